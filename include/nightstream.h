@@ -1,0 +1,121 @@
+/*
+ * nightstream.h -- C ABI of the MI355X Neo folding backend (libnightstream.so).
+ *
+ * The reference (LFDT-Nightstream, Rust) has no FFI layer: its hot-path boundaries are
+ * in-process traits.  Each entry point below replaces one of them with plain pointers and
+ * sizes so a Rust `extern "C"` crate (or ctypes) can bind it.  Conventions:
+ *   - F_q elements are uint64_t canonical values < q = 2^64 - 2^32 + 1.
+ *   - K = F_q[u]/(u^2 - 7) elements are two uint64_t (c0, c1).
+ *   - Digit matrices are int8 row-major D x m (Z[rho*m + c]), D = 54, entries in {-1,0,1}.
+ *   - Commitments are kappa*D uint64_t, data[col*D + row] (neo-ajtai/src/types.rs:17-22).
+ *   - Every function returns 0 on success or a negative NS_E* code; ns_last_error() returns
+ *     the message of the calling thread's last failure (the reference panics or returns
+ *     Err(PiCcsError) in these cases, neo-reductions/src/error.rs:6-21).
+ *   - All work runs on the calling process's current HIP device; nothing falls back to the
+ *     CPU: without a usable GPU every compute entry point returns NS_ENODEV.
+ */
+#ifndef NIGHTSTREAM_H
+#define NIGHTSTREAM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NS_OK 0
+#define NS_EINVAL -1      /* invalid input (PiCcsError::InvalidInput) */
+#define NS_EPARAMS -2     /* parameter / extension policy failure (ParamsError) */
+#define NS_ESUMCHECK -4   /* sum-check invariant failed (PiCcsError::SumcheckError) */
+#define NS_ERANGE -6      /* DEC split out of range (PiCcsError::ProtocolError) */
+#define NS_EPROTOCOL -7   /* other protocol check failed (PiCcsError::ProtocolError) */
+#define NS_ENODEV -10     /* no usable HIP device */
+#define NS_EHIP -11       /* HIP runtime error */
+
+typedef struct ns_ccs ns_ccs;
+typedef struct ns_pp ns_pp;
+typedef struct ns_prover ns_prover;
+typedef struct ns_oracle ns_oracle;
+
+/* ---- runtime ---- */
+const char* ns_last_error(void);
+int ns_device_count(void);
+/* Free a buffer returned by this library. */
+void ns_free(void* p);
+
+/* ---- parameters: NeoParams::goldilocks_auto_r1cs_ccs (neo-params/src/lib.rs:186-222) ----
+ * out[0..10] = q, eta, d, kappa, m, b, k_rho, B, T, s, lambda */
+int ns_params_goldilocks_auto_r1cs_ccs(size_t n_rows, uint64_t out[11]);
+
+/* ---- CCS structure: r1cs_to_ccs (neo-ccs/src/r1cs.rs:13-43), identity-first, n == m ----
+ * CSR per matrix: row_ptr[n+1], col[nnz], val[nnz].  The digest is digest_ccs_matrices
+ * (neo-reductions/src/engines/utils.rs:214-321). */
+int ns_ccs_from_r1cs(size_t n, size_t m, const uint64_t* a_ptr, const uint32_t* a_col, const uint64_t* a_val,
+                     const uint64_t* b_ptr, const uint32_t* b_col, const uint64_t* b_val, const uint64_t* c_ptr,
+                     const uint32_t* c_col, const uint64_t* c_val, ns_ccs** out);
+int ns_ccs_digest(const ns_ccs* s, uint64_t out[4]);
+void ns_ccs_free(ns_ccs* s);
+
+/* ---- Ajtai public parameters: neo_ajtai::setup (neo-ajtai/src/commit.rs:133-156) ----
+ * Samples M in R_q^{kappa x m} from ChaCha8Rng::from_seed(seed) directly in HBM. */
+int ns_pp_setup_seeded(const uint8_t seed[32], size_t m, int keep_coeffs, ns_pp** out);
+/* PP from host coefficients pp[(i*m + j)*54 + u] (kappa = 16). */
+int ns_pp_from_coeffs(const uint64_t* coeffs, size_t m, int keep_coeffs, ns_pp** out);
+/* Copy coefficients of PP rows back (requires keep_coeffs): out[(i*m + j)*54 + u]. */
+int ns_pp_coeffs(const ns_pp* pp, uint64_t* out);
+void ns_pp_free(ns_pp* pp);
+
+/* ---- decomposition: decomp_b_row_major, b = 2 balanced (neo-ajtai/src/decomp.rs:118-158) ---- */
+int ns_decomp_b(const uint64_t* z, size_t m, uint32_t b, int8_t* Z_out);
+
+/* ---- SModuleHomomorphism::commit (neo-ccs/src/traits.rs:8-14; AjtaiSModule::commit,
+ *      neo-ajtai/src/s_module.rs:335-364 -> commit_precomp_ct_row_major, commit.rs:679-744).
+ * Commits n_wit host digit matrices in one batched launch; out[w*kappa*D + col*D + row]. */
+int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit, uint64_t* out);
+
+/* ---- RoundOracle for Pi_CCS (neo-reductions/src/sumcheck.rs:17-34; OptimizedOracle,
+ *      optimized_engine/oracle.rs:1597-1639 new_with_sparse, 2075-2104 trait impl).
+ * Z: k host digit matrices (MCS first, then ME witnesses), uploaded once and kept in HBM.
+ * alpha[ell_d], beta_a[ell_d], beta_r[ell_n], r_inputs[ell_n] (nullable) are K elements. */
+int ns_ccs_oracle_new(const ns_ccs* s, const int8_t* const* Z, size_t k, const uint64_t* alpha,
+                      const uint64_t* beta_a, const uint64_t* beta_r, const uint64_t gamma[2], size_t ell_d,
+                      size_t ell_n, const uint64_t* r_inputs, ns_oracle** out);
+/* evals_at for the row rounds: out[2*i..] = round polynomial at xs[i] (n points). */
+int ns_oracle_evals(ns_oracle* o, const uint64_t* xs, size_t n, uint64_t* out);
+int ns_oracle_fold(ns_oracle* o, const uint64_t r[2]);
+size_t ns_oracle_num_rounds(const ns_oracle* o);
+size_t ns_oracle_degree_bound(const ns_oracle* o);
+/* precompute_for_r (oracle.rs:1659-1783) at row point r[ell_n]: out[(w*t + j)*54 + rho] (K). */
+int ns_oracle_y_eval(ns_oracle* o, const uint64_t* r, uint64_t* out);
+void ns_oracle_free(ns_oracle* o);
+
+/* ---- Pi_RLC Z_mix (optimized_engine/common.rs:1329-1363): zmix = sum_i rot(a_i) Z_i,
+ *      a_i = rho_coeffs[i*54 .. ] in [-2, 2] (neo-reductions/src/common.rs:370-403). */
+int ns_rlc_zmix(const int8_t* rho_coeffs, const int8_t* const* Z, size_t k, size_t m, int32_t* zmix_out);
+/* ---- Pi_DEC split (neo-reductions/src/common.rs:75-268): balanced base-2 digits of zmix
+ *      into k_dec planes; nonzero[i] = plane i has a nonzero digit.  NS_ERANGE if |v| >= 2^k_dec. */
+int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* outs, int* nonzero);
+
+/* ---- shard prover: fold_shard_prove (neo-fold/src/shard.rs:2285-2316) driven as
+ *      FoldingSession::{add_step_io, fold_and_prove} (neo-fold/src/session.rs:1018-1311).
+ * z_steps[s] is step s's full z = [x | w] (length m); x = z[0..m_in].  On success *proof
+ * holds the canonical encoding (little-endian u64 words); free it
+ * with ns_free. */
+int ns_prover_new(const ns_ccs* s, const ns_pp* pp, ns_prover** out);
+int ns_prove_shard(ns_prover* p, const uint64_t* const* z_steps, size_t n_steps, size_t m_in, uint8_t** proof,
+                   size_t* proof_len, double* seconds);
+/* Same, with z_steps_dev[s] device pointers (inputs already resident in HBM). */
+int ns_prove_shard_device(ns_prover* p, const uint64_t* const* z_steps_dev, size_t n_steps, size_t m_in,
+                          uint8_t** proof, size_t* proof_len, double* seconds);
+/* Per-kernel timing of the next proves (HIP events on the prover's stream); classes:
+ * 0 round-0 eval, 1 round eval, 2 table fold, 3 Ajtai GEMM, 4 digit x K GEMM, 5 RLC Z_mix,
+ * 6 DEC split.  ms[i] is the summed duration and counts[i] the number of launches. */
+int ns_prover_set_timing(ns_prover* p, int on);
+int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, size_t n);
+void ns_prover_free(ns_prover* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,286 @@
+"""MI355X-native backend for the Nightstream Neo folding hot path.
+
+The compute lives in libnightstream.so (hand-written HIP for gfx950 plus a C++ host fold loop)
+behind the C ABI in include/nightstream.h; this module binds it with ctypes and mirrors the
+reference's entry points (neo-fold FoldingSession, neo-ajtai commit, the Pi_CCS RoundOracle).
+There is no CPU fallback: every call raises NsError when the library or the GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnightstream.so")
+Q = 0xFFFFFFFF00000001
+D = 54
+KAPPA = 16
+
+EXPORTS = [
+    "ns_last_error", "ns_device_count", "ns_free", "ns_params_goldilocks_auto_r1cs_ccs", "ns_ccs_from_r1cs",
+    "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
+    "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
+    "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
+    "ns_dec_split", "ns_prover_new", "ns_prove_shard", "ns_prover_free",
+]
+
+
+class NsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libnightstream.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NsError(-10, f"{LIB_PATH} is missing: run `python -m nightstream_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, u64p = C.c_void_p, C.c_size_t, C.POINTER(C.c_uint64)
+    L.ns_last_error.restype = C.c_char_p
+    L.ns_device_count.restype = C.c_int
+    L.ns_free.argtypes = [vp]
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if name not in ("ns_last_error", "ns_device_count", "ns_free", "ns_oracle_num_rounds",
+                        "ns_oracle_degree_bound") and not name.endswith("_free"):
+            fn.restype = C.c_int
+    L.ns_oracle_num_rounds.restype = sz
+    L.ns_oracle_degree_bound.restype = sz
+    L.ns_oracle_num_rounds.argtypes = [vp]
+    L.ns_oracle_degree_bound.argtypes = [vp]
+    for n in ("ns_ccs_free", "ns_pp_free", "ns_oracle_free", "ns_prover_free"):
+        getattr(L, n).argtypes = [vp]
+        getattr(L, n).restype = None
+    L.ns_ccs_from_r1cs.argtypes = [sz, sz] + [vp] * 9 + [C.POINTER(vp)]
+    L.ns_ccs_digest.argtypes = [vp, vp]
+    L.ns_pp_setup_seeded.argtypes = [C.c_char_p, sz, C.c_int, C.POINTER(vp)]
+    L.ns_pp_from_coeffs.argtypes = [vp, sz, C.c_int, C.POINTER(vp)]
+    L.ns_pp_coeffs.argtypes = [vp, vp]
+    L.ns_decomp_b.argtypes = [vp, sz, C.c_uint32, vp]
+    L.ns_ajtai_commit_batch.argtypes = [vp, vp, sz, vp]
+    L.ns_ccs_oracle_new.argtypes = [vp, vp, sz, vp, vp, vp, vp, sz, sz, vp, C.POINTER(vp)]
+    L.ns_oracle_evals.argtypes = [vp, vp, sz, vp]
+    L.ns_oracle_fold.argtypes = [vp, vp]
+    L.ns_oracle_y_eval.argtypes = [vp, vp, vp]
+    L.ns_rlc_zmix.argtypes = [vp, vp, sz, sz, vp]
+    L.ns_dec_split.argtypes = [vp, sz, sz, vp, vp]
+    L.ns_prover_new.argtypes = [vp, vp, C.POINTER(vp)]
+    L.ns_prove_shard.argtypes = [vp, vp, sz, sz, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
+                                 C.POINTER(C.c_double)]
+    L.ns_params_goldilocks_auto_r1cs_ccs.argtypes = [sz, vp]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise NsError(rc, lib().ns_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ptr_array(arrays):
+    return (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+def device_count() -> int:
+    return lib().ns_device_count()
+
+
+def params_goldilocks_auto_r1cs_ccs(n_rows: int) -> dict:
+    out = np.zeros(11, np.uint64)
+    _check(lib().ns_params_goldilocks_auto_r1cs_ccs(n_rows, _ptr(out)))
+    keys = ["q", "eta", "d", "kappa", "m", "b", "k_rho", "B", "T", "s", "lambda"]
+    return {k: int(v) for k, v in zip(keys, out)}
+
+
+class Ccs:
+    """Identity-first CCS from R1CS CSR matrices (neo-ccs r1cs_to_ccs)."""
+
+    def __init__(self, inst):
+        self.n, self.m = inst.n, inst.m
+        arrs = []
+        for (rp, ci, v) in (inst.a, inst.b, inst.c):
+            arrs += [np.ascontiguousarray(rp, np.uint64), np.ascontiguousarray(ci, np.uint32),
+                     np.ascontiguousarray(v, np.uint64)]
+        self._keep = arrs
+        h = C.c_void_p()
+        _check(lib().ns_ccs_from_r1cs(inst.n, inst.m, *[_ptr(a) for a in arrs], C.byref(h)))
+        self.h = h
+
+    def digest(self) -> np.ndarray:
+        out = np.zeros(4, np.uint64)
+        _check(lib().ns_ccs_digest(self.h, _ptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ns_ccs_free(self.h)
+            self.h = None
+
+
+class PublicParams:
+    """Ajtai PP resident in HBM (neo_ajtai::setup from a ChaCha8 seed)."""
+
+    def __init__(self, m: int, seed: bytes | None = None, coeffs: np.ndarray | None = None, keep_coeffs=False):
+        self.m = m
+        h = C.c_void_p()
+        if coeffs is not None:
+            c = np.ascontiguousarray(coeffs, np.uint64)
+            _check(lib().ns_pp_from_coeffs(_ptr(c), m, int(keep_coeffs), C.byref(h)))
+        else:
+            assert seed is not None and len(seed) == 32
+            _check(lib().ns_pp_setup_seeded(seed, m, int(keep_coeffs), C.byref(h)))
+        self.h = h
+
+    def coeffs(self) -> np.ndarray:
+        out = np.zeros(KAPPA * self.m * D, np.uint64)
+        _check(lib().ns_pp_coeffs(self.h, _ptr(out)))
+        return out
+
+    def commit(self, Zs) -> np.ndarray:
+        """SModuleHomomorphism::commit for a batch of D x m digit matrices -> [n, kappa*D]."""
+        Zs = [np.ascontiguousarray(z, np.int8).reshape(-1) for z in Zs]
+        out = np.zeros(len(Zs) * KAPPA * D, np.uint64)
+        _check(lib().ns_ajtai_commit_batch(self.h, _ptr_array(Zs), len(Zs), _ptr(out)))
+        return out.reshape(len(Zs), KAPPA * D)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ns_pp_free(self.h)
+            self.h = None
+
+
+def decomp_b(z: np.ndarray, b: int = 2) -> np.ndarray:
+    z = np.ascontiguousarray(z, np.uint64)
+    Z = np.zeros(D * len(z), np.int8)
+    _check(lib().ns_decomp_b(_ptr(z), len(z), b, _ptr(Z)))
+    return Z.reshape(D, len(z))
+
+
+def _kflat(xs) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(xs, np.uint64).reshape(-1))
+
+
+class CcsOracle:
+    """Pi_CCS RoundOracle on the GPU (evals_at / fold / num_rounds / degree_bound)."""
+
+    def __init__(self, ccs: Ccs, Zs, alpha, beta_a, beta_r, gamma, ell_d, ell_n, r_inputs=None):
+        self.Zs = [np.ascontiguousarray(z, np.int8).reshape(-1) for z in Zs]
+        self._k = [_kflat(alpha), _kflat(beta_a), _kflat(beta_r), _kflat(gamma)]
+        rin = None if r_inputs is None else _kflat(r_inputs)
+        self._rin = rin
+        h = C.c_void_p()
+        _check(lib().ns_ccs_oracle_new(ccs.h, _ptr_array(self.Zs), len(self.Zs), *[_ptr(a) for a in self._k],
+                                       ell_d, ell_n, None if rin is None else _ptr(rin), C.byref(h)))
+        self.h = h
+        self.ccs = ccs
+
+    def num_rounds(self) -> int:
+        return lib().ns_oracle_num_rounds(self.h)
+
+    def degree_bound(self) -> int:
+        return lib().ns_oracle_degree_bound(self.h)
+
+    def evals_at(self, xs) -> np.ndarray:
+        xs = _kflat(xs)
+        out = np.zeros_like(xs)
+        _check(lib().ns_oracle_evals(self.h, _ptr(xs), len(xs) // 2, _ptr(out)))
+        return out.reshape(-1, 2)
+
+    def fold(self, r):
+        r = _kflat(r)
+        _check(lib().ns_oracle_fold(self.h, _ptr(r)))
+
+    def y_eval(self, r) -> np.ndarray:
+        r = _kflat(r)
+        out = np.zeros(len(self.Zs) * 4 * D * 2, np.uint64)
+        _check(lib().ns_oracle_y_eval(self.h, _ptr(r), _ptr(out)))
+        return out.reshape(len(self.Zs), 4, D, 2)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ns_oracle_free(self.h)
+            self.h = None
+
+
+def rlc_zmix(rho_coeffs, Zs, m: int) -> np.ndarray:
+    rc = np.ascontiguousarray(np.asarray(rho_coeffs, np.int8).reshape(-1))
+    Zs = [np.ascontiguousarray(z, np.int8).reshape(-1) for z in Zs]
+    out = np.zeros(D * m, np.int32)
+    _check(lib().ns_rlc_zmix(_ptr(rc), _ptr_array(Zs), len(Zs), m, _ptr(out)))
+    return out.reshape(D, m)
+
+
+def dec_split(zmix: np.ndarray, k_dec: int):
+    zmix = np.ascontiguousarray(zmix, np.int32).reshape(-1)
+    m = len(zmix) // D
+    outs = [np.zeros(D * m, np.int8) for _ in range(k_dec)]
+    nz = np.zeros(k_dec, np.int32)
+    _check(lib().ns_dec_split(_ptr(zmix), m, k_dec, _ptr_array(outs), _ptr(nz)))
+    return [o.reshape(D, m) for o in outs], [bool(x) for x in nz]
+
+
+class FoldingSession:
+    """Mirror of neo-fold FoldingSession (session.rs:668-1953) for CCS-only shards.
+
+    new_ajtai_seeded(ccs, seed) -> add_step_io(x, w) per step -> fold_and_prove() returns the
+    canonical proof encoding; the accumulator and witnesses stay in HBM during the fold.
+    """
+
+    def __init__(self, ccs: Ccs, pp: PublicParams):
+        self.ccs, self.pp = ccs, pp
+        h = C.c_void_p()
+        _check(lib().ns_prover_new(ccs.h, pp.h, C.byref(h)))
+        self.h = h
+        self.steps: list[np.ndarray] = []
+        self.m_in: int | None = None
+        self.last_seconds = 0.0
+
+    @classmethod
+    def new_ajtai_seeded(cls, inst, seed: bytes):
+        ccs = Ccs(inst)
+        pp = PublicParams(inst.m, seed=seed)
+        return cls(ccs, pp)
+
+    def add_step_io(self, public_input, witness):
+        x = np.asarray(public_input, np.uint64)
+        w = np.asarray(witness, np.uint64)
+        if self.m_in is None:
+            self.m_in = len(x)
+        elif self.m_in != len(x):
+            raise NsError(-1, "all steps must share the same m_in")
+        z = np.concatenate([x, w]).astype(np.uint64)
+        if len(z) != self.ccs.m:
+            raise NsError(-1, f"len(x) + len(w) = {len(z)} but CCS.m = {self.ccs.m}")
+        self.steps.append(np.ascontiguousarray(z))
+
+    def add_step_z(self, z, m_in=1):
+        self.add_step_io(z[:m_in], z[m_in:])
+
+    def fold_and_prove(self) -> bytes:
+        out = C.POINTER(C.c_uint8)()
+        n = C.c_size_t()
+        sec = C.c_double()
+        _check(lib().ns_prove_shard(self.h, _ptr_array(self.steps), len(self.steps), self.m_in or 0, C.byref(out),
+                                    C.byref(n), C.byref(sec)))
+        data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)
+        lib().ns_free(C.cast(out, C.c_void_p))
+        self.last_seconds = sec.value
+        return data
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ns_prover_free(self.h)
+            self.h = None

@@ -1,0 +1,247 @@
+// ajtai.hip -- balanced decomposition, Ajtai public-parameter generation and the Ajtai
+// commitment as an exact int8 MFMA contraction on gfx950.
+//
+// Commitment (neo-ajtai/src/commit.rs:679-744): for every Ajtai row i < kappa,
+//   c_i = cf( sum_j a_ij * zhat_j  mod Phi_81 ),  zhat_j = sum_t Z[t][j] X^t,  Z[t][j] in {-1,0,1}.
+// Every a_ij coefficient (u64 < q) is split into 8 byte limbs stored with a -128 offset so they
+// fit signed int8.  The polynomial product sum is then
+//   P_i[s] = sum_l 256^l sum_{u+t=s} ( G'_l[u][t] + 128 S[t] ),
+//   G'_l[u][t] = sum_j (limb_l(a_ij[u]) - 128) Z[t][j],   S[t] = sum_j Z[t][j],
+// a GEMM with M = kappa*8*54 limb rows, N = 54 * (#witnesses) digit rows and K = m, computed
+// with v_mfma_i32_16x16x64_i8 and int32 accumulation (|G'| <= 128 m < 2^31 for m <= 2^23).
+// The epilogue sums anti-diagonals in int64, combines limbs in F_q and reduces mod
+// Phi_81 = X^54 + X^27 + 1 using X^81 = 1 and X^54 = -X^27 - 1.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ns {
+
+// ---------------- decomposition ----------------
+__global__ void k_decomp_b2(const uint64_t* __restrict__ z, size_t m, int8_t* __restrict__ Z) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  int64_t a = to_balanced(z[c]);
+  for (int row = 0; row < D; row++) {
+    int64_t dg = (a & 1) == 0 ? 0 : (a >= 0 ? 1 : -1);
+    Z[(size_t)row * m + c] = (int8_t)dg;
+    a = (a - dg) >> 1;
+  }
+}
+
+void launch_decomp_b2(const uint64_t* z, size_t m, int8_t* Z, hipStream_t s) {
+  if (m == 0) return;
+  hipLaunchKernelGGL(k_decomp_b2, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, z, m, Z);
+}
+
+// ---------------- PP limbs ----------------
+// limbs[((i*8 + l)*54 + u) * m + j] = byte_l(pp[(i*m + j)*54 + u]) - 128
+__global__ void k_pp_limbs(const uint64_t* __restrict__ pp, size_t kappa, size_t m, int8_t* __restrict__ limbs) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t i = blockIdx.y;
+  if (j >= m) return;
+  const uint64_t* a = pp + (i * m + j) * D;
+  for (int u = 0; u < D; u++) {
+    uint64_t v = a[u];
+    for (int l = 0; l < 8; l++) {
+      int b = (int)((v >> (8 * l)) & 0xFF) - 128;
+      limbs[((i * 8 + l) * D + u) * m + j] = (int8_t)b;
+    }
+  }
+}
+
+void launch_pp_limbs(const uint64_t* pp, size_t kappa, size_t m, int8_t* limbs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pp_limbs, dim3((unsigned)((m + 255) / 256), (unsigned)kappa), dim3(256), 0, s, pp, kappa, m,
+                     limbs);
+}
+
+// ---------------- ChaCha8 keystream (rand_chacha block layout) ----------------
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int n) { return (x << n) | (x >> (32 - n)); }
+#define NS_QR(a, b, c, d) \
+  a += b; d ^= a; d = rotl32(d, 16); c += d; b ^= c; b = rotl32(b, 12); \
+  a += b; d ^= a; d = rotl32(d, 8);  c += d; b ^= c; b = rotl32(b, 7);
+
+struct ChaKey {
+  uint32_t k[8];
+};
+
+__global__ void k_chacha8_words(ChaKey key, uint64_t first_block, size_t nblocks, uint32_t* __restrict__ out) {
+  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  uint64_t ctr = first_block + b;
+  uint32_t in[16] = {0x61707865, 0x3320646e, 0x79622d32, 0x6b206574, key.k[0], key.k[1], key.k[2], key.k[3],
+                     key.k[4],   key.k[5],   key.k[6],   key.k[7],   (uint32_t)ctr, (uint32_t)(ctr >> 32), 0, 0};
+  uint32_t x[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) x[i] = in[i];
+#pragma unroll
+  for (int r = 0; r < 8; r += 2) {
+    NS_QR(x[0], x[4], x[8], x[12]);
+    NS_QR(x[1], x[5], x[9], x[13]);
+    NS_QR(x[2], x[6], x[10], x[14]);
+    NS_QR(x[3], x[7], x[11], x[15]);
+    NS_QR(x[0], x[5], x[10], x[15]);
+    NS_QR(x[1], x[6], x[11], x[12]);
+    NS_QR(x[2], x[7], x[8], x[13]);
+    NS_QR(x[3], x[4], x[9], x[14]);
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) out[b * 16 + i] = x[i] + in[i];
+}
+
+void launch_chacha8_words(const uint32_t key[8], uint64_t first_block, size_t nblocks, uint32_t* out, hipStream_t s) {
+  ChaKey k;
+  for (int i = 0; i < 8; i++) k.k[i] = key[i];
+  hipLaunchKernelGGL(k_chacha8_words, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, s, k, first_block,
+                     nblocks, out);
+}
+
+// ---------------- Ajtai commit ----------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int AJ_MT = 4;       // 16-row M subtiles per wave (64 limb rows per block)
+constexpr int AJ_NS = 11;      // 16-column N subtiles per wave
+constexpr int AJ_WAVES = 4;    // waves per block
+constexpr int AJ_NBLK = AJ_WAVES * AJ_NS * 16;  // 704 digit rows per block
+constexpr int AJ_MROWS = KAPPA * 8 * D;         // 6912
+
+static size_t aj_splits(size_t m) {
+  size_t s = m / 16384;
+  if (s < 1) s = 1;
+  if (s > 8) s = 8;
+  while (s > 1 && (m / s) % 64 != 0) s--;
+  return s;
+}
+static size_t aj_groups(size_t nw) { return (nw * D + AJ_NBLK - 1) / AJ_NBLK; }
+
+size_t ajtai_workspace_bytes(size_t m, size_t nw) {
+  size_t npad = aj_groups(nw) * AJ_NBLK;
+  return aj_splits(m) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 256;
+}
+
+// grid: x = M tile (64 limb rows), y = N group (704 digit rows), z = K split
+__global__ __launch_bounds__(256) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t m, size_t kc,
+                                                   const int8_t* const* __restrict__ Zs, size_t nw, size_t npad,
+                                                   int32_t* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const size_t m0 = (size_t)blockIdx.x * (AJ_MT * 16);
+  const size_t k_begin = (size_t)blockIdx.z * kc;
+  const int kq = (lane >> 4) * 16;  // byte offset of this lane's k-slice inside a 64-wide step
+  const int r16 = lane & 15;
+
+  const int8_t* arow[AJ_MT];
+#pragma unroll
+  for (int mt = 0; mt < AJ_MT; mt++) arow[mt] = limbs + (m0 + mt * 16 + r16) * m + k_begin + kq;
+
+  const int8_t* brow[AJ_NS];
+  bool bval[AJ_NS];
+  const size_t ncol0 = (size_t)blockIdx.y * AJ_NBLK + (size_t)wave * AJ_NS * 16;
+#pragma unroll
+  for (int ns_ = 0; ns_ < AJ_NS; ns_++) {
+    size_t nc = ncol0 + ns_ * 16 + r16;
+    size_t w = nc / D, t = nc % D;
+    bval[ns_] = w < nw;
+    brow[ns_] = bval[ns_] ? Zs[w] + t * m + k_begin + kq : nullptr;
+  }
+
+  v4i acc[AJ_MT][AJ_NS];
+#pragma unroll
+  for (int mt = 0; mt < AJ_MT; mt++)
+#pragma unroll
+    for (int ns_ = 0; ns_ < AJ_NS; ns_++) acc[mt][ns_] = v4i{0, 0, 0, 0};
+
+  for (size_t k = 0; k < kc; k += 64) {
+    v4i a[AJ_MT], b[AJ_NS];
+#pragma unroll
+    for (int mt = 0; mt < AJ_MT; mt++) a[mt] = *(const v4i*)(arow[mt] + k);
+#pragma unroll
+    for (int ns_ = 0; ns_ < AJ_NS; ns_++) b[ns_] = bval[ns_] ? *(const v4i*)(brow[ns_] + k) : v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int mt = 0; mt < AJ_MT; mt++)
+#pragma unroll
+      for (int ns_ = 0; ns_ < AJ_NS; ns_++)
+        acc[mt][ns_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mt], b[ns_], acc[mt][ns_], 0, 0, 0);
+  }
+
+  // D[row][col]: col = lane % 16, row = 4 * (lane / 16) + v
+  int32_t* base = part + ((size_t)blockIdx.z * AJ_MROWS + m0) * npad + ncol0;
+#pragma unroll
+  for (int mt = 0; mt < AJ_MT; mt++)
+#pragma unroll
+    for (int ns_ = 0; ns_ < AJ_NS; ns_++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        size_t row = mt * 16 + (lane >> 4) * 4 + v;
+        base[row * npad + ns_ * 16 + r16] = acc[mt][ns_][v];
+      }
+}
+
+// S[w*54 + t] = sum_j Z_w[t][j]
+__global__ void k_digit_rowsum(const int8_t* const* __restrict__ Zs, size_t m, int32_t* __restrict__ S) {
+  size_t w = blockIdx.x / D, t = blockIdx.x % D;
+  const int8_t* row = Zs[w] + t * m;
+  int32_t acc = 0;
+  for (size_t j = threadIdx.x; j < m; j += blockDim.x) acc += row[j];
+  __shared__ int32_t red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S[blockIdx.x] = red[0];
+}
+
+// block = (w, i); thread s < 107 computes P[s]; threads r < 54 then reduce mod Phi_81
+__global__ __launch_bounds__(128) void k_ajtai_epilogue(const int32_t* __restrict__ part, size_t splits,
+                                                        size_t npad, const int32_t* __restrict__ S,
+                                                        uint64_t* __restrict__ out) {
+  const size_t w = blockIdx.x / KAPPA, i = blockIdx.x % KAPPA;
+  __shared__ uint64_t P[2 * D - 1];
+  const int s = threadIdx.x;
+  if (s < 2 * D - 1) {
+    int u_lo = s - (D - 1) > 0 ? s - (D - 1) : 0;
+    int u_hi = s < D - 1 ? s : D - 1;
+    int64_t corr = 0;
+    for (int u = u_lo; u <= u_hi; u++) corr += S[w * D + (s - u)];
+    uint64_t acc = 0;
+    for (int l = 0; l < 8; l++) {
+      int64_t g = 0;
+      for (int u = u_lo; u <= u_hi; u++) {
+        size_t row = (i * 8 + l) * D + u;
+        size_t col = w * D + (s - u);
+        for (size_t sp = 0; sp < splits; sp++) g += part[(sp * AJ_MROWS + row) * npad + col];
+      }
+      g += 128 * corr;
+      acc = fadd(acc, fmul(from_i64(g), 1ull << (8 * l)));
+    }
+    P[s] = acc;
+  }
+  __syncthreads();
+  if (s < D) {
+    uint64_t c = P[s];
+    if (s <= 25) c = fadd(c, P[s + 81]);
+    if (s >= 27) c = fsub(c, P[s + 27]);
+    if (s <= 26) c = fsub(c, P[s + 54]);
+    out[(w * KAPPA + i) * D + s] = c;
+  }
+}
+
+void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* const* Zs_dev, size_t nw,
+                  uint64_t* out, void* workspace, hipStream_t s) {
+  if (nw == 0) return;
+  const size_t splits = aj_splits(m), groups = aj_groups(nw), npad = groups * AJ_NBLK;
+  const size_t kc = m / splits;
+  int32_t* part = (int32_t*)workspace;
+  int32_t* S = part + splits * (size_t)AJ_MROWS * npad;
+  hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
+  ktimer_begin(KT_AJTAI_GEMM, s);
+  hipLaunchKernelGGL(k_ajtai_gemm, dim3((unsigned)(AJ_MROWS / (AJ_MT * 16)), (unsigned)groups, (unsigned)splits),
+                     dim3(64 * AJ_WAVES), 0, s, limbs, m, kc, Zs_dev, nw, npad, part);
+  ktimer_end(KT_AJTAI_GEMM, s);
+  hipLaunchKernelGGL(k_ajtai_epilogue, dim3((unsigned)(nw * KAPPA)), dim3(128), 0, s, part, splits, npad, S, out);
+  (void)kappa;
+}
+
+}  // namespace ns

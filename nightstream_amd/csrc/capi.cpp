@@ -1,0 +1,391 @@
+// capi.cpp -- extern "C" entry points declared in include/nightstream.h.
+#include <string.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nightstream.h"
+#include "prover.h"
+
+using namespace ns;
+
+struct ns_ccs {
+  std::unique_ptr<Ccs> s;
+};
+struct ns_pp {
+  std::unique_ptr<Pp> pp;
+};
+struct ns_prover {
+  std::unique_ptr<Prover> p;
+};
+struct ns_oracle {
+  const Ccs* s;
+  hipStream_t st;
+  std::vector<DevBuf> zbufs;
+  std::unique_ptr<CcsOracle> o;
+  size_t ell_d, ell_n, d_sc, k;
+  std::vector<K2> alpha, beta_a, beta_r, r_inputs;
+  K2 gamma;
+  std::vector<K2> row_chals;
+  bool ajtai_ready = false;
+  AjtaiPhase aj;
+  DevBuf ws, vjs;
+};
+
+static thread_local std::string g_err;
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    f();
+    return NS_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    switch (e.code) {
+      case 2: return NS_EPARAMS;
+      case 4: return NS_ESUMCHECK;
+      case 6: return NS_ERANGE;
+      case 5:
+      case 7: return NS_EPROTOCOL;
+      default: return NS_EINVAL;
+    }
+  } catch (const HipError& e) {
+    g_err = std::string("HIP error ") + hipGetErrorString(e.err) + " at " + e.what + " (" + e.file + ":" +
+            std::to_string(e.line) + ")";
+    return e.err == hipErrorNoDevice || e.err == hipErrorInvalidDevice ? NS_ENODEV : NS_EHIP;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NS_EINVAL;
+  }
+}
+
+static void require_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw HipError(hipErrorNoDevice, "hipGetDeviceCount", __FILE__, __LINE__);
+}
+
+static hipStream_t default_stream() {
+  static hipStream_t s = [] {
+    hipStream_t x;
+    NS_CHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    return x;
+  }();
+  return s;
+}
+
+extern "C" {
+
+const char* ns_last_error(void) { return g_err.c_str(); }
+
+int ns_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void ns_free(void* p) { free(p); }
+
+int ns_params_goldilocks_auto_r1cs_ccs(size_t n_rows, uint64_t out[11]) {
+  return guard([&] {
+    Params p = Params::goldilocks_auto_r1cs_ccs(n_rows);
+    uint64_t v[11] = {p.q, p.eta, p.d, p.kappa, p.m, p.b, p.k_rho, p.B, p.T, p.s, p.lambda};
+    memcpy(out, v, sizeof(v));
+  });
+}
+
+int ns_ccs_from_r1cs(size_t n, size_t m, const uint64_t* a_ptr, const uint32_t* a_col, const uint64_t* a_val,
+                     const uint64_t* b_ptr, const uint32_t* b_col, const uint64_t* b_val, const uint64_t* c_ptr,
+                     const uint32_t* c_col, const uint64_t* c_val, ns_ccs** out) {
+  return guard([&] {
+    require_device();
+    auto mk = [&](const uint64_t* p, const uint32_t* c, const uint64_t* v) {
+      HostCsr h;
+      h.ptr.assign(p, p + n + 1);
+      size_t nnz = p[n];
+      h.col.assign(c, c + nnz);
+      h.val.assign(v, v + nnz);
+      for (auto& x : h.val)
+        if (x >= Q) throw Error(1, "matrix value not canonical");
+      return h;
+    };
+    auto s = Ccs::from_r1cs(n, m, mk(a_ptr, a_col, a_val), mk(b_ptr, b_col, b_val), mk(c_ptr, c_col, c_val));
+    *out = new ns_ccs{std::move(s)};
+  });
+}
+
+int ns_ccs_digest(const ns_ccs* s, uint64_t out[4]) {
+  return guard([&] { memcpy(out, s->s->digest, 32); });
+}
+
+void ns_ccs_free(ns_ccs* s) { delete s; }
+
+int ns_pp_setup_seeded(const uint8_t seed[32], size_t m, int keep_coeffs, ns_pp** out) {
+  return guard([&] {
+    require_device();
+    auto pp = Pp::setup_seeded(seed, m, keep_coeffs != 0, default_stream());
+    *out = new ns_pp{std::move(pp)};
+  });
+}
+
+int ns_pp_from_coeffs(const uint64_t* coeffs, size_t m, int keep_coeffs, ns_pp** out) {
+  return guard([&] {
+    require_device();
+    auto pp = Pp::from_coeffs(coeffs, m, keep_coeffs != 0, default_stream());
+    *out = new ns_pp{std::move(pp)};
+  });
+}
+
+int ns_pp_coeffs(const ns_pp* pp, uint64_t* out) {
+  return guard([&] {
+    if (!pp->pp->coeffs.p) throw Error(1, "PP coefficients were not kept");
+    NS_CHECK(hipMemcpy(out, pp->pp->coeffs.p, (size_t)KAPPA * pp->pp->m * D * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+void ns_pp_free(ns_pp* pp) { delete pp; }
+
+int ns_decomp_b(const uint64_t* z, size_t m, uint32_t b, int8_t* Z_out) {
+  return guard([&] {
+    require_device();
+    if (b != 2) throw Error(1, "only b = 2 is implemented");
+    hipStream_t s = default_stream();
+    DevBuf zd(m * 8), Zd((size_t)D * m);
+    NS_CHECK(hipMemcpyAsync(zd.p, z, m * 8, hipMemcpyHostToDevice, s));
+    launch_decomp_b2(zd.as<uint64_t>(), m, Zd.as<int8_t>(), s);
+    NS_CHECK(hipMemcpyAsync(Z_out, Zd.p, (size_t)D * m, hipMemcpyDeviceToHost, s));
+    NS_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+static std::vector<DevBuf> upload_digits(const int8_t* const* Z, size_t n, size_t m, hipStream_t s) {
+  std::vector<DevBuf> out;
+  for (size_t i = 0; i < n; i++) {
+    out.emplace_back((size_t)D * m);
+    NS_CHECK(hipMemcpyAsync(out.back().p, Z[i], (size_t)D * m, hipMemcpyHostToDevice, s));
+  }
+  return out;
+}
+
+int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit, uint64_t* out) {
+  return guard([&] {
+    require_device();
+    const size_t m = pp->pp->m;
+    hipStream_t s = default_stream();
+    auto bufs = upload_digits(Z, n_wit, m, s);
+    std::vector<const int8_t*> ptrs;
+    for (auto& b : bufs) ptrs.push_back(b.as<int8_t>());
+    DevBuf ptrs_d(n_wit * sizeof(void*)), outd(n_wit * KAPPA * D * 8), ws(ajtai_workspace_bytes(m, n_wit));
+    NS_CHECK(hipMemcpyAsync(ptrs_d.p, ptrs.data(), n_wit * sizeof(void*), hipMemcpyHostToDevice, s));
+    ajtai_commit(pp->pp->limbs.as<int8_t>(), KAPPA, m, ptrs_d.as<const int8_t*>(), n_wit, outd.as<uint64_t>(), ws.p,
+                 s);
+    NS_CHECK(hipMemcpyAsync(out, outd.p, n_wit * KAPPA * D * 8, hipMemcpyDeviceToHost, s));
+    NS_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+static std::vector<K2> kvec(const uint64_t* p, size_t n) {
+  std::vector<K2> v(n);
+  for (size_t i = 0; i < n; i++) v[i] = K2{p[2 * i], p[2 * i + 1]};
+  return v;
+}
+
+int ns_ccs_oracle_new(const ns_ccs* s, const int8_t* const* Z, size_t k, const uint64_t* alpha,
+                      const uint64_t* beta_a, const uint64_t* beta_r, const uint64_t gamma[2], size_t ell_d,
+                      size_t ell_n, const uint64_t* r_inputs, ns_oracle** out) {
+  return guard([&] {
+    require_device();
+    if (ell_d != 6) throw Error(1, "ell_d must be 6 (D = 54)");
+    if (k == 0) throw Error(1, "need at least one witness");
+    auto o = std::make_unique<ns_oracle>();
+    o->s = s->s.get();
+    o->st = default_stream();
+    o->zbufs = upload_digits(Z, k, s->s->m, o->st);
+    std::vector<const int8_t*> ptrs;
+    for (auto& b : o->zbufs) ptrs.push_back(b.as<int8_t>());
+    o->alpha = kvec(alpha, ell_d);
+    o->beta_a = kvec(beta_a, ell_d);
+    o->beta_r = kvec(beta_r, ell_n);
+    if (r_inputs) o->r_inputs = kvec(r_inputs, ell_n);
+    o->gamma = K2{gamma[0], gamma[1]};
+    o->ell_d = ell_d;
+    o->ell_n = ell_n;
+    o->k = k;
+    o->d_sc = std::max<size_t>(s->s->f_degree() + 1, 6);
+    o->o = std::make_unique<CcsOracle>(*s->s, ptrs.data(), k, o->alpha.data(), o->beta_a.data(), o->beta_r.data(),
+                                       o->gamma, ell_d, ell_n, r_inputs ? o->r_inputs.data() : nullptr, o->st);
+    *out = o.release();
+  });
+}
+
+int ns_oracle_evals(ns_oracle* o, const uint64_t* xs, size_t n, uint64_t* out) {
+  return guard([&] {
+    std::vector<K2> x = kvec(xs, n), y(n);
+    if (o->row_chals.size() < o->ell_n) {
+      std::vector<K2> c = o->o->round_coeffs();
+      for (size_t i = 0; i < n; i++) {
+        K2 r = c.back();
+        for (size_t d = c.size() - 1; d-- > 0;) r = kadd(kmul(r, x[i]), c[d]);
+        y[i] = r;
+      }
+    } else {
+      if (!o->ajtai_ready) {
+        auto ye = o->o->y_eval(o->row_chals, o->ws, o->vjs);
+        o->aj.init(*o->s, std::move(ye), o->k, o->alpha.data(), o->beta_a.data(), o->beta_r.data(), o->gamma,
+                   o->row_chals, o->r_inputs.empty() ? nullptr : o->r_inputs.data());
+        o->ajtai_ready = true;
+      }
+      if (o->aj.chals.size() >= o->ell_d) throw Error(1, "evals after the last round");
+      y = o->aj.evals(x);
+    }
+    for (size_t i = 0; i < n; i++) {
+      out[2 * i] = y[i].c0;
+      out[2 * i + 1] = y[i].c1;
+    }
+  });
+}
+
+int ns_oracle_fold(ns_oracle* o, const uint64_t r[2]) {
+  return guard([&] {
+    K2 x{r[0], r[1]};
+    if (o->row_chals.size() < o->ell_n) {
+      o->row_chals.push_back(x);
+      o->o->fold(x);
+    } else {
+      if (!o->ajtai_ready) {
+        auto ye = o->o->y_eval(o->row_chals, o->ws, o->vjs);
+        o->aj.init(*o->s, std::move(ye), o->k, o->alpha.data(), o->beta_a.data(), o->beta_r.data(), o->gamma,
+                   o->row_chals, o->r_inputs.empty() ? nullptr : o->r_inputs.data());
+        o->ajtai_ready = true;
+      }
+      o->aj.fold(x);
+    }
+  });
+}
+
+size_t ns_oracle_num_rounds(const ns_oracle* o) { return o->ell_n + o->ell_d; }
+size_t ns_oracle_degree_bound(const ns_oracle* o) { return o->d_sc; }
+
+int ns_oracle_y_eval(ns_oracle* o, const uint64_t* r, uint64_t* out) {
+  return guard([&] {
+    std::vector<K2> rr = kvec(r, o->ell_n);
+    auto y = o->o->y_eval(rr, o->ws, o->vjs);
+    memcpy(out, y.data(), y.size() * sizeof(K2));
+  });
+}
+
+void ns_oracle_free(ns_oracle* o) { delete o; }
+
+int ns_rlc_zmix(const int8_t* rho_coeffs, const int8_t* const* Z, size_t k, size_t m, int32_t* zmix_out) {
+  return guard([&] {
+    require_device();
+    hipStream_t s = default_stream();
+    // rot(a_i) as int8 [k][54][54]
+    std::vector<int8_t> rho(k * D * D);
+    for (size_t i = 0; i < k; i++) {
+      int64_t col[D], nxt[D];
+      for (int r = 0; r < D; r++) {
+        col[r] = rho_coeffs[i * D + r];
+        if (col[r] < -2 || col[r] > 2) throw Error(1, "rho coefficient outside the alphabet [-2, 2]");
+      }
+      for (int j = 0; j < D; j++) {
+        for (int r = 0; r < D; r++) rho[i * D * D + r * D + j] = (int8_t)col[r];
+        int64_t last = col[D - 1];
+        nxt[0] = -last;
+        for (int r = 1; r < D; r++) nxt[r] = col[r - 1];
+        nxt[27] -= last;
+        memcpy(col, nxt, sizeof(col));
+      }
+    }
+    auto bufs = upload_digits(Z, k, m, s);
+    std::vector<const int8_t*> ptrs;
+    for (auto& b : bufs) ptrs.push_back(b.as<int8_t>());
+    DevBuf pd(k * sizeof(void*)), rd(rho.size()), zd((size_t)D * m * 4);
+    NS_CHECK(hipMemcpyAsync(pd.p, ptrs.data(), k * sizeof(void*), hipMemcpyHostToDevice, s));
+    NS_CHECK(hipMemcpyAsync(rd.p, rho.data(), rho.size(), hipMemcpyHostToDevice, s));
+    launch_rlc_zmix(rd.as<int8_t>(), pd.as<const int8_t*>(), k, m, zd.as<int32_t>(), s);
+    NS_CHECK(hipMemcpyAsync(zmix_out, zd.p, (size_t)D * m * 4, hipMemcpyDeviceToHost, s));
+    NS_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* outs, int* nonzero) {
+  return guard([&] {
+    require_device();
+    if (k_dec == 0 || k_dec > 30) throw Error(1, "k_dec out of range");
+    hipStream_t s = default_stream();
+    const size_t len = (size_t)D * m;
+    DevBuf zd(len * 4), flags((k_dec + 1) * 4), pd(k_dec * sizeof(void*));
+    std::vector<DevBuf> planes;
+    std::vector<int8_t*> ptrs;
+    for (size_t i = 0; i < k_dec; i++) {
+      planes.emplace_back(len);
+      ptrs.push_back(planes.back().as<int8_t>());
+    }
+    NS_CHECK(hipMemcpyAsync(zd.p, zmix, len * 4, hipMemcpyHostToDevice, s));
+    NS_CHECK(hipMemcpyAsync(pd.p, ptrs.data(), k_dec * sizeof(void*), hipMemcpyHostToDevice, s));
+    NS_CHECK(hipMemsetAsync(flags.p, 0, (k_dec + 1) * 4, s));
+    launch_dec_split(zd.as<int32_t>(), len, (int)k_dec, pd.as<int8_t* const>(), flags.as<int>(), s);
+    std::vector<int> fl(k_dec + 1);
+    NS_CHECK(hipMemcpyAsync(fl.data(), flags.p, (k_dec + 1) * 4, hipMemcpyDeviceToHost, s));
+    NS_CHECK(hipStreamSynchronize(s));
+    if (fl[k_dec]) throw Error(6, "DEC split: value out of range for k_rho digits");
+    for (size_t i = 0; i < k_dec; i++) {
+      NS_CHECK(hipMemcpy(outs[i], planes[i].p, len, hipMemcpyDeviceToHost));
+      nonzero[i] = fl[i] ? 1 : 0;
+    }
+  });
+}
+
+int ns_prover_new(const ns_ccs* s, const ns_pp* pp, ns_prover** out) {
+  return guard([&] {
+    require_device();
+    auto p = std::make_unique<Prover>(*s->s, *pp->pp);
+    *out = new ns_prover{std::move(p)};
+  });
+}
+
+int ns_prove_shard(ns_prover* p, const uint64_t* const* z_steps, size_t n_steps, size_t m_in, uint8_t** proof,
+                   size_t* proof_len, double* seconds) {
+  return guard([&] {
+    if (n_steps == 0) throw Error(1, "need at least one step");
+    std::vector<uint64_t> enc = p->p->prove_shard(z_steps, n_steps, m_in);
+    uint8_t* buf = (uint8_t*)malloc(enc.size() * 8);
+    if (!buf) throw Error(1, "out of host memory");
+    memcpy(buf, enc.data(), enc.size() * 8);
+    *proof = buf;
+    *proof_len = enc.size() * 8;
+    if (seconds) *seconds = p->p->last_seconds;
+  });
+}
+
+int ns_prove_shard_device(ns_prover* p, const uint64_t* const* z_steps_dev, size_t n_steps, size_t m_in,
+                          uint8_t** proof, size_t* proof_len, double* seconds) {
+  return guard([&] {
+    if (n_steps == 0) throw Error(1, "need at least one step");
+    std::vector<uint64_t> enc = p->p->prove_shard(z_steps_dev, n_steps, m_in, true);
+    uint8_t* buf = (uint8_t*)malloc(enc.size() * 8);
+    if (!buf) throw Error(1, "out of host memory");
+    memcpy(buf, enc.data(), enc.size() * 8);
+    *proof = buf;
+    *proof_len = enc.size() * 8;
+    if (seconds) *seconds = p->p->last_seconds;
+  });
+}
+
+int ns_prover_set_timing(ns_prover* p, int on) {
+  return guard([&] { p->p->timing = on != 0; });
+}
+
+int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, size_t n) {
+  return guard([&] {
+    for (size_t i = 0; i < n && i < (size_t)KT_N; i++) {
+      ms[i] = p->p->kernel_ms[i];
+      counts[i] = p->p->kernel_count[i];
+    }
+  });
+}
+
+void ns_prover_free(ns_prover* p) { delete p; }
+
+}  // extern "C"

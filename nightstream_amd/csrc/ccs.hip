@@ -1,0 +1,560 @@
+// ccs.hip -- Pi_CCS row-phase kernels: chi tables, z1 recomposition, sparse products, the
+// Eval column sums, the per-round univariate polynomial and the in-place table folds, plus the
+// digit x K products y[w][j][rho] = sum_c Z_w[rho][c] v_j[c] (precompute_for_r and DEC children).
+//
+// Round polynomial (optimized_engine/oracle.rs:1047-1178, b = 2): for every row pair
+// (2t, 2t+1) and X the current row bit,
+//   eq_beta(X) * ( f(M z1)(X) + sum_{w,rho} wg[w,rho] * N(y_{w,rho}(X)) ) + gamma^k eq_rin(X) Eval(X),
+//   N(y) = y^3 - y, y(X) = lo + (hi - lo) X,
+// summed over all pairs; the kernel returns the deg_max + 1 coefficients (low -> high).
+// Round 0 reads the int8 digit matrices directly: with lo, hi in {-1,0,1},
+//   N(y(X)) = lo A(X) + hi B(X) + 3 lo^2 hi C(X) + 3 lo hi^2 D(X),
+//   A = -2X + 3X^2 - X^3, B = X^3 - X, C = X - 2X^2 + X^3, D = X^2 - X^3,
+// so the NC part of a pair needs only signed selections of wg (no products).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ns {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ---------------- chi ----------------
+__global__ void k_chi(const K2* __restrict__ bits, int nbits, K2* __restrict__ out) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ((size_t)1 << nbits)) return;
+  K2 w = kone();
+  for (int b = 0; b < nbits; b++) {
+    K2 r = bits[b];
+    w = kmul(w, ((idx >> b) & 1) ? r : ksub(kone(), r));
+  }
+  out[idx] = w;
+}
+
+void launch_chi(const K2* bits, int nbits, K2* out, hipStream_t s) {
+  size_t len = (size_t)1 << nbits;
+  hipLaunchKernelGGL(k_chi, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, bits, nbits, out);
+}
+
+// ---------------- z1 ----------------
+__global__ void k_z1(const int8_t* __restrict__ Z, size_t m, K2* __restrict__ z1) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  int64_t acc = 0;
+  for (int rho = D - 1; rho >= 0; rho--) acc = 2 * acc + Z[(size_t)rho * m + c];
+  z1[c] = K2{from_i64(acc), 0};
+}
+
+void launch_z1(const int8_t* Z1, size_t m, K2* z1, hipStream_t s) {
+  hipLaunchKernelGGL(k_z1, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, Z1, m, z1);
+}
+
+// ---------------- sparse products ----------------
+__global__ void k_spmv(const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ col,
+                       const uint64_t* __restrict__ val, uint32_t n, const K2* __restrict__ x, K2* __restrict__ y,
+                       K2 coeff, int accumulate) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  K2 acc = kzero();
+  for (uint32_t k = ptr[r]; k < ptr[r + 1]; k++) acc = kadd(acc, kscale(x[col[k]], val[k]));
+  acc = kmul(acc, coeff);
+  y[r] = accumulate ? kadd(y[r], acc) : acc;
+}
+
+__global__ void k_copy_scaled(const K2* __restrict__ x, size_t n, K2* __restrict__ y, K2 coeff, int accumulate) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  K2 v = kmul(x[r], coeff);
+  y[r] = accumulate ? kadd(y[r], v) : v;
+}
+
+void launch_spmv(const DevSparse& M, const K2* x, K2* y, K2 coeff, bool accumulate, hipStream_t s) {
+  dim3 g((M.n + 255) / 256);
+  if (M.identity) {
+    size_t cap = M.n < M.m ? M.n : M.m;
+    hipLaunchKernelGGL(k_copy_scaled, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, x, cap, y, coeff,
+                       accumulate ? 1 : 0);
+    return;
+  }
+  hipLaunchKernelGGL(k_spmv, g, dim3(256), 0, s, M.csr_ptr, M.csr_col, M.csr_val, M.n, x, y, coeff,
+                     accumulate ? 1 : 0);
+}
+
+__global__ void k_spmv_t(const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ row,
+                         const uint64_t* __restrict__ val, uint32_t m, const K2* __restrict__ chi, uint32_t n_eff,
+                         K2* __restrict__ v) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  K2 acc = kzero();
+  for (uint32_t k = ptr[c]; k < ptr[c + 1]; k++) {
+    uint32_t r = row[k];
+    if (r < n_eff) acc = kadd(acc, kscale(chi[r], val[k]));
+  }
+  v[c] = acc;
+}
+
+__global__ void k_copy_prefix(const K2* __restrict__ x, size_t cap, size_t m, K2* __restrict__ v) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  v[c] = c < cap ? x[c] : kzero();
+}
+
+void launch_spmv_t(const DevSparse& M, const K2* chi, size_t n_eff, K2* v, hipStream_t s) {
+  dim3 g((M.m + 255) / 256);
+  if (M.identity) {
+    size_t cap = M.m < n_eff ? M.m : n_eff;
+    hipLaunchKernelGGL(k_copy_prefix, g, dim3(256), 0, s, chi, cap, (size_t)M.m, v);
+    return;
+  }
+  hipLaunchKernelGGL(k_spmv_t, g, dim3(256), 0, s, M.csc_ptr, M.csc_row, M.csc_val, M.m, chi, (uint32_t)n_eff, v);
+}
+
+// ---------------- Eval column sums ----------------
+__global__ void k_weighted_colsum(const int8_t* const* __restrict__ Zs, int nw, const K2* __restrict__ W, size_t m,
+                                  K2* __restrict__ S) {
+  __shared__ K2 w_s[16 * D];
+  for (int i = threadIdx.x; i < nw * D; i += blockDim.x) w_s[i] = W[i];
+  __syncthreads();
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  K2 pos = kzero(), neg = kzero();
+  for (int w = 0; w < nw; w++) {
+    const int8_t* z = Zs[w] + c;
+    for (int rho = 0; rho < D; rho++) {
+      int8_t d = z[(size_t)rho * m];
+      if (d > 0) pos = kadd(pos, w_s[w * D + rho]);
+      else if (d < 0) neg = kadd(neg, w_s[w * D + rho]);
+    }
+  }
+  S[c] = ksub(pos, neg);
+}
+
+void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const K2* W, size_t m, K2* S,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_weighted_colsum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, Zs_dev, (int)nw, W, m,
+                     S);
+}
+
+// ---------------- y = Z v via int8 MFMA on byte limbs of v ----------------
+// Vl[(j*2 + comp)*8 + l][c] = byte_l(v_j[c].comp) - 128 (64 rows per 4 matrices).
+__global__ void k_v_limbs(const K2* __restrict__ v, size_t t, size_t m, int8_t* __restrict__ Vl) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t j = blockIdx.y;
+  if (c >= m) return;
+  K2 x = v[j * m + c];
+  uint64_t comp[2] = {x.c0, x.c1};
+#pragma unroll
+  for (int cp = 0; cp < 2; cp++)
+#pragma unroll
+    for (int l = 0; l < 8; l++)
+      Vl[((j * 2 + cp) * 8 + l) * m + c] = (int8_t)((int)((comp[cp] >> (8 * l)) & 0xFF) - 128);
+}
+
+__global__ void k_digit_rowsum2(const int8_t* const* __restrict__ Zs, size_t m, int32_t* __restrict__ S) {
+  size_t w = blockIdx.x / D, t = blockIdx.x % D;
+  const int8_t* row = Zs[w] + t * m;
+  int32_t acc = 0;
+  for (size_t j = threadIdx.x; j < m; j += blockDim.x) acc += row[j];
+  __shared__ int32_t red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S[blockIdx.x] = red[0];
+}
+
+constexpr int YD_SPLIT_MAX = 64;
+
+// grid: x = 64-row tile of digit rows (w, rho), y = K split.  N = 16 * t limb rows (t <= 8).
+template <int NS>
+__global__ __launch_bounds__(256) void k_ydigits_gemm(const int8_t* const* __restrict__ Zs, size_t nrows,
+                                                      const int8_t* __restrict__ Vl, size_t m, size_t kc,
+                                                      int32_t* __restrict__ part, size_t mpad) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t r0 = (size_t)blockIdx.x * 64 + wave * 16;
+  const size_t k_begin = (size_t)blockIdx.y * kc;
+  const int kq = (lane >> 4) * 16, r16 = lane & 15;
+  const size_t myrow = r0 + r16;
+  const bool aval = myrow < nrows;
+  const int8_t* arow = aval ? Zs[myrow / D] + (myrow % D) * m + k_begin + kq : nullptr;
+  const int8_t* brow[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) brow[n] = Vl + (size_t)(n * 16 + r16) * m + k_begin + kq;
+  v4i acc[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) acc[n] = v4i{0, 0, 0, 0};
+  for (size_t k = 0; k < kc; k += 64) {
+    v4i a = aval ? *(const v4i*)(arow + k) : v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      v4i b = *(const v4i*)(brow[n] + k);
+      acc[n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[n], 0, 0, 0);
+    }
+  }
+  int32_t* base = part + ((size_t)blockIdx.y * mpad + r0) * (NS * 16);
+#pragma unroll
+  for (int n = 0; n < NS; n++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) base[((lane >> 4) * 4 + v) * (NS * 16) + n * 16 + r16] = acc[n][v];
+}
+
+// thread per (row, j): y[row][j] = sum_comp,l 256^l (G + 128 rowsum)
+__global__ void k_ydigits_epilogue(const int32_t* __restrict__ part, size_t splits, size_t mpad, size_t nrows,
+                                   size_t t, const int32_t* __restrict__ rowsum, K2* __restrict__ y) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nrows * t) return;
+  size_t row = idx / t, j = idx % t;
+  const size_t ncols = 16 * t;
+  uint64_t comp[2];
+  for (int cp = 0; cp < 2; cp++) {
+    uint64_t acc = 0;
+    for (int l = 0; l < 8; l++) {
+      int64_t g = 0;
+      size_t col = (j * 2 + cp) * 8 + l;
+      for (size_t sp = 0; sp < splits; sp++) g += part[(sp * mpad + row) * ncols + col];
+      g += 128 * (int64_t)rowsum[row];
+      acc = fadd(acc, fmul(from_i64(g), 1ull << (8 * l)));
+    }
+    comp[cp] = acc;
+  }
+  // output layout y[w][j][rho]
+  size_t w = row / D, rho = row % D;
+  y[(w * t + j) * D + rho] = K2{comp[0], comp[1]};
+}
+
+static size_t yd_splits(size_t m) {
+  size_t s = m / 2048;
+  if (s < 1) s = 1;
+  if (s > YD_SPLIT_MAX) s = YD_SPLIT_MAX;
+  while (s > 1 && (m / s) % 64 != 0) s--;
+  return s;
+}
+
+size_t ydigits_ws_bytes(size_t nw, size_t t, size_t m) {
+  size_t nrows = nw * D, mpad = (nrows + 63) / 64 * 64;
+  return yd_splits(m) * mpad * 16 * t * 4 + 16 * t * m + nrows * 4 + 1024;
+}
+
+void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t t, size_t m, K2* y, void* ws,
+                    hipStream_t s) {
+  const size_t nrows = nw * D, mpad = (nrows + 63) / 64 * 64, splits = yd_splits(m), kc = m / splits;
+  int32_t* part = (int32_t*)ws;
+  int8_t* Vl = (int8_t*)(part + splits * mpad * 16 * t);
+  int32_t* rowsum = (int32_t*)(((uintptr_t)(Vl + 16 * t * m) + 255) & ~(uintptr_t)255);
+  hipLaunchKernelGGL(k_v_limbs, dim3((unsigned)((m + 255) / 256), (unsigned)t), dim3(256), 0, s, v, t, m, Vl);
+  hipLaunchKernelGGL(k_digit_rowsum2, dim3((unsigned)nrows), dim3(256), 0, s, Zs_dev, m, rowsum);
+  dim3 g((unsigned)(mpad / 64), (unsigned)splits);
+  ktimer_begin(KT_YD_GEMM, s);
+  switch (t) {
+    case 1: hipLaunchKernelGGL(k_ydigits_gemm<1>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    case 2: hipLaunchKernelGGL(k_ydigits_gemm<2>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    case 3: hipLaunchKernelGGL(k_ydigits_gemm<3>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    case 4: hipLaunchKernelGGL(k_ydigits_gemm<4>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    case 5: hipLaunchKernelGGL(k_ydigits_gemm<5>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    case 6: hipLaunchKernelGGL(k_ydigits_gemm<6>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    case 7: hipLaunchKernelGGL(k_ydigits_gemm<7>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    default: hipLaunchKernelGGL(k_ydigits_gemm<8>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+  }
+  ktimer_end(KT_YD_GEMM, s);
+  size_t nt = nrows * t;
+  hipLaunchKernelGGL(k_ydigits_epilogue, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, part, splits, mpad,
+                     nrows, t, rowsum, y);
+}
+
+// ---------------- round polynomial ----------------
+constexpr int RB = 256;        // threads per block
+constexpr int RMAXBLK = 2048;  // partial slots
+
+struct RoundArgs {
+  size_t len;
+  const K2* eqb;
+  const K2* eqr;
+  const K2* ev;
+  const K2* fv;
+  int nf;
+  const K2* nc;
+  const int8_t* const* digits;
+  size_t m;
+  int k;
+  K2 gamma_k;
+  FPoly f;
+};
+
+__device__ __forceinline__ K2 wave_reduce(K2 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    K2 o;
+    o.c0 = __shfl_down(v.c0, off, 64);
+    o.c1 = __shfl_down(v.c1, off, 64);
+    v = kadd(v, o);
+  }
+  return v;
+}
+
+// coefficient vector of width 5 (deg_max = 4)
+template <bool ROUND0>
+__global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __restrict__ wg, K2* __restrict__ partials) {
+  extern __shared__ K2 sh[];  // wg cache [k*54] then reduction scratch
+  const int nrow = A.k * D;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) sh[i] = wg[i];
+  __syncthreads();
+  const size_t pairs = A.len / 2;
+  K2 c[5];
+#pragma unroll
+  for (int d = 0; d < 5; d++) c[d] = kzero();
+  K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < pairs; t += (size_t)gridDim.x * blockDim.x) {
+    const K2 e0 = A.eqb[2 * t];
+    const K2 e1 = ksub(A.eqb[2 * t + 1], e0);
+    K2 in[5];
+#pragma unroll
+    for (int d = 0; d < 5; d++) in[d] = kzero();
+    // f(M z1)(X)
+    for (int tt = 0; tt < A.f.nterms; tt++) {
+      K2 p[5];
+      p[0] = K2{A.f.coeff[tt], 0};
+#pragma unroll
+      for (int d = 1; d < 5; d++) p[d] = kzero();
+      for (int v = 0; v < A.f.nv[tt]; v++) {
+        const K2* tbl = A.fv + (size_t)A.f.var[tt][v] * A.len;
+        K2 a = tbl[2 * t], b = ksub(tbl[2 * t + 1], a);
+        for (int e = 0; e < A.f.exp[tt][v]; e++) {
+#pragma unroll
+          for (int d = 4; d >= 1; d--) p[d] = kadd(kmul(a, p[d]), kmul(b, p[d - 1]));
+          p[0] = kmul(a, p[0]);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
+    }
+    if (ROUND0) {
+      // signed selections of wg
+      K2 slo = kzero(), shi = kzero(), s3 = kzero(), s4 = kzero();
+      for (int w = 0; w < A.k; w++) {
+        const int8_t* z = A.digits[w] + 2 * t;
+        for (int rho = 0; rho < D; rho++) {
+          int lo = z[(size_t)rho * A.m], hi = z[(size_t)rho * A.m + 1];
+          if ((lo | hi) == 0) continue;
+          K2 wv = sh[w * D + rho];
+          K2 nw = kneg(wv);
+          if (lo) slo = kadd(slo, lo > 0 ? wv : nw);
+          if (hi) shi = kadd(shi, hi > 0 ? wv : nw);
+          if (lo && hi) {
+            s3 = kadd(s3, hi > 0 ? wv : nw);  // lo^2 hi = hi
+            s4 = kadd(s4, lo > 0 ? wv : nw);  // lo hi^2 = lo
+          }
+        }
+      }
+      // c1 = -2 slo - shi + 3 s3 ; c2 = 3 slo - 6 s3 + 3 s4 ; c3 = -slo + shi + 3 s3 - 3 s4
+      K2 s3x3 = kadd(kadd(s3, s3), s3);
+      K2 s4x3 = kadd(kadd(s4, s4), s4);
+      K2 slo2 = kadd(slo, slo);
+      in[1] = kadd(in[1], ksub(ksub(s3x3, slo2), shi));
+      in[2] = kadd(in[2], ksub(kadd(kadd(slo2, slo), s4x3), kadd(s3x3, s3x3)));
+      in[3] = kadd(in[3], ksub(kadd(shi, s3x3), kadd(slo, s4x3)));
+    } else {
+      K2 s1 = kzero(), s2 = kzero(), s3 = kzero(), s4 = kzero(), s5 = kzero(), s6 = kzero();
+      for (int r = 0; r < nrow; r++) {
+        const K2* row = A.nc + (size_t)r * A.len + 2 * t;
+        K2 a = row[0];
+        K2 b = ksub(row[1], a);
+        K2 wv = sh[r];
+        K2 wa = kmul(wv, a), wb = kmul(wv, b);
+        K2 a2 = ksqr(a), b2 = ksqr(b);
+        s1 = kadd(s1, kmul(wa, a2));
+        s2 = kadd(s2, wa);
+        s3 = kadd(s3, kmul(wb, a2));
+        s4 = kadd(s4, wb);
+        s5 = kadd(s5, kmul(wa, b2));
+        s6 = kadd(s6, kmul(wb, b2));
+      }
+      in[0] = kadd(in[0], ksub(s1, s2));
+      in[1] = kadd(in[1], ksub(kadd(kadd(s3, s3), s3), s4));
+      in[2] = kadd(in[2], kadd(kadd(s5, s5), s5));
+      in[3] = kadd(in[3], s6);
+    }
+    c[0] = kadd(c[0], kmul(e0, in[0]));
+#pragma unroll
+    for (int d = 1; d < 5; d++) c[d] = kadd(c[d], kadd(kmul(e0, in[d]), kmul(e1, in[d - 1])));
+    if (A.eqr) {
+      K2 r0 = A.eqr[2 * t], r1 = ksub(A.eqr[2 * t + 1], r0);
+      K2 v0 = A.ev[2 * t], v1 = ksub(A.ev[2 * t + 1], v0);
+      ev0 = kadd(ev0, kmul(r0, v0));
+      ev1 = kadd(ev1, kadd(kmul(r0, v1), kmul(r1, v0)));
+      ev2 = kadd(ev2, kmul(r1, v1));
+    }
+  }
+  if (A.eqr) {
+    c[0] = kadd(c[0], kmul(A.gamma_k, ev0));
+    c[1] = kadd(c[1], kmul(A.gamma_k, ev1));
+    c[2] = kadd(c[2], kmul(A.gamma_k, ev2));
+  }
+  // block reduction
+  K2* red = sh + nrow;  // [4 waves][5]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < 5; d++) {
+    K2 v = wave_reduce(c[d]);
+    if (lane == 0) red[wave * 5 + d] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    K2 v = kzero();
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = kadd(v, red[w * 5 + threadIdx.x]);
+    partials[blockIdx.x * 5 + threadIdx.x] = v;
+  }
+}
+
+__global__ void k_sum_partials(const K2* __restrict__ partials, int nblk, K2* __restrict__ out) {
+  int d = threadIdx.x;
+  if (d >= 5) return;
+  K2 v = kzero();
+  for (int b = 0; b < nblk; b++) v = kadd(v, partials[b * 5 + d]);
+  out[d] = v;
+}
+
+size_t round_ws_bytes() { return (size_t)RMAXBLK * 5 * sizeof(K2); }
+
+void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool round0, K2* out, void* ws,
+                       hipStream_t s) {
+  RoundArgs A;
+  A.len = T.len;
+  A.eqb = T.eqb;
+  A.eqr = T.eqr;
+  A.ev = T.ev;
+  A.fv = T.fv;
+  A.nf = T.nf;
+  A.nc = T.nc;
+  A.digits = T.digits;
+  A.m = T.m;
+  A.k = T.k;
+  A.gamma_k = T.gamma_k;
+  A.f = f;
+  size_t pairs = T.len / 2;
+  size_t nblk = (pairs + RB - 1) / RB;
+  if (nblk > RMAXBLK) nblk = RMAXBLK;
+  if (nblk < 1) nblk = 1;
+  size_t shmem = ((size_t)T.k * D + 4 * 5) * sizeof(K2);
+  K2* partials = (K2*)ws;
+  ktimer_begin(round0 ? KT_ROUND0 : KT_ROUND, s);
+  if (round0)
+    hipLaunchKernelGGL(k_round_eval<true>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials);
+  else
+    hipLaunchKernelGGL(k_round_eval<false>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials);
+  ktimer_end(round0 ? KT_ROUND0 : KT_ROUND, s);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partials, (int)nblk, out);
+}
+
+// ---------------- folds ----------------
+__global__ void k_fold_k(const K2* __restrict__ src, size_t len, K2* __restrict__ dst, K2 r) {
+  size_t half = len / 2;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t row = blockIdx.y;
+  if (i >= half) return;
+  const K2* s = src + row * len;
+  K2 lo = s[2 * i], hi = s[2 * i + 1];
+  dst[row * half + i] = kadd(lo, kmul(ksub(hi, lo), r));
+}
+
+void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStream_t s) {
+  size_t half = len / 2;
+  if (rows == 0 || half == 0) return;
+  ktimer_begin(KT_FOLD, s);
+  hipLaunchKernelGGL(k_fold_k, dim3((unsigned)((half + 255) / 256), (unsigned)rows), dim3(256), 0, s, src, len, dst,
+                     r);
+  ktimer_end(KT_FOLD, s);
+}
+
+__global__ void k_fold_digits(const int8_t* const* __restrict__ digits, size_t m, size_t len, K2* __restrict__ dst,
+                              K2 r) {
+  size_t half = len / 2;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t row = blockIdx.y;  // w*54 + rho
+  if (i >= half) return;
+  const int8_t* z = digits[row / D] + (row % D) * m;
+  // rows beyond the witness width (m < len) are zero (RowStreamState::build pads to 2^ell_n)
+  int lo = (2 * i < m) ? z[2 * i] : 0;
+  int hi = (2 * i + 1 < m) ? z[2 * i + 1] : 0;
+  K2 v = K2{from_i64(lo), 0};
+  v = kadd(v, kmul_small(r, hi - lo));
+  dst[row * half + i] = v;
+}
+
+void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r, hipStream_t s) {
+  size_t half = len / 2;
+  hipLaunchKernelGGL(k_fold_digits, dim3((unsigned)((half + 255) / 256), (unsigned)(k * D)), dim3(256), 0, s,
+                     digits_dev, m, len, dst, r);
+}
+
+// ---------------- Pi_RLC / Pi_DEC ----------------
+__global__ __launch_bounds__(256) void k_rlc_zmix(const int8_t* __restrict__ rho, const int8_t* const* __restrict__ Zs,
+                                                  int k, size_t m, int32_t* __restrict__ zmix) {
+  extern __shared__ int8_t rho_s[];  // [i][kk][r]
+  for (int idx = threadIdx.x; idx < k * D * D; idx += blockDim.x) {
+    int i = idx / (D * D), rem = idx % (D * D), kk = rem / D, r = rem % D;
+    rho_s[idx] = rho[(size_t)i * D * D + r * D + kk];
+  }
+  __syncthreads();
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  int32_t acc[D];
+#pragma unroll
+  for (int r = 0; r < D; r++) acc[r] = 0;
+  for (int i = 0; i < k; i++) {
+    const int8_t* z = Zs[i] + c;
+    for (int kk = 0; kk < D; kk++) {
+      int d = z[(size_t)kk * m];
+      if (d == 0) continue;
+      const int8_t* col = rho_s + (i * D + kk) * D;
+#pragma unroll
+      for (int r = 0; r < D; r++) acc[r] += d * (int)col[r];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < D; r++) zmix[(size_t)r * m + c] = acc[r];
+}
+
+void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
+                     hipStream_t s) {
+  size_t shmem = k * D * D;
+  ktimer_begin(KT_ZMIX, s);
+  hipLaunchKernelGGL(k_rlc_zmix, dim3((unsigned)((m + 255) / 256)), dim3(256), shmem, s, rho_dev, Zs_dev, (int)k, m,
+                     zmix);
+  ktimer_end(KT_ZMIX, s);
+}
+
+__global__ void k_dec_split(const int32_t* __restrict__ zmix, size_t len, int k_dec, int8_t* const* __restrict__ outs,
+                            int* __restrict__ flags) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= len) return;
+  int64_t v = zmix[idx];
+  int64_t B = (int64_t)1 << k_dec;
+  unsigned nz = 0;
+  bool bad = v >= B || v <= -B;
+  for (int i = 0; i < k_dec; i++) {
+    int64_t r = v % 2;
+    outs[i][idx] = (int8_t)r;
+    if (r) nz |= 1u << i;
+    v = (v - r) / 2;
+  }
+  if (v != 0) bad = true;
+  // one flag word per digit plane; atomics on integer flags are exact
+  if (nz) {
+    for (int i = 0; i < k_dec; i++)
+      if ((nz >> i) & 1) {
+        if (flags[i] == 0) atomicOr(&flags[i], 1);
+      }
+  }
+  if (bad) atomicOr(&flags[k_dec], 1);
+}
+
+void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const* outs_dev, int* flags, hipStream_t s) {
+  ktimer_begin(KT_SPLIT, s);
+  hipLaunchKernelGGL(k_dec_split, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, zmix, len, k_dec, outs_dev,
+                     flags);
+  ktimer_end(KT_SPLIT, s);
+}
+
+}  // namespace ns

@@ -1,0 +1,121 @@
+// kernels.h -- launchers for the gfx950 kernels of the Neo folding hot path.
+//
+// All launchers are asynchronous on the given stream.  Device buffers are plain pointers;
+// digit matrices are int8 row-major D x m (Z[rho*m + c]); K elements are ns::K2 (16 bytes).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "field.h"
+
+namespace ns {
+
+constexpr int D = 54;      // ring degree, neo-math/src/ring.rs:14
+constexpr int DPAD = 64;   // 2^ell_d
+constexpr int KAPPA = 16;  // Ajtai rank (Goldilocks preset)
+constexpr int MAX_F_VARS = 8;
+constexpr int MAX_F_TERMS = 8;
+constexpr int MAX_DM = 7;  // max coefficient index of a row-phase round polynomial
+
+#define NS_CHECK(x)                                                                              \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess) throw ns::HipError(e_, #x, __FILE__, __LINE__);                        \
+  } while (0)
+
+struct HipError {
+  hipError_t err;
+  const char* what;
+  const char* file;
+  int line;
+  HipError(hipError_t e, const char* w, const char* f, int l) : err(e), what(w), file(f), line(l) {}
+};
+
+// Per-kernel timing hooks (HIP events on the launch stream); no-ops unless a prover enabled
+// timing on the calling thread.
+enum KernelClass { KT_ROUND0 = 0, KT_ROUND, KT_FOLD, KT_AJTAI_GEMM, KT_YD_GEMM, KT_ZMIX, KT_SPLIT, KT_N };
+void ktimer_begin(int cls, hipStream_t s);
+void ktimer_end(int cls, hipStream_t s);
+
+// Sparse matrix on the device: CSR (rows) for M x and CSC (columns) for M^T x.
+struct DevSparse {
+  uint32_t n = 0, m = 0, nnz = 0;
+  bool identity = false;
+  uint32_t* csr_ptr = nullptr;
+  uint32_t* csr_col = nullptr;
+  uint64_t* csr_val = nullptr;
+  uint32_t* csc_ptr = nullptr;
+  uint32_t* csc_row = nullptr;
+  uint64_t* csc_val = nullptr;
+};
+
+// Compiled CCS polynomial f for the round kernels: sum_t coeff_t * prod_v x_{var}^{exp}.
+struct FPoly {
+  int nterms = 0;
+  uint64_t coeff[MAX_F_TERMS];
+  int nv[MAX_F_TERMS];
+  int var[MAX_F_TERMS][4];
+  int exp[MAX_F_TERMS][4];
+  int deg_max = 4;  // max(4, deg f + 1)
+};
+
+// ---- decomposition (neo-ajtai/src/decomp.rs:118-158, b = 2 balanced) ----
+void launch_decomp_b2(const uint64_t* z, size_t m, int8_t* Z, hipStream_t s);
+
+// ---- Ajtai ----
+// PP coefficients [kappa][m][54] (u64, device) -> int8 limb planes [(i*8+l)*54+u][m] (value - 128).
+void launch_pp_limbs(const uint64_t* pp, size_t kappa, size_t m, int8_t* limbs, hipStream_t s);
+// ChaCha8 keystream for setup(): words [n_words] from (key, counter0); see ajtai.hip.
+void launch_chacha8_words(const uint32_t key[8], uint64_t first_block, size_t nblocks, uint32_t* out, hipStream_t s);
+size_t ajtai_workspace_bytes(size_t m, size_t nw);
+// Commit nw digit matrices (device pointers in Zs_dev, a device array of nw pointers).
+void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* const* Zs_dev, size_t nw,
+                  uint64_t* out /* nw x kappa x 54 */, void* workspace, hipStream_t s);
+
+// ---- chi tables, matrix-vector products ----
+void launch_chi(const K2* bits, int nbits, K2* out, hipStream_t s);  // len 2^nbits
+void launch_z1(const int8_t* Z1, size_t m, K2* z1, hipStream_t s);
+// y[r] = coeff * sum_k val[k] x[col[k]] (+ y[r] if accumulate), r < n
+void launch_spmv(const DevSparse& M, const K2* x, K2* y, K2 coeff, bool accumulate, hipStream_t s);
+// v[c] = sum_{k in col c, row < n_eff} val[k] chi[row]
+void launch_spmv_t(const DevSparse& M, const K2* chi, size_t n_eff, K2* v, hipStream_t s);
+// S[c] = sum_{w, rho} W[w*54+rho] * Z_w[rho][c]
+void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const K2* W, size_t m, K2* S,
+                                  hipStream_t s);
+// y[w][j][rho] = sum_c Z_w[rho][c] v_j[c]  (w < nw, j < t); partial workspace sized by ydigits_ws_bytes
+size_t ydigits_ws_bytes(size_t nw, size_t t, size_t m);
+void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t t, size_t m, K2* y, void* ws,
+                    hipStream_t s);
+
+// ---- Pi_CCS row phase ----
+struct RowTables {
+  size_t len;         // current table length L (power of two)
+  const K2* eqb;      // [L]
+  const K2* eqr;      // [L] or null
+  const K2* ev;       // [L] or null
+  const K2* fv;       // [nf][L]
+  int nf;
+  const K2* nc;       // [k*54][L] (rounds >= 1)
+  const int8_t* const* digits;  // round 0: k device pointers to D x m digit matrices
+  size_t m;           // digit row stride (round 0)
+  int k;
+  K2 gamma_k;
+};
+size_t round_ws_bytes();
+// Writes the deg_max+1 coefficients (low->high) of the round polynomial to out (device).
+void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg /* [k*54] device */, bool round0,
+                       K2* out, void* ws, hipStream_t s);
+// Fold every table with challenge r: dst[i] = src[2i] + r (src[2i+1] - src[2i]).
+void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStream_t s);
+void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r, hipStream_t s);
+
+// ---- Pi_RLC / Pi_DEC ----
+// zmix[r][c] = sum_i sum_kk rho_i[r][kk] Z_i[kk][c]; rho given as int8 [k][54][54]
+void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
+                     hipStream_t s);
+// balanced base-2 digits of zmix into k_dec planes; flags[i] |= plane i nonzero; flags[k_dec] = range error
+void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const* outs_dev, int* flags,
+                      hipStream_t s);
+
+}  // namespace ns

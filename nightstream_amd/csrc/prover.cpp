@@ -1,0 +1,1267 @@
+// prover.cpp -- MI355X shard prover: host fold loop over device-resident state.
+//
+// Restates, in transcript order, neo-fold/src/shard.rs:1725-2283 (fold_shard_prove_impl) for
+// CCS-only steps, with the Pi_CCS round oracle, precompute_for_r, the Ajtai commitment,
+// Pi_RLC's Z_mix and Pi_DEC's digit split and child products on the GPU.  Host-side pieces:
+// transcript (neo-transcript), sum-check drivers and Lagrange interpolation
+// (neo-reductions/src/sumcheck.rs:79-209, 350-456), Ajtai-bit rounds
+// (optimized_engine/oracle.rs:1812-1951), ME outputs (oracle.rs:1955-2072), Pi_RLC on X, y and
+// commitments (optimized_engine/common.rs:1221-1379, session.rs:301-324) and the DEC checks
+// (optimized_engine/common.rs:1575-1628).
+#include "prover.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+namespace ns {
+
+// ---------------- device memory ----------------
+void DevBuf::alloc(size_t n) {
+  release();
+  if (n == 0) n = 256;
+  NS_CHECK(hipMalloc(&p, n));
+  bytes = n;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+template <class T>
+static void h2d(void* dst, const T* src, size_t n, hipStream_t s) {
+  NS_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+static void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
+  NS_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  NS_CHECK(hipStreamSynchronize(s));
+}
+
+// ---------------- kernel timers ----------------
+namespace {
+struct KTimers {
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  int open_cls = -1;
+  hipEvent_t open_ev = nullptr;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      NS_CHECK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void collect(double* ms, uint64_t* cnt) {
+    for (int c = 0; c < KT_N; c++) {
+      ms[c] = 0;
+      cnt[c] = 0;
+    }
+    for (auto& r : recs) {
+      float t = 0;
+      NS_CHECK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.cls] += t;
+      cnt[r.cls]++;
+    }
+  }
+  ~KTimers() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+thread_local KTimers* g_kt = nullptr;
+}  // namespace
+
+void ktimer_begin(int cls, hipStream_t s) {
+  if (!g_kt) return;
+  g_kt->open_cls = cls;
+  g_kt->open_ev = g_kt->get();
+  NS_CHECK(hipEventRecord(g_kt->open_ev, s));
+}
+
+void ktimer_end(int cls, hipStream_t s) {
+  if (!g_kt || g_kt->open_cls != cls) return;
+  hipEvent_t e = g_kt->get();
+  NS_CHECK(hipEventRecord(e, s));
+  g_kt->recs.push_back({cls, g_kt->open_ev, e});
+  g_kt->open_cls = -1;
+}
+
+// ---------------- params (neo-params/src/lib.rs) ----------------
+typedef unsigned __int128 u128;
+static uint32_t bitlen128(u128 x) {
+  uint32_t n = 0;
+  while (x) {
+    n++;
+    x >>= 1;
+  }
+  return n;
+}
+
+bool Params::extension_check(uint32_t ell, uint32_t d_sc, uint32_t* s_min_out, int32_t* slack_out) const {
+  u128 ld = (u128)ell * d_sc;
+  uint32_t smin = 3;
+  if (lambda < 128) {
+    u128 pow2 = (u128)1 << lambda;
+    bool ovf = ld != 0 && pow2 > (~(u128)0) / ld;
+    if (!ovf && (u128)q >= pow2 * ld) smin = 1;
+    else if (!ovf && (u128)q * q >= pow2 * ld) smin = 2;
+  }
+  if (smin > 2) return false;
+  u128 q2 = (u128)q * q, rhs = ((u128)1 << lambda) * ld;
+  int32_t slack;
+  if (q2 < rhs) {
+    slack = -1;
+  } else {
+    slack = (int32_t)bitlen128(q2) - (int32_t)bitlen128(rhs);
+    if (slack >= 0 && slack < 128 && q2 < (rhs << (uint32_t)slack)) slack -= 1;
+  }
+  if (s_min_out) *s_min_out = smin;
+  if (slack_out) *slack_out = slack;
+  return true;
+}
+
+Params Params::goldilocks_auto_r1cs_ccs(size_t n_rows) {
+  Params p;
+  size_t padded = 2;
+  if (n_rows) {
+    padded = 1;
+    while (padded < n_rows) padded <<= 1;
+    if (padded < 2) padded = 2;
+  }
+  u128 prod = (u128)p.d * padded;
+  uint32_t ell = prod <= 1 ? 0 : bitlen128(prod - 1);
+  uint32_t d_sc = 5;
+  for (uint32_t lam = p.lambda; lam >= 96; lam--) {
+    p.lambda = lam;
+    int32_t slack;
+    uint32_t sm;
+    if (p.extension_check(ell, d_sc, &sm, &slack) && slack >= 2) return p;
+  }
+  throw Error(2, "goldilocks_auto_r1cs_ccs: extension degree s >= 3 required");
+}
+
+// ---------------- CCS ----------------
+size_t Ccs::f_degree() const {
+  size_t d = 0;
+  for (auto& e : term_exps) {
+    size_t s = 0;
+    for (auto x : e) s += x;
+    d = std::max(d, s);
+  }
+  return d;
+}
+
+static void ccs_digest(const Ccs& s, uint64_t out[4]) {
+  // neo-reductions/src/engines/utils.rs:214-321
+  const Poseidon2& P = digest_perm();
+  uint64_t st[16] = {0};
+  int absorbed = 0;
+  const char* dom = "neo/ccs/matrices/v1";
+  for (const char* c = dom; *c; c++) {
+    if (absorbed >= 15) {
+      P.permute(st);
+      absorbed = 0;
+    }
+    st[absorbed++] = (uint8_t)*c;
+  }
+  if (absorbed + 3 >= 16) {
+    P.permute(st);
+    absorbed = 0;
+  }
+  st[absorbed] = s.n;
+  st[absorbed + 1] = s.m;
+  st[absorbed + 2] = s.t;
+  P.permute(st);
+  auto emit = [&](uint64_t r, uint64_t c, uint64_t v) {
+    if (absorbed + 3 > 15) {
+      P.permute(st);
+      absorbed = 0;
+    }
+    st[absorbed] = r % Q;
+    st[absorbed + 1] = c % Q;
+    st[absorbed + 2] = v % Q;
+    absorbed += 3;
+  };
+  for (size_t j = 0; j < s.t; j++) {
+    absorbed = 0;
+    st[0] = j;
+    absorbed = 1;
+    if (j == 0) {
+      for (size_t r = 0; r < s.n; r++) emit(r, r, 1);
+    } else {
+      // row-major nonzero order with ascending columns inside a row
+      const HostCsr& M = s.host[j];
+      std::vector<std::pair<uint32_t, uint64_t>> row;
+      for (size_t r = 0; r < s.n; r++) {
+        row.clear();
+        for (uint64_t k = M.ptr[r]; k < M.ptr[r + 1]; k++)
+          if (M.val[k] != 0) row.emplace_back(M.col[k], M.val[k]);
+        std::sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (auto& e : row) emit(r, e.first, e.second);
+      }
+    }
+    P.permute(st);
+  }
+  for (int i = 0; i < 4; i++) out[i] = st[i];
+}
+
+std::unique_ptr<Ccs> Ccs::from_r1cs(size_t n, size_t m, const HostCsr& a, const HostCsr& b, const HostCsr& c) {
+  // neo-ccs/src/r1cs.rs:13-43 (identity-first, f = X1 X2 - X3 with a leading dummy variable)
+  if (n != m) throw Error(1, "Pi_CCS requires a square CCS (n == m) with M_0 = I");
+  if (n < 2 || (n & (n - 1)) != 0) throw Error(1, "n must be a power of two >= 2 on this backend");
+  auto s = std::make_unique<Ccs>();
+  s->n = n;
+  s->m = m;
+  s->t = 4;
+  s->host.resize(4);
+  s->host[1] = a;
+  s->host[2] = b;
+  s->host[3] = c;
+  s->term_coeff = {1, Q - 1};
+  s->term_exps = {{0, 1, 1, 0}, {0, 0, 0, 1}};
+  // compile f over used variables
+  std::vector<int> pos(s->t, -1);
+  for (auto& e : s->term_exps)
+    for (size_t j = 0; j < s->t; j++)
+      if (e[j] && pos[j] < 0) pos[j] = 0;
+  int nf = 0;
+  for (size_t j = 0; j < s->t; j++)
+    if (pos[j] == 0) {
+      pos[j] = nf++;
+      s->fvar_mat.push_back((int)j);
+    }
+  FPoly& f = s->fpoly;
+  f.nterms = (int)s->term_coeff.size();
+  for (int tt = 0; tt < f.nterms; tt++) {
+    f.coeff[tt] = s->term_coeff[tt];
+    f.nv[tt] = 0;
+    for (size_t j = 0; j < s->t; j++)
+      if (s->term_exps[tt][j]) {
+        f.var[tt][f.nv[tt]] = pos[j];
+        f.exp[tt][f.nv[tt]] = (int)s->term_exps[tt][j];
+        f.nv[tt]++;
+      }
+  }
+  f.deg_max = (int)std::max<size_t>(4, s->f_degree() + 1);
+  if (f.deg_max != 4) throw Error(1, "f degree > 3 is not supported by the round kernels");
+  // device matrices
+  s->dev.resize(4);
+  s->dev[0].identity = true;
+  s->dev[0].n = (uint32_t)n;
+  s->dev[0].m = (uint32_t)m;
+  for (size_t j = 1; j < 4; j++) {
+    const HostCsr& H = s->host[j];
+    if (H.ptr.size() != n + 1) throw Error(1, "CSR row pointer has wrong length");
+    size_t nnz = H.ptr[n];
+    std::vector<uint32_t> rp(n + 1), cp(m + 1, 0), rows(nnz);
+    std::vector<uint64_t> cv(nnz);
+    for (size_t r = 0; r <= n; r++) rp[r] = (uint32_t)H.ptr[r];
+    for (size_t k = 0; k < nnz; k++) {
+      if (H.col[k] >= m) throw Error(1, "CSR column index out of range");
+      cp[H.col[k] + 1]++;
+    }
+    for (size_t cc = 0; cc < m; cc++) cp[cc + 1] += cp[cc];
+    std::vector<uint32_t> pos2(cp.begin(), cp.end() - 1);
+    for (size_t r = 0; r < n; r++)
+      for (uint64_t k = H.ptr[r]; k < H.ptr[r + 1]; k++) {
+        uint32_t cc = H.col[k];
+        rows[pos2[cc]] = (uint32_t)r;
+        cv[pos2[cc]] = H.val[k];
+        pos2[cc]++;
+      }
+    DevSparse& D_ = s->dev[j];
+    D_.n = (uint32_t)n;
+    D_.m = (uint32_t)m;
+    D_.nnz = (uint32_t)nnz;
+    auto up = [&](const void* src, size_t bytes) {
+      s->bufs.emplace_back(bytes);
+      NS_CHECK(hipMemcpy(s->bufs.back().p, src, bytes, hipMemcpyHostToDevice));
+      return s->bufs.back().p;
+    };
+    D_.csr_ptr = (uint32_t*)up(rp.data(), rp.size() * 4);
+    D_.csr_col = (uint32_t*)up(H.col.data(), std::max<size_t>(nnz, 1) * 4);
+    D_.csr_val = (uint64_t*)up(H.val.data(), std::max<size_t>(nnz, 1) * 8);
+    D_.csc_ptr = (uint32_t*)up(cp.data(), cp.size() * 4);
+    D_.csc_row = (uint32_t*)up(rows.data(), std::max<size_t>(nnz, 1) * 4);
+    D_.csc_val = (uint64_t*)up(cv.data(), std::max<size_t>(nnz, 1) * 8);
+  }
+  ccs_digest(*s, s->digest);
+  return s;
+}
+
+// ---------------- PP ----------------
+__global__ void k_count_rejects(const uint64_t* x, size_t n, unsigned long long* cnt) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && x[i] >= Q) atomicAdd(cnt, 1ull);
+}
+
+static std::unique_ptr<Pp> finish_pp(std::unique_ptr<Pp> pp, bool keep, hipStream_t s) {
+  pp->limbs.alloc((size_t)KAPPA * 8 * D * pp->m);
+  launch_pp_limbs(pp->coeffs.as<uint64_t>(), KAPPA, pp->m, pp->limbs.as<int8_t>(), s);
+  NS_CHECK(hipStreamSynchronize(s));
+  if (!keep) pp->coeffs.release();
+  return pp;
+}
+
+std::unique_ptr<Pp> Pp::setup_seeded(const uint8_t seed[32], size_t m, bool keep_coeffs, hipStream_t s) {
+  // neo-ajtai/src/commit.rs:133-156: one ChaCha8 stream, rows i, columns j, 54 coefficients,
+  // each a next_u64 draw accepted when < q.  Draw k without rejections is keystream words
+  // (2k, 2k+1); the rare rejected draws are compacted on the host.
+  if (m == 0 || m % 64 != 0) throw Error(1, "Ajtai m must be a nonzero multiple of 64");
+  auto pp = std::make_unique<Pp>();
+  pp->m = m;
+  const size_t N = (size_t)KAPPA * m * D, extra = 1024;
+  const size_t draws = N + extra, nblocks = (2 * draws + 15) / 16;
+  DevBuf words(nblocks * 64);
+  ChaCha8 rng(seed);
+  launch_chacha8_words(rng.key, 0, nblocks, words.as<uint32_t>(), s);
+  DevBuf cnt(8);
+  NS_CHECK(hipMemsetAsync(cnt.p, 0, 8, s));
+  hipLaunchKernelGGL(k_count_rejects, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, words.as<uint64_t>(), N,
+                     cnt.as<unsigned long long>());
+  unsigned long long rej = 0;
+  d2h(&rej, cnt.p, 1, s);
+  if (rej == 0) {
+    pp->coeffs = std::move(words);
+  } else {
+    std::vector<uint64_t> x(draws), out;
+    out.reserve(N);
+    d2h(x.data(), words.p, draws, s);
+    size_t used = 0;
+    while (out.size() < N) {
+      if (used == x.size()) {
+        // continue the stream on the host
+        ChaCha8 tail(seed);
+        for (size_t i = 0; i < used; i++) (void)tail.next_u64();
+        while (out.size() < N) {
+          uint64_t v = tail.next_u64();
+          if (v < Q) out.push_back(v);
+        }
+        break;
+      }
+      uint64_t v = x[used++];
+      if (v < Q) out.push_back(v);
+    }
+    pp->coeffs.alloc(N * 8);
+    h2d(pp->coeffs.p, out.data(), N, s);
+  }
+  return finish_pp(std::move(pp), keep_coeffs, s);
+}
+
+std::unique_ptr<Pp> Pp::from_coeffs(const uint64_t* host, size_t m, bool keep_coeffs, hipStream_t s) {
+  if (m == 0 || m % 64 != 0) throw Error(1, "Ajtai m must be a nonzero multiple of 64");
+  auto pp = std::make_unique<Pp>();
+  pp->m = m;
+  pp->coeffs.alloc((size_t)KAPPA * m * D * 8);
+  h2d(pp->coeffs.p, host, (size_t)KAPPA * m * D, s);
+  return finish_pp(std::move(pp), keep_coeffs, s);
+}
+
+// ---------------- host helpers ----------------
+static std::vector<K2> chi_host(const K2* bits, size_t nbits) {
+  size_t len = (size_t)1 << nbits;
+  std::vector<K2> w(len, kzero());
+  w[0] = kone();
+  for (size_t i = 0; i < nbits; i++) {
+    size_t step = (size_t)1 << i;
+    K2 om = ksub(kone(), bits[i]);
+    for (size_t mask = 0; mask < step; mask++) {
+      K2 v = w[mask];
+      w[mask] = kmul(v, om);
+      w[mask + step] = kmul(v, bits[i]);
+    }
+  }
+  return w;
+}
+
+static K2 eq_mask(size_t mask, const K2* pts, size_t n) {
+  K2 p = kone();
+  for (size_t b = 0; b < n; b++) p = kmul(p, ((mask >> b) & 1) ? pts[b] : ksub(kone(), pts[b]));
+  return p;
+}
+
+static K2 eq_lin(K2 a, K2 b) { return kadd(kmul(ksub(kone(), a), ksub(kone(), b)), kmul(a, b)); }
+
+static K2 poly_eval(const K2* c, size_t n, K2 x) {
+  K2 r = c[n - 1];
+  for (size_t i = n - 1; i-- > 0;) r = kadd(kmul(r, x), c[i]);
+  return r;
+}
+
+// Lagrange interpolation to low->high coefficients (sumcheck.rs:79-121)
+static void interpolate(const K2* xs, const K2* ys, size_t n, K2* coeffs) {
+  for (size_t i = 0; i < n; i++) coeffs[i] = kzero();
+  std::vector<K2> numer(n), next(n);
+  for (size_t i = 0; i < n; i++) {
+    std::fill(numer.begin(), numer.end(), kzero());
+    numer[0] = kone();
+    size_t cur = 0;
+    for (size_t j = 0; j < n; j++) {
+      if (i == j) continue;
+      std::fill(next.begin(), next.end(), kzero());
+      for (size_t d = 0; d <= cur; d++) {
+        next[d + 1] = kadd(next[d + 1], numer[d]);
+        next[d] = kadd(next[d], kmul(kneg(xs[j]), numer[d]));
+      }
+      numer = next;
+      cur++;
+    }
+    K2 denom = kone();
+    for (size_t j = 0; j < n; j++)
+      if (i != j) denom = kmul(denom, ksub(xs[i], xs[j]));
+    K2 scale = kmul(ys[i], kinv(denom));
+    for (size_t d = 0; d <= cur; d++) coeffs[d] = kadd(coeffs[d], kmul(scale, numer[d]));
+  }
+}
+
+// ---------------- Ajtai-bit rounds ----------------
+static void fold_bit_inplace(K2* dg, size_t bit, K2 a) {
+  size_t stride = (size_t)1 << bit, step = stride << 1;
+  for (size_t base = 0; base < (size_t)D; base += step)
+    for (size_t off = 0; off < stride; off++) {
+      size_t i0 = base + off;
+      if (i0 >= (size_t)D) break;
+      size_t i1 = i0 + stride;
+      K2 lo = dg[i0], hi = i1 < (size_t)D ? dg[i1] : kzero();
+      dg[i0] = kadd(lo, kmul(ksub(hi, lo), a));
+    }
+}
+
+void AjtaiPhase::init(const Ccs& s, std::vector<K2> y, size_t k, const K2* alpha_, const K2* beta_a_,
+                      const K2* beta_r, K2 gamma_, const std::vector<K2>& r_time, const K2* r_inputs) {
+  kt = k;
+  t = s.t;
+  y_eval = std::move(y);
+  alpha.assign(alpha_, alpha_ + ell_d);
+  beta_a.assign(beta_a_, beta_a_ + ell_d);
+  gamma = gamma_;
+  chals.clear();
+  // F' = f(m_vals), m_vals[j] = sum_rho 2^rho y_eval[0][j][rho] (= <z1, v_j>)
+  std::vector<K2> mv(t, kzero());
+  K2 pw = kone();
+  for (int rho = 0; rho < D; rho++) {
+    for (size_t j = 0; j < t; j++) mv[j] = kadd(mv[j], kmul(y_eval[(0 * t + j) * D + rho], pw));
+    pw = kadd(pw, pw);
+  }
+  f_prime = kzero();
+  for (size_t tt = 0; tt < s.term_coeff.size(); tt++) {
+    K2 term = K2{s.term_coeff[tt], 0};
+    for (size_t j = 0; j < t; j++)
+      for (uint32_t e = 0; e < s.term_exps[tt][j]; e++) term = kmul(term, mv[j]);
+    f_prime = kadd(f_prime, term);
+  }
+  eq_beta_r = kone();
+  for (size_t i = 0; i < r_time.size(); i++) eq_beta_r = kmul(eq_beta_r, eq_lin(r_time[i], beta_r[i]));
+  has_inputs = r_inputs != nullptr;
+  eq_r_in = kzero();
+  if (has_inputs) {
+    eq_r_in = kone();
+    for (size_t i = 0; i < r_time.size(); i++) eq_r_in = kmul(eq_r_in, eq_lin(r_time[i], r_inputs[i]));
+  }
+}
+
+std::vector<K2> AjtaiPhase::evals(const std::vector<K2>& xs) const {
+  const size_t j = chals.size(), free_a = ell_d - j - 1, tail = (size_t)1 << free_a, hs = (size_t)1 << (j + 1);
+  std::vector<K2> wbt = chi_host(beta_a.data() + j + 1, free_a), wat = chi_host(alpha.data() + j + 1, free_a);
+  K2 eqb_pref = kone(), eqa_pref = kone();
+  for (size_t i = 0; i < j; i++) {
+    eqb_pref = kmul(eqb_pref, eq_lin(chals[i], beta_a[i]));
+    eqa_pref = kmul(eqa_pref, eq_lin(chals[i], alpha[i]));
+  }
+  std::vector<K2> gpi(kt);
+  gpi[0] = kone();
+  for (size_t i = 1; i < kt; i++) gpi[i] = kmul(gpi[i - 1], gamma);
+  K2 gk = kone();
+  for (size_t i = 0; i < kt; i++) gk = kmul(gk, gamma);
+  std::vector<K2> gkj(t);
+  gkj[0] = kone();
+  for (size_t q = 1; q < t; q++) gkj[q] = kmul(gkj[q - 1], gk);
+  std::vector<K2> yp(kt * t * D);
+  for (size_t q = 0; q < kt * t; q++) {
+    memcpy(&yp[q * D], &y_eval[q * D], D * sizeof(K2));
+    for (size_t b = 0; b < j; b++) fold_bit_inplace(&yp[q * D], b, chals[b]);
+  }
+  std::vector<K2> ys(xs.size());
+  K2 tmp[D];
+  for (size_t xi = 0; xi < xs.size(); xi++) {
+    K2 x = xs[xi];
+    K2 eq_beta = kmul(eq_beta_r, kmul(eqb_pref, eq_lin(x, beta_a[j])));
+    K2 eq_ar = has_inputs ? kmul(eq_r_in, kmul(eqa_pref, eq_lin(x, alpha[j]))) : kzero();
+    K2 nc_sum = kzero(), g = gamma;
+    for (size_t i = 0; i < kt; i++) {
+      memcpy(tmp, &yp[(i * t + 0) * D], sizeof(tmp));
+      fold_bit_inplace(tmp, j, x);
+      K2 a = kzero();
+      for (size_t tt = 0; tt < tail; tt++) {
+        size_t id = tt * hs;
+        if (id < (size_t)D) {
+          K2 y = tmp[id];
+          a = kadd(a, kmul(wbt[tt], kmul(y, ksub(ksqr(y), kone()))));
+        }
+      }
+      nc_sum = kadd(nc_sum, kmul(g, a));
+      g = kmul(g, gamma);
+    }
+    K2 o = kmul(eq_beta, kadd(f_prime, nc_sum));
+    if (kt >= 2 && !kiszero(eq_ar)) {
+      K2 inner = kzero();
+      for (size_t jm = 0; jm < t; jm++) {
+        K2 sj = kzero();
+        for (size_t i = 1; i < kt; i++) {
+          memcpy(tmp, &yp[(i * t + jm) * D], sizeof(tmp));
+          fold_bit_inplace(tmp, j, x);
+          K2 yd = kzero();
+          for (size_t tt = 0; tt < tail; tt++) {
+            size_t id = tt * hs;
+            if (id < (size_t)D) yd = kadd(yd, kmul(wat[tt], tmp[id]));
+          }
+          sj = kadd(sj, kmul(kmul(gpi[i], gkj[jm]), yd));
+        }
+        inner = kadd(inner, sj);
+      }
+      o = kadd(o, kmul(eq_ar, kmul(gk, inner)));
+    }
+    ys[xi] = o;
+  }
+  return ys;
+}
+
+// ---------------- CCS oracle on the device ----------------
+CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* alpha, const K2* beta_a,
+                     const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st)
+    : s_(s), st_(st), k_(k), ell_d_(ell_d), ell_n_(ell_n) {
+  const size_t n_pad = (size_t)1 << ell_n, m = s.m, n = s.n;
+  if (n_pad != n) throw Error(1, "row domain must equal n (power of two)");
+  len_ = n_pad;
+  zs_.assign(Zs, Zs + k);
+  Zptrs_.alloc(k * sizeof(void*));
+  h2d(Zptrs_.p, (const void* const*)Zs, k, st);
+  nf_ = (int)s.fvar_mat.size();
+  has_eval_ = k >= 2 && r_inputs != nullptr;
+  nsc_ = 1 + nf_ + (has_eval_ ? 2 : 0);
+  // scalar tables: [eqb][f vars][eqr][ev]
+  sc_[0].alloc((size_t)nsc_ * n_pad * sizeof(K2));
+  sc_[1].alloc((size_t)nsc_ * (n_pad / 2) * sizeof(K2));
+  K2* T = sc_[0].as<K2>();
+  DevBuf bits(64 * sizeof(K2));
+  h2d(bits.p, beta_r, ell_n, st);
+  launch_chi(bits.as<K2>(), (int)ell_n, T, st);
+  DevBuf z1(m * sizeof(K2));
+  launch_z1(Zs[0], m, z1.as<K2>(), st);
+  for (int f = 0; f < nf_; f++) {
+    const DevSparse& M = s.dev[s.fvar_mat[f]];
+    launch_spmv(M, z1.as<K2>(), T + (size_t)(1 + f) * n_pad, kone(), false, st);
+  }
+  gamma_k_ = kone();
+  for (size_t i = 0; i < k; i++) gamma_k_ = kmul(gamma_k_, gamma);
+  if (has_eval_) {
+    DevBuf rb(64 * sizeof(K2));
+    h2d(rb.p, r_inputs, ell_n, st);
+    launch_chi(rb.as<K2>(), (int)ell_n, T + (size_t)(1 + nf_) * n_pad, st);
+    // S_alpha = sum_{i>=1} gamma^i sum_rho chi_alpha[rho] Z_i[rho,:]
+    std::vector<K2> W((k - 1) * D);
+    K2 g = kone();
+    for (size_t i = 1; i < k; i++) {
+      g = kmul(g, gamma);
+      for (int rho = 0; rho < D; rho++) W[(i - 1) * D + rho] = kmul(g, eq_mask(rho, alpha, ell_d));
+    }
+    DevBuf Wd(W.size() * sizeof(K2)), Sa(m * sizeof(K2));
+    h2d(Wd.p, W.data(), W.size(), st);
+    launch_weighted_digit_colsum(Zptrs_.as<const int8_t*>() + 1, k - 1, Wd.as<K2>(), m, Sa.as<K2>(), st);
+    K2* ev = T + (size_t)(2 + nf_) * n_pad;
+    K2 gkj = kone();
+    for (size_t j = 0; j < s.t; j++) {
+      launch_spmv(s.dev[j], Sa.as<K2>(), ev, gkj, j > 0, st);
+      gkj = kmul(gkj, gamma_k_);
+    }
+    NS_CHECK(hipStreamSynchronize(st));
+  }
+  // NC weights wg[w*54 + rho] = chi_{beta_a}(rho) gamma^{w+1}
+  std::vector<K2> wg(k * D);
+  {
+    K2 g = gamma;
+    for (size_t w = 0; w < k; w++) {
+      for (int rho = 0; rho < D; rho++) wg[w * D + rho] = kmul(eq_mask(rho, beta_a, ell_d), g);
+      g = kmul(g, gamma);
+    }
+  }
+  wg_.alloc(wg.size() * sizeof(K2));
+  h2d(wg_.p, wg.data(), wg.size(), st);
+  nc_[0].alloc(k * D * (n_pad / 2) * sizeof(K2));
+  nc_[1].alloc(k * D * std::max<size_t>(n_pad / 4, 1) * sizeof(K2));
+  coeffs_dev_.alloc(8 * sizeof(K2));
+  ws_.alloc(round_ws_bytes());
+  NS_CHECK(hipStreamSynchronize(st));
+}
+
+std::vector<K2> CcsOracle::round_coeffs() {
+  RowTables T;
+  const K2* sc = sc_[cur_sc_].as<K2>();
+  T.len = len_;
+  T.eqb = sc;
+  T.fv = sc + len_;
+  T.nf = nf_;
+  T.eqr = has_eval_ ? sc + (size_t)(1 + nf_) * len_ : nullptr;
+  T.ev = has_eval_ ? sc + (size_t)(2 + nf_) * len_ : nullptr;
+  T.nc = round_ == 0 ? nullptr : nc_[cur_nc_].as<K2>();
+  T.digits = Zptrs_.as<const int8_t*>();
+  T.m = s_.m;
+  T.k = (int)k_;
+  T.gamma_k = gamma_k_;
+  launch_round_eval(T, s_.fpoly, wg_.as<K2>(), round_ == 0, coeffs_dev_.as<K2>(), ws_.p, st_);
+  std::vector<K2> out(5);
+  d2h(out.data(), coeffs_dev_.p, 5, st_);
+  return out;
+}
+
+void CcsOracle::fold(K2 r) {
+  const size_t half = len_ / 2;
+  int nsc_dst = cur_sc_ ^ 1;
+  launch_fold_k(sc_[cur_sc_].as<K2>(), (size_t)nsc_, len_, sc_[nsc_dst].as<K2>(), r, st_);
+  if (round_ == 0) {
+    launch_fold_digits(Zptrs_.as<const int8_t*>(), (int)k_, s_.m, len_, nc_[0].as<K2>(), r, st_);
+    cur_nc_ = 0;
+  } else {
+    int dst = cur_nc_ ^ 1;
+    launch_fold_k(nc_[cur_nc_].as<K2>(), k_ * D, len_, nc_[dst].as<K2>(), r, st_);
+    cur_nc_ = dst;
+  }
+  // the scalar ping-pong buffer 1 holds at most n/2 entries per row: keep tables compact
+  if (nsc_dst == 1 && half > sc_[1].bytes / (nsc_ * sizeof(K2))) throw Error(3, "scalar table overflow");
+  cur_sc_ = nsc_dst;
+  len_ = half;
+  round_++;
+}
+
+std::vector<K2> CcsOracle::y_eval(const std::vector<K2>& r, DevBuf& ws, DevBuf& vjs) {
+  const size_t m = s_.m, t = s_.t, n = s_.n;
+  DevBuf rb(64 * sizeof(K2)), chi(((size_t)1 << ell_n_) * sizeof(K2));
+  h2d(rb.p, r.data(), r.size(), st_);
+  launch_chi(rb.as<K2>(), (int)ell_n_, chi.as<K2>(), st_);
+  vjs.ensure(t * m * sizeof(K2));
+  size_t n_eff = std::min(n, (size_t)1 << ell_n_);
+  for (size_t j = 0; j < t; j++) launch_spmv_t(s_.dev[j], chi.as<K2>(), n_eff, vjs.as<K2>() + j * m, st_);
+  ws.ensure(ydigits_ws_bytes(k_, t, m));
+  DevBuf y(k_ * t * D * sizeof(K2));
+  launch_ydigits(Zptrs_.as<const int8_t*>(), k_, vjs.as<K2>(), t, m, y.as<K2>(), ws.p, st_);
+  std::vector<K2> out(k_ * t * D);
+  d2h(out.data(), y.p, out.size(), st_);
+  return out;
+}
+
+// ---------------- canonical encoding ----------------
+struct Enc {
+  std::vector<uint64_t> v;
+  void u(uint64_t x) { v.push_back(x); }
+  void k(K2 x) {
+    v.push_back(x.c0);
+    v.push_back(x.c1);
+  }
+  void digest(const uint8_t d[32]) {
+    for (int i = 0; i < 4; i++) {
+      uint64_t w = 0;
+      for (int j = 0; j < 8; j++) w |= (uint64_t)d[8 * i + j] << (8 * j);
+      v.push_back(w);
+    }
+  }
+  void me(const MeHost& e) {
+    for (int i = 0; i < KAPPA * D; i++) u(e.c[i]);
+    u(e.m_in);
+    for (auto x : e.X) u(x);
+    u(e.r.size());
+    for (auto& x : e.r) k(x);
+    u(e.t);
+    for (size_t j = 0; j < e.t; j++) {
+      u(DPAD);
+      for (int i = 0; i < DPAD; i++) k(e.y[j * DPAD + i]);
+    }
+    u(e.t);
+    for (auto& x : e.y_scalars) k(x);
+    digest(e.fold_digest);
+    u(0);
+    u(0);
+    u(0);
+  }
+};
+
+// ---------------- prover ----------------
+Prover::Prover(const Ccs& s, const Pp& pp) : s_(s), pp_(pp) {
+  if (pp.m != s.m) throw Error(1, "PP width does not match CCS m");
+  params_ = Params::goldilocks_auto_r1cs_ccs(s.n);
+  NS_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Prover::~Prover() { (void)hipStreamDestroy(stream_); }
+
+void Prover::commit_device(const int8_t* const* Zs_dev, size_t nw, uint64_t* host_out) {
+  ws_ajtai_.ensure(ajtai_workspace_bytes(s_.m, nw));
+  DevBuf ptrs(nw * sizeof(void*)), out(nw * KAPPA * D * 8);
+  h2d(ptrs.p, (const void* const*)Zs_dev, nw, stream_);
+  ajtai_commit(pp_.limbs.as<int8_t>(), KAPPA, s_.m, ptrs.as<const int8_t*>(), nw, out.as<uint64_t>(), ws_ajtai_.p,
+               stream_);
+  d2h(host_out, out.p, nw * KAPPA * D, stream_);
+}
+
+static size_t log2_pow2(size_t n) {
+  size_t l = 0;
+  while (((size_t)1 << l) < n) l++;
+  return l < 1 ? 1 : l;
+}
+
+std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t nsteps, size_t m_in,
+                                          bool z_on_device) {
+  using clk = std::chrono::steady_clock;
+  std::unique_ptr<KTimers> kt;
+  if (timing) {
+    kt = std::make_unique<KTimers>();
+    g_kt = kt.get();
+  }
+  struct Reset {
+    ~Reset() { g_kt = nullptr; }
+  } reset_guard;
+  const Ccs& s = s_;
+  const size_t m = s.m, t = s.t, k_dec = params_.k_rho;
+  const size_t ell_d = 6, ell_n = log2_pow2(s.n), ell = ell_d + ell_n;
+  const size_t d_sc = std::max<size_t>(s.f_degree() + 1, std::max<size_t>(2, 2 * params_.b + 2));
+  uint32_t smin;
+  int32_t slack;
+  if (!params_.extension_check((uint32_t)ell, (uint32_t)d_sc, &smin, &slack) || slack < 0)
+    throw Error(2, "extension policy failed");
+  if (m_in > m) throw Error(1, "m_in exceeds m");
+  hipStream_t st = stream_;
+  auto t0 = clk::now();
+
+  // ---- step intake: digits and MCS commitments for all steps ----
+  std::vector<DevBuf> Zmcs(nsteps);
+  std::vector<std::vector<uint64_t>> x_host(nsteps);
+  {
+    DevBuf zbuf(m * 8);
+    for (size_t i = 0; i < nsteps; i++) {
+      Zmcs[i].alloc((size_t)D * m);
+      const uint64_t* zsrc = z_steps[i];
+      if (!z_on_device) {
+        h2d(zbuf.p, z_steps[i], m, st);
+        zsrc = zbuf.as<uint64_t>();
+        x_host[i].assign(z_steps[i], z_steps[i] + m_in);
+      } else {
+        x_host[i].resize(m_in);
+        if (m_in) d2h(x_host[i].data(), z_steps[i], m_in, st);
+      }
+      launch_decomp_b2(zsrc, m, Zmcs[i].as<int8_t>(), st);
+      if (!z_on_device) NS_CHECK(hipStreamSynchronize(st));
+    }
+    NS_CHECK(hipStreamSynchronize(st));
+  }
+  std::vector<uint64_t> cmcs(nsteps * KAPPA * D);
+  {
+    std::vector<const int8_t*> ptrs(nsteps);
+    for (size_t i = 0; i < nsteps; i++) ptrs[i] = Zmcs[i].as<int8_t>();
+    commit_device(ptrs.data(), nsteps, cmcs.data());
+  }
+
+  Enc pb;
+  pb.u(0x314652504f52504eULL);
+  pb.u(nsteps);
+  Transcript tr("neo.fold/session");
+
+  std::vector<MeHost> acc;
+  std::vector<DevBuf> acc_wit;
+  DevBuf vjs, ws_yd;
+  const size_t xs_n = d_sc + 1;
+  std::vector<K2> xs(xs_n);
+  for (size_t i = 0; i < xs_n; i++) xs[i] = K2{i, 0};
+  std::vector<K2> bpow(D);
+  bpow[0] = kone();
+  for (int i = 1; i < D; i++) bpow[i] = kadd(bpow[i - 1], bpow[i - 1]);
+
+  for (size_t idx = 0; idx < nsteps; idx++) {
+    const size_t nacc = acc.size(), kt = nacc + 1;
+    // absorb_step_memory (no LUT / MEM instances)
+    tr.append_message("step/absorb_memory_start", (const uint8_t*)nullptr, 0);
+    tr.append_msg_u64("step/lut_count", 0);
+    tr.append_msg_u64("step/mem_count", 0);
+    tr.append_message("step/absorb_memory_done", (const uint8_t*)nullptr, 0);
+    // bind_header_and_instances_with_digest
+    tr.append_message("phase/pi-ccs", (const uint8_t*)nullptr, 0);
+    tr.append_message("neo/ccs/header/v1", (const uint8_t*)nullptr, 0);
+    {
+      uint64_t hdr[6] = {64, 2, params_.lambda, ell, d_sc, (uint64_t)(slack < 0 ? -slack : slack)};
+      tr.append_u64s("ccs/header", hdr, 6);
+      uint8_t sign = slack >= 0 ? 1 : 0;
+      tr.append_message("ccs/slack_sign", &sign, 1);
+      tr.append_message("neo/ccs/instances", (const uint8_t*)nullptr, 0);
+      uint64_t dd[3] = {s.n, s.m, s.t};
+      tr.append_u64s("dims", dd, 3);
+      for (int i = 0; i < 4; i++) tr.append_fields("mat_digest", &s.digest[i], 1);
+      tr.append_message("neo/ccs/poly", (const uint8_t*)nullptr, 0);
+      uint64_t ar = s.t, tl = s.term_coeff.size();
+      tr.append_u64s("arity", &ar, 1);
+      tr.append_u64s("terms_len", &tl, 1);
+      std::vector<size_t> order(tl);
+      for (size_t i = 0; i < tl; i++) order[i] = i;
+      std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return s.term_exps[a] < s.term_exps[b]; });
+      for (size_t i : order) {
+        tr.append_fields("coeff", &s.term_coeff[i], 1);
+        std::vector<uint64_t> ex(s.term_exps[i].begin(), s.term_exps[i].end());
+        tr.append_u64s("exps", ex.data(), ex.size());
+      }
+      tr.append_fields("x", x_host[idx].data(), m_in);
+      uint64_t mi = m_in;
+      tr.append_u64s("m_in", &mi, 1);
+      tr.append_fields("c_data", &cmcs[idx * KAPPA * D], KAPPA * D);
+    }
+    // bind_me_inputs
+    tr.append_message("neo/ccs/me_inputs/v2", (const uint8_t*)nullptr, 0);
+    {
+      uint64_t cnt = nacc;
+      tr.append_u64s("me_count", &cnt, 1);
+      std::vector<uint64_t> buf;
+      for (auto& e : acc) {
+        tr.append_fields("c_data_in", e.c, KAPPA * D);
+        uint64_t mi = e.m_in;
+        tr.append_u64s("m_in_in", &mi, 1);
+        buf.clear();
+        for (auto& x : e.r) {
+          buf.push_back(x.c0);
+          buf.push_back(x.c1);
+        }
+        tr.append_fields("r_in", buf.data(), buf.size());
+        buf.clear();
+        for (auto& x : e.y) {
+          buf.push_back(x.c0);
+          buf.push_back(x.c1);
+        }
+        tr.append_fields("y_elem", buf.data(), buf.size());
+      }
+    }
+    // sample_challenges
+    tr.append_message("neo/ccs/chals/v1", (const uint8_t*)nullptr, 0);
+    std::vector<K2> alpha(ell_d), beta(ell);
+    for (size_t i = 0; i < ell_d; i++) {
+      uint64_t c2[2];
+      tr.challenge_fields("chal/k", 2, c2);
+      alpha[i] = K2{c2[0], c2[1]};
+    }
+    for (size_t i = 0; i < ell; i++) {
+      uint64_t c2[2];
+      tr.challenge_fields("chal/k", 2, c2);
+      beta[i] = K2{c2[0], c2[1]};
+    }
+    K2 gamma;
+    {
+      uint64_t c2[2];
+      tr.challenge_fields("chal/k", 2, c2);
+      gamma = K2{c2[0], c2[1]};
+    }
+    const K2* beta_a = beta.data();
+    const K2* beta_r = beta.data() + ell_d;
+    // claimed initial sum T (paper_exact.rs:825-944)
+    K2 T = kzero();
+    if (kt >= 2) {
+      std::vector<K2> chi_a = chi_host(alpha.data(), ell_d);
+      K2 gk = kone();
+      for (size_t i = 0; i < kt; i++) gk = kmul(gk, gamma);
+      K2 inner = kzero();
+      for (size_t j = 0; j < t; j++) {
+        K2 gkj = kone();
+        for (size_t q = 0; q < j; q++) gkj = kmul(gkj, gk);
+        K2 gi = kone();
+        for (size_t a = 0; a < nacc; a++) {
+          gi = kmul(gi, gamma);  // gamma^{i-1}, i = a + 2
+          K2 ye = kzero();
+          for (int rho = 0; rho < DPAD; rho++) ye = kadd(ye, kmul(acc[a].y[j * DPAD + rho], chi_a[rho]));
+          inner = kadd(inner, kmul(kmul(gi, gkj), ye));
+        }
+      }
+      T = kmul(gk, inner);
+    }
+    tr.append_k("sumcheck/initial_sum", T);
+    for (size_t i = 0; i < ell_n; i++) {
+      tr.append_msg_u64("route_a/r_cycle", i);
+      (void)tr.challenge_field("route_a/cycle/0");
+      (void)tr.challenge_field("route_a/cycle/1");
+    }
+    // device oracle
+    std::vector<const int8_t*> Zs(kt);
+    Zs[0] = Zmcs[idx].as<int8_t>();
+    for (size_t a = 0; a < nacc; a++) Zs[1 + a] = acc_wit[a].as<int8_t>();
+    CcsOracle oracle(s, Zs.data(), kt, alpha.data(), beta_a, beta_r, gamma, ell_d, ell_n,
+                     nacc ? acc[0].r.data() : nullptr, st);
+    // batched time sum-check with the single CCS claim
+    tr.append_msg_u64("batched/dynamic_bind/len", 1);
+    tr.append_message("batched/dynamic_bind/claim_label", "ccs/time");
+    tr.append_msg_u64("batched/dynamic_bind/claim_idx", 0);
+    tr.append_msg_u64("batched/dynamic_bind/degree_bound", d_sc);
+    {
+      uint8_t one = 1;
+      tr.append_message("batched/dynamic_bind/is_dynamic", &one, 1);
+    }
+    tr.append_k("batched/dynamic_bind/claimed_sum", T);
+    tr.append_message("sc/domain", "shard/batched_time");
+    tr.append_msg_u64("sc/inst_idx", idx);
+    K2 running = T;
+    std::vector<K2> time_coeffs(ell_n * xs_n), r_time(ell_n);
+    for (size_t rd = 0; rd < ell_n; rd++) {
+      tr.append_msg_u64("batched/round_idx", rd);
+      std::vector<K2> c = oracle.round_coeffs();
+      std::vector<K2> ys(xs_n);
+      for (size_t i = 0; i < xs_n; i++) ys[i] = poly_eval(c.data(), c.size(), xs[i]);
+      if (!keq(kadd(ys[0], ys[1]), running))
+        throw Error(4, "step " + std::to_string(idx) + ": batched round " + std::to_string(rd) + " invariant failed");
+      K2* co = &time_coeffs[rd * xs_n];
+      interpolate(xs.data(), ys.data(), xs_n, co);
+      tr.append_message("batched/claim_label", "ccs/time");
+      tr.append_msg_u64("batched/claim_idx", 0);
+      for (size_t i = 0; i < xs_n; i++) tr.append_k("batched/round/coeff", co[i]);
+      K2 r = tr.challenge_k("batched/challenge/0", "batched/challenge/1");
+      r_time[rd] = r;
+      running = poly_eval(co, xs_n, r);
+      oracle.fold(r);
+    }
+    tr.append_message("sc/domain_end", "shard/batched_time");
+    tr.append_msg_u64("sc/inst_idx_end", idx);
+
+    // precompute_for_r on the device
+    std::vector<K2> y_eval = oracle.y_eval(r_time, ws_yd, vjs);
+    // Ajtai-bit rounds (host)
+    AjtaiPhase aj;
+    aj.init(s, y_eval, kt, alpha.data(), beta_a, beta_r, gamma, r_time, nacc ? acc[0].r.data() : nullptr);
+    tr.append_message("sc/domain", "ccs/ajtai");
+    tr.append_msg_u64("sc/inst_idx", idx);
+    tr.append_k("sc/claimed_sum", running);
+    std::vector<K2> aj_coeffs(ell_d * xs_n);
+    for (size_t rd = 0; rd < ell_d; rd++) {
+      std::vector<K2> ys = aj.evals(xs);
+      if (!keq(kadd(ys[0], ys[1]), running))
+        throw Error(4, "step " + std::to_string(idx) + ": ajtai round " + std::to_string(rd) + " invariant failed");
+      K2* co = &aj_coeffs[rd * xs_n];
+      interpolate(xs.data(), ys.data(), xs_n, co);
+      for (size_t i = 0; i < xs_n; i++) tr.append_k("sumcheck/round/coeff", co[i]);
+      K2 r = tr.challenge_k("sumcheck/challenge/0", "sumcheck/challenge/1");
+      running = poly_eval(co, xs_n, r);
+      aj.fold(r);
+    }
+    const std::vector<K2>& aj_chals = aj.chals;
+    tr.append_message("sc/domain_end", "ccs/ajtai");
+    tr.append_msg_u64("sc/inst_idx_end", idx);
+    uint8_t fold_digest[32];
+    tr.digest32(fold_digest);
+
+    // ME outputs
+    auto head_cols = [&](const int8_t* Zdev, std::vector<uint64_t>& X) {
+      X.assign((size_t)D * m_in, 0);
+      if (!m_in) return;
+      std::vector<int8_t> tmp((size_t)D * m_in);
+      NS_CHECK(hipMemcpy2DAsync(tmp.data(), m_in, Zdev, m, m_in, D, hipMemcpyDeviceToHost, st));
+      NS_CHECK(hipStreamSynchronize(st));
+      for (size_t i = 0; i < tmp.size(); i++) X[i] = from_i64(tmp[i]);
+    };
+    std::vector<MeHost> ccs_out(kt);
+    for (size_t i = 0; i < kt; i++) {
+      MeHost& e = ccs_out[i];
+      e.m_in = m_in;
+      e.t = t;
+      if (i == 0) {
+        memcpy(e.c, &cmcs[idx * KAPPA * D], sizeof(e.c));
+        head_cols(Zmcs[idx].as<int8_t>(), e.X);
+      } else {
+        memcpy(e.c, acc[i - 1].c, sizeof(e.c));
+        e.X = acc[i - 1].X;
+      }
+      e.r = r_time;
+      e.y.assign(t * DPAD, kzero());
+      e.y_scalars.assign(t, kzero());
+      for (size_t j = 0; j < t; j++) {
+        K2 sc = kzero();
+        for (int rho = 0; rho < D; rho++) {
+          K2 v = y_eval[(i * t + j) * D + rho];
+          e.y[j * DPAD + rho] = v;
+          sc = kadd(sc, kmul(v, bpow[rho]));
+        }
+        e.y_scalars[j] = sc;
+      }
+      memcpy(e.fold_digest, fold_digest, 32);
+    }
+
+    // ---- Pi_RLC ----
+    tr.append_message("fold/rlc_inputs/v1", "main");
+    {
+      uint64_t v = idx;
+      tr.append_u64s("step_idx", &v, 1);
+      v = kt;
+      tr.append_u64s("me_count", &v, 1);
+      for (auto& e : ccs_out) {
+        tr.append_fields("c_data", e.c, KAPPA * D);
+        v = e.m_in;
+        tr.append_u64s("m_in", &v, 1);
+        tr.append_message("me_fold_digest", e.fold_digest, 32);
+        for (auto& x : e.r) tr.append_k("r_limb", x);
+        tr.append_fields("X", e.X.data(), e.X.size());
+        for (auto& x : e.y) tr.append_k("y_elem", x);
+        for (auto& x : e.y_scalars) tr.append_k("y_scalar", x);
+        v = 0;
+        tr.append_u64s("c_step_coords_len", &v, 1);
+        tr.append_fields("c_step_coords", nullptr, 0);
+        tr.append_u64s("u_offset", &v, 1);
+        tr.append_u64s("u_len", &v, 1);
+      }
+    }
+    if ((uint64_t)kt * 216u * (params_.b - 1) >= ((uint64_t)1 << params_.k_rho))
+      throw Error(5, "RLC norm bound violated");
+    std::vector<int8_t> rho_a(kt * D);
+    std::vector<uint64_t> rhos(kt * D * D, 0);
+    std::vector<int8_t> rho_i8(kt * D * D, 0);
+    for (size_t i = 0; i < kt; i++) {
+      tr.append_msg_u64("rlc/rot/index", i);
+      // draw_alphabet_vector (neo-reductions/src/common.rs:410-439)
+      static const int8_t A5[5] = {-2, -1, 0, 1, 2};
+      const uint32_t bucket = (65536u / 5u) * 5u;
+      size_t got = 0;
+      uint64_t ctr = i;
+      while (got < (size_t)D) {
+        tr.append_msg_u64("rlc/rot/chunk", ctr);
+        uint8_t dig[32];
+        tr.digest32(dig);
+        for (int w = 0; w < 16 && got < (size_t)D; w++) {
+          uint32_t x = (uint32_t)dig[2 * w] | ((uint32_t)dig[2 * w + 1] << 8);
+          if (x < bucket) rho_a[i * D + got++] = A5[x % 5];
+        }
+        ctr++;
+      }
+      uint64_t* R = &rhos[i * D * D];
+      if (kt == 1) {
+        for (int q = 0; q < D; q++) R[q * D + q] = 1;
+      } else {
+        // rot_from_coeffs: column j = X^j * a mod Phi_81
+        uint64_t col[D], nxt[D];
+        for (int r = 0; r < D; r++) col[r] = from_i64(rho_a[i * D + r]);
+        for (int j = 0; j < D; j++) {
+          for (int r = 0; r < D; r++) R[r * D + j] = col[r];
+          uint64_t last = col[D - 1];
+          nxt[0] = fneg(last);
+          for (int r = 1; r < D; r++) nxt[r] = col[r - 1];
+          nxt[27] = fsub(nxt[27], last);
+          memcpy(col, nxt, sizeof(col));
+        }
+      }
+      for (int q = 0; q < D * D; q++) {
+        int64_t v = to_balanced(R[q]);
+        if (v < -127 || v > 127) throw Error(5, "rotation entry out of int8 range");
+        rho_i8[i * D * D + q] = (int8_t)v;
+      }
+    }
+    MeHost parent;
+    parent.m_in = m_in;
+    parent.t = t;
+    parent.r = r_time;
+    memcpy(parent.fold_digest, fold_digest, 32);
+    parent.X.assign((size_t)D * m_in, 0);
+    parent.y.assign(t * DPAD, kzero());
+    parent.y_scalars.assign(t, kzero());
+    if (kt == 1) {
+      memcpy(parent.c, ccs_out[0].c, sizeof(parent.c));
+      parent.X = ccs_out[0].X;
+      parent.y = ccs_out[0].y;
+    } else {
+      for (size_t i = 0; i < kt; i++) {
+        const uint64_t* R = &rhos[i * D * D];
+        for (int rr = 0; rr < D; rr++)
+          for (int kk = 0; kk < D; kk++) {
+            uint64_t co = R[rr * D + kk];
+            if (!co) continue;
+            for (size_t c = 0; c < m_in; c++)
+              parent.X[rr * m_in + c] = fadd(parent.X[rr * m_in + c], fmul(co, ccs_out[i].X[kk * m_in + c]));
+          }
+        for (size_t j = 0; j < t; j++)
+          for (int rr = 0; rr < D; rr++) {
+            K2 a = kzero();
+            for (int kk = 0; kk < D; kk++) a = kadd(a, kscale(ccs_out[i].y[j * DPAD + kk], R[rr * D + kk]));
+            parent.y[j * DPAD + rr] = kadd(parent.y[j * DPAD + rr], a);
+          }
+      }
+      // commitment mix: sum_i rot(rho_i) * c_i (s_lincomb with the rotation's first column)
+      memset(parent.c, 0, sizeof(parent.c));
+      for (size_t i = 0; i < kt; i++) {
+        const int8_t* a = &rho_a[i * D];
+        for (int col = 0; col < KAPPA; col++) {
+          const uint64_t* cc = &ccs_out[i].c[col * D];
+          uint64_t tmp[2 * D - 1];
+          for (auto& x : tmp) x = 0;
+          for (int u = 0; u < D; u++) {
+            if (!a[u]) continue;
+            for (int v = 0; v < D; v++) {
+              uint64_t p = a[u] > 0 ? cc[v] : fneg(cc[v]);
+              for (int rep = 0; rep < (a[u] > 0 ? a[u] : -a[u]); rep++) tmp[u + v] = fadd(tmp[u + v], p);
+            }
+          }
+          for (int q = 0; q < D; q++) {
+            uint64_t c = tmp[q];
+            if (q <= 25) c = fadd(c, tmp[q + 81]);
+            if (q >= 27) c = fsub(c, tmp[q + 27]);
+            if (q <= 26) c = fsub(c, tmp[q + 54]);
+            parent.c[col * D + q] = fadd(parent.c[col * D + q], c);
+          }
+        }
+      }
+    }
+    for (size_t j = 0; j < t; j++) {
+      K2 sc = kzero();
+      for (int rho = 0; rho < D; rho++) sc = kadd(sc, kmul(parent.y[j * DPAD + rho], bpow[rho]));
+      parent.y_scalars[j] = sc;
+    }
+    // Z_mix on the device
+    DevBuf zmix((size_t)D * m * 4), rho_dev(rho_i8.size()), zptr(kt * sizeof(void*));
+    h2d(rho_dev.p, rho_i8.data(), rho_i8.size(), st);
+    h2d(zptr.p, (const void* const*)Zs.data(), kt, st);
+    launch_rlc_zmix(rho_dev.as<int8_t>(), zptr.as<const int8_t*>(), kt, m, zmix.as<int32_t>(), st);
+
+    // ---- Pi_DEC ----
+    std::vector<DevBuf> planes(k_dec);
+    std::vector<const int8_t*> pptr(k_dec);
+    for (size_t i = 0; i < k_dec; i++) {
+      planes[i].alloc((size_t)D * m);
+      pptr[i] = planes[i].as<int8_t>();
+    }
+    DevBuf pptr_dev(k_dec * sizeof(void*)), flags((k_dec + 1) * 4);
+    h2d(pptr_dev.p, (const void* const*)pptr.data(), k_dec, st);
+    NS_CHECK(hipMemsetAsync(flags.p, 0, (k_dec + 1) * 4, st));
+    launch_dec_split(zmix.as<int32_t>(), (size_t)D * m, (int)k_dec, (int8_t* const*)pptr_dev.p, flags.as<int>(), st);
+    std::vector<int> fl(k_dec + 1);
+    d2h(fl.data(), flags.p, k_dec + 1, st);
+    if (fl[k_dec]) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
+    std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
+    commit_device(pptr.data(), k_dec, child_c.data());
+    // child y = Z_i v_j with the v_j of the shared point r_time (already in vjs)
+    std::vector<K2> y_child(k_dec * t * D);
+    {
+      ws_yd.ensure(ydigits_ws_bytes(k_dec, t, m));
+      DevBuf ydev(y_child.size() * sizeof(K2));
+      launch_ydigits(pptr_dev.as<const int8_t*>(), k_dec, vjs.as<K2>(), t, m, ydev.as<K2>(), ws_yd.p, st);
+      d2h(y_child.data(), ydev.p, y_child.size(), st);
+    }
+    std::vector<MeHost> children(k_dec);
+    for (size_t i = 0; i < k_dec; i++) {
+      MeHost& e = children[i];
+      if (fl[i]) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
+      else memset(e.c, 0, sizeof(e.c));
+      e.m_in = m_in;
+      e.t = t;
+      head_cols(pptr[i], e.X);
+      e.r = parent.r;
+      e.y.assign(t * DPAD, kzero());
+      e.y_scalars.assign(t, kzero());
+      for (size_t j = 0; j < t; j++) {
+        K2 sc = kzero();
+        for (int rho = 0; rho < D; rho++) {
+          K2 v = y_child[(i * t + j) * D + rho];
+          e.y[j * DPAD + rho] = v;
+          sc = kadd(sc, kmul(v, bpow[rho]));
+        }
+        e.y_scalars[j] = sc;
+      }
+      memcpy(e.fold_digest, parent.fold_digest, 32);
+    }
+    {
+      bool ok_y = true, ok_x = true, ok_c = true;
+      for (size_t j = 0; j < t && ok_y; j++)
+        for (int q = 0; q < DPAD; q++) {
+          K2 lhs = kzero();
+          uint64_t pw = 1;
+          for (size_t i = 0; i < k_dec; i++) {
+            lhs = kadd(lhs, kscale(children[i].y[j * DPAD + q], pw));
+            pw = fadd(pw, pw);
+          }
+          if (!keq(lhs, parent.y[j * DPAD + q])) {
+            ok_y = false;
+            break;
+          }
+        }
+      for (size_t q = 0; q < (size_t)D * m_in; q++) {
+        uint64_t lhs = 0, pw = 1;
+        for (size_t i = 0; i < k_dec; i++) {
+          lhs = fadd(lhs, fmul(children[i].X[q], pw));
+          pw = fadd(pw, pw);
+        }
+        if (lhs != parent.X[q]) ok_x = false;
+      }
+      for (int q = 0; q < KAPPA * D; q++) {
+        uint64_t lhs = children[0].c[q], pw = 2;
+        for (size_t i = 1; i < k_dec; i++) {
+          lhs = fadd(lhs, fmul(children[i].c[q], pw));
+          pw = fadd(pw, pw);
+        }
+        if (lhs != parent.c[q]) ok_c = false;
+      }
+      if (!(ok_y && ok_x && ok_c))
+        throw Error(7, "DEC public check failed at step " + std::to_string(idx) + " (y=" + std::to_string(ok_y) +
+                           ", X=" + std::to_string(ok_x) + ", c=" + std::to_string(ok_c) + ")");
+    }
+    tr.append_msg_u64("fold/step_done", idx);
+
+    // ---- encode StepProof ----
+    pb.u(kt);
+    for (auto& e : ccs_out) pb.me(e);
+    pb.u(ell_n + ell_d);
+    for (size_t rd = 0; rd < ell_n; rd++) {
+      pb.u(xs_n);
+      for (size_t i = 0; i < xs_n; i++) pb.k(time_coeffs[rd * xs_n + i]);
+    }
+    for (size_t rd = 0; rd < ell_d; rd++) {
+      pb.u(xs_n);
+      for (size_t i = 0; i < xs_n; i++) pb.k(aj_coeffs[rd * xs_n + i]);
+    }
+    pb.k(T);
+    pb.u(ell_n + ell_d);
+    for (auto& x : r_time) pb.k(x);
+    for (auto& x : aj_chals) pb.k(x);
+    pb.u(ell_d);
+    for (auto& x : alpha) pb.k(x);
+    pb.u(ell_d);
+    for (size_t i = 0; i < ell_d; i++) pb.k(beta_a[i]);
+    pb.u(ell_n);
+    for (size_t i = 0; i < ell_n; i++) pb.k(beta_r[i]);
+    pb.k(gamma);
+    pb.k(running);
+    pb.digest(fold_digest);
+    pb.u(kt);
+    for (auto x : rhos) pb.u(x);
+    pb.me(parent);
+    pb.u(k_dec);
+    for (auto& e : children) pb.me(e);
+    for (int i = 0; i < 6; i++) pb.u(0);
+    pb.u(1);
+    pb.k(T);
+    pb.u(1);
+    pb.u(d_sc);
+    pb.u(1);
+    pb.u(8);
+    for (const char* c = "ccs/time"; *c; c++) pb.u((uint8_t)*c);
+    pb.u(1);
+    pb.u(ell_n);
+    for (size_t rd = 0; rd < ell_n; rd++) {
+      pb.u(xs_n);
+      for (size_t i = 0; i < xs_n; i++) pb.k(time_coeffs[rd * xs_n + i]);
+    }
+    pb.u(0);
+
+    acc = std::move(children);
+    acc_wit = std::move(planes);
+  }
+  pb.u(acc.size());
+  for (auto& e : acc) pb.me(e);
+  NS_CHECK(hipStreamSynchronize(st));
+  last_seconds = std::chrono::duration<double>(clk::now() - t0).count();
+  last_permutations = tr.permutations();
+  if (kt) kt->collect(kernel_ms, kernel_count);
+  return pb.v;
+}
+
+}  // namespace ns

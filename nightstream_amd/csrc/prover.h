@@ -1,0 +1,178 @@
+// prover.h -- host orchestration of the MI355X Neo folding prover: device-resident CCS,
+// Ajtai PP, witnesses and Pi_CCS tables, with the Fiat-Shamir transcript on the host.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "transcript.h"
+
+namespace ns {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Owned device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n);
+  void ensure(size_t n) {
+    if (n > bytes) alloc(n);
+  }
+  void release();
+  template <class T>
+  T* as() const { return (T*)p; }
+};
+
+struct Params {
+  uint64_t q = Q;
+  uint32_t eta = 81, d = 54, kappa = 16;
+  uint64_t m = 1ull << 24;
+  uint32_t b = 2, k_rho = 12;
+  uint64_t B = 4096;
+  uint32_t T = 216, s = 2, lambda = 127;
+  static Params goldilocks_auto_r1cs_ccs(size_t n_rows);  // neo-params/src/lib.rs:186-222
+  bool extension_check(uint32_t ell, uint32_t d_sc, uint32_t* s_min, int32_t* slack) const;
+};
+
+struct HostCsr {
+  std::vector<uint64_t> ptr;
+  std::vector<uint32_t> col;
+  std::vector<uint64_t> val;
+};
+
+// CCS structure: identity-first [I, M_1 .. M_{t-1}] with the sparse polynomial f.
+struct Ccs {
+  size_t n = 0, m = 0, t = 0;
+  std::vector<HostCsr> host;     // host[j] for j >= 1 (index 0 = identity, empty)
+  std::vector<DevSparse> dev;    // dev[j]
+  std::vector<DevBuf> bufs;
+  // f terms
+  std::vector<uint64_t> term_coeff;
+  std::vector<std::vector<uint32_t>> term_exps;
+  FPoly fpoly;                   // compiled over used variables
+  std::vector<int> fvar_mat;     // CCS matrix index of each f-var table
+  uint64_t digest[4];
+  static std::unique_ptr<Ccs> from_r1cs(size_t n, size_t m, const HostCsr& a, const HostCsr& b, const HostCsr& c);
+  size_t f_degree() const;
+};
+
+// Ajtai public parameters resident in HBM as int8 limb planes (see ajtai.hip).
+struct Pp {
+  size_t kappa = KAPPA, m = 0;
+  DevBuf limbs;   // kappa*8*54 rows x m
+  DevBuf coeffs;  // optional u64 [kappa][m][54]
+  static std::unique_ptr<Pp> setup_seeded(const uint8_t seed[32], size_t m, bool keep_coeffs, hipStream_t s);
+  static std::unique_ptr<Pp> from_coeffs(const uint64_t* host_coeffs, size_t m, bool keep_coeffs, hipStream_t s);
+};
+
+struct MeHost {
+  uint64_t c[KAPPA * D];
+  size_t m_in = 0;
+  std::vector<uint64_t> X;   // D x m_in
+  std::vector<K2> r;
+  size_t t = 0;
+  std::vector<K2> y;         // t x 64
+  std::vector<K2> y_scalars; // t
+  uint8_t fold_digest[32];
+};
+
+struct StepTiming {
+  double transcript_s = 0, gpu_wait_s = 0;
+};
+
+class Prover {
+ public:
+  Prover(const Ccs& s, const Pp& pp);
+  ~Prover();
+  // Fold `nsteps` steps given their full z vectors (host, length m each).  Returns the
+  // canonical proof encoding (u64 little-endian words; same layout as the CPU restatement).
+  // z_on_device: z_steps are device pointers (inputs already resident in HBM).
+  std::vector<uint64_t> prove_shard(const uint64_t* const* z_steps, size_t nsteps, size_t m_in,
+                                    bool z_on_device = false);
+  double last_seconds = 0;
+  // Per-kernel-class timing of the last prove (HIP events on the prover stream).
+  bool timing = false;
+  double kernel_ms[KT_N] = {0};
+  uint64_t kernel_count[KT_N] = {0};
+  uint64_t last_permutations = 0;
+  // commit helpers exposed to the C ABI
+  void commit_device(const int8_t* const* Zs_host_ptrs_dev, size_t nw, uint64_t* host_out);
+  hipStream_t stream() const { return stream_; }
+  const Ccs& ccs() const { return s_; }
+  const Pp& pp() const { return pp_; }
+  DevBuf& ws_ajtai() { return ws_ajtai_; }
+
+ private:
+  const Ccs& s_;
+  const Pp& pp_;
+  Params params_;
+  hipStream_t stream_;
+  DevBuf ws_ajtai_, ws_round_, ws_yd_;
+};
+
+// Ajtai-bit rounds of Pi_CCS on the host (optimized_engine/oracle.rs:1812-1951), from the
+// precompute_for_r products y_eval[w][j][rho] at the folded row point.
+struct AjtaiPhase {
+  size_t kt = 0, t = 0, ell_d = 6;
+  std::vector<K2> y_eval, alpha, beta_a, chals;
+  K2 gamma, f_prime, eq_beta_r, eq_r_in;
+  bool has_inputs = false;
+  void init(const Ccs& s, std::vector<K2> y, size_t k, const K2* alpha, const K2* beta_a, const K2* beta_r, K2 gamma,
+            const std::vector<K2>& r_time, const K2* r_inputs);
+  std::vector<K2> evals(const std::vector<K2>& xs) const;
+  void fold(K2 r) { chals.push_back(r); }
+};
+
+// Device-resident Pi_CCS round oracle (RoundOracle, neo-reductions/src/sumcheck.rs:17-34).
+class CcsOracle {
+ public:
+  CcsOracle(const Ccs& s, const int8_t* const* Zs_dev_host, size_t k, const K2* alpha, const K2* beta_a,
+            const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st);
+  // coefficients of the current row round polynomial (deg_max + 1)
+  std::vector<K2> round_coeffs();
+  void fold(K2 r);
+  size_t round() const { return round_; }
+  size_t ell_n() const { return ell_n_; }
+  // precompute_for_r at the folded row point: y_eval[w][j][rho]
+  std::vector<K2> y_eval(const std::vector<K2>& r, DevBuf& ws, DevBuf& vjs_out);
+
+ private:
+  const Ccs& s_;
+  hipStream_t st_;
+  size_t k_, ell_d_, ell_n_, len_, round_ = 0;
+  int nf_, nsc_;
+  bool has_eval_;
+  K2 gamma_k_;
+  DevBuf Zptrs_;          // device array of k digit pointers
+  DevBuf sc_[2];          // scalar tables [nsc][len]
+  DevBuf nc_[2];          // NC tables [k*54][len]
+  int cur_sc_ = 0, cur_nc_ = 0;
+  DevBuf wg_, coeffs_dev_, ws_;
+  std::vector<const int8_t*> zs_;
+};
+
+}  // namespace ns

@@ -1,0 +1,271 @@
+// transcript.cpp -- ChaCha8, Poseidon2 and the Fiat-Shamir transcript (host).
+// See transcript.h for the construction and its parity status.
+#include "transcript.h"
+
+#include <string.h>
+
+namespace ns {
+
+static inline uint32_t rotl(uint32_t x, int n) { return (x << n) | (x >> (32 - n)); }
+#define QR(a, b, c, d)                     \
+  a += b; d ^= a; d = rotl(d, 16);         \
+  c += d; b ^= c; b = rotl(b, 12);         \
+  a += b; d ^= a; d = rotl(d, 8);          \
+  c += d; b ^= c; b = rotl(b, 7);
+
+void chacha8_block(const uint32_t key[8], uint64_t counter, uint32_t out[16]) {
+  uint32_t in[16] = {0x61707865, 0x3320646e, 0x79622d32, 0x6b206574, key[0], key[1], key[2], key[3],
+                     key[4],     key[5],     key[6],     key[7],     (uint32_t)counter, (uint32_t)(counter >> 32),
+                     0,          0};
+  uint32_t x[16];
+  memcpy(x, in, sizeof(x));
+  for (int r = 0; r < 8; r += 2) {
+    QR(x[0], x[4], x[8], x[12]);
+    QR(x[1], x[5], x[9], x[13]);
+    QR(x[2], x[6], x[10], x[14]);
+    QR(x[3], x[7], x[11], x[15]);
+    QR(x[0], x[5], x[10], x[15]);
+    QR(x[1], x[6], x[11], x[12]);
+    QR(x[2], x[7], x[8], x[13]);
+    QR(x[3], x[4], x[9], x[14]);
+  }
+  for (int i = 0; i < 16; i++) out[i] = x[i] + in[i];
+}
+
+ChaCha8::ChaCha8(const uint8_t seed[32]) {
+  for (int i = 0; i < 8; i++)
+    key[i] = (uint32_t)seed[4 * i] | ((uint32_t)seed[4 * i + 1] << 8) | ((uint32_t)seed[4 * i + 2] << 16) |
+             ((uint32_t)seed[4 * i + 3] << 24);
+}
+
+ChaCha8 ChaCha8::from_u64(uint64_t state) {
+  uint8_t seed[32];
+  for (int c = 0; c < 8; c++) {
+    state = state * 6364136223846793005ULL + 11634580027462260723ULL;
+    uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
+    uint32_t rot = (uint32_t)(state >> 59);
+    uint32_t x = (xs >> rot) | (xs << ((32 - rot) & 31));
+    for (int b = 0; b < 4; b++) seed[4 * c + b] = (uint8_t)(x >> (8 * b));
+  }
+  return ChaCha8(seed);
+}
+
+void ChaCha8::refill() {
+  for (int b = 0; b < 4; b++) chacha8_block(key, counter + b, buf + 16 * b);
+  counter += 4;
+}
+
+uint32_t ChaCha8::next_u32() {
+  if (idx >= 64) {
+    refill();
+    idx = 0;
+  }
+  return buf[idx++];
+}
+
+uint64_t ChaCha8::next_u64() {
+  if (idx < 63) {
+    uint64_t v = (uint64_t)buf[idx] | ((uint64_t)buf[idx + 1] << 32);
+    idx += 2;
+    return v;
+  }
+  if (idx >= 64) {
+    refill();
+    idx = 2;
+    return (uint64_t)buf[0] | ((uint64_t)buf[1] << 32);
+  }
+  uint64_t lo = buf[63];
+  refill();
+  idx = 1;
+  return ((uint64_t)buf[0] << 32) | lo;
+}
+
+void ChaCha8::fill_bytes(uint8_t* dst, size_t len) {
+  size_t done = 0;
+  while (done < len) {
+    if (idx >= 64) {
+      refill();
+      idx = 0;
+    }
+    while (done < len && idx < 64) {
+      uint32_t w = buf[idx++];
+      for (int b = 0; b < 4 && done < len; b++) dst[done++] = (uint8_t)(w >> (8 * b));
+    }
+  }
+}
+
+static uint64_t sample_fq(ChaCha8& rng) {
+  for (;;) {
+    uint64_t x = rng.next_u64();
+    if (x < Q) return x;
+  }
+}
+
+void Poseidon2::init(int w, ChaCha8& rng) {
+  width = w;
+  memset(ext_init, 0, sizeof(ext_init));
+  memset(ext_term, 0, sizeof(ext_term));
+  memset(diag, 0, sizeof(diag));
+  for (int r = 0; r < 4; r++)
+    for (int i = 0; i < w; i++) ext_init[r][i] = sample_fq(rng);
+  for (int r = 0; r < 4; r++)
+    for (int i = 0; i < w; i++) ext_term[r][i] = sample_fq(rng);
+  for (int r = 0; r < 22; r++) internal_rc[r] = sample_fq(rng);
+  for (int i = 0; i < w; i++) diag[i] = sample_fq(rng);  // stand-in diagonal, see transcript.h
+}
+
+static inline uint64_t sbox7(uint64_t x) {
+  uint64_t x2 = fmul(x, x);
+  uint64_t x3 = fmul(x2, x);
+  uint64_t x4 = fmul(x2, x2);
+  return fmul(x3, x4);
+}
+
+static inline void mat4(uint64_t* x) {
+  uint64_t t01 = fadd(x[0], x[1]), t23 = fadd(x[2], x[3]);
+  uint64_t t0123 = fadd(t01, t23);
+  uint64_t t01123 = fadd(t0123, x[1]), t01233 = fadd(t0123, x[3]);
+  uint64_t y3 = fadd(t01233, fadd(x[0], x[0]));
+  uint64_t y1 = fadd(t01123, fadd(x[2], x[2]));
+  uint64_t y0 = fadd(t01123, t01);
+  uint64_t y2 = fadd(t01233, t23);
+  x[0] = y0;
+  x[1] = y1;
+  x[2] = y2;
+  x[3] = y3;
+}
+
+static inline void ext_layer(uint64_t* st, int w) {
+  for (int c = 0; c < w; c += 4) mat4(st + c);
+  uint64_t s[4] = {0, 0, 0, 0};
+  for (int c = 0; c < w; c += 4)
+    for (int k = 0; k < 4; k++) s[k] = fadd(s[k], st[c + k]);
+  for (int i = 0; i < w; i++) st[i] = fadd(st[i], s[i & 3]);
+}
+
+void Poseidon2::permute(uint64_t* st) const {
+  const int w = width;
+  ext_layer(st, w);
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < w; i++) st[i] = sbox7(fadd(st[i], ext_init[r][i]));
+    ext_layer(st, w);
+  }
+  for (int r = 0; r < 22; r++) {
+    st[0] = sbox7(fadd(st[0], internal_rc[r]));
+    uint64_t sum = 0;
+    for (int i = 0; i < w; i++) sum = fadd(sum, st[i]);
+    for (int i = 0; i < w; i++) st[i] = fadd(fmul(st[i], diag[i]), sum);
+  }
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < w; i++) st[i] = sbox7(fadd(st[i], ext_term[r][i]));
+    ext_layer(st, w);
+  }
+}
+
+const Poseidon2& transcript_perm() {
+  static Poseidon2 p = [] {
+    static const uint8_t SEED[33] = "neo_poseidon2_goldilocks_seed___";
+    ChaCha8 rng(SEED);
+    Poseidon2 q;
+    q.init(8, rng);
+    return q;
+  }();
+  return p;
+}
+
+const Poseidon2& digest_perm() {
+  static Poseidon2 p = [] {
+    ChaCha8 rng = ChaCha8::from_u64(0x434353445F4D4154ULL);
+    Poseidon2 q;
+    q.init(16, rng);
+    return q;
+  }();
+  return p;
+}
+
+// ---------------- transcript ----------------
+Transcript::Transcript(const char* app_label) {
+  memset(st_, 0, sizeof(st_));
+  perm_ = &transcript_perm();
+  append_message("neo/transcript/v1|poseidon2-goldilocks-w8-r4", (const uint8_t*)app_label, strlen(app_label));
+}
+
+void Transcript::permute() {
+  perm_->permute(st_);
+  absorbed_ = 0;
+  nperm_++;
+}
+
+void Transcript::absorb(uint64_t x) {
+  if (absorbed_ >= 4) permute();
+  st_[absorbed_++] = x;
+}
+
+void Transcript::absorb_label(const char* label) {
+  for (const char* p = label; *p; p++) absorb((uint8_t)*p);
+}
+
+void Transcript::absorb_slice(const uint64_t* in, size_t len) {
+  size_t src = 0;
+  while (absorbed_ < 4 && src < len) st_[absorbed_++] = in[src++];
+  if (absorbed_ == 4) permute();
+  while (len - src >= 4) {
+    st_[0] = in[src];
+    st_[1] = in[src + 1];
+    st_[2] = in[src + 2];
+    st_[3] = in[src + 3];
+    permute();
+    src += 4;
+  }
+  while (src < len) st_[absorbed_++] = in[src++];
+}
+
+void Transcript::append_message(const char* label, const uint8_t* msg, size_t len) {
+  absorb_label(label);
+  absorb(len);
+  for (size_t i = 0; i < len; i++) absorb(msg[i]);
+}
+
+void Transcript::append_msg_u64(const char* label, uint64_t v) {
+  uint8_t b[8];
+  for (int i = 0; i < 8; i++) b[i] = (uint8_t)(v >> (8 * i));
+  append_message(label, b, 8);
+}
+
+void Transcript::append_fields(const char* label, const uint64_t* fs, size_t n) {
+  absorb_label(label);
+  absorb(n);
+  absorb_slice(fs, n);
+}
+
+void Transcript::append_u64s(const char* label, const uint64_t* us, size_t n) {
+  absorb_label(label);
+  absorb(n);
+  for (size_t i = 0; i < n; i++) absorb(us[i] % Q);
+}
+
+uint64_t Transcript::challenge_field(const char* label) {
+  append_message("chal/label", (const uint8_t*)label, strlen(label));
+  absorb(1);
+  permute();
+  return st_[0];
+}
+
+void Transcript::challenge_fields(const char* label, size_t n, uint64_t* out) {
+  append_message("chal/label", (const uint8_t*)label, strlen(label));
+  size_t got = 0;
+  while (got < n) {
+    absorb(1);
+    permute();
+    for (size_t i = 0; i < 4 && got < n; i++) out[got++] = st_[i];
+  }
+}
+
+void Transcript::digest32(uint8_t out[32]) {
+  absorb(1);
+  permute();
+  for (int i = 0; i < 4; i++)
+    for (int b = 0; b < 8; b++) out[8 * i + b] = (uint8_t)(st_[i] >> (8 * b));
+}
+
+}  // namespace ns

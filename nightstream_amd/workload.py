@@ -1,0 +1,112 @@
+"""Synthetic workloads for the Neo folding hot path.
+
+R1CS generator (the benchmark workloads' description): n constraints x m variables with n == m.
+A and B have 3 nonzeros per row at distinct uniform columns with uniform nonzero Goldilocks
+coefficients; C has one nonzero per row at a uniform column j != 0 with coefficient
+(Az)_i (Bz)_i / z_j, so A z o B z = C z holds.  z[0] = 1 is the single public input and the
+other entries are uniform in the balanced range |z| <= 2^54 - 1: with b = 2 and D = 54 the
+balanced decomposition (neo-ajtai/src/decomp.rs:118-158) represents exactly that range, so a
+z entry uniform in all of F_q would not recompose (neo-ajtai/tests/decomp_representability_limit.rs).
+
+The instance is fed unchanged to the MI355X prover and to the CPU restatement.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+Q = 0xFFFFFFFF00000001
+ZMAX = (1 << 54) - 1
+
+
+def _rand_u64(rng: np.random.Generator, size: int) -> np.ndarray:
+    hi = rng.integers(0, 1 << 32, size=size, dtype=np.uint64)
+    lo = rng.integers(0, 1 << 32, size=size, dtype=np.uint64)
+    return (hi << np.uint64(32)) | lo
+
+
+def rand_field(rng: np.random.Generator, size: int, nonzero: bool = True) -> np.ndarray:
+    """Uniform F_q elements by rejection (x < q, and x != 0 when nonzero)."""
+    out = np.empty(size, dtype=np.uint64)
+    filled = 0
+    while filled < size:
+        x = _rand_u64(rng, size - filled)
+        ok = x < np.uint64(Q)
+        if nonzero:
+            ok &= x != 0
+        x = x[ok]
+        out[filled:filled + len(x)] = x
+        filled += len(x)
+    return out
+
+
+def rand_balanced_z(rng: np.random.Generator, m: int) -> np.ndarray:
+    """z[0] = 1, z[1:] uniform in [-(2^54-1), 2^54-1] mapped into F_q."""
+    mag = rng.integers(-ZMAX, ZMAX + 1, size=m, dtype=np.int64)
+    z = np.where(mag >= 0, mag.astype(np.uint64), (np.uint64(Q) - (-mag).astype(np.uint64)))
+    z = z.astype(np.uint64)
+    z[0] = 1
+    return z
+
+
+class R1CS:
+    """CSR matrices A, B, C (n x m) and a satisfying witness z (z[0] = 1, m_in = 1)."""
+
+    def __init__(self, n, m, a, b, c, z):
+        self.n, self.m = n, m
+        self.a, self.b, self.c = a, b, c  # (rowptr u64[n+1], cols u32[nnz], vals u64[nnz])
+        self.z = z
+        self.m_in = 1
+
+
+def gen_r1cs(n: int, seed: int, nsteps: int = 1) -> tuple[R1CS, list[np.ndarray]]:
+    """One R1CS instance and the witnesses of `nsteps` fold steps.
+
+    A shard proves many steps against one CCS structure.  C's coefficients are solved from z,
+    so a second independent uniform z would not satisfy the same C; every step of the shard
+    therefore folds the same satisfying z.  Proving work is independent of witness values
+    (exact field arithmetic, digit planes of the same distribution), and every step still
+    folds a different accumulator because each step draws fresh challenges.
+    """
+    m = n
+    rng = np.random.default_rng(seed)
+    z = rand_balanced_z(rng, m)
+    zi = [int(v) for v in z]
+
+    def rand_rows():
+        cols = np.empty((n, 3), dtype=np.uint32)
+        for i in range(n):
+            cols[i] = rng.choice(m, size=3, replace=False)
+        cols.sort(axis=1)
+        vals = rand_field(rng, 3 * n).reshape(n, 3)
+        return cols, vals
+
+    a_cols, a_vals = rand_rows()
+    b_cols, b_vals = rand_rows()
+    c_cols = np.empty(n, dtype=np.uint32)
+    c_vals = np.empty(n, dtype=np.uint64)
+    jj = rng.integers(1, m, size=n)
+    for i in range(n):
+        az = sum(int(a_vals[i, k]) * zi[int(a_cols[i, k])] for k in range(3)) % Q
+        bz = sum(int(b_vals[i, k]) * zi[int(b_cols[i, k])] for k in range(3)) % Q
+        j = int(jj[i])
+        while zi[j] == 0:
+            j = 1 + (j % (m - 1))
+        c_cols[i] = j
+        c_vals[i] = (az * bz % Q) * pow(zi[j], Q - 2, Q) % Q
+    rp3 = np.arange(0, 3 * n + 1, 3, dtype=np.uint64)
+    rp1 = np.arange(0, n + 1, dtype=np.uint64)
+    a = (rp3, a_cols.reshape(-1).copy(), a_vals.reshape(-1).copy())
+    b = (rp3, b_cols.reshape(-1).copy(), b_vals.reshape(-1).copy())
+    c = (rp1, c_cols, c_vals)
+    inst = R1CS(n, m, a, b, c, z)
+    return inst, [z.copy() for _ in range(nsteps)]
+
+
+def check_r1cs(inst: R1CS, z: np.ndarray) -> bool:
+    zi = [int(v) for v in z]
+
+    def mv(mat, i):
+        rp, ci, v = mat
+        return sum(int(v[k]) * zi[int(ci[k])] for k in range(int(rp[i]), int(rp[i + 1]))) % Q
+
+    return all(mv(inst.a, i) * mv(inst.b, i) % Q == mv(inst.c, i) for i in range(inst.n))

@@ -1,0 +1,259 @@
+"""ctypes bindings to the CPU restatement (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline.  The MI355X product library never loads this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+Q = 0xFFFFFFFF00000001
+D = 54
+
+_lib = None
+
+
+class KX(C.Structure):
+    """K element passed by value (struct { uint64_t c0, c1; })."""
+    _fields_ = [("c0", C.c_uint64), ("c1", C.c_uint64)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+        i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        L.or_fq_mul_batch.argtypes = [u64p, u64p, u64p, C.c_size_t]
+        L.or_k_mul_batch.argtypes = [u64p, u64p, u64p, C.c_size_t]
+        L.or_k_inv_batch.argtypes = [u64p, u64p, C.c_size_t]
+        L.or_rq_mul.argtypes = [u64p, u64p, u64p]
+        L.or_rot_step.argtypes = [u64p, u64p]
+        L.or_chacha_ietf_block.argtypes = [u32p, C.c_uint32, u32p, C.c_int, u32p]
+        L.or_chacha_block.argtypes = [u32p, C.c_uint64, C.c_uint64, C.c_int, u32p]
+        L.or_p2_permute8.argtypes = [u64p]
+        L.or_decomp_b2_row_major.argtypes = [u64p, C.c_size_t, i8p]
+        L.or_ajtai_setup.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, u64p]
+        L.or_ajtai_commit.argtypes = [u64p, C.c_size_t, C.c_size_t, i8p, u64p]
+        L.or_ccs_from_r1cs.restype = C.c_void_p
+        L.or_ccs_from_r1cs.argtypes = [C.c_size_t, C.c_size_t] + [u64p, u32p, u64p] * 3
+        L.or_ccs_free.argtypes = [C.c_void_p]
+        L.or_ccs_digest.argtypes = [C.c_void_p, u64p]
+        L.or_chi_table.argtypes = [u64p, C.c_size_t, u64p]
+        L.or_rowstate_build.restype = C.c_void_p
+        L.or_rowstate_build.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, u64p, u64p, u64p,
+                                        KX, C.c_size_t, C.c_size_t, C.c_void_p]
+        L.or_rowstate_free.argtypes = [C.c_void_p]
+        L.or_rowstate_round_coeffs.restype = C.c_size_t
+        L.or_rowstate_round_coeffs.argtypes = [C.c_void_p, u64p]
+        L.or_rowstate_evals.argtypes = [C.c_void_p, u64p, C.c_size_t, u64p]
+        L.or_rowstate_fold.argtypes = [C.c_void_p, KX]
+        L.or_rowstate_len.restype = C.c_size_t
+        L.or_rowstate_len.argtypes = [C.c_void_p]
+        L.or_rowstate_export.restype = C.c_size_t
+        L.or_rowstate_export.argtypes = [C.c_void_p, C.c_int, C.c_size_t, u64p, C.c_size_t]
+        L.or_precompute_y.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, u64p, C.c_size_t, u64p]
+        L.or_vjs.argtypes = [C.c_void_p, u64p, C.c_size_t, u64p]
+        L.or_interpolate.argtypes = [u64p, u64p, C.c_size_t, u64p]
+        L.or_rot_from_coeffs.argtypes = [i8p, u64p]
+        L.or_rlc_zmix.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, i32p]
+        L.or_split_b.restype = C.c_int
+        L.or_split_b.argtypes = [i32p, C.c_size_t, C.c_size_t, C.c_void_p, C.POINTER(C.c_int)]
+        L.or_prover_new.restype = C.c_void_p
+        L.or_prover_new.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.or_prover_free.argtypes = [C.c_void_p]
+        L.or_prove_shard.restype = C.c_int
+        L.or_prove_shard.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                     C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t),
+                                     C.POINTER(C.c_double)]
+        L.or_free.argtypes = [C.c_void_p]
+        L.or_last_error.restype = C.c_char_p
+        L.or_params_auto_r1cs.restype = C.c_int
+        L.or_extension_check.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptrs(arrays):
+    arr = (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+    return arr
+
+
+def _k2(x):
+    return KX(int(x[0]), int(x[1]))
+
+
+class Ccs:
+    def __init__(self, inst):
+        L = lib()
+        self._keep = [inst.a, inst.b, inst.c]
+        args = []
+        for (rp, ci, v) in (inst.a, inst.b, inst.c):
+            args += [np.ascontiguousarray(rp, np.uint64), np.ascontiguousarray(ci, np.uint32),
+                     np.ascontiguousarray(v, np.uint64)]
+        self._args = args
+        self.ptr = L.or_ccs_from_r1cs(inst.n, inst.m, *args)
+        self.n, self.m, self.t = inst.n, inst.m, 4
+
+    def digest(self):
+        out = np.zeros(4, np.uint64)
+        lib().or_ccs_digest(self.ptr, out)
+        return out
+
+    def __del__(self):
+        try:
+            lib().or_ccs_free(self.ptr)
+        except Exception:
+            pass
+
+
+def decomp(z: np.ndarray) -> np.ndarray:
+    z = np.ascontiguousarray(z, np.uint64)
+    Z = np.zeros(D * len(z), np.int8)
+    lib().or_decomp_b2_row_major(z, len(z), Z)
+    return Z.reshape(D, len(z))
+
+
+def ajtai_setup(seed: bytes, kappa: int, m: int) -> np.ndarray:
+    pp = np.zeros(kappa * m * D, np.uint64)
+    lib().or_ajtai_setup(seed, kappa, m, pp)
+    return pp
+
+
+def ajtai_commit(pp: np.ndarray, kappa: int, m: int, Z: np.ndarray) -> np.ndarray:
+    out = np.zeros(kappa * D, np.uint64)
+    lib().or_ajtai_commit(np.ascontiguousarray(pp, np.uint64), kappa, m, np.ascontiguousarray(Z, np.int8).reshape(-1), out)
+    return out
+
+
+class RowState:
+    """Pi_CCS row-phase oracle state (optimized_engine/oracle.rs RowStreamState)."""
+
+    def __init__(self, ccs: Ccs, Zs, alpha, beta_a, beta_r, gamma, ell_d, ell_n, r_inputs=None):
+        L = lib()
+        self.Zs = [np.ascontiguousarray(z, np.int8).reshape(-1) for z in Zs]
+        self._zp = _ptrs(self.Zs)
+        a = np.ascontiguousarray(np.asarray(alpha, np.uint64).reshape(-1))
+        ba = np.ascontiguousarray(np.asarray(beta_a, np.uint64).reshape(-1))
+        br = np.ascontiguousarray(np.asarray(beta_r, np.uint64).reshape(-1))
+        self._keep = [a, ba, br]
+        rin = None
+        if r_inputs is not None:
+            rin = np.ascontiguousarray(np.asarray(r_inputs, np.uint64).reshape(-1))
+            self._keep.append(rin)
+        self.ptr = L.or_rowstate_build(ccs.ptr, self._zp, len(self.Zs), a, ba, br, _k2(gamma), ell_d, ell_n,
+                                       None if rin is None else rin.ctypes.data)
+
+    def coeffs(self):
+        out = np.zeros(32, np.uint64)
+        n = lib().or_rowstate_round_coeffs(self.ptr, out)
+        return out[:2 * n].reshape(n, 2)
+
+    def evals(self, xs):
+        xs = np.ascontiguousarray(np.asarray(xs, np.uint64).reshape(-1))
+        out = np.zeros_like(xs)
+        lib().or_rowstate_evals(self.ptr, xs, len(xs) // 2, out)
+        return out.reshape(-1, 2)
+
+    def fold(self, r):
+        lib().or_rowstate_fold(self.ptr, _k2(r))
+
+    def length(self):
+        return lib().or_rowstate_len(self.ptr)
+
+    def export(self, which, idx=0):
+        cap = self.length() * D
+        out = np.zeros(2 * cap, np.uint64)
+        n = lib().or_rowstate_export(self.ptr, which, idx, out, cap)
+        return out[:2 * n].reshape(n, 2)
+
+    def __del__(self):
+        try:
+            lib().or_rowstate_free(self.ptr)
+        except Exception:
+            pass
+
+
+def precompute_y(ccs: Ccs, Zs, r, ell_n):
+    Zs = [np.ascontiguousarray(z, np.int8).reshape(-1) for z in Zs]
+    r = np.ascontiguousarray(np.asarray(r, np.uint64).reshape(-1))
+    out = np.zeros(len(Zs) * ccs.t * D * 2, np.uint64)
+    lib().or_precompute_y(ccs.ptr, _ptrs(Zs), len(Zs), r, ell_n, out)
+    return out.reshape(len(Zs), ccs.t, D, 2)
+
+
+def vjs(ccs: Ccs, r, ell_n):
+    r = np.ascontiguousarray(np.asarray(r, np.uint64).reshape(-1))
+    out = np.zeros(ccs.t * ccs.m * 2, np.uint64)
+    lib().or_vjs(ccs.ptr, r, ell_n, out)
+    return out.reshape(ccs.t, ccs.m, 2)
+
+
+def rot_from_coeffs(a) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, np.int8))
+    out = np.zeros(D * D, np.uint64)
+    lib().or_rot_from_coeffs(a, out)
+    return out.reshape(D, D)
+
+
+def rlc_zmix(rhos, Zs, m):
+    rhos = [np.ascontiguousarray(r, np.uint64).reshape(-1) for r in rhos]
+    Zs = [np.ascontiguousarray(z, np.int8).reshape(-1) for z in Zs]
+    out = np.zeros(D * m, np.int32)
+    lib().or_rlc_zmix(_ptrs(rhos), _ptrs(Zs), len(Zs), m, out)
+    return out.reshape(D, m)
+
+
+def split_b(zmix: np.ndarray, k: int):
+    zmix = np.ascontiguousarray(zmix, np.int32).reshape(-1)
+    outs = [np.zeros(len(zmix), np.int8) for _ in range(k)]
+    nz = (C.c_int * k)()
+    rc = lib().or_split_b(zmix, len(zmix), k, _ptrs(outs), nz)
+    if rc != 0:
+        raise ValueError("DEC split out of range")
+    return outs, [bool(x) for x in nz]
+
+
+class Prover:
+    """CPU shard prover (fold loop restatement); proves with the canonical proof encoding."""
+
+    def __init__(self, inst, pp_seed: bytes, n_threads: int = 0):
+        self.ccs = Ccs(inst)
+        self.ptr = lib().or_prover_new(self.ccs.ptr, pp_seed, n_threads)
+        if not self.ptr:
+            raise RuntimeError(lib().or_last_error().decode())
+
+    def prove(self, zs, m_in=1):
+        zs = [np.ascontiguousarray(z, np.uint64) for z in zs]
+        out = C.POINTER(C.c_uint8)()
+        n = C.c_size_t()
+        sec = C.c_double()
+        rc = lib().or_prove_shard(self.ptr, _ptrs(zs), len(zs), m_in, C.byref(out), C.byref(n), C.byref(sec))
+        if rc != 0:
+            raise RuntimeError(f"oracle prove failed ({rc}): {lib().or_last_error().decode()}")
+        data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)
+        lib().or_free(out)
+        return data, sec.value
+
+    def __del__(self):
+        try:
+            lib().or_prover_free(self.ptr)
+        except Exception:
+            pass

@@ -1,0 +1,948 @@
+/*
+ * shard.c -- CCS-only shard fold loop (Route A) on the CPU, with the canonical proof encoding.
+ * TEST INFRASTRUCTURE ONLY (checker for nightstream_amd; see neo_oracle.h).  bench.py times
+ * this file's prover as the CPU baseline ("port").
+ *
+ * Restates, in transcript order:
+ *   neo-fold/src/session.rs:1018-1061        step intake: z -> balanced digits -> Ajtai commit
+ *   neo-fold/src/shard.rs:1725-2283          fold_shard_prove_impl (CCS-only steps)
+ *   neo-fold/src/memory_sidecar/memory.rs:82-124   absorb_step_memory_impl with no LUT/MEM
+ *   neo-reductions/src/engines/utils.rs:29-64      build_dims_and_policy
+ *   neo-reductions/src/engines/utils.rs:80-135     bind_header_and_instances_with_digest
+ *   neo-reductions/src/engines/utils.rs:138-181    bind_me_inputs (v2 framing)
+ *   neo-reductions/src/engines/utils.rs:184-212    sample_challenges
+ *   neo-reductions/src/engines/paper_exact_engine/paper_exact.rs:825-944  claimed_initial_sum
+ *   neo-memory/src/ts_common.rs:22-37               sample_ext_point (r_cycle)
+ *   neo-fold/src/memory_sidecar/transcript.rs:37-64 bind_batched_dynamic_claims
+ *   neo-fold/src/memory_sidecar/sumcheck_ds.rs:11-77 sc_start / sc_end wrappers
+ *   neo-reductions/src/sumcheck.rs:124-209, 350-456 single and batched sum-check provers
+ *   optimized_engine/oracle.rs:1812-1951            evals_ajtai_phase (Ajtai-bit rounds)
+ *   optimized_engine/oracle.rs:1955-2072            build_me_outputs_from_ajtai_precomp
+ *   neo-fold/src/shard.rs:1125-1331                 bind_rlc_inputs, prove_rlc_dec_lane
+ *   neo-reductions/src/common.rs:370-573            rot_from_coeffs, draw_alphabet_vector, sample_rot_rhos_n
+ *   optimized_engine/common.rs:1221-1379            rlc_reduction_optimized; api.rs:131-211 (k = 1 path)
+ *   neo-reductions/src/common.rs:75-268             split_b_matrix_k_with_nonzero_flags
+ *   optimized_engine/common.rs:1445-1628            DEC children, ok_y / ok_X checks
+ *   neo-fold/src/session.rs:301-324                 default mixers (s_lincomb, combine_b_pows)
+ * Pinned by: tests/test_oracle_shard.py (every step's sum-checks verify with p(0)+p(1)
+ * invariants, the Ajtai-phase final value equals the paper-exact terminal identity, DEC
+ * recomposition checks hold, RLC y equals Z_mix * v_j).  Transcript bytes: parity unpinned.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "neo_oracle.h"
+
+#define D NEO_D
+#define DPAD 64
+#define KAPPA 16
+
+static char g_err[512];
+const char* or_last_error(void) { return g_err; }
+void or_free(void* p) { free(p); }
+
+static const kx K0 = {0, 0};
+static const kx K1 = {1, 0};
+
+/* ---------------- growable u64 buffer for the canonical encoding ---------------- */
+typedef struct { uint64_t* v; size_t n, cap; } u64buf;
+static void ub_push(u64buf* b, uint64_t x) {
+  if (b->n == b->cap) {
+    b->cap = b->cap ? b->cap * 2 : 4096;
+    b->v = (uint64_t*)realloc(b->v, b->cap * sizeof(uint64_t));
+  }
+  b->v[b->n++] = x;
+}
+static void ub_k(u64buf* b, kx x) { ub_push(b, x.c0); ub_push(b, x.c1); }
+static void ub_digest(u64buf* b, const uint8_t d[32]) {
+  for (int i = 0; i < 4; i++) {
+    uint64_t w = 0;
+    for (int j = 0; j < 8; j++) w |= (uint64_t)d[8 * i + j] << (8 * j);
+    ub_push(b, w);
+  }
+}
+
+/* ---------------- ME instance ---------------- */
+typedef struct {
+  fq c[KAPPA * D];
+  size_t m_in;
+  fq* X;           /* D x m_in */
+  size_t ell_n;
+  kx* r;
+  size_t t;
+  kx* y;           /* t x 64 */
+  kx* y_scalars;   /* t */
+  uint8_t fold_digest[32];
+} me_t;
+
+static void me_init(me_t* e, size_t m_in, size_t ell_n, size_t t) {
+  memset(e, 0, sizeof(*e));
+  e->m_in = m_in;
+  e->ell_n = ell_n;
+  e->t = t;
+  e->X = (fq*)calloc(D * (m_in ? m_in : 1), sizeof(fq));
+  e->r = (kx*)calloc(ell_n ? ell_n : 1, sizeof(kx));
+  e->y = (kx*)calloc(t * DPAD, sizeof(kx));
+  e->y_scalars = (kx*)calloc(t, sizeof(kx));
+}
+static void me_free(me_t* e) {
+  free(e->X);
+  free(e->r);
+  free(e->y);
+  free(e->y_scalars);
+}
+static void me_copy(me_t* dst, const me_t* src) {
+  me_init(dst, src->m_in, src->ell_n, src->t);
+  memcpy(dst->c, src->c, sizeof(src->c));
+  memcpy(dst->X, src->X, D * src->m_in * sizeof(fq));
+  memcpy(dst->r, src->r, src->ell_n * sizeof(kx));
+  memcpy(dst->y, src->y, src->t * DPAD * sizeof(kx));
+  memcpy(dst->y_scalars, src->y_scalars, src->t * sizeof(kx));
+  memcpy(dst->fold_digest, src->fold_digest, 32);
+}
+static void ser_me(u64buf* b, const me_t* e) {
+  for (int i = 0; i < KAPPA * D; i++) ub_push(b, e->c[i]);
+  ub_push(b, e->m_in);
+  for (size_t i = 0; i < D * e->m_in; i++) ub_push(b, e->X[i]);
+  ub_push(b, e->ell_n);
+  for (size_t i = 0; i < e->ell_n; i++) ub_k(b, e->r[i]);
+  ub_push(b, e->t);
+  for (size_t j = 0; j < e->t; j++) {
+    ub_push(b, DPAD);
+    for (int i = 0; i < DPAD; i++) ub_k(b, e->y[j * DPAD + i]);
+  }
+  ub_push(b, e->t);
+  for (size_t j = 0; j < e->t; j++) ub_k(b, e->y_scalars[j]);
+  ub_digest(b, e->fold_digest);
+  ub_push(b, 0); /* c_step_coords.len() */
+  ub_push(b, 0); /* u_offset */
+  ub_push(b, 0); /* u_len */
+}
+
+/* ---------------- transcript helpers ---------------- */
+static void tr_msg_u64(or_tr* t, const char* label, uint64_t v) {
+  uint8_t b[8];
+  for (int i = 0; i < 8; i++) b[i] = (uint8_t)(v >> (8 * i));
+  or_tr_append_message(t, label, b, 8);
+}
+static void tr_msg_empty(or_tr* t, const char* label) { or_tr_append_message(t, label, NULL, 0); }
+static void tr_k(or_tr* t, const char* label, kx x) {
+  fq f[2] = {x.c0, x.c1};
+  or_tr_append_fields(t, label, f, 2);
+}
+static kx tr_chal_k2(or_tr* t, const char* l0, const char* l1) {
+  kx r;
+  r.c0 = or_tr_challenge_field(t, l0);
+  r.c1 = or_tr_challenge_field(t, l1);
+  return r;
+}
+
+/* ---------------- prover ---------------- */
+struct or_prover {
+  const or_ccs* s;
+  or_params params;
+  fq* pp; /* kappa * m * 54 */
+  fq digest[4];
+  int n_threads;
+};
+
+or_prover* or_prover_new(const or_ccs* s, const uint8_t pp_seed[32], int n_threads) {
+  or_prover* p = (or_prover*)calloc(1, sizeof(or_prover));
+  p->s = s;
+  if (or_params_auto_r1cs(s->n, &p->params) != 0) {
+    snprintf(g_err, sizeof(g_err), "params: unsupported n");
+    free(p);
+    return NULL;
+  }
+  p->pp = (fq*)malloc((size_t)KAPPA * s->m * D * sizeof(fq));
+  or_ajtai_setup(pp_seed, KAPPA, s->m, p->pp);
+  or_ccs_digest(s, p->digest);
+  p->n_threads = n_threads;
+  return p;
+}
+
+void or_prover_free(or_prover* p) {
+  if (!p) return;
+  free(p->pp);
+  free(p);
+}
+
+static size_t log2_ceil_pow2(size_t n) {
+  size_t p = 1, l = 0;
+  while (p < n) {
+    p <<= 1;
+    l++;
+  }
+  if (p < 2) l = 1;
+  return l;
+}
+
+/* Ajtai-phase round evaluation (oracle.rs:1812-1951). */
+typedef struct {
+  size_t k_total, t, ell_d;
+  const kx* y_eval;  /* k_total*t*54 */
+  kx f_prime, eq_beta_r, eq_r_inputs;
+  int has_inputs;
+  const kx *alpha, *beta_a;
+  kx gamma;
+  kx ajtai_chals[8];
+  size_t nchals;
+} ajtai_ctx;
+
+static kx eq_lin(kx a, kx b) { return or_k_add(or_k_mul(or_k_sub(K1, a), or_k_sub(K1, b)), or_k_mul(a, b)); }
+
+static void fold_bit_inplace(kx* dg, size_t bit, kx a) {
+  size_t stride = (size_t)1 << bit, step = stride << 1;
+  for (size_t base = 0; base < D; base += step)
+    for (size_t off = 0; off < stride; off++) {
+      size_t i0 = base + off;
+      if (i0 >= D) break;
+      size_t i1 = i0 + stride;
+      kx lo = dg[i0], hi = i1 < D ? dg[i1] : K0;
+      dg[i0] = or_k_add(lo, or_k_mul(or_k_sub(hi, lo), a));
+    }
+}
+
+static void ajtai_evals(const ajtai_ctx* c, const kx* xs, size_t nx, kx* out) {
+  size_t j = c->nchals, ell_d = c->ell_d, kt = c->k_total, t = c->t;
+  size_t free_a = ell_d - j - 1, tail_len = (size_t)1 << free_a, head_stride = (size_t)1 << (j + 1);
+  kx wbt[64], wat[64];
+  or_chi_table(c->beta_a + j + 1, free_a, wbt);
+  or_chi_table(c->alpha + j + 1, free_a, wat);
+  kx eqb_pref = K1, eqa_pref = K1;
+  for (size_t i = 0; i < j; i++) {
+    eqb_pref = or_k_mul(eqb_pref, eq_lin(c->ajtai_chals[i], c->beta_a[i]));
+    eqa_pref = or_k_mul(eqa_pref, eq_lin(c->ajtai_chals[i], c->alpha[i]));
+  }
+  kx* gpi = (kx*)malloc(kt * sizeof(kx));
+  gpi[0] = K1;
+  for (size_t i = 1; i < kt; i++) gpi[i] = or_k_mul(gpi[i - 1], c->gamma);
+  kx gk = K1;
+  for (size_t i = 0; i < kt; i++) gk = or_k_mul(gk, c->gamma);
+  kx gkj[8];
+  gkj[0] = K1;
+  for (size_t i = 1; i < t; i++) gkj[i] = or_k_mul(gkj[i - 1], gk);
+  /* prefold y_nc (= y_eval[i][0]) and y_eval by the fixed prefix bits */
+  kx* ync = (kx*)malloc(kt * D * sizeof(kx));
+  kx* yev = (kx*)malloc(kt * t * D * sizeof(kx));
+  for (size_t i = 0; i < kt; i++) {
+    memcpy(ync + i * D, c->y_eval + (i * t + 0) * D, D * sizeof(kx));
+    for (size_t b = 0; b < j; b++) fold_bit_inplace(ync + i * D, b, c->ajtai_chals[b]);
+    for (size_t jm = 0; jm < t; jm++) {
+      memcpy(yev + (i * t + jm) * D, c->y_eval + (i * t + jm) * D, D * sizeof(kx));
+      for (size_t b = 0; b < j; b++) fold_bit_inplace(yev + (i * t + jm) * D, b, c->ajtai_chals[b]);
+    }
+  }
+  kx tmp[D];
+  for (size_t xi = 0; xi < nx; xi++) {
+    kx x = xs[xi];
+    kx eq_beta = or_k_mul(c->eq_beta_r, or_k_mul(eqb_pref, eq_lin(x, c->beta_a[j])));
+    kx eq_ar = c->has_inputs ? or_k_mul(c->eq_r_inputs, or_k_mul(eqa_pref, eq_lin(x, c->alpha[j]))) : K0;
+    kx nc_sum = K0, g = c->gamma;
+    for (size_t i = 0; i < kt; i++) {
+      memcpy(tmp, ync + i * D, sizeof(tmp));
+      fold_bit_inplace(tmp, j, x);
+      kx acc = K0;
+      for (size_t tt = 0; tt < tail_len; tt++) {
+        size_t idx = tt * head_stride;
+        if (idx < D) {
+          kx y = tmp[idx];
+          kx ny = or_k_mul(y, or_k_sub(or_k_mul(y, y), K1)); /* y (y^2 - 1) */
+          acc = or_k_add(acc, or_k_mul(wbt[tt], ny));
+        }
+      }
+      nc_sum = or_k_add(nc_sum, or_k_mul(g, acc));
+      g = or_k_mul(g, c->gamma);
+    }
+    kx o = or_k_mul(eq_beta, or_k_add(c->f_prime, nc_sum));
+    if (kt >= 2 && !or_k_eq(eq_ar, K0)) {
+      kx inner = K0;
+      for (size_t jm = 0; jm < t; jm++) {
+        kx sj = K0;
+        for (size_t i = 1; i < kt; i++) {
+          memcpy(tmp, yev + (i * t + jm) * D, sizeof(tmp));
+          fold_bit_inplace(tmp, j, x);
+          kx yd = K0;
+          for (size_t tt = 0; tt < tail_len; tt++) {
+            size_t idx = tt * head_stride;
+            if (idx < D) yd = or_k_add(yd, or_k_mul(wat[tt], tmp[idx]));
+          }
+          sj = or_k_add(sj, or_k_mul(or_k_mul(gpi[i], gkj[jm]), yd));
+        }
+        inner = or_k_add(inner, sj);
+      }
+      o = or_k_add(o, or_k_mul(eq_ar, or_k_mul(gk, inner)));
+    }
+    out[xi] = o;
+  }
+  free(gpi);
+  free(ync);
+  free(yev);
+}
+
+/* f evaluated at m_vals (SparsePoly::eval_in_ext) */
+static kx eval_f(const or_ccs* s, const kx* mv) {
+  kx acc = K0;
+  for (size_t tt = 0; tt < s->nterms; tt++) {
+    kx term = or_k_from(s->term_coeff[tt]);
+    for (size_t j = 0; j < s->t; j++)
+      for (uint32_t e = 0; e < s->term_exps[tt][j]; e++) term = or_k_mul(term, mv[j]);
+    acc = or_k_add(acc, term);
+  }
+  return acc;
+}
+
+static kx eq_points(const kx* p, const kx* q, size_t n) {
+  kx acc = K1;
+  for (size_t i = 0; i < n; i++) acc = or_k_mul(acc, eq_lin(p[i], q[i]));
+  return acc;
+}
+
+static void draw_alphabet(or_tr* t, size_t need, int8_t* out, uint64_t seed) {
+  static const int8_t A5[5] = {-2, -1, 0, 1, 2};
+  const uint32_t bucket = (65536u / 5u) * 5u;
+  size_t got = 0;
+  uint64_t ctr = seed;
+  while (got < need) {
+    tr_msg_u64(t, "rlc/rot/chunk", ctr);
+    uint8_t dig[32];
+    or_tr_digest32(t, dig);
+    for (int w = 0; w < 16; w++) {
+      uint32_t x = (uint32_t)dig[2 * w] | ((uint32_t)dig[2 * w + 1] << 8);
+      if (x < bucket) {
+        out[got++] = A5[x % 5];
+        if (got == need) break;
+      }
+    }
+    ctr++;
+  }
+}
+
+void or_rot_from_coeffs(const int8_t* a, fq* rho) {
+  fq col[D], nxt[D];
+  for (int r = 0; r < D; r++) col[r] = or_fq_from_i64(a[r]);
+  for (int j = 0; j < D; j++) {
+    for (int r = 0; r < D; r++) rho[r * D + j] = col[r];
+    or_rot_step(col, nxt);
+    memcpy(col, nxt, sizeof(col));
+  }
+}
+
+void or_rlc_zmix(const fq* const* rhos, const int8_t* const* Zs, size_t k, size_t m, int32_t* zmix) {
+  memset(zmix, 0, D * m * sizeof(int32_t));
+#pragma omp parallel for schedule(static)
+  for (long rr = 0; rr < D; rr++) {
+    for (size_t i = 0; i < k; i++)
+      for (size_t kk = 0; kk < D; kk++) {
+        int64_t coeff = or_fq_to_balanced(rhos[i][(size_t)rr * D + kk]);
+        if (!coeff) continue;
+        const int8_t* zr = Zs[i] + kk * m;
+        int32_t* out = zmix + (size_t)rr * m;
+        for (size_t c = 0; c < m; c++) out[c] += (int32_t)coeff * zr[c];
+      }
+  }
+}
+
+int or_split_b(const int32_t* zmix, size_t len, size_t k, int8_t* const* outs, int* nonzero) {
+  for (size_t i = 0; i < k; i++) {
+    memset(outs[i], 0, len);
+    nonzero[i] = 0;
+  }
+  int64_t B = (int64_t)1 << k;
+  for (size_t idx = 0; idx < len; idx++) {
+    int64_t v = zmix[idx];
+    if (v >= B || v <= -B) return -1;
+    for (size_t i = 0; i < k; i++) {
+      if (v == 0) break;
+      int64_t r = v % 2, q = (v - r) / 2;
+      if (r != 0) {
+        outs[i][idx] = (int8_t)r;
+        nonzero[i] = 1;
+      }
+      v = q;
+    }
+    if (v != 0) return -1;
+  }
+  return 0;
+}
+
+static int dims(const or_prover* p, size_t* ell_d, size_t* ell_n, size_t* ell, size_t* d_sc) {
+  const or_ccs* s = p->s;
+  *ell_d = 6;
+  *ell_n = log2_ceil_pow2(s->n);
+  *ell = *ell_d + *ell_n;
+  size_t fdeg = 0;
+  for (size_t tt = 0; tt < s->nterms; tt++) {
+    size_t d = 0;
+    for (size_t j = 0; j < s->t; j++) d += s->term_exps[tt][j];
+    if (d > fdeg) fdeg = d;
+  }
+  size_t a = fdeg + 1, bb = 2 * p->params.b + 2;
+  if (bb < 2) bb = 2;
+  *d_sc = a > bb ? a : bb;
+  return 0;
+}
+
+int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t m_in, uint8_t** out,
+                   size_t* out_len, double* seconds_per_step) {
+  const or_ccs* s = p->s;
+  size_t m = s->m, t = s->t, k_dec = p->params.k_rho;
+  size_t ell_d, ell_n, ell, d_sc;
+  dims(p, &ell_d, &ell_n, &ell, &d_sc);
+  uint32_t smin;
+  int32_t slack;
+  if (or_extension_check(&p->params, (uint32_t)ell, (uint32_t)d_sc, &smin, &slack) != 0 || slack < 0) {
+    snprintf(g_err, sizeof(g_err), "extension policy failed");
+    return -1;
+  }
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+
+  /* step intake (session.rs:1018-1061): digits + MCS commitment */
+  int8_t** Zmcs = (int8_t**)malloc(nsteps * sizeof(int8_t*));
+  fq(*cmcs)[KAPPA * D] = malloc(nsteps * sizeof(*cmcs));
+  for (size_t st = 0; st < nsteps; st++) {
+    Zmcs[st] = (int8_t*)malloc(D * m);
+    or_decomp_b2_row_major(z_steps[st], m, Zmcs[st]);
+    or_ajtai_commit(p->pp, KAPPA, m, Zmcs[st], cmcs[st]);
+  }
+
+  u64buf pb = {0};
+  ub_push(&pb, 0x314652504f52504eULL); /* "NPROPRF1" tag */
+  ub_push(&pb, nsteps);
+
+  or_tr tr;
+  or_tr_new(&tr, (const uint8_t*)"neo.fold/session", 16);
+
+  size_t nacc = 0;
+  me_t* acc = NULL;
+  int8_t** acc_wit = NULL;
+  const size_t xs_n = d_sc + 1;
+  kx xs[16];
+  for (size_t i = 0; i < xs_n; i++) xs[i] = or_k_from(i);
+  int rc = 0;
+
+  for (size_t idx = 0; idx < nsteps; idx++) {
+    /* absorb_step_memory with no LUT / MEM instances */
+    tr_msg_empty(&tr, "step/absorb_memory_start");
+    tr_msg_u64(&tr, "step/lut_count", 0);
+    tr_msg_u64(&tr, "step/mem_count", 0);
+    tr_msg_empty(&tr, "step/absorb_memory_done");
+
+    size_t k = nacc + 1;
+    /* bind_header_and_instances_with_digest */
+    tr_msg_empty(&tr, "phase/pi-ccs");
+    tr_msg_empty(&tr, "neo/ccs/header/v1");
+    {
+      uint64_t hdr[6] = {64, 2, p->params.lambda, ell, d_sc, (uint64_t)(slack < 0 ? -slack : slack)};
+      or_tr_append_u64s(&tr, "ccs/header", hdr, 6);
+      uint8_t sign = slack >= 0 ? 1 : 0;
+      or_tr_append_message(&tr, "ccs/slack_sign", &sign, 1);
+      tr_msg_empty(&tr, "neo/ccs/instances");
+      uint64_t dd[3] = {s->n, s->m, s->t};
+      or_tr_append_u64s(&tr, "dims", dd, 3);
+      for (int i = 0; i < 4; i++) or_tr_append_fields(&tr, "mat_digest", &p->digest[i], 1);
+      /* absorb_sparse_polynomial: terms sorted by exps (lexicographic) */
+      tr_msg_empty(&tr, "neo/ccs/poly");
+      uint64_t ar = s->t;
+      or_tr_append_u64s(&tr, "arity", &ar, 1);
+      uint64_t tl = s->nterms;
+      or_tr_append_u64s(&tr, "terms_len", &tl, 1);
+      size_t order[16];
+      for (size_t i = 0; i < s->nterms; i++) order[i] = i;
+      for (size_t i = 0; i < s->nterms; i++)
+        for (size_t j2 = i + 1; j2 < s->nterms; j2++) {
+          int less = 0;
+          for (size_t q = 0; q < s->t; q++) {
+            if (s->term_exps[order[j2]][q] != s->term_exps[order[i]][q]) {
+              less = s->term_exps[order[j2]][q] < s->term_exps[order[i]][q];
+              break;
+            }
+          }
+          if (less) {
+            size_t tmp = order[i];
+            order[i] = order[j2];
+            order[j2] = tmp;
+          }
+        }
+      for (size_t i = 0; i < s->nterms; i++) {
+        or_tr_append_fields(&tr, "coeff", &s->term_coeff[order[i]], 1);
+        uint64_t ex[8];
+        for (size_t q = 0; q < s->t; q++) ex[q] = s->term_exps[order[i]][q];
+        or_tr_append_u64s(&tr, "exps", ex, s->t);
+      }
+      /* the MCS instance */
+      fq xf[64];
+      for (size_t i = 0; i < m_in; i++) xf[i] = z_steps[idx][i];
+      or_tr_append_fields(&tr, "x", xf, m_in);
+      uint64_t mi = m_in;
+      or_tr_append_u64s(&tr, "m_in", &mi, 1);
+      or_tr_append_fields(&tr, "c_data", cmcs[idx], KAPPA * D);
+    }
+    /* bind_me_inputs */
+    tr_msg_empty(&tr, "neo/ccs/me_inputs/v2");
+    {
+      uint64_t cnt = nacc;
+      or_tr_append_u64s(&tr, "me_count", &cnt, 1);
+      for (size_t a = 0; a < nacc; a++) {
+        or_tr_append_fields(&tr, "c_data_in", acc[a].c, KAPPA * D);
+        uint64_t mi = acc[a].m_in;
+        or_tr_append_u64s(&tr, "m_in_in", &mi, 1);
+        fq* rf = (fq*)malloc(2 * acc[a].ell_n * sizeof(fq));
+        for (size_t i = 0; i < acc[a].ell_n; i++) {
+          rf[2 * i] = acc[a].r[i].c0;
+          rf[2 * i + 1] = acc[a].r[i].c1;
+        }
+        or_tr_append_fields(&tr, "r_in", rf, 2 * acc[a].ell_n);
+        free(rf);
+        fq* yf = (fq*)malloc(2 * acc[a].t * DPAD * sizeof(fq));
+        for (size_t i = 0; i < acc[a].t * DPAD; i++) {
+          yf[2 * i] = acc[a].y[i].c0;
+          yf[2 * i + 1] = acc[a].y[i].c1;
+        }
+        or_tr_append_fields(&tr, "y_elem", yf, 2 * acc[a].t * DPAD);
+        free(yf);
+      }
+    }
+    /* sample_challenges */
+    tr_msg_empty(&tr, "neo/ccs/chals/v1");
+    kx alpha[8], beta[64], gamma;
+    for (size_t i = 0; i < ell_d; i++) {
+      fq c2[2];
+      or_tr_challenge_fields(&tr, "chal/k", 2, c2);
+      alpha[i].c0 = c2[0];
+      alpha[i].c1 = c2[1];
+    }
+    for (size_t i = 0; i < ell; i++) {
+      fq c2[2];
+      or_tr_challenge_fields(&tr, "chal/k", 2, c2);
+      beta[i].c0 = c2[0];
+      beta[i].c1 = c2[1];
+    }
+    {
+      fq c2[2];
+      or_tr_challenge_fields(&tr, "chal/k", 2, c2);
+      gamma.c0 = c2[0];
+      gamma.c1 = c2[1];
+    }
+    const kx* beta_a = beta;
+    const kx* beta_r = beta + ell_d;
+    /* claimed initial sum T */
+    kx T = K0;
+    {
+      size_t kt = 1 + nacc;
+      if (kt >= 2) {
+        kx chi_a[DPAD];
+        or_chi_table(alpha, ell_d, chi_a);
+        kx gk = K1;
+        for (size_t i = 0; i < kt; i++) gk = or_k_mul(gk, gamma);
+        kx inner = K0;
+        for (size_t j = 0; j < t; j++)
+          for (size_t a = 0; a < nacc; a++) {
+            size_t i = a + 2;
+            kx ye = K0;
+            for (size_t rho = 0; rho < DPAD; rho++) ye = or_k_add(ye, or_k_mul(acc[a].y[j * DPAD + rho], chi_a[rho]));
+            kx w = K1;
+            for (size_t q = 0; q < i - 1; q++) w = or_k_mul(w, gamma);
+            for (size_t q = 0; q < j; q++) w = or_k_mul(w, gk);
+            inner = or_k_add(inner, or_k_mul(w, ye));
+          }
+        T = or_k_mul(gk, inner);
+      }
+    }
+    tr_k(&tr, "sumcheck/initial_sum", T);
+    /* r_cycle (sampled, unused by CCS-only steps) */
+    for (size_t i = 0; i < ell_n; i++) {
+      tr_msg_u64(&tr, "route_a/r_cycle", i);
+      (void)or_tr_challenge_field(&tr, "route_a/cycle/0");
+      (void)or_tr_challenge_field(&tr, "route_a/cycle/1");
+    }
+    /* oracle */
+    size_t kt = k;
+    const int8_t** Zs = (const int8_t**)malloc(kt * sizeof(int8_t*));
+    Zs[0] = Zmcs[idx];
+    for (size_t a = 0; a < nacc; a++) Zs[1 + a] = acc_wit[a];
+    or_rowstate* rs = or_rowstate_build(s, Zs, kt, alpha, beta_a, beta_r, gamma, ell_d, ell_n, nacc ? acc[0].r : NULL);
+
+    /* batched time sum-check (one CCS claim) */
+    tr_msg_u64(&tr, "batched/dynamic_bind/len", 1);
+    or_tr_append_message(&tr, "batched/dynamic_bind/claim_label", (const uint8_t*)"ccs/time", 8);
+    tr_msg_u64(&tr, "batched/dynamic_bind/claim_idx", 0);
+    tr_msg_u64(&tr, "batched/dynamic_bind/degree_bound", d_sc);
+    {
+      uint8_t one = 1;
+      or_tr_append_message(&tr, "batched/dynamic_bind/is_dynamic", &one, 1);
+    }
+    tr_k(&tr, "batched/dynamic_bind/claimed_sum", T);
+    or_tr_append_message(&tr, "sc/domain", (const uint8_t*)"shard/batched_time", 18);
+    tr_msg_u64(&tr, "sc/inst_idx", idx);
+    kx running = T;
+    kx* time_coeffs = (kx*)malloc(ell_n * xs_n * sizeof(kx));
+    kx* r_time = (kx*)malloc(ell_n * sizeof(kx));
+    for (size_t rd = 0; rd < ell_n; rd++) {
+      tr_msg_u64(&tr, "batched/round_idx", rd);
+      kx ys[16];
+      or_rowstate_evals(rs, xs, xs_n, ys);
+      if (!or_k_eq(or_k_add(ys[0], ys[1]), running)) {
+        snprintf(g_err, sizeof(g_err), "step %zu: batched round %zu invariant failed", idx, rd);
+        rc = -2;
+        break;
+      }
+      kx* co = time_coeffs + rd * xs_n;
+      or_interpolate(xs, ys, xs_n, co);
+      or_tr_append_message(&tr, "batched/claim_label", (const uint8_t*)"ccs/time", 8);
+      tr_msg_u64(&tr, "batched/claim_idx", 0);
+      for (size_t i = 0; i < xs_n; i++) tr_k(&tr, "batched/round/coeff", co[i]);
+      kx r = tr_chal_k2(&tr, "batched/challenge/0", "batched/challenge/1");
+      r_time[rd] = r;
+      running = or_poly_eval(co, xs_n, r);
+      or_rowstate_fold(rs, r);
+    }
+    or_rowstate_free(rs);
+    if (rc) break;
+    or_tr_append_message(&tr, "sc/domain_end", (const uint8_t*)"shard/batched_time", 18);
+    tr_msg_u64(&tr, "sc/inst_idx_end", idx);
+
+    /* Ajtai rounds */
+    kx* y_eval = (kx*)malloc(kt * t * D * sizeof(kx));
+    or_precompute_y(s, Zs, kt, r_time, ell_n, y_eval);
+    ajtai_ctx ac;
+    memset(&ac, 0, sizeof(ac));
+    ac.k_total = kt;
+    ac.t = t;
+    ac.ell_d = ell_d;
+    ac.y_eval = y_eval;
+    {
+      kx mv[8];
+      fq pw = 1;
+      for (size_t j = 0; j < t; j++) mv[j] = K0;
+      for (size_t rho = 0; rho < D; rho++) {
+        for (size_t j = 0; j < t; j++) mv[j] = or_k_add(mv[j], or_k_scale(y_eval[(0 * t + j) * D + rho], pw));
+        pw = or_fq_add(pw, pw);
+      }
+      ac.f_prime = eval_f(s, mv);
+    }
+    ac.eq_beta_r = eq_points(r_time, beta_r, ell_n);
+    ac.has_inputs = nacc > 0;
+    ac.eq_r_inputs = nacc ? eq_points(r_time, acc[0].r, ell_n) : K0;
+    ac.alpha = alpha;
+    ac.beta_a = beta_a;
+    ac.gamma = gamma;
+    kx ajtai_initial = running;
+    or_tr_append_message(&tr, "sc/domain", (const uint8_t*)"ccs/ajtai", 9);
+    tr_msg_u64(&tr, "sc/inst_idx", idx);
+    tr_k(&tr, "sc/claimed_sum", ajtai_initial);
+    kx* aj_coeffs = (kx*)malloc(ell_d * xs_n * sizeof(kx));
+    for (size_t rd = 0; rd < ell_d; rd++) {
+      kx ys[16];
+      ajtai_evals(&ac, xs, xs_n, ys);
+      if (!or_k_eq(or_k_add(ys[0], ys[1]), running)) {
+        snprintf(g_err, sizeof(g_err), "step %zu: ajtai round %zu invariant failed", idx, rd);
+        rc = -3;
+        break;
+      }
+      kx* co = aj_coeffs + rd * xs_n;
+      or_interpolate(xs, ys, xs_n, co);
+      for (size_t i = 0; i < xs_n; i++) tr_k(&tr, "sumcheck/round/coeff", co[i]);
+      kx r = tr_chal_k2(&tr, "sumcheck/challenge/0", "sumcheck/challenge/1");
+      running = or_poly_eval(co, xs_n, r);
+      ac.ajtai_chals[ac.nchals++] = r;
+    }
+    if (rc) break;
+    or_tr_append_message(&tr, "sc/domain_end", (const uint8_t*)"ccs/ajtai", 9);
+    tr_msg_u64(&tr, "sc/inst_idx_end", idx);
+    uint8_t fold_digest[32];
+    or_tr_digest32(&tr, fold_digest);
+
+    /* ME outputs */
+    me_t* ccs_out = (me_t*)calloc(kt, sizeof(me_t));
+    fq bpow[D];
+    bpow[0] = 1;
+    for (int i = 1; i < D; i++) bpow[i] = or_fq_add(bpow[i - 1], bpow[i - 1]);
+    for (size_t i = 0; i < kt; i++) {
+      me_init(&ccs_out[i], m_in, ell_n, t);
+      if (i == 0) {
+        memcpy(ccs_out[i].c, cmcs[idx], sizeof(ccs_out[i].c));
+        for (size_t r = 0; r < D; r++)
+          for (size_t c = 0; c < m_in; c++) ccs_out[i].X[r * m_in + c] = or_fq_from_i64(Zmcs[idx][r * m + c]);
+      } else {
+        memcpy(ccs_out[i].c, acc[i - 1].c, sizeof(ccs_out[i].c));
+        memcpy(ccs_out[i].X, acc[i - 1].X, D * m_in * sizeof(fq));
+      }
+      memcpy(ccs_out[i].r, r_time, ell_n * sizeof(kx));
+      for (size_t j = 0; j < t; j++) {
+        kx sc = K0;
+        for (size_t rho = 0; rho < D; rho++) {
+          kx v = y_eval[(i * t + j) * D + rho];
+          ccs_out[i].y[j * DPAD + rho] = v;
+          sc = or_k_add(sc, or_k_scale(v, bpow[rho]));
+        }
+        ccs_out[i].y_scalars[j] = sc;
+      }
+      memcpy(ccs_out[i].fold_digest, fold_digest, 32);
+    }
+
+
+    /* ---- Pi_RLC (main lane): bind_rlc_inputs (shard.rs:1125-1168) ---- */
+    or_tr_append_message(&tr, "fold/rlc_inputs/v1", (const uint8_t*)"main", 4);
+    {
+      uint64_t v = idx;
+      or_tr_append_u64s(&tr, "step_idx", &v, 1);
+      v = kt;
+      or_tr_append_u64s(&tr, "me_count", &v, 1);
+      for (size_t i = 0; i < kt; i++) {
+        const me_t* e = &ccs_out[i];
+        or_tr_append_fields(&tr, "c_data", e->c, KAPPA * D);
+        v = e->m_in;
+        or_tr_append_u64s(&tr, "m_in", &v, 1);
+        or_tr_append_message(&tr, "me_fold_digest", e->fold_digest, 32);
+        for (size_t q = 0; q < e->ell_n; q++) tr_k(&tr, "r_limb", e->r[q]);
+        or_tr_append_fields(&tr, "X", e->X, D * e->m_in);
+        for (size_t q = 0; q < e->t * DPAD; q++) tr_k(&tr, "y_elem", e->y[q]);
+        for (size_t q = 0; q < e->t; q++) tr_k(&tr, "y_scalar", e->y_scalars[q]);
+        v = 0;
+        or_tr_append_u64s(&tr, "c_step_coords_len", &v, 1);
+        or_tr_append_fields(&tr, "c_step_coords", NULL, 0);
+        or_tr_append_u64s(&tr, "u_offset", &v, 1);
+        or_tr_append_u64s(&tr, "u_len", &v, 1);
+      }
+    }
+    /* sample_rot_rhos_n (common.rs:467-573) */
+    if ((uint64_t)kt * 216u * (p->params.b - 1) >= ((uint64_t)1 << p->params.k_rho)) {
+      snprintf(g_err, sizeof(g_err), "RLC norm bound violated for count=%zu", kt);
+      rc = -4;
+      break;
+    }
+    fq** rhos = (fq**)malloc(kt * sizeof(fq*));
+    int8_t(*rho_a)[D] = malloc(kt * sizeof(*rho_a));
+    for (size_t i = 0; i < kt; i++) {
+      tr_msg_u64(&tr, "rlc/rot/index", i);
+      draw_alphabet(&tr, D, rho_a[i], i);
+      rhos[i] = (fq*)calloc(D * D, sizeof(fq));
+      if (kt == 1) {
+        for (int q = 0; q < D; q++) rhos[i][q * D + q] = 1;
+      } else {
+        or_rot_from_coeffs(rho_a[i], rhos[i]);
+      }
+    }
+    /* rlc_reduction_optimized / k = 1 fast path */
+    me_t parent;
+    me_init(&parent, m_in, ell_n, t);
+    memcpy(parent.r, ccs_out[0].r, ell_n * sizeof(kx));
+    memcpy(parent.fold_digest, ccs_out[0].fold_digest, 32);
+    int32_t* zmix = (int32_t*)malloc(D * m * sizeof(int32_t));
+    if (kt == 1) {
+      memcpy(parent.c, ccs_out[0].c, sizeof(parent.c));
+      memcpy(parent.X, ccs_out[0].X, D * m_in * sizeof(fq));
+      memcpy(parent.y, ccs_out[0].y, t * DPAD * sizeof(kx));
+      for (size_t i = 0; i < D * m; i++) zmix[i] = Zs[0][i];
+    } else {
+      for (size_t i = 0; i < kt; i++)
+        for (size_t rr = 0; rr < D; rr++)
+          for (size_t kk = 0; kk < D; kk++) {
+            fq co = rhos[i][rr * D + kk];
+            if (!co) continue;
+            for (size_t c = 0; c < m_in; c++)
+              parent.X[rr * m_in + c] = or_fq_add(parent.X[rr * m_in + c], or_fq_mul(co, ccs_out[i].X[kk * m_in + c]));
+          }
+      for (size_t j = 0; j < t; j++)
+        for (size_t i = 0; i < kt; i++)
+          for (size_t rr = 0; rr < D; rr++) {
+            kx a = K0;
+            for (size_t kk = 0; kk < D; kk++)
+              a = or_k_add(a, or_k_scale(ccs_out[i].y[j * DPAD + kk], rhos[i][rr * D + kk]));
+            parent.y[j * DPAD + rr] = or_k_add(parent.y[j * DPAD + rr], a);
+          }
+      or_rlc_zmix((const fq* const*)rhos, Zs, kt, m, zmix);
+      /* mix_rhos_commits: s_lincomb(rot_matrix_to_rq(rho_i), c_i) */
+      fq term[KAPPA * D], acoef[D];
+      memset(parent.c, 0, sizeof(parent.c));
+      for (size_t i = 0; i < kt; i++) {
+        for (int q = 0; q < D; q++) acoef[q] = rhos[i][q * D + 0];
+        or_s_mul(acoef, ccs_out[i].c, KAPPA, term);
+        for (int q = 0; q < KAPPA * D; q++) parent.c[q] = or_fq_add(parent.c[q], term[q]);
+      }
+    }
+    for (size_t j = 0; j < t; j++) {
+      kx sc = K0;
+      for (size_t rho = 0; rho < D; rho++) sc = or_k_add(sc, or_k_scale(parent.y[j * DPAD + rho], bpow[rho]));
+      parent.y_scalars[j] = sc;
+    }
+
+    /* ---- Pi_DEC ---- */
+    int8_t** zsplit = (int8_t**)malloc(k_dec * sizeof(int8_t*));
+    int nonzero[64];
+    for (size_t i = 0; i < k_dec; i++) zsplit[i] = (int8_t*)malloc(D * m);
+    if (or_split_b(zmix, D * m, k_dec, zsplit, nonzero) != 0) {
+      snprintf(g_err, sizeof(g_err), "step %zu: DEC split out of range", idx);
+      rc = -5;
+      break;
+    }
+    me_t* children = (me_t*)calloc(k_dec, sizeof(me_t));
+    kx* y_child = (kx*)malloc(k_dec * t * D * sizeof(kx));
+    or_precompute_y(s, (const int8_t* const*)zsplit, k_dec, parent.r, ell_n, y_child);
+    for (size_t i = 0; i < k_dec; i++) {
+      me_init(&children[i], m_in, ell_n, t);
+      if (nonzero[i]) or_ajtai_commit(p->pp, KAPPA, m, zsplit[i], children[i].c);
+      for (size_t r = 0; r < D; r++)
+        for (size_t c = 0; c < m_in; c++) children[i].X[r * m_in + c] = or_fq_from_i64(zsplit[i][r * m + c]);
+      memcpy(children[i].r, parent.r, ell_n * sizeof(kx));
+      for (size_t j = 0; j < t; j++) {
+        kx sc = K0;
+        for (size_t rho = 0; rho < D; rho++) {
+          kx v = y_child[(i * t + j) * D + rho];
+          children[i].y[j * DPAD + rho] = v;
+          sc = or_k_add(sc, or_k_scale(v, bpow[rho]));
+        }
+        children[i].y_scalars[j] = sc;
+      }
+      memcpy(children[i].fold_digest, parent.fold_digest, 32);
+    }
+    /* ok_y, ok_X, ok_c */
+    {
+      int ok_y = 1, ok_x = 1, ok_c = 1;
+      for (size_t j = 0; j < t && ok_y; j++)
+        for (size_t q = 0; q < DPAD; q++) {
+          kx lhs = K0;
+          fq pw = 1;
+          for (size_t i = 0; i < k_dec; i++) {
+            lhs = or_k_add(lhs, or_k_scale(children[i].y[j * DPAD + q], pw));
+            pw = or_fq_add(pw, pw);
+          }
+          if (!or_k_eq(lhs, parent.y[j * DPAD + q])) {
+            ok_y = 0;
+            break;
+          }
+        }
+      for (size_t q = 0; q < D * m_in; q++) {
+        fq lhs = 0, pw = 1;
+        for (size_t i = 0; i < k_dec; i++) {
+          lhs = or_fq_add(lhs, or_fq_mul(children[i].X[q], pw));
+          pw = or_fq_add(pw, pw);
+        }
+        if (lhs != parent.X[q]) ok_x = 0;
+      }
+      for (size_t q = 0; q < KAPPA * D; q++) {
+        fq lhs = children[0].c[q], pw = 2;
+        for (size_t i = 1; i < k_dec; i++) {
+          lhs = or_fq_add(lhs, or_fq_mul(children[i].c[q], pw));
+          pw = or_fq_add(pw, pw);
+        }
+        if (lhs != parent.c[q]) ok_c = 0;
+      }
+      if (!(ok_y && ok_x && ok_c)) {
+        snprintf(g_err, sizeof(g_err), "DEC public check failed at step %zu (y=%d, X=%d, c=%d)", idx, ok_y, ok_x, ok_c);
+        rc = -6;
+        break;
+      }
+    }
+    tr_msg_u64(&tr, "fold/step_done", idx);
+
+    /* ---- serialize StepProof ---- */
+    ub_push(&pb, kt);
+    for (size_t i = 0; i < kt; i++) ser_me(&pb, &ccs_out[i]);
+    /* PiCcsProof */
+    ub_push(&pb, ell_n + ell_d);
+    for (size_t rd = 0; rd < ell_n; rd++) {
+      ub_push(&pb, xs_n);
+      for (size_t i = 0; i < xs_n; i++) ub_k(&pb, time_coeffs[rd * xs_n + i]);
+    }
+    for (size_t rd = 0; rd < ell_d; rd++) {
+      ub_push(&pb, xs_n);
+      for (size_t i = 0; i < xs_n; i++) ub_k(&pb, aj_coeffs[rd * xs_n + i]);
+    }
+    ub_k(&pb, T);
+    ub_push(&pb, ell_n + ell_d);
+    for (size_t rd = 0; rd < ell_n; rd++) ub_k(&pb, r_time[rd]);
+    for (size_t rd = 0; rd < ell_d; rd++) ub_k(&pb, ac.ajtai_chals[rd]);
+    ub_push(&pb, ell_d);
+    for (size_t i = 0; i < ell_d; i++) ub_k(&pb, alpha[i]);
+    ub_push(&pb, ell_d);
+    for (size_t i = 0; i < ell_d; i++) ub_k(&pb, beta_a[i]);
+    ub_push(&pb, ell_n);
+    for (size_t i = 0; i < ell_n; i++) ub_k(&pb, beta_r[i]);
+    ub_k(&pb, gamma);
+    ub_k(&pb, running);
+    ub_digest(&pb, fold_digest);
+    /* rlc_rhos, rlc_parent, dec_children */
+    ub_push(&pb, kt);
+    for (size_t i = 0; i < kt; i++)
+      for (int q = 0; q < D * D; q++) ub_push(&pb, rhos[i][q]);
+    ser_me(&pb, &parent);
+    ub_push(&pb, k_dec);
+    for (size_t i = 0; i < k_dec; i++) ser_me(&pb, &children[i]);
+    /* mem sidecar (empty): cpu_me_claims_val, shout_addr_pre{claimed_sums, active_mask, round_polys, r_addr}, proofs */
+    ub_push(&pb, 0);
+    ub_push(&pb, 0);
+    ub_push(&pb, 0);
+    ub_push(&pb, 0);
+    ub_push(&pb, 0);
+    ub_push(&pb, 0);
+    /* batched_time: claimed_sums, degree_bounds, labels, round_polys */
+    ub_push(&pb, 1);
+    ub_k(&pb, T);
+    ub_push(&pb, 1);
+    ub_push(&pb, d_sc);
+    ub_push(&pb, 1);
+    ub_push(&pb, 8);
+    for (int i = 0; i < 8; i++) ub_push(&pb, (uint8_t)"ccs/time"[i]);
+    ub_push(&pb, 1);
+    ub_push(&pb, ell_n);
+    for (size_t rd = 0; rd < ell_n; rd++) {
+      ub_push(&pb, xs_n);
+      for (size_t i = 0; i < xs_n; i++) ub_k(&pb, time_coeffs[rd * xs_n + i]);
+    }
+    /* val_fold: None */
+    ub_push(&pb, 0);
+
+    /* ---- advance the accumulator ---- */
+    for (size_t a = 0; a < nacc; a++) {
+      me_free(&acc[a]);
+      free(acc_wit[a]);
+    }
+    free(acc);
+    free(acc_wit);
+    nacc = k_dec;
+    acc = children;
+    acc_wit = zsplit;
+    for (size_t i = 0; i < kt; i++) me_free(&ccs_out[i]);
+    free(ccs_out);
+    me_free(&parent);
+    for (size_t i = 0; i < kt; i++) free(rhos[i]);
+    free(rhos);
+    free(rho_a);
+    free(zmix);
+    free(y_child);
+    free(y_eval);
+    free(aj_coeffs);
+    free(time_coeffs);
+    free(r_time);
+    free(Zs);
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (seconds_per_step)
+    *seconds_per_step = ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) / (double)nsteps;
+  /* output obligations (main) */
+  ub_push(&pb, nacc);
+  for (size_t a = 0; a < nacc; a++) ser_me(&pb, &acc[a]);
+  for (size_t a = 0; a < nacc; a++) {
+    me_free(&acc[a]);
+    free(acc_wit[a]);
+  }
+  free(acc);
+  free(acc_wit);
+  for (size_t st = 0; st < nsteps; st++) free(Zmcs[st]);
+  free(Zmcs);
+  free(cmcs);
+  if (rc == 0) {
+    *out = (uint8_t*)pb.v;
+    *out_len = pb.n * sizeof(uint64_t);
+  } else {
+    free(pb.v);
+    *out = NULL;
+    *out_len = 0;
+  }
+  (void)me_copy;
+  return rc;
+}

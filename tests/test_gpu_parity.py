@@ -1,0 +1,131 @@
+"""GPU parity: every kernel path of libnightstream.so against the CPU restatement (oracle/).
+
+All arithmetic is exact, so the bar is bit-identity (field elements, digits, commitments,
+round polynomials, proof bytes) on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+from nightstream_amd.workload import Q, gen_r1cs, rand_balanced_z, rand_field
+
+pytestmark = pytest.mark.gpu
+D = 54
+
+
+def rand_k(rng, n):
+    return rand_field(rng, 2 * n, nonzero=False).reshape(n, 2)
+
+
+def rand_digits(rng, m, density=(0.25, 0.5, 0.25)):
+    return rng.choice(np.array([-1, 0, 1], np.int8), size=(D, m), p=density)
+
+
+def test_decomp_matches_oracle(ns, oracle):
+    rng = np.random.default_rng(1)
+    m = 4096
+    z = rand_balanced_z(rng, m)
+    z[1:9] = [0, 1, Q - 1, (1 << 54) - 1, Q - ((1 << 54) - 1), 2, Q - 2, 3]
+    assert np.array_equal(ns.decomp_b(z), oracle.decomp(z))
+    # values outside the representable range are truncated identically
+    zf = rand_field(rng, 1024, nonzero=False)
+    assert np.array_equal(ns.decomp_b(zf), oracle.decomp(zf))
+
+
+def test_pp_setup_matches_oracle(ns, oracle):
+    seed = bytes(range(32))
+    m = 256
+    pp = ns.PublicParams(m, seed=seed, keep_coeffs=True)
+    assert np.array_equal(pp.coeffs(), oracle.ajtai_setup(seed, 16, m))
+
+
+@pytest.mark.parametrize("m,nw", [(1024, 1), (1024, 3), (2048, 13), (2048, 16)])
+def test_commit_matches_oracle(ns, oracle, m, nw):
+    rng = np.random.default_rng(m + nw)
+    seed = bytes([7] * 32)
+    pp_host = oracle.ajtai_setup(seed, 16, m)
+    pp = ns.PublicParams(m, coeffs=pp_host)
+    Zs = [rand_digits(rng, m) for _ in range(nw)]
+    got = pp.commit(Zs)
+    for w in range(nw):
+        assert np.array_equal(got[w], oracle.ajtai_commit(pp_host, 16, m, Zs[w])), f"witness {w}"
+
+
+def test_commit_extreme_digits(ns, oracle):
+    m = 1024
+    seed = bytes([9] * 32)
+    pp_host = oracle.ajtai_setup(seed, 16, m)
+    pp = ns.PublicParams(m, coeffs=pp_host)
+    Zs = [np.ones((D, m), np.int8), -np.ones((D, m), np.int8), np.zeros((D, m), np.int8)]
+    got = pp.commit(Zs)
+    for w in range(3):
+        assert np.array_equal(got[w], oracle.ajtai_commit(pp_host, 16, m, Zs[w]))
+    assert not got[2].any()
+
+
+def _ccs_pair(ns, oracle, n, seed):
+    inst, zs = gen_r1cs(n, seed, 1)
+    return inst, zs, ns.Ccs(inst), oracle.Ccs(inst)
+
+
+@pytest.mark.parametrize("k,with_inputs", [(1, False), (3, True)])
+def test_ccs_oracle_rounds_match(ns, oracle, k, with_inputs):
+    n = 1 << 10
+    ell_n, ell_d = 10, 6
+    rng = np.random.default_rng(10 + k)
+    inst, zs, gccs, occs = _ccs_pair(ns, oracle, n, 5)
+    Zs = [oracle.decomp(zs[0])] + [rand_digits(rng, n) for _ in range(k - 1)]
+    alpha, beta_a, beta_r = rand_k(rng, ell_d), rand_k(rng, ell_d), rand_k(rng, ell_n)
+    gamma = rand_k(rng, 1)[0]
+    r_in = rand_k(rng, ell_n) if with_inputs else None
+    go = ns.CcsOracle(gccs, Zs, alpha, beta_a, beta_r, gamma, ell_d, ell_n, r_in)
+    oo = oracle.RowState(occs, Zs, alpha, beta_a, beta_r, gamma, ell_d, ell_n, r_in)
+    assert go.num_rounds() == ell_n + ell_d and go.degree_bound() == 6
+    xs = np.array([[x, 0] for x in range(7)], np.uint64)
+    r_time = []
+    for rd in range(ell_n):
+        g = go.evals_at(xs)
+        o = oo.evals(xs)
+        assert np.array_equal(g, o), f"round {rd}"
+        r = rand_k(rng, 1)[0]
+        r_time.append(r)
+        go.fold(r)
+        oo.fold(r)
+    y_g = go.y_eval(np.array(r_time))
+    y_o = oracle.precompute_y(occs, Zs, np.array(r_time), ell_n)
+    assert np.array_equal(y_g, y_o)
+
+
+def test_zmix_and_split_match(ns, oracle):
+    rng = np.random.default_rng(3)
+    m, k = 2048, 13
+    Zs = [rand_digits(rng, m) for _ in range(k)]
+    a = rng.integers(-2, 3, size=(k, D)).astype(np.int8)
+    rhos = [oracle.rot_from_coeffs(a[i]) for i in range(k)]
+    zg = ns.rlc_zmix(a, Zs, m)
+    zo = oracle.rlc_zmix(rhos, Zs, m)
+    assert np.array_equal(zg, zo)
+    pg, nzg = ns.dec_split(zg, 12)
+    po, nzo = oracle.split_b(zo, 12)
+    assert nzg == nzo
+    for i in range(12):
+        assert np.array_equal(pg[i].reshape(-1), po[i]), f"plane {i}"
+
+
+def test_dec_split_rejects_out_of_range(ns):
+    z = np.zeros((D, 64), np.int32)
+    z[3, 5] = 4096
+    with pytest.raises(ns.NsError):
+        ns.dec_split(z, 12)
+
+
+@pytest.mark.parametrize("n,steps", [(1 << 10, 4)])
+def test_prove_shard_bit_identical(ns, oracle, n, steps):
+    inst, zs = gen_r1cs(n, 1, steps)
+    seed = bytes([42] * 32)
+    sess = ns.FoldingSession(ns.Ccs(inst), ns.PublicParams(n, seed=seed))
+    for z in zs:
+        sess.add_step_z(z)
+    proof_gpu = sess.fold_and_prove()
+    proof_cpu, _ = oracle.Prover(inst, seed).prove(zs)
+    assert len(proof_gpu) == len(proof_cpu)
+    assert proof_gpu == proof_cpu
