@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Fold-steps/s of the MI355X Neo prover on the 2^16-row R1CS shard.
+
+A "step" is one fold step (neo-fold/src/shard.rs:1803-2273: Pi_CCS sum-check, Ajtai rounds, ME
+outputs, Pi_RLC, Pi_DEC with 12 child commitments) including that step's MCS commitment.  The
+timed region proves one shard of --steps fold steps (default 16, n = m = 2^16: the
+"plain method at scale on one GPU" workload) with the step witnesses already resident in HBM.
+
+Printed JSON line: metric/value/unit plus
+  roofline     -- the kernel class with the largest summed duration, measured live with HIP events
+                  on the prover's stream over the timed region, against MI355X peaks;
+  cpu_baseline -- the CPU restatement (oracle/, "port") timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PP_SEED = bytes([0x4E, 0x53] + [0x07] * 30)
+HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0
+KT = ["round0_eval", "round_eval", "table_fold", "ajtai_gemm", "ydigits_gemm", "rlc_zmix", "dec_split"]
+KERNEL_NAMES = {
+    "round0_eval": "k_round_eval<true>",
+    "round_eval": "k_round_eval<false>",
+    "table_fold": "k_fold_k",
+    "ajtai_gemm": "k_ajtai_gemm",
+    "ydigits_gemm": "k_ydigits_gemm<4>",
+    "rlc_zmix": "k_rlc_zmix",
+    "dec_split": "k_dec_split",
+}
+D = 54
+
+
+def algorithmic_work(n: int, steps: int, nw_mcs: int):
+    """Algorithmic bytes (or int8 ops) per kernel class summed over one shard of `steps`.
+
+    Pi_CCS tables per row at round r (length L = n >> r): k*54 NC entries + nsc scalar tables
+    (eq_beta, 3 f-var tables, plus eq_rin and Eval once k > 1), 16 B per K entry; round 0 reads
+    the int8 digit matrices (1 B per digit) instead of NC K entries.
+    """
+    ell_n = n.bit_length() - 1
+    out = {c: 0.0 for c in KT}
+    for s in range(steps):
+        k = 1 if s == 0 else 13
+        nsc = 4 if s == 0 else 6
+        out["round0_eval"] += k * D * n + nsc * 16 * n
+        for r in range(1, ell_n):
+            L = n >> r
+            out["round_eval"] += (k * D + nsc) * 16 * L
+        for r in range(ell_n):
+            L = n >> r
+            rows = nsc if r == 0 else (k * D + nsc)
+            out["table_fold"] += rows * 16 * (L + L // 2)
+        # y products: k witnesses at the row point + 12 DEC children, t = 4 matrices
+        for nw in (k, 12):
+            out["ydigits_gemm"] += 2.0 * (nw * D) * 64 * n
+        out["rlc_zmix"] += (k * D * n + D * n * 4) if k > 1 else D * n * 5
+        out["dec_split"] += D * n * 4 + 12 * D * n
+        out["ajtai_gemm"] += 2.0 * 6912 * D * 12 * n  # 12 DEC children per step
+    out["ajtai_gemm"] += 2.0 * 6912 * D * nw_mcs * n  # MCS commitments of all steps (one launch)
+    return out
+
+
+def kernel_bound(cls: str):
+    if cls in ("ajtai_gemm", "ydigits_gemm"):
+        return "mfma", "TFLOP/s", INT8_PEAK_TOPS
+    return "hbm", "GB/s", HBM_PEAK_GBS
+
+
+def cpu_baseline(inst, z, cpu_steps: int):
+    from oracle import pyoracle
+
+    pyoracle.build()
+    prover = pyoracle.Prover(inst, PP_SEED)
+    t0 = time.perf_counter()
+    prover.prove([z] * cpu_steps)
+    wall = time.perf_counter() - t0
+    secs = prover.step_seconds
+    # steady state: step 1 folds k = 13 claims like every later step; step 0 folds k = 1
+    sample_idx = 1 if cpu_steps >= 2 else 0
+    return {
+        "value": 1.0 / secs[sample_idx],
+        "unit": "fold-steps/s",
+        "cores": pyoracle.num_threads(),
+        "kind": "port",
+        "sample": (f"CPU restatement (oracle/) of the same 2^{inst.n.bit_length() - 1}-row shard: fold step "
+                   f"{sample_idx} (k = {13 if sample_idx else 1} claims), {secs[sample_idx]:.2f} s; "
+                   f"{cpu_steps} steps + MCS commits took {wall:.1f} s wall"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 16)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="fold steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (independent shards "
+                                   "shard across GPUs as separate jobs, see DESIGN.md)"}))
+        sys.exit(2)
+    if args.steps < 1:
+        sys.exit("--steps must be >= 1")
+
+    import torch
+
+    import nightstream_amd as ns
+    from nightstream_amd.workload import gen_r1cs
+
+    if not torch.cuda.is_available():
+        sys.exit("no GPU visible")
+    dev = torch.device("cuda:0")
+    t_setup = time.perf_counter()
+    inst, zs = gen_r1cs(args.n, args.seed, 1)
+    z = zs[0]
+    ccs = ns.Ccs(inst)
+    pp = ns.PublicParams(args.n, seed=PP_SEED)
+    sess = ns.FoldingSession(ccs, pp)
+    z_dev = torch.from_numpy(z.view(np.int64).copy()).to(dev)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+
+    if args.warmup > 0:
+        sess.prove_device([z_dev.data_ptr()] * args.warmup)
+    torch.cuda.synchronize()
+    sess.set_timing(True)
+    t0 = time.perf_counter()
+    proof = sess.prove_device([z_dev.data_ptr()] * args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sess.set_timing(False)
+    elapsed = t1 - t0
+    ms, counts = sess.kernel_stats()
+    phases = sess.phase_stats()
+
+    work = algorithmic_work(args.n, args.steps, args.steps)
+    dom = max(range(len(KT)), key=lambda i: ms[i])
+    cls = KT[dom]
+    bound, unit, peak = kernel_bound(cls)
+    avg_ms = ms[dom] / max(counts[dom], 1)
+    per_launch = work[cls] / max(counts[dom], 1)
+    achieved = (per_launch / (avg_ms * 1e-3)) / (1e9 if unit == "GB/s" else 1e12)
+    roofline = {
+        "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+        "frac": round(achieved / peak, 5), "traffic": None,
+        "kernel": KERNEL_NAMES[cls], "launches": int(counts[dom]), "avg_ms": round(avg_ms, 5),
+        "per_launch_algorithmic": per_launch,
+        "kernel_ms": {KT[i]: round(ms[i], 4) for i in range(len(KT))},
+    }
+
+    cpu = None
+    if args.cpu_steps > 0:
+        cpu = cpu_baseline(inst, z, args.cpu_steps)
+
+    value = args.steps / elapsed
+    line = {
+        "metric": "fold-steps/s at 2^16-row CCS chunk vs CPU; p50/p99 job latency under load",
+        "value": round(value, 4),
+        "unit": "fold-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks F_q / F_q^2; int8 MFMA with int32 accumulation for exact contractions)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain shard: R1CS n = m = 2^{args.n.bit_length() - 1}, {args.steps} fold steps, "
+                        "default Goldilocks params (D=54, kappa=16, b=2, k_rho=12), CCS-only",
+            "n": args.n, "m": args.n, "fold_steps": args.steps, "m_in": 1,
+            "proof_bytes": len(proof), "setup_s": round(setup_s, 2),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "host_phases_s": phases,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

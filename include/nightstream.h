@@ -113,6 +113,10 @@ int ns_prove_shard_device(ns_prover* p, const uint64_t* const* z_steps_dev, size
  * 6 DEC split.  ms[i] is the summed duration and counts[i] the number of launches. */
 int ns_prover_set_timing(ns_prover* p, int on);
 int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, size_t n);
+/* Host wall-clock seconds per phase of the last prove (12 phases: bind inputs, oracle build,
+ * row rounds, y products, Ajtai rounds, ME outputs, RLC binding, Z_mix, DEC, encode, intake,
+ * accumulator swap). */
+int ns_prover_phase_stats(const ns_prover* p, double* seconds, size_t n);
 void ns_prover_free(ns_prover* p);
 
 #ifdef __cplusplus

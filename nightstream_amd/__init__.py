@@ -23,7 +23,8 @@ EXPORTS = [
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
-    "ns_dec_split", "ns_prover_new", "ns_prove_shard", "ns_prover_free",
+    "ns_dec_split", "ns_prover_new", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
+    "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_free",
 ]
 
 
@@ -76,6 +77,10 @@ def lib():
     L.ns_prover_new.argtypes = [vp, vp, C.POINTER(vp)]
     L.ns_prove_shard.argtypes = [vp, vp, sz, sz, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
                                  C.POINTER(C.c_double)]
+    L.ns_prove_shard_device.argtypes = L.ns_prove_shard.argtypes
+    L.ns_prover_set_timing.argtypes = [vp, C.c_int]
+    L.ns_prover_kernel_stats.argtypes = [vp, vp, vp, sz]
+    L.ns_prover_phase_stats.argtypes = [vp, vp, sz]
     L.ns_params_goldilocks_auto_r1cs_ccs.argtypes = [sz, vp]
     _lib = L
     return L
@@ -279,6 +284,36 @@ class FoldingSession:
         lib().ns_free(C.cast(out, C.c_void_p))
         self.last_seconds = sec.value
         return data
+
+    def prove_device(self, z_ptrs, m_in: int = 1) -> bytes:
+        """fold_and_prove over step witnesses already resident in HBM (device pointers to u64[m])."""
+        arr = (C.c_void_p * len(z_ptrs))(*z_ptrs)
+        out = C.POINTER(C.c_uint8)()
+        n = C.c_size_t()
+        sec = C.c_double()
+        _check(lib().ns_prove_shard_device(self.h, arr, len(z_ptrs), m_in, C.byref(out), C.byref(n),
+                                           C.byref(sec)))
+        data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)
+        lib().ns_free(C.cast(out, C.c_void_p))
+        self.last_seconds = sec.value
+        return data
+
+    def set_timing(self, on: bool):
+        _check(lib().ns_prover_set_timing(self.h, int(on)))
+
+    def kernel_stats(self):
+        ms = np.zeros(7, np.float64)
+        cnt = np.zeros(7, np.uint64)
+        _check(lib().ns_prover_kernel_stats(self.h, _ptr(ms), _ptr(cnt), 7))
+        return ms.tolist(), [int(c) for c in cnt]
+
+    PHASES = ["bind_inputs", "oracle_build", "row_rounds", "y_products", "ajtai_rounds", "me_outputs",
+              "rlc_bind", "zmix", "dec", "encode", "intake", "acc_swap"]
+
+    def phase_stats(self) -> dict:
+        out = np.zeros(12, np.float64)
+        _check(lib().ns_prover_phase_stats(self.h, _ptr(out), 12))
+        return {k: round(float(v), 5) for k, v in zip(self.PHASES, out)}
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -386,6 +386,12 @@ int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, siz
   });
 }
 
+int ns_prover_phase_stats(const ns_prover* p, double* seconds, size_t n) {
+  return guard([&] {
+    for (size_t i = 0; i < n && i < (size_t)Prover::NPHASE; i++) seconds[i] = p->p->phase_s[i];
+  });
+}
+
 void ns_prover_free(ns_prover* p) { delete p; }
 
 }  // extern "C"

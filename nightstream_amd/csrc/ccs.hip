@@ -292,58 +292,75 @@ __device__ __forceinline__ K2 wave_reduce(K2 v) {
   return v;
 }
 
-// coefficient vector of width 5 (deg_max = 4)
+// coefficient vector of width 5 (deg_max = 4).  Work item = (row pair t, row group g): the NC
+// sum over the k*54 digit rows of a pair is split into G groups so that short tables still fill
+// the GPU; every group multiplies its partial NC polynomial by the pair's eq_beta(X) (the
+// round polynomial is linear in the NC sum), and group 0 adds the f and Eval terms.
 template <bool ROUND0>
-__global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __restrict__ wg, K2* __restrict__ partials) {
+__global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
+                                                   int G) {
   extern __shared__ K2 sh[];  // wg cache [k*54] then reduction scratch
   const int nrow = A.k * D;
   for (int i = threadIdx.x; i < nrow; i += blockDim.x) sh[i] = wg[i];
   __syncthreads();
   const size_t pairs = A.len / 2;
+  const int R = (nrow + G - 1) / G;
   K2 c[5];
 #pragma unroll
   for (int d = 0; d < 5; d++) c[d] = kzero();
   K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < pairs; t += (size_t)gridDim.x * blockDim.x) {
+  const size_t total = pairs * (size_t)G;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = idx % pairs;
+    const int g = (int)(idx / pairs);
+    const int r_lo = g * R, r_hi = min(nrow, r_lo + R);
     const K2 e0 = A.eqb[2 * t];
     const K2 e1 = ksub(A.eqb[2 * t + 1], e0);
     K2 in[5];
 #pragma unroll
     for (int d = 0; d < 5; d++) in[d] = kzero();
-    // f(M z1)(X)
-    for (int tt = 0; tt < A.f.nterms; tt++) {
-      K2 p[5];
-      p[0] = K2{A.f.coeff[tt], 0};
+    if (g == 0) {
+      // f(M z1)(X)
+      for (int tt = 0; tt < A.f.nterms; tt++) {
+        K2 p[5];
+        p[0] = K2{A.f.coeff[tt], 0};
 #pragma unroll
-      for (int d = 1; d < 5; d++) p[d] = kzero();
-      for (int v = 0; v < A.f.nv[tt]; v++) {
-        const K2* tbl = A.fv + (size_t)A.f.var[tt][v] * A.len;
-        K2 a = tbl[2 * t], b = ksub(tbl[2 * t + 1], a);
-        for (int e = 0; e < A.f.exp[tt][v]; e++) {
+        for (int d = 1; d < 5; d++) p[d] = kzero();
+        for (int v = 0; v < A.f.nv[tt]; v++) {
+          const K2* tbl = A.fv + (size_t)A.f.var[tt][v] * A.len;
+          K2 a = tbl[2 * t], b = ksub(tbl[2 * t + 1], a);
+          for (int e = 0; e < A.f.exp[tt][v]; e++) {
 #pragma unroll
-          for (int d = 4; d >= 1; d--) p[d] = kadd(kmul(a, p[d]), kmul(b, p[d - 1]));
-          p[0] = kmul(a, p[0]);
+            for (int d = 4; d >= 1; d--) p[d] = kadd(kmul(a, p[d]), kmul(b, p[d - 1]));
+            p[0] = kmul(a, p[0]);
+          }
         }
-      }
 #pragma unroll
-      for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
+        for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
+      }
+      if (A.eqr) {
+        K2 r0 = A.eqr[2 * t], r1 = ksub(A.eqr[2 * t + 1], r0);
+        K2 v0 = A.ev[2 * t], v1 = ksub(A.ev[2 * t + 1], v0);
+        ev0 = kadd(ev0, kmul(r0, v0));
+        ev1 = kadd(ev1, kadd(kmul(r0, v1), kmul(r1, v0)));
+        ev2 = kadd(ev2, kmul(r1, v1));
+      }
     }
     if (ROUND0) {
       // signed selections of wg
       K2 slo = kzero(), shi = kzero(), s3 = kzero(), s4 = kzero();
-      for (int w = 0; w < A.k; w++) {
-        const int8_t* z = A.digits[w] + 2 * t;
-        for (int rho = 0; rho < D; rho++) {
-          int lo = z[(size_t)rho * A.m], hi = z[(size_t)rho * A.m + 1];
-          if ((lo | hi) == 0) continue;
-          K2 wv = sh[w * D + rho];
-          K2 nw = kneg(wv);
-          if (lo) slo = kadd(slo, lo > 0 ? wv : nw);
-          if (hi) shi = kadd(shi, hi > 0 ? wv : nw);
-          if (lo && hi) {
-            s3 = kadd(s3, hi > 0 ? wv : nw);  // lo^2 hi = hi
-            s4 = kadd(s4, lo > 0 ? wv : nw);  // lo hi^2 = lo
-          }
+      for (int r = r_lo; r < r_hi; r++) {
+        const int w = r / D, rho = r - w * D;
+        const int8_t* z = A.digits[w] + (size_t)rho * A.m + 2 * t;
+        int lo = z[0], hi = z[1];
+        if ((lo | hi) == 0) continue;
+        K2 wv = sh[r];
+        K2 nw = kneg(wv);
+        if (lo) slo = kadd(slo, lo > 0 ? wv : nw);
+        if (hi) shi = kadd(shi, hi > 0 ? wv : nw);
+        if (lo && hi) {
+          s3 = kadd(s3, hi > 0 ? wv : nw);  // lo^2 hi = hi
+          s4 = kadd(s4, lo > 0 ? wv : nw);  // lo hi^2 = lo
         }
       }
       // c1 = -2 slo - shi + 3 s3 ; c2 = 3 slo - 6 s3 + 3 s4 ; c3 = -slo + shi + 3 s3 - 3 s4
@@ -355,7 +372,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
       in[3] = kadd(in[3], ksub(kadd(shi, s3x3), kadd(slo, s4x3)));
     } else {
       K2 s1 = kzero(), s2 = kzero(), s3 = kzero(), s4 = kzero(), s5 = kzero(), s6 = kzero();
-      for (int r = 0; r < nrow; r++) {
+      for (int r = r_lo; r < r_hi; r++) {
         const K2* row = A.nc + (size_t)r * A.len + 2 * t;
         K2 a = row[0];
         K2 b = ksub(row[1], a);
@@ -377,13 +394,6 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
     c[0] = kadd(c[0], kmul(e0, in[0]));
 #pragma unroll
     for (int d = 1; d < 5; d++) c[d] = kadd(c[d], kadd(kmul(e0, in[d]), kmul(e1, in[d - 1])));
-    if (A.eqr) {
-      K2 r0 = A.eqr[2 * t], r1 = ksub(A.eqr[2 * t + 1], r0);
-      K2 v0 = A.ev[2 * t], v1 = ksub(A.ev[2 * t + 1], v0);
-      ev0 = kadd(ev0, kmul(r0, v0));
-      ev1 = kadd(ev1, kadd(kmul(r0, v1), kmul(r1, v0)));
-      ev2 = kadd(ev2, kmul(r1, v1));
-    }
   }
   if (A.eqr) {
     c[0] = kadd(c[0], kmul(A.gamma_k, ev0));
@@ -433,16 +443,20 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   A.gamma_k = T.gamma_k;
   A.f = f;
   size_t pairs = T.len / 2;
-  size_t nblk = (pairs + RB - 1) / RB;
+  // split the k*54 NC rows of a pair across G work items when the table is short
+  const size_t target = (size_t)RMAXBLK * RB / 2;
+  int G = 1;
+  while ((size_t)G * 2 <= (size_t)T.k * D && pairs * (size_t)G * 2 <= target) G *= 2;
+  size_t nblk = (pairs * (size_t)G + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
   size_t shmem = ((size_t)T.k * D + 4 * 5) * sizeof(K2);
   K2* partials = (K2*)ws;
   ktimer_begin(round0 ? KT_ROUND0 : KT_ROUND, s);
   if (round0)
-    hipLaunchKernelGGL(k_round_eval<true>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials);
+    hipLaunchKernelGGL(k_round_eval<true>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
   else
-    hipLaunchKernelGGL(k_round_eval<false>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials);
+    hipLaunchKernelGGL(k_round_eval<false>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
   ktimer_end(round0 ? KT_ROUND0 : KT_ROUND, s);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partials, (int)nblk, out);
 }

@@ -738,6 +738,17 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   if (m_in > m) throw Error(1, "m_in exceeds m");
   hipStream_t st = stream_;
   auto t0 = clk::now();
+  for (int i = 0; i < NPHASE; i++) phase_s[i] = 0;
+  auto tmark = clk::now();
+  int cur_phase = -1;
+  auto phase_mark = [&](int next) {
+    auto now = clk::now();
+    if (cur_phase >= 0) phase_s[cur_phase] += std::chrono::duration<double>(now - tmark).count();
+    tmark = now;
+    cur_phase = next;
+  };
+  phase_mark(10);  // step intake
+
 
   // ---- step intake: digits and MCS commitments for all steps ----
   std::vector<DevBuf> Zmcs(nsteps);
@@ -789,6 +800,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     tr.append_msg_u64("step/lut_count", 0);
     tr.append_msg_u64("step/mem_count", 0);
     tr.append_message("step/absorb_memory_done", (const uint8_t*)nullptr, 0);
+    phase_mark(0);
     // bind_header_and_instances_with_digest
     tr.append_message("phase/pi-ccs", (const uint8_t*)nullptr, 0);
     tr.append_message("neo/ccs/header/v1", (const uint8_t*)nullptr, 0);
@@ -889,12 +901,14 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       (void)tr.challenge_field("route_a/cycle/0");
       (void)tr.challenge_field("route_a/cycle/1");
     }
+    phase_mark(1);
     // device oracle
     std::vector<const int8_t*> Zs(kt);
     Zs[0] = Zmcs[idx].as<int8_t>();
     for (size_t a = 0; a < nacc; a++) Zs[1 + a] = acc_wit[a].as<int8_t>();
     CcsOracle oracle(s, Zs.data(), kt, alpha.data(), beta_a, beta_r, gamma, ell_d, ell_n,
                      nacc ? acc[0].r.data() : nullptr, st);
+    phase_mark(2);
     // batched time sum-check with the single CCS claim
     tr.append_msg_u64("batched/dynamic_bind/len", 1);
     tr.append_message("batched/dynamic_bind/claim_label", "ccs/time");
@@ -929,8 +943,10 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     tr.append_message("sc/domain_end", "shard/batched_time");
     tr.append_msg_u64("sc/inst_idx_end", idx);
 
+    phase_mark(3);
     // precompute_for_r on the device
     std::vector<K2> y_eval = oracle.y_eval(r_time, ws_yd, vjs);
+    phase_mark(4);
     // Ajtai-bit rounds (host)
     AjtaiPhase aj;
     aj.init(s, y_eval, kt, alpha.data(), beta_a, beta_r, gamma, r_time, nacc ? acc[0].r.data() : nullptr);
@@ -955,6 +971,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     uint8_t fold_digest[32];
     tr.digest32(fold_digest);
 
+    phase_mark(5);
     // ME outputs
     auto head_cols = [&](const int8_t* Zdev, std::vector<uint64_t>& X) {
       X.assign((size_t)D * m_in, 0);
@@ -991,6 +1008,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       memcpy(e.fold_digest, fold_digest, 32);
     }
 
+    phase_mark(6);
     // ---- Pi_RLC ----
     tr.append_message("fold/rlc_inputs/v1", "main");
     {
@@ -1117,12 +1135,14 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       for (int rho = 0; rho < D; rho++) sc = kadd(sc, kmul(parent.y[j * DPAD + rho], bpow[rho]));
       parent.y_scalars[j] = sc;
     }
+    phase_mark(7);
     // Z_mix on the device
     DevBuf zmix((size_t)D * m * 4), rho_dev(rho_i8.size()), zptr(kt * sizeof(void*));
     h2d(rho_dev.p, rho_i8.data(), rho_i8.size(), st);
     h2d(zptr.p, (const void* const*)Zs.data(), kt, st);
     launch_rlc_zmix(rho_dev.as<int8_t>(), zptr.as<const int8_t*>(), kt, m, zmix.as<int32_t>(), st);
 
+    phase_mark(8);
     // ---- Pi_DEC ----
     std::vector<DevBuf> planes(k_dec);
     std::vector<const int8_t*> pptr(k_dec);
@@ -1204,6 +1224,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
         throw Error(7, "DEC public check failed at step " + std::to_string(idx) + " (y=" + std::to_string(ok_y) +
                            ", X=" + std::to_string(ok_x) + ", c=" + std::to_string(ok_c) + ")");
     }
+    phase_mark(9);
     tr.append_msg_u64("fold/step_done", idx);
 
     // ---- encode StepProof ----
@@ -1254,7 +1275,9 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
 
     acc = std::move(children);
     acc_wit = std::move(planes);
+    phase_mark(11);
   }
+  phase_mark(-1);
   pb.u(acc.size());
   for (auto& e : acc) pb.me(e);
   NS_CHECK(hipStreamSynchronize(st));

@@ -116,6 +116,11 @@ class Prover {
   double last_seconds = 0;
   // Per-kernel-class timing of the last prove (HIP events on the prover stream).
   bool timing = false;
+  // Host wall time per phase of the last prove: 0 bind header/ME inputs/challenges, 1 oracle
+  // build, 2 row rounds, 3 y products, 4 Ajtai rounds, 5 ME outputs, 6 RLC binding + rhos,
+  // 7 Z_mix, 8 DEC, 9 encode, 10 step intake (decomp + MCS commits), 11 accumulator swap.
+  static constexpr int NPHASE = 12;
+  double phase_s[NPHASE] = {0};
   double kernel_ms[KT_N] = {0};
   uint64_t kernel_count[KT_N] = {0};
   uint64_t last_permutations = 0;

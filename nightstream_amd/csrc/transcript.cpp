@@ -114,52 +114,61 @@ void Poseidon2::init(int w, ChaCha8& rng) {
   for (int i = 0; i < w; i++) diag[i] = sample_fq(rng);  // stand-in diagonal, see transcript.h
 }
 
+typedef unsigned __int128 u128;
+static inline uint64_t red(u128 x) { return reduce128((uint64_t)x, (uint64_t)(x >> 64)); }
+
 static inline uint64_t sbox7(uint64_t x) {
-  uint64_t x2 = fmul(x, x);
-  uint64_t x3 = fmul(x2, x);
-  uint64_t x4 = fmul(x2, x2);
-  return fmul(x3, x4);
+  uint64_t x2 = red((u128)x * x);
+  uint64_t x3 = red((u128)x2 * x);
+  uint64_t x4 = red((u128)x2 * x2);
+  return red((u128)x3 * x4);
 }
 
-static inline void mat4(uint64_t* x) {
-  uint64_t t01 = fadd(x[0], x[1]), t23 = fadd(x[2], x[3]);
-  uint64_t t0123 = fadd(t01, t23);
-  uint64_t t01123 = fadd(t0123, x[1]), t01233 = fadd(t0123, x[3]);
-  uint64_t y3 = fadd(t01233, fadd(x[0], x[0]));
-  uint64_t y1 = fadd(t01123, fadd(x[2], x[2]));
-  uint64_t y0 = fadd(t01123, t01);
-  uint64_t y2 = fadd(t01233, t23);
-  x[0] = y0;
-  x[1] = y1;
-  x[2] = y2;
-  x[3] = y3;
+// External linear layer: M4 = circ(2,3,1,1) on every 4-lane chunk, then out[i] += sum of lane
+// (i mod 4) over all chunks.  Accumulated in 128 bits and reduced once per output.
+template <int W>
+static inline void ext_layer(uint64_t* st) {
+  u128 y[W];
+  for (int c = 0; c < W; c += 4) {
+    u128 x0 = st[c], x1 = st[c + 1], x2 = st[c + 2], x3 = st[c + 3];
+    u128 t01 = x0 + x1, t23 = x2 + x3, t0123 = t01 + t23;
+    y[c + 0] = t0123 + t01 + x1;       // 2 x0 + 3 x1 + x2 + x3
+    y[c + 1] = t0123 + x1 + 2 * x2;    // x0 + 2 x1 + 3 x2 + x3
+    y[c + 2] = t0123 + t23 + x3;       // x0 + x1 + 2 x2 + 3 x3
+    y[c + 3] = t0123 + x3 + 2 * x0;    // 3 x0 + x1 + x2 + 2 x3
+  }
+  u128 sums[4];
+  for (int k = 0; k < 4; k++) {
+    sums[k] = 0;
+    for (int c = 0; c < W; c += 4) sums[k] += y[c + k];
+  }
+  for (int i = 0; i < W; i++) st[i] = red(y[i] + sums[i & 3]);
 }
 
-static inline void ext_layer(uint64_t* st, int w) {
-  for (int c = 0; c < w; c += 4) mat4(st + c);
-  uint64_t s[4] = {0, 0, 0, 0};
-  for (int c = 0; c < w; c += 4)
-    for (int k = 0; k < 4; k++) s[k] = fadd(s[k], st[c + k]);
-  for (int i = 0; i < w; i++) st[i] = fadd(st[i], s[i & 3]);
+template <int W>
+static inline void permute_w(const Poseidon2& P, uint64_t* st) {
+  ext_layer<W>(st);
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < W; i++) st[i] = sbox7(red((u128)st[i] + P.ext_init[r][i]));
+    ext_layer<W>(st);
+  }
+  for (int r = 0; r < 22; r++) {
+    u128 rest = 0;
+    for (int i = 1; i < W; i++) rest += st[i];
+    uint64_t x7 = sbox7(red((u128)st[0] + P.internal_rc[r]));
+    uint64_t sum = red(rest + x7);
+    st[0] = red((u128)x7 * P.diag[0] + sum);
+    for (int i = 1; i < W; i++) st[i] = red((u128)st[i] * P.diag[i] + sum);
+  }
+  for (int r = 0; r < 4; r++) {
+    for (int i = 0; i < W; i++) st[i] = sbox7(red((u128)st[i] + P.ext_term[r][i]));
+    ext_layer<W>(st);
+  }
 }
 
 void Poseidon2::permute(uint64_t* st) const {
-  const int w = width;
-  ext_layer(st, w);
-  for (int r = 0; r < 4; r++) {
-    for (int i = 0; i < w; i++) st[i] = sbox7(fadd(st[i], ext_init[r][i]));
-    ext_layer(st, w);
-  }
-  for (int r = 0; r < 22; r++) {
-    st[0] = sbox7(fadd(st[0], internal_rc[r]));
-    uint64_t sum = 0;
-    for (int i = 0; i < w; i++) sum = fadd(sum, st[i]);
-    for (int i = 0; i < w; i++) st[i] = fadd(fmul(st[i], diag[i]), sum);
-  }
-  for (int r = 0; r < 4; r++) {
-    for (int i = 0; i < w; i++) st[i] = sbox7(fadd(st[i], ext_term[r][i]));
-    ext_layer(st, w);
-  }
+  if (width == 8) permute_w<8>(*this, st);
+  else permute_w<16>(*this, st);
 }
 
 const Poseidon2& transcript_perm() {

@@ -175,6 +175,10 @@ void or_prover_free(or_prover* p);
 int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t m_in,
                    uint8_t** out, size_t* out_len, double* seconds_per_step);
 void or_free(void* ptr);
+/* Wall time of fold step i of the last or_prove_shard call (the MCS commits of all steps are
+ * done before step 0 and counted in neither). */
+double or_last_step_seconds(size_t i);
+int or_num_threads(void);
 const char* or_last_error(void);
 
 #ifdef __cplusplus

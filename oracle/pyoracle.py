@@ -85,6 +85,10 @@ def lib():
                                      C.POINTER(C.c_double)]
         L.or_free.argtypes = [C.c_void_p]
         L.or_last_error.restype = C.c_char_p
+        L.or_last_step_seconds.restype = C.c_double
+        L.or_last_step_seconds.argtypes = [C.c_size_t]
+        L.or_num_threads.restype = C.c_int
+        L.or_tr_permutation_count.restype = C.c_uint64
         L.or_params_auto_r1cs.restype = C.c_int
         L.or_extension_check.restype = C.c_int
         _lib = L
@@ -122,6 +126,10 @@ class Ccs:
             lib().or_ccs_free(self.ptr)
         except Exception:
             pass
+
+
+def num_threads() -> int:
+    return lib().or_num_threads()
 
 
 def decomp(z: np.ndarray) -> np.ndarray:
@@ -250,6 +258,7 @@ class Prover:
             raise RuntimeError(f"oracle prove failed ({rc}): {lib().or_last_error().decode()}")
         data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)
         lib().or_free(out)
+        self.step_seconds = [lib().or_last_step_seconds(i) for i in range(len(zs))]
         return data, sec.value
 
     def __del__(self):

@@ -33,12 +33,25 @@
 #include <string.h>
 #include <time.h>
 #include "neo_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define D NEO_D
 #define DPAD 64
 #define KAPPA 16
 
 static char g_err[512];
+static double g_step_seconds[4096];
+static size_t g_nstep_seconds = 0;
+double or_last_step_seconds(size_t i) { return i < g_nstep_seconds ? g_step_seconds[i] : -1.0; }
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 const char* or_last_error(void) { return g_err; }
 void or_free(void* p) { free(p); }
 
@@ -423,6 +436,9 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
   for (size_t i = 0; i < xs_n; i++) xs[i] = or_k_from(i);
   int rc = 0;
 
+  g_nstep_seconds = 0;
+  struct timespec ts_step;
+  clock_gettime(CLOCK_MONOTONIC, &ts_step);
   for (size_t idx = 0; idx < nsteps; idx++) {
     /* absorb_step_memory with no LUT / MEM instances */
     tr_msg_empty(&tr, "step/absorb_memory_start");
@@ -896,6 +912,14 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
     /* val_fold: None */
     ub_push(&pb, 0);
 
+    {
+      struct timespec now;
+      clock_gettime(CLOCK_MONOTONIC, &now);
+      if (g_nstep_seconds < 4096)
+        g_step_seconds[g_nstep_seconds++] =
+            (double)(now.tv_sec - ts_step.tv_sec) + 1e-9 * (double)(now.tv_nsec - ts_step.tv_nsec);
+      ts_step = now;
+    }
     /* ---- advance the accumulator ---- */
     for (size_t a = 0; a < nacc; a++) {
       me_free(&acc[a]);
