@@ -20,6 +20,9 @@ FLAGS = [
     # packed FP32 instructions stay off (no float code here, kept for consistency)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+    # host code: AVX2/BMI2 baseline (both the build host and the MI355X hosts support it); the
+    # znver4 schedule keeps the 64x64->128 Goldilocks reductions of the host Poseidon2 tight
+    "-Xarch_host", "-march=x86-64-v3", "-Xarch_host", "-mtune=znver4",
 ]
 
 

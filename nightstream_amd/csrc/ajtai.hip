@@ -177,45 +177,57 @@ __global__ __launch_bounds__(256) void k_ajtai_gemm(const int8_t* __restrict__ l
       }
 }
 
-// S[w*54 + t] = sum_j Z_w[t][j]
-__global__ void k_digit_rowsum(const int8_t* const* __restrict__ Zs, size_t m, int32_t* __restrict__ S) {
-  size_t w = blockIdx.x / D, t = blockIdx.x % D;
-  const int8_t* row = Zs[w] + t * m;
-  int32_t acc = 0;
-  for (size_t j = threadIdx.x; j < m; j += blockDim.x) acc += row[j];
-  __shared__ int32_t red[256];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) S[blockIdx.x] = red[0];
+__device__ __forceinline__ int sum4_i8a(int x) {
+  return (x << 24 >> 24) + (x << 16 >> 24) + (x << 8 >> 24) + (x >> 24);
 }
 
-// block = (w, i); thread s < 107 computes P[s]; threads r < 54 then reduce mod Phi_81
-__global__ __launch_bounds__(128) void k_ajtai_epilogue(const int32_t* __restrict__ part, size_t splits,
+// S[w*54 + t] = sum_j Z_w[t][j]; 16-byte loads (m is a multiple of 64)
+__global__ __launch_bounds__(256) void k_digit_rowsum(const int8_t* const* __restrict__ Zs, size_t m,
+                                                      int32_t* __restrict__ S) {
+  size_t w = blockIdx.x / D, t = blockIdx.x % D;
+  const int4* row = (const int4*)(Zs[w] + t * m);
+  int32_t acc = 0;
+  for (size_t j = threadIdx.x; j < m / 16; j += blockDim.x) {
+    int4 v = row[j];
+    acc += sum4_i8a(v.x) + sum4_i8a(v.y) + sum4_i8a(v.z) + sum4_i8a(v.w);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  __shared__ int32_t red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) S[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// block = (w, i), 8 waves: wave l sums limb l's anti-diagonals G_l[s] (int64) for s < 107;
+// then 107 threads combine the limbs in F_q and 54 threads reduce mod Phi_81.
+__global__ __launch_bounds__(512) void k_ajtai_epilogue(const int32_t* __restrict__ part, size_t splits,
                                                         size_t npad, const int32_t* __restrict__ S,
                                                         uint64_t* __restrict__ out) {
   const size_t w = blockIdx.x / KAPPA, i = blockIdx.x % KAPPA;
+  __shared__ int64_t G[8][2 * D - 1];
+  __shared__ int64_t corr[2 * D - 1];
   __shared__ uint64_t P[2 * D - 1];
-  const int s = threadIdx.x;
-  if (s < 2 * D - 1) {
+  const int lane = threadIdx.x & 63, l = threadIdx.x >> 6;
+  for (int s = lane; s < 2 * D - 1; s += 64) {
     int u_lo = s - (D - 1) > 0 ? s - (D - 1) : 0;
     int u_hi = s < D - 1 ? s : D - 1;
-    int64_t corr = 0;
-    for (int u = u_lo; u <= u_hi; u++) corr += S[w * D + (s - u)];
-    uint64_t acc = 0;
-    for (int l = 0; l < 8; l++) {
-      int64_t g = 0;
-      for (int u = u_lo; u <= u_hi; u++) {
-        size_t row = (i * 8 + l) * D + u;
-        size_t col = w * D + (s - u);
-        for (size_t sp = 0; sp < splits; sp++) g += part[(sp * AJ_MROWS + row) * npad + col];
-      }
-      g += 128 * corr;
-      acc = fadd(acc, fmul(from_i64(g), 1ull << (8 * l)));
+    int64_t g = 0;
+    for (size_t sp = 0; sp < splits; sp++) {
+      const int32_t* base = part + (sp * AJ_MROWS + (i * 8 + l) * D) * npad + w * D + s;
+      for (int u = u_lo; u <= u_hi; u++) g += base[(size_t)u * npad - u];
     }
+    G[l][s] = g;
+    if (l == 0) {
+      int64_t c = 0;
+      for (int u = u_lo; u <= u_hi; u++) c += S[w * D + (s - u)];
+      corr[s] = c;
+    }
+  }
+  __syncthreads();
+  const int s = threadIdx.x;
+  if (s < 2 * D - 1) {
+    uint64_t acc = 0;
+    for (int ll = 0; ll < 8; ll++) acc = fadd(acc, fmul(from_i64(G[ll][s] + 128 * corr[s]), 1ull << (8 * ll)));
     P[s] = acc;
   }
   __syncthreads();
@@ -240,7 +252,7 @@ void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* con
   hipLaunchKernelGGL(k_ajtai_gemm, dim3((unsigned)(AJ_MROWS / (AJ_MT * 16)), (unsigned)groups, (unsigned)splits),
                      dim3(64 * AJ_WAVES), 0, s, limbs, m, kc, Zs_dev, nw, npad, part);
   ktimer_end(KT_AJTAI_GEMM, s);
-  hipLaunchKernelGGL(k_ajtai_epilogue, dim3((unsigned)(nw * KAPPA)), dim3(128), 0, s, part, splits, npad, S, out);
+  hipLaunchKernelGGL(k_ajtai_epilogue, dim3((unsigned)(nw * KAPPA)), dim3(512), 0, s, part, splits, npad, S, out);
   (void)kappa;
 }
 

@@ -110,8 +110,8 @@ void launch_spmv_t(const DevSparse& M, const K2* chi, size_t n_eff, K2* v, hipSt
 }
 
 // ---------------- Eval column sums ----------------
-__global__ void k_weighted_colsum(const int8_t* const* __restrict__ Zs, int nw, const K2* __restrict__ W, size_t m,
-                                  K2* __restrict__ S) {
+__global__ __launch_bounds__(256) void k_weighted_colsum(const int8_t* const* __restrict__ Zs, int nw,
+                                                         const K2* __restrict__ W, size_t m, K2* __restrict__ S) {
   __shared__ K2 w_s[16 * D];
   for (int i = threadIdx.x; i < nw * D; i += blockDim.x) w_s[i] = W[i];
   __syncthreads();
@@ -120,10 +120,15 @@ __global__ void k_weighted_colsum(const int8_t* const* __restrict__ Zs, int nw, 
   K2 pos = kzero(), neg = kzero();
   for (int w = 0; w < nw; w++) {
     const int8_t* z = Zs[w] + c;
+    int8_t d[D];
+#pragma unroll
+    for (int rho = 0; rho < D; rho++) d[rho] = z[(size_t)rho * m];
+#pragma unroll
     for (int rho = 0; rho < D; rho++) {
-      int8_t d = z[(size_t)rho * m];
-      if (d > 0) pos = kadd(pos, w_s[w * D + rho]);
-      else if (d < 0) neg = kadd(neg, w_s[w * D + rho]);
+      K2 wv = w_s[w * D + rho];
+      K2 zero = kzero();
+      pos = kadd(pos, d[rho] > 0 ? wv : zero);
+      neg = kadd(neg, d[rho] < 0 ? wv : zero);
     }
   }
   S[c] = ksub(pos, neg);
@@ -150,19 +155,25 @@ __global__ void k_v_limbs(const K2* __restrict__ v, size_t t, size_t m, int8_t* 
       Vl[((j * 2 + cp) * 8 + l) * m + c] = (int8_t)((int)((comp[cp] >> (8 * l)) & 0xFF) - 128);
 }
 
-__global__ void k_digit_rowsum2(const int8_t* const* __restrict__ Zs, size_t m, int32_t* __restrict__ S) {
+__device__ __forceinline__ int sum4_i8(int x) {
+  return (x << 24 >> 24) + (x << 16 >> 24) + (x << 8 >> 24) + (x >> 24);
+}
+
+// S[w*54 + t] = sum_j Z_w[t][j]; 16-byte loads (m is a multiple of 64)
+__global__ __launch_bounds__(256) void k_digit_rowsum2(const int8_t* const* __restrict__ Zs, size_t m,
+                                                       int32_t* __restrict__ S) {
   size_t w = blockIdx.x / D, t = blockIdx.x % D;
-  const int8_t* row = Zs[w] + t * m;
+  const int4* row = (const int4*)(Zs[w] + t * m);
   int32_t acc = 0;
-  for (size_t j = threadIdx.x; j < m; j += blockDim.x) acc += row[j];
-  __shared__ int32_t red[256];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
+  for (size_t j = threadIdx.x; j < m / 16; j += blockDim.x) {
+    int4 v = row[j];
+    acc += sum4_i8(v.x) + sum4_i8(v.y) + sum4_i8(v.z) + sum4_i8(v.w);
   }
-  if (threadIdx.x == 0) S[blockIdx.x] = red[0];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  __shared__ int32_t red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) S[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 constexpr int YD_SPLIT_MAX = 64;
@@ -417,12 +428,16 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
   }
 }
 
-__global__ void k_sum_partials(const K2* __restrict__ partials, int nblk, K2* __restrict__ out) {
-  int d = threadIdx.x;
-  if (d >= 5) return;
+// one block per coefficient: strided partial sums, wave reductions, then the 4 wave results
+__global__ __launch_bounds__(256) void k_sum_partials(const K2* __restrict__ partials, int nblk, K2* __restrict__ out) {
+  const int d = blockIdx.x;
   K2 v = kzero();
-  for (int b = 0; b < nblk; b++) v = kadd(v, partials[b * 5 + d]);
-  out[d] = v;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = kadd(v, partials[b * 5 + d]);
+  __shared__ K2 red[4];
+  v = wave_reduce(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[d] = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
 }
 
 size_t round_ws_bytes() { return (size_t)RMAXBLK * 5 * sizeof(K2); }
@@ -458,7 +473,7 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   else
     hipLaunchKernelGGL(k_round_eval<false>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
   ktimer_end(round0 ? KT_ROUND0 : KT_ROUND, s);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partials, (int)nblk, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
 }
 
 // ---------------- folds ----------------
@@ -518,8 +533,11 @@ __global__ __launch_bounds__(256) void k_rlc_zmix(const int8_t* __restrict__ rho
   for (int r = 0; r < D; r++) acc[r] = 0;
   for (int i = 0; i < k; i++) {
     const int8_t* z = Zs[i] + c;
+    int8_t dz[D];
+#pragma unroll
+    for (int kk = 0; kk < D; kk++) dz[kk] = z[(size_t)kk * m];
     for (int kk = 0; kk < D; kk++) {
-      int d = z[(size_t)kk * m];
+      int d = dz[kk];
       if (d == 0) continue;
       const int8_t* col = rho_s + (i * D + kk) * D;
 #pragma unroll
@@ -539,29 +557,44 @@ void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t 
   ktimer_end(KT_ZMIX, s);
 }
 
-__global__ void k_dec_split(const int32_t* __restrict__ zmix, size_t len, int k_dec, int8_t* const* __restrict__ outs,
-                            int* __restrict__ flags) {
+__global__ __launch_bounds__(256) void k_dec_split(const int32_t* __restrict__ zmix, size_t len, int k_dec,
+                                                   int8_t* const* __restrict__ outs, int* __restrict__ flags) {
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= len) return;
-  int64_t v = zmix[idx];
-  int64_t B = (int64_t)1 << k_dec;
-  unsigned nz = 0;
-  bool bad = v >= B || v <= -B;
-  for (int i = 0; i < k_dec; i++) {
-    int64_t r = v % 2;
-    outs[i][idx] = (int8_t)r;
-    if (r) nz |= 1u << i;
-    v = (v - r) / 2;
+  unsigned nz = 0, bad = 0;
+  if (idx < len) {
+    int v = zmix[idx];
+    int B = 1 << k_dec;
+    bad = (v >= B || v <= -B) ? 1u : 0u;
+    for (int i = 0; i < k_dec; i++) {
+      int r = v % 2;  // balanced base-2 digit: sign of v when v is odd
+      outs[i][idx] = (int8_t)r;
+      nz |= (r != 0 ? 1u : 0u) << i;
+      v = (v - r) / 2;
+    }
+    if (v != 0) bad = 1;
   }
-  if (v != 0) bad = true;
-  // one flag word per digit plane; atomics on integer flags are exact
-  if (nz) {
+  // OR across the block, then one atomic per flag word (integer atomics are exact)
+  for (int off = 32; off > 0; off >>= 1) {
+    nz |= __shfl_down(nz, off, 64);
+    bad |= __shfl_down(bad, off, 64);
+  }
+  __shared__ unsigned red[8];
+  if ((threadIdx.x & 63) == 0) {
+    red[(threadIdx.x >> 6) * 2] = nz;
+    red[(threadIdx.x >> 6) * 2 + 1] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned a = 0, b = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      a |= red[2 * w];
+      b |= red[2 * w + 1];
+    }
+    // most blocks find the flags already raised: skip the contended atomic then
     for (int i = 0; i < k_dec; i++)
-      if ((nz >> i) & 1) {
-        if (flags[i] == 0) atomicOr(&flags[i], 1);
-      }
+      if (((a >> i) & 1) && __atomic_load_n(&flags[i], __ATOMIC_RELAXED) == 0) atomicOr(&flags[i], 1);
+    if (b) atomicOr(&flags[k_dec], 1);
   }
-  if (bad) atomicOr(&flags[k_dec], 1);
 }
 
 void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const* outs_dev, int* flags, hipStream_t s) {

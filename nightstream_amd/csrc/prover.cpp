@@ -536,36 +536,42 @@ std::vector<K2> AjtaiPhase::evals(const std::vector<K2>& xs) const {
 
 // ---------------- CCS oracle on the device ----------------
 CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* alpha, const K2* beta_a,
-                     const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st)
+                     const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st,
+                     OracleWs* ws)
     : s_(s), st_(st), k_(k), ell_d_(ell_d), ell_n_(ell_n) {
   const size_t n_pad = (size_t)1 << ell_n, m = s.m, n = s.n;
   if (n_pad != n) throw Error(1, "row domain must equal n (power of two)");
+  if (!ws) {
+    own_ = std::make_unique<OracleWs>();
+    ws = own_.get();
+  }
+  w_ = ws;
   len_ = n_pad;
   zs_.assign(Zs, Zs + k);
-  Zptrs_.alloc(k * sizeof(void*));
-  h2d(Zptrs_.p, (const void* const*)Zs, k, st);
+  w_->Zptrs.ensure(k * sizeof(void*));
+  h2d(w_->Zptrs.p, (const void* const*)Zs, k, st);
   nf_ = (int)s.fvar_mat.size();
   has_eval_ = k >= 2 && r_inputs != nullptr;
   nsc_ = 1 + nf_ + (has_eval_ ? 2 : 0);
   // scalar tables: [eqb][f vars][eqr][ev]
-  sc_[0].alloc((size_t)nsc_ * n_pad * sizeof(K2));
-  sc_[1].alloc((size_t)nsc_ * (n_pad / 2) * sizeof(K2));
-  K2* T = sc_[0].as<K2>();
-  DevBuf bits(64 * sizeof(K2));
-  h2d(bits.p, beta_r, ell_n, st);
-  launch_chi(bits.as<K2>(), (int)ell_n, T, st);
-  DevBuf z1(m * sizeof(K2));
-  launch_z1(Zs[0], m, z1.as<K2>(), st);
+  w_->sc[0].ensure((size_t)nsc_ * n_pad * sizeof(K2));
+  w_->sc[1].ensure((size_t)nsc_ * (n_pad / 2) * sizeof(K2));
+  K2* T = w_->sc[0].as<K2>();
+  w_->bits.ensure(128 * sizeof(K2));
+  K2* bits = w_->bits.as<K2>();
+  h2d(bits, beta_r, ell_n, st);
+  launch_chi(bits, (int)ell_n, T, st);
+  w_->z1.ensure(m * sizeof(K2));
+  launch_z1(Zs[0], m, w_->z1.as<K2>(), st);
   for (int f = 0; f < nf_; f++) {
     const DevSparse& M = s.dev[s.fvar_mat[f]];
-    launch_spmv(M, z1.as<K2>(), T + (size_t)(1 + f) * n_pad, kone(), false, st);
+    launch_spmv(M, w_->z1.as<K2>(), T + (size_t)(1 + f) * n_pad, kone(), false, st);
   }
   gamma_k_ = kone();
   for (size_t i = 0; i < k; i++) gamma_k_ = kmul(gamma_k_, gamma);
   if (has_eval_) {
-    DevBuf rb(64 * sizeof(K2));
-    h2d(rb.p, r_inputs, ell_n, st);
-    launch_chi(rb.as<K2>(), (int)ell_n, T + (size_t)(1 + nf_) * n_pad, st);
+    h2d(bits + 64, r_inputs, ell_n, st);
+    launch_chi(bits + 64, (int)ell_n, T + (size_t)(1 + nf_) * n_pad, st);
     // S_alpha = sum_{i>=1} gamma^i sum_rho chi_alpha[rho] Z_i[rho,:]
     std::vector<K2> W((k - 1) * D);
     K2 g = kone();
@@ -573,69 +579,69 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
       g = kmul(g, gamma);
       for (int rho = 0; rho < D; rho++) W[(i - 1) * D + rho] = kmul(g, eq_mask(rho, alpha, ell_d));
     }
-    DevBuf Wd(W.size() * sizeof(K2)), Sa(m * sizeof(K2));
-    h2d(Wd.p, W.data(), W.size(), st);
-    launch_weighted_digit_colsum(Zptrs_.as<const int8_t*>() + 1, k - 1, Wd.as<K2>(), m, Sa.as<K2>(), st);
+    w_->Wd.ensure(W.size() * sizeof(K2));
+    w_->Sa.ensure(m * sizeof(K2));
+    h2d(w_->Wd.p, W.data(), W.size(), st);
+    launch_weighted_digit_colsum(w_->Zptrs.as<const int8_t*>() + 1, k - 1, w_->Wd.as<K2>(), m, w_->Sa.as<K2>(), st);
     K2* ev = T + (size_t)(2 + nf_) * n_pad;
     K2 gkj = kone();
     for (size_t j = 0; j < s.t; j++) {
-      launch_spmv(s.dev[j], Sa.as<K2>(), ev, gkj, j > 0, st);
+      launch_spmv(s.dev[j], w_->Sa.as<K2>(), ev, gkj, j > 0, st);
       gkj = kmul(gkj, gamma_k_);
     }
-    NS_CHECK(hipStreamSynchronize(st));
   }
   // NC weights wg[w*54 + rho] = chi_{beta_a}(rho) gamma^{w+1}
   std::vector<K2> wg(k * D);
   {
     K2 g = gamma;
+    std::vector<K2> wb(D);
+    for (int rho = 0; rho < D; rho++) wb[rho] = eq_mask(rho, beta_a, ell_d);
     for (size_t w = 0; w < k; w++) {
-      for (int rho = 0; rho < D; rho++) wg[w * D + rho] = kmul(eq_mask(rho, beta_a, ell_d), g);
+      for (int rho = 0; rho < D; rho++) wg[w * D + rho] = kmul(wb[rho], g);
       g = kmul(g, gamma);
     }
   }
-  wg_.alloc(wg.size() * sizeof(K2));
-  h2d(wg_.p, wg.data(), wg.size(), st);
-  nc_[0].alloc(k * D * (n_pad / 2) * sizeof(K2));
-  nc_[1].alloc(k * D * std::max<size_t>(n_pad / 4, 1) * sizeof(K2));
-  coeffs_dev_.alloc(8 * sizeof(K2));
-  ws_.alloc(round_ws_bytes());
-  NS_CHECK(hipStreamSynchronize(st));
+  w_->wg.ensure(wg.size() * sizeof(K2));
+  h2d(w_->wg.p, wg.data(), wg.size(), st);
+  w_->nc[0].ensure(k * D * (n_pad / 2) * sizeof(K2));
+  w_->nc[1].ensure(k * D * std::max<size_t>(n_pad / 4, 1) * sizeof(K2));
+  w_->coeffs.ensure(8 * sizeof(K2));
+  w_->ws.ensure(round_ws_bytes());
+  NS_CHECK(hipStreamSynchronize(st));  // host vectors above go out of scope
 }
 
 std::vector<K2> CcsOracle::round_coeffs() {
   RowTables T;
-  const K2* sc = sc_[cur_sc_].as<K2>();
+  const K2* sc = w_->sc[cur_sc_].as<K2>();
   T.len = len_;
   T.eqb = sc;
   T.fv = sc + len_;
   T.nf = nf_;
   T.eqr = has_eval_ ? sc + (size_t)(1 + nf_) * len_ : nullptr;
   T.ev = has_eval_ ? sc + (size_t)(2 + nf_) * len_ : nullptr;
-  T.nc = round_ == 0 ? nullptr : nc_[cur_nc_].as<K2>();
-  T.digits = Zptrs_.as<const int8_t*>();
+  T.nc = round_ == 0 ? nullptr : w_->nc[cur_nc_].as<K2>();
+  T.digits = w_->Zptrs.as<const int8_t*>();
   T.m = s_.m;
   T.k = (int)k_;
   T.gamma_k = gamma_k_;
-  launch_round_eval(T, s_.fpoly, wg_.as<K2>(), round_ == 0, coeffs_dev_.as<K2>(), ws_.p, st_);
+  launch_round_eval(T, s_.fpoly, w_->wg.as<K2>(), round_ == 0, w_->coeffs.as<K2>(), w_->ws.p, st_);
   std::vector<K2> out(5);
-  d2h(out.data(), coeffs_dev_.p, 5, st_);
+  d2h(out.data(), w_->coeffs.p, 5, st_);
   return out;
 }
 
 void CcsOracle::fold(K2 r) {
   const size_t half = len_ / 2;
   int nsc_dst = cur_sc_ ^ 1;
-  launch_fold_k(sc_[cur_sc_].as<K2>(), (size_t)nsc_, len_, sc_[nsc_dst].as<K2>(), r, st_);
+  launch_fold_k(w_->sc[cur_sc_].as<K2>(), (size_t)nsc_, len_, w_->sc[nsc_dst].as<K2>(), r, st_);
   if (round_ == 0) {
-    launch_fold_digits(Zptrs_.as<const int8_t*>(), (int)k_, s_.m, len_, nc_[0].as<K2>(), r, st_);
+    launch_fold_digits(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_, w_->nc[0].as<K2>(), r, st_);
     cur_nc_ = 0;
   } else {
     int dst = cur_nc_ ^ 1;
-    launch_fold_k(nc_[cur_nc_].as<K2>(), k_ * D, len_, nc_[dst].as<K2>(), r, st_);
+    launch_fold_k(w_->nc[cur_nc_].as<K2>(), k_ * D, len_, w_->nc[dst].as<K2>(), r, st_);
     cur_nc_ = dst;
   }
-  // the scalar ping-pong buffer 1 holds at most n/2 entries per row: keep tables compact
-  if (nsc_dst == 1 && half > sc_[1].bytes / (nsc_ * sizeof(K2))) throw Error(3, "scalar table overflow");
   cur_sc_ = nsc_dst;
   len_ = half;
   round_++;
@@ -643,17 +649,18 @@ void CcsOracle::fold(K2 r) {
 
 std::vector<K2> CcsOracle::y_eval(const std::vector<K2>& r, DevBuf& ws, DevBuf& vjs) {
   const size_t m = s_.m, t = s_.t, n = s_.n;
-  DevBuf rb(64 * sizeof(K2)), chi(((size_t)1 << ell_n_) * sizeof(K2));
-  h2d(rb.p, r.data(), r.size(), st_);
-  launch_chi(rb.as<K2>(), (int)ell_n_, chi.as<K2>(), st_);
+  w_->bits.ensure(128 * sizeof(K2));
+  w_->chi.ensure(((size_t)1 << ell_n_) * sizeof(K2));
+  h2d(w_->bits.p, r.data(), r.size(), st_);
+  launch_chi(w_->bits.as<K2>(), (int)ell_n_, w_->chi.as<K2>(), st_);
   vjs.ensure(t * m * sizeof(K2));
   size_t n_eff = std::min(n, (size_t)1 << ell_n_);
-  for (size_t j = 0; j < t; j++) launch_spmv_t(s_.dev[j], chi.as<K2>(), n_eff, vjs.as<K2>() + j * m, st_);
+  for (size_t j = 0; j < t; j++) launch_spmv_t(s_.dev[j], w_->chi.as<K2>(), n_eff, vjs.as<K2>() + j * m, st_);
   ws.ensure(ydigits_ws_bytes(k_, t, m));
-  DevBuf y(k_ * t * D * sizeof(K2));
-  launch_ydigits(Zptrs_.as<const int8_t*>(), k_, vjs.as<K2>(), t, m, y.as<K2>(), ws.p, st_);
+  w_->ydev.ensure(k_ * t * D * sizeof(K2));
+  launch_ydigits(w_->Zptrs.as<const int8_t*>(), k_, vjs.as<K2>(), t, m, w_->ydev.as<K2>(), ws.p, st_);
   std::vector<K2> out(k_ * t * D);
-  d2h(out.data(), y.p, out.size(), st_);
+  d2h(out.data(), w_->ydev.p, out.size(), st_);
   return out;
 }
 
@@ -703,11 +710,12 @@ Prover::~Prover() { (void)hipStreamDestroy(stream_); }
 
 void Prover::commit_device(const int8_t* const* Zs_dev, size_t nw, uint64_t* host_out) {
   ws_ajtai_.ensure(ajtai_workspace_bytes(s_.m, nw));
-  DevBuf ptrs(nw * sizeof(void*)), out(nw * KAPPA * D * 8);
-  h2d(ptrs.p, (const void* const*)Zs_dev, nw, stream_);
-  ajtai_commit(pp_.limbs.as<int8_t>(), KAPPA, s_.m, ptrs.as<const int8_t*>(), nw, out.as<uint64_t>(), ws_ajtai_.p,
-               stream_);
-  d2h(host_out, out.p, nw * KAPPA * D, stream_);
+  ptrs_.ensure(nw * sizeof(void*));
+  cout_.ensure(nw * KAPPA * D * 8);
+  h2d(ptrs_.p, (const void* const*)Zs_dev, nw, stream_);
+  ajtai_commit(pp_.limbs.as<int8_t>(), KAPPA, s_.m, ptrs_.as<const int8_t*>(), nw, cout_.as<uint64_t>(),
+               ws_ajtai_.p, stream_);
+  d2h(host_out, cout_.p, nw * KAPPA * D, stream_);
 }
 
 static size_t log2_pow2(size_t n) {
@@ -751,12 +759,14 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
 
 
   // ---- step intake: digits and MCS commitments for all steps ----
-  std::vector<DevBuf> Zmcs(nsteps);
+  if (zmcs_.size() < nsteps) zmcs_.resize(nsteps);
+  std::vector<DevBuf>& Zmcs = zmcs_;
   std::vector<std::vector<uint64_t>> x_host(nsteps);
   {
-    DevBuf zbuf(m * 8);
+    zbuf_.ensure(m * 8);
+    DevBuf& zbuf = zbuf_;
     for (size_t i = 0; i < nsteps; i++) {
-      Zmcs[i].alloc((size_t)D * m);
+      Zmcs[i].ensure((size_t)D * m);
       const uint64_t* zsrc = z_steps[i];
       if (!z_on_device) {
         h2d(zbuf.p, z_steps[i], m, st);
@@ -784,8 +794,10 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   Transcript tr("neo.fold/session");
 
   std::vector<MeHost> acc;
-  std::vector<DevBuf> acc_wit;
-  DevBuf vjs, ws_yd;
+  std::vector<const int8_t*> acc_wit;  // digit planes of the accumulator (planes_[cur_planes])
+  int cur_planes = 0;
+  DevBuf& vjs = vjs_;
+  DevBuf& ws_yd = ws_yd_;
   const size_t xs_n = d_sc + 1;
   std::vector<K2> xs(xs_n);
   for (size_t i = 0; i < xs_n; i++) xs[i] = K2{i, 0};
@@ -905,9 +917,9 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     // device oracle
     std::vector<const int8_t*> Zs(kt);
     Zs[0] = Zmcs[idx].as<int8_t>();
-    for (size_t a = 0; a < nacc; a++) Zs[1 + a] = acc_wit[a].as<int8_t>();
+    for (size_t a = 0; a < nacc; a++) Zs[1 + a] = acc_wit[a];
     CcsOracle oracle(s, Zs.data(), kt, alpha.data(), beta_a, beta_r, gamma, ell_d, ell_n,
-                     nacc ? acc[0].r.data() : nullptr, st);
+                     nacc ? acc[0].r.data() : nullptr, st, &ows_);
     phase_mark(2);
     // batched time sum-check with the single CCS claim
     tr.append_msg_u64("batched/dynamic_bind/len", 1);
@@ -1137,20 +1149,26 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     }
     phase_mark(7);
     // Z_mix on the device
-    DevBuf zmix((size_t)D * m * 4), rho_dev(rho_i8.size()), zptr(kt * sizeof(void*));
+    zmix_.ensure((size_t)D * m * 4);
+    rho_dev_.ensure(rho_i8.size());
+    zptr_.ensure(kt * sizeof(void*));
+    DevBuf &zmix = zmix_, &rho_dev = rho_dev_, &zptr = zptr_;
     h2d(rho_dev.p, rho_i8.data(), rho_i8.size(), st);
     h2d(zptr.p, (const void* const*)Zs.data(), kt, st);
     launch_rlc_zmix(rho_dev.as<int8_t>(), zptr.as<const int8_t*>(), kt, m, zmix.as<int32_t>(), st);
 
     phase_mark(8);
     // ---- Pi_DEC ----
-    std::vector<DevBuf> planes(k_dec);
+    if (k_dec > 16) throw Error(1, "k_rho > 16 is not supported");
+    const int nxt_planes = cur_planes ^ 1;
     std::vector<const int8_t*> pptr(k_dec);
     for (size_t i = 0; i < k_dec; i++) {
-      planes[i].alloc((size_t)D * m);
-      pptr[i] = planes[i].as<int8_t>();
+      planes_[nxt_planes][i].ensure((size_t)D * m);
+      pptr[i] = planes_[nxt_planes][i].as<int8_t>();
     }
-    DevBuf pptr_dev(k_dec * sizeof(void*)), flags((k_dec + 1) * 4);
+    pptr_dev_.ensure(k_dec * sizeof(void*));
+    flags_.ensure((k_dec + 1) * 4);
+    DevBuf &pptr_dev = pptr_dev_, &flags = flags_;
     h2d(pptr_dev.p, (const void* const*)pptr.data(), k_dec, st);
     NS_CHECK(hipMemsetAsync(flags.p, 0, (k_dec + 1) * 4, st));
     launch_dec_split(zmix.as<int32_t>(), (size_t)D * m, (int)k_dec, (int8_t* const*)pptr_dev.p, flags.as<int>(), st);
@@ -1163,9 +1181,9 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     std::vector<K2> y_child(k_dec * t * D);
     {
       ws_yd.ensure(ydigits_ws_bytes(k_dec, t, m));
-      DevBuf ydev(y_child.size() * sizeof(K2));
-      launch_ydigits(pptr_dev.as<const int8_t*>(), k_dec, vjs.as<K2>(), t, m, ydev.as<K2>(), ws_yd.p, st);
-      d2h(y_child.data(), ydev.p, y_child.size(), st);
+      ydev_.ensure(y_child.size() * sizeof(K2));
+      launch_ydigits(pptr_dev.as<const int8_t*>(), k_dec, vjs.as<K2>(), t, m, ydev_.as<K2>(), ws_yd.p, st);
+      d2h(y_child.data(), ydev_.p, y_child.size(), st);
     }
     std::vector<MeHost> children(k_dec);
     for (size_t i = 0; i < k_dec; i++) {
@@ -1274,7 +1292,8 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     pb.u(0);
 
     acc = std::move(children);
-    acc_wit = std::move(planes);
+    acc_wit = pptr;
+    cur_planes = nxt_planes;
     phase_mark(11);
   }
   phase_mark(-1);

@@ -100,6 +100,11 @@ struct MeHost {
   uint8_t fold_digest[32];
 };
 
+// Device buffers of a Pi_CCS oracle, reused across fold steps (grow-only).
+struct OracleWs {
+  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev;
+};
+
 struct StepTiming {
   double transcript_s = 0, gpu_wait_s = 0;
 };
@@ -137,6 +142,10 @@ class Prover {
   Params params_;
   hipStream_t stream_;
   DevBuf ws_ajtai_, ws_round_, ws_yd_;
+  // buffers reused across fold steps
+  OracleWs ows_;
+  DevBuf planes_[2][16], zmix_, rho_dev_, zptr_, pptr_dev_, flags_, ydev_, ptrs_, cout_, zbuf_, vjs_;
+  std::vector<DevBuf> zmcs_;
 };
 
 // Ajtai-bit rounds of Pi_CCS on the host (optimized_engine/oracle.rs:1812-1951), from the
@@ -156,7 +165,8 @@ struct AjtaiPhase {
 class CcsOracle {
  public:
   CcsOracle(const Ccs& s, const int8_t* const* Zs_dev_host, size_t k, const K2* alpha, const K2* beta_a,
-            const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st);
+            const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st,
+            OracleWs* ws = nullptr);
   // coefficients of the current row round polynomial (deg_max + 1)
   std::vector<K2> round_coeffs();
   void fold(K2 r);
@@ -172,11 +182,9 @@ class CcsOracle {
   int nf_, nsc_;
   bool has_eval_;
   K2 gamma_k_;
-  DevBuf Zptrs_;          // device array of k digit pointers
-  DevBuf sc_[2];          // scalar tables [nsc][len]
-  DevBuf nc_[2];          // NC tables [k*54][len]
+  std::unique_ptr<OracleWs> own_;
+  OracleWs* w_;           // Zptrs: k digit pointers; sc: scalar tables [nsc][len]; nc: NC [k*54][len]
   int cur_sc_ = 0, cur_nc_ = 0;
-  DevBuf wg_, coeffs_dev_, ws_;
   std::vector<const int8_t*> zs_;
 };
 

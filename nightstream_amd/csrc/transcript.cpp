@@ -115,7 +115,19 @@ void Poseidon2::init(int w, ChaCha8& rng) {
 }
 
 typedef unsigned __int128 u128;
-static inline uint64_t red(u128 x) { return reduce128((uint64_t)x, (uint64_t)(x >> 64)); }
+// Partial reduction of any 128-bit value to a congruent u64 (possibly in [q, 2^64)): the borrow of
+// lo - hi_hi has probability ~2^-32 for product outputs and stays a predictable branch; the carry
+// of the second addition is folded branch-free.  Permutation outputs are canonicalized once.
+static inline uint64_t red(u128 x) {
+  uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+  uint64_t hi_hi = hi >> 32, hi_lo = hi & EPS;
+  uint64_t t0;
+  if (__builtin_expect(__builtin_sub_overflow(lo, hi_hi, &t0), 0)) t0 -= EPS;
+  uint64_t t1 = (hi_lo << 32) - hi_lo;
+  uint64_t t2;
+  bool c = __builtin_add_overflow(t0, t1, &t2);
+  return t2 + ((uint64_t)0 - (uint64_t)c & EPS);
+}
 
 static inline uint64_t sbox7(uint64_t x) {
   uint64_t x2 = red((u128)x * x);
@@ -164,6 +176,7 @@ static inline void permute_w(const Poseidon2& P, uint64_t* st) {
     for (int i = 0; i < W; i++) st[i] = sbox7(red((u128)st[i] + P.ext_term[r][i]));
     ext_layer<W>(st);
   }
+  for (int i = 0; i < W; i++) st[i] = st[i] >= Q ? st[i] - Q : st[i];
 }
 
 void Poseidon2::permute(uint64_t* st) const {

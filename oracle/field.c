@@ -17,22 +17,25 @@
 static inline fq reduce128(unsigned __int128 x) {
   uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
   uint64_t hi_hi = hi >> 32, hi_lo = hi & 0xFFFFFFFFULL;
-  /* x = lo + 2^64 hi_lo + 2^96 hi_hi ; 2^64 = 2^32-1, 2^96 = -1 (mod q) */
+  /* x = lo + 2^64 hi_lo + 2^96 hi_hi ; 2^64 = 2^32-1, 2^96 = -1 (mod q).  Carries and borrows
+     become masks (random data would otherwise mispredict). */
   uint64_t t0 = lo - hi_hi;
-  if (lo < hi_hi) t0 -= 0xFFFFFFFFULL; /* borrow: add q == subtract (2^32-1) mod 2^64 */
+  t0 -= (0 - (uint64_t)(lo < hi_hi)) & 0xFFFFFFFFULL;
   uint64_t t1 = (hi_lo << 32) - hi_lo;
   uint64_t t2 = t0 + t1;
-  if (t2 < t0) t2 += 0xFFFFFFFFULL;
-  if (t2 >= NEO_Q) t2 -= NEO_Q;
-  return t2;
+  t2 += (0 - (uint64_t)(t2 < t1)) & 0xFFFFFFFFULL;
+  return t2 >= NEO_Q ? t2 - NEO_Q : t2;
 }
 
 fq or_fq_add(fq a, fq b) {
   uint64_t s = a + b;
-  if (s < a || s >= NEO_Q) s -= NEO_Q;
-  return s;
+  s += (0 - (uint64_t)(s < a)) & 0xFFFFFFFFULL;
+  return s >= NEO_Q ? s - NEO_Q : s;
 }
-fq or_fq_sub(fq a, fq b) { return a >= b ? a - b : a + (NEO_Q - b); }
+fq or_fq_sub(fq a, fq b) {
+  uint64_t d = a - b;
+  return d - ((0 - (uint64_t)(a < b)) & 0xFFFFFFFFULL);
+}
 fq or_fq_neg(fq a) { return a ? NEO_Q - a : 0; }
 fq or_fq_mul(fq a, fq b) { return reduce128((unsigned __int128)a * b); }
 fq or_fq_pow(fq a, uint64_t e) {
