@@ -110,6 +110,7 @@ int or_extension_check(const or_params* p, uint32_t ell, uint32_t d_sc, uint32_t
 /* ---------------- Ajtai (neo-ajtai) ---------------- */
 void or_decomp_b2_row_major(const fq* z, size_t m, int8_t* Z);               /* decomp.rs:118-158 */
 void or_ajtai_setup(const uint8_t seed[32], size_t kappa, size_t m, fq* pp); /* commit.rs:133-156; pp[(i*m+j)*54+u] */
+/* Z: balanced base-2 digits in {-1, 0, 1} (row-major [54][m]); other values are ignored. */
 void or_ajtai_commit(const fq* pp, size_t kappa, size_t m, const int8_t* Z, fq* out); /* commit.rs:679-744 */
 void or_s_mul(const fq* rho_coeffs, const fq* c, size_t kappa, fq* out);     /* commit.rs:458-471 */
 
