@@ -7,8 +7,9 @@ timed region proves one shard of --steps fold steps (default 16, n = m = 2^16: t
 "plain method at scale on one GPU" workload) with the step witnesses already resident in HBM.
 
 Printed JSON line: metric/value/unit plus
-  roofline     -- the kernel class with the largest summed duration, measured live with HIP events
-                  on the prover's stream over the timed region, against MI355X peaks;
+  roofline     -- the kernel class with the largest summed duration, timed live by start/stop events
+                  that hipExtLaunchKernelGGL records on the prover's stream over the timed region,
+                  against MI355X peaks; traffic from the committed FETCH_SIZE/WRITE_SIZE summary;
   cpu_baseline -- the CPU restatement (oracle/, "port") timed on this host on a bounded sample.
 """
 from __future__ import annotations
@@ -106,10 +107,22 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 16)
     ap.add_argument("--cpu-steps", type=int, default=2, help="fold steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 plain element-wise copies of 512 MiB (int64 negate: 512 MiB read + 512 MiB "
+                         "written each) as the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.arange(1 << 26, dtype=torch.int64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (independent shards "
-                                   "shard across GPUs as separate jobs, see DESIGN.md)"}))
+                                   "shard across GPUs as separate jobs, one process per GPU)"}))
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
@@ -152,9 +165,15 @@ def main():
     avg_ms = ms[dom] / max(counts[dom], 1)
     per_launch = work[cls] / max(counts[dom], 1)
     achieved = (per_launch / (avg_ms * 1e-3)) / (1e9 if unit == "GB/s" else 1e12)
+    # HBM bytes per launch of this kernel, from the FETCH_SIZE / WRITE_SIZE summary of the default
+    # bench command (copy-kernel calibrated); only meaningful for the workload it was taken on
+    traffic = None
+    if args.n == 1 << 16 and args.steps == 16:
+        from nightstream_amd.profiling import load_traffic
+        traffic = load_traffic(os.path.join(ROOT, "profiles", "traffic_plain_shard_2e16.md"), KERNEL_NAMES[cls])
     roofline = {
         "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-        "frac": round(achieved / peak, 5), "traffic": None,
+        "frac": round(achieved / peak, 5), "traffic": traffic,
         "kernel": KERNEL_NAMES[cls], "launches": int(counts[dom]), "avg_ms": round(avg_ms, 5),
         "per_launch_algorithmic": per_launch,
         "kernel_ms": {KT[i]: round(ms[i], 4) for i in range(len(KT))},

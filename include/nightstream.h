@@ -100,7 +100,7 @@ int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* out
 /* ---- shard prover: fold_shard_prove (neo-fold/src/shard.rs:2285-2316) driven as
  *      FoldingSession::{add_step_io, fold_and_prove} (neo-fold/src/session.rs:1018-1311).
  * z_steps[s] is step s's full z = [x | w] (length m); x = z[0..m_in].  On success *proof
- * holds the canonical encoding (little-endian u64 words); free it
+ * holds the canonical encoding (little-endian u64 words, nightstream_amd/proof.py); free it
  * with ns_free. */
 int ns_prover_new(const ns_ccs* s, const ns_pp* pp, ns_prover** out);
 int ns_prove_shard(ns_prover* p, const uint64_t* const* z_steps, size_t n_steps, size_t m_in, uint8_t** proof,

@@ -248,10 +248,9 @@ void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* con
   int32_t* part = (int32_t*)workspace;
   int32_t* S = part + splits * (size_t)AJ_MROWS * npad;
   hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
-  ktimer_begin(KT_AJTAI_GEMM, s);
-  hipLaunchKernelGGL(k_ajtai_gemm, dim3((unsigned)(AJ_MROWS / (AJ_MT * 16)), (unsigned)groups, (unsigned)splits),
-                     dim3(64 * AJ_WAVES), 0, s, limbs, m, kc, Zs_dev, nw, npad, part);
-  ktimer_end(KT_AJTAI_GEMM, s);
+  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm,
+                  dim3((unsigned)(AJ_MROWS / (AJ_MT * 16)), (unsigned)groups, (unsigned)splits), dim3(64 * AJ_WAVES),
+                  0, s, limbs, m, kc, Zs_dev, nw, npad, part);
   hipLaunchKernelGGL(k_ajtai_epilogue, dim3((unsigned)(nw * KAPPA)), dim3(512), 0, s, part, splits, npad, S, out);
   (void)kappa;
 }

@@ -257,18 +257,18 @@ void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t 
   hipLaunchKernelGGL(k_v_limbs, dim3((unsigned)((m + 255) / 256), (unsigned)t), dim3(256), 0, s, v, t, m, Vl);
   hipLaunchKernelGGL(k_digit_rowsum2, dim3((unsigned)nrows), dim3(256), 0, s, Zs_dev, m, rowsum);
   dim3 g((unsigned)(mpad / 64), (unsigned)splits);
-  ktimer_begin(KT_YD_GEMM, s);
+#define NS_YD(T) NS_LAUNCH_TIMED(KT_YD_GEMM, k_ydigits_gemm<T>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad)
   switch (t) {
-    case 1: hipLaunchKernelGGL(k_ydigits_gemm<1>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
-    case 2: hipLaunchKernelGGL(k_ydigits_gemm<2>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
-    case 3: hipLaunchKernelGGL(k_ydigits_gemm<3>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
-    case 4: hipLaunchKernelGGL(k_ydigits_gemm<4>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
-    case 5: hipLaunchKernelGGL(k_ydigits_gemm<5>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
-    case 6: hipLaunchKernelGGL(k_ydigits_gemm<6>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
-    case 7: hipLaunchKernelGGL(k_ydigits_gemm<7>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
-    default: hipLaunchKernelGGL(k_ydigits_gemm<8>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); break;
+    case 1: NS_YD(1); break;
+    case 2: NS_YD(2); break;
+    case 3: NS_YD(3); break;
+    case 4: NS_YD(4); break;
+    case 5: NS_YD(5); break;
+    case 6: NS_YD(6); break;
+    case 7: NS_YD(7); break;
+    default: NS_YD(8); break;
   }
-  ktimer_end(KT_YD_GEMM, s);
+#undef NS_YD
   size_t nt = nrows * t;
   hipLaunchKernelGGL(k_ydigits_epilogue, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, part, splits, mpad,
                      nrows, t, rowsum, y);
@@ -467,12 +467,10 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   if (nblk < 1) nblk = 1;
   size_t shmem = ((size_t)T.k * D + 4 * 5) * sizeof(K2);
   K2* partials = (K2*)ws;
-  ktimer_begin(round0 ? KT_ROUND0 : KT_ROUND, s);
   if (round0)
-    hipLaunchKernelGGL(k_round_eval<true>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
+    NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<true>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
   else
-    hipLaunchKernelGGL(k_round_eval<false>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
-  ktimer_end(round0 ? KT_ROUND0 : KT_ROUND, s);
+    NS_LAUNCH_TIMED(KT_ROUND, k_round_eval<false>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
   hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
 }
 
@@ -490,10 +488,8 @@ __global__ void k_fold_k(const K2* __restrict__ src, size_t len, K2* __restrict_
 void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStream_t s) {
   size_t half = len / 2;
   if (rows == 0 || half == 0) return;
-  ktimer_begin(KT_FOLD, s);
-  hipLaunchKernelGGL(k_fold_k, dim3((unsigned)((half + 255) / 256), (unsigned)rows), dim3(256), 0, s, src, len, dst,
-                     r);
-  ktimer_end(KT_FOLD, s);
+  NS_LAUNCH_TIMED(KT_FOLD, k_fold_k, dim3((unsigned)((half + 255) / 256), (unsigned)rows), dim3(256), 0, s, src, len,
+                  dst, r);
 }
 
 __global__ void k_fold_digits(const int8_t* const* __restrict__ digits, size_t m, size_t len, K2* __restrict__ dst,
@@ -551,10 +547,8 @@ __global__ __launch_bounds__(256) void k_rlc_zmix(const int8_t* __restrict__ rho
 void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
                      hipStream_t s) {
   size_t shmem = k * D * D;
-  ktimer_begin(KT_ZMIX, s);
-  hipLaunchKernelGGL(k_rlc_zmix, dim3((unsigned)((m + 255) / 256)), dim3(256), shmem, s, rho_dev, Zs_dev, (int)k, m,
-                     zmix);
-  ktimer_end(KT_ZMIX, s);
+  NS_LAUNCH_TIMED(KT_ZMIX, k_rlc_zmix, dim3((unsigned)((m + 255) / 256)), dim3(256), shmem, s, rho_dev, Zs_dev, (int)k,
+                  m, zmix);
 }
 
 __global__ __launch_bounds__(256) void k_dec_split(const int32_t* __restrict__ zmix, size_t len, int k_dec,
@@ -598,10 +592,8 @@ __global__ __launch_bounds__(256) void k_dec_split(const int32_t* __restrict__ z
 }
 
 void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const* outs_dev, int* flags, hipStream_t s) {
-  ktimer_begin(KT_SPLIT, s);
-  hipLaunchKernelGGL(k_dec_split, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, zmix, len, k_dec, outs_dev,
-                     flags);
-  ktimer_end(KT_SPLIT, s);
+  NS_LAUNCH_TIMED(KT_SPLIT, k_dec_split, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, zmix, len, k_dec,
+                  outs_dev, flags);
 }
 
 }  // namespace ns

@@ -4,6 +4,7 @@
 // digit matrices are int8 row-major D x m (Z[rho*m + c]); K elements are ns::K2 (16 bytes).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -32,11 +33,19 @@ struct HipError {
   HipError(hipError_t e, const char* w, const char* f, int l) : err(e), what(w), file(f), line(l) {}
 };
 
-// Per-kernel timing hooks (HIP events on the launch stream); no-ops unless a prover enabled
-// timing on the calling thread.
+// Per-kernel timing: when a prover enabled timing on the calling thread, ktimer_events hands out
+// a start/stop event pair that hipExtLaunchKernelGGL records at the dispatch's own start and end
+// (so host submission gaps are not counted); otherwise it returns false and the launch is plain.
 enum KernelClass { KT_ROUND0 = 0, KT_ROUND, KT_FOLD, KT_AJTAI_GEMM, KT_YD_GEMM, KT_ZMIX, KT_SPLIT, KT_N };
-void ktimer_begin(int cls, hipStream_t s);
-void ktimer_end(int cls, hipStream_t s);
+bool ktimer_events(int cls, hipEvent_t* start, hipEvent_t* stop);
+#define NS_LAUNCH_TIMED(cls, kernel, grid, block, shmem, stream, ...)                                   \
+  do {                                                                                                \
+    hipEvent_t ns_ev_a_ = nullptr, ns_ev_b_ = nullptr;                                                \
+    if (ktimer_events((cls), &ns_ev_a_, &ns_ev_b_))                                                   \
+      hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, ns_ev_a_, ns_ev_b_, 0, __VA_ARGS__); \
+    else                                                                                              \
+      hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                            \
+  } while (0)
 
 // Sparse matrix on the device: CSR (rows) for M x and CSC (columns) for M^T x.
 struct DevSparse {

@@ -48,8 +48,6 @@ struct KTimers {
     hipEvent_t a, b;
   };
   std::vector<Rec> recs;
-  int open_cls = -1;
-  hipEvent_t open_ev = nullptr;
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   hipEvent_t get() {
@@ -79,19 +77,12 @@ struct KTimers {
 thread_local KTimers* g_kt = nullptr;
 }  // namespace
 
-void ktimer_begin(int cls, hipStream_t s) {
-  if (!g_kt) return;
-  g_kt->open_cls = cls;
-  g_kt->open_ev = g_kt->get();
-  NS_CHECK(hipEventRecord(g_kt->open_ev, s));
-}
-
-void ktimer_end(int cls, hipStream_t s) {
-  if (!g_kt || g_kt->open_cls != cls) return;
-  hipEvent_t e = g_kt->get();
-  NS_CHECK(hipEventRecord(e, s));
-  g_kt->recs.push_back({cls, g_kt->open_ev, e});
-  g_kt->open_cls = -1;
+bool ktimer_events(int cls, hipEvent_t* start, hipEvent_t* stop) {
+  if (!g_kt) return false;
+  *start = g_kt->get();
+  *stop = g_kt->get();
+  g_kt->recs.push_back({cls, *start, *stop});
+  return true;
 }
 
 // ---------------- params (neo-params/src/lib.rs) ----------------
