@@ -182,6 +182,44 @@ double or_last_step_seconds(size_t i);
 int or_num_threads(void);
 const char* or_last_error(void);
 
+
+/* ---------------- Twist / Shout oracles (neo-memory/src/twist_oracle.rs) ----------------
+ * Sparse time columns: nnz entries (idx[i], vals[2i..2i+1] as K), over a length-2^ell_n domain.
+ * All oracles share evals/fold/num_rounds/degree_bound (RoundOracle, sumcheck.rs:17-34). */
+typedef struct {
+  size_t nnz;
+  const uint64_t* idx;
+  const uint64_t* vals;
+} or_sparse_in;
+typedef struct or_ts or_ts;
+or_ts* or_ts_product(const kx* const* factors, size_t nf, size_t len, size_t degree_bound);
+or_ts* or_ts_shout_value(const kx* r_cycle, size_t ell_n, or_sparse_in has_lookup, or_sparse_in val, kx* claim);
+or_ts* or_ts_index_adapter(const kx* r_cycle, size_t ell_n, or_sparse_in has_lookup, const or_sparse_in* addr_bits,
+                           size_t ell_addr, const kx* r_addr, kx* claim);
+or_ts* or_ts_bitness(const kx* r_cycle, size_t ell_n, const or_sparse_in* cols, const kx* weights, size_t ncols);
+or_ts* or_ts_twist_read(const kx* r_cycle, size_t ell_n, or_sparse_in has_read, or_sparse_in rv,
+                        const or_sparse_in* ra_bits, or_sparse_in has_write, or_sparse_in inc,
+                        const or_sparse_in* wa_bits, size_t ell_addr, const kx* r_addr, kx init_at_r_addr);
+or_ts* or_ts_twist_write(const kx* r_cycle, size_t ell_n, or_sparse_in has_write, or_sparse_in wv, or_sparse_in inc,
+                         const or_sparse_in* wa_bits, size_t ell_addr, const kx* r_addr, kx init_at_r_addr);
+or_ts* or_ts_val_eval(size_t ell_n, const or_sparse_in* wa_bits, size_t ell_addr, or_sparse_in has_write,
+                      or_sparse_in inc, const kx* r_addr, const kx* r_time, kx* claim);
+or_ts* or_ts_total_inc(size_t ell_n, const or_sparse_in* wa_bits, size_t ell_addr, or_sparse_in has_write,
+                       or_sparse_in inc, const kx* r_addr, kx* claim);
+or_ts* or_ts_read_addr(const uint64_t* init_addrs, const kx* init_vals, size_t ninit, const kx* r_cycle, size_t ell_n,
+                       or_sparse_in has_read, or_sparse_in rv, const or_sparse_in* ra_bits, or_sparse_in has_write,
+                       const or_sparse_in* wa_bits, or_sparse_in inc, size_t ell_addr);
+or_ts* or_ts_write_addr(const uint64_t* init_addrs, const kx* init_vals, size_t ninit, const kx* r_cycle,
+                        size_t ell_n, or_sparse_in has_write, or_sparse_in wv, const or_sparse_in* wa_bits,
+                        or_sparse_in inc, size_t ell_addr);
+or_ts* or_ts_addr_lookup(const or_sparse_in* addr_bits, size_t ell_addr, or_sparse_in has_lookup, const kx* table,
+                         size_t table_len, const kx* r_cycle, size_t ell_n, kx* claim);
+void or_ts_evals(or_ts* o, const kx* xs, size_t nx, kx* ys);
+void or_ts_fold(or_ts* o, kx r);
+size_t or_ts_num_rounds(const or_ts* o);
+size_t or_ts_degree_bound(const or_ts* o);
+void or_ts_free(or_ts* o);
+
 #ifdef __cplusplus
 }
 #endif
