@@ -266,3 +266,165 @@ class Prover:
             lib().or_prover_free(self.ptr)
         except Exception:
             pass
+
+
+# ---------------- Twist / Shout oracles (oracle/twist.c) ----------------
+class SparseIn(C.Structure):
+    _fields_ = [("nnz", C.c_size_t), ("idx", C.POINTER(C.c_uint64)), ("vals", C.POINTER(C.c_uint64))]
+
+
+def _kbuf(ks):
+    a = np.array([v for k in ks for v in (int(k[0]), int(k[1]))] or [0, 0], dtype=np.uint64)
+    return a
+
+
+class _Keep:
+    def __init__(self):
+        self.items = []
+
+    def sparse(self, col):
+        """col: (idx array, list of K) -> SparseIn (keeps the buffers alive)."""
+        idx = np.ascontiguousarray(np.asarray(col[0], np.uint64).reshape(-1)) if len(col[0]) else np.zeros(1, np.uint64)
+        vals = _kbuf(col[1])
+        self.items += [idx, vals]
+        return SparseIn(len(col[0]), idx.ctypes.data_as(C.POINTER(C.c_uint64)),
+                        vals.ctypes.data_as(C.POINTER(C.c_uint64)))
+
+    def sparse_arr(self, cols):
+        arr = (SparseIn * max(1, len(cols)))(*[self.sparse(c) for c in cols])
+        self.items.append(arr)
+        return arr
+
+    def ks(self, ks):
+        a = _kbuf(ks)
+        self.items.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+
+def _ts_lib():
+    L = lib()
+    if getattr(L, "_ts_bound", False):
+        return L
+    vp, sz = C.c_void_p, C.c_size_t
+    KP = C.POINTER(KX)
+    L.or_ts_product.restype = vp
+    L.or_ts_product.argtypes = [vp, sz, sz, sz]
+    L.or_ts_shout_value.restype = vp
+    L.or_ts_shout_value.argtypes = [vp, sz, SparseIn, SparseIn, KP]
+    L.or_ts_index_adapter.restype = vp
+    L.or_ts_index_adapter.argtypes = [vp, sz, SparseIn, vp, sz, vp, KP]
+    L.or_ts_bitness.restype = vp
+    L.or_ts_bitness.argtypes = [vp, sz, vp, vp, sz]
+    L.or_ts_twist_read.restype = vp
+    L.or_ts_twist_read.argtypes = [vp, sz, SparseIn, SparseIn, vp, SparseIn, SparseIn, vp, sz, vp, KX]
+    L.or_ts_twist_write.restype = vp
+    L.or_ts_twist_write.argtypes = [vp, sz, SparseIn, SparseIn, SparseIn, vp, sz, vp, KX]
+    L.or_ts_val_eval.restype = vp
+    L.or_ts_val_eval.argtypes = [sz, vp, sz, SparseIn, SparseIn, vp, vp, KP]
+    L.or_ts_total_inc.restype = vp
+    L.or_ts_total_inc.argtypes = [sz, vp, sz, SparseIn, SparseIn, vp, KP]
+    L.or_ts_read_addr.restype = vp
+    L.or_ts_read_addr.argtypes = [vp, vp, sz, vp, sz, SparseIn, SparseIn, vp, SparseIn, vp, SparseIn, sz]
+    L.or_ts_write_addr.restype = vp
+    L.or_ts_write_addr.argtypes = [vp, vp, sz, vp, sz, SparseIn, SparseIn, vp, SparseIn, sz]
+    L.or_ts_addr_lookup.restype = vp
+    L.or_ts_addr_lookup.argtypes = [vp, sz, SparseIn, vp, sz, vp, sz, KP]
+    L.or_ts_evals.argtypes = [vp, vp, sz, vp]
+    L.or_ts_fold.argtypes = [vp, KX]
+    L.or_ts_num_rounds.restype = sz
+    L.or_ts_num_rounds.argtypes = [vp]
+    L.or_ts_degree_bound.restype = sz
+    L.or_ts_degree_bound.argtypes = [vp]
+    L.or_ts_free.argtypes = [vp]
+    L._ts_bound = True
+    return L
+
+
+class TsOracle:
+    """A Twist/Shout round oracle of the CPU restatement.  Columns are (idx, [K]) sparse pairs."""
+
+    def __init__(self, kind, *args):
+        L = _ts_lib()
+        k = _Keep()
+        self._keep = k
+        claim = KX(0, 0)
+        cp = C.byref(claim)
+        if kind == "product":
+            factors, degree_bound = args
+            bufs = [_kbuf(f) for f in factors]
+            k.items += bufs
+            ptrs = (C.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+            k.items.append(ptrs)
+            self.h = L.or_ts_product(ptrs, len(factors), len(factors[0]), degree_bound)
+        elif kind == "shout_value":
+            r_cycle, has_lookup, val = args
+            self.h = L.or_ts_shout_value(k.ks(r_cycle), len(r_cycle), k.sparse(has_lookup), k.sparse(val), cp)
+        elif kind == "index_adapter":
+            r_cycle, has_lookup, addr_bits, r_addr = args
+            self.h = L.or_ts_index_adapter(k.ks(r_cycle), len(r_cycle), k.sparse(has_lookup), k.sparse_arr(addr_bits),
+                                           len(addr_bits), k.ks(r_addr), cp)
+        elif kind == "bitness":
+            r_cycle, cols, weights = args
+            self.h = L.or_ts_bitness(k.ks(r_cycle), len(r_cycle), k.sparse_arr(cols), k.ks(weights), len(cols))
+        elif kind == "twist_read":
+            r_cycle, has_read, rv, ra_bits, has_write, inc, wa_bits, r_addr, init_at = args
+            self.h = L.or_ts_twist_read(k.ks(r_cycle), len(r_cycle), k.sparse(has_read), k.sparse(rv),
+                                        k.sparse_arr(ra_bits), k.sparse(has_write), k.sparse(inc),
+                                        k.sparse_arr(wa_bits), len(r_addr), k.ks(r_addr), KX(*init_at))
+        elif kind == "twist_write":
+            r_cycle, has_write, wv, inc, wa_bits, r_addr, init_at = args
+            self.h = L.or_ts_twist_write(k.ks(r_cycle), len(r_cycle), k.sparse(has_write), k.sparse(wv),
+                                         k.sparse(inc), k.sparse_arr(wa_bits), len(r_addr), k.ks(r_addr), KX(*init_at))
+        elif kind == "val_eval":
+            ell_n, wa_bits, has_write, inc, r_addr, r_time = args
+            self.h = L.or_ts_val_eval(ell_n, k.sparse_arr(wa_bits), len(wa_bits), k.sparse(has_write), k.sparse(inc),
+                                      k.ks(r_addr), k.ks(r_time), cp)
+        elif kind == "total_inc":
+            ell_n, wa_bits, has_write, inc, r_addr = args
+            self.h = L.or_ts_total_inc(ell_n, k.sparse_arr(wa_bits), len(wa_bits), k.sparse(has_write),
+                                       k.sparse(inc), k.ks(r_addr), cp)
+        elif kind == "read_addr":
+            init, r_cycle, has_read, rv, ra_bits, has_write, wa_bits, inc = args
+            ia = np.array([a for a, _ in init] or [0], np.uint64)
+            k.items.append(ia)
+            self.h = L.or_ts_read_addr(ia.ctypes.data, k.ks([v for _, v in init]), len(init), k.ks(r_cycle),
+                                       len(r_cycle), k.sparse(has_read), k.sparse(rv), k.sparse_arr(ra_bits),
+                                       k.sparse(has_write), k.sparse_arr(wa_bits), k.sparse(inc), len(ra_bits))
+        elif kind == "write_addr":
+            init, r_cycle, has_write, wv, wa_bits, inc = args
+            ia = np.array([a for a, _ in init] or [0], np.uint64)
+            k.items.append(ia)
+            self.h = L.or_ts_write_addr(ia.ctypes.data, k.ks([v for _, v in init]), len(init), k.ks(r_cycle),
+                                        len(r_cycle), k.sparse(has_write), k.sparse(wv), k.sparse_arr(wa_bits),
+                                        k.sparse(inc), len(wa_bits))
+        elif kind == "addr_lookup":
+            addr_bits, has_lookup, table, r_cycle = args
+            self.h = L.or_ts_addr_lookup(k.sparse_arr(addr_bits), len(addr_bits), k.sparse(has_lookup), k.ks(table),
+                                         len(table), k.ks(r_cycle), len(r_cycle), cp)
+        else:
+            raise ValueError(kind)
+        if not self.h:
+            raise RuntimeError("oracle construction failed")
+        self.claim = (claim.c0, claim.c1)
+
+    def num_rounds(self):
+        return _ts_lib().or_ts_num_rounds(self.h)
+
+    def degree_bound(self):
+        return _ts_lib().or_ts_degree_bound(self.h)
+
+    def evals(self, xs):
+        xb = _kbuf(xs)
+        out = np.zeros(2 * len(xs), np.uint64)
+        _ts_lib().or_ts_evals(self.h, xb.ctypes.data, len(xs), out.ctypes.data)
+        o = [int(v) for v in out]
+        return [(o[2 * i], o[2 * i + 1]) for i in range(len(xs))]
+
+    def fold(self, r):
+        _ts_lib().or_ts_fold(self.h, KX(int(r[0]), int(r[1])))
+
+    def __del__(self):
+        try:
+            _ts_lib().or_ts_free(self.h)
+        except Exception:
+            pass
