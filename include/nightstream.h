@@ -119,6 +119,65 @@ int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, siz
 int ns_prover_phase_stats(const ns_prover* p, double* seconds, size_t n);
 void ns_prover_free(ns_prover* p);
 
+/* ---- Twist/Shout round oracles (neo-memory/src/twist_oracle.rs).  Each handle implements the
+ *      RoundOracle contract (neo-reductions/src/sumcheck.rs:17-34) through ns_ts_evals /
+ *      ns_ts_fold / ns_ts_num_rounds / ns_ts_degree_bound.  Sparse time columns are SparseIdxVec
+ *      (neo-memory/src/sparse_time.rs:10-24): nnz strictly increasing indices in [0, 2^ell_n)
+ *      and their K values (vals[2i], vals[2i+1]).  claim, where given, receives the sum the
+ *      reference constructor returns. */
+typedef struct {
+  size_t nnz;
+  const uint64_t* idx;
+  const uint64_t* vals;
+} ns_sparse_col;
+typedef struct ns_ts ns_ts;
+/* ProductRoundOracle::new (twist_oracle.rs:56-76): nf dense factors of length len (power of 2). */
+int ns_ts_product_new(const uint64_t* const* factors, size_t nf, size_t len, size_t degree_bound, ns_ts** out);
+/* ShoutValueOracleSparse::new (twist_oracle.rs:213-245). */
+int ns_ts_shout_value_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_lookup, ns_sparse_col val,
+                          uint64_t claim[2], ns_ts** out);
+/* IndexAdapterOracleSparseTime::new_with_gate (twist_oracle.rs:396-442). */
+int ns_ts_index_adapter_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_lookup,
+                            const ns_sparse_col* addr_bits, size_t ell_addr, const uint64_t* r_addr, uint64_t claim[2],
+                            ns_ts** out);
+/* LazyWeightedBitnessOracleSparseTime::new_with_cycle (twist_oracle.rs:525-541). */
+int ns_ts_bitness_new(const uint64_t* r_cycle, size_t ell_n, const ns_sparse_col* cols, const uint64_t* weights,
+                      size_t ncols, ns_ts** out);
+/* TwistReadCheckOracleSparseTime::new (twist_oracle.rs:641-684). */
+int ns_ts_twist_read_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_read, ns_sparse_col rv,
+                         const ns_sparse_col* ra_bits, ns_sparse_col has_write, ns_sparse_col inc,
+                         const ns_sparse_col* wa_bits, size_t ell_addr, const uint64_t* r_addr,
+                         const uint64_t init_at_r_addr[2], ns_ts** out);
+/* TwistWriteCheckOracleSparseTime::new (twist_oracle.rs:787-822). */
+int ns_ts_twist_write_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_write, ns_sparse_col wv,
+                          ns_sparse_col inc, const ns_sparse_col* wa_bits, size_t ell_addr, const uint64_t* r_addr,
+                          const uint64_t init_at_r_addr[2], ns_ts** out);
+/* TwistValEvalOracleSparseTime::new (twist_oracle.rs:927-972); r_time has ell_n entries. */
+int ns_ts_val_eval_new(size_t ell_n, const ns_sparse_col* wa_bits, size_t ell_addr, ns_sparse_col has_write,
+                       ns_sparse_col inc, const uint64_t* r_addr, const uint64_t* r_time, uint64_t claim[2],
+                       ns_ts** out);
+/* TwistTotalIncOracleSparseTime::new (twist_oracle.rs:1063-1104). */
+int ns_ts_total_inc_new(size_t ell_n, const ns_sparse_col* wa_bits, size_t ell_addr, ns_sparse_col has_write,
+                        ns_sparse_col inc, const uint64_t* r_addr, uint64_t claim[2], ns_ts** out);
+/* TwistReadCheckAddrOracleSparseTime::new (twist_oracle.rs:1282-1378); init: (addr, K) pairs. */
+int ns_ts_read_addr_new(const uint64_t* init_addrs, const uint64_t* init_vals, size_t ninit, const uint64_t* r_cycle,
+                        size_t ell_n, ns_sparse_col has_read, ns_sparse_col rv, const ns_sparse_col* ra_bits,
+                        ns_sparse_col has_write, const ns_sparse_col* wa_bits, ns_sparse_col inc, size_t ell_addr,
+                        ns_ts** out);
+/* TwistWriteCheckAddrOracleSparseTime::new (twist_oracle.rs:1489-1547). */
+int ns_ts_write_addr_new(const uint64_t* init_addrs, const uint64_t* init_vals, size_t ninit,
+                         const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_write, ns_sparse_col wv,
+                         const ns_sparse_col* wa_bits, ns_sparse_col inc, size_t ell_addr, ns_ts** out);
+/* AddressLookupOracle::new (twist_oracle.rs:1684-1736): table of table_len K entries. */
+int ns_ts_addr_lookup_new(const ns_sparse_col* addr_bits, size_t ell_addr, ns_sparse_col has_lookup,
+                          const uint64_t* table, size_t table_len, const uint64_t* r_cycle, size_t ell_n,
+                          uint64_t claim[2], ns_ts** out);
+int ns_ts_evals(ns_ts* o, const uint64_t* xs, size_t n, uint64_t* out);
+int ns_ts_fold(ns_ts* o, const uint64_t r[2]);
+size_t ns_ts_num_rounds(const ns_ts* o);
+size_t ns_ts_degree_bound(const ns_ts* o);
+void ns_ts_free(ns_ts* o);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,10 @@ EXPORTS = [
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
     "ns_dec_split", "ns_prover_new", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_free",
+    "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
+    "ns_ts_twist_read_new", "ns_ts_twist_write_new", "ns_ts_val_eval_new", "ns_ts_total_inc_new",
+    "ns_ts_read_addr_new", "ns_ts_write_addr_new", "ns_ts_addr_lookup_new", "ns_ts_evals", "ns_ts_fold",
+    "ns_ts_num_rounds", "ns_ts_degree_bound", "ns_ts_free",
 ]
 
 
@@ -52,7 +56,8 @@ def lib():
     for name in EXPORTS:
         fn = getattr(L, name)
         if name not in ("ns_last_error", "ns_device_count", "ns_free", "ns_oracle_num_rounds",
-                        "ns_oracle_degree_bound") and not name.endswith("_free"):
+                        "ns_oracle_degree_bound", "ns_ts_num_rounds", "ns_ts_degree_bound") \
+                and not name.endswith("_free"):
             fn.restype = C.c_int
     L.ns_oracle_num_rounds.restype = sz
     L.ns_oracle_degree_bound.restype = sz

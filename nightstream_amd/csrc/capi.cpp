@@ -7,6 +7,7 @@
 
 #include "../../include/nightstream.h"
 #include "prover.h"
+#include "twist.h"
 
 using namespace ns;
 
@@ -395,3 +396,192 @@ int ns_prover_phase_stats(const ns_prover* p, double* seconds, size_t n) {
 void ns_prover_free(ns_prover* p) { delete p; }
 
 }  // extern "C"
+
+// ---------------- Twist / Shout oracles ----------------
+struct ns_ts {
+  std::unique_ptr<TsOracleGpu> o;
+};
+
+static SparseCol to_col(const ns_sparse_col& c) {
+  SparseCol s;
+  s.idx.resize(c.nnz);
+  s.val.resize(c.nnz);
+  for (size_t i = 0; i < c.nnz; i++) {
+    if (c.idx[i] > 0xFFFFFFFFull) throw Error(1, "sparse column index exceeds 32 bits");
+    s.idx[i] = (uint32_t)c.idx[i];
+    s.val[i] = K2{c.vals[2 * i], c.vals[2 * i + 1]};
+    if (s.val[i].c0 >= Q || s.val[i].c1 >= Q) throw Error(1, "sparse column value is not canonical");
+  }
+  return s;
+}
+
+static std::vector<SparseCol> to_cols(const ns_sparse_col* c, size_t n) {
+  std::vector<SparseCol> v;
+  for (size_t i = 0; i < n; i++) v.push_back(to_col(c[i]));
+  return v;
+}
+
+template <class F>
+static int ts_make(ns_ts** out, uint64_t* claim, F&& f) {
+  return guard([&] {
+    if (!out) throw Error(1, "null output handle");
+    auto h = std::make_unique<ns_ts>();
+    h->o = std::make_unique<TsOracleGpu>();
+    f(*h->o);
+    if (claim) {
+      K2 c = h->o->claim();
+      claim[0] = c.c0;
+      claim[1] = c.c1;
+    }
+    *out = h.release();
+  });
+}
+
+extern "C" int ns_ts_product_new(const uint64_t* const* factors, size_t nf, size_t len, size_t degree_bound,
+                                 ns_ts** out) {
+  return ts_make(out, nullptr, [&](TsOracleGpu& o) {
+    std::vector<std::vector<K2>> fs;
+    for (size_t f = 0; f < nf; f++) fs.push_back(kvec(factors[f], len));
+    o.init_product(fs, degree_bound);
+  });
+}
+
+extern "C" int ns_ts_shout_value_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_lookup,
+                                     ns_sparse_col val, uint64_t claim[2], ns_ts** out) {
+  return ts_make(out, claim, [&](TsOracleGpu& o) {
+    SparseCol g = to_col(has_lookup), a = to_col(val);
+    o.init_time(TSK_SHOUT_VALUE, kvec(r_cycle, ell_n), ell_n, &g, &a, nullptr, {}, {}, {}, kzero(), nullptr, nullptr,
+                nullptr);
+  });
+}
+
+extern "C" int ns_ts_index_adapter_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_lookup,
+                                       const ns_sparse_col* addr_bits, size_t ell_addr, const uint64_t* r_addr,
+                                       uint64_t claim[2], ns_ts** out) {
+  return ts_make(out, claim, [&](TsOracleGpu& o) {
+    SparseCol g = to_col(has_lookup);
+    o.init_time(TSK_INDEX_ADAPTER, kvec(r_cycle, ell_n), ell_n, &g, nullptr, nullptr, to_cols(addr_bits, ell_addr),
+                kvec(r_addr, ell_addr), {}, kzero(), nullptr, nullptr, nullptr);
+  });
+}
+
+extern "C" int ns_ts_bitness_new(const uint64_t* r_cycle, size_t ell_n, const ns_sparse_col* cols,
+                                 const uint64_t* weights, size_t ncols, ns_ts** out) {
+  return ts_make(out, nullptr, [&](TsOracleGpu& o) {
+    o.init_time(TSK_BITNESS, kvec(r_cycle, ell_n), ell_n, nullptr, nullptr, nullptr, to_cols(cols, ncols), {},
+                kvec(weights, ncols), kzero(), nullptr, nullptr, nullptr);
+  });
+}
+
+extern "C" int ns_ts_twist_read_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_read, ns_sparse_col rv,
+                                    const ns_sparse_col* ra_bits, ns_sparse_col has_write, ns_sparse_col inc,
+                                    const ns_sparse_col* wa_bits, size_t ell_addr, const uint64_t* r_addr,
+                                    const uint64_t init_at_r_addr[2], ns_ts** out) {
+  return ts_make(out, nullptr, [&](TsOracleGpu& o) {
+    SparseCol g = to_col(has_read), a = to_col(rv), hw = to_col(has_write), ic = to_col(inc);
+    std::vector<SparseCol> wb = to_cols(wa_bits, ell_addr);
+    o.init_time(TSK_TWIST_READ, kvec(r_cycle, ell_n), ell_n, &g, &a, nullptr, to_cols(ra_bits, ell_addr),
+                kvec(r_addr, ell_addr), {}, K2{init_at_r_addr[0], init_at_r_addr[1]}, &hw, &ic, &wb);
+  });
+}
+
+extern "C" int ns_ts_twist_write_new(const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_write,
+                                     ns_sparse_col wv, ns_sparse_col inc, const ns_sparse_col* wa_bits,
+                                     size_t ell_addr, const uint64_t* r_addr, const uint64_t init_at_r_addr[2],
+                                     ns_ts** out) {
+  return ts_make(out, nullptr, [&](TsOracleGpu& o) {
+    SparseCol g = to_col(has_write), a = to_col(wv), b = to_col(inc);
+    std::vector<SparseCol> wb = to_cols(wa_bits, ell_addr);
+    o.init_time(TSK_TWIST_WRITE, kvec(r_cycle, ell_n), ell_n, &g, &a, &b, wb, kvec(r_addr, ell_addr), {},
+                K2{init_at_r_addr[0], init_at_r_addr[1]}, &g, &b, &wb);
+  });
+}
+
+extern "C" int ns_ts_val_eval_new(size_t ell_n, const ns_sparse_col* wa_bits, size_t ell_addr,
+                                  ns_sparse_col has_write, ns_sparse_col inc, const uint64_t* r_addr,
+                                  const uint64_t* r_time, uint64_t claim[2], ns_ts** out) {
+  return ts_make(out, claim, [&](TsOracleGpu& o) {
+    SparseCol g = to_col(has_write), b = to_col(inc);
+    o.init_time(TSK_VAL_EVAL, kvec(r_time, ell_n), ell_n, &g, nullptr, &b, to_cols(wa_bits, ell_addr),
+                kvec(r_addr, ell_addr), {}, kzero(), nullptr, nullptr, nullptr);
+  });
+}
+
+extern "C" int ns_ts_total_inc_new(size_t ell_n, const ns_sparse_col* wa_bits, size_t ell_addr,
+                                   ns_sparse_col has_write, ns_sparse_col inc, const uint64_t* r_addr,
+                                   uint64_t claim[2], ns_ts** out) {
+  return ts_make(out, claim, [&](TsOracleGpu& o) {
+    SparseCol g = to_col(has_write), b = to_col(inc);
+    o.init_time(TSK_TOTAL_INC, {}, ell_n, &g, nullptr, &b, to_cols(wa_bits, ell_addr), kvec(r_addr, ell_addr), {},
+                kzero(), nullptr, nullptr, nullptr);
+  });
+}
+
+static void init_lists(const uint64_t* init_addrs, const uint64_t* init_vals, size_t ninit, std::vector<uint32_t>& a,
+                       std::vector<K2>& v) {
+  for (size_t i = 0; i < ninit; i++) {
+    if (init_addrs[i] > 0xFFFFFFFFull) throw Error(1, "init address exceeds 32 bits");
+    a.push_back((uint32_t)init_addrs[i]);
+    v.push_back(K2{init_vals[2 * i], init_vals[2 * i + 1]});
+  }
+}
+
+extern "C" int ns_ts_read_addr_new(const uint64_t* init_addrs, const uint64_t* init_vals, size_t ninit,
+                                   const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_read, ns_sparse_col rv,
+                                   const ns_sparse_col* ra_bits, ns_sparse_col has_write,
+                                   const ns_sparse_col* wa_bits, ns_sparse_col inc, size_t ell_addr, ns_ts** out) {
+  return ts_make(out, nullptr, [&](TsOracleGpu& o) {
+    std::vector<uint32_t> ia;
+    std::vector<K2> iv;
+    init_lists(init_addrs, init_vals, ninit, ia, iv);
+    SparseCol hr = to_col(has_read), r = to_col(rv), hw = to_col(has_write), ic = to_col(inc);
+    std::vector<SparseCol> rb = to_cols(ra_bits, ell_addr), wb = to_cols(wa_bits, ell_addr);
+    o.init_addr(TSK_READ_ADDR, ia, iv, kvec(r_cycle, ell_n), ell_n, &hr, &r, &rb, hw, ic, wb, ic, ell_addr);
+  });
+}
+
+extern "C" int ns_ts_write_addr_new(const uint64_t* init_addrs, const uint64_t* init_vals, size_t ninit,
+                                    const uint64_t* r_cycle, size_t ell_n, ns_sparse_col has_write, ns_sparse_col wv,
+                                    const ns_sparse_col* wa_bits, ns_sparse_col inc, size_t ell_addr, ns_ts** out) {
+  return ts_make(out, nullptr, [&](TsOracleGpu& o) {
+    std::vector<uint32_t> ia;
+    std::vector<K2> iv;
+    init_lists(init_addrs, init_vals, ninit, ia, iv);
+    SparseCol hw = to_col(has_write), w = to_col(wv), ic = to_col(inc);
+    std::vector<SparseCol> wb = to_cols(wa_bits, ell_addr);
+    o.init_addr(TSK_WRITE_ADDR, ia, iv, kvec(r_cycle, ell_n), ell_n, nullptr, nullptr, nullptr, hw, w, wb, ic,
+                ell_addr);
+  });
+}
+
+extern "C" int ns_ts_addr_lookup_new(const ns_sparse_col* addr_bits, size_t ell_addr, ns_sparse_col has_lookup,
+                                     const uint64_t* table, size_t table_len, const uint64_t* r_cycle, size_t ell_n,
+                                     uint64_t claim[2], ns_ts** out) {
+  return ts_make(out, claim, [&](TsOracleGpu& o) {
+    o.init_lookup(to_cols(addr_bits, ell_addr), to_col(has_lookup), kvec(table, table_len), kvec(r_cycle, ell_n),
+                  ell_n);
+  });
+}
+
+extern "C" int ns_ts_evals(ns_ts* o, const uint64_t* xs, size_t n, uint64_t* out) {
+  return guard([&] {
+    if (!o) throw Error(1, "null oracle");
+    std::vector<K2> x = kvec(xs, n), y(n);
+    o->o->evals(x.data(), n, y.data());
+    for (size_t i = 0; i < n; i++) {
+      out[2 * i] = y[i].c0;
+      out[2 * i + 1] = y[i].c1;
+    }
+  });
+}
+
+extern "C" int ns_ts_fold(ns_ts* o, const uint64_t r[2]) {
+  return guard([&] {
+    if (!o) throw Error(1, "null oracle");
+    o->o->fold(K2{r[0], r[1]});
+  });
+}
+
+extern "C" size_t ns_ts_num_rounds(const ns_ts* o) { return o ? o->o->num_rounds() : 0; }
+extern "C" size_t ns_ts_degree_bound(const ns_ts* o) { return o ? o->o->degree_bound() : 0; }
+extern "C" void ns_ts_free(ns_ts* o) { delete o; }

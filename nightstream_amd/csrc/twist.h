@@ -1,0 +1,99 @@
+// twist.h -- device-resident Twist/Shout round oracles (neo-memory/src/twist_oracle.rs).
+//
+// Time-lane oracles keep their sparse columns as one union support (sorted time indices) with
+// the columns' values aligned to it in HBM; a round evaluates every sibling pair once and a
+// fold writes the parents through a prefix-sum compaction.  Address-lane oracles replay memory
+// per address bucket with a stable radix sort by bucket and a K-valued prefix scan instead of
+// the reference's hash map.  Round polynomials leave the GPU as coefficient vectors.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "prover.h"
+
+namespace ns {
+
+struct SparseCol {
+  std::vector<uint32_t> idx;  // strictly increasing
+  std::vector<K2> val;        // nonzero
+};
+
+enum TsKind {
+  TSK_PRODUCT = 0,
+  TSK_SHOUT_VALUE,
+  TSK_INDEX_ADAPTER,
+  TSK_BITNESS,
+  TSK_TWIST_READ,
+  TSK_TWIST_WRITE,
+  TSK_VAL_EVAL,
+  TSK_TOTAL_INC,
+  TSK_READ_ADDR,
+  TSK_WRITE_ADDR,
+  TSK_ADDR_LOOKUP
+};
+
+constexpr int TS_MAX_COEF = 40;  // degree bound 3 + ell_addr <= 39
+
+class TsOracleGpu {
+ public:
+  TsOracleGpu();
+  ~TsOracleGpu();
+  TsOracleGpu(const TsOracleGpu&) = delete;
+  TsOracleGpu& operator=(const TsOracleGpu&) = delete;
+
+  // constructors (the claim, where the reference returns one, is the round-0 hypercube sum)
+  void init_product(const std::vector<std::vector<K2>>& factors, size_t degree_bound);
+  void init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t ell_n, const SparseCol* gate, const SparseCol* a,
+                 const SparseCol* b, const std::vector<SparseCol>& bits, const std::vector<K2>& r_addr,
+                 const std::vector<K2>& weights, K2 init_at_r_addr, const SparseCol* term_write,
+                 const SparseCol* term_inc, const std::vector<SparseCol>* term_bits);
+  void init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs, const std::vector<K2>& init_vals,
+                 const std::vector<K2>& r_cycle, size_t ell_n, const SparseCol* has_read, const SparseCol* rv,
+                 const std::vector<SparseCol>* ra_bits, const SparseCol& has_write, const SparseCol& wv_or_inc,
+                 const std::vector<SparseCol>& wa_bits, const SparseCol& inc, size_t ell_addr);
+  void init_lookup(const std::vector<SparseCol>& addr_bits, const SparseCol& has_lookup, const std::vector<K2>& table,
+                   const std::vector<K2>& r_cycle, size_t ell_n);
+
+  size_t num_rounds() const;
+  size_t degree_bound() const { return degree_bound_; }
+  void evals(const K2* xs, size_t n, K2* out);
+  void fold(K2 r);
+  K2 claim();
+
+ private:
+  void round_coeffs(std::vector<K2>& c);
+  void time_round(std::vector<K2>& c);
+  void time_fold(K2 r);
+  void addr_round(std::vector<K2>& c, bool final);
+  void product_round(std::vector<K2>& c);
+  K2 time_final();
+
+  TsKind kind_ = TSK_PRODUCT;
+  hipStream_t st_ = nullptr;
+  size_t degree_bound_ = 0, ell_n_ = 0, ell_addr_ = 0, bit_idx_ = 0;
+  K2 prefix_eq_ = K2{1, 0};
+  std::vector<K2> r_cycle_, r_addr_, chals_;
+  // time lanes: columns [gate, a, b, bits...] over the union support
+  size_t n_ = 0, cap_ = 1, len_ = 0, ncol_ = 0, nbits_ = 0;
+  int col_a_ = -1, col_b_ = -1;
+  DevBuf idx_[2], cols_[2], flags_, pos_, scan_tmp_, partials_, coef_dev_, consts_;
+  int cur_ = 0;
+  size_t scan_tmp_bytes_ = 0;
+  // Twist val_pre terms (sorted by time) and their exclusive prefix sums
+  K2 init_at_ = K2{0, 0};
+  size_t nterms_ = 0;
+  DevBuf term_t_, term_v_, term_p_;
+  std::vector<uint32_t> term_t_h_;
+  std::vector<K2> term_v_h_;
+  // product oracle
+  size_t nf_ = 0, flen_ = 0;
+  DevBuf fac_[2];
+  // address lanes
+  size_t nev_ = 0, ninit_tab_ = 0;
+  DevBuf ev_eq_, ev_hr_, ev_rv_, ev_hw_, ev_inc_, ev_ra_, ev_wa_, ev_rw_, ev_ww_, it_[2];
+  DevBuf keys_[2], vals_[2], dsort_, dscan_, sort_tmp_;
+  size_t sort_tmp_bytes_ = 0;
+};
+
+}  // namespace ns

@@ -1,0 +1,1109 @@
+// twist.hip -- Twist/Shout round oracles on the device (neo-memory/src/twist_oracle.rs).
+//
+// Time lanes (ShoutValue, IndexAdapter, LazyWeightedBitness, Twist read/write checks, ValEval,
+// TotalInc; twist_oracle.rs:200-1196): one union support of the sparse columns, a pair kernel
+// that builds each sibling pair's round polynomial as a product of linear factors and reduces
+// the coefficients, and a fold that writes parents through an exclusive-scan compaction
+// (sparse_time.rs:69-116).  Twist's Val_pre(t) at a pair's children is
+// init + sum_{u < block} inc_u + sum_{u in block} inc_u * LT_low(u, t): terms below the pair's
+// time block contribute exactly 1, terms above exactly 0 (twist_oracle.rs:328-342), so each
+// round touches every increment term once instead of once per pair.
+// Address lanes (twist_oracle.rs:1259-1660): per round, a stable radix sort of the write events
+// by address bucket and a K-valued inclusive scan replace the sequential hash-map replay; each
+// read queries its two sibling buckets by binary search.
+// Product / address-lookup (twist_oracle.rs:56-183, 1662-1736): dense factors, pair kernel and
+// in-place fold; the lookup's weight table is built by sorting lookups by address.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+#include "twist.h"
+
+namespace ns {
+
+constexpr int TB = 256;
+constexpr int TMAXBLK = 256;
+
+__device__ __forceinline__ K2 kinterp(K2 f0, K2 f1, K2 x) { return kadd(f0, kmul(ksub(f1, f0), x)); }
+__device__ __forceinline__ K2 eq1(K2 a, K2 b) {
+  return kadd(kmul(ksub(kone(), a), ksub(kone(), b)), kmul(a, b));
+}
+// bit_ops.rs:5-9
+__device__ __forceinline__ K2 eq_bit_affine(K2 bit, K2 u) {
+  return kadd(kmul(bit, ksub(kadd(u, u), kone())), ksub(kone(), u));
+}
+
+template <int MAXC>
+__device__ __forceinline__ void poly_mul_lin(K2 (&p)[MAXC], int& deg, K2 a, K2 b) {
+  // p <- p * (a + b x)
+#pragma unroll
+  for (int d = MAXC - 1; d >= 1; d--)
+    if (d <= deg + 1) p[d] = kadd(kmul(p[d], a), kmul(p[d - 1], b));
+  p[0] = kmul(p[0], a);
+  deg++;
+}
+
+__device__ __forceinline__ K2 wave_sum(K2 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    K2 o;
+    o.c0 = __shfl_down(v.c0, off, 64);
+    o.c1 = __shfl_down(v.c1, off, 64);
+    v = kadd(v, o);
+  }
+  return v;
+}
+
+// Block-reduce `ncoef` coefficients into partials[blockIdx.x * ncoef + d].
+template <int MAXC>
+__device__ void block_store(K2 (&acc)[MAXC], int ncoef, K2* partials) {
+  __shared__ K2 red[TB / 64][MAXC];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 0; d < MAXC; d++) {
+    if (d < ncoef) {
+      K2 v = wave_sum(acc[d]);
+      if (lane == 0) red[wave][d] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ncoef) {
+    K2 v = kzero();
+    for (int w = 0; w < TB / 64; w++) v = kadd(v, red[w][threadIdx.x]);
+    partials[blockIdx.x * ncoef + threadIdx.x] = v;
+  }
+}
+
+__global__ void k_ts_sum(const K2* __restrict__ partials, int nblk, int ncoef, K2* __restrict__ out) {
+  const int d = blockIdx.x;
+  K2 v = kzero();
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = kadd(v, partials[b * ncoef + d]);
+  __shared__ K2 red[4];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[d] = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
+}
+
+// ---------------- time lanes ----------------
+struct TsTimeArgs {
+  int kind;
+  uint32_t n, stride;
+  const uint32_t* idx;
+  const K2* cols;  // [col][stride]: 0 gate, col_a, col_b, then nbits bit columns at bit0
+  int col_a, col_b, bit0, nbits;
+  const K2* consts;  // r_cycle[ell] | chals[ell] | r_addr[nbits] | weights[nbits]
+  int ell, bit_idx;
+  K2 prefix_eq;
+  K2 init_at;
+  const uint32_t* term_t;
+  const K2* term_v;
+  const K2* term_p;
+  uint32_t nterms;
+  int ncoef;
+};
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint64_t key) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// LT(u, t) restricted to positions b..0 (t_b = c Boolean, t_i = chals[i] below):
+// lt_eval_at_bool_index (twist_oracle.rs:328-342) once every higher bit matched.
+__device__ K2 lt_low(uint32_t u, int b, int c, const K2* chals) {
+  K2 suffix = kone(), acc = kzero();
+  if (((u >> b) & 1) == 0) {
+    if (c) acc = suffix;
+    suffix = c ? kzero() : suffix;
+  } else {
+    suffix = c ? suffix : kzero();
+  }
+  for (int i = b - 1; i >= 0; i--) {
+    K2 t = chals[i];
+    if (((u >> i) & 1) == 0) {
+      acc = kadd(acc, kmul(t, suffix));
+      suffix = kmul(suffix, ksub(kone(), t));
+    } else {
+      suffix = kmul(suffix, t);
+    }
+  }
+  return acc;
+}
+
+__device__ K2 val_pre_child(const TsTimeArgs& A, uint64_t pair, int c, const K2* chals) {
+  const int b = A.bit_idx;
+  uint64_t lo_key = pair << (b + 1), hi_key = (pair + 1) << (b + 1);
+  uint32_t lo = lower_bound_u32(A.term_t, 0, A.nterms, lo_key);
+  uint32_t hi = lower_bound_u32(A.term_t, lo, A.nterms, hi_key);
+  K2 v = kadd(A.init_at, A.term_p[lo]);
+  for (uint32_t k = lo; k < hi; k++) v = kadd(v, kmul(A.term_v[k], lt_low(A.term_t[k], b, c, chals)));
+  return v;
+}
+
+// lt_eval (mle.rs:7-24) at the child time point (chals | c | bits of pair) against r_time
+__device__ K2 lt_child(const TsTimeArgs& A, uint64_t pair, int c, const K2* r_time, const K2* chals) {
+  K2 suffix = kone(), acc = kzero();
+  for (int i = A.ell - 1; i >= 0; i--) {
+    K2 jp;
+    if (i < A.bit_idx) jp = chals[i];
+    else if (i == A.bit_idx) jp = c ? kone() : kzero();
+    else jp = ((pair >> (i - A.bit_idx - 1)) & 1) ? kone() : kzero();
+    K2 j = r_time[i];
+    acc = kadd(acc, kmul(kmul(ksub(kone(), jp), j), suffix));
+    suffix = kmul(suffix, eq1(jp, j));
+  }
+  return acc;
+}
+
+template <int MAXC>
+__global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restrict__ partials) {
+  const K2* r_cycle = A.consts;
+  const K2* chals = A.consts + A.ell;
+  const K2* r_addr = A.consts + 2 * A.ell;
+  const K2* weights = r_addr + A.nbits;
+  K2 acc[MAXC];
+#pragma unroll
+  for (int d = 0; d < MAXC; d++) acc[d] = kzero();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+    const uint32_t id = A.idx[i];
+    const uint64_t pair = id >> 1;
+    if (i > 0 && (A.idx[i - 1] >> 1) == pair) continue;  // the even sibling owns the pair
+    uint32_t i0 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu;
+    if ((id & 1) == 0) {
+      i0 = i;
+      if (i + 1 < A.n && A.idx[i + 1] == id + 1) i1 = i + 1;
+    } else {
+      i1 = i;
+    }
+#define COLV(c, which) ((which) != 0xFFFFFFFFu ? A.cols[(size_t)(c) * A.stride + (which)] : kzero())
+    // chi_cycle_children (twist_oracle.rs:185-198)
+    K2 chi0 = kone(), chi1 = kone();
+    const bool has_chi = A.kind != TSK_VAL_EVAL && A.kind != TSK_TOTAL_INC;
+    if (has_chi) {
+      K2 suffix = kone();
+      int shift = 0;
+      for (int b = A.bit_idx + 1; b < A.ell; b++, shift++)
+        suffix = kmul(suffix, ((pair >> shift) & 1) ? r_cycle[b] : ksub(kone(), r_cycle[b]));
+      K2 r = r_cycle[A.bit_idx];
+      chi0 = kmul(kmul(A.prefix_eq, ksub(kone(), r)), suffix);
+      chi1 = kmul(kmul(A.prefix_eq, r), suffix);
+    }
+    K2 p[MAXC];
+#pragma unroll
+    for (int d = 0; d < MAXC; d++) p[d] = kzero();
+    p[0] = kone();
+    int deg = 0;
+    if (A.kind == TSK_BITNESS) {
+      // chi(x) * sum_c w_c b_c(x) (b_c(x) - 1), b_c(x) = b0 + db x
+      K2 q0 = kzero(), q1 = kzero(), q2 = kzero();
+      for (int c = 0; c < A.nbits; c++) {
+        K2 b0 = COLV(A.bit0 + c, i0), b1 = COLV(A.bit0 + c, i1);
+        if (kiszero(b0) && kiszero(b1)) continue;
+        K2 db = ksub(b1, b0), w = weights[c];
+        K2 b0m1 = ksub(b0, kone());
+        q0 = kadd(q0, kmul(w, kmul(b0, b0m1)));
+        q1 = kadd(q1, kmul(w, kadd(kmul(b0, db), kmul(db, b0m1))));
+        q2 = kadd(q2, kmul(w, kmul(db, db)));
+      }
+      K2 dc = ksub(chi1, chi0);
+      p[0] = kmul(chi0, q0);
+      p[1] = kadd(kmul(chi0, q1), kmul(dc, q0));
+      p[2] = kadd(kmul(chi0, q2), kmul(dc, q1));
+      p[3] = kmul(dc, q2);
+    } else {
+      K2 g0 = COLV(0, i0), g1 = COLV(0, i1);
+      if (kiszero(g0) && kiszero(g1)) continue;
+      poly_mul_lin<MAXC>(p, deg, g0, ksub(g1, g0));
+      if (has_chi) poly_mul_lin<MAXC>(p, deg, chi0, ksub(chi1, chi0));
+      K2 f0 = kzero(), f1 = kzero();
+      bool has_f = true;
+      switch (A.kind) {
+        case TSK_SHOUT_VALUE:
+          f0 = COLV(A.col_a, i0);
+          f1 = COLV(A.col_a, i1);
+          break;
+        case TSK_INDEX_ADAPTER:
+          has_f = false;
+          break;
+        case TSK_TWIST_READ: {
+          K2 vp0 = val_pre_child(A, pair, 0, chals), vp1 = val_pre_child(A, pair, 1, chals);
+          f0 = ksub(vp0, COLV(A.col_a, i0));
+          f1 = ksub(vp1, COLV(A.col_a, i1));
+          break;
+        }
+        case TSK_TWIST_WRITE: {
+          K2 vp0 = val_pre_child(A, pair, 0, chals), vp1 = val_pre_child(A, pair, 1, chals);
+          f0 = ksub(ksub(COLV(A.col_a, i0), vp0), COLV(A.col_b, i0));
+          f1 = ksub(ksub(COLV(A.col_a, i1), vp1), COLV(A.col_b, i1));
+          break;
+        }
+        case TSK_VAL_EVAL: {
+          f0 = COLV(A.col_b, i0);
+          f1 = COLV(A.col_b, i1);
+          K2 h0 = lt_child(A, pair, 0, r_cycle, chals), h1 = lt_child(A, pair, 1, r_cycle, chals);
+          poly_mul_lin<MAXC>(p, deg, h0, ksub(h1, h0));
+          break;
+        }
+        case TSK_TOTAL_INC:
+          f0 = COLV(A.col_b, i0);
+          f1 = COLV(A.col_b, i1);
+          break;
+        default:
+          break;
+      }
+      if (has_f) poly_mul_lin<MAXC>(p, deg, f0, ksub(f1, f0));
+      for (int b = 0; b < A.nbits; b++) {
+        K2 e0 = eq_bit_affine(COLV(A.bit0 + b, i0), r_addr[b]);
+        K2 e1 = eq_bit_affine(COLV(A.bit0 + b, i1), r_addr[b]);
+        poly_mul_lin<MAXC>(p, deg, e0, ksub(e1, e0));
+      }
+    }
+#undef COLV
+#pragma unroll
+    for (int d = 0; d < MAXC; d++) acc[d] = kadd(acc[d], p[d]);
+  }
+  block_store<MAXC>(acc, A.ncoef, partials);
+}
+
+__global__ void k_ts_heads(const uint32_t* __restrict__ idx, uint32_t n, uint32_t* __restrict__ flags) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = (i == 0 || (idx[i - 1] >> 1) != (idx[i] >> 1)) ? 1u : 0u;
+}
+
+__global__ void k_ts_fold(const uint32_t* __restrict__ idx, const K2* __restrict__ cols, uint32_t n, uint32_t stride,
+                          int ncol, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, K2 r,
+                          uint32_t* __restrict__ nidx, K2* __restrict__ ncols, uint32_t* __restrict__ nout) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i == n - 1) nout[0] = pos[i] + flags[i];
+  if (!flags[i]) return;
+  const uint32_t id = idx[i];
+  uint32_t i0 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu;
+  if ((id & 1) == 0) {
+    i0 = i;
+    if (i + 1 < n && idx[i + 1] == id + 1) i1 = i + 1;
+  } else {
+    i1 = i;
+  }
+  const uint32_t p = pos[i];
+  nidx[p] = id >> 1;
+  for (int c = 0; c < ncol; c++) {
+    K2 v0 = i0 != 0xFFFFFFFFu ? cols[(size_t)c * stride + i0] : kzero();
+    K2 v1 = i1 != 0xFFFFFFFFu ? cols[(size_t)c * stride + i1] : kzero();
+    ncols[(size_t)c * stride + p] = kinterp(v0, v1, r);
+  }
+}
+
+// sum_k term_v[k] * lt_eval_at_bool_index(term_t[k], chals) (the fully bound Val_pre)
+__global__ __launch_bounds__(TB) void k_ts_valpre_full(const uint32_t* __restrict__ tt, const K2* __restrict__ tv,
+                                                       uint32_t nterms, const K2* __restrict__ chals, int ell,
+                                                       K2* __restrict__ partials) {
+  K2 acc[1] = {kzero()};
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nterms; k += gridDim.x * blockDim.x) {
+    uint32_t u = tt[k];
+    K2 suffix = kone(), lt = kzero();
+    for (int i = ell - 1; i >= 0; i--) {
+      K2 t = chals[i];
+      if (((u >> i) & 1) == 0) {
+        lt = kadd(lt, kmul(t, suffix));
+        suffix = kmul(suffix, ksub(kone(), t));
+      } else {
+        suffix = kmul(suffix, t);
+      }
+    }
+    acc[0] = kadd(acc[0], kmul(tv[k], lt));
+  }
+  block_store<1>(acc, 1, partials);
+}
+
+// chi_at_index(r, t) * scale[i] for a list of times (scale may be null)
+__global__ void k_ts_chi_at(const uint32_t* __restrict__ ts, uint32_t n, const K2* __restrict__ r, int ell,
+                            const K2* __restrict__ scale, K2* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t t = ts[i];
+  K2 acc = kone();
+  for (int b = 0; b < ell; b++) acc = kmul(acc, ((t >> b) & 1) ? r[b] : ksub(kone(), r[b]));
+  out[i] = scale ? kmul(acc, scale[i]) : acc;
+}
+
+// ---------------- product / lookup ----------------
+template <int MAXC>
+__global__ __launch_bounds__(TB) void k_prod_round(const K2* __restrict__ fac, int nf, size_t stride, size_t half,
+                                                   K2* __restrict__ partials) {
+  K2 acc[MAXC];
+#pragma unroll
+  for (int d = 0; d < MAXC; d++) acc[d] = kzero();
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < half; i += (size_t)gridDim.x * blockDim.x) {
+    K2 p[MAXC];
+#pragma unroll
+    for (int d = 0; d < MAXC; d++) p[d] = kzero();
+    p[0] = kone();
+    int deg = 0;
+    for (int f = 0; f < nf; f++) {
+      K2 a = fac[f * stride + 2 * i], b = fac[f * stride + 2 * i + 1];
+      poly_mul_lin<MAXC>(p, deg, a, ksub(b, a));
+    }
+#pragma unroll
+    for (int d = 0; d < MAXC; d++) acc[d] = kadd(acc[d], p[d]);
+  }
+  block_store<MAXC>(acc, nf + 1, partials);
+}
+
+__global__ void k_prod_fold(const K2* __restrict__ src, K2* __restrict__ dst, int nf, size_t stride, size_t half, K2 r) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  int f = blockIdx.y;
+  if (i >= half || f >= nf) return;
+  dst[f * stride + i] = kinterp(src[f * stride + 2 * i], src[f * stride + 2 * i + 1], r);
+}
+
+// W[a] = sum of weights of the lookups at address a (lookups sorted by address)
+__global__ void k_lookup_weights(const uint32_t* __restrict__ saddr, const uint32_t* __restrict__ sidx, uint32_t n,
+                                 const K2* __restrict__ w, uint32_t pow2_addr, K2* __restrict__ W) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= pow2_addr) return;
+  uint32_t lo = lower_bound_u32(saddr, 0, n, a), hi = lower_bound_u32(saddr, lo, n, (uint64_t)a + 1);
+  K2 acc = kzero();
+  for (uint32_t j = lo; j < hi; j++) acc = kadd(acc, w[sidx[j]]);
+  W[a] = acc;
+}
+
+// ---------------- address lanes ----------------
+struct KAddOp {
+  __device__ __forceinline__ K2 operator()(const K2& a, const K2& b) const { return kadd(a, b); }
+};
+
+__global__ void k_addr_keys(const K2* __restrict__ hw, const K2* __restrict__ inc, const K2* __restrict__ ww,
+                            const uint32_t* __restrict__ wa, uint32_t nev, int bit, uint32_t sentinel,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, K2* __restrict__ dval) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nev) return;
+  bool w = !kiszero(hw[e]);
+  keys[e] = w ? (bit >= 0 ? (wa[e] >> bit) : 0u) : sentinel;
+  vals[e] = e;
+  dval[e] = w ? kmul(kmul(inc[e], hw[e]), ww[e]) : kzero();
+}
+
+__global__ void k_addr_gather(const uint32_t* __restrict__ svals, const K2* __restrict__ dval, uint32_t nev,
+                              K2* __restrict__ out) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nev) return;
+  out[j] = dval[svals[j]];
+}
+
+// mem value of bucket k just before event e: it[k] + sum of bucket-k write deltas of earlier events
+__device__ K2 bucket_before(const uint32_t* skeys, const uint32_t* svals, const K2* scan, uint32_t nev, uint32_t k,
+                            uint32_t e, const K2* it) {
+  uint32_t lo = lower_bound_u32(skeys, 0, nev, k), hi = lower_bound_u32(skeys, lo, nev, (uint64_t)k + 1);
+  uint32_t j = lower_bound_u32(svals, lo, hi, e);  // first event >= e in the bucket
+  K2 s = kzero();
+  if (j > lo) s = ksub(scan[j - 1], lo > 0 ? scan[lo - 1] : kzero());
+  return kadd(it[k], s);
+}
+
+struct AddrArgs {
+  uint32_t nev;
+  int is_read, bit, final_;
+  const K2 *eq, *hr, *rv, *hw, *inc, *rw, *ww;  // rv holds wv for the write check
+  const uint32_t *ra, *wa;
+  const uint32_t *skeys, *svals;
+  const K2* scan;
+  const K2* it;
+};
+
+__global__ __launch_bounds__(TB) void k_addr_round(AddrArgs A, K2* __restrict__ partials) {
+  K2 acc[3] = {kzero(), kzero(), kzero()};
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < A.nev; e += gridDim.x * blockDim.x) {
+    K2 gate = A.is_read ? A.hr[e] : A.hw[e];
+    if (kiszero(gate)) continue;
+    uint32_t addr = A.is_read ? A.ra[e] : A.wa[e];
+    K2 c = kmul(kmul(A.eq[e], gate), A.is_read ? A.rw[e] : A.ww[e]);
+    if (A.final_) {
+      K2 mem = bucket_before(A.skeys, A.svals, A.scan, A.nev, 0, e, A.it);
+      K2 diff = A.is_read ? ksub(mem, A.rv[e]) : ksub(ksub(A.rv[e], mem), A.inc[e]);
+      acc[0] = kadd(acc[0], kmul(c, diff));
+      continue;
+    }
+    uint32_t k0 = (addr >> (A.bit + 1)) << 1;
+    K2 v0 = bucket_before(A.skeys, A.svals, A.scan, A.nev, k0, e, A.it);
+    K2 v1 = bucket_before(A.skeys, A.svals, A.scan, A.nev, k0 + 1, e, A.it);
+    K2 dv = ksub(v1, v0);
+    // read: (v0 - rv) + dv x ; write: (wv - inc - v0) - dv x
+    K2 base = A.is_read ? ksub(v0, A.rv[e]) : ksub(ksub(A.rv[e], A.inc[e]), v0);
+    K2 slope = A.is_read ? dv : kneg(dv);
+    if ((addr >> A.bit) & 1) {  // x * (base + slope x)
+      acc[1] = kadd(acc[1], kmul(c, base));
+      acc[2] = kadd(acc[2], kmul(c, slope));
+    } else {  // (1 - x) * (base + slope x)
+      acc[0] = kadd(acc[0], kmul(c, base));
+      acc[1] = kadd(acc[1], kmul(c, ksub(slope, base)));
+      acc[2] = ksub(acc[2], kmul(c, slope));
+    }
+  }
+  block_store<3>(acc, 3, partials);
+}
+
+__global__ void k_addr_fold_w(K2* __restrict__ w, const uint32_t* __restrict__ addr, uint32_t n, int bit, K2 r) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  w[e] = kmul(w[e], ((addr[e] >> bit) & 1) ? r : ksub(kone(), r));
+}
+
+// ======================= host =======================
+static unsigned nblocks(size_t n, size_t cap = TMAXBLK) {
+  size_t b = (n + TB - 1) / TB;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+template <class T>
+static void upload(DevBuf& d, const std::vector<T>& h, hipStream_t st, size_t min_elems = 1) {
+  size_t n = std::max(h.size(), min_elems);
+  d.ensure(n * sizeof(T));
+  if (!h.empty()) NS_CHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+TsOracleGpu::TsOracleGpu() {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw HipError(hipErrorNoDevice, "no device", __FILE__, __LINE__);
+  NS_CHECK(hipStreamCreate(&st_));
+}
+
+TsOracleGpu::~TsOracleGpu() {
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+static SparseCol empty_col() { return SparseCol{}; }
+
+void TsOracleGpu::init_product(const std::vector<std::vector<K2>>& factors, size_t degree_bound) {
+  kind_ = TSK_PRODUCT;
+  nf_ = factors.size();
+  if (nf_ == 0 || nf_ + 1 > (size_t)TS_MAX_COEF) throw Error(1, "product oracle: 1..39 factors required");
+  flen_ = factors[0].size();
+  if (flen_ == 0 || (flen_ & (flen_ - 1))) throw Error(1, "product oracle: factor length must be a power of two");
+  std::vector<K2> flat(nf_ * flen_);
+  for (size_t f = 0; f < nf_; f++) {
+    if (factors[f].size() != flen_) throw Error(1, "product oracle: all factors must have the same length");
+    std::copy(factors[f].begin(), factors[f].end(), flat.begin() + f * flen_);
+  }
+  upload(fac_[0], flat, st_);
+  fac_[1].ensure(flat.size() * sizeof(K2));
+  cur_ = 0;
+  degree_bound_ = degree_bound;
+  size_t l = 0;
+  while (((size_t)1 << l) < flen_) l++;
+  ell_n_ = l;
+  len_ = flen_;
+}
+
+// union support of the columns, values aligned; columns: gate, [a], [b], bits...
+void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t ell_n, const SparseCol* gate,
+                            const SparseCol* a, const SparseCol* b, const std::vector<SparseCol>& bits,
+                            const std::vector<K2>& r_addr, const std::vector<K2>& weights, K2 init_at_r_addr,
+                            const SparseCol* term_write, const SparseCol* term_inc,
+                            const std::vector<SparseCol>* term_bits) {
+  kind_ = kind;
+  if (ell_n < 1 || ell_n > 30) throw Error(1, "time-lane oracle: 1 <= ell_n <= 30");
+  ell_n_ = ell_n;
+  len_ = (size_t)1 << ell_n;
+  r_cycle_ = r_cycle;
+  r_cycle_.resize(ell_n, kzero());
+  r_addr_ = r_addr;
+  nbits_ = bits.size();
+  if (kind != TSK_BITNESS && r_addr_.size() != nbits_) throw Error(1, "time-lane oracle: r_addr length != bit columns");
+  std::vector<const SparseCol*> cs;
+  SparseCol none = empty_col();
+  cs.push_back(gate ? gate : &none);
+  col_a_ = col_b_ = -1;
+  if (a) {
+    col_a_ = (int)cs.size();
+    cs.push_back(a);
+  }
+  if (b) {
+    col_b_ = (int)cs.size();
+    cs.push_back(b);
+  }
+  for (auto& c : bits) cs.push_back(&c);
+  ncol_ = cs.size();
+  std::vector<uint32_t> u;
+  for (auto* c : cs) {
+    if (c->idx.size() != c->val.size()) throw Error(1, "sparse column: idx/val length mismatch");
+    for (auto t : c->idx) {
+      if (t >= len_) throw Error(1, "sparse column index out of range");
+      u.push_back(t);
+    }
+  }
+  std::sort(u.begin(), u.end());
+  u.erase(std::unique(u.begin(), u.end()), u.end());
+  n_ = u.size();
+  cap_ = std::max<size_t>(n_, 1);
+  std::vector<K2> vals(ncol_ * cap_, kzero());
+  for (size_t c = 0; c < ncol_; c++) {
+    const SparseCol* col = cs[c];
+    for (size_t j = 0; j < col->idx.size(); j++) {
+      if (j && col->idx[j] <= col->idx[j - 1]) throw Error(1, "sparse column indices must be strictly increasing");
+      size_t p = std::lower_bound(u.begin(), u.end(), col->idx[j]) - u.begin();
+      vals[c * cap_ + p] = col->val[j];
+    }
+  }
+  upload(idx_[0], u, st_);
+  upload(cols_[0], vals, st_);
+  idx_[1].ensure(std::max<size_t>(n_, 1) * 4);
+  cols_[1].ensure(vals.size() * sizeof(K2));
+  flags_.ensure(std::max<size_t>(n_, 1) * 4 + 8);
+  pos_.ensure(std::max<size_t>(n_, 1) * 4 + 8);
+  size_t tmp = 0;
+  NS_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)std::max<size_t>(n_, 1), st_));
+  scan_tmp_.ensure(tmp + 16);
+  scan_tmp_bytes_ = tmp;
+  cur_ = 0;
+  // constants: r_cycle | chals | r_addr | weights
+  consts_.ensure((2 * ell_n + 2 * std::max(nbits_, weights.size()) + 4) * sizeof(K2));
+  std::vector<K2> cst(r_cycle_);
+  cst.resize(2 * ell_n, kzero());
+  if (kind == TSK_BITNESS) {
+    cst.resize(2 * ell_n + nbits_, kzero());
+    cst.insert(cst.end(), weights.begin(), weights.end());
+  } else {
+    cst.insert(cst.end(), r_addr_.begin(), r_addr_.end());
+  }
+  NS_CHECK(hipMemcpyAsync(consts_.p, cst.data(), cst.size() * sizeof(K2), hipMemcpyHostToDevice, st_));
+  switch (kind) {
+    case TSK_SHOUT_VALUE: degree_bound_ = 3; break;
+    case TSK_INDEX_ADAPTER: degree_bound_ = 2 + nbits_; break;
+    case TSK_BITNESS: degree_bound_ = 3; break;
+    case TSK_TWIST_READ:
+    case TSK_TWIST_WRITE:
+    case TSK_VAL_EVAL: degree_bound_ = 3 + nbits_; break;
+    case TSK_TOTAL_INC: degree_bound_ = 2 + nbits_; break;
+    default: throw Error(1, "not a time-lane oracle kind");
+  }
+  if (degree_bound_ + 1 > (size_t)TS_MAX_COEF) throw Error(1, "time-lane oracle: too many address bits");
+  partials_.ensure((size_t)TMAXBLK * TS_MAX_COEF * sizeof(K2));
+  coef_dev_.ensure(TS_MAX_COEF * sizeof(K2));
+  // Twist Val_pre increment terms at r_addr (twist_oracle.rs:344-371), sorted by time
+  init_at_ = init_at_r_addr;
+  nterms_ = 0;
+  if (kind == TSK_TWIST_READ || kind == TSK_TWIST_WRITE) {
+    const SparseCol& hw = *term_write;
+    const SparseCol& ic = *term_inc;
+    const std::vector<SparseCol>& wb = *term_bits;
+    if (wb.size() != r_addr_.size()) throw Error(1, "twist: wa_bits length != r_addr length");
+    auto get = [](const SparseCol& c, uint32_t t) {
+      auto it = std::lower_bound(c.idx.begin(), c.idx.end(), t);
+      return (it != c.idx.end() && *it == t) ? c.val[it - c.idx.begin()] : kzero();
+    };
+    term_t_h_.clear();
+    term_v_h_.clear();
+    for (size_t i = 0; i < hw.idx.size(); i++) {
+      uint32_t t = hw.idx[i];
+      K2 h = hw.val[i], it = get(ic, t);
+      if (kiszero(h) || kiszero(it)) continue;
+      K2 eq = kone();
+      for (size_t bb = 0; bb < wb.size(); bb++) {
+        K2 bit = get(wb[bb], t), uu = r_addr_[bb];
+        eq = kmul(eq, kadd(kmul(bit, ksub(kadd(uu, uu), kone())), ksub(kone(), uu)));
+      }
+      K2 v = kmul(kmul(h, it), eq);
+      if (!kiszero(v)) {
+        term_t_h_.push_back(t);
+        term_v_h_.push_back(v);
+      }
+    }
+    nterms_ = term_t_h_.size();
+    std::vector<K2> pre(nterms_ + 1, kzero());
+    for (size_t k = 0; k < nterms_; k++) pre[k + 1] = kadd(pre[k], term_v_h_[k]);
+    upload(term_t_, term_t_h_, st_);
+    upload(term_v_, term_v_h_, st_);
+    upload(term_p_, pre, st_);
+  }
+  NS_CHECK(hipStreamSynchronize(st_));
+}
+
+size_t TsOracleGpu::num_rounds() const {
+  switch (kind_) {
+    case TSK_PRODUCT:
+    case TSK_ADDR_LOOKUP: {
+      size_t l = 0;
+      while (((size_t)1 << l) < len_) l++;
+      return l;
+    }
+    case TSK_TOTAL_INC: {
+      size_t l = 0;
+      while (((size_t)1 << l) < len_) l++;
+      return l;
+    }
+    case TSK_READ_ADDR:
+    case TSK_WRITE_ADDR:
+      return ell_addr_ > bit_idx_ ? ell_addr_ - bit_idx_ : 0;
+    default:
+      return ell_n_ > bit_idx_ ? ell_n_ - bit_idx_ : 0;
+  }
+}
+
+void TsOracleGpu::time_round(std::vector<K2>& c) {
+  const int ncoef = (int)degree_bound_ + 1;
+  c.assign(ncoef, kzero());
+  if (n_ == 0) return;
+  // chals into the constants block
+  if (!chals_.empty())
+    NS_CHECK(hipMemcpyAsync(consts_.as<K2>() + ell_n_, chals_.data(), chals_.size() * sizeof(K2),
+                            hipMemcpyHostToDevice, st_));
+  TsTimeArgs A;
+  A.kind = kind_;
+  A.n = (uint32_t)n_;
+  A.stride = (uint32_t)cap_;
+  A.idx = idx_[cur_].as<uint32_t>();
+  A.cols = cols_[cur_].as<K2>();
+  A.col_a = col_a_;
+  A.col_b = col_b_;
+  A.bit0 = (int)(ncol_ - nbits_);
+  A.nbits = (int)nbits_;
+  A.consts = consts_.as<K2>();
+  A.ell = (int)ell_n_;
+  A.bit_idx = (int)bit_idx_;
+  A.prefix_eq = prefix_eq_;
+  A.init_at = init_at_;
+  A.term_t = term_t_.as<uint32_t>();
+  A.term_v = term_v_.as<K2>();
+  A.term_p = term_p_.as<K2>();
+  A.nterms = (uint32_t)nterms_;
+  A.ncoef = ncoef;
+  unsigned nb = nblocks(n_);
+  if (ncoef <= 8)
+    hipLaunchKernelGGL(k_ts_time_round<8>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
+  else if (ncoef <= 16)
+    hipLaunchKernelGGL(k_ts_time_round<16>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
+  else if (ncoef <= 24)
+    hipLaunchKernelGGL(k_ts_time_round<24>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
+  else
+    hipLaunchKernelGGL(k_ts_time_round<TS_MAX_COEF>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
+  hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_dev_.as<K2>());
+  NS_CHECK(hipMemcpyAsync(c.data(), coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  NS_CHECK(hipStreamSynchronize(st_));
+}
+
+void TsOracleGpu::time_fold(K2 r) {
+  if (n_ > 0) {
+    const uint32_t n = (uint32_t)n_;
+    const uint32_t stride = (uint32_t)cap_;
+    unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_ts_heads, dim3(g), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(), n, flags_.as<uint32_t>());
+    size_t tmp = 0;
+    NS_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flags_.as<uint32_t>(), pos_.as<uint32_t>(), (int)n, st_));
+    if (tmp > scan_tmp_.bytes) throw Error(1, "scan workspace too small");
+    NS_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp_.p, tmp, flags_.as<uint32_t>(), pos_.as<uint32_t>(), (int)n, st_));
+    uint32_t* nout = pos_.as<uint32_t>() + n;
+    hipLaunchKernelGGL(k_ts_fold, dim3(g), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(), cols_[cur_].as<K2>(), n,
+                       stride, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(), r, idx_[cur_ ^ 1].as<uint32_t>(),
+                       cols_[cur_ ^ 1].as<K2>(), nout);
+    uint32_t nn = 0;
+    NS_CHECK(hipMemcpyAsync(&nn, nout, 4, hipMemcpyDeviceToHost, st_));
+    NS_CHECK(hipStreamSynchronize(st_));
+    n_ = nn;
+    cur_ ^= 1;
+  }
+  len_ >>= 1;
+}
+
+// value of a fully folded time-lane oracle (twist_oracle.rs singleton branches)
+K2 TsOracleGpu::time_final() {
+  std::vector<K2> v(ncol_, kzero());
+  if (n_ > 0) {
+    const uint32_t stride = (uint32_t)cap_;
+    for (size_t c = 0; c < ncol_; c++)
+      NS_CHECK(hipMemcpyAsync(&v[c], cols_[cur_].as<K2>() + c * stride, sizeof(K2), hipMemcpyDeviceToHost, st_));
+    NS_CHECK(hipStreamSynchronize(st_));
+  }
+  const size_t bit0 = ncol_ - nbits_;
+  if (kind_ == TSK_BITNESS) {
+    // weights live in the constants block after r_cycle | chals | (nbits slots)
+    std::vector<K2> w(nbits_);
+    if (nbits_)
+      NS_CHECK(hipMemcpy(w.data(), consts_.as<K2>() + 2 * ell_n_ + nbits_, nbits_ * sizeof(K2), hipMemcpyDeviceToHost));
+    K2 acc = kzero();
+    for (size_t c = 0; c < nbits_; c++) {
+      K2 b = v[bit0 + c];
+      acc = kadd(acc, kmul(kmul(w[c], b), ksub(b, kone())));
+    }
+    return kmul(prefix_eq_, acc);
+  }
+  K2 eq = kone();
+  for (size_t b = 0; b < nbits_; b++) {
+    K2 bit = v[bit0 + b], u = r_addr_[b];
+    eq = kmul(eq, kadd(kmul(bit, ksub(kadd(u, u), kone())), ksub(kone(), u)));
+  }
+  K2 g = v[0];
+  auto colv = [&](int c) { return c >= 0 ? v[c] : kzero(); };
+  switch (kind_) {
+    case TSK_SHOUT_VALUE: return kmul(kmul(prefix_eq_, g), colv(col_a_));
+    case TSK_INDEX_ADAPTER: return kmul(kmul(prefix_eq_, g), eq);
+    case TSK_TWIST_READ:
+    case TSK_TWIST_WRITE: {
+      K2 vp = init_at_;
+      if (nterms_) {
+        NS_CHECK(hipMemcpyAsync(consts_.as<K2>() + ell_n_, chals_.data(), chals_.size() * sizeof(K2),
+                                hipMemcpyHostToDevice, st_));
+        unsigned nb = nblocks(nterms_);
+        hipLaunchKernelGGL(k_ts_valpre_full, dim3(nb), dim3(TB), 0, st_, term_t_.as<uint32_t>(), term_v_.as<K2>(),
+                           (uint32_t)nterms_, consts_.as<K2>() + ell_n_, (int)ell_n_, partials_.as<K2>());
+        hipLaunchKernelGGL(k_ts_sum, dim3(1), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, 1, coef_dev_.as<K2>());
+        K2 s;
+        NS_CHECK(hipMemcpyAsync(&s, coef_dev_.p, sizeof(K2), hipMemcpyDeviceToHost, st_));
+        NS_CHECK(hipStreamSynchronize(st_));
+        vp = kadd(vp, s);
+      }
+      if (kind_ == TSK_TWIST_READ) return kmul(kmul(kmul(prefix_eq_, g), ksub(vp, colv(col_a_))), eq);
+      return kmul(kmul(kmul(prefix_eq_, g), ksub(ksub(colv(col_a_), vp), colv(col_b_))), eq);
+    }
+    case TSK_VAL_EVAL: {
+      // lt_eval(challenges, r_time), mle.rs:7-24
+      K2 suffix = kone(), lt = kzero();
+      for (size_t i = ell_n_; i-- > 0;) {
+        K2 jp = chals_[i], j = r_cycle_[i];
+        lt = kadd(lt, kmul(kmul(ksub(kone(), jp), j), suffix));
+        suffix = kmul(suffix, kadd(kmul(ksub(kone(), jp), ksub(kone(), j)), kmul(jp, j)));
+      }
+      return kmul(kmul(kmul(g, colv(col_b_)), eq), lt);
+    }
+    case TSK_TOTAL_INC: return kmul(kmul(g, colv(col_b_)), eq);
+    default: return kzero();
+  }
+}
+
+void TsOracleGpu::product_round(std::vector<K2>& c) {
+  const int ncoef = (int)nf_ + 1;
+  c.assign(ncoef, kzero());
+  size_t half = len_ / 2;
+  unsigned nb = nblocks(half);
+  partials_.ensure((size_t)TMAXBLK * TS_MAX_COEF * sizeof(K2));
+  coef_dev_.ensure(TS_MAX_COEF * sizeof(K2));
+  if (ncoef <= 8)
+    hipLaunchKernelGGL(k_prod_round<8>, dim3(nb), dim3(TB), 0, st_, fac_[cur_].as<K2>(), (int)nf_, flen_, half,
+                       partials_.as<K2>());
+  else
+    hipLaunchKernelGGL(k_prod_round<TS_MAX_COEF>, dim3(nb), dim3(TB), 0, st_, fac_[cur_].as<K2>(), (int)nf_, flen_, half,
+                       partials_.as<K2>());
+  hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_dev_.as<K2>());
+  NS_CHECK(hipMemcpyAsync(c.data(), coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  NS_CHECK(hipStreamSynchronize(st_));
+}
+
+void TsOracleGpu::init_lookup(const std::vector<SparseCol>& addr_bits, const SparseCol& has_lookup,
+                              const std::vector<K2>& table, const std::vector<K2>& r_cycle, size_t ell_n) {
+  const size_t ell_addr = addr_bits.size();
+  if (ell_addr > 24) throw Error(1, "address lookup: ell_addr <= 24");
+  const size_t pow2_addr = (size_t)1 << ell_addr;
+  const size_t len = (size_t)1 << ell_n;
+  // lookups: nonzero gates; addresses from the bit columns (a bit counts when it equals ONE)
+  std::vector<uint32_t> ts, addr;
+  std::vector<K2> gates;
+  for (size_t i = 0; i < has_lookup.idx.size(); i++) {
+    if (has_lookup.idx[i] >= len) throw Error(1, "sparse column index out of range");
+    if (kiszero(has_lookup.val[i])) continue;
+    uint32_t t = has_lookup.idx[i], a = 0;
+    for (size_t b = 0; b < ell_addr; b++) {
+      auto& c = addr_bits[b];
+      auto it = std::lower_bound(c.idx.begin(), c.idx.end(), t);
+      if (it != c.idx.end() && *it == t && keq(c.val[it - c.idx.begin()], kone())) a |= 1u << b;
+    }
+    ts.push_back(t);
+    addr.push_back(a);
+    gates.push_back(has_lookup.val[i]);
+  }
+  const uint32_t n = (uint32_t)ts.size();
+  std::vector<K2> tab(pow2_addr, kzero());
+  for (size_t a = 0; a < std::min(pow2_addr, table.size()); a++) tab[a] = table[a];
+  nf_ = 2;
+  flen_ = pow2_addr;
+  fac_[0].ensure(2 * pow2_addr * sizeof(K2));
+  fac_[1].ensure(2 * pow2_addr * sizeof(K2));
+  NS_CHECK(hipMemcpyAsync(fac_[0].p, tab.data(), pow2_addr * sizeof(K2), hipMemcpyHostToDevice, st_));
+  K2* W = fac_[0].as<K2>() + pow2_addr;
+  if (n == 0) {
+    NS_CHECK(hipMemsetAsync(W, 0, pow2_addr * sizeof(K2), st_));
+  } else {
+    DevBuf dts, dg, dw, da, dv, sa, sv, rc, tmp;
+    upload(dts, ts, st_);
+    upload(dg, gates, st_);
+    upload(da, addr, st_);
+    std::vector<K2> rcy(r_cycle);
+    rcy.resize(std::max<size_t>(ell_n, 1), kzero());
+    upload(rc, rcy, st_);
+    dw.ensure(n * sizeof(K2));
+    hipLaunchKernelGGL(k_ts_chi_at, dim3((n + 255) / 256), dim3(256), 0, st_, dts.as<uint32_t>(), n, rc.as<K2>(),
+                       (int)ell_n, dg.as<K2>(), dw.as<K2>());
+    std::vector<uint32_t> iota(n);
+    for (uint32_t i = 0; i < n; i++) iota[i] = i;
+    upload(dv, iota, st_);
+    sa.ensure(n * 4);
+    sv.ensure(n * 4);
+    size_t tb = 0;
+    NS_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, da.as<uint32_t>(), sa.as<uint32_t>(), dv.as<uint32_t>(),
+                                                sv.as<uint32_t>(), (int)n, 0, (int)ell_addr + 1, st_));
+    tmp.ensure(tb + 16);
+    NS_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, da.as<uint32_t>(), sa.as<uint32_t>(), dv.as<uint32_t>(),
+                                                sv.as<uint32_t>(), (int)n, 0, (int)ell_addr + 1, st_));
+    hipLaunchKernelGGL(k_lookup_weights, dim3((unsigned)((pow2_addr + 255) / 256)), dim3(256), 0, st_,
+                       sa.as<uint32_t>(), sv.as<uint32_t>(), n, dw.as<K2>(), (uint32_t)pow2_addr, W);
+    NS_CHECK(hipStreamSynchronize(st_));
+  }
+  NS_CHECK(hipStreamSynchronize(st_));
+  kind_ = TSK_ADDR_LOOKUP;
+  degree_bound_ = 2;
+  cur_ = 0;
+  len_ = pow2_addr;
+  ell_n_ = ell_addr;
+}
+
+void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs, const std::vector<K2>& init_vals,
+                            const std::vector<K2>& r_cycle, size_t ell_n, const SparseCol* has_read,
+                            const SparseCol* rv, const std::vector<SparseCol>* ra_bits, const SparseCol& has_write,
+                            const SparseCol& wv_or_inc, const std::vector<SparseCol>& wa_bits, const SparseCol& inc,
+                            size_t ell_addr) {
+  kind_ = kind;
+  if (ell_addr > 24) throw Error(1, "address lane: ell_addr <= 24");
+  ell_addr_ = ell_addr;
+  ell_n_ = ell_n;
+  const size_t len = (size_t)1 << ell_n, pow2_addr = (size_t)1 << ell_addr;
+  const bool rd = kind == TSK_READ_ADDR;
+  auto get = [](const SparseCol& c, uint32_t t) {
+    auto it = std::lower_bound(c.idx.begin(), c.idx.end(), t);
+    return (it != c.idx.end() && *it == t) ? c.val[it - c.idx.begin()] : kzero();
+  };
+  auto addr_at = [&](const std::vector<SparseCol>& bits, uint32_t t) {
+    uint32_t a = 0;
+    for (size_t b = 0; b < bits.size(); b++)
+      if (keq(get(bits[b], t), kone())) a |= 1u << b;
+    return a;
+  };
+  auto nz_times = [len](const SparseCol& c) {
+    std::vector<uint32_t> v;
+    for (size_t i = 0; i < c.idx.size(); i++) {
+      if (c.idx[i] >= len) throw Error(1, "sparse column index out of range");
+      if (!kiszero(c.val[i])) v.push_back(c.idx[i]);
+    }
+    return v;
+  };
+  std::vector<uint32_t> times = nz_times(has_write);
+  if (rd) {
+    std::vector<uint32_t> r = nz_times(*has_read), m;
+    std::merge(r.begin(), r.end(), times.begin(), times.end(), std::back_inserter(m));
+    m.erase(std::unique(m.begin(), m.end()), m.end());
+    times.swap(m);
+  }
+  nev_ = times.size();
+  std::vector<K2> hr(nev_), rvv(nev_), hw(nev_), ic(nev_), one(nev_, kone());
+  std::vector<uint32_t> ra(nev_, 0), wa(nev_, 0);
+  for (size_t e = 0; e < nev_; e++) {
+    uint32_t t = times[e];
+    hw[e] = get(has_write, t);
+    ic[e] = get(inc, t);
+    if (rd) {
+      hr[e] = get(*has_read, t);
+      rvv[e] = get(*rv, t);
+      ra[e] = kiszero(hr[e]) ? 0 : addr_at(*ra_bits, t);
+      wa[e] = kiszero(hw[e]) ? 0 : addr_at(wa_bits, t);
+    } else {
+      hr[e] = kzero();
+      rvv[e] = get(wv_or_inc, t);  // wv
+      wa[e] = addr_at(wa_bits, t);
+    }
+  }
+  const size_t ne = std::max<size_t>(nev_, 1);
+  upload(ev_hr_, hr, st_, 1);
+  upload(ev_rv_, rvv, st_, 1);
+  upload(ev_hw_, hw, st_, 1);
+  upload(ev_inc_, ic, st_, 1);
+  upload(ev_ra_, ra, st_, 1);
+  upload(ev_wa_, wa, st_, 1);
+  upload(ev_rw_, one, st_, 1);
+  upload(ev_ww_, one, st_, 1);
+  ev_eq_.ensure(ne * sizeof(K2));
+  DevBuf dts, rc;
+  upload(dts, times, st_, 1);
+  std::vector<K2> rcy(r_cycle);
+  rcy.resize(std::max<size_t>(ell_n, 1), kzero());
+  upload(rc, rcy, st_);
+  if (nev_)
+    hipLaunchKernelGGL(k_ts_chi_at, dim3((unsigned)((nev_ + 255) / 256)), dim3(256), 0, st_, dts.as<uint32_t>(),
+                       (uint32_t)nev_, rc.as<K2>(), (int)ell_n, (const K2*)nullptr, ev_eq_.as<K2>());
+  // dense init table over addresses
+  std::vector<K2> it(pow2_addr, kzero());
+  for (size_t i = 0; i < init_addrs.size(); i++) {
+    if (init_addrs[i] >= pow2_addr) throw Error(1, "init address out of range");
+    it[init_addrs[i]] = kadd(it[init_addrs[i]], init_vals[i]);
+  }
+  upload(it_[0], it, st_);
+  it_[1].ensure(pow2_addr * sizeof(K2));
+  ninit_tab_ = pow2_addr;
+  for (int i = 0; i < 2; i++) {
+    keys_[i].ensure(ne * 4);
+    vals_[i].ensure(ne * 4);
+  }
+  dsort_.ensure(ne * sizeof(K2) * 2);
+  dscan_.ensure(ne * sizeof(K2));
+  size_t tb = 0, tb2 = 0;
+  NS_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys_[0].as<uint32_t>(), keys_[1].as<uint32_t>(),
+                                              vals_[0].as<uint32_t>(), vals_[1].as<uint32_t>(), (int)ne, 0,
+                                              (int)ell_addr + 1, st_));
+  NS_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, tb2, dsort_.as<K2>(), dscan_.as<K2>(), KAddOp(), (int)ne, st_));
+  sort_tmp_bytes_ = std::max(tb, tb2);
+  sort_tmp_.ensure(sort_tmp_bytes_ + 16);
+  partials_.ensure((size_t)TMAXBLK * TS_MAX_COEF * sizeof(K2));
+  coef_dev_.ensure(TS_MAX_COEF * sizeof(K2));
+  cur_ = 0;
+  bit_idx_ = 0;
+  degree_bound_ = 2;
+  NS_CHECK(hipStreamSynchronize(st_));
+}
+
+void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
+  c.assign(3, kzero());
+  if (nev_ == 0) return;
+  const uint32_t n = (uint32_t)nev_;
+  const int bit = final ? -1 : (int)bit_idx_;
+  const uint32_t sentinel = final ? 1u : (uint32_t)((ninit_tab_ >> bit_idx_));
+  unsigned g = (n + 255) / 256;
+  K2* dval = dsort_.as<K2>() + n;
+  hipLaunchKernelGGL(k_addr_keys, dim3(g), dim3(256), 0, st_, ev_hw_.as<K2>(), ev_inc_.as<K2>(), ev_ww_.as<K2>(),
+                     ev_wa_.as<uint32_t>(), n, bit, sentinel, keys_[0].as<uint32_t>(), vals_[0].as<uint32_t>(), dval);
+  size_t tb = sort_tmp_bytes_;
+  NS_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp_.p, tb, keys_[0].as<uint32_t>(), keys_[1].as<uint32_t>(),
+                                              vals_[0].as<uint32_t>(), vals_[1].as<uint32_t>(), (int)n, 0,
+                                              (int)ell_addr_ + 1, st_));
+  hipLaunchKernelGGL(k_addr_gather, dim3(g), dim3(256), 0, st_, vals_[1].as<uint32_t>(), dval, n, dsort_.as<K2>());
+  tb = sort_tmp_bytes_;
+  NS_CHECK(hipcub::DeviceScan::InclusiveScan(sort_tmp_.p, tb, dsort_.as<K2>(), dscan_.as<K2>(), KAddOp(), (int)n, st_));
+  AddrArgs A;
+  A.nev = n;
+  A.is_read = kind_ == TSK_READ_ADDR;
+  A.bit = (int)bit_idx_;
+  A.final_ = final ? 1 : 0;
+  A.eq = ev_eq_.as<K2>();
+  A.hr = ev_hr_.as<K2>();
+  A.rv = ev_rv_.as<K2>();
+  A.hw = ev_hw_.as<K2>();
+  A.inc = ev_inc_.as<K2>();
+  A.rw = ev_rw_.as<K2>();
+  A.ww = ev_ww_.as<K2>();
+  A.ra = ev_ra_.as<uint32_t>();
+  A.wa = ev_wa_.as<uint32_t>();
+  A.skeys = keys_[1].as<uint32_t>();
+  A.svals = vals_[1].as<uint32_t>();
+  A.scan = dscan_.as<K2>();
+  A.it = it_[cur_].as<K2>();
+  unsigned nb = nblocks(n);
+  hipLaunchKernelGGL(k_addr_round, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
+  hipLaunchKernelGGL(k_ts_sum, dim3(3), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, 3, coef_dev_.as<K2>());
+  NS_CHECK(hipMemcpyAsync(c.data(), coef_dev_.p, 3 * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  NS_CHECK(hipStreamSynchronize(st_));
+}
+
+void TsOracleGpu::round_coeffs(std::vector<K2>& c) {
+  switch (kind_) {
+    case TSK_PRODUCT:
+    case TSK_ADDR_LOOKUP:
+      product_round(c);
+      return;
+    case TSK_READ_ADDR:
+    case TSK_WRITE_ADDR:
+      addr_round(c, false);
+      return;
+    default:
+      time_round(c);
+  }
+}
+
+void TsOracleGpu::evals(const K2* xs, size_t n, K2* out) {
+  if (num_rounds() == 0) {
+    K2 v = kzero();
+    if (kind_ == TSK_PRODUCT || kind_ == TSK_ADDR_LOOKUP) {
+      std::vector<K2> f(nf_);
+      for (size_t i = 0; i < nf_; i++)
+        NS_CHECK(hipMemcpyAsync(&f[i], fac_[cur_].as<K2>() + i * flen_, sizeof(K2), hipMemcpyDeviceToHost, st_));
+      NS_CHECK(hipStreamSynchronize(st_));
+      v = kone();
+      for (auto& x : f) v = kmul(v, x);
+    } else if (kind_ == TSK_READ_ADDR || kind_ == TSK_WRITE_ADDR) {
+      std::vector<K2> c;
+      addr_round(c, true);
+      v = c[0];
+    } else {
+      v = time_final();
+    }
+    for (size_t i = 0; i < n; i++) out[i] = v;
+    return;
+  }
+  std::vector<K2> c;
+  round_coeffs(c);
+  for (size_t i = 0; i < n; i++) {
+    K2 acc = kzero();
+    for (size_t d = c.size(); d-- > 0;) acc = kadd(kmul(acc, xs[i]), c[d]);
+    out[i] = acc;
+  }
+}
+
+K2 TsOracleGpu::claim() {
+  std::vector<K2> c;
+  round_coeffs(c);
+  // p(0) + p(1) = 2 c0 + c1 + ... + c_d
+  K2 s = c.empty() ? kzero() : c[0];
+  for (auto& x : c) s = kadd(s, x);
+  return s;
+}
+
+void TsOracleGpu::fold(K2 r) {
+  if (num_rounds() == 0) return;
+  switch (kind_) {
+    case TSK_PRODUCT:
+    case TSK_ADDR_LOOKUP: {
+      size_t half = len_ / 2;
+      hipLaunchKernelGGL(k_prod_fold, dim3((unsigned)((half + 255) / 256), (unsigned)nf_), dim3(256), 0, st_,
+                         fac_[cur_].as<K2>(), fac_[cur_ ^ 1].as<K2>(), (int)nf_, flen_, half, r);
+      NS_CHECK(hipStreamSynchronize(st_));
+      cur_ ^= 1;
+      len_ = half;
+      chals_.push_back(r);
+      return;
+    }
+    case TSK_READ_ADDR:
+    case TSK_WRITE_ADDR: {
+      const uint32_t n = (uint32_t)nev_;
+      if (n) {
+        unsigned g = (n + 255) / 256;
+        if (kind_ == TSK_READ_ADDR)
+          hipLaunchKernelGGL(k_addr_fold_w, dim3(g), dim3(256), 0, st_, ev_rw_.as<K2>(), ev_ra_.as<uint32_t>(), n,
+                             (int)bit_idx_, r);
+        hipLaunchKernelGGL(k_addr_fold_w, dim3(g), dim3(256), 0, st_, ev_ww_.as<K2>(), ev_wa_.as<uint32_t>(), n,
+                           (int)bit_idx_, r);
+      }
+      size_t half = (ninit_tab_ >> bit_idx_) / 2;
+      hipLaunchKernelGGL(k_prod_fold, dim3((unsigned)((half + 255) / 256), 1), dim3(256), 0, st_, it_[cur_].as<K2>(),
+                         it_[cur_ ^ 1].as<K2>(), 1, (size_t)0, half, r);
+      NS_CHECK(hipStreamSynchronize(st_));
+      cur_ ^= 1;
+      bit_idx_++;
+      return;
+    }
+    case TSK_TOTAL_INC:
+      time_fold(r);
+      return;
+    default:
+      break;
+  }
+  if (kind_ != TSK_VAL_EVAL) {
+    K2 rc = r_cycle_[bit_idx_];
+    prefix_eq_ = kmul(prefix_eq_, kadd(kmul(ksub(kone(), r), ksub(kone(), rc)), kmul(r, rc)));
+  }
+  time_fold(r);
+  chals_.push_back(r);
+  bit_idx_++;
+}
+
+}  // namespace ns

@@ -1,0 +1,172 @@
+"""Twist/Shout round oracles on the GPU (ctypes mirror of the ns_ts_* C ABI).
+
+Each TsOracle implements the RoundOracle contract of neo-reductions/src/sumcheck.rs:17-34
+(evals / fold / num_rounds / degree_bound).  Sparse time columns are passed as
+(indices, [K values]) pairs, the SparseIdxVec of neo-memory/src/sparse_time.rs; K values are
+(c0, c1) tuples of canonical Goldilocks integers.  The constructors take the arguments of the
+reference constructors (neo-memory/src/twist_oracle.rs) in the same order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import NsError, _check, lib
+
+
+class SparseCol(C.Structure):
+    _fields_ = [("nnz", C.c_size_t), ("idx", C.POINTER(C.c_uint64)), ("vals", C.POINTER(C.c_uint64))]
+
+
+def _kbuf(ks):
+    return np.array([v for k in ks for v in (int(k[0]), int(k[1]))] or [0, 0], dtype=np.uint64)
+
+
+class _Keep:
+    def __init__(self):
+        self.items = []
+
+    def col(self, c):
+        idx = np.ascontiguousarray(np.asarray(c[0], np.uint64).reshape(-1)) if len(c[0]) else np.zeros(1, np.uint64)
+        vals = _kbuf(c[1])
+        self.items += [idx, vals]
+        return SparseCol(len(c[0]), idx.ctypes.data_as(C.POINTER(C.c_uint64)),
+                         vals.ctypes.data_as(C.POINTER(C.c_uint64)))
+
+    def cols(self, cs):
+        arr = (SparseCol * max(1, len(cs)))(*[self.col(c) for c in cs])
+        self.items.append(arr)
+        return arr
+
+    def ks(self, ks):
+        a = _kbuf(ks)
+        self.items.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+
+_BOUND = False
+
+
+def _bind():
+    global _BOUND
+    L = lib()
+    if _BOUND:
+        return L
+    vp, sz, S = C.c_void_p, C.c_size_t, SparseCol
+    out = C.POINTER(vp)
+    L.ns_ts_product_new.argtypes = [vp, sz, sz, sz, out]
+    L.ns_ts_shout_value_new.argtypes = [vp, sz, S, S, vp, out]
+    L.ns_ts_index_adapter_new.argtypes = [vp, sz, S, vp, sz, vp, vp, out]
+    L.ns_ts_bitness_new.argtypes = [vp, sz, vp, vp, sz, out]
+    L.ns_ts_twist_read_new.argtypes = [vp, sz, S, S, vp, S, S, vp, sz, vp, vp, out]
+    L.ns_ts_twist_write_new.argtypes = [vp, sz, S, S, S, vp, sz, vp, vp, out]
+    L.ns_ts_val_eval_new.argtypes = [sz, vp, sz, S, S, vp, vp, vp, out]
+    L.ns_ts_total_inc_new.argtypes = [sz, vp, sz, S, S, vp, vp, out]
+    L.ns_ts_read_addr_new.argtypes = [vp, vp, sz, vp, sz, S, S, vp, S, vp, S, sz, out]
+    L.ns_ts_write_addr_new.argtypes = [vp, vp, sz, vp, sz, S, S, vp, S, sz, out]
+    L.ns_ts_addr_lookup_new.argtypes = [vp, sz, S, vp, sz, vp, sz, vp, out]
+    L.ns_ts_evals.argtypes = [vp, vp, sz, vp]
+    L.ns_ts_fold.argtypes = [vp, vp]
+    L.ns_ts_num_rounds.argtypes = [vp]
+    L.ns_ts_num_rounds.restype = sz
+    L.ns_ts_degree_bound.argtypes = [vp]
+    L.ns_ts_degree_bound.restype = sz
+    L.ns_ts_free.argtypes = [vp]
+    L.ns_ts_free.restype = None
+    _BOUND = True
+    return L
+
+
+class TsOracle:
+    """Device-resident Twist/Shout oracle; `kind` names the reference oracle (see module doc)."""
+
+    def __init__(self, kind, *args):
+        L = _bind()
+        k = _Keep()
+        self._keep = k
+        h = C.c_void_p()
+        claim = np.zeros(2, np.uint64)
+        cp = claim.ctypes.data_as(C.c_void_p)
+        if kind == "product":
+            factors, degree_bound = args
+            bufs = [_kbuf(f) for f in factors]
+            k.items += bufs
+            ptrs = (C.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+            k.items.append(ptrs)
+            rc = L.ns_ts_product_new(ptrs, len(factors), len(factors[0]), degree_bound, C.byref(h))
+        elif kind == "shout_value":
+            r_cycle, has_lookup, val = args
+            rc = L.ns_ts_shout_value_new(k.ks(r_cycle), len(r_cycle), k.col(has_lookup), k.col(val), cp, C.byref(h))
+        elif kind == "index_adapter":
+            r_cycle, has_lookup, addr_bits, r_addr = args
+            rc = L.ns_ts_index_adapter_new(k.ks(r_cycle), len(r_cycle), k.col(has_lookup), k.cols(addr_bits),
+                                           len(addr_bits), k.ks(r_addr), cp, C.byref(h))
+        elif kind == "bitness":
+            r_cycle, cols, weights = args
+            rc = L.ns_ts_bitness_new(k.ks(r_cycle), len(r_cycle), k.cols(cols), k.ks(weights), len(cols), C.byref(h))
+        elif kind == "twist_read":
+            r_cycle, has_read, rv, ra_bits, has_write, inc, wa_bits, r_addr, init_at = args
+            rc = L.ns_ts_twist_read_new(k.ks(r_cycle), len(r_cycle), k.col(has_read), k.col(rv), k.cols(ra_bits),
+                                        k.col(has_write), k.col(inc), k.cols(wa_bits), len(r_addr), k.ks(r_addr),
+                                        k.ks([init_at]), C.byref(h))
+        elif kind == "twist_write":
+            r_cycle, has_write, wv, inc, wa_bits, r_addr, init_at = args
+            rc = L.ns_ts_twist_write_new(k.ks(r_cycle), len(r_cycle), k.col(has_write), k.col(wv), k.col(inc),
+                                         k.cols(wa_bits), len(r_addr), k.ks(r_addr), k.ks([init_at]), C.byref(h))
+        elif kind == "val_eval":
+            ell_n, wa_bits, has_write, inc, r_addr, r_time = args
+            rc = L.ns_ts_val_eval_new(ell_n, k.cols(wa_bits), len(wa_bits), k.col(has_write), k.col(inc),
+                                      k.ks(r_addr), k.ks(r_time), cp, C.byref(h))
+        elif kind == "total_inc":
+            ell_n, wa_bits, has_write, inc, r_addr = args
+            rc = L.ns_ts_total_inc_new(ell_n, k.cols(wa_bits), len(wa_bits), k.col(has_write), k.col(inc),
+                                       k.ks(r_addr), cp, C.byref(h))
+        elif kind in ("read_addr", "write_addr"):
+            init = args[0]
+            ia = np.array([a for a, _ in init] or [0], np.uint64)
+            k.items.append(ia)
+            iv = k.ks([v for _, v in init])
+            if kind == "read_addr":
+                _, r_cycle, has_read, rv, ra_bits, has_write, wa_bits, inc = args
+                rc = L.ns_ts_read_addr_new(ia.ctypes.data, iv, len(init), k.ks(r_cycle), len(r_cycle), k.col(has_read),
+                                           k.col(rv), k.cols(ra_bits), k.col(has_write), k.cols(wa_bits), k.col(inc),
+                                           len(ra_bits), C.byref(h))
+            else:
+                _, r_cycle, has_write, wv, wa_bits, inc = args
+                rc = L.ns_ts_write_addr_new(ia.ctypes.data, iv, len(init), k.ks(r_cycle), len(r_cycle),
+                                            k.col(has_write), k.col(wv), k.cols(wa_bits), k.col(inc), len(wa_bits),
+                                            C.byref(h))
+        elif kind == "addr_lookup":
+            addr_bits, has_lookup, table, r_cycle = args
+            rc = L.ns_ts_addr_lookup_new(k.cols(addr_bits), len(addr_bits), k.col(has_lookup), k.ks(table), len(table),
+                                         k.ks(r_cycle), len(r_cycle), cp, C.byref(h))
+        else:
+            raise NsError(-1, f"unknown oracle kind {kind}")
+        _check(rc)
+        self.h = h
+        self.claim = (int(claim[0]), int(claim[1]))
+
+    def num_rounds(self) -> int:
+        return _bind().ns_ts_num_rounds(self.h)
+
+    def degree_bound(self) -> int:
+        return _bind().ns_ts_degree_bound(self.h)
+
+    def evals(self, xs):
+        xb = _kbuf(xs)
+        out = np.zeros(2 * len(xs), np.uint64)
+        _check(_bind().ns_ts_evals(self.h, xb.ctypes.data, len(xs), out.ctypes.data))
+        o = [int(v) for v in out]
+        return [(o[2 * i], o[2 * i + 1]) for i in range(len(xs))]
+
+    def fold(self, r):
+        rb = np.array([int(r[0]), int(r[1])], np.uint64)
+        _check(_bind().ns_ts_fold(self.h, rb.ctypes.data))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _bind().ns_ts_free(self.h)
+        except Exception:
+            pass

@@ -110,3 +110,92 @@ def check_r1cs(inst: R1CS, z: np.ndarray) -> bool:
         return sum(int(v[k]) * zi[int(ci[k])] for k in range(int(rp[i]), int(rp[i + 1]))) % Q
 
     return all(mv(inst.a, i) * mv(inst.b, i) % Q == mv(inst.c, i) for i in range(inst.n))
+
+
+# ---------------- Twist / Shout traces ----------------
+def _kconst(v):
+    return (int(v) % Q, 0)
+
+
+def sparse_bits(addrs, active, ell_addr):
+    """Bit columns of addresses at the active times: column b holds ONE where bit b is set."""
+    cols = []
+    ts = np.nonzero(active)[0]
+    for b in range(ell_addr):
+        sel = ts[((addrs[ts] >> b) & 1) == 1]
+        cols.append((sel.astype(np.uint64), [(1, 0)] * len(sel)))
+    return cols
+
+
+def gen_memory_trace(ell_n: int, ell_addr: int, p_access: float, write_frac: float, seed: int, zipf_s: float = 0.0):
+    """Synthetic Twist memory lane over 2^ell_n cycles: each cycle accesses memory with probability
+    p_access (a read or, with probability write_frac, a write), at a uniform address or a Zipf(s)
+    address when zipf_s > 0; written values are uniform u32.  Returns the sparse columns of the
+    Twist oracles (has_read, rv, ra_bits, has_write, wv, inc, wa_bits) plus the initial memory.
+    rv is the value read (memory contents at that time); inc = wv - old value."""
+    rng = np.random.default_rng(seed)
+    T, A = 1 << ell_n, 1 << ell_addr
+    if zipf_s > 0:
+        ranks = np.arange(1, A + 1, dtype=np.float64)
+        p = ranks ** (-zipf_s)
+        p /= p.sum()
+        perm = rng.permutation(A)
+        addrs = perm[rng.choice(A, size=T, p=p)]
+    else:
+        addrs = rng.integers(0, A, size=T)
+    access = rng.random(T) < p_access
+    is_w = access & (rng.random(T) < write_frac)
+    is_r = access & ~is_w
+    init = {int(a): int(v) for a, v in zip(rng.choice(A, size=min(A, 64), replace=False),
+                                              rng.integers(0, 1 << 32, size=min(A, 64)))}
+    mem = dict(init)
+    rv, wv, inc = [], [], []
+    for t in range(T):
+        a = int(addrs[t])
+        if is_r[t]:
+            rv.append(mem.get(a, 0))
+        if is_w[t]:
+            v = int(rng.integers(0, 1 << 32))
+            wv.append(v)
+            inc.append((v - mem.get(a, 0)) % Q)
+            mem[a] = v
+    tr = np.nonzero(is_r)[0].astype(np.uint64)
+    tw = np.nonzero(is_w)[0].astype(np.uint64)
+    one = [(1, 0)]
+    return dict(
+        ell_n=ell_n, ell_addr=ell_addr,
+        has_read=(tr, one * len(tr)), rv=(tr[np.array(rv, dtype=np.int64) != 0] if len(tr) else tr,
+                                          [_kconst(v) for v in rv if v != 0]),
+        ra_bits=sparse_bits(addrs, is_r, ell_addr),
+        has_write=(tw, one * len(tw)), wv=(tw[np.array(wv, dtype=np.int64) != 0] if len(tw) else tw,
+                                           [_kconst(v) for v in wv if v != 0]),
+        inc=(tw[np.array(inc, dtype=object) != 0] if len(tw) else tw, [(v, 0) for v in inc if v != 0]),
+        wa_bits=sparse_bits(addrs, is_w, ell_addr),
+        init=sorted((a, (v % Q, 0)) for a, v in init.items()),
+    )
+
+
+def gen_lookup_trace(ell_n: int, seed: int):
+    """Shout lane over 2^ell_n cycles into the 2^16-entry table op(a, b) for 8-bit a, b, with the
+    op chosen per lookup among ADD / XOR / AND (the address is a | b << 8; the table is the
+    selected op's, so three tables and three lookup lanes).  Returns per op: (addr_bits,
+    has_lookup, val, table)."""
+    rng = np.random.default_rng(seed)
+    T = 1 << ell_n
+    a = rng.integers(0, 256, size=T)
+    b = rng.integers(0, 256, size=T)
+    op = rng.integers(0, 3, size=T)
+    grid_a = np.arange(1 << 16) & 0xFF
+    grid_b = np.arange(1 << 16) >> 8
+    tables = [(grid_a + grid_b), (grid_a ^ grid_b), (grid_a & grid_b)]
+    lanes = []
+    addr = a | (b << 8)
+    for o in range(3):
+        act = op == o
+        ts = np.nonzero(act)[0].astype(np.uint64)
+        vals = tables[o][addr[act]]
+        nz = vals != 0
+        lanes.append(dict(addr_bits=sparse_bits(addr, act, 16), has_lookup=(ts, [(1, 0)] * len(ts)),
+                          val=(ts[nz], [(int(v), 0) for v in vals[nz]]),
+                          table=[(int(v), 0) for v in tables[o]]))
+    return lanes

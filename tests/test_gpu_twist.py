@@ -1,0 +1,102 @@
+"""GPU Twist/Shout oracles against the CPU restatement, through the C ABI, on synthetic
+RISC-V-like lanes: register file (32 words), RAM (2^16 words, Zipf(1.1) addresses, 35% of
+cycles, 40% writes) and the 8x8-bit ADD/XOR/AND lookup table.  Every round's evaluations (at the
+degree+1 nodes and a random point of K), every claim and the final values must be identical."""
+import numpy as np
+import pytest
+
+from kmath import rand_k
+
+from nightstream_amd.workload import gen_lookup_trace, gen_memory_trace
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def twist(ns):
+    from nightstream_amd import twist
+    return twist
+
+
+def drive(g, c, rng, label):
+    assert g.num_rounds() == c.num_rounds(), label
+    assert g.degree_bound() == c.degree_bound(), label
+    assert g.claim == c.claim or c.claim == (0, 0), label
+    rounds = c.num_rounds()
+    for rd in range(rounds):
+        deg = c.degree_bound()
+        xs = [(v, 0) for v in range(deg + 1)] + [rand_k(rng)]
+        yc = c.evals(xs)
+        yg = g.evals(xs)
+        assert yg == yc, f"{label}: round {rd}"
+        r = rand_k(rng)
+        g.fold(r)
+        c.fold(r)
+    assert g.num_rounds() == c.num_rounds() == 0
+    x = [rand_k(rng)]
+    assert g.evals(x) == c.evals(x), f"{label}: final value"
+
+
+def both(twist, oracle, kind, *args):
+    return twist.TsOracle(kind, *args), oracle.TsOracle(kind, *args)
+
+
+@pytest.mark.parametrize("ell_addr,ell_n,p,wf,zipf", [(5, 10, 1.0, 0.34, 0.0), (16, 10, 0.35, 0.4, 1.1)])
+def test_twist_time_lanes(twist, oracle, ell_addr, ell_n, p, wf, zipf):
+    tr = gen_memory_trace(ell_n, ell_addr, p, wf, seed=7 + ell_addr, zipf_s=zipf)
+    rng = np.random.default_rng(ell_addr)
+    r_cycle = [rand_k(rng) for _ in range(ell_n)]
+    r_addr = [rand_k(rng) for _ in range(ell_addr)]
+    init_at = rand_k(rng)
+    print(f"twist time lanes ell_addr={ell_addr} ell_n={ell_n}", flush=True)
+    g, c = both(twist, oracle, "twist_read", r_cycle, tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"],
+                tr["inc"], tr["wa_bits"], r_addr, init_at)
+    drive(g, c, rng, "twist_read")
+    g, c = both(twist, oracle, "twist_write", r_cycle, tr["has_write"], tr["wv"], tr["inc"], tr["wa_bits"], r_addr,
+                init_at)
+    drive(g, c, rng, "twist_write")
+    r_time = [rand_k(rng) for _ in range(ell_n)]
+    g, c = both(twist, oracle, "val_eval", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], r_addr, r_time)
+    drive(g, c, rng, "val_eval")
+    g, c = both(twist, oracle, "total_inc", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], r_addr)
+    drive(g, c, rng, "total_inc")
+    w = [rand_k(rng) for _ in tr["ra_bits"]]
+    g, c = both(twist, oracle, "bitness", r_cycle, tr["ra_bits"], w)
+    drive(g, c, rng, "bitness")
+
+
+@pytest.mark.parametrize("ell_addr,ell_n", [(5, 12), (16, 12)])
+def test_twist_address_lanes(twist, oracle, ell_addr, ell_n):
+    tr = gen_memory_trace(ell_n, ell_addr, 1.0 if ell_addr == 5 else 0.35, 0.4, seed=3 + ell_addr,
+                          zipf_s=0.0 if ell_addr == 5 else 1.1)
+    rng = np.random.default_rng(50 + ell_addr)
+    r_cycle = [rand_k(rng) for _ in range(ell_n)]
+    print(f"twist address lanes ell_addr={ell_addr} ell_n={ell_n}", flush=True)
+    g, c = both(twist, oracle, "read_addr", tr["init"], r_cycle, tr["has_read"], tr["rv"], tr["ra_bits"],
+                tr["has_write"], tr["wa_bits"], tr["inc"])
+    drive(g, c, rng, "read_addr")
+    g, c = both(twist, oracle, "write_addr", tr["init"], r_cycle, tr["has_write"], tr["wv"], tr["wa_bits"], tr["inc"])
+    drive(g, c, rng, "write_addr")
+
+
+def test_shout_lanes(twist, oracle):
+    ell_n = 12
+    lanes = gen_lookup_trace(ell_n, seed=11)
+    rng = np.random.default_rng(60)
+    r_cycle = [rand_k(rng) for _ in range(ell_n)]
+    for i, ln in enumerate(lanes):
+        print(f"shout lane {i}", flush=True)
+        r_addr = [rand_k(rng) for _ in range(16)]
+        g, c = both(twist, oracle, "shout_value", r_cycle, ln["has_lookup"], ln["val"])
+        drive(g, c, rng, "shout_value")
+        g, c = both(twist, oracle, "index_adapter", r_cycle, ln["has_lookup"], ln["addr_bits"], r_addr)
+        drive(g, c, rng, "index_adapter")
+        g, c = both(twist, oracle, "addr_lookup", ln["addr_bits"], ln["has_lookup"], ln["table"], r_cycle)
+        drive(g, c, rng, "addr_lookup")
+
+
+def test_product_oracle(twist, oracle):
+    rng = np.random.default_rng(70)
+    facs = [[rand_k(rng) for _ in range(1 << 10)] for _ in range(3)]
+    g, c = both(twist, oracle, "product", facs, 3)
+    drive(g, c, rng, "product")
