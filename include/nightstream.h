@@ -178,6 +178,20 @@ size_t ns_ts_num_rounds(const ns_ts* o);
 size_t ns_ts_degree_bound(const ns_ts* o);
 void ns_ts_free(ns_ts* o);
 
+/* ---- shared CPU bus of the memory sidecar.  Z: host digit matrix [54][m] of the CPU witness;
+ *      bus_cell(col, j) = bus_base + col*chunk + j, time_index(j) = m_in + j
+ *      (neo-memory/src/cpu/bus_layout.rs:63-80). */
+/* decode_vector (neo-memory/src/ajtai.rs:31-55): z[c] = sum_rho 2^rho Z[rho][c]. */
+int ns_bus_decode(const int8_t* Z, size_t m, uint64_t* z_out);
+/* decode_cpu_z_to_k + build_time_sparse_from_bus_col for every column (cpu_bus.rs:969-1016):
+ * column c's nonzero entries go to idx_out[c*chunk ..] / vals_out[2*(c*chunk) ..], count nnz_out[c]. */
+int ns_bus_time_cols(const int8_t* Z, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                     size_t steps_len, size_t ell_n, uint64_t* idx_out, uint64_t* vals_out, size_t* nnz_out);
+/* append_bus_openings_to_me_instance (cpu_bus.rs:195-320): y_out[(c*64 + rho)*2 ..] and
+ * y_scalars_out[c*2 ..] for every bus column at the ME point r[ell_n]. */
+int ns_bus_openings(const int8_t* Z, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                    const uint64_t* r, size_t ell_n, uint64_t* y_out, uint64_t* y_scalars_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -585,3 +585,37 @@ extern "C" int ns_ts_fold(ns_ts* o, const uint64_t r[2]) {
 extern "C" size_t ns_ts_num_rounds(const ns_ts* o) { return o ? o->o->num_rounds() : 0; }
 extern "C" size_t ns_ts_degree_bound(const ns_ts* o) { return o ? o->o->degree_bound() : 0; }
 extern "C" void ns_ts_free(ns_ts* o) { delete o; }
+
+// ---------------- shared CPU bus ----------------
+extern "C" int ns_bus_decode(const int8_t* Z, size_t m, uint64_t* z_out) {
+  return guard([&] {
+    require_device();
+    bus_decode(Z, m, z_out);
+  });
+}
+
+extern "C" int ns_bus_time_cols(const int8_t* Z, size_t m, size_t m_in, size_t bus_base, size_t chunk,
+                                size_t bus_cols, size_t steps_len, size_t ell_n, uint64_t* idx_out, uint64_t* vals_out,
+                                size_t* nnz_out) {
+  return guard([&] {
+    require_device();
+    bus_time_cols(Z, m, m_in, bus_base, chunk, bus_cols, steps_len, ell_n, idx_out, vals_out, nnz_out);
+  });
+}
+
+extern "C" int ns_bus_openings(const int8_t* Z, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                               const uint64_t* r, size_t ell_n, uint64_t* y_out, uint64_t* y_scalars_out) {
+  return guard([&] {
+    require_device();
+    std::vector<K2> rr = kvec(r, ell_n), y(bus_cols * 64), ys(bus_cols);
+    bus_openings(Z, m, m_in, bus_base, chunk, bus_cols, rr.data(), ell_n, y.data(), ys.data());
+    for (size_t i = 0; i < y.size(); i++) {
+      y_out[2 * i] = y[i].c0;
+      y_out[2 * i + 1] = y[i].c1;
+    }
+    for (size_t i = 0; i < ys.size(); i++) {
+      y_scalars_out[2 * i] = ys[i].c0;
+      y_scalars_out[2 * i + 1] = ys[i].c1;
+    }
+  });
+}

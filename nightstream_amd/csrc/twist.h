@@ -96,4 +96,12 @@ class TsOracleGpu {
   size_t sort_tmp_bytes_ = 0;
 };
 
+// Shared CPU bus helpers (neo-fold/src/memory_sidecar/cpu_bus.rs): Z is the host digit matrix
+// [54][m] of the CPU witness.
+void bus_decode(const int8_t* Z_host, size_t m, uint64_t* z_out);
+void bus_time_cols(const int8_t* Z_host, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                   size_t steps_len, size_t ell_n, uint64_t* idx_out, uint64_t* vals_out, size_t* nnz_out);
+void bus_openings(const int8_t* Z_host, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                  const K2* r, size_t ell_n, K2* y_out, K2* ys_out);
+
 }  // namespace ns

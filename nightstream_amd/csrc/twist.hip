@@ -453,6 +453,85 @@ __global__ void k_addr_fold_w(K2* __restrict__ w, const uint32_t* __restrict__ a
   w[e] = kmul(w[e], ((addr[e] >> bit) & 1) ? r : ksub(kone(), r));
 }
 
+// ---------------- shared CPU bus (memory_sidecar/cpu_bus.rs) ----------------
+// z[c] = sum_rho 2^rho Z[rho][c]; |z| < 2^54 so the signed sum is exact in int64
+__device__ __forceinline__ uint64_t decode_col(const int8_t* __restrict__ Z, size_t m, size_t c) {
+  int64_t acc = 0;
+  for (int rho = D - 1; rho >= 0; rho--) acc = 2 * acc + Z[(size_t)rho * m + c];
+  return from_i64(acc);
+}
+
+__global__ void k_bus_decode(const int8_t* __restrict__ Z, size_t m, uint64_t* __restrict__ z) {
+  size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (c < m) z[c] = decode_col(Z, m, c);
+}
+
+// One block per bus column: decode its chunk cells and compact the nonzero ones (time-ordered).
+__global__ __launch_bounds__(256) void k_bus_time_cols(const int8_t* __restrict__ Z, size_t m, uint32_t m_in,
+                                                       size_t bus_base, uint32_t chunk, uint32_t steps_len,
+                                                       uint32_t* __restrict__ idx_out, K2* __restrict__ vals_out,
+                                                       uint32_t* __restrict__ nnz_out) {
+  const uint32_t col = blockIdx.x;
+  __shared__ uint32_t wave_cnt[4];
+  __shared__ uint32_t base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t j0 = 0; j0 < steps_len; j0 += 256) {
+    uint32_t j = j0 + threadIdx.x;
+    uint64_t v = 0;
+    if (j < steps_len) v = decode_col(Z, m, bus_base + (size_t)col * chunk + j);
+    bool nz = j < steps_len && v != 0;
+    unsigned long long bal = __ballot(nz);
+    uint32_t before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_cnt[wave] = __popcll(bal);
+    __syncthreads();
+    uint32_t off = base;
+    for (int w = 0; w < wave; w++) off += wave_cnt[w];
+    if (nz) {
+      uint32_t p = off + before;
+      idx_out[(size_t)col * chunk + p] = m_in + j;
+      vals_out[(size_t)col * chunk + p] = K2{v, 0};
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nnz_out[col] = base;
+}
+
+// y[col][rho] = sum_j w[j] Z[rho][bus_cell(col, j)]: one block per (col, rho)
+__global__ __launch_bounds__(256) void k_bus_open(const int8_t* __restrict__ Z, size_t m, size_t bus_base,
+                                                  uint32_t chunk, const K2* __restrict__ w, K2* __restrict__ y) {
+  const uint32_t col = blockIdx.x, rho = blockIdx.y;
+  const int8_t* row = Z + (size_t)rho * m + bus_base + (size_t)col * chunk;
+  K2 acc = kzero();
+  for (uint32_t j = threadIdx.x; j < chunk; j += blockDim.x) {
+    int8_t d = row[j];
+    if (d > 0) acc = kadd(acc, w[j]);
+    else if (d < 0) acc = ksub(acc, w[j]);
+  }
+  __shared__ K2 red[4];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) y[(size_t)col * 64 + rho] = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
+}
+
+// y_scalar[col] = sum_rho 2^rho y[col][rho]; rows rho >= 54 of the 64-padded y are zeroed
+__global__ void k_bus_scalars(K2* __restrict__ y, uint32_t bus_cols, K2* __restrict__ ys) {
+  uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= bus_cols) return;
+  K2 s = kzero();
+  uint64_t pw = 1;
+  for (int rho = 0; rho < D; rho++) {
+    s = kadd(s, kscale(y[(size_t)col * 64 + rho], pw));
+    pw = fadd(pw, pw);
+  }
+  for (int rho = D; rho < 64; rho++) y[(size_t)col * 64 + rho] = kzero();
+  ys[col] = s;
+}
+
 // ======================= host =======================
 static unsigned nblocks(size_t n, size_t cap = TMAXBLK) {
   size_t b = (n + TB - 1) / TB;
@@ -1104,6 +1183,75 @@ void TsOracleGpu::fold(K2 r) {
   time_fold(r);
   chals_.push_back(r);
   bit_idx_++;
+}
+
+// ---------------- shared CPU bus (host) ----------------
+static void bus_check(size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols, size_t pow2) {
+  if (chunk == 0) throw Error(1, "bus: chunk_size must be >= 1");
+  if (chunk > 0xFFFFFFFFull || m_in + chunk > pow2) throw Error(1, "bus: time index out of range for ell_n");
+  if (bus_base + bus_cols * chunk > m) throw Error(1, "bus: bus region exceeds the witness");
+}
+
+void bus_decode(const int8_t* Z_host, size_t m, uint64_t* z_out) {
+  hipStream_t st;
+  NS_CHECK(hipStreamCreate(&st));
+  DevBuf dz(D * m), dout(m * 8);
+  NS_CHECK(hipMemcpyAsync(dz.p, Z_host, D * m, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_bus_decode, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, dz.as<int8_t>(), m,
+                     dout.as<uint64_t>());
+  NS_CHECK(hipMemcpyAsync(z_out, dout.p, m * 8, hipMemcpyDeviceToHost, st));
+  NS_CHECK(hipStreamSynchronize(st));
+  NS_CHECK(hipStreamDestroy(st));
+}
+
+void bus_time_cols(const int8_t* Z_host, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                   size_t steps_len, size_t ell_n, uint64_t* idx_out, uint64_t* vals_out, size_t* nnz_out) {
+  if (ell_n > 31) throw Error(1, "bus: ell_n <= 31");
+  bus_check(m, m_in, bus_base, chunk, bus_cols, (size_t)1 << ell_n);
+  if (steps_len > chunk) throw Error(1, "bus: steps_len > chunk_size");
+  if (bus_cols == 0) return;
+  hipStream_t st;
+  NS_CHECK(hipStreamCreate(&st));
+  DevBuf dz(D * m), di(bus_cols * chunk * 4), dv(bus_cols * chunk * sizeof(K2)), dn(bus_cols * 4);
+  NS_CHECK(hipMemcpyAsync(dz.p, Z_host, D * m, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_bus_time_cols, dim3((unsigned)bus_cols), dim3(256), 0, st, dz.as<int8_t>(), m, (uint32_t)m_in,
+                     bus_base, (uint32_t)chunk, (uint32_t)steps_len, di.as<uint32_t>(), dv.as<K2>(), dn.as<uint32_t>());
+  std::vector<uint32_t> hi(bus_cols * chunk), hn(bus_cols);
+  NS_CHECK(hipMemcpyAsync(hi.data(), di.p, hi.size() * 4, hipMemcpyDeviceToHost, st));
+  NS_CHECK(hipMemcpyAsync(vals_out, dv.p, bus_cols * chunk * sizeof(K2), hipMemcpyDeviceToHost, st));
+  NS_CHECK(hipMemcpyAsync(hn.data(), dn.p, bus_cols * 4, hipMemcpyDeviceToHost, st));
+  NS_CHECK(hipStreamSynchronize(st));
+  NS_CHECK(hipStreamDestroy(st));
+  for (size_t c = 0; c < bus_cols; c++) {
+    nnz_out[c] = hn[c];
+    for (size_t j = 0; j < hn[c]; j++) idx_out[c * chunk + j] = hi[c * chunk + j];
+  }
+}
+
+void bus_openings(const int8_t* Z_host, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                  const K2* r, size_t ell_n, K2* y_out, K2* ys_out) {
+  if (ell_n == 0 || ell_n > 31) throw Error(1, "bus openings require 1 <= ell_n <= 31");
+  bus_check(m, m_in, bus_base, chunk, bus_cols, (size_t)1 << ell_n);
+  if (bus_cols == 0) return;
+  hipStream_t st;
+  NS_CHECK(hipStreamCreate(&st));
+  DevBuf dz(D * m), dt(chunk * 4), dr(ell_n * sizeof(K2)), dw(chunk * sizeof(K2)), dy(bus_cols * 64 * sizeof(K2)),
+      dys(bus_cols * sizeof(K2));
+  std::vector<uint32_t> ts(chunk);
+  for (size_t j = 0; j < chunk; j++) ts[j] = (uint32_t)(m_in + j);
+  NS_CHECK(hipMemcpyAsync(dz.p, Z_host, D * m, hipMemcpyHostToDevice, st));
+  NS_CHECK(hipMemcpyAsync(dt.p, ts.data(), chunk * 4, hipMemcpyHostToDevice, st));
+  NS_CHECK(hipMemcpyAsync(dr.p, r, ell_n * sizeof(K2), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_ts_chi_at, dim3((unsigned)((chunk + 255) / 256)), dim3(256), 0, st, dt.as<uint32_t>(),
+                     (uint32_t)chunk, dr.as<K2>(), (int)ell_n, (const K2*)nullptr, dw.as<K2>());
+  hipLaunchKernelGGL(k_bus_open, dim3((unsigned)bus_cols, D), dim3(256), 0, st, dz.as<int8_t>(), m, bus_base,
+                     (uint32_t)chunk, dw.as<K2>(), dy.as<K2>());
+  hipLaunchKernelGGL(k_bus_scalars, dim3((unsigned)((bus_cols + 63) / 64)), dim3(64), 0, st, dy.as<K2>(),
+                     (uint32_t)bus_cols, dys.as<K2>());
+  NS_CHECK(hipMemcpyAsync(y_out, dy.p, bus_cols * 64 * sizeof(K2), hipMemcpyDeviceToHost, st));
+  NS_CHECK(hipMemcpyAsync(ys_out, dys.p, bus_cols * sizeof(K2), hipMemcpyDeviceToHost, st));
+  NS_CHECK(hipStreamSynchronize(st));
+  NS_CHECK(hipStreamDestroy(st));
 }
 
 }  // namespace ns

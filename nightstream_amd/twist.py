@@ -170,3 +170,48 @@ class TsOracle:
                 _bind().ns_ts_free(self.h)
         except Exception:
             pass
+
+
+# ---------------- shared CPU bus (neo-fold/src/memory_sidecar/cpu_bus.rs) ----------------
+def bus_decode(Z) -> np.ndarray:
+    """decode_vector: z[c] = sum_rho 2^rho Z[rho][c] for a [54][m] int8 digit matrix."""
+    Z = np.ascontiguousarray(Z, np.int8)
+    out = np.zeros(Z.shape[1], np.uint64)
+    L = lib()
+    L.ns_bus_decode.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+    _check(L.ns_bus_decode(Z.ctypes.data, Z.shape[1], out.ctypes.data))
+    return out
+
+
+def bus_time_cols(Z, m_in, bus_base, chunk, bus_cols, steps_len, ell_n):
+    """build_time_sparse_from_bus_col for every bus column: [(idx array, [K values])]."""
+    Z = np.ascontiguousarray(Z, np.int8)
+    idx = np.zeros(max(1, bus_cols * chunk), np.uint64)
+    vals = np.zeros(max(2, 2 * bus_cols * chunk), np.uint64)
+    nnz = np.zeros(max(1, bus_cols), np.uint64)
+    L = lib()
+    L.ns_bus_time_cols.argtypes = [C.c_void_p] + [C.c_size_t] * 7 + [C.c_void_p] * 3
+    _check(L.ns_bus_time_cols(Z.ctypes.data, Z.shape[1], m_in, bus_base, chunk, bus_cols, steps_len, ell_n,
+                              idx.ctypes.data, vals.ctypes.data, nnz.ctypes.data))
+    out = []
+    for c in range(bus_cols):
+        n = int(nnz[c])
+        base = c * chunk
+        out.append((idx[base:base + n].copy(),
+                    [(int(vals[2 * (base + i)]), int(vals[2 * (base + i) + 1])) for i in range(n)]))
+    return out
+
+
+def bus_openings(Z, m_in, bus_base, chunk, bus_cols, r):
+    """append_bus_openings_to_me_instance: (y rows padded to 64, y_scalars) per bus column."""
+    Z = np.ascontiguousarray(Z, np.int8)
+    rb = _kbuf(r)
+    y = np.zeros(max(2, bus_cols * 128), np.uint64)
+    ys = np.zeros(max(2, bus_cols * 2), np.uint64)
+    L = lib()
+    L.ns_bus_openings.argtypes = [C.c_void_p] + [C.c_size_t] * 5 + [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+    _check(L.ns_bus_openings(Z.ctypes.data, Z.shape[1], m_in, bus_base, chunk, bus_cols, rb.ctypes.data, len(r),
+                             y.ctypes.data, ys.ctypes.data))
+    yl, ysl = [int(v) for v in y], [int(v) for v in ys]
+    return ([[(yl[2 * (c * 64 + i)], yl[2 * (c * 64 + i) + 1]) for i in range(64)] for c in range(bus_cols)],
+            [(ysl[2 * c], ysl[2 * c + 1]) for c in range(bus_cols)])

@@ -220,6 +220,13 @@ size_t or_ts_num_rounds(const or_ts* o);
 size_t or_ts_degree_bound(const or_ts* o);
 void or_ts_free(or_ts* o);
 
+/* ---------------- shared CPU bus (neo-memory/src/cpu/bus_layout.rs, memory_sidecar/cpu_bus.rs) ---- */
+void or_bus_decode(const int8_t* Z, size_t m, fq* z);
+long or_bus_time_col(const fq* z, size_t zlen, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                     size_t col_id, size_t steps_len, size_t pow2_cycle, uint64_t* idx_out, kx* vals_out);
+int or_bus_openings(const int8_t* Z, size_t m, size_t m_in, size_t bus_base, size_t chunk, size_t bus_cols,
+                    const kx* r, size_t ell_n, kx* y, kx* y_scalars);
+
 #ifdef __cplusplus
 }
 #endif

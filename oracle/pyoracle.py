@@ -428,3 +428,47 @@ class TsOracle:
             _ts_lib().or_ts_free(self.h)
         except Exception:
             pass
+
+
+# ---------------- shared CPU bus (oracle/bus.c) ----------------
+def bus_decode(Z):
+    Z = np.ascontiguousarray(Z, np.int8)
+    m = Z.shape[1]
+    out = np.zeros(m, np.uint64)
+    L = lib()
+    L.or_bus_decode.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+    L.or_bus_decode(Z.ctypes.data, m, out.ctypes.data)
+    return out
+
+
+def bus_time_col(z, m_in, bus_base, chunk, bus_cols, col_id, steps_len, pow2_cycle):
+    z = np.ascontiguousarray(z, np.uint64)
+    idx = np.zeros(chunk + 1, np.uint64)
+    vals = np.zeros(2 * chunk + 2, np.uint64)
+    L = lib()
+    L.or_bus_time_col.restype = C.c_long
+    L.or_bus_time_col.argtypes = [C.c_void_p] + [C.c_size_t] * 8 + [C.c_void_p, C.c_void_p]
+    n = L.or_bus_time_col(z.ctypes.data, len(z), m_in, bus_base, chunk, bus_cols, col_id, steps_len, pow2_cycle,
+                          idx.ctypes.data, vals.ctypes.data)
+    if n < 0:
+        raise ValueError("bus time column: invalid layout")
+    return idx[:n].copy(), [(int(vals[2 * i]), int(vals[2 * i + 1])) for i in range(n)]
+
+
+def bus_openings(Z, m_in, bus_base, chunk, bus_cols, r):
+    Z = np.ascontiguousarray(Z, np.int8)
+    m = Z.shape[1]
+    rb = np.array([v for k in r for v in (int(k[0]), int(k[1]))], np.uint64)
+    y = np.zeros(bus_cols * 64 * 2, np.uint64)
+    ys = np.zeros(bus_cols * 2, np.uint64)
+    L = lib()
+    L.or_bus_openings.restype = C.c_int
+    L.or_bus_openings.argtypes = [C.c_void_p] + [C.c_size_t] * 5 + [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+    rc = L.or_bus_openings(Z.ctypes.data, m, m_in, bus_base, chunk, bus_cols, rb.ctypes.data, len(r), y.ctypes.data,
+                           ys.ctypes.data)
+    if rc != 0:
+        raise ValueError("bus openings: invalid layout")
+    yl = [int(v) for v in y]
+    ysl = [int(v) for v in ys]
+    return ([[(yl[2 * (c * 64 + i)], yl[2 * (c * 64 + i) + 1]) for i in range(64)] for c in range(bus_cols)],
+            [(ysl[2 * c], ysl[2 * c + 1]) for c in range(bus_cols)])

@@ -100,3 +100,28 @@ def test_product_oracle(twist, oracle):
     facs = [[rand_k(rng) for _ in range(1 << 10)] for _ in range(3)]
     g, c = both(twist, oracle, "product", facs, 3)
     drive(g, c, rng, "product")
+
+
+@pytest.mark.parametrize("chunk,bus_cols,steps_len", [(4096, 40, 4096), (4096, 40, 3000), (64, 7, 64)])
+def test_bus_helpers(twist, oracle, chunk, bus_cols, steps_len):
+    """decode / build_time_sparse_from_bus_col / append_bus_openings_to_me_instance on a 2^12-cycle
+    chunk with 40 bus columns (two Twist lanes of 5 address bits, one Twist of 16, one Shout of 16)."""
+    from kmath import D
+    rng = np.random.default_rng(chunk + bus_cols + steps_len)
+    m_in = 1
+    m = bus_cols * chunk + 1024
+    Z = rng.integers(-1, 2, size=(D, m)).astype(np.int8)
+    Z[:, 1024:1024 + chunk // 2] = 0  # a run of zero cells: empty entries in the time columns
+    bus_base = m - bus_cols * chunk
+    ell_n = (m_in + chunk - 1).bit_length()
+    print(f"bus chunk={chunk} cols={bus_cols} steps={steps_len}", flush=True)
+    z = twist.bus_decode(Z)
+    assert np.array_equal(z, oracle.bus_decode(Z))
+    cols = twist.bus_time_cols(Z, m_in, bus_base, chunk, bus_cols, steps_len, ell_n)
+    for c in range(bus_cols):
+        ci, cv = oracle.bus_time_col(z, m_in, bus_base, chunk, bus_cols, c, steps_len, 1 << ell_n)
+        assert np.array_equal(cols[c][0], ci) and cols[c][1] == cv, f"column {c}"
+    r = [rand_k(rng) for _ in range(ell_n)]
+    yg, ysg = twist.bus_openings(Z, m_in, bus_base, chunk, bus_cols, r)
+    yc, ysc = oracle.bus_openings(Z, m_in, bus_base, chunk, bus_cols, r)
+    assert yg == yc and ysg == ysc
