@@ -107,6 +107,13 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 16)
     ap.add_argument("--cpu-steps", type=int, default=2, help="fold steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", choices=["shard", "batch", "serve"], default="shard",
+                    help="shard: one 2^16-row shard (default); batch: 128 shards of 2^10..2^16 rows, plain path vs "
+                         "a pool of concurrent prover contexts; serve: Poisson arrivals with continuous admission")
+    ap.add_argument("--workers", type=int, default=16, help="concurrent prover contexts (batch / serve)")
+    ap.add_argument("--shards", type=int, default=128, help="batch: number of shards")
+    ap.add_argument("--rate-duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--rates", type=str, default="", help="serve: comma-separated arrival rates (jobs/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 plain element-wise copies of 512 MiB (int64 negate: 512 MiB read + 512 MiB "
                          "written each) as the reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -126,6 +133,10 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import torch
 
@@ -206,6 +217,123 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_phases_s": phases,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _stderr(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bench_batch(args):
+    """128 independent shards (rows 2^10..2^16 log-uniform, 4-16 fold steps): the plain path (one
+    shard at a time in one prover context) against --workers concurrent contexts; every proof of
+    the batched run must equal the plain run's."""
+    import torch
+
+    from nightstream_amd.batch import ShardFactory, prove_batched, prove_sequential
+
+    rng = np.random.default_rng(args.seed)
+    fac = ShardFactory(PP_SEED)
+    t0 = time.perf_counter()
+    specs = [(int(rng.integers(10, 17)), int(rng.integers(4, 17))) for _ in range(args.shards)]
+    cache = {}
+    shards = []
+    for i, (e, steps) in enumerate(specs):
+        key = (e, i % 4)  # up to 4 distinct R1CS instances per size
+        if key not in cache:
+            cache[key] = fac.make(1 << e, 1, 1000 + e * 16 + (i % 4))
+        base = cache[key]
+        from nightstream_amd.batch import Shard
+        shards.append(Shard(base.n, steps, base.seed, base.ccs, base.pp, base.z_dev))
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    total_steps = sum(s.steps for s in shards)
+    _stderr(f"batch: {len(shards)} shards, {total_steps} fold steps, {len(cache)} instances, setup {setup:.1f} s")
+    # warm both paths on one small shard (PP limbs, kernel loads)
+    prove_sequential(shards[:1])
+    seq, t_seq = prove_sequential(shards)
+    _stderr(f"batch: plain path {t_seq:.2f} s")
+    bat, t_bat = prove_batched(shards, args.workers)
+    _stderr(f"batch: {args.workers} contexts {t_bat:.2f} s")
+    identical = all(a == b for a, b in zip(seq, bat))
+    line = {
+        "metric": "fold-steps/s, 128-shard batch (2^10..2^16 rows, 4-16 steps)",
+        "value": round(total_steps / t_bat, 3), "unit": "fold-steps/s", "n_gpus": 1,
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "batch", "shards": len(shards), "fold_steps": total_steps, "workers": args.workers,
+                   "sizes_log2": [e for e, _ in specs], "steps": [s for _, s in specs]},
+        "plain_path_fold_steps_per_s": round(total_steps / t_seq, 3),
+        "speedup_vs_plain": round(t_seq / t_bat, 3),
+        "seconds": {"plain": round(t_seq, 3), "batched": round(t_bat, 3), "setup": round(setup, 1)},
+        "proofs_bit_identical": identical,
+    }
+    print(json.dumps(line), flush=True)
+    if not identical:
+        sys.exit("batched proofs differ from the plain path")
+
+
+def bench_serve(args):
+    """Open-loop Poisson arrivals; jobs of 2^12..2^18 rows (log-uniform) with 1-32 fold steps;
+    admission by free worker and by estimated HBM footprint under a 240 GB cap.  For each rate:
+    fold-steps/s and p50/p99 latency, for the pooled server and for the plain single-context
+    server; the sweep stops once the pooled p99 exceeds 5 s."""
+    import torch
+
+    from nightstream_amd.batch import Shard, ShardFactory, pp_bytes, serve
+
+    rng = np.random.default_rng(args.seed)
+    fac = ShardFactory(PP_SEED)
+    t0 = time.perf_counter()
+    base = {}
+    for e in range(12, 19):
+        for v in range(2):
+            base[(e, v)] = fac.make(1 << e, 1, 2000 + e * 4 + v)
+    torch.cuda.synchronize()
+    jobs = []
+    for i in range(4096):
+        e = int(rng.integers(12, 19))
+        b = base[(e, i % 2)]
+        jobs.append(Shard(b.n, int(rng.integers(1, 33)), b.seed, b.ccs, b.pp, b.z_dev))
+    pp_res = float(sum(pp_bytes(1 << e) for e in range(12, 19)))
+    _stderr(f"serve: pool of {len(base)} instances ready in {time.perf_counter() - t0:.1f} s")
+    # reference proofs (alone) for the first jobs, to compare against the mixed runs
+    solo = {}
+    rates = [float(r) for r in args.rates.split(",")] if args.rates else [5.0, 10.0, 20.0, 40.0, 80.0]
+    out = []
+    for rate in rates:
+        rec = {"rate_jobs_per_s": rate}
+        for label, workers in (("pooled", args.workers), ("plain", 1)):
+            res = serve(jobs, rate, args.rate_duration, workers, 240e9, seed=int(rate * 1000), pp_resident=pp_res)
+            if not res:
+                continue
+            lat = np.array([r.latency for r in res])
+            span = max(r.finish for r in res) - min(r.arrival for r in res)
+            steps = sum(r.steps for r in res)
+            rec[label] = {"jobs": len(res), "fold_steps_per_s": round(steps / span, 3),
+                          "p50_s": round(float(np.percentile(lat, 50)), 3),
+                          "p99_s": round(float(np.percentile(lat, 99)), 3)}
+            if label == "pooled":
+                for r in res[:8]:
+                    solo.setdefault(r.job, r.proof)
+            _stderr(f"serve: rate {rate} {label}: {rec[label]}")
+            if label == "pooled" and rec[label]["p99_s"] > 5.0:
+                break
+        out.append(rec)
+        if rec.get("pooled", {}).get("p99_s", 0) > 5.0:
+            break
+    # each mixed result equals the same job proved alone
+    from nightstream_amd.batch import ProverPool
+    alone = ProverPool(1)
+    same = all(alone.prove(jobs[j]) == p for j, p in solo.items())
+    line = {
+        "metric": "served fold-steps/s and p50/p99 job latency (Poisson arrivals, 2^12..2^18 rows, 1-32 steps)",
+        "value": max((r["pooled"]["fold_steps_per_s"] for r in out if "pooled" in r), default=None),
+        "unit": "fold-steps/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "serve", "workers": args.workers, "rate_duration_s": args.rate_duration,
+                   "hbm_cap_bytes": 240e9},
+        "rates": out, "mixed_equals_alone": same,
     }
     print(json.dumps(line), flush=True)
 

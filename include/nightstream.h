@@ -103,6 +103,10 @@ int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* out
  * holds the canonical encoding (little-endian u64 words, nightstream_amd/proof.py); free it
  * with ns_free. */
 int ns_prover_new(const ns_ccs* s, const ns_pp* pp, ns_prover** out);
+/* Point an existing prover at another CCS / PP pair, keeping its device buffers (grown on
+ * demand): a serving worker proves job after job without reallocating.  No reference
+ * counterpart (the reference builds a FoldingSession per shard, session.rs:668-1953). */
+int ns_prover_rebind(ns_prover* p, const ns_ccs* s, const ns_pp* pp);
 int ns_prove_shard(ns_prover* p, const uint64_t* const* z_steps, size_t n_steps, size_t m_in, uint8_t** proof,
                    size_t* proof_len, double* seconds);
 /* Same, with z_steps_dev[s] device pointers (inputs already resident in HBM). */

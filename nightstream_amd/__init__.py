@@ -23,7 +23,7 @@ EXPORTS = [
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
-    "ns_dec_split", "ns_prover_new", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
+    "ns_dec_split", "ns_prover_new", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_free",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
     "ns_ts_twist_read_new", "ns_ts_twist_write_new", "ns_ts_val_eval_new", "ns_ts_total_inc_new",
@@ -80,6 +80,7 @@ def lib():
     L.ns_rlc_zmix.argtypes = [vp, vp, sz, sz, vp]
     L.ns_dec_split.argtypes = [vp, sz, sz, vp, vp]
     L.ns_prover_new.argtypes = [vp, vp, C.POINTER(vp)]
+    L.ns_prover_rebind.argtypes = [vp, vp, vp]
     L.ns_prove_shard.argtypes = [vp, vp, sz, sz, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
                                  C.POINTER(C.c_double)]
     L.ns_prove_shard_device.argtypes = L.ns_prove_shard.argtypes
@@ -302,6 +303,12 @@ class FoldingSession:
         lib().ns_free(C.cast(out, C.c_void_p))
         self.last_seconds = sec.value
         return data
+
+    def rebind(self, ccs: "Ccs", pp: "PublicParams"):
+        """Reuse this session's device buffers for another CCS / PP (same semantics as a new session)."""
+        _check(lib().ns_prover_rebind(self.h, ccs.h, pp.h))
+        self.ccs, self.pp = ccs, pp
+        self.steps, self.m_in = [], None
 
     def set_timing(self, on: bool):
         _check(lib().ns_prover_set_timing(self.h, int(on)))

@@ -1,0 +1,244 @@
+"""Batched and served proving: many independent shards on one GPU.
+
+A shard's fold steps are sequential (each step's accumulator feeds the next), and each step's
+Fiat-Shamir transcript is host work of roughly constant size (about 17.8k Poseidon2
+permutations, independent of n).  Throughput across shards therefore comes from running many
+sessions at once: every worker thread owns one prover context (its own HIP stream and device
+buffers, reused from job to job through `rebind`), drives its shard's transcript on its own core,
+and its kernels overlap with the other workers' on the GPU.  ctypes releases the GIL for the
+whole `prove` call, so the Python layer only dispatches.
+
+  prove_sequential(shards)          the plain path: one shard at a time, one prover context
+  prove_batched(shards, workers)    the same shards through a pool of `workers` contexts
+  serve(jobs, rate, ...)            open-loop Poisson arrivals with continuous admission by slot
+                                    and by estimated HBM footprint; per-job latency
+Every proof is the same byte string whichever path produced it (each shard has its own
+transcript and the kernels are deterministic).
+"""
+from __future__ import annotations
+
+import dataclasses
+import queue
+import threading
+import time
+
+import numpy as np
+
+from .workload import Q, R1CS, rand_balanced_z, rand_field
+
+D = 54
+
+
+def gen_r1cs_fast(n: int, seed: int):
+    """The R1CS generator's distribution (A, B: 3 distinct uniform columns per row with uniform
+    nonzero coefficients; C: one column j != 0 per row with coefficient (Az)_i (Bz)_i / z_j;
+    z[0] = 1, other entries uniform in |z| <= 2^54 - 1), vectorised for large batches."""
+    rng = np.random.default_rng(seed)
+    m = n
+    z = rand_balanced_z(rng, m)
+    zo = z.astype(object)
+
+    def rows3():
+        cols = rng.integers(0, m, size=(n, 3))
+        while True:
+            dup = (cols[:, 0] == cols[:, 1]) | (cols[:, 0] == cols[:, 2]) | (cols[:, 1] == cols[:, 2])
+            if not dup.any():
+                break
+            cols[dup] = rng.integers(0, m, size=(int(dup.sum()), 3))
+        cols.sort(axis=1)
+        vals = rand_field(rng, 3 * n).reshape(n, 3)
+        return cols.astype(np.uint32), vals
+
+    a_cols, a_vals = rows3()
+    b_cols, b_vals = rows3()
+    az = (a_vals.astype(object) * zo[a_cols.astype(np.int64)]).sum(axis=1) % Q
+    bz = (b_vals.astype(object) * zo[b_cols.astype(np.int64)]).sum(axis=1) % Q
+    jj = rng.integers(1, m, size=n)
+    zero = z[jj] == 0
+    while zero.any():
+        jj[zero] = 1 + (jj[zero] % (m - 1))
+        zero = z[jj] == 0
+    inv = np.array([pow(int(v), Q - 2, Q) for v in z[jj]], dtype=object)
+    c_vals = np.array((az * bz % Q) * inv % Q, dtype=np.uint64)
+    rp3 = np.arange(0, 3 * n + 1, 3, dtype=np.uint64)
+    rp1 = np.arange(0, n + 1, dtype=np.uint64)
+    a = (rp3, a_cols.reshape(-1).copy(), a_vals.reshape(-1).copy())
+    b = (rp3, b_cols.reshape(-1).copy(), b_vals.reshape(-1).copy())
+    c = (rp1, jj.astype(np.uint32), c_vals)
+    return R1CS(n, m, a, b, c, z), z
+
+
+def footprint_bytes(n: int, k: int = 13) -> int:
+    """HBM a proving job needs beyond the shared PP: digit planes of k witnesses and 2 x 16 child
+    planes, the round-1 NC tables (double-buffered), Z_mix and the scalar tables."""
+    m = n
+    return int(k * D * m + 2 * 16 * D * m + 2 * (k * D + 6) * 16 * (n // 2) + 4 * D * m + 8 * 16 * n)
+
+
+def pp_bytes(m: int) -> int:
+    return 6912 * m
+
+
+@dataclasses.dataclass
+class Shard:
+    n: int
+    steps: int
+    seed: int
+    ccs: object = None
+    pp: object = None
+    z_dev: object = None  # device tensor holding z (u64[m])
+
+    def z_ptrs(self):
+        return [self.z_dev.data_ptr()] * self.steps
+
+
+class ShardFactory:
+    """Builds shards on the device, sharing one Ajtai PP per width m."""
+
+    def __init__(self, pp_seed: bytes, device="cuda:0"):
+        import torch
+
+        import nightstream_amd as ns
+        self.ns, self.torch = ns, torch
+        self.pp_seed = pp_seed
+        self.device = device
+        self.pps = {}
+
+    def pp(self, m):
+        if m not in self.pps:
+            self.pps[m] = self.ns.PublicParams(m, seed=self.pp_seed)
+        return self.pps[m]
+
+    def make(self, n: int, steps: int, seed: int) -> Shard:
+        inst, z = gen_r1cs_fast(n, seed)
+        sh = Shard(n, steps, seed)
+        sh.ccs = self.ns.Ccs(inst)
+        sh.pp = self.pp(n)
+        sh.z_dev = self.torch.from_numpy(z.view(np.int64).copy()).to(self.device)
+        return sh
+
+
+class ProverPool:
+    """`workers` prover contexts, each driven by its own thread."""
+
+    def __init__(self, workers: int):
+        self.workers = workers
+        self._local = threading.local()
+        self._all = []
+        self._lock = threading.Lock()
+
+    def session(self, shard: Shard):
+        import nightstream_amd as ns
+        s = getattr(self._local, "s", None)
+        if s is None:
+            s = ns.FoldingSession(shard.ccs, shard.pp)
+            self._local.s = s
+            with self._lock:
+                self._all.append(s)
+        else:
+            s.rebind(shard.ccs, shard.pp)
+        return s
+
+    def prove(self, shard: Shard) -> bytes:
+        return self.session(shard).prove_device(shard.z_ptrs())
+
+
+def prove_sequential(shards):
+    """The plain per-shard path: one context, one shard at a time."""
+    pool = ProverPool(1)
+    out = []
+    t0 = time.perf_counter()
+    for sh in shards:
+        out.append(pool.prove(sh))
+    return out, time.perf_counter() - t0
+
+
+def prove_batched(shards, workers: int):
+    """All shards through `workers` concurrent contexts (largest first to balance the tail)."""
+    import concurrent.futures as cf
+    pool = ProverPool(workers)
+    order = sorted(range(len(shards)), key=lambda i: -shards[i].n * shards[i].steps)
+    out = [None] * len(shards)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = {ex.submit(pool.prove, shards[i]): i for i in order}
+        for f in cf.as_completed(futs):
+            out[futs[f]] = f.result()
+    return out, time.perf_counter() - t0
+
+
+@dataclasses.dataclass
+class JobResult:
+    job: int
+    n: int
+    steps: int
+    arrival: float
+    start: float
+    finish: float
+    proof: bytes
+
+    @property
+    def latency(self):
+        return self.finish - self.arrival
+
+
+def serve(jobs, rate: float, duration: float, workers: int, hbm_cap: float, seed: int = 0, pp_resident: float = 0.0):
+    """Open-loop Poisson arrivals at `rate` jobs/s for `duration` seconds, drawing jobs in order
+    from `jobs` (a list of Shard, cycled).  A job starts when a worker is free and the summed
+    footprint of running jobs plus its own stays within `hbm_cap` bytes (continuous admission:
+    finished jobs free their slot immediately).  Returns the completed JobResult list; jobs that
+    arrive after `duration` are not generated, and every admitted job runs to completion."""
+    rng = np.random.default_rng(seed)
+    arrivals = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        arrivals.append(t)
+    pool = ProverPool(workers)
+    pending = queue.Queue()
+    results = []
+    res_lock = threading.Lock()
+    cond = threading.Condition()
+    state = {"mem": pp_resident, "free": workers}
+    t0 = time.perf_counter()
+
+    def worker():
+        while True:
+            item = pending.get()
+            if item is None:
+                return
+            idx, sh, arr, fp = item
+            start = time.perf_counter() - t0
+            proof = pool.prove(sh)
+            fin = time.perf_counter() - t0
+            with res_lock:
+                results.append(JobResult(idx, sh.n, sh.steps, arr, start, fin, proof))
+            with cond:
+                state["mem"] -= fp
+                state["free"] += 1
+                cond.notify_all()
+
+    threads = [threading.Thread(target=worker, daemon=True) for _ in range(workers)]
+    for th in threads:
+        th.start()
+    for i, arr in enumerate(arrivals):
+        now = time.perf_counter() - t0
+        if arr > now:
+            time.sleep(arr - now)
+        sh = jobs[i % len(jobs)]
+        fp = footprint_bytes(sh.n)
+        with cond:
+            # FIFO admission: wait for a slot and for HBM headroom
+            while state["free"] == 0 or state["mem"] + fp > hbm_cap:
+                cond.wait()
+            state["free"] -= 1
+            state["mem"] += fp
+        pending.put((i, sh, arr, fp))
+    for _ in threads:
+        pending.put(None)
+    for th in threads:
+        th.join()
+    results.sort(key=lambda r: r.job)
+    return results

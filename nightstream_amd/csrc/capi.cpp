@@ -346,6 +346,13 @@ int ns_prover_new(const ns_ccs* s, const ns_pp* pp, ns_prover** out) {
   });
 }
 
+int ns_prover_rebind(ns_prover* p, const ns_ccs* s, const ns_pp* pp) {
+  return guard([&] {
+    if (!p || !s || !pp) throw Error(1, "null handle");
+    p->p->rebind(*s->s, *pp->pp);
+  });
+}
+
 int ns_prove_shard(ns_prover* p, const uint64_t* const* z_steps, size_t n_steps, size_t m_in, uint8_t** proof,
                    size_t* proof_len, double* seconds) {
   return guard([&] {

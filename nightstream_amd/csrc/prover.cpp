@@ -691,20 +691,28 @@ struct Enc {
 };
 
 // ---------------- prover ----------------
-Prover::Prover(const Ccs& s, const Pp& pp) : s_(s), pp_(pp) {
+Prover::Prover(const Ccs& s, const Pp& pp) : s_(&s), pp_(&pp) {
   if (pp.m != s.m) throw Error(1, "PP width does not match CCS m");
   params_ = Params::goldilocks_auto_r1cs_ccs(s.n);
   NS_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 }
 
+void Prover::rebind(const Ccs& s, const Pp& pp) {
+  if (pp.m != s.m) throw Error(1, "PP width does not match CCS m");
+  NS_CHECK(hipStreamSynchronize(stream_));
+  s_ = &s;
+  pp_ = &pp;
+  params_ = Params::goldilocks_auto_r1cs_ccs(s.n);
+}
+
 Prover::~Prover() { (void)hipStreamDestroy(stream_); }
 
 void Prover::commit_device(const int8_t* const* Zs_dev, size_t nw, uint64_t* host_out) {
-  ws_ajtai_.ensure(ajtai_workspace_bytes(s_.m, nw));
+  ws_ajtai_.ensure(ajtai_workspace_bytes(s_->m, nw));
   ptrs_.ensure(nw * sizeof(void*));
   cout_.ensure(nw * KAPPA * D * 8);
   h2d(ptrs_.p, (const void* const*)Zs_dev, nw, stream_);
-  ajtai_commit(pp_.limbs.as<int8_t>(), KAPPA, s_.m, ptrs_.as<const int8_t*>(), nw, cout_.as<uint64_t>(),
+  ajtai_commit(pp_->limbs.as<int8_t>(), KAPPA, s_->m, ptrs_.as<const int8_t*>(), nw, cout_.as<uint64_t>(),
                ws_ajtai_.p, stream_);
   d2h(host_out, cout_.p, nw * KAPPA * D, stream_);
 }
@@ -726,7 +734,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   struct Reset {
     ~Reset() { g_kt = nullptr; }
   } reset_guard;
-  const Ccs& s = s_;
+  const Ccs& s = *s_;
   const size_t m = s.m, t = s.t, k_dec = params_.k_rho;
   const size_t ell_d = 6, ell_n = log2_pow2(s.n), ell = ell_d + ell_n;
   const size_t d_sc = std::max<size_t>(s.f_degree() + 1, std::max<size_t>(2, 2 * params_.b + 2));

@@ -132,13 +132,15 @@ class Prover {
   // commit helpers exposed to the C ABI
   void commit_device(const int8_t* const* Zs_host_ptrs_dev, size_t nw, uint64_t* host_out);
   hipStream_t stream() const { return stream_; }
-  const Ccs& ccs() const { return s_; }
-  const Pp& pp() const { return pp_; }
+  const Ccs& ccs() const { return *s_; }
+  const Pp& pp() const { return *pp_; }
+  // Reuse this prover's device buffers (grown on demand) for another CCS / PP pair.
+  void rebind(const Ccs& s, const Pp& pp);
   DevBuf& ws_ajtai() { return ws_ajtai_; }
 
  private:
-  const Ccs& s_;
-  const Pp& pp_;
+  const Ccs* s_;
+  const Pp* pp_;
   Params params_;
   hipStream_t stream_;
   DevBuf ws_ajtai_, ws_round_, ws_yd_;
