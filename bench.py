@@ -138,31 +138,28 @@ def main():
     if args.workload == "serve":
         return bench_serve(args)
 
-    import torch
-
     import nightstream_amd as ns
     from nightstream_amd.workload import gen_r1cs
 
-    if not torch.cuda.is_available():
+    if ns.device_count() < 1:
         sys.exit("no GPU visible")
-    dev = torch.device("cuda:0")
     t_setup = time.perf_counter()
     inst, zs = gen_r1cs(args.n, args.seed, 1)
     z = zs[0]
     ccs = ns.Ccs(inst)
     pp = ns.PublicParams(args.n, seed=PP_SEED)
     sess = ns.FoldingSession(ccs, pp)
-    z_dev = torch.from_numpy(z.view(np.int64).copy()).to(dev)
-    torch.cuda.synchronize()
+    z_dev = ns.DeviceBuffer.from_array(z)
+    ns.device_synchronize()
     setup_s = time.perf_counter() - t_setup
 
     if args.warmup > 0:
         sess.prove_device([z_dev.data_ptr()] * args.warmup)
-    torch.cuda.synchronize()
+    ns.device_synchronize()
     sess.set_timing(True)
     t0 = time.perf_counter()
     proof = sess.prove_device([z_dev.data_ptr()] * args.steps)
-    torch.cuda.synchronize()
+    ns.device_synchronize()
     t1 = time.perf_counter()
     sess.set_timing(False)
     elapsed = t1 - t0
@@ -229,8 +226,7 @@ def bench_batch(args):
     """128 independent shards (rows 2^10..2^16 log-uniform, 4-16 fold steps): the plain path (one
     shard at a time in one prover context) against --workers concurrent contexts; every proof of
     the batched run must equal the plain run's."""
-    import torch
-
+    import nightstream_amd as ns
     from nightstream_amd.batch import ShardFactory, prove_batched, prove_sequential
 
     rng = np.random.default_rng(args.seed)
@@ -246,7 +242,7 @@ def bench_batch(args):
         base = cache[key]
         from nightstream_amd.batch import Shard
         shards.append(Shard(base.n, steps, base.seed, base.ccs, base.pp, base.z_dev))
-    torch.cuda.synchronize()
+    ns.device_synchronize()
     setup = time.perf_counter() - t0
     total_steps = sum(s.steps for s in shards)
     _stderr(f"batch: {len(shards)} shards, {total_steps} fold steps, {len(cache)} instances, setup {setup:.1f} s")
@@ -278,8 +274,7 @@ def bench_serve(args):
     admission by free worker and by estimated HBM footprint under a 240 GB cap.  For each rate:
     fold-steps/s and p50/p99 latency, for the pooled server and for the plain single-context
     server; the sweep stops once the pooled p99 exceeds 5 s."""
-    import torch
-
+    import nightstream_amd as ns
     from nightstream_amd.batch import Shard, ShardFactory, pp_bytes, serve
 
     rng = np.random.default_rng(args.seed)
@@ -289,7 +284,7 @@ def bench_serve(args):
     for e in range(12, 19):
         for v in range(2):
             base[(e, v)] = fac.make(1 << e, 1, 2000 + e * 4 + v)
-    torch.cuda.synchronize()
+    ns.device_synchronize()
     jobs = []
     for i in range(4096):
         e = int(rng.integers(12, 19))

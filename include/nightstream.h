@@ -43,6 +43,13 @@ const char* ns_last_error(void);
 int ns_device_count(void);
 /* Free a buffer returned by this library. */
 void ns_free(void* p);
+/* Device memory on this library's HIP runtime, for callers that keep witnesses resident in HBM
+ * (ns_prove_shard_device).  Runtime plumbing with no reference counterpart. */
+int ns_device_alloc(size_t bytes, void** out);
+int ns_device_free(void* p);
+int ns_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int ns_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int ns_device_synchronize(void);
 
 /* ---- parameters: NeoParams::goldilocks_auto_r1cs_ccs (neo-params/src/lib.rs:186-222) ----
  * out[0..10] = q, eta, d, kappa, m, b, k_rho, B, T, s, lambda */

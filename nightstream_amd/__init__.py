@@ -19,7 +19,8 @@ D = 54
 KAPPA = 16
 
 EXPORTS = [
-    "ns_last_error", "ns_device_count", "ns_free", "ns_params_goldilocks_auto_r1cs_ccs", "ns_ccs_from_r1cs",
+    "ns_last_error", "ns_device_count", "ns_free", "ns_device_alloc", "ns_device_free", "ns_memcpy_h2d",
+    "ns_memcpy_d2h", "ns_device_synchronize", "ns_params_goldilocks_auto_r1cs_ccs", "ns_ccs_from_r1cs",
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
@@ -81,6 +82,10 @@ def lib():
     L.ns_dec_split.argtypes = [vp, sz, sz, vp, vp]
     L.ns_prover_new.argtypes = [vp, vp, C.POINTER(vp)]
     L.ns_prover_rebind.argtypes = [vp, vp, vp]
+    L.ns_device_alloc.argtypes = [sz, C.POINTER(vp)]
+    L.ns_device_free.argtypes = [vp]
+    L.ns_memcpy_h2d.argtypes = [vp, vp, sz]
+    L.ns_memcpy_d2h.argtypes = [vp, vp, sz]
     L.ns_prove_shard.argtypes = [vp, vp, sz, sz, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
                                  C.POINTER(C.c_double)]
     L.ns_prove_shard_device.argtypes = L.ns_prove_shard.argtypes
@@ -97,6 +102,40 @@ def _check(rc: int):
         raise NsError(rc, lib().ns_last_error().decode(errors="replace"))
 
 
+class DeviceBuffer:
+    """HBM allocation on the library's own HIP runtime (freed with the object)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        _check(lib().ns_device_alloc(nbytes, C.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _check(lib().ns_memcpy_h2d(b.ptr, a.ctypes.data, a.nbytes))
+        return b
+
+    def to_array(self, dtype, count) -> np.ndarray:
+        out = np.empty(count, dtype)
+        _check(lib().ns_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                lib().ns_device_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -107,6 +146,10 @@ def _ptr_array(arrays):
 
 def device_count() -> int:
     return lib().ns_device_count()
+
+
+def device_synchronize():
+    _check(lib().ns_device_synchronize())
 
 
 def params_goldilocks_auto_r1cs_ccs(n_rows: int) -> dict:

@@ -86,7 +86,7 @@ class Shard:
     seed: int
     ccs: object = None
     pp: object = None
-    z_dev: object = None  # device tensor holding z (u64[m])
+    z_dev: object = None  # DeviceBuffer holding z (u64[m])
 
     def z_ptrs(self):
         return [self.z_dev.data_ptr()] * self.steps
@@ -95,13 +95,10 @@ class Shard:
 class ShardFactory:
     """Builds shards on the device, sharing one Ajtai PP per width m."""
 
-    def __init__(self, pp_seed: bytes, device="cuda:0"):
-        import torch
-
+    def __init__(self, pp_seed: bytes):
         import nightstream_amd as ns
-        self.ns, self.torch = ns, torch
+        self.ns = ns
         self.pp_seed = pp_seed
-        self.device = device
         self.pps = {}
 
     def pp(self, m):
@@ -114,7 +111,7 @@ class ShardFactory:
         sh = Shard(n, steps, seed)
         sh.ccs = self.ns.Ccs(inst)
         sh.pp = self.pp(n)
-        sh.z_dev = self.torch.from_numpy(z.view(np.int64).copy()).to(self.device)
+        sh.z_dev = self.ns.DeviceBuffer.from_array(z)
         return sh
 
 

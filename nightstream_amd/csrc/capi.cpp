@@ -626,3 +626,31 @@ extern "C" int ns_bus_openings(const int8_t* Z, size_t m, size_t m_in, size_t bu
     }
   });
 }
+
+// ---------------- device memory ----------------
+extern "C" int ns_device_alloc(size_t bytes, void** out) {
+  return guard([&] {
+    require_device();
+    if (!out) throw Error(1, "null output pointer");
+    *out = nullptr;
+    NS_CHECK(hipMalloc(out, bytes ? bytes : 1));
+  });
+}
+
+extern "C" int ns_device_free(void* p) {
+  return guard([&] {
+    if (p) NS_CHECK(hipFree(p));
+  });
+}
+
+extern "C" int ns_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes) {
+  return guard([&] { NS_CHECK(hipMemcpy(dst_dev, src_host, bytes, hipMemcpyHostToDevice)); });
+}
+
+extern "C" int ns_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes) {
+  return guard([&] { NS_CHECK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost)); });
+}
+
+extern "C" int ns_device_synchronize(void) {
+  return guard([&] { NS_CHECK(hipDeviceSynchronize()); });
+}
