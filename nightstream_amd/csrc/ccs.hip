@@ -596,4 +596,18 @@ void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const*
                   outs_dev, flags);
 }
 
+// ---------------- head columns (ME public inputs X) ----------------
+__global__ void k_head_cols(PlanePtrs planes, int np, size_t m, int m_in, int8_t* __restrict__ out) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= np * D * m_in) return;
+  int p = idx / (D * m_in), r = (idx / m_in) % D, c = idx % m_in;
+  out[idx] = planes.p[p][(size_t)r * m + c];
+}
+
+void launch_head_cols(const PlanePtrs& planes, int np, size_t m, int m_in, int8_t* out, hipStream_t s) {
+  int n = np * D * m_in;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_head_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, planes, np, m, m_in, out);
+}
+
 }  // namespace ns

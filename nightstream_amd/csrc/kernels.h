@@ -124,6 +124,11 @@ void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t
 void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
                      hipStream_t s);
 // balanced base-2 digits of zmix into k_dec planes; flags[i] |= plane i nonzero; flags[k_dec] = range error
+// Gather the first m_in columns of up to 16 digit planes [54][m]: out[(p*54 + rho)*m_in + c].
+struct PlanePtrs {
+  const int8_t* p[16];
+};
+void launch_head_cols(const PlanePtrs& planes, int np, size_t m, int m_in, int8_t* out, hipStream_t s);
 void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const* outs_dev, int* flags,
                       hipStream_t s);
 

@@ -34,10 +34,37 @@ template <class T>
 static void h2d(void* dst, const T* src, size_t n, hipStream_t s) {
   NS_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
+// Device-to-host copies land in a per-thread pinned staging buffer (a direct DMA, no runtime
+// staging through pageable memory) and are then copied to the caller's memory.
+namespace {
+struct PinnedStage {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t n) {
+    if (n > bytes) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      size_t want = n < ((size_t)1 << 20) ? ((size_t)1 << 20) : n;
+      NS_CHECK(hipHostMalloc(&p, want, hipHostMallocDefault));
+      bytes = want;
+    }
+    return p;
+  }
+  ~PinnedStage() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+thread_local PinnedStage g_stage;
+}  // namespace
+
 template <class T>
 static void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
-  NS_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  const size_t bytes = n * sizeof(T);
+  if (bytes == 0) return;
+  void* stage = g_stage.get(bytes);
+  NS_CHECK(hipMemcpyAsync(stage, src, bytes, hipMemcpyDeviceToHost, s));
   NS_CHECK(hipStreamSynchronize(s));
+  memcpy(dst, stage, bytes);
 }
 
 // ---------------- kernel timers ----------------
@@ -984,13 +1011,19 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
 
     phase_mark(5);
     // ME outputs
-    auto head_cols = [&](const int8_t* Zdev, std::vector<uint64_t>& X) {
-      X.assign((size_t)D * m_in, 0);
-      if (!m_in) return;
-      std::vector<int8_t> tmp((size_t)D * m_in);
-      NS_CHECK(hipMemcpy2DAsync(tmp.data(), m_in, Zdev, m, m_in, D, hipMemcpyDeviceToHost, st));
-      NS_CHECK(hipStreamSynchronize(st));
-      for (size_t i = 0; i < tmp.size(); i++) X[i] = from_i64(tmp[i]);
+    // X = the first m_in columns of each digit plane, gathered on the device in one launch
+    auto head_cols = [&](const int8_t* const* planes, size_t np, std::vector<std::vector<uint64_t>*> Xs) {
+      for (size_t p = 0; p < np; p++) Xs[p]->assign((size_t)D * m_in, 0);
+      if (!m_in || !np) return;
+      if (np > 16) throw Error(1, "at most 16 planes per head-column gather");
+      PlanePtrs pl{};
+      for (size_t p = 0; p < np; p++) pl.p[p] = planes[p];
+      head_.ensure(np * D * m_in);
+      launch_head_cols(pl, (int)np, m, (int)m_in, head_.as<int8_t>(), st);
+      std::vector<int8_t> tmp(np * D * m_in);
+      d2h(tmp.data(), head_.p, tmp.size(), st);
+      for (size_t p = 0; p < np; p++)
+        for (size_t i = 0; i < (size_t)D * m_in; i++) (*Xs[p])[i] = from_i64(tmp[p * D * m_in + i]);
     };
     std::vector<MeHost> ccs_out(kt);
     for (size_t i = 0; i < kt; i++) {
@@ -999,7 +1032,8 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       e.t = t;
       if (i == 0) {
         memcpy(e.c, &cmcs[idx * KAPPA * D], sizeof(e.c));
-        head_cols(Zmcs[idx].as<int8_t>(), e.X);
+        const int8_t* pl0 = Zmcs[idx].as<int8_t>();
+        head_cols(&pl0, 1, {&e.X});
       } else {
         memcpy(e.c, acc[i - 1].c, sizeof(e.c));
         e.X = acc[i - 1].X;
@@ -1109,12 +1143,34 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
             for (size_t c = 0; c < m_in; c++)
               parent.X[rr * m_in + c] = fadd(parent.X[rr * m_in + c], fmul(co, ccs_out[i].X[kk * m_in + c]));
           }
-        for (size_t j = 0; j < t; j++)
-          for (int rr = 0; rr < D; rr++) {
-            K2 a = kzero();
-            for (int kk = 0; kk < D; kk++) a = kadd(a, kscale(ccs_out[i].y[j * DPAD + kk], R[rr * D + kk]));
-            parent.y[j * DPAD + rr] = kadd(parent.y[j * DPAD + rr], a);
+      }
+      // y_j = sum_i rot(a_i) y_ij: rot(a) v is a(X) v(X) mod Phi_81, so accumulate the products of
+      // the small coefficients a_i in [-2, 2] with y_ij as signed additions and reduce once.
+      for (size_t j = 0; j < t; j++) {
+        K2 tmp[2 * D - 1];
+        for (auto& x : tmp) x = kzero();
+        for (size_t i = 0; i < kt; i++) {
+          const int8_t* a = &rho_a[i * D];
+          const K2* y = &ccs_out[i].y[j * DPAD];
+          K2 y2[D];
+          for (int v = 0; v < D; v++) y2[v] = kadd(y[v], y[v]);
+          for (int u = 0; u < D; u++) {
+            const int au = a[u];
+            if (!au) continue;
+            const K2* src = (au == 1 || au == -1) ? y : y2;
+            if (au > 0)
+              for (int v = 0; v < D; v++) tmp[u + v] = kadd(tmp[u + v], src[v]);
+            else
+              for (int v = 0; v < D; v++) tmp[u + v] = ksub(tmp[u + v], src[v]);
           }
+        }
+        for (int q = 0; q < D; q++) {
+          K2 c = tmp[q];
+          if (q <= 25) c = kadd(c, tmp[q + 81]);
+          if (q >= 27) c = ksub(c, tmp[q + 27]);
+          if (q <= 26) c = ksub(c, tmp[q + 54]);
+          parent.y[j * DPAD + q] = c;
+        }
       }
       // commitment mix: sum_i rot(rho_i) * c_i (s_lincomb with the rotation's first column)
       memset(parent.c, 0, sizeof(parent.c));
@@ -1185,13 +1241,17 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       d2h(y_child.data(), ydev_.p, y_child.size(), st);
     }
     std::vector<MeHost> children(k_dec);
+    {
+      std::vector<std::vector<uint64_t>*> xs;
+      for (auto& c : children) xs.push_back(&c.X);
+      head_cols(pptr.data(), k_dec, xs);
+    }
     for (size_t i = 0; i < k_dec; i++) {
       MeHost& e = children[i];
       if (fl[i]) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
       else memset(e.c, 0, sizeof(e.c));
       e.m_in = m_in;
       e.t = t;
-      head_cols(pptr[i], e.X);
       e.r = parent.r;
       e.y.assign(t * DPAD, kzero());
       e.y_scalars.assign(t, kzero());

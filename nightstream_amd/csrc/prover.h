@@ -146,7 +146,7 @@ class Prover {
   DevBuf ws_ajtai_, ws_round_, ws_yd_;
   // buffers reused across fold steps
   OracleWs ows_;
-  DevBuf planes_[2][16], zmix_, rho_dev_, zptr_, pptr_dev_, flags_, ydev_, ptrs_, cout_, zbuf_, vjs_;
+  DevBuf planes_[2][16], zmix_, rho_dev_, zptr_, pptr_dev_, flags_, ydev_, ptrs_, cout_, zbuf_, vjs_, head_;
   std::vector<DevBuf> zmcs_;
 };
 
