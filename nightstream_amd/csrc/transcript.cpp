@@ -2,6 +2,7 @@
 // See transcript.h for the construction and its parity status.
 #include "transcript.h"
 
+#include <immintrin.h>
 #include <string.h>
 
 namespace ns {
@@ -101,6 +102,8 @@ static uint64_t sample_fq(ChaCha8& rng) {
   }
 }
 
+__attribute__((target("avx512f"))) static void permute8_avx512(const Poseidon2& P, uint64_t* st);
+
 void Poseidon2::init(int w, ChaCha8& rng) {
   width = w;
   memset(ext_init, 0, sizeof(ext_init));
@@ -112,6 +115,29 @@ void Poseidon2::init(int w, ChaCha8& rng) {
     for (int i = 0; i < w; i++) ext_term[r][i] = sample_fq(rng);
   for (int r = 0; r < 22; r++) internal_rc[r] = sample_fq(rng);
   for (int i = 0; i < w; i++) diag[i] = sample_fq(rng);  // stand-in diagonal, see transcript.h
+  if (w == 8) {
+    for (int r = 0; r < 4; r++)
+      for (int i = 0; i < 8; i++) {
+        vec_rc[r][i] = ext_init[r][i];
+        vec_rc[4 + r][i] = ext_term[r][i];
+      }
+    diag0p1 = diag[0] + 1;  // diag[0] < q, so no wrap
+    use_avx512 = false;
+    if (__builtin_cpu_supports("avx512f")) {
+      // enable the vector path only if it reproduces the scalar permutation on a set of states
+      use_avx512 = true;
+      uint64_t a[8], b[8], seed = 0x9E3779B97F4A7C15ull;
+      for (int t = 0; t < 64 && use_avx512; t++) {
+        for (int i = 0; i < 8; i++) {
+          seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+          a[i] = b[i] = (t & 1) ? seed % Q : Q - 1 - (seed & 15);
+        }
+        permute_scalar(a);
+        permute8_avx512(*this, b);
+        if (memcmp(a, b, sizeof a) != 0) use_avx512 = false;
+      }
+    }
+  }
 }
 
 typedef unsigned __int128 u128;
@@ -179,9 +205,110 @@ static inline void permute_w(const Poseidon2& P, uint64_t* st) {
   for (int i = 0; i < W; i++) st[i] = st[i] >= Q ? st[i] - Q : st[i];
 }
 
-void Poseidon2::permute(uint64_t* st) const {
+void Poseidon2::permute_scalar(uint64_t* st) const {
   if (width == 8) permute_w<8>(*this, st);
   else permute_w<16>(*this, st);
+}
+
+// ---- width 8 with AVX-512F: the 8-lane state is one zmm register ----
+// External rounds run on the vector (Goldilocks products from four 32x32 multiplies, the
+// circulant layer on 32-bit halves so every sum is exact, one reduction per lane).  In the
+// internal rounds the s-box chain of lane 0 stays scalar while lanes 1..7 are updated in the
+// vector, off the chain: rest' = sum_{i>0} mu_i s_i + 7 * sum.  Same outputs as permute_w<8>.
+#define NS_AVX512 __attribute__((target("avx512f")))
+NS_AVX512 static inline __m512i v_red(__m512i lo, __m512i hi) {
+  const __m512i m32 = _mm512_set1_epi64(0xFFFFFFFFll);
+  __m512i hhi = _mm512_srli_epi64(hi, 32), hlo = _mm512_and_si512(hi, m32);
+  __m512i t0 = _mm512_sub_epi64(lo, hhi);
+  t0 = _mm512_mask_sub_epi64(t0, _mm512_cmplt_epu64_mask(lo, hhi), t0, m32);
+  __m512i t1 = _mm512_sub_epi64(_mm512_slli_epi64(hlo, 32), hlo);
+  __m512i t2 = _mm512_add_epi64(t0, t1);
+  return _mm512_mask_add_epi64(t2, _mm512_cmplt_epu64_mask(t2, t1), t2, m32);
+}
+NS_AVX512 static inline __m512i v_mul(__m512i a, __m512i b) {
+  const __m512i m32 = _mm512_set1_epi64(0xFFFFFFFFll);
+  __m512i ah = _mm512_srli_epi64(a, 32), bh = _mm512_srli_epi64(b, 32);
+  __m512i ll = _mm512_mul_epu32(a, b), lh = _mm512_mul_epu32(a, bh), hl = _mm512_mul_epu32(ah, b),
+          hh = _mm512_mul_epu32(ah, bh);
+  __m512i t = _mm512_add_epi64(lh, _mm512_srli_epi64(ll, 32));
+  __m512i u = _mm512_add_epi64(hl, _mm512_and_si512(t, m32));
+  __m512i lo = _mm512_or_si512(_mm512_slli_epi64(u, 32), _mm512_and_si512(ll, m32));
+  __m512i hi = _mm512_add_epi64(_mm512_add_epi64(hh, _mm512_srli_epi64(t, 32)), _mm512_srli_epi64(u, 32));
+  return v_red(lo, hi);
+}
+// a + b for a < 2^64 and b < q: a wrap past 2^64 adds 2^64 mod q = 2^32 - 1 and cannot wrap again
+NS_AVX512 static inline __m512i v_addq(__m512i a, __m512i b) {
+  __m512i s = _mm512_add_epi64(a, b);
+  return _mm512_mask_add_epi64(s, _mm512_cmplt_epu64_mask(s, b), s, _mm512_set1_epi64(0xFFFFFFFFll));
+}
+NS_AVX512 static inline u128 v_hsum(__m512i x) {
+  const __m512i m32 = _mm512_set1_epi64(0xFFFFFFFFll);
+  uint64_t L = (uint64_t)_mm512_reduce_add_epi64(_mm512_and_si512(x, m32));
+  uint64_t H = (uint64_t)_mm512_reduce_add_epi64(_mm512_srli_epi64(x, 32));
+  return ((u128)H << 32) + L;
+}
+NS_AVX512 static inline __m512i v_sbox(__m512i x) {
+  __m512i x2 = v_mul(x, x);
+  return v_mul(v_mul(x2, x), v_mul(x2, x2));
+}
+// circ(2,3,1,1) per 4-lane chunk, then out_i = y_i + (y_i + y_{i xor 4}), plus rc; exact on halves
+NS_AVX512 static inline __m512i v_ext(__m512i x, __m512i rc) {
+  const __m512i m32 = _mm512_set1_epi64(0xFFFFFFFFll);
+  const __m512i p1 = _mm512_set_epi64(4, 7, 6, 5, 0, 3, 2, 1);
+  const __m512i p2 = _mm512_set_epi64(5, 4, 7, 6, 1, 0, 3, 2);
+  const __m512i p3 = _mm512_set_epi64(6, 5, 4, 7, 2, 1, 0, 3);
+  const __m512i sw = _mm512_set_epi64(3, 2, 1, 0, 7, 6, 5, 4);
+  __m512i L = _mm512_and_si512(x, m32), H = _mm512_srli_epi64(x, 32);
+  __m512i l1 = _mm512_permutexvar_epi64(p1, L), h1 = _mm512_permutexvar_epi64(p1, H);
+  __m512i SL = _mm512_add_epi64(_mm512_add_epi64(L, l1),
+                                _mm512_add_epi64(_mm512_permutexvar_epi64(p2, L), _mm512_permutexvar_epi64(p3, L)));
+  __m512i SH = _mm512_add_epi64(_mm512_add_epi64(H, h1),
+                                _mm512_add_epi64(_mm512_permutexvar_epi64(p2, H), _mm512_permutexvar_epi64(p3, H)));
+  __m512i yL = _mm512_add_epi64(_mm512_add_epi64(SL, L), _mm512_add_epi64(l1, l1));
+  __m512i yH = _mm512_add_epi64(_mm512_add_epi64(SH, H), _mm512_add_epi64(h1, h1));
+  __m512i oL = _mm512_add_epi64(_mm512_add_epi64(yL, yL), _mm512_permutexvar_epi64(sw, yL));
+  __m512i oH = _mm512_add_epi64(_mm512_add_epi64(yH, yH), _mm512_permutexvar_epi64(sw, yH));
+  oL = _mm512_add_epi64(oL, _mm512_and_si512(rc, m32));
+  oH = _mm512_add_epi64(oH, _mm512_srli_epi64(rc, 32));
+  // oH * 2^32 + oL with oH, oL < 2^38: low 64 bits plus a small top part times 2^64 = 2^32 - 1
+  __m512i lo = _mm512_add_epi64(_mm512_slli_epi64(oH, 32), oL);
+  __mmask8 c = _mm512_cmplt_epu64_mask(lo, oL);
+  __m512i top = _mm512_add_epi64(_mm512_srli_epi64(oH, 32), _mm512_maskz_set1_epi64(c, 1));
+  __m512i add = _mm512_sub_epi64(_mm512_slli_epi64(top, 32), top);
+  __m512i r = _mm512_add_epi64(lo, add);
+  return _mm512_mask_add_epi64(r, _mm512_cmplt_epu64_mask(r, add), r, m32);
+}
+NS_AVX512 static void permute8_avx512(const Poseidon2& P, uint64_t* st) {
+  __m512i x = _mm512_loadu_si512(st);
+  x = v_ext(x, _mm512_load_si512(P.vec_rc[0]));
+  for (int r = 0; r < 4; r++) x = v_ext(v_sbox(x), r < 3 ? _mm512_load_si512(P.vec_rc[r + 1]) : _mm512_setzero_si512());
+  const __m512i mu = _mm512_mask_blend_epi64(1, _mm512_loadu_si512(P.diag), _mm512_setzero_si512());
+  uint64_t st0 = (uint64_t)_mm_cvtsi128_si64(_mm512_castsi512_si128(x));
+  __m512i v = _mm512_mask_blend_epi64(1, x, _mm512_setzero_si512());
+  uint64_t s0 = red((u128)st0 + P.internal_rc[0]);
+  uint64_t restr = red(v_hsum(v));
+  for (int r = 0; r < 22; r++) {
+    __m512i mv = v_mul(mu, v);
+    u128 W = v_hsum(mv);
+    uint64_t x7 = sbox7(s0);
+    uint64_t sum = red((u128)restr + x7);
+    v = _mm512_mask_blend_epi64(1, v_addq(mv, _mm512_set1_epi64((long long)sum)), _mm512_setzero_si512());
+    u128 acc = (u128)x7 * P.diag0p1 + restr;
+    if (r < 21) acc += P.internal_rc[r + 1];
+    s0 = red(acc);
+    restr = red(W + (u128)sum * 7);
+  }
+  x = _mm512_mask_blend_epi64(1, v, _mm512_set1_epi64((long long)s0));
+  for (int r = 0; r < 4; r++)
+    x = v_ext(v_sbox(v_addq(x, _mm512_load_si512(P.vec_rc[4 + r]))), _mm512_setzero_si512());
+  const __m512i q = _mm512_set1_epi64((long long)Q);
+  x = _mm512_mask_sub_epi64(x, _mm512_cmpge_epu64_mask(x, q), x, q);
+  _mm512_storeu_si512(st, x);
+}
+
+void Poseidon2::permute(uint64_t* st) const {
+  if (width == 8 && use_avx512) permute8_avx512(*this, st);
+  else permute_scalar(st);
 }
 
 const Poseidon2& transcript_perm() {

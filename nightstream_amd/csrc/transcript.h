@@ -41,8 +41,13 @@ struct Poseidon2 {
   uint64_t ext_term[4][16];
   uint64_t internal_rc[22];
   uint64_t diag[16];
+  // width-8 AVX-512 path: the 8 external round-constant vectors and diag[0] + 1
+  alignas(64) uint64_t vec_rc[8][8];
+  uint64_t diag0p1 = 0;
+  bool use_avx512 = false;
   void init(int w, ChaCha8& rng);
   void permute(uint64_t* st) const;
+  void permute_scalar(uint64_t* st) const;
 };
 const Poseidon2& transcript_perm();
 const Poseidon2& digest_perm();
