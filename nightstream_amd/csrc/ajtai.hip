@@ -116,7 +116,8 @@ static size_t aj_groups(size_t nw) { return (nw * D + AJ_NBLK - 1) / AJ_NBLK; }
 
 size_t ajtai_workspace_bytes(size_t m, size_t nw) {
   size_t npad = aj_groups(nw) * AJ_NBLK;
-  return aj_splits(m) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 256;
+  size_t sp = aj_splits(m);
+  return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
 }
 
 // grid: x = M tile (64 limb rows), y = N group (704 digit rows), z = K split
@@ -251,7 +252,14 @@ void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* con
   NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm,
                   dim3((unsigned)(AJ_MROWS / (AJ_MT * 16)), (unsigned)groups, (unsigned)splits), dim3(64 * AJ_WAVES),
                   0, s, limbs, m, kc, Zs_dev, nw, npad, part);
-  hipLaunchKernelGGL(k_ajtai_epilogue, dim3((unsigned)(nw * KAPPA)), dim3(512), 0, s, part, splits, npad, S, out);
+  const int32_t* G = part;
+  if (splits > 1) {
+    // reduce the K splits with coalesced loads before the anti-diagonal epilogue
+    int32_t* red = (int32_t*)(((uintptr_t)(S + nw * D) + 255) & ~(uintptr_t)255);
+    launch_sum_splits(part, splits, (size_t)AJ_MROWS * npad, red, s);
+    G = red;
+  }
+  hipLaunchKernelGGL(k_ajtai_epilogue, dim3((unsigned)(nw * KAPPA)), dim3(512), 0, s, G, (size_t)1, npad, S, out);
   (void)kappa;
 }
 

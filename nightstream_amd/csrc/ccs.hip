@@ -110,6 +110,97 @@ void launch_spmv_t(const DevSparse& M, const K2* chi, size_t n_eff, K2* v, hipSt
 }
 
 // ---------------- Eval column sums ----------------
+// ---------------- contractions over digit rows on int8 MFMA ----------------
+// out[r][c] = sum_i sum_{kk<54} A_i[r][kk] Z_i[kk][c] (int32) for r < arows <= MT*16, with A_i the
+// int8 matrix A[(i*arows + r)*54 + kk].  A block owns 256 columns: per instance it stages A_i and
+// the 54 x 256 digit tile in LDS (the tile transposed so that each lane's B fragment is 16
+// contiguous digit rows of one column) and issues MT x 4 v_mfma_i32_16x16x64_i8 per wave
+// (K = 64 = the 54 digit rows zero-padded).  Requires m % 256 == 0.
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+template <int MT>
+__global__ __launch_bounds__(256) void k_rowcontract(const int8_t* __restrict__ A, int arows,
+                                                     const int8_t* const* __restrict__ Zs, int k, size_t m,
+                                                     int32_t* __restrict__ out) {
+  constexpr int NB = 256, TP = 80;
+  __shared__ __align__(16) int8_t T[NB * TP];
+  __shared__ __align__(16) int8_t Ra[MT * 16 * TP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t c0 = (size_t)blockIdx.x * NB;
+  const int kq = (lane >> 4) * 16, r16 = lane & 15;
+  v4i_t acc[MT][4];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) acc[mt][nt] = v4i_t{0, 0, 0, 0};
+  for (int i = 0; i < k; i++) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < MT * 16 * 64; e += 256) {
+      int r = e >> 6, kk = e & 63;
+      Ra[r * TP + kk] = (r < arows && kk < D) ? A[((size_t)i * arows + r) * D + kk] : (int8_t)0;
+    }
+    const int8_t* Z = Zs[i];
+    for (int e = threadIdx.x; e < 64 * (NB / 4); e += 256) {
+      int kk = e / (NB / 4), w4 = e % (NB / 4);
+      uint32_t v = kk < D ? *(const uint32_t*)(Z + (size_t)kk * m + c0 + 4 * w4) : 0u;
+      int8_t* t = T + (4 * w4) * TP + kk;
+      t[0] = (int8_t)(v & 0xFF);
+      t[TP] = (int8_t)((v >> 8) & 0xFF);
+      t[2 * TP] = (int8_t)((v >> 16) & 0xFF);
+      t[3 * TP] = (int8_t)(v >> 24);
+    }
+    __syncthreads();
+    v4i_t b[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) b[nt] = *(const v4i_t*)(T + (wave * 64 + nt * 16 + r16) * TP + kq);
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+      v4i_t a = *(const v4i_t*)(Ra + (mt * 16 + r16) * TP + kq);
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int row = mt * 16 + 4 * (lane >> 4) + v;
+        if (row < arows) out[(size_t)row * m + c0 + wave * 64 + nt * 16 + r16] = acc[mt][nt][v];
+      }
+}
+
+// A_i rows for the weighted column sum: bytes of W.c0 and W.c1 (each minus 128), then a row of ones
+__global__ void k_w_limbs(const K2* __restrict__ W, int nw, int8_t* __restrict__ A) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nw * 17 * D) return;
+  int i = idx / (17 * D), r = (idx / D) % 17, rho = idx % D;
+  K2 w = W[i * D + rho];
+  int8_t v;
+  if (r < 8) v = (int8_t)((int)((w.c0 >> (8 * r)) & 0xFF) - 128);
+  else if (r < 16) v = (int8_t)((int)((w.c1 >> (8 * (r - 8))) & 0xFF) - 128);
+  else v = 1;
+  A[idx] = v;
+}
+
+// S[c] = (sum_l 256^l (G_l + 128 G_16), same for c1): undo the -128 byte offset with the digit sum
+__global__ void k_colsum_epilogue(const int32_t* __restrict__ G, size_t m, K2* __restrict__ S) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  const int64_t corr = 128 * (int64_t)G[16 * m + c];
+  uint64_t comp[2];
+#pragma unroll
+  for (int cp = 0; cp < 2; cp++) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int l = 0; l < 8; l++) acc = fadd(acc, fmul(from_i64(G[(size_t)(cp * 8 + l) * m + c] + corr), 1ull << (8 * l)));
+    comp[cp] = acc;
+  }
+  S[c] = K2{comp[0], comp[1]};
+}
+
+size_t colsum_ws_bytes(size_t nw, size_t m) { return nw * 17 * D + 256 + 17 * m * 4 + 256; }
+
 __global__ __launch_bounds__(256) void k_weighted_colsum(const int8_t* const* __restrict__ Zs, int nw,
                                                          const K2* __restrict__ W, size_t m, K2* __restrict__ S) {
   __shared__ K2 w_s[16 * D];
@@ -134,10 +225,23 @@ __global__ __launch_bounds__(256) void k_weighted_colsum(const int8_t* const* __
   S[c] = ksub(pos, neg);
 }
 
-void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const K2* W, size_t m, K2* S,
+void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const K2* W, size_t m, K2* S, void* ws,
                                   hipStream_t s) {
-  hipLaunchKernelGGL(k_weighted_colsum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, Zs_dev, (int)nw, W, m,
-                     S);
+  if (nw == 0) {
+    NS_CHECK(hipMemsetAsync(S, 0, m * sizeof(K2), s));
+    return;
+  }
+  if (m % 256 != 0 || !ws) {
+    hipLaunchKernelGGL(k_weighted_colsum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, Zs_dev, (int)nw, W, m,
+                       S);
+    return;
+  }
+  int8_t* A = (int8_t*)ws;
+  int32_t* G = (int32_t*)(((uintptr_t)(A + nw * 17 * D) + 255) & ~(uintptr_t)255);
+  int na = (int)(nw * 17 * D);
+  hipLaunchKernelGGL(k_w_limbs, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, W, (int)nw, A);
+  hipLaunchKernelGGL(k_rowcontract<2>, dim3((unsigned)(m / 256)), dim3(256), 0, s, A, 17, Zs_dev, (int)nw, m, G);
+  hipLaunchKernelGGL(k_colsum_epilogue, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, G, m, S);
 }
 
 // ---------------- y = Z v via int8 MFMA on byte limbs of v ----------------
@@ -235,6 +339,19 @@ __global__ void k_ydigits_epilogue(const int32_t* __restrict__ part, size_t spli
   y[(w * t + j) * D + rho] = K2{comp[0], comp[1]};
 }
 
+// out[e] = sum_sp part[sp * n + e]: the K-split partials reduced with coalesced loads
+__global__ void k_sum_splits_i32(const int32_t* __restrict__ part, size_t splits, size_t n, int32_t* __restrict__ out) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int32_t acc = 0;
+  for (size_t sp = 0; sp < splits; sp++) acc += part[sp * n + e];
+  out[e] = acc;
+}
+
+void launch_sum_splits(const int32_t* part, size_t splits, size_t n, int32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_splits_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, splits, n, out);
+}
+
 static size_t yd_splits(size_t m) {
   size_t s = m / 2048;
   if (s < 1) s = 1;
@@ -245,7 +362,7 @@ static size_t yd_splits(size_t m) {
 
 size_t ydigits_ws_bytes(size_t nw, size_t t, size_t m) {
   size_t nrows = nw * D, mpad = (nrows + 63) / 64 * 64;
-  return yd_splits(m) * mpad * 16 * t * 4 + 16 * t * m + nrows * 4 + 1024;
+  return (yd_splits(m) + 1) * mpad * 16 * t * 4 + 16 * t * m + nrows * 4 + 2048;
 }
 
 void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t t, size_t m, K2* y, void* ws,
@@ -254,6 +371,7 @@ void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t 
   int32_t* part = (int32_t*)ws;
   int8_t* Vl = (int8_t*)(part + splits * mpad * 16 * t);
   int32_t* rowsum = (int32_t*)(((uintptr_t)(Vl + 16 * t * m) + 255) & ~(uintptr_t)255);
+  int32_t* red = (int32_t*)(((uintptr_t)(rowsum + nrows) + 255) & ~(uintptr_t)255);
   hipLaunchKernelGGL(k_v_limbs, dim3((unsigned)((m + 255) / 256), (unsigned)t), dim3(256), 0, s, v, t, m, Vl);
   hipLaunchKernelGGL(k_digit_rowsum2, dim3((unsigned)nrows), dim3(256), 0, s, Zs_dev, m, rowsum);
   dim3 g((unsigned)(mpad / 64), (unsigned)splits);
@@ -270,7 +388,8 @@ void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t 
   }
 #undef NS_YD
   size_t nt = nrows * t;
-  hipLaunchKernelGGL(k_ydigits_epilogue, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, part, splits, mpad,
+  launch_sum_splits(part, splits, mpad * 16 * t, red, s);
+  hipLaunchKernelGGL(k_ydigits_epilogue, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, red, (size_t)1, mpad,
                      nrows, t, rowsum, y);
 }
 
@@ -546,6 +665,11 @@ __global__ __launch_bounds__(256) void k_rlc_zmix(const int8_t* __restrict__ rho
 
 void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
                      hipStream_t s) {
+  if (m % 256 == 0) {
+    NS_LAUNCH_TIMED(KT_ZMIX, k_rowcontract<4>, dim3((unsigned)(m / 256)), dim3(256), 0, s, rho_dev, (int)D, Zs_dev,
+                    (int)k, m, zmix);
+    return;
+  }
   size_t shmem = k * D * D;
   NS_LAUNCH_TIMED(KT_ZMIX, k_rlc_zmix, dim3((unsigned)((m + 255) / 256)), dim3(256), shmem, s, rho_dev, Zs_dev, (int)k,
                   m, zmix);

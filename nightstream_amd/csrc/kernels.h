@@ -90,8 +90,9 @@ void launch_spmv(const DevSparse& M, const K2* x, K2* y, K2 coeff, bool accumula
 // v[c] = sum_{k in col c, row < n_eff} val[k] chi[row]
 void launch_spmv_t(const DevSparse& M, const K2* chi, size_t n_eff, K2* v, hipStream_t s);
 // S[c] = sum_{w, rho} W[w*54+rho] * Z_w[rho][c]
-void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const K2* W, size_t m, K2* S,
+void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const K2* W, size_t m, K2* S, void* ws,
                                   hipStream_t s);
+size_t colsum_ws_bytes(size_t nw, size_t m);
 // y[w][j][rho] = sum_c Z_w[rho][c] v_j[c]  (w < nw, j < t); partial workspace sized by ydigits_ws_bytes
 size_t ydigits_ws_bytes(size_t nw, size_t t, size_t m);
 void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t t, size_t m, K2* y, void* ws,
@@ -124,6 +125,8 @@ void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t
 void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
                      hipStream_t s);
 // balanced base-2 digits of zmix into k_dec planes; flags[i] |= plane i nonzero; flags[k_dec] = range error
+// out[e] = sum over K splits of part[sp * n + e] (int32)
+void launch_sum_splits(const int32_t* part, size_t splits, size_t n, int32_t* out, hipStream_t s);
 // Gather the first m_in columns of up to 16 digit planes [54][m]: out[(p*54 + rho)*m_in + c].
 struct PlanePtrs {
   const int8_t* p[16];

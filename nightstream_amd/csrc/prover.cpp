@@ -600,7 +600,9 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
     w_->Wd.ensure(W.size() * sizeof(K2));
     w_->Sa.ensure(m * sizeof(K2));
     h2d(w_->Wd.p, W.data(), W.size(), st);
-    launch_weighted_digit_colsum(w_->Zptrs.as<const int8_t*>() + 1, k - 1, w_->Wd.as<K2>(), m, w_->Sa.as<K2>(), st);
+    w_->cs_ws.ensure(colsum_ws_bytes(k - 1, m));
+    launch_weighted_digit_colsum(w_->Zptrs.as<const int8_t*>() + 1, k - 1, w_->Wd.as<K2>(), m, w_->Sa.as<K2>(),
+                                 w_->cs_ws.p, st);
     K2* ev = T + (size_t)(2 + nf_) * n_pad;
     K2 gkj = kone();
     for (size_t j = 0; j < s.t; j++) {

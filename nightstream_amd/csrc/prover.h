@@ -102,7 +102,7 @@ struct MeHost {
 
 // Device buffers of a Pi_CCS oracle, reused across fold steps (grow-only).
 struct OracleWs {
-  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev;
+  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws;
 };
 
 struct StepTiming {
