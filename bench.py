@@ -28,14 +28,15 @@ sys.path.insert(0, ROOT)
 PP_SEED = bytes([0x4E, 0x53] + [0x07] * 30)
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0
-KT = ["round0_eval", "round_eval", "table_fold", "ajtai_gemm", "ydigits_gemm", "rlc_zmix", "dec_split"]
+KT = ["round0_eval", "round_eval", "table_fold", "ajtai_gemm", "ydigits_gemm", "rlc_zmix", "dec_split", "round1_eval"]
 KERNEL_NAMES = {
-    "round0_eval": "k_round_eval<true>",
-    "round_eval": "k_round_eval<false>",
+    "round0_eval": "k_round_eval<0>",
+    "round1_eval": "k_round_eval<1>",
+    "round_eval": "k_round_eval<2>",
     "table_fold": "k_fold_k",
     "ajtai_gemm": "k_ajtai_gemm",
     "ydigits_gemm": "k_ydigits_gemm<4>",
-    "rlc_zmix": "k_rlc_zmix",
+    "rlc_zmix": "k_rowcontract<4>",
     "dec_split": "k_dec_split",
 }
 D = 54
@@ -45,8 +46,9 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
     """Algorithmic bytes (or int8 ops) per kernel class summed over one shard of `steps`.
 
     Pi_CCS tables per row at round r (length L = n >> r): k*54 NC entries + nsc scalar tables
-    (eq_beta, 3 f-var tables, plus eq_rin and Eval once k > 1), 16 B per K entry; round 0 reads
-    the int8 digit matrices (1 B per digit) instead of NC K entries.
+    (eq_beta, 3 f-var tables, plus eq_rin and Eval once k > 1), 16 B per K entry.  Rounds 0 and 1
+    read the int8 digit matrices (1 B per digit, all n digits of a row each) instead of NC K
+    entries; the fold after round 1 writes the NC tables of length n/4 from the digits.
     """
     ell_n = n.bit_length() - 1
     out = {c: 0.0 for c in KT}
@@ -54,13 +56,17 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
         k = 1 if s == 0 else 13
         nsc = 4 if s == 0 else 6
         out["round0_eval"] += k * D * n + nsc * 16 * n
-        for r in range(1, ell_n):
+        out["round1_eval"] += k * D * n + nsc * 16 * (n // 2)
+        for r in range(2, ell_n):
             L = n >> r
             out["round_eval"] += (k * D + nsc) * 16 * L
         for r in range(ell_n):
             L = n >> r
-            rows = nsc if r == 0 else (k * D + nsc)
-            out["table_fold"] += rows * 16 * (L + L // 2)
+            out["table_fold"] += nsc * 16 * (L + L // 2)
+            if r == 1:
+                out["table_fold"] += k * D * n + k * D * 16 * (n // 4)
+            elif r >= 2:
+                out["table_fold"] += k * D * 16 * (L + L // 2)
         # y products: k witnesses at the row point + 12 DEC children, t = 4 matrices
         for nw in (k, 12):
             out["ydigits_gemm"] += 2.0 * (nw * D) * 64 * n

@@ -410,6 +410,7 @@ struct RoundArgs {
   int k;
   K2 gamma_k;
   FPoly f;
+  const K2* pc;  // round 1: [20][4] coefficient polynomials of the digit monomials
 };
 
 __device__ __forceinline__ K2 wave_reduce(K2 v) {
@@ -426,12 +427,17 @@ __device__ __forceinline__ K2 wave_reduce(K2 v) {
 // sum over the k*54 digit rows of a pair is split into G groups so that short tables still fill
 // the GPU; every group multiplies its partial NC polynomial by the pair's eq_beta(X) (the
 // round polynomial is linear in the NC sum), and group 0 adds the f and Eval terms.
-template <bool ROUND0>
+// MODE 0: round 0 over the int8 digits; MODE 1: round 1 over the int8 digits (tables folded once
+// by r0, see below); MODE 2: K tables.
+template <int MODE>
 __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
                                                    int G) {
-  extern __shared__ K2 sh[];  // wg cache [k*54] then reduction scratch
+  extern __shared__ K2 sh[];  // wg cache [k*54], round-1 monomial polynomials [80], reduction scratch
   const int nrow = A.k * D;
   for (int i = threadIdx.x; i < nrow; i += blockDim.x) sh[i] = wg[i];
+  K2* pcs = sh + nrow;
+  if (MODE == 1)
+    for (int i = threadIdx.x; i < 80; i += blockDim.x) pcs[i] = A.pc[i];
   __syncthreads();
   const size_t pairs = A.len / 2;
   const int R = (nrow + G - 1) / G;
@@ -476,7 +482,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
         ev2 = kadd(ev2, kmul(r1, v1));
       }
     }
-    if (ROUND0) {
+    if (MODE == 0) {
       // signed selections of wg
       K2 slo = kzero(), shi = kzero(), s3 = kzero(), s4 = kzero();
       for (int r = r_lo; r < r_hi; r++) {
@@ -500,6 +506,46 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
       in[1] = kadd(in[1], ksub(ksub(s3x3, slo2), shi));
       in[2] = kadd(in[2], ksub(kadd(kadd(slo2, slo), s4x3), kadd(s3x3, s3x3)));
       in[3] = kadd(in[3], ksub(kadd(shi, s3x3), kadd(slo, s4x3)));
+    } else if (MODE == 1) {
+      // Round 1 straight from the digits: with alpha = 1 - r0, beta = r0 the pair's table values
+      // are a = alpha d0 + beta d1, a' = alpha d2 + beta d3 (d_i = digits 4t..4t+3), and
+      // N(a + (a' - a) X) = sum over the 20 digit monomials {d_i, |d_i| d_j (i != j),
+      // d_i d_j d_l (i < j < l)} of a fixed cubic in X (expanding (sum_i L_i(X) d_i)^3 with
+      // d^3 = d, d^2 = |d|).  Each row adds +-w to the monomials it hits; the cubics are applied
+      // once per work item.
+      K2 acc[20];
+#pragma unroll
+      for (int j = 0; j < 20; j++) acc[j] = kzero();
+      for (int r = r_lo; r < r_hi; r++) {
+        const int w = r / D, rho = r - w * D;
+        const uint32_t q = *(const uint32_t*)(A.digits[w] + (size_t)rho * A.m + 4 * t);
+        if (q == 0) continue;
+        const int d[4] = {(int)(int8_t)(q & 0xFF), (int)(int8_t)((q >> 8) & 0xFF), (int)(int8_t)((q >> 16) & 0xFF),
+                          (int)(int8_t)(q >> 24)};
+        const K2 wv = sh[r], nw = kneg(wv);
+        int mono[20];
+#pragma unroll
+        for (int i = 0; i < 4; i++) mono[i] = d[i];
+        int idx = 4;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (i != j) mono[idx++] = (d[i] & 1) * d[j];
+        mono[16] = d[0] * d[1] * d[2];
+        mono[17] = d[0] * d[1] * d[3];
+        mono[18] = d[0] * d[2] * d[3];
+        mono[19] = d[1] * d[2] * d[3];
+#pragma unroll
+        for (int j = 0; j < 20; j++) {
+          const K2 v = mono[j] > 0 ? wv : (mono[j] < 0 ? nw : kzero());
+          acc[j] = kadd(acc[j], v);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 20; j++)
+#pragma unroll
+        for (int dd = 0; dd < 4; dd++) in[dd] = kadd(in[dd], kmul(pcs[j * 4 + dd], acc[j]));
     } else {
       K2 s1 = kzero(), s2 = kzero(), s3 = kzero(), s4 = kzero(), s5 = kzero(), s6 = kzero();
       for (int r = r_lo; r < r_hi; r++) {
@@ -531,7 +577,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
     c[2] = kadd(c[2], kmul(A.gamma_k, ev2));
   }
   // block reduction
-  K2* red = sh + nrow;  // [4 waves][5]
+  K2* red = sh + nrow + 80;  // [4 waves][5]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
 #pragma unroll
@@ -578,18 +624,22 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   A.f = f;
   size_t pairs = T.len / 2;
   // split the k*54 NC rows of a pair across G work items when the table is short
-  const size_t target = (size_t)RMAXBLK * RB / 2;
+  // (round 1 from digits applies 80 coefficient products per work item: keep its items long)
+  const size_t target = T.digit_round1 ? (size_t)RB * 512 : (size_t)RMAXBLK * RB / 2;
   int G = 1;
   while ((size_t)G * 2 <= (size_t)T.k * D && pairs * (size_t)G * 2 <= target) G *= 2;
   size_t nblk = (pairs * (size_t)G + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
-  size_t shmem = ((size_t)T.k * D + 4 * 5) * sizeof(K2);
+  size_t shmem = ((size_t)T.k * D + 80 + 4 * 5) * sizeof(K2);
   K2* partials = (K2*)ws;
+  A.pc = T.pc;
   if (round0)
-    NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<true>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
+    NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<0>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
+  else if (T.digit_round1)
+    NS_LAUNCH_TIMED(KT_ROUND1, k_round_eval<1>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
   else
-    NS_LAUNCH_TIMED(KT_ROUND, k_round_eval<false>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
+    NS_LAUNCH_TIMED(KT_ROUND, k_round_eval<2>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
   hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
 }
 
@@ -624,6 +674,32 @@ __global__ void k_fold_digits(const int8_t* const* __restrict__ digits, size_t m
   K2 v = K2{from_i64(lo), 0};
   v = kadd(v, kmul_small(r, hi - lo));
   dst[row * half + i] = v;
+}
+
+// Tables after rounds 0 and 1 straight from the digits: T2[i] = sum_b chi_{(r0, r1)}(b) D[4i + b].
+__global__ void k_fold_digits2(const int8_t* const* __restrict__ digits, size_t m, size_t len, K2* __restrict__ dst,
+                               K2 w00, K2 w10, K2 w01, K2 w11) {
+  size_t quarter = len / 4;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t row = blockIdx.y;
+  if (i >= quarter) return;
+  const int8_t* z = digits[row / D] + (row % D) * m;
+  int d[4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) d[b] = (4 * i + b < m) ? z[4 * i + b] : 0;
+  const K2 w[4] = {w00, w10, w01, w11};
+  K2 v = kzero();
+#pragma unroll
+  for (int b = 0; b < 4; b++) v = kadd(v, d[b] > 0 ? w[b] : (d[b] < 0 ? kneg(w[b]) : kzero()));
+  dst[row * quarter + i] = v;
+}
+
+void launch_fold_digits2(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1,
+                         hipStream_t s) {
+  size_t quarter = len / 4;
+  const K2 a0 = ksub(kone(), r0), a1 = ksub(kone(), r1);
+  NS_LAUNCH_TIMED(KT_FOLD, k_fold_digits2, dim3((unsigned)((quarter + 255) / 256), (unsigned)(k * D)), dim3(256), 0,
+                  s, digits_dev, m, len, dst, kmul(a0, a1), kmul(r0, a1), kmul(a0, r1), kmul(r0, r1));
 }
 
 void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r, hipStream_t s) {

@@ -47,8 +47,43 @@ NS_HD uint64_t fsub(uint64_t a, uint64_t b) {
   return d;
 }
 NS_HD uint64_t fneg(uint64_t a) { return a ? Q - a : 0; }
+#if defined(__HIP_DEVICE_COMPILE__)
+// 64 x 64 -> 128 from four 32 x 32 + 64 multiply-adds (v_mad_u64_u32); every partial sum fits in
+// 64 bits, so no carry chains are needed.
+__device__ __forceinline__ void mul_wide(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32), bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
+  const uint64_t p0 = (uint64_t)al * bl;
+  const uint64_t p1 = (uint64_t)al * bh + (p0 >> 32);
+  const uint64_t p2 = (uint64_t)ah * bl + (uint32_t)p1;
+  hi = (uint64_t)ah * bh + (p1 >> 32) + (p2 >> 32);
+  lo = (p2 << 32) | (uint32_t)p0;
+}
+// reduce128 with hi_lo * (2^32 - 1) formed from 32-bit pieces instead of a multiply
+__device__ __forceinline__ uint64_t reduce128_dev(uint64_t lo, uint64_t hi) {
+  const uint32_t hl = (uint32_t)hi, hh = (uint32_t)(hi >> 32);
+  uint64_t t0 = lo - hh;
+  t0 -= (uint64_t)0 - (uint64_t)(lo < hh) & EPS;
+  const uint64_t t1 = ((uint64_t)(hl - (uint32_t)(hl != 0)) << 32) | (uint32_t)(0u - hl);
+  uint64_t t2 = t0 + t1;
+  t2 += (uint64_t)0 - (uint64_t)(t2 < t1) & EPS;
+  return t2 >= Q ? t2 - Q : t2;
+}
+NS_HD uint64_t fmul(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  mul_wide(a, b, lo, hi);
+  return reduce128_dev(lo, hi);
+}
+// 7 a = 8 a - a as a 128-bit value
+NS_HD uint64_t fmul7(uint64_t a) {
+  const uint64_t a8 = a << 3;
+  const uint64_t lo = a8 - a;
+  const uint64_t hi = (a >> 61) - (uint64_t)(a8 < a);
+  return reduce128_dev(lo, hi);
+}
+#else
 NS_HD uint64_t fmul(uint64_t a, uint64_t b) { return reduce128(a * b, mulhi64(a, b)); }
 NS_HD uint64_t fmul7(uint64_t a) { return reduce128(a * 7ull, mulhi64(a, 7ull)); }
+#endif
 NS_HD uint64_t fdouble(uint64_t a) { return fadd(a, a); }
 // Signed small integer into F_q.
 NS_HD uint64_t from_i64(int64_t v) { return v >= 0 ? (uint64_t)v : Q - (uint64_t)(-v); }

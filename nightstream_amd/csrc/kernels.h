@@ -36,7 +36,7 @@ struct HipError {
 // Per-kernel timing: when a prover enabled timing on the calling thread, ktimer_events hands out
 // a start/stop event pair that hipExtLaunchKernelGGL records at the dispatch's own start and end
 // (so host submission gaps are not counted); otherwise it returns false and the launch is plain.
-enum KernelClass { KT_ROUND0 = 0, KT_ROUND, KT_FOLD, KT_AJTAI_GEMM, KT_YD_GEMM, KT_ZMIX, KT_SPLIT, KT_N };
+enum KernelClass { KT_ROUND0 = 0, KT_ROUND, KT_FOLD, KT_AJTAI_GEMM, KT_YD_GEMM, KT_ZMIX, KT_SPLIT, KT_ROUND1, KT_N };
 bool ktimer_events(int cls, hipEvent_t* start, hipEvent_t* stop);
 #define NS_LAUNCH_TIMED(cls, kernel, grid, block, shmem, stream, ...)                                   \
   do {                                                                                                \
@@ -111,6 +111,8 @@ struct RowTables {
   size_t m;           // digit row stride (round 0)
   int k;
   K2 gamma_k;
+  bool digit_round1 = false;  // round 1 evaluated from the digits (tables folded once, by r0)
+  const K2* pc = nullptr;     // [20][4] monomial coefficient polynomials for digit_round1
 };
 size_t round_ws_bytes();
 // Writes the deg_max+1 coefficients (low->high) of the round polynomial to out (device).
@@ -118,6 +120,8 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg /* [k*54
                        K2* out, void* ws, hipStream_t s);
 // Fold every table with challenge r: dst[i] = src[2i] + r (src[2i+1] - src[2i]).
 void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStream_t s);
+void launch_fold_digits2(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1,
+                         hipStream_t s);
 void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r, hipStream_t s);
 
 // ---- Pi_RLC / Pi_DEC ----

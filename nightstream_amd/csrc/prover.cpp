@@ -8,6 +8,8 @@
 // (optimized_engine/oracle.rs:1812-1951), ME outputs (oracle.rs:1955-2072), Pi_RLC on X, y and
 // commitments (optimized_engine/common.rs:1221-1379, session.rs:301-324) and the DEC checks
 // (optimized_engine/common.rs:1575-1628).
+#include <array>
+
 #include "prover.h"
 
 #include <string.h>
@@ -639,11 +641,43 @@ std::vector<K2> CcsOracle::round_coeffs() {
   T.nf = nf_;
   T.eqr = has_eval_ ? sc + (size_t)(1 + nf_) * len_ : nullptr;
   T.ev = has_eval_ ? sc + (size_t)(2 + nf_) * len_ : nullptr;
-  T.nc = round_ == 0 ? nullptr : w_->nc[cur_nc_].as<K2>();
+  T.nc = round_ <= 1 ? nullptr : w_->nc[cur_nc_].as<K2>();
   T.digits = w_->Zptrs.as<const int8_t*>();
   T.m = s_.m;
   T.k = (int)k_;
   T.gamma_k = gamma_k_;
+  if (round_ == 1) {
+    // cubic coefficients of the 20 digit monomials for N((1-X) a + X a') with
+    // a = (1-r0) d0 + r0 d1, a' = (1-r0) d2 + r0 d3
+    const K2 al = ksub(kone(), r0_), be = r0_;
+    typedef std::array<K2, 4> P4;
+    auto lin = [](K2 c0, K2 c1) { return P4{c0, c1, kzero(), kzero()}; };
+    auto mul = [](const P4& a, const P4& b) {
+      P4 o{kzero(), kzero(), kzero(), kzero()};
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; i + j < 4; j++) o[i + j] = kadd(o[i + j], kmul(a[i], b[j]));
+      return o;
+    };
+    auto scale = [](const P4& a, uint64_t c) { return P4{kscale(a[0], c), kscale(a[1], c), kscale(a[2], c), kscale(a[3], c)}; };
+    const P4 L[4] = {lin(al, kneg(al)), lin(be, kneg(be)), lin(kzero(), al), lin(kzero(), be)};
+    std::vector<K2> pc;
+    pc.reserve(80);
+    auto push = [&](const P4& p) { pc.insert(pc.end(), p.begin(), p.end()); };
+    for (int i = 0; i < 4; i++) {
+      P4 c = mul(mul(L[i], L[i]), L[i]);
+      for (int d = 0; d < 4; d++) c[d] = ksub(c[d], L[i][d]);
+      push(c);
+    }
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++)
+        if (i != j) push(scale(mul(mul(L[i], L[i]), L[j]), 3));
+    const int trip[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    for (auto& tr : trip) push(scale(mul(mul(L[tr[0]], L[tr[1]]), L[tr[2]]), 6));
+    w_->pc.ensure(80 * sizeof(K2));
+    h2d(w_->pc.p, pc.data(), pc.size(), st_);
+    T.digit_round1 = true;
+    T.pc = w_->pc.as<K2>();
+  }
   launch_round_eval(T, s_.fpoly, w_->wg.as<K2>(), round_ == 0, w_->coeffs.as<K2>(), w_->ws.p, st_);
   std::vector<K2> out(5);
   d2h(out.data(), w_->coeffs.p, 5, st_);
@@ -655,7 +689,14 @@ void CcsOracle::fold(K2 r) {
   int nsc_dst = cur_sc_ ^ 1;
   launch_fold_k(w_->sc[cur_sc_].as<K2>(), (size_t)nsc_, len_, w_->sc[nsc_dst].as<K2>(), r, st_);
   if (round_ == 0) {
-    launch_fold_digits(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_, w_->nc[0].as<K2>(), r, st_);
+    // the NC tables stay as digits through round 1 (evaluated from them with r0)
+    r0_ = r;
+    if (half < 2) {  // a one-round table: materialize now
+      launch_fold_digits(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_, w_->nc[0].as<K2>(), r, st_);
+      cur_nc_ = 0;
+    }
+  } else if (round_ == 1 && len_ * 2 >= 4) {
+    launch_fold_digits2(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_ * 2, w_->nc[0].as<K2>(), r0_, r, st_);
     cur_nc_ = 0;
   } else {
     int dst = cur_nc_ ^ 1;

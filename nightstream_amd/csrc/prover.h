@@ -102,7 +102,7 @@ struct MeHost {
 
 // Device buffers of a Pi_CCS oracle, reused across fold steps (grow-only).
 struct OracleWs {
-  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws;
+  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc;
 };
 
 struct StepTiming {
@@ -188,6 +188,7 @@ class CcsOracle {
   OracleWs* w_;           // Zptrs: k digit pointers; sc: scalar tables [nsc][len]; nc: NC [k*54][len]
   int cur_sc_ = 0, cur_nc_ = 0;
   std::vector<const int8_t*> zs_;
+  K2 r0_ = K2{0, 0};  // round-0 challenge: round 1 is evaluated from the digits with it
 };
 
 }  // namespace ns
