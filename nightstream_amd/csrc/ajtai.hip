@@ -106,7 +106,8 @@ constexpr int AJ_NBLK = AJ_WAVES * AJ_NS * 16;  // 704 digit rows per block
 constexpr int AJ_MROWS = KAPPA * 8 * D;         // 6912
 
 static size_t aj_splits(size_t m) {
-  size_t s = m / 16384;
+  // 108 M tiles x splits blocks: 8 splits give 3.4 blocks per CU on 256 CUs (less tail than 4)
+  size_t s = m / 8192;
   if (s < 1) s = 1;
   if (s > 8) s = 8;
   while (s > 1 && (m / s) % 64 != 0) s--;
@@ -152,17 +153,28 @@ __global__ __launch_bounds__(256) void k_ajtai_gemm(const int8_t* __restrict__ l
 #pragma unroll
     for (int ns_ = 0; ns_ < AJ_NS; ns_++) acc[mt][ns_] = v4i{0, 0, 0, 0};
 
+  // one wave per SIMD: the next k-step's operands are loaded before this step's MFMAs issue
+  v4i a[AJ_MT], b[AJ_NS];
+#pragma unroll
+  for (int mt = 0; mt < AJ_MT; mt++) a[mt] = *(const v4i*)(arow[mt]);
+#pragma unroll
+  for (int ns_ = 0; ns_ < AJ_NS; ns_++) b[ns_] = bval[ns_] ? *(const v4i*)(brow[ns_]) : v4i{0, 0, 0, 0};
   for (size_t k = 0; k < kc; k += 64) {
-    v4i a[AJ_MT], b[AJ_NS];
+    v4i an[AJ_MT], bn[AJ_NS];
+    const size_t kn = k + 64 < kc ? k + 64 : k;  // last step reloads its own (in-bounds) slice
 #pragma unroll
-    for (int mt = 0; mt < AJ_MT; mt++) a[mt] = *(const v4i*)(arow[mt] + k);
+    for (int mt = 0; mt < AJ_MT; mt++) an[mt] = *(const v4i*)(arow[mt] + kn);
 #pragma unroll
-    for (int ns_ = 0; ns_ < AJ_NS; ns_++) b[ns_] = bval[ns_] ? *(const v4i*)(brow[ns_] + k) : v4i{0, 0, 0, 0};
+    for (int ns_ = 0; ns_ < AJ_NS; ns_++) bn[ns_] = bval[ns_] ? *(const v4i*)(brow[ns_] + kn) : v4i{0, 0, 0, 0};
 #pragma unroll
     for (int mt = 0; mt < AJ_MT; mt++)
 #pragma unroll
       for (int ns_ = 0; ns_ < AJ_NS; ns_++)
         acc[mt][ns_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mt], b[ns_], acc[mt][ns_], 0, 0, 0);
+#pragma unroll
+    for (int mt = 0; mt < AJ_MT; mt++) a[mt] = an[mt];
+#pragma unroll
+    for (int ns_ = 0; ns_ < AJ_NS; ns_++) b[ns_] = bn[ns_];
   }
 
   // D[row][col]: col = lane % 16, row = 4 * (lane / 16) + v

@@ -445,17 +445,20 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
 #pragma unroll
   for (int d = 0; d < 5; d++) c[d] = kzero();
   K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
-  const size_t total = pairs * (size_t)G;
+  // round 1 splits each (pair, row group) item into two halves of the 20 digit monomials
+  const int H = MODE == 1 ? 2 : 1;
+  const size_t total = pairs * (size_t)G * H;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const size_t t = idx % pairs;
-    const int g = (int)(idx / pairs);
+    const int gh = (int)(idx / pairs);
+    const int g = gh / H, h = gh % H;
     const int r_lo = g * R, r_hi = min(nrow, r_lo + R);
     const K2 e0 = A.eqb[2 * t];
     const K2 e1 = ksub(A.eqb[2 * t + 1], e0);
     K2 in[5];
 #pragma unroll
     for (int d = 0; d < 5; d++) in[d] = kzero();
-    if (g == 0) {
+    if (g == 0 && h == 0) {
       // f(M z1)(X)
       for (int tt = 0; tt < A.f.nterms; tt++) {
         K2 p[5];
@@ -513,39 +516,86 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
       // d_i d_j d_l (i < j < l)} of a fixed cubic in X (expanding (sum_i L_i(X) d_i)^3 with
       // d^3 = d, d^2 = |d|).  Each row adds +-w to the monomials it hits; the cubics are applied
       // once per work item.
-      K2 acc[20];
-#pragma unroll
-      for (int j = 0; j < 20; j++) acc[j] = kzero();
-      for (int r = r_lo; r < r_hi; r++) {
-        const int w = r / D, rho = r - w * D;
-        const uint32_t q = *(const uint32_t*)(A.digits[w] + (size_t)rho * A.m + 4 * t);
-        if (q == 0) continue;
-        const int d[4] = {(int)(int8_t)(q & 0xFF), (int)(int8_t)((q >> 8) & 0xFF), (int)(int8_t)((q >> 16) & 0xFF),
-                          (int)(int8_t)(q >> 24)};
-        const K2 wv = sh[r], nw = kneg(wv);
-        int mono[20];
-#pragma unroll
-        for (int i = 0; i < 4; i++) mono[i] = d[i];
-        int idx = 4;
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-          for (int j = 0; j < 4; j++)
-            if (i != j) mono[idx++] = (d[i] & 1) * d[j];
-        mono[16] = d[0] * d[1] * d[2];
-        mono[17] = d[0] * d[1] * d[3];
-        mono[18] = d[0] * d[2] * d[3];
-        mono[19] = d[1] * d[2] * d[3];
-#pragma unroll
-        for (int j = 0; j < 20; j++) {
-          const K2 v = mono[j] > 0 ? wv : (mono[j] < 0 ? nw : kzero());
-          acc[j] = kadd(acc[j], v);
+#define NS_UPD(acc, mv)                                         \
+  {                                                             \
+    const int mm = (mv);                                        \
+    acc = kadd(acc, mm > 0 ? wv : (mm < 0 ? nw : kzero()));     \
+  }
+#define NS_APPLY(j, acc)                                        \
+  {                                                             \
+    in[0] = kadd(in[0], kmul(pcs[(j)*4 + 0], acc));             \
+    in[1] = kadd(in[1], kmul(pcs[(j)*4 + 1], acc));             \
+    in[2] = kadd(in[2], kmul(pcs[(j)*4 + 2], acc));             \
+    in[3] = kadd(in[3], kmul(pcs[(j)*4 + 3], acc));             \
+  }
+      if (h == 0) {
+        K2 a0 = kzero(), a1 = kzero(), a2 = kzero(), a3 = kzero(), a4 = kzero(), a5 = kzero(), a6 = kzero(), a7 = kzero(), a8 = kzero(), a9 = kzero();
+        for (int r = r_lo; r < r_hi; r++) {
+          const int w = r / D, rho = r - w * D;
+          const uint32_t q = *(const uint32_t*)(A.digits[w] + (size_t)rho * A.m + 4 * t);
+          if (q == 0) continue;
+          const int d0 = (int)(int8_t)(q & 0xFF), d1 = (int)(int8_t)((q >> 8) & 0xFF),
+                    d2 = (int)(int8_t)((q >> 16) & 0xFF), d3 = (int)(int8_t)(q >> 24);
+          const int e0 = d0 & 1, e1 = d1 & 1, e2 = d2 & 1, e3 = d3 & 1;
+          (void)e0; (void)e1; (void)e2; (void)e3;
+          const K2 wv = sh[r], nw = kneg(wv);
+          NS_UPD(a0, d0)
+          NS_UPD(a1, d1)
+          NS_UPD(a2, d2)
+          NS_UPD(a3, d3)
+          NS_UPD(a4, e0 * d1)
+          NS_UPD(a5, e0 * d2)
+          NS_UPD(a6, e0 * d3)
+          NS_UPD(a7, e1 * d0)
+          NS_UPD(a8, e1 * d2)
+          NS_UPD(a9, e1 * d3)
         }
+        NS_APPLY(0, a0)
+        NS_APPLY(1, a1)
+        NS_APPLY(2, a2)
+        NS_APPLY(3, a3)
+        NS_APPLY(4, a4)
+        NS_APPLY(5, a5)
+        NS_APPLY(6, a6)
+        NS_APPLY(7, a7)
+        NS_APPLY(8, a8)
+        NS_APPLY(9, a9)
       }
-#pragma unroll
-      for (int j = 0; j < 20; j++)
-#pragma unroll
-        for (int dd = 0; dd < 4; dd++) in[dd] = kadd(in[dd], kmul(pcs[j * 4 + dd], acc[j]));
+      else {
+        K2 a10 = kzero(), a11 = kzero(), a12 = kzero(), a13 = kzero(), a14 = kzero(), a15 = kzero(), a16 = kzero(), a17 = kzero(), a18 = kzero(), a19 = kzero();
+        for (int r = r_lo; r < r_hi; r++) {
+          const int w = r / D, rho = r - w * D;
+          const uint32_t q = *(const uint32_t*)(A.digits[w] + (size_t)rho * A.m + 4 * t);
+          if (q == 0) continue;
+          const int d0 = (int)(int8_t)(q & 0xFF), d1 = (int)(int8_t)((q >> 8) & 0xFF),
+                    d2 = (int)(int8_t)((q >> 16) & 0xFF), d3 = (int)(int8_t)(q >> 24);
+          const int e0 = d0 & 1, e1 = d1 & 1, e2 = d2 & 1, e3 = d3 & 1;
+          (void)e0; (void)e1; (void)e2; (void)e3;
+          const K2 wv = sh[r], nw = kneg(wv);
+          NS_UPD(a10, e2 * d0)
+          NS_UPD(a11, e2 * d1)
+          NS_UPD(a12, e2 * d3)
+          NS_UPD(a13, e3 * d0)
+          NS_UPD(a14, e3 * d1)
+          NS_UPD(a15, e3 * d2)
+          NS_UPD(a16, d0 * d1 * d2)
+          NS_UPD(a17, d0 * d1 * d3)
+          NS_UPD(a18, d0 * d2 * d3)
+          NS_UPD(a19, d1 * d2 * d3)
+        }
+        NS_APPLY(10, a10)
+        NS_APPLY(11, a11)
+        NS_APPLY(12, a12)
+        NS_APPLY(13, a13)
+        NS_APPLY(14, a14)
+        NS_APPLY(15, a15)
+        NS_APPLY(16, a16)
+        NS_APPLY(17, a17)
+        NS_APPLY(18, a18)
+        NS_APPLY(19, a19)
+      }
+#undef NS_UPD
+#undef NS_APPLY
     } else {
       K2 s1 = kzero(), s2 = kzero(), s3 = kzero(), s4 = kzero(), s5 = kzero(), s6 = kzero();
       for (int r = r_lo; r < r_hi; r++) {
@@ -628,7 +678,8 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   const size_t target = T.digit_round1 ? (size_t)RB * 512 : (size_t)RMAXBLK * RB / 2;
   int G = 1;
   while ((size_t)G * 2 <= (size_t)T.k * D && pairs * (size_t)G * 2 <= target) G *= 2;
-  size_t nblk = (pairs * (size_t)G + RB - 1) / RB;
+  const size_t H = T.digit_round1 ? 2 : 1;
+  size_t nblk = (pairs * (size_t)G * H + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
   size_t shmem = ((size_t)T.k * D + 80 + 4 * 5) * sizeof(K2);
