@@ -99,11 +99,15 @@ void launch_chacha8_words(const uint32_t key[8], uint64_t first_block, size_t nb
 // ---------------- Ajtai commit ----------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int AJ_MT = 4;       // 16-row M subtiles per wave (64 limb rows per block)
+constexpr int AJ_MT = 4;       // 16-row M subtiles per wave
 constexpr int AJ_NS = 11;      // 16-column N subtiles per wave
-constexpr int AJ_WAVES = 4;    // waves per block
-constexpr int AJ_NBLK = AJ_WAVES * AJ_NS * 16;  // 704 digit rows per block
-constexpr int AJ_MROWS = KAPPA * 8 * D;         // 6912
+// Block tile 256 limb rows x 352 digit rows, 8 waves as 4 (M) x 2 (N), each wave 64 x 176; the
+// block's A and B slices of every 64-wide K step go through double-buffered LDS once, so each
+// PP byte read from L2 feeds 352 columns and each digit byte 256 rows.
+constexpr int AJ_BM = 256, AJ_BN = 352, AJ_BK = 64, AJ_LDP = 80;  // LDS row pitch (bytes)
+constexpr int AJ_THREADS = 512;
+constexpr int AJ_NBLK = AJ_BN;                   // digit rows per N group
+constexpr int AJ_MROWS = KAPPA * 8 * D;          // 6912 = 27 x 256
 
 static size_t aj_splits(size_t m) {
   // 108 M tiles x splits blocks: 8 splits give 3.4 blocks per CU on 256 CUs (less tail than 4)
@@ -121,31 +125,53 @@ size_t ajtai_workspace_bytes(size_t m, size_t nw) {
   return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
 }
 
-// grid: x = M tile (64 limb rows), y = N group (704 digit rows), z = K split
-__global__ __launch_bounds__(256) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t m, size_t kc,
-                                                   const int8_t* const* __restrict__ Zs, size_t nw, size_t npad,
-                                                   int32_t* __restrict__ part) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const size_t m0 = (size_t)blockIdx.x * (AJ_MT * 16);
+// grid: x = M tile (256 limb rows), y = N group (352 digit rows), z = K split
+__global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t m, size_t kc,
+                                                          const int8_t* const* __restrict__ Zs, size_t nw,
+                                                          size_t npad, int32_t* __restrict__ part) {
+  __shared__ __align__(16) int8_t As[2][AJ_BM * AJ_LDP];
+  __shared__ __align__(16) int8_t Bs[2][AJ_BN * AJ_LDP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 3, wn = wave >> 2;
+  const size_t m0 = (size_t)blockIdx.x * AJ_BM;
+  const size_t n0 = (size_t)blockIdx.y * AJ_BN;
   const size_t k_begin = (size_t)blockIdx.z * kc;
-  const int kq = (lane >> 4) * 16;  // byte offset of this lane's k-slice inside a 64-wide step
-  const int r16 = lane & 15;
-
-  const int8_t* arow[AJ_MT];
+  const int kq = (lane >> 4) * 16, r16 = lane & 15;
+  // global -> LDS staging: A has 256 x 4 chunks of 16 B (2 per thread), B 352 x 4 (<= 3 per thread)
+  const int8_t* asrc[2];
+  int adst[2];
 #pragma unroll
-  for (int mt = 0; mt < AJ_MT; mt++) arow[mt] = limbs + (m0 + mt * 16 + r16) * m + k_begin + kq;
-
-  const int8_t* brow[AJ_NS];
-  bool bval[AJ_NS];
-  const size_t ncol0 = (size_t)blockIdx.y * AJ_NBLK + (size_t)wave * AJ_NS * 16;
-#pragma unroll
-  for (int ns_ = 0; ns_ < AJ_NS; ns_++) {
-    size_t nc = ncol0 + ns_ * 16 + r16;
-    size_t w = nc / D, t = nc % D;
-    bval[ns_] = w < nw;
-    brow[ns_] = bval[ns_] ? Zs[w] + t * m + k_begin + kq : nullptr;
+  for (int c = 0; c < 2; c++) {
+    int chunk = tid + c * AJ_THREADS, row = chunk >> 2, col = (chunk & 3) * 16;
+    asrc[c] = limbs + (m0 + row) * m + k_begin + col;
+    adst[c] = row * AJ_LDP + col;
   }
+  const int8_t* bsrc[3];
+  int bdst[3];
+  bool bok[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    int chunk = tid + c * AJ_THREADS, row = chunk >> 2, col = (chunk & 3) * 16;
+    size_t nc = n0 + row;
+    size_t w = nc / D, t = nc % D;
+    bok[c] = chunk < AJ_BN * 4 && w < nw;
+    bsrc[c] = bok[c] ? Zs[w] + t * m + k_begin + col : nullptr;
+    bdst[c] = chunk < AJ_BN * 4 ? row * AJ_LDP + col : -1;
+  }
+  v4i ra[2], rb[3];
+  auto gload = [&](size_t k) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) ra[c] = *(const v4i*)(asrc[c] + k);
+#pragma unroll
+    for (int c = 0; c < 3; c++) rb[c] = bok[c] ? *(const v4i*)(bsrc[c] + k) : v4i{0, 0, 0, 0};
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) *(v4i*)(&As[buf][adst[c]]) = ra[c];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if (bdst[c] >= 0) *(v4i*)(&Bs[buf][bdst[c]]) = rb[c];
+  };
 
   v4i acc[AJ_MT][AJ_NS];
 #pragma unroll
@@ -153,32 +179,34 @@ __global__ __launch_bounds__(256) void k_ajtai_gemm(const int8_t* __restrict__ l
 #pragma unroll
     for (int ns_ = 0; ns_ < AJ_NS; ns_++) acc[mt][ns_] = v4i{0, 0, 0, 0};
 
-  // one wave per SIMD: the next k-step's operands are loaded before this step's MFMAs issue
-  v4i a[AJ_MT], b[AJ_NS];
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (size_t k = 0; k < kc; k += AJ_BK) {
+    const bool more = k + AJ_BK < kc;
+    if (more) gload(k + AJ_BK);
+    const int8_t* A = As[buf] + (wm * 64 + r16) * AJ_LDP + kq;
+    const int8_t* B = Bs[buf] + (wn * 176 + r16) * AJ_LDP + kq;
+    v4i a[AJ_MT];
 #pragma unroll
-  for (int mt = 0; mt < AJ_MT; mt++) a[mt] = *(const v4i*)(arow[mt]);
+    for (int mt = 0; mt < AJ_MT; mt++) a[mt] = *(const v4i*)(A + mt * 16 * AJ_LDP);
 #pragma unroll
-  for (int ns_ = 0; ns_ < AJ_NS; ns_++) b[ns_] = bval[ns_] ? *(const v4i*)(brow[ns_]) : v4i{0, 0, 0, 0};
-  for (size_t k = 0; k < kc; k += 64) {
-    v4i an[AJ_MT], bn[AJ_NS];
-    const size_t kn = k + 64 < kc ? k + 64 : k;  // last step reloads its own (in-bounds) slice
+    for (int ns_ = 0; ns_ < AJ_NS; ns_++) {
+      v4i b = *(const v4i*)(B + ns_ * 16 * AJ_LDP);
 #pragma unroll
-    for (int mt = 0; mt < AJ_MT; mt++) an[mt] = *(const v4i*)(arow[mt] + kn);
-#pragma unroll
-    for (int ns_ = 0; ns_ < AJ_NS; ns_++) bn[ns_] = bval[ns_] ? *(const v4i*)(brow[ns_] + kn) : v4i{0, 0, 0, 0};
-#pragma unroll
-    for (int mt = 0; mt < AJ_MT; mt++)
-#pragma unroll
-      for (int ns_ = 0; ns_ < AJ_NS; ns_++)
-        acc[mt][ns_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mt], b[ns_], acc[mt][ns_], 0, 0, 0);
-#pragma unroll
-    for (int mt = 0; mt < AJ_MT; mt++) a[mt] = an[mt];
-#pragma unroll
-    for (int ns_ = 0; ns_ < AJ_NS; ns_++) b[ns_] = bn[ns_];
+      for (int mt = 0; mt < AJ_MT; mt++)
+        acc[mt][ns_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mt], b, acc[mt][ns_], 0, 0, 0);
+    }
+    if (more) {
+      lstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
   }
 
   // D[row][col]: col = lane % 16, row = 4 * (lane / 16) + v
-  int32_t* base = part + ((size_t)blockIdx.z * AJ_MROWS + m0) * npad + ncol0;
+  int32_t* base = part + ((size_t)blockIdx.z * AJ_MROWS + m0 + wm * 64) * npad + n0 + wn * 176;
 #pragma unroll
   for (int mt = 0; mt < AJ_MT; mt++)
 #pragma unroll
@@ -261,9 +289,8 @@ void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* con
   int32_t* part = (int32_t*)workspace;
   int32_t* S = part + splits * (size_t)AJ_MROWS * npad;
   hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
-  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm,
-                  dim3((unsigned)(AJ_MROWS / (AJ_MT * 16)), (unsigned)groups, (unsigned)splits), dim3(64 * AJ_WAVES),
-                  0, s, limbs, m, kc, Zs_dev, nw, npad, part);
+  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)(AJ_MROWS / AJ_BM), (unsigned)groups, (unsigned)splits),
+                  dim3(AJ_THREADS), 0, s, limbs, m, kc, Zs_dev, nw, npad, part);
   const int32_t* G = part;
   if (splits > 1) {
     // reduce the K splits with coalesced loads before the anti-diagonal epilogue
