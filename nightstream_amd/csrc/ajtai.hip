@@ -99,114 +99,151 @@ void launch_chacha8_words(const uint32_t key[8], uint64_t first_block, size_t nb
 // ---------------- Ajtai commit ----------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int AJ_MT = 4;       // 16-row M subtiles per wave
-constexpr int AJ_NS = 11;      // 16-column N subtiles per wave
-// Block tile 256 limb rows x 352 digit rows, 8 waves as 4 (M) x 2 (N), each wave 64 x 176; the
-// block's A and B slices of every 64-wide K step go through double-buffered LDS once, so each
-// PP byte read from L2 feeds 352 columns and each digit byte 256 rows.
-constexpr int AJ_BM = 256, AJ_BN = 352, AJ_BK = 64, AJ_LDP = 80;  // LDS row pitch (bytes)
+constexpr int AJ_MT = 3;   // 16-row M subtiles per wave
+constexpr int AJ_NS = 11;  // 16-column N subtiles per wave
+// Block tile 192 limb rows x 352 digit rows, 8 waves as 4 (M) x 2 (N), each wave 48 x 176.  The
+// block's A and B slices of every 64-wide K step go through double-buffered LDS once (each PP
+// byte read from L2 feeds 352 columns, each digit byte 192 rows).  Global loads run two K steps
+// ahead through two register sets, so a step's loads have two steps of MFMA work to arrive.
+constexpr int AJ_BM = 192, AJ_BN = 352, AJ_BK = 64, AJ_LDP = 80;  // LDS row pitch (bytes)
 constexpr int AJ_THREADS = 512;
 constexpr int AJ_NBLK = AJ_BN;                   // digit rows per N group
-constexpr int AJ_MROWS = KAPPA * 8 * D;          // 6912 = 27 x 256
+constexpr int AJ_MROWS = KAPPA * 8 * D;          // 6912 = 36 x 192
+constexpr int AJ_MBLK = AJ_MROWS / AJ_BM;        // 36
+constexpr int AJ_ACH = AJ_BM * 4, AJ_BCH = AJ_BN * 4;  // 16-byte chunks per K step
 
-static size_t aj_splits(size_t m) {
-  // 108 M tiles x splits blocks: 8 splits give 3.4 blocks per CU on 256 CUs (less tail than 4)
-  size_t s = m / 8192;
-  if (s < 1) s = 1;
-  if (s > 8) s = 8;
-  while (s > 1 && (m / s) % 64 != 0) s--;
-  return s;
-}
 static size_t aj_groups(size_t nw) { return (nw * D + AJ_NBLK - 1) / AJ_NBLK; }
+// K splits: one block per CU at a time (87 KB of LDS), so pick the split count that wastes the
+// least of the last round of 256 blocks: minimise ceil(blocks / 256) / splits (fewest splits on
+// ties, since every split adds a partial-sum plane), with at least 16 K steps per split.
+static size_t aj_splits(size_t m, size_t nw) {
+  const size_t nsteps = m / AJ_BK, per = (size_t)AJ_MBLK * aj_groups(nw);
+  size_t best = 1;
+  double best_t = 1e30;
+  for (size_t sp = 1; sp <= 16 && sp * 16 <= nsteps; sp++) {
+    const double t = (double)((per * sp + 255) / 256) / (double)sp;
+    if (t < best_t - 1e-12) {
+      best_t = t;
+      best = sp;
+    }
+  }
+  return best;
+}
 
 size_t ajtai_workspace_bytes(size_t m, size_t nw) {
   size_t npad = aj_groups(nw) * AJ_NBLK;
-  size_t sp = aj_splits(m);
+  size_t sp = aj_splits(m, nw);
   return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
 }
 
-// grid: x = M tile (256 limb rows), y = N group (352 digit rows), z = K split
-__global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t m, size_t kc,
+// grid: x = M tile (192 limb rows), y = N group (352 digit rows), z = K split (steps
+// [z nsteps / S, (z + 1) nsteps / S) of 64 columns)
+__global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t m, size_t nsteps,
                                                           const int8_t* const* __restrict__ Zs, size_t nw,
                                                           size_t npad, int32_t* __restrict__ part) {
+  typedef const __attribute__((address_space(1))) v4i gv4i;
   __shared__ __align__(16) int8_t As[2][AJ_BM * AJ_LDP];
   __shared__ __align__(16) int8_t Bs[2][AJ_BN * AJ_LDP];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave & 3, wn = wave >> 2;
   const size_t m0 = (size_t)blockIdx.x * AJ_BM;
   const size_t n0 = (size_t)blockIdx.y * AJ_BN;
-  const size_t k_begin = (size_t)blockIdx.z * kc;
+  const size_t S = gridDim.z;
+  const size_t s_begin = (size_t)blockIdx.z * nsteps / S, s_end = ((size_t)blockIdx.z + 1) * nsteps / S;
   const int kq = (lane >> 4) * 16, r16 = lane & 15;
-  // global -> LDS staging: A has 256 x 4 chunks of 16 B (2 per thread), B 352 x 4 (<= 3 per thread)
+  // staging slots: A chunk tid + 512 c (c < 2, valid below 768), B chunk tid + 512 c (c < 3,
+  // valid below 1408 and for digit rows of real witnesses); invalid slots load a valid address
+  // so every thread issues the same loads, and store nothing (A) or zeros (B)
   const int8_t* asrc[2];
   int adst[2];
 #pragma unroll
   for (int c = 0; c < 2; c++) {
-    int chunk = tid + c * AJ_THREADS, row = chunk >> 2, col = (chunk & 3) * 16;
-    asrc[c] = limbs + (m0 + row) * m + k_begin + col;
-    adst[c] = row * AJ_LDP + col;
+    const int chunk = tid + c * AJ_THREADS, ok = chunk < AJ_ACH;
+    const int row = ok ? chunk >> 2 : 0, col = (chunk & 3) * 16;
+    asrc[c] = limbs + (m0 + row) * m + col;
+    adst[c] = ok ? row * AJ_LDP + col : -1;
   }
   const int8_t* bsrc[3];
   int bdst[3];
   bool bok[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    int chunk = tid + c * AJ_THREADS, row = chunk >> 2, col = (chunk & 3) * 16;
-    size_t nc = n0 + row;
-    size_t w = nc / D, t = nc % D;
-    bok[c] = chunk < AJ_BN * 4 && w < nw;
-    bsrc[c] = bok[c] ? Zs[w] + t * m + k_begin + col : nullptr;
-    bdst[c] = chunk < AJ_BN * 4 ? row * AJ_LDP + col : -1;
+    const int chunk = tid + c * AJ_THREADS, inb = chunk < AJ_BCH;
+    const int row = inb ? chunk >> 2 : 0, col = (chunk & 3) * 16;
+    const size_t nc = n0 + row, w = nc / D, t = nc % D;
+    bok[c] = inb && w < nw;
+    bsrc[c] = bok[c] ? Zs[w] + t * m + col : Zs[0] + col;
+    bdst[c] = inb ? row * AJ_LDP + col : -1;
   }
-  v4i ra[2], rb[3];
-  auto gload = [&](size_t k) {
-#pragma unroll
-    for (int c = 0; c < 2; c++) ra[c] = *(const v4i*)(asrc[c] + k);
-#pragma unroll
-    for (int c = 0; c < 3; c++) rb[c] = bok[c] ? *(const v4i*)(bsrc[c] + k) : v4i{0, 0, 0, 0};
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int c = 0; c < 2; c++) *(v4i*)(&As[buf][adst[c]]) = ra[c];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-      if (bdst[c] >= 0) *(v4i*)(&Bs[buf][bdst[c]]) = rb[c];
-  };
+  v4i ra0[2], rb0[3], ra1[2], rb1[3];
+#define AJ_GLOAD(RA, RB, st)                                                        \
+  {                                                                                 \
+    const size_t ko = (size_t)(st) * AJ_BK;                                         \
+    _Pragma("unroll") for (int c = 0; c < 2; c++) RA[c] = *(gv4i*)(asrc[c] + ko);   \
+    _Pragma("unroll") for (int c = 0; c < 3; c++) RB[c] = *(gv4i*)(bsrc[c] + ko);   \
+  }
+#define AJ_LSTORE(RA, RB, buf)                                                      \
+  {                                                                                 \
+    _Pragma("unroll") for (int c = 0; c < 2; c++) if (adst[c] >= 0) *(v4i*)(&As[buf][adst[c]]) = RA[c]; \
+    _Pragma("unroll") for (int c = 0; c < 3; c++)                                   \
+      if (bdst[c] >= 0) *(v4i*)(&Bs[buf][bdst[c]]) = bok[c] ? RB[c] : v4i{0, 0, 0, 0}; \
+  }
 
   v4i acc[AJ_MT][AJ_NS];
 #pragma unroll
   for (int mt = 0; mt < AJ_MT; mt++)
 #pragma unroll
     for (int ns_ = 0; ns_ < AJ_NS; ns_++) acc[mt][ns_] = v4i{0, 0, 0, 0};
-
-  gload(0);
-  lstore(0);
-  __syncthreads();
-  int buf = 0;
-  for (size_t k = 0; k < kc; k += AJ_BK) {
-    const bool more = k + AJ_BK < kc;
-    if (more) gload(k + AJ_BK);
-    const int8_t* A = As[buf] + (wm * 64 + r16) * AJ_LDP + kq;
+  auto compute = [&](int buf) {
+    const int8_t* A = As[buf] + (wm * 48 + r16) * AJ_LDP + kq;
     const int8_t* B = Bs[buf] + (wn * 176 + r16) * AJ_LDP + kq;
     v4i a[AJ_MT];
 #pragma unroll
     for (int mt = 0; mt < AJ_MT; mt++) a[mt] = *(const v4i*)(A + mt * 16 * AJ_LDP);
 #pragma unroll
     for (int ns_ = 0; ns_ < AJ_NS; ns_++) {
-      v4i b = *(const v4i*)(B + ns_ * 16 * AJ_LDP);
+      const v4i b = *(const v4i*)(B + ns_ * 16 * AJ_LDP);
 #pragma unroll
       for (int mt = 0; mt < AJ_MT; mt++)
         acc[mt][ns_] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mt], b, acc[mt][ns_], 0, 0, 0);
     }
-    if (more) {
-      lstore(buf ^ 1);
-      __syncthreads();
-      buf ^= 1;
-    }
+  };
+
+  // Steps past s_end - 1 re-read step s_end - 1 (loaded, never stored).  The loop body has no
+  // exit between a prefetch and its store, and the scheduling barriers keep each prefetch issued
+  // before the MFMAs of the step it overlaps.
+  const size_t last = s_end - 1;
+  AJ_GLOAD(ra0, rb0, s_begin);
+  AJ_GLOAD(ra1, rb1, min(s_begin + 1, last));
+  AJ_LSTORE(ra0, rb0, 0);
+  __syncthreads();
+  size_t st = s_begin;
+  for (; st + 2 < s_end; st += 2) {
+    // step st from buffer 0 (set 1 holds st + 1): prefetch st + 2 into set 0
+    AJ_GLOAD(ra0, rb0, st + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(0);
+    AJ_LSTORE(ra1, rb1, 1);
+    __syncthreads();
+    // step st + 1 from buffer 1 (set 0 holds st + 2): prefetch st + 3 into set 1
+    AJ_GLOAD(ra1, rb1, min(st + 3, last));
+    __builtin_amdgcn_sched_barrier(0);
+    compute(1);
+    AJ_LSTORE(ra0, rb0, 0);
+    __syncthreads();
   }
+  // one or two steps left: st in buffer 0, st + 1 (if any) in set 1
+  compute(0);
+  if (st + 1 < s_end) {
+    AJ_LSTORE(ra1, rb1, 1);
+    __syncthreads();
+    compute(1);
+  }
+#undef AJ_GLOAD
+#undef AJ_LSTORE
 
   // D[row][col]: col = lane % 16, row = 4 * (lane / 16) + v
-  int32_t* base = part + ((size_t)blockIdx.z * AJ_MROWS + m0 + wm * 64) * npad + n0 + wn * 176;
+  int32_t* base = part + ((size_t)blockIdx.z * AJ_MROWS + m0 + wm * 48) * npad + n0 + wn * 176;
 #pragma unroll
   for (int mt = 0; mt < AJ_MT; mt++)
 #pragma unroll
@@ -284,13 +321,13 @@ __global__ __launch_bounds__(512) void k_ajtai_epilogue(const int32_t* __restric
 void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* const* Zs_dev, size_t nw,
                   uint64_t* out, void* workspace, hipStream_t s) {
   if (nw == 0) return;
-  const size_t splits = aj_splits(m), groups = aj_groups(nw), npad = groups * AJ_NBLK;
-  const size_t kc = m / splits;
+  const size_t splits = aj_splits(m, nw), groups = aj_groups(nw), npad = groups * AJ_NBLK;
+  const size_t nsteps = m / AJ_BK;
   int32_t* part = (int32_t*)workspace;
   int32_t* S = part + splits * (size_t)AJ_MROWS * npad;
   hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
-  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)(AJ_MROWS / AJ_BM), (unsigned)groups, (unsigned)splits),
-                  dim3(AJ_THREADS), 0, s, limbs, m, kc, Zs_dev, nw, npad, part);
+  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)AJ_MBLK, (unsigned)groups, (unsigned)splits),
+                  dim3(AJ_THREADS), 0, s, limbs, m, nsteps, Zs_dev, nw, npad, part);
   const int32_t* G = part;
   if (splits > 1) {
     // reduce the K splits with coalesced loads before the anti-diagonal epilogue

@@ -281,8 +281,8 @@ int ns_rlc_zmix(const int8_t* rho_coeffs, const int8_t* const* Z, size_t k, size
   return guard([&] {
     require_device();
     hipStream_t s = default_stream();
-    // rot(a_i) as int8 [k][54][54]
-    std::vector<int8_t> rho(k * D * D);
+    // rot(a_i) as int8 [k][64][64] (row r, column j; zero beyond 54)
+    std::vector<int8_t> rho(k * 64 * 64, 0);
     for (size_t i = 0; i < k; i++) {
       int64_t col[D], nxt[D];
       for (int r = 0; r < D; r++) {
@@ -290,7 +290,7 @@ int ns_rlc_zmix(const int8_t* rho_coeffs, const int8_t* const* Z, size_t k, size
         if (col[r] < -2 || col[r] > 2) throw Error(1, "rho coefficient outside the alphabet [-2, 2]");
       }
       for (int j = 0; j < D; j++) {
-        for (int r = 0; r < D; r++) rho[i * D * D + r * D + j] = (int8_t)col[r];
+        for (int r = 0; r < D; r++) rho[(i * 64 + r) * 64 + j] = (int8_t)col[r];
         int64_t last = col[D - 1];
         nxt[0] = -last;
         for (int r = 1; r < D; r++) nxt[r] = col[r - 1];

@@ -111,75 +111,93 @@ void launch_spmv_t(const DevSparse& M, const K2* chi, size_t n_eff, K2* v, hipSt
 
 // ---------------- Eval column sums ----------------
 // ---------------- contractions over digit rows on int8 MFMA ----------------
-// out[r][c] = sum_i sum_{kk<54} A_i[r][kk] Z_i[kk][c] (int32) for r < arows <= MT*16, with A_i the
-// int8 matrix A[(i*arows + r)*54 + kk].  A block owns 256 columns: per instance it stages A_i and
-// the 54 x 256 digit tile in LDS (the tile transposed so that each lane's B fragment is 16
-// contiguous digit rows of one column) and issues MT x 4 v_mfma_i32_16x16x64_i8 per wave
-// (K = 64 = the 54 digit rows zero-padded).  Requires m % 256 == 0.
+// out[r][c] = sum_i sum_{kk<54} A_i[r][kk] Z_i[kk][c] (int32) for r < arows <= MT*16.  A_i is int8
+// [MT*16][64] at A + i*MT*16*64, zero beyond arows rows and 54 columns.  A block (4 waves) owns
+// 64 columns; wave w takes the instances i = w, w+4, ...  Lane L loads digit rows
+// 16(L/16)..+15 of the columns c0 + 4(L%16) .. +3 as dwords (16 lanes read 64 contiguous bytes
+// of a row) and transposes them in registers into four v_mfma_i32_16x16x64_i8 B fragments:
+// byte j of fragment n is row 16(L/16)+j of column c0 + 4(L%16) + n, so column j of MFMA tile
+// n is global column c0 + 4j + n and each lane ends up owning four adjacent output columns
+// (one 16-byte store per row).  The waves' partial sums meet in LDS.  Requires m % 64 == 0.
 typedef int v4i_t __attribute__((ext_vector_type(4)));
+constexpr int RC_NB = 64;
+constexpr int RC_AP = 64;  // padded row length of A_i
+__device__ __forceinline__ uint32_t byte_col(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3, int n) {
+  const int s = 8 * n;
+  return ((v0 >> s) & 0xFFu) | (((v1 >> s) & 0xFFu) << 8) | (((v2 >> s) & 0xFFu) << 16) | (((v3 >> s) & 0xFFu) << 24);
+}
 template <int MT>
 __global__ __launch_bounds__(256) void k_rowcontract(const int8_t* __restrict__ A, int arows,
                                                      const int8_t* const* __restrict__ Zs, int k, size_t m,
                                                      int32_t* __restrict__ out) {
-  constexpr int NB = 256, TP = 80;
-  __shared__ __align__(16) int8_t T[NB * TP];
-  __shared__ __align__(16) int8_t Ra[MT * 16 * TP];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const size_t c0 = (size_t)blockIdx.x * NB;
-  const int kq = (lane >> 4) * 16, r16 = lane & 15;
+  __shared__ v4i_t red[3][MT * 4][64];
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const size_t c0 = (size_t)blockIdx.x * RC_NB;
+  const int kq = lane >> 4, cg = lane & 15;
   v4i_t acc[MT][4];
 #pragma unroll
   for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) acc[mt][nt] = v4i_t{0, 0, 0, 0};
-  for (int i = 0; i < k; i++) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < MT * 16 * 64; e += 256) {
-      int r = e >> 6, kk = e & 63;
-      Ra[r * TP + kk] = (r < arows && kk < D) ? A[((size_t)i * arows + r) * D + kk] : (int8_t)0;
+    for (int n = 0; n < 4; n++) acc[mt][n] = v4i_t{0, 0, 0, 0};
+  for (int i = wave; i < k; i += 4) {
+    typedef const __attribute__((address_space(1))) uint32_t gu32;
+    const int8_t* Z = Zs[i] + c0 + 4 * cg;
+    uint32_t v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int rho = 16 * kq + j;
+      const uint32_t x = *(gu32*)(Z + (size_t)min(rho, D - 1) * m);  // rows past 53 read row 53
+      v[j] = rho < D ? x : 0u;
     }
-    const int8_t* Z = Zs[i];
-    for (int e = threadIdx.x; e < 64 * (NB / 4); e += 256) {
-      int kk = e / (NB / 4), w4 = e % (NB / 4);
-      uint32_t v = kk < D ? *(const uint32_t*)(Z + (size_t)kk * m + c0 + 4 * w4) : 0u;
-      int8_t* t = T + (4 * w4) * TP + kk;
-      t[0] = (int8_t)(v & 0xFF);
-      t[TP] = (int8_t)((v >> 8) & 0xFF);
-      t[2 * TP] = (int8_t)((v >> 16) & 0xFF);
-      t[3 * TP] = (int8_t)(v >> 24);
-    }
-    __syncthreads();
     v4i_t b[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) b[nt] = *(const v4i_t*)(T + (wave * 64 + nt * 16 + r16) * TP + kq);
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) b[n][q] = (int)byte_col(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3], n);
+    const int8_t* Ai = A + (size_t)i * MT * 16 * RC_AP + cg * RC_AP + 16 * kq;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++) {
-      v4i_t a = *(const v4i_t*)(Ra + (mt * 16 + r16) * TP + kq);
+      const v4i_t a = *(const v4i_t*)(Ai + mt * 16 * RC_AP);
 #pragma unroll
-      for (int nt = 0; nt < 4; nt++) acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+      for (int n = 0; n < 4; n++) acc[mt][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[n], acc[mt][n], 0, 0, 0);
     }
   }
+  if (wave > 0)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int n = 0; n < 4; n++) red[wave - 1][mt * 4 + n][lane] = acc[mt][n];
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int w = 0; w < 3; w++)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int n = 0; n < 4; n++) acc[mt][n] += red[w][mt * 4 + n][lane];
 #pragma unroll
   for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        int row = mt * 16 + 4 * (lane >> 4) + v;
-        if (row < arows) out[(size_t)row * m + c0 + wave * 64 + nt * 16 + r16] = acc[mt][nt][v];
-      }
+    for (int v = 0; v < 4; v++) {
+      const int row = mt * 16 + 4 * kq + v;
+      if (row < arows)
+        *(v4i_t*)(out + (size_t)row * m + c0 + 4 * cg) = v4i_t{acc[mt][0][v], acc[mt][1][v], acc[mt][2][v], acc[mt][3][v]};
+    }
 }
 
-// A_i rows for the weighted column sum: bytes of W.c0 and W.c1 (each minus 128), then a row of ones
+// A_i rows for the weighted column sum: bytes of W.c0 and W.c1 (each minus 128), then a row of
+// ones; int8 [32][64] per instance, zero beyond row 17 and column 54
 __global__ void k_w_limbs(const K2* __restrict__ W, int nw, int8_t* __restrict__ A) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nw * 17 * D) return;
-  int i = idx / (17 * D), r = (idx / D) % 17, rho = idx % D;
-  K2 w = W[i * D + rho];
-  int8_t v;
-  if (r < 8) v = (int8_t)((int)((w.c0 >> (8 * r)) & 0xFF) - 128);
-  else if (r < 16) v = (int8_t)((int)((w.c1 >> (8 * (r - 8))) & 0xFF) - 128);
-  else v = 1;
+  if (idx >= nw * 32 * RC_AP) return;
+  int i = idx / (32 * RC_AP), r = (idx / RC_AP) % 32, rho = idx % RC_AP;
+  int8_t v = 0;
+  if (r < 17 && rho < D) {
+    K2 w = W[i * D + rho];
+    if (r < 8) v = (int8_t)((int)((w.c0 >> (8 * r)) & 0xFF) - 128);
+    else if (r < 16) v = (int8_t)((int)((w.c1 >> (8 * (r - 8))) & 0xFF) - 128);
+    else v = 1;
+  }
   A[idx] = v;
 }
 
@@ -199,7 +217,7 @@ __global__ void k_colsum_epilogue(const int32_t* __restrict__ G, size_t m, K2* _
   S[c] = K2{comp[0], comp[1]};
 }
 
-size_t colsum_ws_bytes(size_t nw, size_t m) { return nw * 17 * D + 256 + 17 * m * 4 + 256; }
+size_t colsum_ws_bytes(size_t nw, size_t m) { return nw * 32 * RC_AP + 256 + 17 * m * 4 + 256; }
 
 __global__ __launch_bounds__(256) void k_weighted_colsum(const int8_t* const* __restrict__ Zs, int nw,
                                                          const K2* __restrict__ W, size_t m, K2* __restrict__ S) {
@@ -231,16 +249,16 @@ void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const 
     NS_CHECK(hipMemsetAsync(S, 0, m * sizeof(K2), s));
     return;
   }
-  if (m % 256 != 0 || !ws) {
+  if (m % RC_NB != 0 || !ws) {
     hipLaunchKernelGGL(k_weighted_colsum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, Zs_dev, (int)nw, W, m,
                        S);
     return;
   }
   int8_t* A = (int8_t*)ws;
-  int32_t* G = (int32_t*)(((uintptr_t)(A + nw * 17 * D) + 255) & ~(uintptr_t)255);
-  int na = (int)(nw * 17 * D);
+  int32_t* G = (int32_t*)(((uintptr_t)(A + nw * 32 * RC_AP) + 255) & ~(uintptr_t)255);
+  int na = (int)(nw * 32 * RC_AP);
   hipLaunchKernelGGL(k_w_limbs, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, W, (int)nw, A);
-  hipLaunchKernelGGL(k_rowcontract<2>, dim3((unsigned)(m / 256)), dim3(256), 0, s, A, 17, Zs_dev, (int)nw, m, G);
+  hipLaunchKernelGGL(k_rowcontract<2>, dim3((unsigned)(m / RC_NB)), dim3(256), 0, s, A, 17, Zs_dev, (int)nw, m, G);
   hipLaunchKernelGGL(k_colsum_epilogue, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, G, m, S);
 }
 
@@ -432,33 +450,52 @@ __device__ __forceinline__ K2 wave_reduce(K2 v) {
 template <int MODE>
 __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
                                                    int G) {
-  extern __shared__ K2 sh[];  // wg cache [k*54], round-1 monomial polynomials [80], reduction scratch
+  // LDS: wg [k*54], round-1 monomial polynomials [80], reduction scratch [20], then -wg [k*54]
+  // (rounds 0 and 1) or 7 wg.c1 [k*54] (K tables), then 7 pc.c1 [80]
+  extern __shared__ K2 sh[];
   const int nrow = A.k * D;
-  for (int i = threadIdx.x; i < nrow; i += blockDim.x) sh[i] = wg[i];
   K2* pcs = sh + nrow;
+  K2* nsh = sh + nrow + 100;
+  uint64_t* w7s = (uint64_t*)(sh + nrow + 100);
+  uint64_t* pcs7 = (uint64_t*)(sh + 2 * nrow + 100);
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+    const K2 w = wg[i];
+    sh[i] = w;
+    if (MODE == 2)
+      w7s[i] = fmul7(w.c1);
+    else
+      nsh[i] = kneg(w);
+  }
+  __shared__ const int8_t* dptr[32];  // digit planes of the k instances (rounds 0 and 1)
+  if (MODE != 2 && threadIdx.x < A.k && threadIdx.x < 32) dptr[threadIdx.x] = A.digits[threadIdx.x];
   if (MODE == 1)
-    for (int i = threadIdx.x; i < 80; i += blockDim.x) pcs[i] = A.pc[i];
+    for (int i = threadIdx.x; i < 80; i += blockDim.x) {
+      const K2 p = A.pc[i];
+      pcs[i] = p;
+      pcs7[i] = fmul7(p.c1);
+    }
   __syncthreads();
   const size_t pairs = A.len / 2;
   const int R = (nrow + G - 1) / G;
+  // groups 0..G-1 take R digit rows each; group G takes the f and Eval terms of the pair
   K2 c[5];
 #pragma unroll
   for (int d = 0; d < 5; d++) c[d] = kzero();
   K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
   // round 1 splits each (pair, row group) item into two halves of the 20 digit monomials
   const int H = MODE == 1 ? 2 : 1;
-  const size_t total = pairs * (size_t)G * H;
+  const size_t total = pairs * ((size_t)G * H + 1);
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const size_t t = idx % pairs;
     const int gh = (int)(idx / pairs);
     const int g = gh / H, h = gh % H;
-    const int r_lo = g * R, r_hi = min(nrow, r_lo + R);
+    const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
     const K2 e0 = A.eqb[2 * t];
     const K2 e1 = ksub(A.eqb[2 * t + 1], e0);
     K2 in[5];
 #pragma unroll
     for (int d = 0; d < 5; d++) in[d] = kzero();
-    if (g == 0 && h == 0) {
+    if (g == G) {
       // f(M z1)(X)
       for (int tt = 0; tt < A.f.nterms; tt++) {
         K2 p[5];
@@ -485,23 +522,36 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
         ev2 = kadd(ev2, kmul(r1, v1));
       }
     }
-    if (MODE == 0) {
-      // signed selections of wg
-      K2 slo = kzero(), shi = kzero(), s3 = kzero(), s4 = kzero();
-      for (int r = r_lo; r < r_hi; r++) {
-        const int w = r / D, rho = r - w * D;
-        const int8_t* z = A.digits[w] + (size_t)rho * A.m + 2 * t;
-        int lo = z[0], hi = z[1];
-        if ((lo | hi) == 0) continue;
-        K2 wv = sh[r];
-        K2 nw = kneg(wv);
-        if (lo) slo = kadd(slo, lo > 0 ? wv : nw);
-        if (hi) shi = kadd(shi, hi > 0 ? wv : nw);
-        if (lo && hi) {
-          s3 = kadd(s3, hi > 0 ? wv : nw);  // lo^2 hi = hi
-          s4 = kadd(s4, lo > 0 ? wv : nw);  // lo hi^2 = lo
+    if (g == G) {
+      // no rows
+    } else if (MODE == 0) {
+      // signed selections of wg, summed lazily (+w or q - w) and reduced once per work item
+      KS1 zlo = ks1_zero(), zhi = ks1_zero(), z3 = ks1_zero(), z4 = ks1_zero();
+      for (int r0 = r_lo; r0 < r_hi; r0 += 8) {
+        // eight rows' digit pairs in flight at once (rows past r_hi re-read row r_hi - 1, then count as 0)
+        uint32_t qd[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int rr = min(r0 + u, r_hi - 1), w = rr / D, rho = rr - w * D;
+          const uint32_t x = *(const uint16_t*)(dptr[w] + (size_t)rho * A.m + 2 * t);
+          qd[u] = r0 + u < r_hi ? x : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+        const int r = min(r0 + u, r_hi - 1);
+        if (qd[u] == 0) continue;
+        const int lo = (int)(int8_t)(qd[u] & 0xFF), hi = (int)(int8_t)(qd[u] >> 8);
+        const K2 wv = sh[r], nw = nsh[r];
+        const K2 vlo = lo > 0 ? wv : (lo < 0 ? nw : kzero());
+        const K2 vhi = hi > 0 ? wv : (hi < 0 ? nw : kzero());
+        const bool both = lo != 0 && hi != 0;
+        ks1_add(zlo, vlo);
+        ks1_add(zhi, vhi);
+        ks1_add(z3, both ? vhi : kzero());  // lo^2 hi = hi
+        ks1_add(z4, both ? vlo : kzero());  // lo hi^2 = lo
         }
       }
+      const K2 slo = ks1_reduce(zlo), shi = ks1_reduce(zhi), s3 = ks1_reduce(z3), s4 = ks1_reduce(z4);
       // c1 = -2 slo - shi + 3 s3 ; c2 = 3 slo - 6 s3 + 3 s4 ; c3 = -slo + shi + 3 s3 - 3 s4
       K2 s3x3 = kadd(kadd(s3, s3), s3);
       K2 s4x3 = kadd(kadd(s4, s4), s4);
@@ -519,26 +569,37 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
 #define NS_UPD(acc, mv)                                         \
   {                                                             \
     const int mm = (mv);                                        \
-    acc = kadd(acc, mm > 0 ? wv : (mm < 0 ? nw : kzero()));     \
+    ks1_add(acc, mm > 0 ? wv : (mm < 0 ? nw : kzero()));        \
   }
-#define NS_APPLY(j, acc)                                        \
-  {                                                             \
-    in[0] = kadd(in[0], kmul(pcs[(j)*4 + 0], acc));             \
-    in[1] = kadd(in[1], kmul(pcs[(j)*4 + 1], acc));             \
-    in[2] = kadd(in[2], kmul(pcs[(j)*4 + 2], acc));             \
-    in[3] = kadd(in[3], kmul(pcs[(j)*4 + 3], acc));             \
+#define NS_APPLY(j, acc)                                                         \
+  {                                                                              \
+    const K2 av = ks1_reduce(acc);                                               \
+    in[0] = kadd(in[0], kmul_w7(pcs[(j)*4 + 0], pcs7[(j)*4 + 0], av));           \
+    in[1] = kadd(in[1], kmul_w7(pcs[(j)*4 + 1], pcs7[(j)*4 + 1], av));           \
+    in[2] = kadd(in[2], kmul_w7(pcs[(j)*4 + 2], pcs7[(j)*4 + 2], av));           \
+    in[3] = kadd(in[3], kmul_w7(pcs[(j)*4 + 3], pcs7[(j)*4 + 3], av));           \
   }
       if (h == 0) {
-        K2 a0 = kzero(), a1 = kzero(), a2 = kzero(), a3 = kzero(), a4 = kzero(), a5 = kzero(), a6 = kzero(), a7 = kzero(), a8 = kzero(), a9 = kzero();
-        for (int r = r_lo; r < r_hi; r++) {
-          const int w = r / D, rho = r - w * D;
-          const uint32_t q = *(const uint32_t*)(A.digits[w] + (size_t)rho * A.m + 4 * t);
-          if (q == 0) continue;
+        KS1 a0 = ks1_zero(), a1 = ks1_zero(), a2 = ks1_zero(), a3 = ks1_zero(), a4 = ks1_zero(), a5 = ks1_zero(),
+            a6 = ks1_zero(), a7 = ks1_zero(), a8 = ks1_zero(), a9 = ks1_zero();
+        for (int r0 = r_lo; r0 < r_hi; r0 += 8) {
+          uint32_t qd[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int rr = min(r0 + u, r_hi - 1), w = rr / D, rho = rr - w * D;
+            const uint32_t x = *(const uint32_t*)(dptr[w] + (size_t)rho * A.m + 4 * t);
+            qd[u] = r0 + u < r_hi ? x : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+          const int r = min(r0 + u, r_hi - 1);
+          const uint32_t q = qd[u];
+          if (q == 0) continue;  // most digit words of the DEC children are zero
           const int d0 = (int)(int8_t)(q & 0xFF), d1 = (int)(int8_t)((q >> 8) & 0xFF),
                     d2 = (int)(int8_t)((q >> 16) & 0xFF), d3 = (int)(int8_t)(q >> 24);
           const int e0 = d0 & 1, e1 = d1 & 1, e2 = d2 & 1, e3 = d3 & 1;
           (void)e0; (void)e1; (void)e2; (void)e3;
-          const K2 wv = sh[r], nw = kneg(wv);
+          const K2 wv = sh[r], nw = nsh[r];
           NS_UPD(a0, d0)
           NS_UPD(a1, d1)
           NS_UPD(a2, d2)
@@ -549,6 +610,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
           NS_UPD(a7, e1 * d0)
           NS_UPD(a8, e1 * d2)
           NS_UPD(a9, e1 * d3)
+          }
         }
         NS_APPLY(0, a0)
         NS_APPLY(1, a1)
@@ -562,16 +624,26 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
         NS_APPLY(9, a9)
       }
       else {
-        K2 a10 = kzero(), a11 = kzero(), a12 = kzero(), a13 = kzero(), a14 = kzero(), a15 = kzero(), a16 = kzero(), a17 = kzero(), a18 = kzero(), a19 = kzero();
-        for (int r = r_lo; r < r_hi; r++) {
-          const int w = r / D, rho = r - w * D;
-          const uint32_t q = *(const uint32_t*)(A.digits[w] + (size_t)rho * A.m + 4 * t);
-          if (q == 0) continue;
+        KS1 a10 = ks1_zero(), a11 = ks1_zero(), a12 = ks1_zero(), a13 = ks1_zero(), a14 = ks1_zero(),
+            a15 = ks1_zero(), a16 = ks1_zero(), a17 = ks1_zero(), a18 = ks1_zero(), a19 = ks1_zero();
+        for (int r0 = r_lo; r0 < r_hi; r0 += 8) {
+          uint32_t qd[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int rr = min(r0 + u, r_hi - 1), w = rr / D, rho = rr - w * D;
+            const uint32_t x = *(const uint32_t*)(dptr[w] + (size_t)rho * A.m + 4 * t);
+            qd[u] = r0 + u < r_hi ? x : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+          const int r = min(r0 + u, r_hi - 1);
+          const uint32_t q = qd[u];
+          if (q == 0) continue;  // most digit words of the DEC children are zero
           const int d0 = (int)(int8_t)(q & 0xFF), d1 = (int)(int8_t)((q >> 8) & 0xFF),
                     d2 = (int)(int8_t)((q >> 16) & 0xFF), d3 = (int)(int8_t)(q >> 24);
           const int e0 = d0 & 1, e1 = d1 & 1, e2 = d2 & 1, e3 = d3 & 1;
           (void)e0; (void)e1; (void)e2; (void)e3;
-          const K2 wv = sh[r], nw = kneg(wv);
+          const K2 wv = sh[r], nw = nsh[r];
           NS_UPD(a10, e2 * d0)
           NS_UPD(a11, e2 * d1)
           NS_UPD(a12, e2 * d3)
@@ -582,6 +654,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
           NS_UPD(a17, d0 * d1 * d3)
           NS_UPD(a18, d0 * d2 * d3)
           NS_UPD(a19, d1 * d2 * d3)
+          }
         }
         NS_APPLY(10, a10)
         NS_APPLY(11, a11)
@@ -597,29 +670,45 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
 #undef NS_UPD
 #undef NS_APPLY
     } else {
-      K2 s1 = kzero(), s2 = kzero(), s3 = kzero(), s4 = kzero(), s5 = kzero(), s6 = kzero();
+      // w N(a + b X) = w (a^3 - a) + w (3 a^2 b - b) X + 3 w a b^2 X^2 + w b^3 X^3: wa, wb, a^2, b^2
+      // reduced per row, the four products and two sums accumulated lazily over the rows
+      KS2 z1 = ks2_zero(), z3 = ks2_zero(), z5 = ks2_zero(), z6 = ks2_zero();
+      KS1 z2 = ks1_zero(), z4 = ks1_zero();
+      const K2* rowp = A.nc + (size_t)r_lo * A.len + 2 * t;
+      K2 n0 = r_lo < r_hi ? rowp[0] : kzero(), n1 = r_lo < r_hi ? rowp[1] : kzero();
       for (int r = r_lo; r < r_hi; r++) {
-        const K2* row = A.nc + (size_t)r * A.len + 2 * t;
-        K2 a = row[0];
-        K2 b = ksub(row[1], a);
-        K2 wv = sh[r];
-        K2 wa = kmul(wv, a), wb = kmul(wv, b);
-        K2 a2 = ksqr(a), b2 = ksqr(b);
-        s1 = kadd(s1, kmul(wa, a2));
-        s2 = kadd(s2, wa);
-        s3 = kadd(s3, kmul(wb, a2));
-        s4 = kadd(s4, wb);
-        s5 = kadd(s5, kmul(wa, b2));
-        s6 = kadd(s6, kmul(wb, b2));
+        const K2 a = n0;
+        const K2 b = ksub(n1, a);
+        if (r + 1 < r_hi) {  // next row's pair in flight during this row's arithmetic
+          rowp += A.len;
+          n0 = rowp[0];
+          n1 = rowp[1];
+        }
+        const K2 wv = sh[r];
+        const uint64_t w7 = w7s[r];
+        const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
+        const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
+        ks2_mac(z1, wa, a2);
+        ks1_add(z2, wa);
+        ks2_mac(z3, wb, a2);
+        ks1_add(z4, wb);
+        ks2_mac(z5, wa, b2);
+        ks2_mac(z6, wb, b2);
       }
+      const K2 s1 = ks2_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2_reduce(z3), s4 = ks1_reduce(z4),
+               s5 = ks2_reduce(z5), s6 = ks2_reduce(z6);
       in[0] = kadd(in[0], ksub(s1, s2));
       in[1] = kadd(in[1], ksub(kadd(kadd(s3, s3), s3), s4));
       in[2] = kadd(in[2], kadd(kadd(s5, s5), s5));
       in[3] = kadd(in[3], s6);
     }
-    c[0] = kadd(c[0], kmul(e0, in[0]));
+    // times eq_beta(X) = e0 + e1 X (in[4] is nonzero only for the f group)
+    const uint64_t e07 = fmul7(e0.c1), e17 = fmul7(e1.c1);
+    c[0] = kadd(c[0], kmul_w7(e0, e07, in[0]));
 #pragma unroll
-    for (int d = 1; d < 5; d++) c[d] = kadd(c[d], kadd(kmul(e0, in[d]), kmul(e1, in[d - 1])));
+    for (int d = 1; d < 4; d++) c[d] = kadd(c[d], kadd(kmul_w7(e0, e07, in[d]), kmul_w7(e1, e17, in[d - 1])));
+    c[4] = kadd(c[4], kmul_w7(e1, e17, in[3]));
+    if (g == G) c[4] = kadd(c[4], kmul_w7(e0, e07, in[4]));
   }
   if (A.eqr) {
     c[0] = kadd(c[0], kmul(A.gamma_k, ev0));
@@ -673,24 +762,28 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   A.gamma_k = T.gamma_k;
   A.f = f;
   size_t pairs = T.len / 2;
-  // split the k*54 NC rows of a pair across G work items when the table is short
-  // (round 1 from digits applies 80 coefficient products per work item: keep its items long)
-  const size_t target = T.digit_round1 ? (size_t)RB * 512 : (size_t)RMAXBLK * RB / 2;
-  int G = 1;
-  while ((size_t)G * 2 <= (size_t)T.k * D && pairs * (size_t)G * 2 <= target) G *= 2;
+  // Split the k*54 NC rows of a pair across G work items: long tables get long items (each item
+  // ends with a few reductions and, in round 1, 40 coefficient products), short tables get as
+  // many items as it takes to fill the GPU, down to one row per item.
+  const size_t nrow = (size_t)T.k * D;
+  const size_t target = round0 ? (size_t)RB * 1024 : (size_t)RB * 512;
+  size_t G = 1;
+  while (G < nrow && pairs * G < target) G *= 2;
+  if (G > nrow) G = nrow;
+  G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
   const size_t H = T.digit_round1 ? 2 : 1;
-  size_t nblk = (pairs * (size_t)G * H + RB - 1) / RB;
+  size_t nblk = (pairs * (G * H + 1) + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
-  size_t shmem = ((size_t)T.k * D + 80 + 4 * 5) * sizeof(K2);
+  size_t shmem = ((size_t)T.k * D * 2 + 80 + 4 * 5 + 40) * sizeof(K2);
   K2* partials = (K2*)ws;
   A.pc = T.pc;
   if (round0)
-    NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<0>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
+    NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<0>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
   else if (T.digit_round1)
-    NS_LAUNCH_TIMED(KT_ROUND1, k_round_eval<1>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
+    NS_LAUNCH_TIMED(KT_ROUND1, k_round_eval<1>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
   else
-    NS_LAUNCH_TIMED(KT_ROUND, k_round_eval<2>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, G);
+    NS_LAUNCH_TIMED(KT_ROUND, k_round_eval<2>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
   hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
 }
 
@@ -762,10 +855,10 @@ void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t
 // ---------------- Pi_RLC / Pi_DEC ----------------
 __global__ __launch_bounds__(256) void k_rlc_zmix(const int8_t* __restrict__ rho, const int8_t* const* __restrict__ Zs,
                                                   int k, size_t m, int32_t* __restrict__ zmix) {
-  extern __shared__ int8_t rho_s[];  // [i][kk][r]
+  extern __shared__ int8_t rho_s[];  // [i][kk][r] from the padded [i][64][64] rows
   for (int idx = threadIdx.x; idx < k * D * D; idx += blockDim.x) {
     int i = idx / (D * D), rem = idx % (D * D), kk = rem / D, r = rem % D;
-    rho_s[idx] = rho[(size_t)i * D * D + r * D + kk];
+    rho_s[idx] = rho[((size_t)i * RC_AP + r) * RC_AP + kk];
   }
   __syncthreads();
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -792,8 +885,8 @@ __global__ __launch_bounds__(256) void k_rlc_zmix(const int8_t* __restrict__ rho
 
 void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
                      hipStream_t s) {
-  if (m % 256 == 0) {
-    NS_LAUNCH_TIMED(KT_ZMIX, k_rowcontract<4>, dim3((unsigned)(m / 256)), dim3(256), 0, s, rho_dev, (int)D, Zs_dev,
+  if (m % RC_NB == 0) {
+    NS_LAUNCH_TIMED(KT_ZMIX, k_rowcontract<4>, dim3((unsigned)(m / RC_NB)), dim3(256), 0, s, rho_dev, (int)D, Zs_dev,
                     (int)k, m, zmix);
     return;
   }

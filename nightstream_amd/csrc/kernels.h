@@ -126,6 +126,7 @@ void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t
 
 // ---- Pi_RLC / Pi_DEC ----
 // zmix[r][c] = sum_i sum_kk rho_i[r][kk] Z_i[kk][c]; rho given as int8 [k][54][54]
+// rho_dev: the k rotation matrices as int8 [k][64][64] (row r, column kk; zero beyond 54)
 void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
                      hipStream_t s);
 // balanced base-2 digits of zmix into k_dec planes; flags[i] |= plane i nonzero; flags[k_dec] = range error

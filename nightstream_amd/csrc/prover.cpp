@@ -1124,7 +1124,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       throw Error(5, "RLC norm bound violated");
     std::vector<int8_t> rho_a(kt * D);
     std::vector<uint64_t> rhos(kt * D * D, 0);
-    std::vector<int8_t> rho_i8(kt * D * D, 0);
+    std::vector<int8_t> rho_i8(kt * 64 * 64, 0);  // [i][row][col] padded to 64 x 64
     for (size_t i = 0; i < kt; i++) {
       tr.append_msg_u64("rlc/rot/index", i);
       // draw_alphabet_vector (neo-reductions/src/common.rs:410-439)
@@ -1161,7 +1161,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       for (int q = 0; q < D * D; q++) {
         int64_t v = to_balanced(R[q]);
         if (v < -127 || v > 127) throw Error(5, "rotation entry out of int8 range");
-        rho_i8[i * D * D + q] = (int8_t)v;
+        rho_i8[(i * 64 + q / D) * 64 + q % D] = (int8_t)v;
       }
     }
     MeHost parent;
