@@ -128,6 +128,8 @@ int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, siz
  * row rounds, y products, Ajtai rounds, ME outputs, RLC binding, Z_mix, DEC, encode, intake,
  * accumulator swap). */
 int ns_prover_phase_stats(const ns_prover* p, double* seconds, size_t n);
+/* Poseidon2 permutations of the host transcript per phase of the last prove (same 12 phases). */
+int ns_prover_phase_perms(const ns_prover* p, uint64_t* perms, size_t n);
 void ns_prover_free(ns_prover* p);
 
 /* ---- Twist/Shout round oracles (neo-memory/src/twist_oracle.rs).  Each handle implements the

@@ -25,7 +25,7 @@ EXPORTS = [
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
     "ns_dec_split", "ns_prover_new", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
-    "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_free",
+    "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
     "ns_ts_twist_read_new", "ns_ts_twist_write_new", "ns_ts_val_eval_new", "ns_ts_total_inc_new",
     "ns_ts_read_addr_new", "ns_ts_write_addr_new", "ns_ts_addr_lookup_new", "ns_ts_evals", "ns_ts_fold",
@@ -92,6 +92,7 @@ def lib():
     L.ns_prover_set_timing.argtypes = [vp, C.c_int]
     L.ns_prover_kernel_stats.argtypes = [vp, vp, vp, sz]
     L.ns_prover_phase_stats.argtypes = [vp, vp, sz]
+    L.ns_prover_phase_perms.argtypes = [vp, vp, sz]
     L.ns_params_goldilocks_auto_r1cs_ccs.argtypes = [sz, vp]
     _lib = L
     return L
@@ -369,6 +370,12 @@ class FoldingSession:
         out = np.zeros(12, np.float64)
         _check(lib().ns_prover_phase_stats(self.h, _ptr(out), 12))
         return {k: round(float(v), 5) for k, v in zip(self.PHASES, out)}
+
+    def phase_perms(self) -> dict:
+        """Poseidon2 permutations of the host transcript per phase of the last prove."""
+        out = np.zeros(12, np.uint64)
+        _check(lib().ns_prover_phase_perms(self.h, _ptr(out), 12))
+        return {k: int(v) for k, v in zip(self.PHASES, out)}
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -400,6 +400,12 @@ int ns_prover_phase_stats(const ns_prover* p, double* seconds, size_t n) {
   });
 }
 
+int ns_prover_phase_perms(const ns_prover* p, uint64_t* perms, size_t n) {
+  return guard([&] {
+    for (size_t i = 0; i < n && i < (size_t)Prover::NPHASE; i++) perms[i] = p->p->phase_perm[i];
+  });
+}
+
 void ns_prover_free(ns_prover* p) { delete p; }
 
 }  // extern "C"

@@ -815,13 +815,20 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   if (m_in > m) throw Error(1, "m_in exceeds m");
   hipStream_t st = stream_;
   auto t0 = clk::now();
-  for (int i = 0; i < NPHASE; i++) phase_s[i] = 0;
+  for (int i = 0; i < NPHASE; i++) phase_s[i] = 0, phase_perm[i] = 0;
   auto tmark = clk::now();
   int cur_phase = -1;
+  const Transcript* trp = nullptr;
+  uint64_t pmark = 0;
   auto phase_mark = [&](int next) {
     auto now = clk::now();
-    if (cur_phase >= 0) phase_s[cur_phase] += std::chrono::duration<double>(now - tmark).count();
+    const uint64_t pn = trp ? trp->permutations() : 0;
+    if (cur_phase >= 0) {
+      phase_s[cur_phase] += std::chrono::duration<double>(now - tmark).count();
+      phase_perm[cur_phase] += pn - pmark;
+    }
     tmark = now;
+    pmark = pn;
     cur_phase = next;
   };
   phase_mark(10);  // step intake
@@ -861,6 +868,8 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   pb.u(0x314652504f52504eULL);
   pb.u(nsteps);
   Transcript tr("neo.fold/session");
+  trp = &tr;
+  pmark = tr.permutations();
 
   std::vector<MeHost> acc;
   std::vector<const int8_t*> acc_wit;  // digit planes of the accumulator (planes_[cur_planes])

@@ -126,6 +126,7 @@ class Prover {
   // 7 Z_mix, 8 DEC, 9 encode, 10 step intake (decomp + MCS commits), 11 accumulator swap.
   static constexpr int NPHASE = 12;
   double phase_s[NPHASE] = {0};
+  uint64_t phase_perm[NPHASE] = {0};
   double kernel_ms[KT_N] = {0};
   uint64_t kernel_count[KT_N] = {0};
   uint64_t last_permutations = 0;
