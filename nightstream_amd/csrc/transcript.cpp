@@ -290,10 +290,18 @@ NS_AVX512 static void permute8_avx512(const Poseidon2& P, uint64_t* st) {
   for (int r = 0; r < 22; r++) {
     __m512i mv = v_mul(mu, v);
     u128 W = v_hsum(mv);
-    uint64_t x7 = sbox7(s0);
+    // s0 -> s0^7 (diag0 + 1) in three dependent multiplies: s0 (diag0 + 1) is formed beside s0^2
+    // and carried through s0^3 (diag0 + 1) = s0^2 * it; s0^7 itself comes off the chain
+    const uint64_t xc = red((u128)s0 * P.diag0p1);
+    const uint64_t x2 = red((u128)s0 * s0);
+    const uint64_t x3 = red((u128)x2 * s0);
+    const uint64_t x3c = red((u128)x2 * xc);
+    const uint64_t x4 = red((u128)x2 * x2);
+    const uint64_t x7 = red((u128)x3 * x4);
+    u128 acc = (u128)x3c * x4;
     uint64_t sum = red((u128)restr + x7);
     v = _mm512_mask_blend_epi64(1, v_addq(mv, _mm512_set1_epi64((long long)sum)), _mm512_setzero_si512());
-    u128 acc = (u128)x7 * P.diag0p1 + restr;
+    acc = (u128)red(acc) + restr;
     if (r < 21) acc += P.internal_rc[r + 1];
     s0 = red(acc);
     restr = red(W + (u128)sum * 7);
