@@ -35,17 +35,20 @@ void launch_decomp_b2(const uint64_t* z, size_t m, int8_t* Z, hipStream_t s) {
 }
 
 // ---------------- PP limbs ----------------
-// limbs[((i*8 + l)*54 + u) * m + j] = byte_l(pp[(i*m + j)*54 + u]) - 128
+// Row (i*8 + l)*54 + u of the 6912-row limb matrix holds byte_l(pp[(i*m + j)*54 + u]) - 128 at
+// column j.  Stored K-step-major: limbs[((j / 64) * 6912 + row) * 64 + j % 64], so the 64-column
+// slice of every row for one K step of the commit GEMM is one contiguous 442 KB region.
 __global__ void k_pp_limbs(const uint64_t* __restrict__ pp, size_t kappa, size_t m, int8_t* __restrict__ limbs) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t i = blockIdx.y;
   if (j >= m) return;
   const uint64_t* a = pp + (i * m + j) * D;
+  int8_t* col = limbs + (j >> 6) * (size_t)(KAPPA * 8 * D) * 64 + (j & 63);
   for (int u = 0; u < D; u++) {
     uint64_t v = a[u];
     for (int l = 0; l < 8; l++) {
       int b = (int)((v >> (8 * l)) & 0xFF) - 128;
-      limbs[((i * 8 + l) * D + u) * m + j] = (int8_t)b;
+      col[((i * 8 + l) * D + u) * 64] = (int8_t)b;
     }
   }
 }
@@ -101,16 +104,21 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int AJ_MT = 3;   // 16-row M subtiles per wave
 constexpr int AJ_NS = 11;  // 16-column N subtiles per wave
-// Block tile 192 limb rows x 352 digit rows, 8 waves as 4 (M) x 2 (N), each wave 48 x 176.  The
-// block's A and B slices of every 64-wide K step go through double-buffered LDS once (each PP
-// byte read from L2 feeds 352 columns, each digit byte 192 rows).  Global loads run two K steps
-// ahead through two register sets, so a step's loads have two steps of MFMA work to arrive.
+// Block tile 192 limb rows x 352 digit rows, 8 waves as 4 (M) x 2 (N), each wave 48 x 176: per
+// 64-wide K step a block stages 34 KB into LDS for 264 MFMAs (a 96 x 704 tile, which reads each
+// PP byte once instead of twice, stages 51 KB and measured slower).  The blocks resident on one
+// XCD (see xcd_remap) work on one K range at a time, so the digit tile of a K step is served from
+// that XCD's L2.  A and B slices go through double-buffered LDS (pitch 80 B: the 16 rows of a
+// fragment read hit all 64 banks once); global loads run two K steps ahead through two register
+// sets.
 constexpr int AJ_BM = 192, AJ_BN = 352, AJ_BK = 64, AJ_LDP = 80;  // LDS row pitch (bytes)
 constexpr int AJ_THREADS = 512;
 constexpr int AJ_NBLK = AJ_BN;                   // digit rows per N group
 constexpr int AJ_MROWS = KAPPA * 8 * D;          // 6912 = 36 x 192
 constexpr int AJ_MBLK = AJ_MROWS / AJ_BM;        // 36
-constexpr int AJ_ACH = AJ_BM * 4, AJ_BCH = AJ_BN * 4;  // 16-byte chunks per K step
+constexpr int AJ_ACH = AJ_BM * 4, AJ_BCH = AJ_BN * 4;  // 16-byte chunks per K step: 768, 1408
+constexpr int AJ_ASL = (AJ_ACH + AJ_THREADS - 1) / AJ_THREADS;  // A chunks per thread
+constexpr int AJ_BSL = (AJ_BCH + AJ_THREADS - 1) / AJ_THREADS;  // B chunks per thread
 
 static size_t aj_groups(size_t nw) { return (nw * D + AJ_NBLK - 1) / AJ_NBLK; }
 // K splits: one block per CU at a time (87 KB of LDS), so pick the split count that wastes the
@@ -136,57 +144,72 @@ size_t ajtai_workspace_bytes(size_t m, size_t nw) {
   return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
 }
 
-// grid: x = M tile (192 limb rows), y = N group (352 digit rows), z = K split (steps
-// [z nsteps / S, (z + 1) nsteps / S) of 64 columns)
-__global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t m, size_t nsteps,
-                                                          const int8_t* const* __restrict__ Zs, size_t nw,
-                                                          size_t npad, int32_t* __restrict__ part) {
+// Workgroups go to the 8 XCDs round-robin (hardware block b runs on XCD b % 8).  The remap gives
+// each XCD a contiguous run of logical tiles, ordered (K split, M tile, N group), so the blocks
+// an XCD runs together share a K range.
+__device__ inline unsigned xcd_remap(unsigned b, unsigned nb) {
+  const unsigned per = (nb + 7) / 8, tall = (nb % 8) ? nb % 8 : 8;
+  const unsigned x = b % 8, l = b / 8;
+  return x < tall ? x * per + l : tall * per + (x - tall) * (per - 1) + l;
+}
+
+// 1-D grid of 36 x groups x S blocks; split z covers K steps [z nsteps / S, (z + 1) nsteps / S)
+__global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t nsteps,
+                                                          const int8_t* const* __restrict__ Zs, size_t nw, size_t m,
+                                                          size_t npad, int groups, int S,
+                                                          int32_t* __restrict__ part) {
   typedef const __attribute__((address_space(1))) v4i gv4i;
   __shared__ __align__(16) int8_t As[2][AJ_BM * AJ_LDP];
   __shared__ __align__(16) int8_t Bs[2][AJ_BN * AJ_LDP];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = wave & 3, wn = wave >> 2;
-  const size_t m0 = (size_t)blockIdx.x * AJ_BM;
-  const size_t n0 = (size_t)blockIdx.y * AJ_BN;
-  const size_t S = gridDim.z;
-  const size_t s_begin = (size_t)blockIdx.z * nsteps / S, s_end = ((size_t)blockIdx.z + 1) * nsteps / S;
+  constexpr int WMW = AJ_BM / 48;  // waves along M
+  const int wm = wave % WMW, wn = wave / WMW;
+  const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned per_split = (unsigned)AJ_MBLK * (unsigned)groups;
+  const unsigned split = lid / per_split, rem = lid % per_split;
+  const size_t m0 = (size_t)(rem / groups) * AJ_BM;
+  const size_t n0 = (size_t)(rem % groups) * AJ_BN;
+  const size_t s_begin = (size_t)split * nsteps / S, s_end = ((size_t)split + 1) * nsteps / S;
   const int kq = (lane >> 4) * 16, r16 = lane & 15;
-  // staging slots: A chunk tid + 512 c (c < 2, valid below 768), B chunk tid + 512 c (c < 3,
-  // valid below 1408 and for digit rows of real witnesses); invalid slots load a valid address
-  // so every thread issues the same loads, and store nothing (A) or zeros (B)
-  const int8_t* asrc[2];
-  int adst[2];
+  // staging: A chunk tid (valid below 384), B chunks tid + 512 c (valid below 2816 and for
+  // digit rows of real witnesses); invalid slots load a valid address so every thread issues
+  // the same loads, and store nothing (A) or zeros (B)
+  // Chunk e of a K step is row crow(e), 16-byte column (e & 3): four lanes read one row's 64
+  // contiguous bytes, and a 16-lane group writes rows {g, g+4, g+8, g+12} of its 16-row unit,
+  // whose 80-byte LDS rows start on 16 distinct 4-bank groups (conflict-free stores).
+  auto crow = [](int e) { return (e >> 6) * 16 + ((e >> 2) & 3) * 4 + ((e >> 4) & 3); };
+  constexpr size_t ASTEP = (size_t)AJ_MROWS * 64;  // bytes per K step of the limb matrix
+  const int8_t* asrc[AJ_ASL];
+  int adst[AJ_ASL];
 #pragma unroll
-  for (int c = 0; c < 2; c++) {
+  for (int c = 0; c < AJ_ASL; c++) {
     const int chunk = tid + c * AJ_THREADS, ok = chunk < AJ_ACH;
-    const int row = ok ? chunk >> 2 : 0, col = (chunk & 3) * 16;
-    asrc[c] = limbs + (m0 + row) * m + col;
-    adst[c] = ok ? row * AJ_LDP + col : -1;
+    asrc[c] = limbs + (ok ? (m0 + crow(chunk)) * 64 + (chunk & 3) * 16 : 0);
+    adst[c] = ok ? crow(chunk) * AJ_LDP + (chunk & 3) * 16 : -1;
   }
-  const int8_t* bsrc[3];
-  int bdst[3];
-  bool bok[3];
+  const int8_t* bsrc[AJ_BSL];
+  int bdst[AJ_BSL];
+  bool bok[AJ_BSL];
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
+  for (int c = 0; c < AJ_BSL; c++) {
     const int chunk = tid + c * AJ_THREADS, inb = chunk < AJ_BCH;
-    const int row = inb ? chunk >> 2 : 0, col = (chunk & 3) * 16;
+    const int row = inb ? crow(chunk) : 0, col = (chunk & 3) * 16;
     const size_t nc = n0 + row, w = nc / D, t = nc % D;
     bok[c] = inb && w < nw;
     bsrc[c] = bok[c] ? Zs[w] + t * m + col : Zs[0] + col;
     bdst[c] = inb ? row * AJ_LDP + col : -1;
   }
-  v4i ra0[2], rb0[3], ra1[2], rb1[3];
-#define AJ_GLOAD(RA, RB, st)                                                        \
-  {                                                                                 \
-    const size_t ko = (size_t)(st) * AJ_BK;                                         \
-    _Pragma("unroll") for (int c = 0; c < 2; c++) RA[c] = *(gv4i*)(asrc[c] + ko);   \
-    _Pragma("unroll") for (int c = 0; c < 3; c++) RB[c] = *(gv4i*)(bsrc[c] + ko);   \
+  v4i ra0[AJ_ASL], rb0[AJ_BSL], ra1[AJ_ASL], rb1[AJ_BSL];
+#define AJ_GLOAD(RA, RB, st)                                                                \
+  {                                                                                         \
+    _Pragma("unroll") for (int c = 0; c < AJ_ASL; c++) RA[c] = *(gv4i*)(asrc[c] + (size_t)(st) * ASTEP); \
+    _Pragma("unroll") for (int c = 0; c < AJ_BSL; c++) RB[c] = *(gv4i*)(bsrc[c] + (size_t)(st) * AJ_BK); \
   }
-#define AJ_LSTORE(RA, RB, buf)                                                      \
-  {                                                                                 \
-    _Pragma("unroll") for (int c = 0; c < 2; c++) if (adst[c] >= 0) *(v4i*)(&As[buf][adst[c]]) = RA[c]; \
-    _Pragma("unroll") for (int c = 0; c < 3; c++)                                   \
-      if (bdst[c] >= 0) *(v4i*)(&Bs[buf][bdst[c]]) = bok[c] ? RB[c] : v4i{0, 0, 0, 0}; \
+#define AJ_LSTORE(RA, RB, buf)                                                              \
+  {                                                                                         \
+    _Pragma("unroll") for (int c = 0; c < AJ_ASL; c++) if (adst[c] >= 0) *(v4i*)(&As[buf][adst[c]]) = RA[c]; \
+    _Pragma("unroll") for (int c = 0; c < AJ_BSL; c++)                                      \
+      if (bdst[c] >= 0) *(v4i*)(&Bs[buf][bdst[c]]) = bok[c] ? RB[c] : v4i{0, 0, 0, 0};      \
   }
 
   v4i acc[AJ_MT][AJ_NS];
@@ -243,7 +266,7 @@ __global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restr
 #undef AJ_LSTORE
 
   // D[row][col]: col = lane % 16, row = 4 * (lane / 16) + v
-  int32_t* base = part + ((size_t)blockIdx.z * AJ_MROWS + m0 + wm * 48) * npad + n0 + wn * 176;
+  int32_t* base = part + ((size_t)split * AJ_MROWS + m0 + wm * 48) * npad + n0 + wn * 176;
 #pragma unroll
   for (int mt = 0; mt < AJ_MT; mt++)
 #pragma unroll
@@ -326,8 +349,8 @@ void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* con
   int32_t* part = (int32_t*)workspace;
   int32_t* S = part + splits * (size_t)AJ_MROWS * npad;
   hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
-  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)AJ_MBLK, (unsigned)groups, (unsigned)splits),
-                  dim3(AJ_THREADS), 0, s, limbs, m, nsteps, Zs_dev, nw, npad, part);
+  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)(AJ_MBLK * groups * splits)), dim3(AJ_THREADS), 0, s,
+                  limbs, nsteps, Zs_dev, nw, m, npad, (int)groups, (int)splits, part);
   const int32_t* G = part;
   if (splits > 1) {
     // reduce the K splits with coalesced loads before the anti-diagonal epilogue

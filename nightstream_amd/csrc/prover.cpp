@@ -488,8 +488,26 @@ void AjtaiPhase::init(const Ccs& s, std::vector<K2> y, size_t k, const K2* alpha
   }
 }
 
+void AjtaiPhase::fold(K2 r) {
+  // fold digit bit j = chals.size() at the entries that later rounds read (multiples of 2^(j+1))
+  const size_t j = chals.size(), stride = (size_t)1 << j, step = stride << 1;
+  for (size_t q = 0; q < kt * t; q++) {
+    K2* y = &y_eval[q * D];
+    for (size_t i0 = 0; i0 < (size_t)D; i0 += step) {
+      const K2 lo = y[i0], hi = i0 + stride < (size_t)D ? y[i0 + stride] : kzero();
+      y[i0] = kadd(lo, kmul(ksub(hi, lo), r));
+    }
+  }
+  chals.push_back(r);
+}
+
+// Round j of the Ajtai-bit sum-check (optimized_engine/oracle.rs:1812-1951) as a polynomial in
+// X: with a = y[id], b = y[id + 2^j] - a for id = tt 2^(j+1) (y folded by the earlier
+// challenges), N(a + bX) = (a^3 - a) + (3a^2 b - b) X + 3ab^2 X^2 + b^3 X^3, and the eq factors
+// and the Eval term are linear in X.  The values at xs are evaluations of that polynomial.
 std::vector<K2> AjtaiPhase::evals(const std::vector<K2>& xs) const {
   const size_t j = chals.size(), free_a = ell_d - j - 1, tail = (size_t)1 << free_a, hs = (size_t)1 << (j + 1);
+  const size_t half = (size_t)1 << j;
   std::vector<K2> wbt = chi_host(beta_a.data() + j + 1, free_a), wat = chi_host(alpha.data() + j + 1, free_a);
   K2 eqb_pref = kone(), eqa_pref = kone();
   for (size_t i = 0; i < j; i++) {
@@ -504,53 +522,67 @@ std::vector<K2> AjtaiPhase::evals(const std::vector<K2>& xs) const {
   std::vector<K2> gkj(t);
   gkj[0] = kone();
   for (size_t q = 1; q < t; q++) gkj[q] = kmul(gkj[q - 1], gk);
-  std::vector<K2> yp(kt * t * D);
-  for (size_t q = 0; q < kt * t; q++) {
-    memcpy(&yp[q * D], &y_eval[q * D], D * sizeof(K2));
-    for (size_t b = 0; b < j; b++) fold_bit_inplace(&yp[q * D], b, chals[b]);
+  auto ab = [&](size_t q, size_t tt, K2& a, K2& b) {
+    const size_t id = tt * hs;
+    const K2* y = &y_eval[q * D];
+    a = y[id];
+    b = ksub(id + half < (size_t)D ? y[id + half] : kzero(), a);
+  };
+  // NC(X) = sum_i gamma^(i+1) sum_tt wbt[tt] N(a + bX), cubic
+  K2 nc[4] = {kzero(), kzero(), kzero(), kzero()};
+  K2 g = gamma;
+  for (size_t i = 0; i < kt; i++) {
+    K2 c[4] = {kzero(), kzero(), kzero(), kzero()};
+    for (size_t tt = 0; tt < tail && tt * hs < (size_t)D; tt++) {
+      K2 a, b;
+      ab(i * t + 0, tt, a, b);
+      const K2 a2 = ksqr(a), b2 = ksqr(b), w = wbt[tt];
+      const K2 wa = kmul(w, a), wb = kmul(w, b);
+      c[0] = kadd(c[0], ksub(kmul(wa, a2), wa));
+      const K2 t3 = kmul(wb, a2);
+      c[1] = kadd(c[1], ksub(kadd(kadd(t3, t3), t3), wb));
+      const K2 t5 = kmul(wa, b2);
+      c[2] = kadd(c[2], kadd(kadd(t5, t5), t5));
+      c[3] = kadd(c[3], kmul(wb, b2));
+    }
+    for (int d = 0; d < 4; d++) nc[d] = kadd(nc[d], kmul(g, c[d]));
+    g = kmul(g, gamma);
+  }
+  // eq_beta(X) = eq_beta_r eqb_pref ((1 - beta) + (2 beta - 1) X), likewise eq_ar with alpha
+  const K2 bj = beta_a[j], aj_ = alpha[j];
+  const K2 cb = kmul(eq_beta_r, eqb_pref), ca = has_inputs ? kmul(eq_r_in, eqa_pref) : kzero();
+  const K2 eb0 = kmul(cb, ksub(kone(), bj)), eb1 = kmul(cb, ksub(kadd(bj, bj), kone()));
+  const K2 ea0 = kmul(ca, ksub(kone(), aj_)), ea1 = kmul(ca, ksub(kadd(aj_, aj_), kone()));
+  K2 o[5] = {kzero(), kzero(), kzero(), kzero(), kzero()};
+  K2 fn[4] = {kadd(f_prime, nc[0]), nc[1], nc[2], nc[3]};
+  for (int d = 0; d < 4; d++) {
+    o[d] = kadd(o[d], kmul(eb0, fn[d]));
+    o[d + 1] = kadd(o[d + 1], kmul(eb1, fn[d]));
+  }
+  if (kt >= 2 && has_inputs) {
+    // inner(X) = sum_jm sum_{i >= 1} gamma^i gk^jm sum_tt wat[tt] (a + bX), linear
+    K2 in0 = kzero(), in1 = kzero();
+    for (size_t jm = 0; jm < t; jm++)
+      for (size_t i = 1; i < kt; i++) {
+        K2 s0 = kzero(), s1 = kzero();
+        for (size_t tt = 0; tt < tail && tt * hs < (size_t)D; tt++) {
+          K2 a, b;
+          ab(i * t + jm, tt, a, b);
+          s0 = kadd(s0, kmul(wat[tt], a));
+          s1 = kadd(s1, kmul(wat[tt], b));
+        }
+        const K2 wgt = kmul(gpi[i], gkj[jm]);
+        in0 = kadd(in0, kmul(wgt, s0));
+        in1 = kadd(in1, kmul(wgt, s1));
+      }
+    in0 = kmul(gk, in0);
+    in1 = kmul(gk, in1);
+    o[0] = kadd(o[0], kmul(ea0, in0));
+    o[1] = kadd(o[1], kadd(kmul(ea0, in1), kmul(ea1, in0)));
+    o[2] = kadd(o[2], kmul(ea1, in1));
   }
   std::vector<K2> ys(xs.size());
-  K2 tmp[D];
-  for (size_t xi = 0; xi < xs.size(); xi++) {
-    K2 x = xs[xi];
-    K2 eq_beta = kmul(eq_beta_r, kmul(eqb_pref, eq_lin(x, beta_a[j])));
-    K2 eq_ar = has_inputs ? kmul(eq_r_in, kmul(eqa_pref, eq_lin(x, alpha[j]))) : kzero();
-    K2 nc_sum = kzero(), g = gamma;
-    for (size_t i = 0; i < kt; i++) {
-      memcpy(tmp, &yp[(i * t + 0) * D], sizeof(tmp));
-      fold_bit_inplace(tmp, j, x);
-      K2 a = kzero();
-      for (size_t tt = 0; tt < tail; tt++) {
-        size_t id = tt * hs;
-        if (id < (size_t)D) {
-          K2 y = tmp[id];
-          a = kadd(a, kmul(wbt[tt], kmul(y, ksub(ksqr(y), kone()))));
-        }
-      }
-      nc_sum = kadd(nc_sum, kmul(g, a));
-      g = kmul(g, gamma);
-    }
-    K2 o = kmul(eq_beta, kadd(f_prime, nc_sum));
-    if (kt >= 2 && !kiszero(eq_ar)) {
-      K2 inner = kzero();
-      for (size_t jm = 0; jm < t; jm++) {
-        K2 sj = kzero();
-        for (size_t i = 1; i < kt; i++) {
-          memcpy(tmp, &yp[(i * t + jm) * D], sizeof(tmp));
-          fold_bit_inplace(tmp, j, x);
-          K2 yd = kzero();
-          for (size_t tt = 0; tt < tail; tt++) {
-            size_t id = tt * hs;
-            if (id < (size_t)D) yd = kadd(yd, kmul(wat[tt], tmp[id]));
-          }
-          sj = kadd(sj, kmul(kmul(gpi[i], gkj[jm]), yd));
-        }
-        inner = kadd(inner, sj);
-      }
-      o = kadd(o, kmul(eq_ar, kmul(gk, inner)));
-    }
-    ys[xi] = o;
-  }
+  for (size_t xi = 0; xi < xs.size(); xi++) ys[xi] = poly_eval(o, 5, xs[xi]);
   return ys;
 }
 

@@ -153,6 +153,9 @@ class Prover {
 
 // Ajtai-bit rounds of Pi_CCS on the host (optimized_engine/oracle.rs:1812-1951), from the
 // precompute_for_r products y_eval[w][j][rho] at the folded row point.
+// y_eval is folded by each round's challenge as it arrives (only the entries at multiples of
+// 2^round stay meaningful), and a round's values come from the polynomial in X built once per
+// round, so each round costs O(kt * t * 54 / 2^round) K products.
 struct AjtaiPhase {
   size_t kt = 0, t = 0, ell_d = 6;
   std::vector<K2> y_eval, alpha, beta_a, chals;
@@ -161,7 +164,7 @@ struct AjtaiPhase {
   void init(const Ccs& s, std::vector<K2> y, size_t k, const K2* alpha, const K2* beta_a, const K2* beta_r, K2 gamma,
             const std::vector<K2>& r_time, const K2* r_inputs);
   std::vector<K2> evals(const std::vector<K2>& xs) const;
-  void fold(K2 r) { chals.push_back(r); }
+  void fold(K2 r);
 };
 
 // Device-resident Pi_CCS round oracle (RoundOracle, neo-reductions/src/sumcheck.rs:17-34).
