@@ -130,6 +130,12 @@ int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, siz
 int ns_prover_phase_stats(const ns_prover* p, double* seconds, size_t n);
 /* Poseidon2 permutations of the host transcript per phase of the last prove (same 12 phases). */
 int ns_prover_phase_perms(const ns_prover* p, uint64_t* perms, size_t n);
+
+/* ---- host Poseidon2 (neo-ccs/src/crypto/poseidon2_goldilocks.rs:40-53 permutation()) ----
+ * Permutes state[width] in place: width 8 is the transcript permutation
+ * (neo-transcript/src/poseidon2.rs), width 16 the CCS-digest one.  force_scalar = 1 runs the
+ * portable path even where the AVX-512 one is selected (both give the same output).  Host only. */
+int ns_poseidon2_permute(uint64_t* state, size_t width, int force_scalar);
 void ns_prover_free(ns_prover* p);
 
 /* ---- Twist/Shout round oracles (neo-memory/src/twist_oracle.rs).  Each handle implements the

@@ -26,6 +26,7 @@ EXPORTS = [
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
     "ns_dec_split", "ns_prover_new", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
+    "ns_poseidon2_permute",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
     "ns_ts_twist_read_new", "ns_ts_twist_write_new", "ns_ts_val_eval_new", "ns_ts_total_inc_new",
     "ns_ts_read_addr_new", "ns_ts_write_addr_new", "ns_ts_addr_lookup_new", "ns_ts_evals", "ns_ts_fold",
@@ -93,6 +94,7 @@ def lib():
     L.ns_prover_kernel_stats.argtypes = [vp, vp, vp, sz]
     L.ns_prover_phase_stats.argtypes = [vp, vp, sz]
     L.ns_prover_phase_perms.argtypes = [vp, vp, sz]
+    L.ns_poseidon2_permute.argtypes = [vp, sz, C.c_int]
     L.ns_params_goldilocks_auto_r1cs_ccs.argtypes = [sz, vp]
     _lib = L
     return L
@@ -151,6 +153,14 @@ def device_count() -> int:
 
 def device_synchronize():
     _check(lib().ns_device_synchronize())
+
+
+def poseidon2_permute(state, force_scalar: bool = False) -> np.ndarray:
+    """The host Poseidon2 permutation (width 8: transcript, width 16: CCS digest) of a state of
+    canonical Goldilocks elements; returns the permuted copy."""
+    st = np.ascontiguousarray(np.asarray(state, dtype=np.uint64)).copy()
+    _check(lib().ns_poseidon2_permute(_ptr(st), st.size, 1 if force_scalar else 0))
+    return st
 
 
 def params_goldilocks_auto_r1cs_ccs(n_rows: int) -> dict:

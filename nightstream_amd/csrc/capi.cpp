@@ -406,6 +406,17 @@ int ns_prover_phase_perms(const ns_prover* p, uint64_t* perms, size_t n) {
   });
 }
 
+int ns_poseidon2_permute(uint64_t* state, size_t width, int force_scalar) {
+  return guard([&] {
+    if (width != 8 && width != 16) throw Error(1, "Poseidon2 width must be 8 or 16");
+    for (size_t i = 0; i < width; i++)
+      if (state[i] >= Q) throw Error(1, "state element is not a canonical field element");
+    const Poseidon2& P = width == 8 ? transcript_perm() : digest_perm();
+    if (force_scalar) P.permute_scalar(state);
+    else P.permute(state);
+  });
+}
+
 void ns_prover_free(ns_prover* p) { delete p; }
 
 }  // extern "C"

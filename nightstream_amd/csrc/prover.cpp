@@ -416,45 +416,7 @@ static K2 poly_eval(const K2* c, size_t n, K2 x) {
   return r;
 }
 
-// Lagrange interpolation to low->high coefficients (sumcheck.rs:79-121)
-static void interpolate(const K2* xs, const K2* ys, size_t n, K2* coeffs) {
-  for (size_t i = 0; i < n; i++) coeffs[i] = kzero();
-  std::vector<K2> numer(n), next(n);
-  for (size_t i = 0; i < n; i++) {
-    std::fill(numer.begin(), numer.end(), kzero());
-    numer[0] = kone();
-    size_t cur = 0;
-    for (size_t j = 0; j < n; j++) {
-      if (i == j) continue;
-      std::fill(next.begin(), next.end(), kzero());
-      for (size_t d = 0; d <= cur; d++) {
-        next[d + 1] = kadd(next[d + 1], numer[d]);
-        next[d] = kadd(next[d], kmul(kneg(xs[j]), numer[d]));
-      }
-      numer = next;
-      cur++;
-    }
-    K2 denom = kone();
-    for (size_t j = 0; j < n; j++)
-      if (i != j) denom = kmul(denom, ksub(xs[i], xs[j]));
-    K2 scale = kmul(ys[i], kinv(denom));
-    for (size_t d = 0; d <= cur; d++) coeffs[d] = kadd(coeffs[d], kmul(scale, numer[d]));
-  }
-}
-
 // ---------------- Ajtai-bit rounds ----------------
-static void fold_bit_inplace(K2* dg, size_t bit, K2 a) {
-  size_t stride = (size_t)1 << bit, step = stride << 1;
-  for (size_t base = 0; base < (size_t)D; base += step)
-    for (size_t off = 0; off < stride; off++) {
-      size_t i0 = base + off;
-      if (i0 >= (size_t)D) break;
-      size_t i1 = i0 + stride;
-      K2 lo = dg[i0], hi = i1 < (size_t)D ? dg[i1] : kzero();
-      dg[i0] = kadd(lo, kmul(ksub(hi, lo), a));
-    }
-}
-
 void AjtaiPhase::init(const Ccs& s, std::vector<K2> y, size_t k, const K2* alpha_, const K2* beta_a_,
                       const K2* beta_r, K2 gamma_, const std::vector<K2>& r_time, const K2* r_inputs) {
   kt = k;
@@ -488,6 +450,51 @@ void AjtaiPhase::init(const Ccs& s, std::vector<K2> y, size_t k, const K2* alpha
   }
 }
 
+// out = sum_i a_i(X) v_i(X) mod Phi_81 (X^54 = -X^27 - 1) for a_i with 54 coefficients in
+// [-2, 2] (a[i*54 + u]) and v_i in F_q^54.  The 32-bit halves of v are accumulated in int64
+// (|sum| < 2^33 * 16 * 54 < 2^43, no carries, vectorizable adds) and reduced once per
+// coefficient of the product.
+static void mix_small_coeffs(const int8_t* a, const uint64_t* const* v, size_t kt, uint64_t* out) {
+  int64_t lo[2 * D - 1] = {0}, hi[2 * D - 1] = {0};
+  for (size_t i = 0; i < kt; i++) {
+    uint32_t vl[D], vh[D];
+    for (int x = 0; x < D; x++) {
+      vl[x] = (uint32_t)v[i][x];
+      vh[x] = (uint32_t)(v[i][x] >> 32);
+    }
+    for (int u = 0; u < D; u++) {
+      const int au = a[i * D + u];
+      int64_t* L = lo + u;
+      int64_t* H = hi + u;
+      switch (au) {
+        case 1:
+          for (int x = 0; x < D; x++) L[x] += vl[x], H[x] += vh[x];
+          break;
+        case -1:
+          for (int x = 0; x < D; x++) L[x] -= vl[x], H[x] -= vh[x];
+          break;
+        case 2:
+          for (int x = 0; x < D; x++) L[x] += 2 * (int64_t)vl[x], H[x] += 2 * (int64_t)vh[x];
+          break;
+        case -2:
+          for (int x = 0; x < D; x++) L[x] -= 2 * (int64_t)vl[x], H[x] -= 2 * (int64_t)vh[x];
+          break;
+        default:
+          break;
+      }
+    }
+  }
+  uint64_t P[2 * D - 1];
+  for (int q = 0; q < 2 * D - 1; q++) P[q] = fadd(from_i64(lo[q]), fmul(from_i64(hi[q]), 1ull << 32));
+  for (int q = 0; q < D; q++) {
+    uint64_t c = P[q];
+    if (q <= 25) c = fadd(c, P[q + 81]);
+    if (q >= 27) c = fsub(c, P[q + 27]);
+    if (q <= 26) c = fsub(c, P[q + 54]);
+    out[q] = c;
+  }
+}
+
 void AjtaiPhase::fold(K2 r) {
   // fold digit bit j = chals.size() at the entries that later rounds read (multiples of 2^(j+1))
   const size_t j = chals.size(), stride = (size_t)1 << j, step = stride << 1;
@@ -506,6 +513,13 @@ void AjtaiPhase::fold(K2 r) {
 // challenges), N(a + bX) = (a^3 - a) + (3a^2 b - b) X + 3ab^2 X^2 + b^3 X^3, and the eq factors
 // and the Eval term are linear in X.  The values at xs are evaluations of that polynomial.
 std::vector<K2> AjtaiPhase::evals(const std::vector<K2>& xs) const {
+  const std::vector<K2> o = coeffs();
+  std::vector<K2> ys(xs.size());
+  for (size_t xi = 0; xi < xs.size(); xi++) ys[xi] = poly_eval(o.data(), o.size(), xs[xi]);
+  return ys;
+}
+
+std::vector<K2> AjtaiPhase::coeffs() const {
   const size_t j = chals.size(), free_a = ell_d - j - 1, tail = (size_t)1 << free_a, hs = (size_t)1 << (j + 1);
   const size_t half = (size_t)1 << j;
   std::vector<K2> wbt = chi_host(beta_a.data() + j + 1, free_a), wat = chi_host(alpha.data() + j + 1, free_a);
@@ -581,9 +595,7 @@ std::vector<K2> AjtaiPhase::evals(const std::vector<K2>& xs) const {
     o[1] = kadd(o[1], kadd(kmul(ea0, in1), kmul(ea1, in0)));
     o[2] = kadd(o[2], kmul(ea1, in1));
   }
-  std::vector<K2> ys(xs.size());
-  for (size_t xi = 0; xi < xs.size(); xi++) ys[xi] = poly_eval(o, 5, xs[xi]);
-  return ys;
+  return std::vector<K2>(o, o + 5);
 }
 
 // ---------------- CCS oracle on the device ----------------
@@ -1048,12 +1060,13 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     for (size_t rd = 0; rd < ell_n; rd++) {
       tr.append_msg_u64("batched/round_idx", rd);
       std::vector<K2> c = oracle.round_coeffs();
-      std::vector<K2> ys(xs_n);
-      for (size_t i = 0; i < xs_n; i++) ys[i] = poly_eval(c.data(), c.size(), xs[i]);
-      if (!keq(kadd(ys[0], ys[1]), running))
-        throw Error(4, "step " + std::to_string(idx) + ": batched round " + std::to_string(rd) + " invariant failed");
+      if (c.size() > xs_n) throw Error(7, "round polynomial exceeds the degree bound");
+      // the reference interpolates the values at 0..d_sc; the polynomial of degree < d_sc + 1
+      // through them is c itself, padded with zeros
       K2* co = &time_coeffs[rd * xs_n];
-      interpolate(xs.data(), ys.data(), xs_n, co);
+      for (size_t i = 0; i < xs_n; i++) co[i] = i < c.size() ? c[i] : kzero();
+      if (!keq(kadd(poly_eval(co, xs_n, xs[0]), poly_eval(co, xs_n, xs[1])), running))
+        throw Error(4, "step " + std::to_string(idx) + ": batched round " + std::to_string(rd) + " invariant failed");
       tr.append_message("batched/claim_label", "ccs/time");
       tr.append_msg_u64("batched/claim_idx", 0);
       for (size_t i = 0; i < xs_n; i++) tr.append_k("batched/round/coeff", co[i]);
@@ -1077,11 +1090,11 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     tr.append_k("sc/claimed_sum", running);
     std::vector<K2> aj_coeffs(ell_d * xs_n);
     for (size_t rd = 0; rd < ell_d; rd++) {
-      std::vector<K2> ys = aj.evals(xs);
-      if (!keq(kadd(ys[0], ys[1]), running))
-        throw Error(4, "step " + std::to_string(idx) + ": ajtai round " + std::to_string(rd) + " invariant failed");
+      const std::vector<K2> c = aj.coeffs();
       K2* co = &aj_coeffs[rd * xs_n];
-      interpolate(xs.data(), ys.data(), xs_n, co);
+      for (size_t i = 0; i < xs_n; i++) co[i] = i < c.size() ? c[i] : kzero();
+      if (!keq(kadd(poly_eval(co, xs_n, xs[0]), poly_eval(co, xs_n, xs[1])), running))
+        throw Error(4, "step " + std::to_string(idx) + ": ajtai round " + std::to_string(rd) + " invariant failed");
       for (size_t i = 0; i < xs_n; i++) tr.append_k("sumcheck/round/coeff", co[i]);
       K2 r = tr.challenge_k("sumcheck/challenge/0", "sumcheck/challenge/1");
       running = poly_eval(co, xs_n, r);
@@ -1228,57 +1241,24 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
               parent.X[rr * m_in + c] = fadd(parent.X[rr * m_in + c], fmul(co, ccs_out[i].X[kk * m_in + c]));
           }
       }
-      // y_j = sum_i rot(a_i) y_ij: rot(a) v is a(X) v(X) mod Phi_81, so accumulate the products of
-      // the small coefficients a_i in [-2, 2] with y_ij as signed additions and reduce once.
+      // y_j = sum_i rot(a_i) y_ij and c = sum_i rot(a_i) c_i: rot(a) v is a(X) v(X) mod Phi_81
+      // with the small coefficients a_i in [-2, 2] (mix_small_coeffs)
+      std::vector<const uint64_t*> vp(kt);
+      std::vector<uint64_t> vbuf(kt * D * 2), outc(D);
       for (size_t j = 0; j < t; j++) {
-        K2 tmp[2 * D - 1];
-        for (auto& x : tmp) x = kzero();
-        for (size_t i = 0; i < kt; i++) {
-          const int8_t* a = &rho_a[i * D];
-          const K2* y = &ccs_out[i].y[j * DPAD];
-          K2 y2[D];
-          for (int v = 0; v < D; v++) y2[v] = kadd(y[v], y[v]);
-          for (int u = 0; u < D; u++) {
-            const int au = a[u];
-            if (!au) continue;
-            const K2* src = (au == 1 || au == -1) ? y : y2;
-            if (au > 0)
-              for (int v = 0; v < D; v++) tmp[u + v] = kadd(tmp[u + v], src[v]);
-            else
-              for (int v = 0; v < D; v++) tmp[u + v] = ksub(tmp[u + v], src[v]);
+        for (int comp = 0; comp < 2; comp++) {
+          for (size_t i = 0; i < kt; i++) {
+            const K2* y = &ccs_out[i].y[j * DPAD];
+            for (int v = 0; v < D; v++) vbuf[i * D + v] = comp ? y[v].c1 : y[v].c0;
+            vp[i] = &vbuf[i * D];
           }
-        }
-        for (int q = 0; q < D; q++) {
-          K2 c = tmp[q];
-          if (q <= 25) c = kadd(c, tmp[q + 81]);
-          if (q >= 27) c = ksub(c, tmp[q + 27]);
-          if (q <= 26) c = ksub(c, tmp[q + 54]);
-          parent.y[j * DPAD + q] = c;
+          mix_small_coeffs(rho_a.data(), vp.data(), kt, outc.data());
+          for (int q = 0; q < D; q++) (comp ? parent.y[j * DPAD + q].c1 : parent.y[j * DPAD + q].c0) = outc[q];
         }
       }
-      // commitment mix: sum_i rot(rho_i) * c_i (s_lincomb with the rotation's first column)
-      memset(parent.c, 0, sizeof(parent.c));
-      for (size_t i = 0; i < kt; i++) {
-        const int8_t* a = &rho_a[i * D];
-        for (int col = 0; col < KAPPA; col++) {
-          const uint64_t* cc = &ccs_out[i].c[col * D];
-          uint64_t tmp[2 * D - 1];
-          for (auto& x : tmp) x = 0;
-          for (int u = 0; u < D; u++) {
-            if (!a[u]) continue;
-            for (int v = 0; v < D; v++) {
-              uint64_t p = a[u] > 0 ? cc[v] : fneg(cc[v]);
-              for (int rep = 0; rep < (a[u] > 0 ? a[u] : -a[u]); rep++) tmp[u + v] = fadd(tmp[u + v], p);
-            }
-          }
-          for (int q = 0; q < D; q++) {
-            uint64_t c = tmp[q];
-            if (q <= 25) c = fadd(c, tmp[q + 81]);
-            if (q >= 27) c = fsub(c, tmp[q + 27]);
-            if (q <= 26) c = fsub(c, tmp[q + 54]);
-            parent.c[col * D + q] = fadd(parent.c[col * D + q], c);
-          }
-        }
+      for (int col = 0; col < KAPPA; col++) {
+        for (size_t i = 0; i < kt; i++) vp[i] = &ccs_out[i].c[col * D];
+        mix_small_coeffs(rho_a.data(), vp.data(), kt, &parent.c[col * D]);
       }
     }
     for (size_t j = 0; j < t; j++) {

@@ -164,6 +164,7 @@ struct AjtaiPhase {
   void init(const Ccs& s, std::vector<K2> y, size_t k, const K2* alpha, const K2* beta_a, const K2* beta_r, K2 gamma,
             const std::vector<K2>& r_time, const K2* r_inputs);
   std::vector<K2> evals(const std::vector<K2>& xs) const;
+  std::vector<K2> coeffs() const;  // the round polynomial, degree <= 4, low to high
   void fold(K2 r);
 };
 

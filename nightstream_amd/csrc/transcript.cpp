@@ -115,13 +115,13 @@ void Poseidon2::init(int w, ChaCha8& rng) {
     for (int i = 0; i < w; i++) ext_term[r][i] = sample_fq(rng);
   for (int r = 0; r < 22; r++) internal_rc[r] = sample_fq(rng);
   for (int i = 0; i < w; i++) diag[i] = sample_fq(rng);  // stand-in diagonal, see transcript.h
+  diag0p1 = diag[0] + 1;                                    // diag[0] < q, so no wrap
   if (w == 8) {
     for (int r = 0; r < 4; r++)
       for (int i = 0; i < 8; i++) {
         vec_rc[r][i] = ext_init[r][i];
         vec_rc[4 + r][i] = ext_term[r][i];
       }
-    diag0p1 = diag[0] + 1;  // diag[0] < q, so no wrap
     use_avx512 = false;
     if (__builtin_cpu_supports("avx512f")) {
       // enable the vector path only if it reproduces the scalar permutation on a set of states
@@ -190,14 +190,25 @@ static inline void permute_w(const Poseidon2& P, uint64_t* st) {
     for (int i = 0; i < W; i++) st[i] = sbox7(red((u128)st[i] + P.ext_init[r][i]));
     ext_layer<W>(st);
   }
+  // Internal rounds: s0' = x7 d0 + sum = x7 (d0 + 1) + rest with rest = sum_{i>0} s_i, so lane 0
+  // does not wait for sum; x7 (d0 + 1) = (x^2 * x (d0 + 1)) * x^4 keeps it three multiplies deep.
+  uint64_t x = red((u128)st[0] + P.internal_rc[0]);
   for (int r = 0; r < 22; r++) {
     u128 rest = 0;
     for (int i = 1; i < W; i++) rest += st[i];
-    uint64_t x7 = sbox7(red((u128)st[0] + P.internal_rc[r]));
-    uint64_t sum = red(rest + x7);
-    st[0] = red((u128)x7 * P.diag[0] + sum);
+    const uint64_t xc = red((u128)x * P.diag0p1);
+    const uint64_t x2 = red((u128)x * x);
+    const uint64_t x3 = red((u128)x2 * x);
+    const uint64_t x3c = red((u128)x2 * xc);
+    const uint64_t x4 = red((u128)x2 * x2);
+    const uint64_t x7 = red((u128)x3 * x4);
+    u128 nx = (u128)red((u128)x3c * x4) + rest;
+    if (r < 21) nx += P.internal_rc[r + 1];
+    const uint64_t sum = red(rest + x7);
     for (int i = 1; i < W; i++) st[i] = red((u128)st[i] * P.diag[i] + sum);
+    x = red(nx);
   }
+  st[0] = x;
   for (int r = 0; r < 4; r++) {
     for (int i = 0; i < W; i++) st[i] = sbox7(red((u128)st[i] + P.ext_term[r][i]));
     ext_layer<W>(st);

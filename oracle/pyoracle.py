@@ -49,6 +49,8 @@ def lib():
         L.or_chacha_ietf_block.argtypes = [u32p, C.c_uint32, u32p, C.c_int, u32p]
         L.or_chacha_block.argtypes = [u32p, C.c_uint64, C.c_uint64, C.c_int, u32p]
         L.or_p2_permute8.argtypes = [u64p]
+        L.or_p2_digest_perm.restype = C.c_void_p
+        L.or_p2_permute.argtypes = [C.c_void_p, u64p]
         L.or_decomp_b2_row_major.argtypes = [u64p, C.c_size_t, i8p]
         L.or_ajtai_setup.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, u64p]
         L.or_ajtai_commit.argtypes = [u64p, C.c_size_t, C.c_size_t, i8p, u64p]
@@ -130,6 +132,16 @@ class Ccs:
 
 def num_threads() -> int:
     return lib().or_num_threads()
+
+
+def p2_permute(state) -> np.ndarray:
+    """Poseidon2 of the oracle: width 8 (transcript) or 16 (CCS digest); returns a copy."""
+    st = np.ascontiguousarray(np.asarray(state, dtype=np.uint64)).copy()
+    if st.size == 8:
+        lib().or_p2_permute8(st)
+    else:
+        lib().or_p2_permute(lib().or_p2_digest_perm(), st)
+    return st
 
 
 def decomp(z: np.ndarray) -> np.ndarray:
