@@ -28,7 +28,8 @@ sys.path.insert(0, ROOT)
 PP_SEED = bytes([0x4E, 0x53] + [0x07] * 30)
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0
-KT = ["round0_eval", "round_eval", "table_fold", "ajtai_gemm", "ydigits_gemm", "rlc_zmix", "dec_split", "round1_eval"]
+KT = ["round0_eval", "round_eval", "table_fold", "ajtai_gemm", "ydigits_gemm", "rlc_zmix", "dec_split", "round1_eval",
+      "round_fused"]
 KERNEL_NAMES = {
     "round0_eval": "k_round_eval<0>",
     "round1_eval": "k_round_eval<1>",
@@ -38,6 +39,7 @@ KERNEL_NAMES = {
     "ydigits_gemm": "k_ydigits_gemm<4>",
     "rlc_zmix": "k_rowcontract<4>",
     "dec_split": "k_dec_split",
+    "round_fused": "k_round_fused",
 }
 D = 54
 
@@ -48,7 +50,9 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
     Pi_CCS tables per row at round r (length L = n >> r): k*54 NC entries + nsc scalar tables
     (eq_beta, 3 f-var tables, plus eq_rin and Eval once k > 1), 16 B per K entry.  Rounds 0 and 1
     read the int8 digit matrices (1 B per digit, all n digits of a row each) instead of NC K
-    entries; the fold after round 1 writes the NC tables of length n/4 from the digits.
+    entries; the fold after round 1 writes the NC tables of length n/4 from the digits.  Round 2
+    reads its tables; each round r >= 3 is one fused launch that reads the round r-1 tables
+    (2L entries) and writes the folded ones (L entries).
     """
     ell_n = n.bit_length() - 1
     out = {c: 0.0 for c in KT}
@@ -57,16 +61,17 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
         nsc = 4 if s == 0 else 6
         out["round0_eval"] += k * D * n + nsc * 16 * n
         out["round1_eval"] += k * D * n + nsc * 16 * (n // 2)
-        for r in range(2, ell_n):
+        out["round_eval"] += (k * D + nsc) * 16 * (n >> 2)
+        for r in range(3, ell_n):
             L = n >> r
-            out["round_eval"] += (k * D + nsc) * 16 * L
-        for r in range(ell_n):
+            out["round_fused"] += (k * D + nsc) * 16 * (2 * L + L)
+        # separate folds: scalar tables after rounds 0 and 1, the NC tables from the digits after
+        # round 1, and the last round's fold (flushed before y)
+        for r in (0, 1):
             L = n >> r
             out["table_fold"] += nsc * 16 * (L + L // 2)
-            if r == 1:
-                out["table_fold"] += k * D * n + k * D * 16 * (n // 4)
-            elif r >= 2:
-                out["table_fold"] += k * D * 16 * (L + L // 2)
+        out["table_fold"] += k * D * n + k * D * 16 * (n // 4)
+        out["table_fold"] += (k * D + nsc) * 16 * (2 + 1)
         # y products: k witnesses at the row point + 12 DEC children, t = 4 matrices
         for nw in (k, 12):
             out["ydigits_gemm"] += 2.0 * (nw * D) * 64 * n

@@ -120,8 +120,9 @@ int ns_prove_shard(ns_prover* p, const uint64_t* const* z_steps, size_t n_steps,
 int ns_prove_shard_device(ns_prover* p, const uint64_t* const* z_steps_dev, size_t n_steps, size_t m_in,
                           uint8_t** proof, size_t* proof_len, double* seconds);
 /* Per-kernel timing of the next proves (HIP events on the prover's stream); classes:
- * 0 round-0 eval, 1 round eval (rounds >= 2), 2 table fold, 3 Ajtai GEMM, 4 digit x K GEMM, 5 RLC Z_mix,
- * 6 DEC split, 7 round-1 eval.  ms[i] is the summed duration and counts[i] the number of launches. */
+ * 0 round-0 eval, 1 round-2 eval, 2 table fold, 3 Ajtai GEMM, 4 digit x K GEMM, 5 RLC Z_mix,
+ * 6 DEC split, 7 round-1 eval, 8 fused fold + round (rounds >= 3).  ms[i] is the summed
+ * duration and counts[i] the number of launches. */
 int ns_prover_set_timing(ns_prover* p, int on);
 int ns_prover_kernel_stats(const ns_prover* p, double* ms, uint64_t* counts, size_t n);
 /* Host wall-clock seconds per phase of the last prove (12 phases: bind inputs, oracle build,

@@ -368,9 +368,9 @@ class FoldingSession:
         _check(lib().ns_prover_set_timing(self.h, int(on)))
 
     def kernel_stats(self):
-        ms = np.zeros(8, np.float64)
-        cnt = np.zeros(8, np.uint64)
-        _check(lib().ns_prover_kernel_stats(self.h, _ptr(ms), _ptr(cnt), 8))
+        ms = np.zeros(9, np.float64)
+        cnt = np.zeros(9, np.uint64)
+        _check(lib().ns_prover_kernel_stats(self.h, _ptr(ms), _ptr(cnt), 9))
         return ms.tolist(), [int(c) for c in cnt]
 
     PHASES = ["bind_inputs", "oracle_build", "row_rounds", "y_products", "ajtai_rounds", "me_outputs",

@@ -787,6 +787,159 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
 }
 
+// ---------------- fused fold + round on the K tables (rounds >= 3) ----------------
+// Work item (pair t, row group g) folds its rows' entries 4t..4t+3 of the previous tables by the
+// previous challenge r into entries 2t, 2t+1 of the new tables (written for the next round) and
+// evaluates the round polynomial on them; the f group also folds the scalar tables.  Each block
+// leaves its 5 partial coefficients for k_sum_partials (the kernel boundary orders them; a
+// device-scope fence per block would write back L2).  Same values as launch_fold_k followed by
+// launch_round_eval.
+__device__ __forceinline__ K2 fold2(K2 lo, K2 hi, K2 r, uint64_t r7) { return kadd(lo, kmul_w7(r, r7, ksub(hi, lo))); }
+
+__global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
+                                                     int G) {
+  // row weights come from global memory (11 KB, cache-resident): a short round's block needs a
+  // few of them, so staging all k*54 in LDS would cost more than it saves
+  __shared__ K2 red[4 * 5];
+  const int nrow = A.k * D;
+  const K2 r = A.r;
+  const uint64_t r7 = A.r7;
+  const size_t pairs = A.len / 2, L2 = 2 * A.len;
+  const int R = (nrow + G - 1) / G;
+  K2 c[5];
+#pragma unroll
+  for (int d = 0; d < 5; d++) c[d] = kzero();
+  K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
+  const size_t total = pairs * ((size_t)G + 1);
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = idx % pairs;
+    const int g = (int)(idx / pairs);
+    const K2* eq = A.sc_src + 4 * t;
+    const K2 elo = fold2(eq[0], eq[1], r, r7), ehi = fold2(eq[2], eq[3], r, r7);
+    const K2 e0 = elo, e1 = ksub(ehi, elo);
+    K2 in[5];
+#pragma unroll
+    for (int d = 0; d < 5; d++) in[d] = kzero();
+    if (g == G) {
+      // fold the scalar tables at 2t, 2t+1, then evaluate f(Mz)(X) and the Eval term on the
+      // folded values (recomputed where needed: no runtime-indexed register arrays)
+      auto folded = [&](int q, K2& lo, K2& hi) {
+        const K2* src = A.sc_src + (size_t)q * L2 + 4 * t;
+        lo = fold2(src[0], src[1], r, r7);
+        hi = fold2(src[2], src[3], r, r7);
+      };
+      for (int q = 0; q < A.nsc; q++) {
+        K2 lo, hi;
+        if (q == 0) {
+          lo = elo;
+          hi = ehi;
+        } else {
+          folded(q, lo, hi);
+        }
+        K2* dst = A.sc_dst + (size_t)q * A.len + 2 * t;
+        dst[0] = lo;
+        dst[1] = hi;
+      }
+      for (int tt = 0; tt < A.f.nterms; tt++) {
+        K2 p[5];
+        p[0] = K2{A.f.coeff[tt], 0};
+#pragma unroll
+        for (int d = 1; d < 5; d++) p[d] = kzero();
+        for (int v = 0; v < A.f.nv[tt]; v++) {
+          K2 lo, hi;
+          folded(1 + A.f.var[tt][v], lo, hi);
+          const K2 a = lo, b = ksub(hi, lo);
+          for (int e = 0; e < A.f.exp[tt][v]; e++) {
+#pragma unroll
+            for (int d = 4; d >= 1; d--) p[d] = kadd(kmul(a, p[d]), kmul(b, p[d - 1]));
+            p[0] = kmul(a, p[0]);
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
+      }
+      if (A.has_eval) {
+        K2 rl, rh, vl, vh;
+        folded(1 + A.nf, rl, rh);
+        folded(2 + A.nf, vl, vh);
+        const K2 r0_ = rl, r1_ = ksub(rh, rl);
+        const K2 v0 = vl, v1 = ksub(vh, vl);
+        ev0 = kadd(ev0, kmul(r0_, v0));
+        ev1 = kadd(ev1, kadd(kmul(r0_, v1), kmul(r1_, v0)));
+        ev2 = kadd(ev2, kmul(r1_, v1));
+      }
+    } else {
+      const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
+      KS2 z1 = ks2_zero(), z3 = ks2_zero(), z5 = ks2_zero(), z6 = ks2_zero();
+      KS1 z2 = ks1_zero(), z4 = ks1_zero();
+      for (int row = r_lo; row < r_hi; row++) {
+        const K2* src = A.nc_src + (size_t)row * L2 + 4 * t;
+        const K2 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
+        const K2 a = fold2(u0, u1, r, r7), hv = fold2(u2, u3, r, r7);
+        K2* dst = A.nc_dst + (size_t)row * A.len + 2 * t;
+        dst[0] = a;
+        dst[1] = hv;
+        const K2 b = ksub(hv, a);
+        const K2 wv = wg[row];
+        const uint64_t w7 = fmul7(wv.c1);
+        const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
+        const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
+        ks2_mac(z1, wa, a2);
+        ks1_add(z2, wa);
+        ks2_mac(z3, wb, a2);
+        ks1_add(z4, wb);
+        ks2_mac(z5, wa, b2);
+        ks2_mac(z6, wb, b2);
+      }
+      const K2 s1 = ks2_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2_reduce(z3), s4 = ks1_reduce(z4),
+               s5 = ks2_reduce(z5), s6 = ks2_reduce(z6);
+      in[0] = ksub(s1, s2);
+      in[1] = ksub(kadd(kadd(s3, s3), s3), s4);
+      in[2] = kadd(kadd(s5, s5), s5);
+      in[3] = s6;
+    }
+    const uint64_t e07 = fmul7(e0.c1), e17 = fmul7(e1.c1);
+    c[0] = kadd(c[0], kmul_w7(e0, e07, in[0]));
+#pragma unroll
+    for (int d = 1; d < 4; d++) c[d] = kadd(c[d], kadd(kmul_w7(e0, e07, in[d]), kmul_w7(e1, e17, in[d - 1])));
+    c[4] = kadd(c[4], kmul_w7(e1, e17, in[3]));
+    if (g == G) c[4] = kadd(c[4], kmul_w7(e0, e07, in[4]));
+  }
+  if (A.has_eval) {
+    c[0] = kadd(c[0], kmul(A.gamma_k, ev0));
+    c[1] = kadd(c[1], kmul(A.gamma_k, ev1));
+    c[2] = kadd(c[2], kmul(A.gamma_k, ev2));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < 5; d++) {
+    K2 v = wave_reduce(c[d]);
+    if (lane == 0) red[wave * 5 + d] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    K2 v = kzero();
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = kadd(v, red[w * 5 + threadIdx.x]);
+    partials[blockIdx.x * 5 + threadIdx.x] = v;
+  }
+}
+
+void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s) {
+  const size_t pairs = A.len / 2, nrow = (size_t)A.k * D;
+  const size_t target = (size_t)RB * 512;
+  size_t G = 1;
+  while (G < nrow && pairs * G < target) G *= 2;
+  if (G > nrow) G = nrow;
+  G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
+  size_t nblk = (pairs * (G + 1) + RB - 1) / RB;
+  if (nblk > RMAXBLK) nblk = RMAXBLK;
+  if (nblk < 1) nblk = 1;
+  K2* partials = (K2*)ws;
+  NS_LAUNCH_TIMED(KT_FUSED, k_round_fused, dim3((unsigned)nblk), dim3(RB), 0, s, A, wg, partials, (int)G);
+  hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
+}
+
 // ---------------- folds ----------------
 __global__ void k_fold_k(const K2* __restrict__ src, size_t len, K2* __restrict__ dst, K2 r) {
   size_t half = len / 2;

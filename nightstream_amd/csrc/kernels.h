@@ -36,7 +36,18 @@ struct HipError {
 // Per-kernel timing: when a prover enabled timing on the calling thread, ktimer_events hands out
 // a start/stop event pair that hipExtLaunchKernelGGL records at the dispatch's own start and end
 // (so host submission gaps are not counted); otherwise it returns false and the launch is plain.
-enum KernelClass { KT_ROUND0 = 0, KT_ROUND, KT_FOLD, KT_AJTAI_GEMM, KT_YD_GEMM, KT_ZMIX, KT_SPLIT, KT_ROUND1, KT_N };
+enum KernelClass {
+  KT_ROUND0 = 0,
+  KT_ROUND,
+  KT_FOLD,
+  KT_AJTAI_GEMM,
+  KT_YD_GEMM,
+  KT_ZMIX,
+  KT_SPLIT,
+  KT_ROUND1,
+  KT_FUSED,
+  KT_N
+};
 bool ktimer_events(int cls, hipEvent_t* start, hipEvent_t* stop);
 #define NS_LAUNCH_TIMED(cls, kernel, grid, block, shmem, stream, ...)                                   \
   do {                                                                                                \
@@ -118,6 +129,25 @@ size_t round_ws_bytes();
 // Writes the deg_max+1 coefficients (low->high) of the round polynomial to out (device).
 void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg /* [k*54] device */, bool round0,
                        K2* out, void* ws, hipStream_t s);
+// Rounds >= 3: fold every table (scalar [nsc][2 len] -> [nsc][len], NC [k*54][2 len] ->
+// [k*54][len]) by r and write the round polynomial's coefficients to out (one fused fold+round
+// launch and the partial sum).  nsc <= 8.
+struct FusedArgs {
+  size_t len;  // table length after the fold
+  const K2* sc_src;
+  K2* sc_dst;
+  int nsc, nf;
+  bool has_eval;
+  const K2* nc_src;
+  K2* nc_dst;
+  int k;
+  K2 gamma_k;
+  FPoly f;
+  K2 r;
+  uint64_t r7;  // 7 r.c1
+};
+constexpr int FUSED_MAX_SC = 8;
+void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s);
 // Fold every table with challenge r: dst[i] = src[2i] + r (src[2i+1] - src[2i]).
 void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStream_t s);
 void launch_fold_digits2(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1,

@@ -677,6 +677,29 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
 }
 
 std::vector<K2> CcsOracle::round_coeffs() {
+  if (pending_) {
+    FusedArgs F;
+    F.len = len_;
+    F.sc_src = w_->sc[cur_sc_].as<K2>();
+    F.sc_dst = w_->sc[cur_sc_ ^ 1].as<K2>();
+    F.nsc = nsc_;
+    F.nf = nf_;
+    F.has_eval = has_eval_;
+    F.nc_src = w_->nc[cur_nc_].as<K2>();
+    F.nc_dst = w_->nc[cur_nc_ ^ 1].as<K2>();
+    F.k = (int)k_;
+    F.gamma_k = gamma_k_;
+    F.f = s_.fpoly;
+    F.r = pending_r_;
+    F.r7 = fmul7(pending_r_.c1);
+    K2* mb = w_->mbox.get<K2>(8);
+    launch_round_fused(F, w_->wg.as<K2>(), mb, w_->ws.p, st_);
+    cur_sc_ ^= 1;
+    cur_nc_ ^= 1;
+    pending_ = false;
+    NS_CHECK(hipStreamSynchronize(st_));
+    return std::vector<K2>(mb, mb + 5);
+  }
   RowTables T;
   const K2* sc = w_->sc[cur_sc_].as<K2>();
   T.len = len_;
@@ -722,14 +745,32 @@ std::vector<K2> CcsOracle::round_coeffs() {
     T.digit_round1 = true;
     T.pc = w_->pc.as<K2>();
   }
-  launch_round_eval(T, s_.fpoly, w_->wg.as<K2>(), round_ == 0, w_->coeffs.as<K2>(), w_->ws.p, st_);
-  std::vector<K2> out(5);
-  d2h(out.data(), w_->coeffs.p, 5, st_);
-  return out;
+  K2* mb = w_->mbox.get<K2>(8);
+  launch_round_eval(T, s_.fpoly, w_->wg.as<K2>(), round_ == 0, mb, w_->ws.p, st_);
+  NS_CHECK(hipStreamSynchronize(st_));
+  return std::vector<K2>(mb, mb + 5);
+}
+
+void CcsOracle::flush() {
+  if (!pending_) return;
+  const size_t prev = len_ * 2;
+  launch_fold_k(w_->sc[cur_sc_].as<K2>(), (size_t)nsc_, prev, w_->sc[cur_sc_ ^ 1].as<K2>(), pending_r_, st_);
+  launch_fold_k(w_->nc[cur_nc_].as<K2>(), k_ * D, prev, w_->nc[cur_nc_ ^ 1].as<K2>(), pending_r_, st_);
+  cur_sc_ ^= 1;
+  cur_nc_ ^= 1;
+  pending_ = false;
 }
 
 void CcsOracle::fold(K2 r) {
+  flush();
   const size_t half = len_ / 2;
+  if (round_ >= 2 && nsc_ <= FUSED_MAX_SC && half >= 1) {
+    pending_ = true;
+    pending_r_ = r;
+    len_ = half;
+    round_++;
+    return;
+  }
   int nsc_dst = cur_sc_ ^ 1;
   launch_fold_k(w_->sc[cur_sc_].as<K2>(), (size_t)nsc_, len_, w_->sc[nsc_dst].as<K2>(), r, st_);
   if (round_ == 0) {
@@ -753,6 +794,7 @@ void CcsOracle::fold(K2 r) {
 }
 
 std::vector<K2> CcsOracle::y_eval(const std::vector<K2>& r, DevBuf& ws, DevBuf& vjs) {
+  flush();
   const size_t m = s_.m, t = s_.t, n = s_.n;
   w_->bits.ensure(128 * sizeof(K2));
   w_->chi.ensure(((size_t)1 << ell_n_) * sizeof(K2));

@@ -101,8 +101,26 @@ struct MeHost {
 };
 
 // Device buffers of a Pi_CCS oracle, reused across fold steps (grow-only).
+// Pinned host memory the device writes directly (round coefficients): no copy, and the host
+// reads it after the stream synchronizes.
+struct HostMailbox {
+  void* p = nullptr;
+  HostMailbox() = default;
+  HostMailbox(const HostMailbox&) = delete;
+  HostMailbox& operator=(const HostMailbox&) = delete;
+  ~HostMailbox() {
+    if (p) (void)hipHostFree(p);
+  }
+  template <class T>
+  T* get(size_t n) {
+    if (!p) NS_CHECK(hipHostMalloc(&p, std::max<size_t>(n * sizeof(T), 256), hipHostMallocDefault));
+    return (T*)p;
+  }
+};
+
 struct OracleWs {
   DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc;
+  HostMailbox mbox;  // 8 K2: the round polynomial's coefficients
 };
 
 struct StepTiming {
@@ -194,6 +212,11 @@ class CcsOracle {
   int cur_sc_ = 0, cur_nc_ = 0;
   std::vector<const int8_t*> zs_;
   K2 r0_ = K2{0, 0};  // round-0 challenge: round 1 is evaluated from the digits with it
+  // From round 2 on, fold(r) only records r: the next round_coeffs folds and evaluates in one
+  // launch (launch_round_fused), and anything else that reads the tables flushes it first.
+  bool pending_ = false;
+  K2 pending_r_ = K2{0, 0};
+  void flush();
 };
 
 }  // namespace ns
