@@ -118,7 +118,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 16)
     ap.add_argument("--cpu-steps", type=int, default=2, help="fold steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", choices=["shard", "batch", "serve"], default="shard",
+    ap.add_argument("--workload", choices=["shard", "batch", "serve", "twist"], default="shard",
                     help="shard: one 2^16-row shard (default); batch: 128 shards of 2^10..2^16 rows, plain path vs "
                          "a pool of concurrent prover contexts; serve: Poisson arrivals with continuous admission")
     ap.add_argument("--workers", type=int, default=16, help="concurrent prover contexts (batch / serve)")
@@ -146,6 +146,8 @@ def main():
         sys.exit("--steps must be >= 1")
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "twist":
+        return bench_twist(args)
     if args.workload == "serve":
         return bench_serve(args)
 
@@ -350,6 +352,136 @@ def bench_serve(args):
         "rates": out, "mixed_equals_alone": same,
     }
     print(json.dumps(line), flush=True)
+
+
+Q_FIELD = 0xFFFFFFFF00000001
+
+
+def twist_chunk_lanes(seed: int, ell_n: int = 12):
+    """The memory lanes of one config-3 chunk of 2^ell_n cycles: two register-file Twist
+    instances (32 words, one read port each, writes on the first), the RAM Twist (2^16 words,
+    35% of cycles, Zipf(1.1) addresses, 40% writes) and the three Shout lookup lanes
+    (8-bit x 8-bit ADD / XOR / AND tables of 2^16 entries)."""
+    from nightstream_amd.workload import gen_lookup_trace, gen_memory_trace
+    regs_a = gen_memory_trace(ell_n, 5, 1.0, 0.5, seed=seed * 8 + 1)
+    regs_b = gen_memory_trace(ell_n, 5, 1.0, 0.0, seed=seed * 8 + 2)
+    ram = gen_memory_trace(ell_n, 16, 0.35, 0.4, seed=seed * 8 + 3, zipf_s=1.1)
+    return [regs_a, regs_b, ram], gen_lookup_trace(ell_n, seed=seed * 8 + 4)
+
+
+def twist_chunk_oracles(cls, twists, shouts, rng, ell_n: int = 12):
+    """Every Twist/Shout oracle of one chunk (time lanes, address lanes, lookups) as (label,
+    constructor) pairs over class `cls` (the GPU TsOracle or the CPU restatement's)."""
+    def rand_kk(r):
+        return (int(r.integers(0, 1 << 63)) * 2 % Q_FIELD, int(r.integers(0, 1 << 63)) * 2 % Q_FIELD)
+    out = []
+    for i, tr in enumerate(twists):
+        ea = tr["ell_addr"]
+        r_cycle = [rand_kk(rng) for _ in range(ell_n)]
+        r_addr = [rand_kk(rng) for _ in range(ea)]
+        init_at = rand_kk(rng)
+        r_time = [rand_kk(rng) for _ in range(ell_n)]
+        w = [rand_kk(rng) for _ in tr["ra_bits"]]
+        out += [
+            (f"twist{i}/read", lambda tr=tr, a=(r_cycle, r_addr, init_at): cls(
+                "twist_read", a[0], tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["inc"], tr["wa_bits"],
+                a[1], a[2])),
+            (f"twist{i}/write", lambda tr=tr, a=(r_cycle, r_addr, init_at): cls(
+                "twist_write", a[0], tr["has_write"], tr["wv"], tr["inc"], tr["wa_bits"], a[1], a[2])),
+            (f"twist{i}/val_eval", lambda tr=tr, a=(r_addr, r_time): cls(
+                "val_eval", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], a[0], a[1])),
+            (f"twist{i}/total_inc", lambda tr=tr, a=r_addr: cls(
+                "total_inc", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], a)),
+            (f"twist{i}/bitness", lambda tr=tr, a=(r_cycle, w): cls("bitness", a[0], tr["ra_bits"], a[1])),
+            (f"twist{i}/read_addr", lambda tr=tr, a=r_cycle: cls(
+                "read_addr", tr["init"], a, tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["wa_bits"],
+                tr["inc"])),
+            (f"twist{i}/write_addr", lambda tr=tr, a=r_cycle: cls(
+                "write_addr", tr["init"], a, tr["has_write"], tr["wv"], tr["wa_bits"], tr["inc"])),
+        ]
+    for j, ln in enumerate(shouts):
+        r_cycle = [rand_kk(rng) for _ in range(ell_n)]
+        r_addr = [rand_kk(rng) for _ in range(16)]
+        out += [
+            (f"shout{j}/value", lambda ln=ln, a=r_cycle: cls("shout_value", a, ln["has_lookup"], ln["val"])),
+            (f"shout{j}/index_adapter", lambda ln=ln, a=(r_cycle, r_addr): cls(
+                "index_adapter", a[0], ln["has_lookup"], ln["addr_bits"], a[1])),
+            (f"shout{j}/addr_lookup", lambda ln=ln, a=r_cycle: cls(
+                "addr_lookup", ln["addr_bits"], ln["has_lookup"], ln["table"], a)),
+        ]
+    return out
+
+
+def drive_oracle(o, chals):
+    """All rounds of one oracle: values at 0..deg per round, folded by chals; returns them."""
+    vals = []
+    for rd in range(o.num_rounds()):
+        vals.append(o.evals([(v, 0) for v in range(o.degree_bound() + 1)]))
+        o.fold(chals[rd])
+    return vals
+
+
+def bench_twist(args):
+    """Config-3 memory sidecar, Twist/Shout round oracles only: 16 chunks of 2^12 cycles, every
+    oracle of a chunk constructed and driven through all its rounds on the GPU with seeded
+    challenges (no transcript), timed per chunk; the CPU restatement runs chunk 0 as the
+    baseline, and chunk 0's values must be identical on both."""
+    import nightstream_amd as ns
+    from nightstream_amd import twist
+    from oracle import pyoracle
+    rng_ch = np.random.default_rng(args.seed)
+    chunks = [twist_chunk_lanes(c) for c in range(args.steps)]
+    chals = [[(int(a) * 2 % Q_FIELD, int(b) * 2 % Q_FIELD) for a, b in rng_ch.integers(0, 1 << 63, size=(40, 2))]
+             for _ in range(64)]
+
+    def run(cls, c, check=None):
+        twists, shouts = chunks[c]
+        rng = np.random.default_rng(1000 + c)
+        vals, per = [], {}
+        for i, (label, make) in enumerate(twist_chunk_oracles(cls, twists, shouts, rng)):
+            t0 = time.perf_counter()
+            o = make()
+            v = drive_oracle(o, chals[i])
+            per[label] = time.perf_counter() - t0
+            vals.append(v)
+        return vals, per
+
+    for c in range(min(args.warmup, len(chunks))):
+        run(twist.TsOracle, c)
+    ns.device_synchronize()
+    t0 = time.perf_counter()
+    per_all = {}
+    gpu_vals0 = None
+    for c in range(len(chunks)):
+        v, per = run(twist.TsOracle, c)
+        if c == 0:
+            gpu_vals0 = v
+        for k, x in per.items():
+            per_all[k] = per_all.get(k, 0.0) + x
+    ns.device_synchronize()
+    t_gpu = time.perf_counter() - t0
+    pyoracle.build()
+    t1 = time.perf_counter()
+    cpu_vals0, _ = run(pyoracle.TsOracle, 0)
+    t_cpu = time.perf_counter() - t1
+    identical = cpu_vals0 == gpu_vals0
+    n_or = len(per_all)
+    line = {
+        "metric": "Twist/Shout round oracles of a config-3 chunk (2^12 cycles), chunks/s",
+        "value": round(len(chunks) / t_gpu, 3), "unit": "chunks/s", "n_gpus": 1, "steps": len(chunks),
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u64 (Goldilocks F_q^2)", "data": "synthetic",
+        "config": {"workload": "twist: 2 register-file Twists (32 words), RAM Twist (2^16 words, 35%, Zipf 1.1, "
+                               "40% writes), 3 Shout lanes (8x8-bit ADD/XOR/AND); 2^12-cycle chunks",
+                   "oracles_per_chunk": n_or, "chunks": len(chunks)},
+        "ms_per_chunk": round(t_gpu / len(chunks) * 1e3, 3),
+        "ms_per_oracle_per_chunk": {k: round(v / len(chunks) * 1e3, 3) for k, v in per_all.items()},
+        "cpu_baseline": {"value": round(1.0 / t_cpu, 4), "unit": "chunks/s", "cores": 1, "kind": "port",
+                         "sample": "CPU restatement (oracle/twist.c) of chunk 0, every oracle, single thread"},
+        "values_bit_identical_chunk0": identical,
+    }
+    print(json.dumps(line), flush=True)
+    if not identical:
+        sys.exit("GPU and CPU Twist/Shout values differ")
 
 
 if __name__ == "__main__":

@@ -84,6 +84,7 @@ class TsOracleGpu {
   K2 init_at_ = K2{0, 0};
   size_t nterms_ = 0;
   DevBuf term_t_, term_v_, term_p_;
+  DevBuf lt_c_[2], lt_s_[2], lt_tmp_;  // per-round LT_low contributions and their prefix sums
   std::vector<uint32_t> term_t_h_;
   std::vector<K2> term_v_h_;
   // product oracle

@@ -98,6 +98,8 @@ struct TsTimeArgs {
   const uint32_t* term_t;
   const K2* term_v;
   const K2* term_p;
+  const K2* lt_s0;  // inclusive prefix sums over the terms of term_v * lt_low(term_t, bit_idx, c)
+  const K2* lt_s1;
   uint32_t nterms;
   int ncoef;
 };
@@ -133,14 +135,30 @@ __device__ K2 lt_low(uint32_t u, int b, int c, const K2* chals) {
   return acc;
 }
 
-__device__ K2 val_pre_child(const TsTimeArgs& A, uint64_t pair, int c, const K2* chals) {
+// val_pre at the child time point (twist_oracle.rs:313-361): every term before the pair's time
+// block contributes its full value (exclusive prefix term_p), the terms inside the block their
+// LT_low-weighted value, summed as a difference of the per-term prefix sums lt_s0 / lt_s1.
+__device__ K2 val_pre_child(const TsTimeArgs& A, uint64_t pair, int c) {
   const int b = A.bit_idx;
   uint64_t lo_key = pair << (b + 1), hi_key = (pair + 1) << (b + 1);
   uint32_t lo = lower_bound_u32(A.term_t, 0, A.nterms, lo_key);
   uint32_t hi = lower_bound_u32(A.term_t, lo, A.nterms, hi_key);
   K2 v = kadd(A.init_at, A.term_p[lo]);
-  for (uint32_t k = lo; k < hi; k++) v = kadd(v, kmul(A.term_v[k], lt_low(A.term_t[k], b, c, chals)));
+  if (hi > lo) {
+    const K2* S = c ? A.lt_s1 : A.lt_s0;
+    v = kadd(v, lo > 0 ? ksub(S[hi - 1], S[lo - 1]) : S[hi - 1]);
+  }
   return v;
+}
+
+// per-term LT_low contributions of one round, for both children (one thread per term)
+__global__ void k_lt_contrib(const uint32_t* __restrict__ term_t, const K2* __restrict__ term_v, uint32_t nterms,
+                             int b, const K2* __restrict__ chals, K2* __restrict__ out0, K2* __restrict__ out1) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nterms) return;
+  const K2 v = term_v[k];
+  out0[k] = kmul(v, lt_low(term_t[k], b, 0, chals));
+  out1[k] = kmul(v, lt_low(term_t[k], b, 1, chals));
 }
 
 // lt_eval (mle.rs:7-24) at the child time point (chals | c | bits of pair) against r_time
@@ -229,13 +247,13 @@ __global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restri
           has_f = false;
           break;
         case TSK_TWIST_READ: {
-          K2 vp0 = val_pre_child(A, pair, 0, chals), vp1 = val_pre_child(A, pair, 1, chals);
+          K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
           f0 = ksub(vp0, COLV(A.col_a, i0));
           f1 = ksub(vp1, COLV(A.col_a, i1));
           break;
         }
         case TSK_TWIST_WRITE: {
-          K2 vp0 = val_pre_child(A, pair, 0, chals), vp1 = val_pre_child(A, pair, 1, chals);
+          K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
           f0 = ksub(ksub(COLV(A.col_a, i0), vp0), COLV(A.col_b, i0));
           f1 = ksub(ksub(COLV(A.col_a, i1), vp1), COLV(A.col_b, i1));
           break;
@@ -753,6 +771,26 @@ void TsOracleGpu::time_round(std::vector<K2>& c) {
   A.term_p = term_p_.as<K2>();
   A.nterms = (uint32_t)nterms_;
   A.ncoef = ncoef;
+  A.lt_s0 = A.lt_s1 = nullptr;
+  if ((kind_ == TSK_TWIST_READ || kind_ == TSK_TWIST_WRITE) && nterms_ > 0) {
+    const uint32_t nt = (uint32_t)nterms_;
+    for (int c = 0; c < 2; c++) {
+      lt_c_[c].ensure(nterms_ * sizeof(K2));
+      lt_s_[c].ensure(nterms_ * sizeof(K2));
+    }
+    hipLaunchKernelGGL(k_lt_contrib, dim3((nt + 255) / 256), dim3(256), 0, st_, term_t_.as<uint32_t>(),
+                       term_v_.as<K2>(), nt, (int)bit_idx_, consts_.as<K2>() + ell_n_, lt_c_[0].as<K2>(),
+                       lt_c_[1].as<K2>());
+    size_t tb = 0;
+    NS_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, tb, lt_c_[0].as<K2>(), lt_s_[0].as<K2>(), KAddOp(), (int)nt,
+                                               st_));
+    lt_tmp_.ensure(tb + 16);
+    for (int c = 0; c < 2; c++)
+      NS_CHECK(hipcub::DeviceScan::InclusiveScan(lt_tmp_.p, tb, lt_c_[c].as<K2>(), lt_s_[c].as<K2>(), KAddOp(),
+                                                 (int)nt, st_));
+    A.lt_s0 = lt_s_[0].as<K2>();
+    A.lt_s1 = lt_s_[1].as<K2>();
+  }
   unsigned nb = nblocks(n_);
   if (ncoef <= 8)
     hipLaunchKernelGGL(k_ts_time_round<8>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
