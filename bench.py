@@ -412,20 +412,31 @@ def twist_chunk_oracles(cls, twists, shouts, rng, ell_n: int = 12):
     return out
 
 
-def drive_oracle(o, chals):
-    """All rounds of one oracle: values at 0..deg per round, folded by chals; returns them."""
-    vals = []
-    for rd in range(o.num_rounds()):
-        vals.append(o.evals([(v, 0) for v in range(o.degree_bound() + 1)]))
-        o.fold(chals[rd])
+def drive_group(os, chals, batch=None):
+    """All rounds of oracles with the same number of rounds and shared challenges (the round loop
+    of run_batched_sumcheck_prover): with `batch` (the GPU module) every round of the group is one
+    batch_round / batch_fold; without it each oracle is evaluated at 0..deg and folded in turn.
+    Returns per oracle the list of per-round values."""
+    vals = [[] for _ in os]
+    for rd in range(os[0].num_rounds() if os else 0):
+        if batch is not None:
+            for i, y in enumerate(batch.batch_round(os)):
+                vals[i].append(y)
+            batch.batch_fold(os, chals[rd])
+        else:
+            for i, o in enumerate(os):
+                vals[i].append(o.evals([(v, 0) for v in range(o.degree_bound() + 1)]))
+                o.fold(chals[rd])
     return vals
 
 
 def bench_twist(args):
-    """Config-3 memory sidecar, Twist/Shout round oracles only: 16 chunks of 2^12 cycles, every
-    oracle of a chunk constructed and driven through all its rounds on the GPU with seeded
-    challenges (no transcript), timed per chunk; the CPU restatement runs chunk 0 as the
-    baseline, and chunk 0's values must be identical on both."""
+    """Config-3 memory sidecar, Twist/Shout round oracles only: 2^12-cycle chunks; every oracle
+    of a chunk is constructed, the oracles are grouped by number of rounds (12 time-lane rounds,
+    5 or 16 address-lane rounds) and each group is driven through all its rounds with shared
+    seeded challenges (no transcript), on the GPU with the batched driver, timed per chunk.  The
+    same chunks are also timed one oracle at a time on the GPU; the CPU restatement runs chunk 0
+    as the baseline, and chunk 0's values must be identical on all three."""
     import nightstream_amd as ns
     from nightstream_amd import twist
     from oracle import pyoracle
@@ -434,47 +445,56 @@ def bench_twist(args):
     chals = [[(int(a) * 2 % Q_FIELD, int(b) * 2 % Q_FIELD) for a, b in rng_ch.integers(0, 1 << 63, size=(40, 2))]
              for _ in range(64)]
 
-    def run(cls, c, check=None):
+    def run(cls, c, batch=None):
         twists, shouts = chunks[c]
         rng = np.random.default_rng(1000 + c)
-        vals, per = [], {}
-        for i, (label, make) in enumerate(twist_chunk_oracles(cls, twists, shouts, rng)):
-            t0 = time.perf_counter()
-            o = make()
-            v = drive_oracle(o, chals[i])
-            per[label] = time.perf_counter() - t0
-            vals.append(v)
-        return vals, per
+        t0 = time.perf_counter()
+        made = [(label, make()) for label, make in twist_chunk_oracles(cls, twists, shouts, rng)]
+        per = {"construct": time.perf_counter() - t0}
+        groups = {}
+        for i, (_, o) in enumerate(made):
+            groups.setdefault(o.num_rounds(), []).append(i)
+        vals = [[] for _ in made]
+        for nr, idxs in groups.items():
+            t1 = time.perf_counter()
+            for i, v in zip(idxs, drive_group([made[i][1] for i in idxs], chals[nr], batch)):
+                vals[i] = v
+            per[f"{nr} rounds x {len(idxs)} oracles"] = time.perf_counter() - t1
+        return vals, per, len(made)
 
-    for c in range(min(args.warmup, len(chunks))):
-        run(twist.TsOracle, c)
-    ns.device_synchronize()
-    t0 = time.perf_counter()
-    per_all = {}
-    gpu_vals0 = None
-    for c in range(len(chunks)):
-        v, per = run(twist.TsOracle, c)
-        if c == 0:
-            gpu_vals0 = v
-        for k, x in per.items():
-            per_all[k] = per_all.get(k, 0.0) + x
-    ns.device_synchronize()
-    t_gpu = time.perf_counter() - t0
+    def timed(batch):
+        for c in range(min(args.warmup, len(chunks))):
+            run(twist.TsOracle, c, batch)
+        ns.device_synchronize()
+        t0 = time.perf_counter()
+        per_all, vals0, n_or = {}, None, 0
+        for c in range(len(chunks)):
+            v, per, n_or = run(twist.TsOracle, c, batch)
+            if c == 0:
+                vals0 = v
+            for k, x in per.items():
+                per_all[k] = per_all.get(k, 0.0) + x
+        ns.device_synchronize()
+        return time.perf_counter() - t0, per_all, vals0, n_or
+
+    t_gpu, per_all, gpu_vals0, n_or = timed(twist)
+    t_one, _, one_vals0, _ = timed(None)
     pyoracle.build()
     t1 = time.perf_counter()
-    cpu_vals0, _ = run(pyoracle.TsOracle, 0)
+    cpu_vals0, _, _ = run(pyoracle.TsOracle, 0)
     t_cpu = time.perf_counter() - t1
-    identical = cpu_vals0 == gpu_vals0
-    n_or = len(per_all)
+    identical = cpu_vals0 == gpu_vals0 == one_vals0
     line = {
         "metric": "Twist/Shout round oracles of a config-3 chunk (2^12 cycles), chunks/s",
         "value": round(len(chunks) / t_gpu, 3), "unit": "chunks/s", "n_gpus": 1, "steps": len(chunks),
         "higher_is_better": True, "vs_baseline": None, "dtype": "u64 (Goldilocks F_q^2)", "data": "synthetic",
         "config": {"workload": "twist: 2 register-file Twists (32 words), RAM Twist (2^16 words, 35%, Zipf 1.1, "
-                               "40% writes), 3 Shout lanes (8x8-bit ADD/XOR/AND); 2^12-cycle chunks",
+                               "40% writes), 3 Shout lanes (8x8-bit ADD/XOR/AND); 2^12-cycle chunks; oracles "
+                               "grouped by number of rounds, each group a batched sum-check",
                    "oracles_per_chunk": n_or, "chunks": len(chunks)},
         "ms_per_chunk": round(t_gpu / len(chunks) * 1e3, 3),
-        "ms_per_oracle_per_chunk": {k: round(v / len(chunks) * 1e3, 3) for k, v in per_all.items()},
+        "ms_per_chunk_one_oracle_at_a_time": round(t_one / len(chunks) * 1e3, 3),
+        "ms_per_phase_per_chunk": {k: round(v / len(chunks) * 1e3, 3) for k, v in per_all.items()},
         "cpu_baseline": {"value": round(1.0 / t_cpu, 4), "unit": "chunks/s", "cores": 1, "kind": "port",
                          "sample": "CPU restatement (oracle/twist.c) of chunk 0, every oracle, single thread"},
         "values_bit_identical_chunk0": identical,

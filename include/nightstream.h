@@ -194,6 +194,15 @@ int ns_ts_addr_lookup_new(const ns_sparse_col* addr_bits, size_t ell_addr, ns_sp
                           uint64_t claim[2], ns_ts** out);
 int ns_ts_evals(ns_ts* o, const uint64_t* xs, size_t n, uint64_t* out);
 int ns_ts_fold(ns_ts* o, const uint64_t r[2]);
+/* One round of the batched sum-check with shared challenges
+ * (run_batched_sumcheck_prover, neo-reductions/src/sumcheck.rs:350-456, the evals_at loop at
+ * :390-407): every oracle's round is enqueued on its own stream before the host waits for any.
+ * For each oracle in order, its round polynomial's values at x = 0, 1, ..., degree_bound are
+ * written to out (degree_bound + 1 K values, 2 words each, oracles back to back).  Every oracle
+ * must have the same number of rounds left, at least one (the reference panics otherwise). */
+int ns_ts_batch_round(ns_ts* const* os, size_t n, uint64_t* out);
+/* Fold every oracle with the shared challenge r (sumcheck.rs:436-442). */
+int ns_ts_batch_fold(ns_ts* const* os, size_t n, const uint64_t r[2]);
 size_t ns_ts_num_rounds(const ns_ts* o);
 size_t ns_ts_degree_bound(const ns_ts* o);
 void ns_ts_free(ns_ts* o);

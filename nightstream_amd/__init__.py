@@ -30,6 +30,7 @@ EXPORTS = [
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
     "ns_ts_twist_read_new", "ns_ts_twist_write_new", "ns_ts_val_eval_new", "ns_ts_total_inc_new",
     "ns_ts_read_addr_new", "ns_ts_write_addr_new", "ns_ts_addr_lookup_new", "ns_ts_evals", "ns_ts_fold",
+    "ns_ts_batch_round", "ns_ts_batch_fold",
     "ns_ts_num_rounds", "ns_ts_degree_bound", "ns_ts_free", "ns_bus_decode", "ns_bus_time_cols", "ns_bus_openings",
 ]
 

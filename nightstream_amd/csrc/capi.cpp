@@ -606,6 +606,43 @@ extern "C" int ns_ts_fold(ns_ts* o, const uint64_t r[2]) {
   });
 }
 
+static std::vector<TsOracleGpu*> ts_handles(ns_ts* const* os, size_t n) {
+  if (n && !os) throw Error(1, "null oracle array");
+  std::vector<TsOracleGpu*> v(n);
+  for (size_t i = 0; i < n; i++) {
+    if (!os[i]) throw Error(1, "null oracle");
+    v[i] = os[i]->o.get();
+  }
+  return v;
+}
+
+extern "C" int ns_ts_batch_round(ns_ts* const* os, size_t n, uint64_t* out) {
+  return guard([&] {
+    std::vector<TsOracleGpu*> v = ts_handles(os, n);
+    std::vector<std::vector<K2>> cs;
+    ts_batch_round(v.data(), n, cs);
+    size_t w = 0;
+    for (size_t i = 0; i < n; i++) {
+      const size_t deg = v[i]->degree_bound();
+      // values at x = 0..deg of the coefficient vector (Horner), deg + 1 per oracle
+      for (size_t x = 0; x <= deg; x++) {
+        const K2 xk{(uint64_t)x, 0};
+        K2 acc = kzero();
+        for (size_t d = cs[i].size(); d-- > 0;) acc = kadd(kmul(acc, xk), cs[i][d]);
+        out[w++] = acc.c0;
+        out[w++] = acc.c1;
+      }
+    }
+  });
+}
+
+extern "C" int ns_ts_batch_fold(ns_ts* const* os, size_t n, const uint64_t r[2]) {
+  return guard([&] {
+    std::vector<TsOracleGpu*> v = ts_handles(os, n);
+    ts_batch_fold(v.data(), n, K2{r[0], r[1]});
+  });
+}
+
 extern "C" size_t ns_ts_num_rounds(const ns_ts* o) { return o ? o->o->num_rounds() : 0; }
 extern "C" size_t ns_ts_degree_bound(const ns_ts* o) { return o ? o->o->degree_bound() : 0; }
 extern "C" void ns_ts_free(ns_ts* o) { delete o; }

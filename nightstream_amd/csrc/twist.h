@@ -35,6 +35,26 @@ enum TsKind {
 
 constexpr int TS_MAX_COEF = 40;  // degree bound 3 + ell_addr <= 39
 
+// Device memory of a Twist/Shout oracle, drawn from a process-wide cache of blocks in
+// power-of-two size classes instead of hipMalloc / hipFree: a chunk builds and drops ~30 oracles,
+// and every hipFree waits for the whole device.  A block returns to the cache only when no work
+// can still use it: an oracle drains its stream before its buffers are released, and a block
+// replaced by a larger one while the oracle is live waits for the device first.  Contents of a
+// block are undefined when it is handed out, as with hipMalloc.
+struct TsBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  TsBuf() = default;
+  TsBuf(const TsBuf&) = delete;
+  TsBuf& operator=(const TsBuf&) = delete;
+  ~TsBuf();
+  void ensure(size_t n);
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
 class TsOracleGpu {
  public:
   TsOracleGpu();
@@ -61,41 +81,62 @@ class TsOracleGpu {
   void fold(K2 r);
   K2 claim();
 
- private:
+  // A round and a fold in two halves, for drivers that step many oracles together: launch_*
+  // enqueues the work on this oracle's stream (the round's coefficients are copied to pinned
+  // host memory); finish_* waits for it.  round_coeffs = launch_round + finish_round.
+  void launch_round();
+  void finish_round(std::vector<K2>& c);
   void round_coeffs(std::vector<K2>& c);
-  void time_round(std::vector<K2>& c);
-  void time_fold(K2 r);
+  void launch_fold(K2 r);
+  void finish_fold();
+
+ private:
+  bool time_round_launch();
+  void time_fold_launch(K2 r);
+  bool addr_round_launch(bool final);
   void addr_round(std::vector<K2>& c, bool final);
-  void product_round(std::vector<K2>& c);
+  bool product_round_launch();
   K2 time_final();
 
   TsKind kind_ = TSK_PRODUCT;
-  hipStream_t st_ = nullptr;
+  hipStream_t st_ = nullptr;  // from the same cache as the blocks, like the pinned mailbox
+  K2* coef_h_ = nullptr;      // pinned: TS_MAX_COEF coefficients, then the folded support size
+  int round_ncoef_ = 0;      // coefficients of the launched round
+  bool round_dev_ = false;   // the launched round copies them from the device
+  bool fold_pending_ = false;  // a launched time-lane fold has not reported its support size
   size_t degree_bound_ = 0, ell_n_ = 0, ell_addr_ = 0, bit_idx_ = 0;
   K2 prefix_eq_ = K2{1, 0};
   std::vector<K2> r_cycle_, r_addr_, chals_;
   // time lanes: columns [gate, a, b, bits...] over the union support
   size_t n_ = 0, cap_ = 1, len_ = 0, ncol_ = 0, nbits_ = 0;
   int col_a_ = -1, col_b_ = -1;
-  DevBuf idx_[2], cols_[2], flags_, pos_, scan_tmp_, partials_, coef_dev_, consts_;
+  TsBuf idx_[2], cols_[2], flags_, pos_, scan_tmp_, partials_, coef_dev_, consts_;
   int cur_ = 0;
   size_t scan_tmp_bytes_ = 0;
   // Twist val_pre terms (sorted by time) and their exclusive prefix sums
   K2 init_at_ = K2{0, 0};
   size_t nterms_ = 0;
-  DevBuf term_t_, term_v_, term_p_;
-  DevBuf lt_c_[2], lt_s_[2], lt_tmp_;  // per-round LT_low contributions and their prefix sums
+  TsBuf term_t_, term_v_, term_p_;
+  TsBuf lt_c_[2], lt_s_[2], lt_tmp_;  // per-round LT_low contributions and their prefix sums
   std::vector<uint32_t> term_t_h_;
   std::vector<K2> term_v_h_;
   // product oracle
   size_t nf_ = 0, flen_ = 0;
-  DevBuf fac_[2];
+  TsBuf fac_[2];
   // address lanes
   size_t nev_ = 0, ninit_tab_ = 0;
-  DevBuf ev_eq_, ev_hr_, ev_rv_, ev_hw_, ev_inc_, ev_ra_, ev_wa_, ev_rw_, ev_ww_, it_[2];
-  DevBuf keys_[2], vals_[2], dsort_, dscan_, sort_tmp_;
+  TsBuf ev_eq_, ev_hr_, ev_rv_, ev_hw_, ev_inc_, ev_ra_, ev_wa_, ev_rw_, ev_ww_, it_[2];
+  TsBuf keys_[2], vals_[2], dsort_, dscan_, sort_tmp_;
+  TsBuf scratch_[9];  // constructors' temporaries, kept until the oracle is dropped
   size_t sort_tmp_bytes_ = 0;
 };
+
+// One round of the batched sum-check (neo-reductions/src/sumcheck.rs:388-407): every oracle's
+// round is enqueued before the host waits for any; coeffs[i] receives oracle i's coefficients.
+// All oracles must have the same number of rounds, at least one.
+void ts_batch_round(TsOracleGpu* const* os, size_t n, std::vector<std::vector<K2>>& coeffs);
+// Fold every oracle with the shared challenge (sumcheck.rs:436-442).
+void ts_batch_fold(TsOracleGpu* const* os, size_t n, K2 r);
 
 // Shared CPU bus helpers (neo-fold/src/memory_sidecar/cpu_bus.rs): Z is the host digit matrix
 // [54][m] of the CPU witness.

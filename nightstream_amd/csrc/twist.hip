@@ -16,6 +16,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "twist.h"
 
@@ -284,6 +286,11 @@ __global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restri
     for (int d = 0; d < MAXC; d++) acc[d] = kadd(acc[d], p[d]);
   }
   block_store<MAXC>(acc, A.ncoef, partials);
+}
+
+// one K value into device memory (a round challenge into an oracle's constants block)
+__global__ void k_put_k2(K2* __restrict__ dst, K2 v) {
+  if (threadIdx.x == 0) *dst = v;
 }
 
 __global__ void k_ts_heads(const uint32_t* __restrict__ idx, uint32_t n, uint32_t* __restrict__ flags) {
@@ -558,8 +565,123 @@ static unsigned nblocks(size_t n, size_t cap = TMAXBLK) {
   return (unsigned)b;
 }
 
+// ---------------- block / stream / mailbox cache (one device per process) ----------------
+namespace {
+class TsPool {
+ public:
+  static TsPool& get() {
+    static TsPool* pool = new TsPool;  // never destroyed: no HIP calls during process exit
+    return *pool;
+  }
+  static size_t size_class(size_t n) {
+    size_t c = 256;
+    while (c < n) c <<= 1;
+    return c;
+  }
+  void* take(size_t cap) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = blocks_.find(cap);
+      if (it != blocks_.end()) {
+        void* p = it->second;
+        blocks_.erase(it);
+        cached_ -= cap;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, cap) != hipSuccess) {
+      (void)hipGetLastError();
+      flush();
+      NS_CHECK(hipMalloc(&p, cap));
+    }
+    return p;
+  }
+  void give(void* p, size_t cap) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (cached_ + cap <= kMaxCached) {
+        blocks_.emplace(cap, p);
+        cached_ += cap;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+  void flush() {
+    std::multimap<size_t, void*> all;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      all.swap(blocks_);
+      cached_ = 0;
+    }
+    for (auto& b : all) (void)hipFree(b.second);
+  }
+  hipStream_t stream() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (!streams_.empty()) {
+        hipStream_t s = streams_.back();
+        streams_.pop_back();
+        return s;
+      }
+    }
+    hipStream_t s = nullptr;
+    NS_CHECK(hipStreamCreate(&s));
+    return s;
+  }
+  void put_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> g(mu_);
+    streams_.push_back(s);
+  }
+  K2* mailbox() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (!mboxes_.empty()) {
+        K2* m = mboxes_.back();
+        mboxes_.pop_back();
+        return m;
+      }
+    }
+    void* p = nullptr;
+    NS_CHECK(hipHostMalloc(&p, (TS_MAX_COEF + 1) * sizeof(K2), hipHostMallocDefault));
+    return (K2*)p;
+  }
+  void put_mailbox(K2* m) {
+    std::lock_guard<std::mutex> g(mu_);
+    mboxes_.push_back(m);
+  }
+
+ private:
+  static constexpr size_t kMaxCached = (size_t)4 << 30;
+  std::mutex mu_;
+  std::multimap<size_t, void*> blocks_;
+  size_t cached_ = 0;
+  std::vector<hipStream_t> streams_;
+  std::vector<K2*> mboxes_;
+};
+}  // namespace
+
+TsBuf::~TsBuf() {
+  if (p) TsPool::get().give(p, bytes);
+}
+
+void TsBuf::ensure(size_t n) {
+  if (n <= bytes && p) return;
+  if (p) {
+    // the old block may still be read by work in flight on the owner's stream
+    NS_CHECK(hipDeviceSynchronize());
+    TsPool::get().give(p, bytes);
+    p = nullptr;
+    bytes = 0;
+  }
+  const size_t cap = TsPool::size_class(n);
+  p = TsPool::get().take(cap);
+  bytes = cap;
+}
+
 template <class T>
-static void upload(DevBuf& d, const std::vector<T>& h, hipStream_t st, size_t min_elems = 1) {
+static void upload(TsBuf& d, const std::vector<T>& h, hipStream_t st, size_t min_elems = 1) {
   size_t n = std::max(h.size(), min_elems);
   d.ensure(n * sizeof(T));
   if (!h.empty()) NS_CHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
@@ -568,11 +690,17 @@ static void upload(DevBuf& d, const std::vector<T>& h, hipStream_t st, size_t mi
 TsOracleGpu::TsOracleGpu() {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw HipError(hipErrorNoDevice, "no device", __FILE__, __LINE__);
-  NS_CHECK(hipStreamCreate(&st_));
+  st_ = TsPool::get().stream();
+  coef_h_ = TsPool::get().mailbox();
 }
 
+// the stream drains before the members' blocks go back to the cache
 TsOracleGpu::~TsOracleGpu() {
-  if (st_) (void)hipStreamDestroy(st_);
+  if (st_) {
+    (void)hipStreamSynchronize(st_);
+    TsPool::get().put_stream(st_);
+  }
+  if (coef_h_) TsPool::get().put_mailbox(coef_h_);
 }
 
 static SparseCol empty_col() { return SparseCol{}; }
@@ -743,14 +871,11 @@ size_t TsOracleGpu::num_rounds() const {
   }
 }
 
-void TsOracleGpu::time_round(std::vector<K2>& c) {
+// enqueue one round of a time-lane oracle (the challenges so far are already in the constants
+// block: launch_fold stores each one); false when the support is empty and the round is zero
+bool TsOracleGpu::time_round_launch() {
   const int ncoef = (int)degree_bound_ + 1;
-  c.assign(ncoef, kzero());
-  if (n_ == 0) return;
-  // chals into the constants block
-  if (!chals_.empty())
-    NS_CHECK(hipMemcpyAsync(consts_.as<K2>() + ell_n_, chals_.data(), chals_.size() * sizeof(K2),
-                            hipMemcpyHostToDevice, st_));
+  if (n_ == 0) return false;
   TsTimeArgs A;
   A.kind = kind_;
   A.n = (uint32_t)n_;
@@ -801,11 +926,12 @@ void TsOracleGpu::time_round(std::vector<K2>& c) {
   else
     hipLaunchKernelGGL(k_ts_time_round<TS_MAX_COEF>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
   hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_dev_.as<K2>());
-  NS_CHECK(hipMemcpyAsync(c.data(), coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
-  NS_CHECK(hipStreamSynchronize(st_));
+  NS_CHECK(hipMemcpyAsync(coef_h_, coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  return true;
 }
 
-void TsOracleGpu::time_fold(K2 r) {
+// enqueue a time-lane fold; the folded support size arrives in the mailbox (finish_fold)
+void TsOracleGpu::time_fold_launch(K2 r) {
   if (n_ > 0) {
     const uint32_t n = (uint32_t)n_;
     const uint32_t stride = (uint32_t)cap_;
@@ -819,10 +945,8 @@ void TsOracleGpu::time_fold(K2 r) {
     hipLaunchKernelGGL(k_ts_fold, dim3(g), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(), cols_[cur_].as<K2>(), n,
                        stride, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(), r, idx_[cur_ ^ 1].as<uint32_t>(),
                        cols_[cur_ ^ 1].as<K2>(), nout);
-    uint32_t nn = 0;
-    NS_CHECK(hipMemcpyAsync(&nn, nout, 4, hipMemcpyDeviceToHost, st_));
-    NS_CHECK(hipStreamSynchronize(st_));
-    n_ = nn;
+    NS_CHECK(hipMemcpyAsync(coef_h_ + TS_MAX_COEF, nout, 4, hipMemcpyDeviceToHost, st_));
+    fold_pending_ = true;
     cur_ ^= 1;
   }
   len_ >>= 1;
@@ -864,8 +988,6 @@ K2 TsOracleGpu::time_final() {
     case TSK_TWIST_WRITE: {
       K2 vp = init_at_;
       if (nterms_) {
-        NS_CHECK(hipMemcpyAsync(consts_.as<K2>() + ell_n_, chals_.data(), chals_.size() * sizeof(K2),
-                                hipMemcpyHostToDevice, st_));
         unsigned nb = nblocks(nterms_);
         hipLaunchKernelGGL(k_ts_valpre_full, dim3(nb), dim3(TB), 0, st_, term_t_.as<uint32_t>(), term_v_.as<K2>(),
                            (uint32_t)nterms_, consts_.as<K2>() + ell_n_, (int)ell_n_, partials_.as<K2>());
@@ -893,9 +1015,8 @@ K2 TsOracleGpu::time_final() {
   }
 }
 
-void TsOracleGpu::product_round(std::vector<K2>& c) {
+bool TsOracleGpu::product_round_launch() {
   const int ncoef = (int)nf_ + 1;
-  c.assign(ncoef, kzero());
   size_t half = len_ / 2;
   unsigned nb = nblocks(half);
   partials_.ensure((size_t)TMAXBLK * TS_MAX_COEF * sizeof(K2));
@@ -907,8 +1028,8 @@ void TsOracleGpu::product_round(std::vector<K2>& c) {
     hipLaunchKernelGGL(k_prod_round<TS_MAX_COEF>, dim3(nb), dim3(TB), 0, st_, fac_[cur_].as<K2>(), (int)nf_, flen_, half,
                        partials_.as<K2>());
   hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_dev_.as<K2>());
-  NS_CHECK(hipMemcpyAsync(c.data(), coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
-  NS_CHECK(hipStreamSynchronize(st_));
+  NS_CHECK(hipMemcpyAsync(coef_h_, coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  return true;
 }
 
 void TsOracleGpu::init_lookup(const std::vector<SparseCol>& addr_bits, const SparseCol& has_lookup,
@@ -945,7 +1066,8 @@ void TsOracleGpu::init_lookup(const std::vector<SparseCol>& addr_bits, const Spa
   if (n == 0) {
     NS_CHECK(hipMemsetAsync(W, 0, pow2_addr * sizeof(K2), st_));
   } else {
-    DevBuf dts, dg, dw, da, dv, sa, sv, rc, tmp;
+    TsBuf &dts = scratch_[0], &dg = scratch_[1], &dw = scratch_[2], &da = scratch_[3], &dv = scratch_[4],
+          &sa = scratch_[5], &sv = scratch_[6], &rc = scratch_[7], &tmp = scratch_[8];
     upload(dts, ts, st_);
     upload(dg, gates, st_);
     upload(da, addr, st_);
@@ -1042,7 +1164,7 @@ void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs
   upload(ev_rw_, one, st_, 1);
   upload(ev_ww_, one, st_, 1);
   ev_eq_.ensure(ne * sizeof(K2));
-  DevBuf dts, rc;
+  TsBuf &dts = scratch_[0], &rc = scratch_[1];
   upload(dts, times, st_, 1);
   std::vector<K2> rcy(r_cycle);
   rcy.resize(std::max<size_t>(ell_n, 1), kzero());
@@ -1080,9 +1202,8 @@ void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs
   NS_CHECK(hipStreamSynchronize(st_));
 }
 
-void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
-  c.assign(3, kzero());
-  if (nev_ == 0) return;
+bool TsOracleGpu::addr_round_launch(bool final) {
+  if (nev_ == 0) return false;
   const uint32_t n = (uint32_t)nev_;
   const int bit = final ? -1 : (int)bit_idx_;
   const uint32_t sentinel = final ? 1u : (uint32_t)((ninit_tab_ >> bit_idx_));
@@ -1118,26 +1239,59 @@ void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
   unsigned nb = nblocks(n);
   hipLaunchKernelGGL(k_addr_round, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
   hipLaunchKernelGGL(k_ts_sum, dim3(3), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, 3, coef_dev_.as<K2>());
-  NS_CHECK(hipMemcpyAsync(c.data(), coef_dev_.p, 3 * sizeof(K2), hipMemcpyDeviceToHost, st_));
-  NS_CHECK(hipStreamSynchronize(st_));
+  NS_CHECK(hipMemcpyAsync(coef_h_, coef_dev_.p, 3 * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  return true;
 }
 
-void TsOracleGpu::round_coeffs(std::vector<K2>& c) {
+void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
+  finish_fold();
+  round_ncoef_ = 3;
+  round_dev_ = addr_round_launch(final);
+  finish_round(c);
+}
+
+void TsOracleGpu::launch_round() {
+  finish_fold();
   switch (kind_) {
     case TSK_PRODUCT:
     case TSK_ADDR_LOOKUP:
-      product_round(c);
+      round_ncoef_ = (int)nf_ + 1;
+      round_dev_ = product_round_launch();
       return;
     case TSK_READ_ADDR:
     case TSK_WRITE_ADDR:
-      addr_round(c, false);
+      round_ncoef_ = 3;
+      round_dev_ = addr_round_launch(false);
       return;
     default:
-      time_round(c);
+      round_ncoef_ = (int)degree_bound_ + 1;
+      round_dev_ = time_round_launch();
   }
 }
 
+void TsOracleGpu::finish_round(std::vector<K2>& c) {
+  c.assign(round_ncoef_, kzero());
+  if (round_dev_) {
+    NS_CHECK(hipStreamSynchronize(st_));
+    std::copy(coef_h_, coef_h_ + round_ncoef_, c.begin());
+  }
+  round_dev_ = false;
+}
+
+void TsOracleGpu::round_coeffs(std::vector<K2>& c) {
+  launch_round();
+  finish_round(c);
+}
+
+void TsOracleGpu::finish_fold() {
+  if (!fold_pending_) return;
+  NS_CHECK(hipStreamSynchronize(st_));
+  n_ = *(const uint32_t*)(coef_h_ + TS_MAX_COEF);
+  fold_pending_ = false;
+}
+
 void TsOracleGpu::evals(const K2* xs, size_t n, K2* out) {
+  finish_fold();
   if (num_rounds() == 0) {
     K2 v = kzero();
     if (kind_ == TSK_PRODUCT || kind_ == TSK_ADDR_LOOKUP) {
@@ -1176,6 +1330,14 @@ K2 TsOracleGpu::claim() {
 }
 
 void TsOracleGpu::fold(K2 r) {
+  launch_fold(r);
+  finish_fold();
+}
+
+// every fold is stream-ordered before the next round's kernels; only a time-lane fold leaves
+// host state (the support size) to collect
+void TsOracleGpu::launch_fold(K2 r) {
+  finish_fold();
   if (num_rounds() == 0) return;
   switch (kind_) {
     case TSK_PRODUCT:
@@ -1183,7 +1345,6 @@ void TsOracleGpu::fold(K2 r) {
       size_t half = len_ / 2;
       hipLaunchKernelGGL(k_prod_fold, dim3((unsigned)((half + 255) / 256), (unsigned)nf_), dim3(256), 0, st_,
                          fac_[cur_].as<K2>(), fac_[cur_ ^ 1].as<K2>(), (int)nf_, flen_, half, r);
-      NS_CHECK(hipStreamSynchronize(st_));
       cur_ ^= 1;
       len_ = half;
       chals_.push_back(r);
@@ -1203,13 +1364,12 @@ void TsOracleGpu::fold(K2 r) {
       size_t half = (ninit_tab_ >> bit_idx_) / 2;
       hipLaunchKernelGGL(k_prod_fold, dim3((unsigned)((half + 255) / 256), 1), dim3(256), 0, st_, it_[cur_].as<K2>(),
                          it_[cur_ ^ 1].as<K2>(), 1, (size_t)0, half, r);
-      NS_CHECK(hipStreamSynchronize(st_));
       cur_ ^= 1;
       bit_idx_++;
       return;
     }
     case TSK_TOTAL_INC:
-      time_fold(r);
+      time_fold_launch(r);
       return;
     default:
       break;
@@ -1218,9 +1378,31 @@ void TsOracleGpu::fold(K2 r) {
     K2 rc = r_cycle_[bit_idx_];
     prefix_eq_ = kmul(prefix_eq_, kadd(kmul(ksub(kone(), r), ksub(kone(), rc)), kmul(r, rc)));
   }
-  time_fold(r);
+  time_fold_launch(r);
+  // constants block: r_cycle (ell_n) | challenges (ell_n slots); at most ell_n folds
+  hipLaunchKernelGGL(k_put_k2, dim3(1), dim3(64), 0, st_, consts_.as<K2>() + ell_n_ + chals_.size(), r);
   chals_.push_back(r);
   bit_idx_++;
+}
+
+void ts_batch_round(TsOracleGpu* const* os, size_t n, std::vector<std::vector<K2>>& coeffs) {
+  if (n == 0) throw Error(1, "batched round: no oracles");
+  for (size_t i = 0; i < n; i++)
+    if (!os[i]) throw Error(1, "batched round: null oracle");
+  const size_t rounds = os[0]->num_rounds();
+  for (size_t i = 0; i < n; i++)
+    if (os[i]->num_rounds() != rounds) throw Error(1, "batched round: every oracle must have the same number of rounds");
+  if (rounds == 0) throw Error(1, "batched round: no rounds left");
+  coeffs.resize(n);
+  for (size_t i = 0; i < n; i++) os[i]->launch_round();
+  for (size_t i = 0; i < n; i++) os[i]->finish_round(coeffs[i]);
+}
+
+void ts_batch_fold(TsOracleGpu* const* os, size_t n, K2 r) {
+  for (size_t i = 0; i < n; i++)
+    if (!os[i]) throw Error(1, "batched fold: null oracle");
+  for (size_t i = 0; i < n; i++) os[i]->launch_fold(r);
+  for (size_t i = 0; i < n; i++) os[i]->finish_fold();
 }
 
 // ---------------- shared CPU bus (host) ----------------

@@ -3,7 +3,8 @@
 Each TsOracle implements the RoundOracle contract of neo-reductions/src/sumcheck.rs:17-34
 (evals / fold / num_rounds / degree_bound).  Sparse time columns are passed as
 (indices, [K values]) pairs, the SparseIdxVec of neo-memory/src/sparse_time.rs; K values are
-(c0, c1) tuples of canonical Goldilocks integers.  The constructors take the arguments of the
+(c0, c1) tuples of canonical Goldilocks integers or, passed without conversion, an (n, 2) uint64
+array.  The constructors take the arguments of the
 reference constructors (neo-memory/src/twist_oracle.rs) in the same order.
 """
 from __future__ import annotations
@@ -20,6 +21,10 @@ class SparseCol(C.Structure):
 
 
 def _kbuf(ks):
+    """K values as a flat uint64 buffer [c0, c1, ...]: an (n, 2) uint64 array is used as it is,
+    any other sequence of (c0, c1) pairs is converted element by element."""
+    if isinstance(ks, np.ndarray) and ks.dtype == np.uint64 and ks.ndim == 2 and ks.shape[1] == 2:
+        return np.ascontiguousarray(ks).reshape(-1) if len(ks) else np.zeros(2, np.uint64)
     return np.array([v for k in ks for v in (int(k[0]), int(k[1]))] or [0, 0], dtype=np.uint64)
 
 
@@ -68,6 +73,8 @@ def _bind():
     L.ns_ts_addr_lookup_new.argtypes = [vp, sz, S, vp, sz, vp, sz, vp, out]
     L.ns_ts_evals.argtypes = [vp, vp, sz, vp]
     L.ns_ts_fold.argtypes = [vp, vp]
+    L.ns_ts_batch_round.argtypes = [vp, sz, vp]
+    L.ns_ts_batch_fold.argtypes = [vp, sz, vp]
     L.ns_ts_num_rounds.argtypes = [vp]
     L.ns_ts_num_rounds.restype = sz
     L.ns_ts_degree_bound.argtypes = [vp]
@@ -170,6 +177,30 @@ class TsOracle:
                 _bind().ns_ts_free(self.h)
         except Exception:
             pass
+
+
+def _handles(oracles):
+    return (C.c_void_p * max(1, len(oracles)))(*[o.h.value for o in oracles])
+
+
+def batch_round(oracles):
+    """One round of the batched sum-check (neo-reductions/src/sumcheck.rs:390-407): for every
+    oracle, its round polynomial's values at 0..degree_bound; all rounds are enqueued before
+    the host waits.  Every oracle must have the same number of rounds left."""
+    degs = [o.degree_bound() for o in oracles]
+    out = np.zeros(2 * max(1, sum(d + 1 for d in degs)), np.uint64)
+    _check(_bind().ns_ts_batch_round(_handles(oracles), len(oracles), out.ctypes.data))
+    o, res, w = [int(v) for v in out], [], 0
+    for d in degs:
+        res.append([(o[2 * (w + i)], o[2 * (w + i) + 1]) for i in range(d + 1)])
+        w += d + 1
+    return res
+
+
+def batch_fold(oracles, r):
+    """Fold every oracle with the shared challenge r (sumcheck.rs:436-442)."""
+    rb = np.array([int(r[0]), int(r[1])], np.uint64)
+    _check(_bind().ns_ts_batch_fold(_handles(oracles), len(oracles), rb.ctypes.data))
 
 
 # ---------------- shared CPU bus (neo-fold/src/memory_sidecar/cpu_bus.rs) ----------------

@@ -117,13 +117,27 @@ def _kconst(v):
     return (int(v) % Q, 0)
 
 
+def kvals(vs) -> np.ndarray:
+    """Base-field values as K = F_q^2 elements: an (n, 2) uint64 array (c0 = v mod q, c1 = 0)."""
+    out = np.zeros((len(vs), 2), np.uint64)
+    if len(vs):
+        out[:, 0] = np.array([int(v) % Q for v in vs], dtype=np.uint64)
+    return out
+
+
+def kones(n: int) -> np.ndarray:
+    out = np.zeros((n, 2), np.uint64)
+    out[:, 0] = 1
+    return out
+
+
 def sparse_bits(addrs, active, ell_addr):
     """Bit columns of addresses at the active times: column b holds ONE where bit b is set."""
     cols = []
     ts = np.nonzero(active)[0]
     for b in range(ell_addr):
         sel = ts[((addrs[ts] >> b) & 1) == 1]
-        cols.append((sel.astype(np.uint64), [(1, 0)] * len(sel)))
+        cols.append((sel.astype(np.uint64), kones(len(sel))))
     return cols
 
 
@@ -161,15 +175,14 @@ def gen_memory_trace(ell_n: int, ell_addr: int, p_access: float, write_frac: flo
             mem[a] = v
     tr = np.nonzero(is_r)[0].astype(np.uint64)
     tw = np.nonzero(is_w)[0].astype(np.uint64)
-    one = [(1, 0)]
     return dict(
         ell_n=ell_n, ell_addr=ell_addr,
-        has_read=(tr, one * len(tr)), rv=(tr[np.array(rv, dtype=np.int64) != 0] if len(tr) else tr,
-                                          [_kconst(v) for v in rv if v != 0]),
+        has_read=(tr, kones(len(tr))), rv=(tr[np.array(rv, dtype=np.int64) != 0] if len(tr) else tr,
+                                           kvals([v for v in rv if v != 0])),
         ra_bits=sparse_bits(addrs, is_r, ell_addr),
-        has_write=(tw, one * len(tw)), wv=(tw[np.array(wv, dtype=np.int64) != 0] if len(tw) else tw,
-                                           [_kconst(v) for v in wv if v != 0]),
-        inc=(tw[np.array(inc, dtype=object) != 0] if len(tw) else tw, [(v, 0) for v in inc if v != 0]),
+        has_write=(tw, kones(len(tw))), wv=(tw[np.array(wv, dtype=np.int64) != 0] if len(tw) else tw,
+                                            kvals([v for v in wv if v != 0])),
+        inc=(tw[np.array(inc, dtype=object) != 0] if len(tw) else tw, kvals([v for v in inc if v != 0])),
         wa_bits=sparse_bits(addrs, is_w, ell_addr),
         init=sorted((a, (v % Q, 0)) for a, v in init.items()),
     )
@@ -195,7 +208,6 @@ def gen_lookup_trace(ell_n: int, seed: int):
         ts = np.nonzero(act)[0].astype(np.uint64)
         vals = tables[o][addr[act]]
         nz = vals != 0
-        lanes.append(dict(addr_bits=sparse_bits(addr, act, 16), has_lookup=(ts, [(1, 0)] * len(ts)),
-                          val=(ts[nz], [(int(v), 0) for v in vals[nz]]),
-                          table=[(int(v), 0) for v in tables[o]]))
+        lanes.append(dict(addr_bits=sparse_bits(addr, act, 16), has_lookup=(ts, kones(len(ts))),
+                          val=(ts[nz], kvals(vals[nz])), table=kvals(tables[o])))
     return lanes

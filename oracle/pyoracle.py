@@ -286,8 +286,11 @@ class SparseIn(C.Structure):
 
 
 def _kbuf(ks):
-    a = np.array([v for k in ks for v in (int(k[0]), int(k[1]))] or [0, 0], dtype=np.uint64)
-    return a
+    """K values as a flat uint64 buffer [c0, c1, ...]: an (n, 2) uint64 array is used as it is,
+    any other sequence of (c0, c1) pairs is converted element by element."""
+    if isinstance(ks, np.ndarray) and ks.dtype == np.uint64 and ks.ndim == 2 and ks.shape[1] == 2:
+        return np.ascontiguousarray(ks).reshape(-1) if len(ks) else np.zeros(2, np.uint64)
+    return np.array([v for k in ks for v in (int(k[0]), int(k[1]))] or [0, 0], dtype=np.uint64)
 
 
 class _Keep:

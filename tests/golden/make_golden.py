@@ -140,7 +140,7 @@ def twist_vectors(cls, tr, r_cycle, r_addr, init_at, rng=None, rs=None):
 def trace_sha(tr) -> np.ndarray:
     def canon(v):
         if isinstance(v, np.ndarray):
-            return [int(x) for x in v.reshape(-1)]
+            return canon(v.tolist())
         if isinstance(v, (list, tuple)):
             return [canon(x) for x in v]
         return int(v)

@@ -95,6 +95,61 @@ def test_shout_lanes(twist, oracle):
         drive(g, c, rng, "addr_lookup")
 
 
+def test_batched_sumcheck(twist, oracle):
+    """The batched driver (run_batched_sumcheck_prover's round loop, shared challenges): the time
+    lanes of a RAM Twist and a Shout lane (12 rounds each), then the address lanes of the same
+    Twist and the Shout table (16 rounds each), each group stepped with ns_ts_batch_round /
+    ns_ts_batch_fold on the GPU and one oracle at a time on the CPU; every value must be
+    identical, and so must the final values."""
+    ell_n = 12
+    tr = gen_memory_trace(ell_n, 16, 0.35, 0.4, seed=31, zipf_s=1.1)
+    ln = gen_lookup_trace(ell_n, seed=32)[1]
+    rng = np.random.default_rng(80)
+    rk = lambda n: [rand_k(rng) for _ in range(n)]  # noqa: E731
+    r_cycle, r_addr, init_at, r_time, w, r_addr_s = rk(ell_n), rk(16), rand_k(rng), rk(ell_n), rk(16), rk(16)
+    time_group = [
+        ("twist_read", r_cycle, tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["inc"], tr["wa_bits"],
+         r_addr, init_at),
+        ("twist_write", r_cycle, tr["has_write"], tr["wv"], tr["inc"], tr["wa_bits"], r_addr, init_at),
+        ("val_eval", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], r_addr, r_time),
+        ("total_inc", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], r_addr),
+        ("bitness", r_cycle, tr["ra_bits"], w),
+        ("shout_value", r_cycle, ln["has_lookup"], ln["val"]),
+        ("index_adapter", r_cycle, ln["has_lookup"], ln["addr_bits"], r_addr_s),
+    ]
+    addr_group = [
+        ("read_addr", tr["init"], r_cycle, tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["wa_bits"],
+         tr["inc"]),
+        ("write_addr", tr["init"], r_cycle, tr["has_write"], tr["wv"], tr["wa_bits"], tr["inc"]),
+        ("addr_lookup", ln["addr_bits"], ln["has_lookup"], ln["table"], r_cycle),
+    ]
+    for name, group in (("time lanes", time_group), ("address lanes", addr_group)):
+        print(f"batched {name}: {len(group)} oracles", flush=True)
+        gs = [twist.TsOracle(*a) for a in group]
+        cs = [oracle.TsOracle(*a) for a in group]
+        rounds = cs[0].num_rounds()
+        assert all(g.num_rounds() == c.num_rounds() == rounds for g, c in zip(gs, cs))
+        for rd in range(rounds):
+            got = twist.batch_round(gs)
+            for i, c in enumerate(cs):
+                want = c.evals([(v, 0) for v in range(c.degree_bound() + 1)])
+                assert got[i] == want, f"{name}: {group[i][0]} round {rd}"
+            r = rand_k(rng)
+            twist.batch_fold(gs, r)
+            for c in cs:
+                c.fold(r)
+        x = [rand_k(rng)]
+        for i, (g, c) in enumerate(zip(gs, cs)):
+            assert g.num_rounds() == 0
+            assert g.evals(x) == c.evals(x), f"{name}: {group[i][0]} final value"
+        with pytest.raises(twist.NsError):
+            twist.batch_round(gs)  # no rounds left
+    # oracles with different numbers of rounds cannot share a batch (sumcheck.rs:357-368)
+    mixed = [twist.TsOracle(*time_group[5]), twist.TsOracle(*addr_group[2])]
+    with pytest.raises(twist.NsError):
+        twist.batch_round(mixed)
+
+
 def test_product_oracle(twist, oracle):
     rng = np.random.default_rng(70)
     facs = [[rand_k(rng) for _ in range(1 << 10)] for _ in range(3)]
