@@ -103,6 +103,7 @@ class TsOracleGpu {
   K2* coef_h_ = nullptr;      // pinned: TS_MAX_COEF coefficients, then the folded support size
   int round_ncoef_ = 0;      // coefficients of the launched round
   bool round_dev_ = false;   // the launched round copies them from the device
+  bool round_points_ = false;  // ... as values at 0 .. round_ncoef_-1 rather than coefficients
   bool fold_pending_ = false;  // a launched time-lane fold has not reported its support size
   size_t degree_bound_ = 0, ell_n_ = 0, ell_addr_ = 0, bit_idx_ = 0;
   K2 prefix_eq_ = K2{1, 0};

@@ -178,114 +178,170 @@ __device__ K2 lt_child(const TsTimeArgs& A, uint64_t pair, int c, const K2* r_ti
   return acc;
 }
 
-template <int MAXC>
-__global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restrict__ partials) {
+// pair of union-support entry i and its children's positions (0xFFFFFFFF where absent); false
+// for the odd sibling of a present even child (the even one owns the pair)
+__device__ __forceinline__ bool ts_pair_at(const TsTimeArgs& A, uint32_t i, uint64_t& pair, uint32_t& i0,
+                                           uint32_t& i1) {
+  const uint32_t id = A.idx[i];
+  pair = id >> 1;
+  if (i > 0 && (A.idx[i - 1] >> 1) == pair) return false;
+  i0 = i1 = 0xFFFFFFFFu;
+  if ((id & 1) == 0) {
+    i0 = i;
+    if (i + 1 < A.n && A.idx[i + 1] == id + 1) i1 = i + 1;
+  } else {
+    i1 = i;
+  }
+  return true;
+}
+
+__device__ __forceinline__ K2 ts_col(const TsTimeArgs& A, int c, uint32_t which) {
+  return which != 0xFFFFFFFFu ? A.cols[(size_t)c * A.stride + which] : kzero();
+}
+
+// chi_cycle_children (twist_oracle.rs:185-198)
+__device__ __forceinline__ void ts_chi(const TsTimeArgs& A, uint64_t pair, K2& chi0, K2& chi1) {
+  const K2* r_cycle = A.consts;
+  K2 suffix = kone();
+  int shift = 0;
+  for (int b = A.bit_idx + 1; b < A.ell; b++, shift++)
+    suffix = kmul(suffix, ((pair >> shift) & 1) ? r_cycle[b] : ksub(kone(), r_cycle[b]));
+  const K2 r = r_cycle[A.bit_idx];
+  chi0 = kmul(kmul(A.prefix_eq, ksub(kone(), r)), suffix);
+  chi1 = kmul(kmul(A.prefix_eq, r), suffix);
+}
+
+// The summand of entry i's pair, for every kind but bitness, as a product of factors linear in
+// the round variable X: fac(value at X = 0, slope) once per factor.  false when another entry
+// owns the pair or the gate is zero on both children (the summand is zero).
+template <class F>
+__device__ __forceinline__ bool ts_pair_factors(const TsTimeArgs& A, uint32_t i, F&& fac) {
   const K2* r_cycle = A.consts;
   const K2* chals = A.consts + A.ell;
   const K2* r_addr = A.consts + 2 * A.ell;
-  const K2* weights = r_addr + A.nbits;
+  uint64_t pair;
+  uint32_t i0, i1;
+  if (!ts_pair_at(A, i, pair, i0, i1)) return false;
+  const K2 g0 = ts_col(A, 0, i0), g1 = ts_col(A, 0, i1);
+  if (kiszero(g0) && kiszero(g1)) return false;
+  fac(g0, ksub(g1, g0));
+  if (A.kind != TSK_VAL_EVAL && A.kind != TSK_TOTAL_INC) {
+    K2 chi0, chi1;
+    ts_chi(A, pair, chi0, chi1);
+    fac(chi0, ksub(chi1, chi0));
+  }
+  K2 f0 = kzero(), f1 = kzero();
+  bool has_f = true;
+  switch (A.kind) {
+    case TSK_SHOUT_VALUE:
+      f0 = ts_col(A, A.col_a, i0);
+      f1 = ts_col(A, A.col_a, i1);
+      break;
+    case TSK_INDEX_ADAPTER:
+      has_f = false;
+      break;
+    case TSK_TWIST_READ: {
+      const K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
+      f0 = ksub(vp0, ts_col(A, A.col_a, i0));
+      f1 = ksub(vp1, ts_col(A, A.col_a, i1));
+      break;
+    }
+    case TSK_TWIST_WRITE: {
+      const K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
+      f0 = ksub(ksub(ts_col(A, A.col_a, i0), vp0), ts_col(A, A.col_b, i0));
+      f1 = ksub(ksub(ts_col(A, A.col_a, i1), vp1), ts_col(A, A.col_b, i1));
+      break;
+    }
+    case TSK_VAL_EVAL: {
+      f0 = ts_col(A, A.col_b, i0);
+      f1 = ts_col(A, A.col_b, i1);
+      const K2 h0 = lt_child(A, pair, 0, r_cycle, chals), h1 = lt_child(A, pair, 1, r_cycle, chals);
+      fac(h0, ksub(h1, h0));
+      break;
+    }
+    case TSK_TOTAL_INC:
+      f0 = ts_col(A, A.col_b, i0);
+      f1 = ts_col(A, A.col_b, i1);
+      break;
+    default:
+      break;
+  }
+  if (has_f) fac(f0, ksub(f1, f0));
+  for (int b = 0; b < A.nbits; b++) {
+    const K2 e0 = eq_bit_affine(ts_col(A, A.bit0 + b, i0), r_addr[b]);
+    const K2 e1 = eq_bit_affine(ts_col(A, A.bit0 + b, i1), r_addr[b]);
+    fac(e0, ksub(e1, e0));
+  }
+  return true;
+}
+
+// running product of linear factors as a coefficient vector (degree < MAXC)
+template <int MAXC>
+struct LinProd {
+  K2 p[MAXC];
+  int deg;
+  __device__ __forceinline__ void operator()(K2 a, K2 b) { poly_mul_lin<MAXC>(p, deg, a, b); }
+};
+
+// One time-lane round as coefficients (degree < MAXC): every thread multiplies out its pairs'
+// products; bitness, whose summand is not a product of linear factors, runs only here.
+template <int MAXC>
+__global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restrict__ partials) {
+  const K2* weights = A.consts + 2 * A.ell + A.nbits;
   K2 acc[MAXC];
 #pragma unroll
   for (int d = 0; d < MAXC; d++) acc[d] = kzero();
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
-    const uint32_t id = A.idx[i];
-    const uint64_t pair = id >> 1;
-    if (i > 0 && (A.idx[i - 1] >> 1) == pair) continue;  // the even sibling owns the pair
-    uint32_t i0 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu;
-    if ((id & 1) == 0) {
-      i0 = i;
-      if (i + 1 < A.n && A.idx[i + 1] == id + 1) i1 = i + 1;
-    } else {
-      i1 = i;
-    }
-#define COLV(c, which) ((which) != 0xFFFFFFFFu ? A.cols[(size_t)(c) * A.stride + (which)] : kzero())
-    // chi_cycle_children (twist_oracle.rs:185-198)
-    K2 chi0 = kone(), chi1 = kone();
-    const bool has_chi = A.kind != TSK_VAL_EVAL && A.kind != TSK_TOTAL_INC;
-    if (has_chi) {
-      K2 suffix = kone();
-      int shift = 0;
-      for (int b = A.bit_idx + 1; b < A.ell; b++, shift++)
-        suffix = kmul(suffix, ((pair >> shift) & 1) ? r_cycle[b] : ksub(kone(), r_cycle[b]));
-      K2 r = r_cycle[A.bit_idx];
-      chi0 = kmul(kmul(A.prefix_eq, ksub(kone(), r)), suffix);
-      chi1 = kmul(kmul(A.prefix_eq, r), suffix);
-    }
-    K2 p[MAXC];
+    LinProd<MAXC> lp;
+    K2(&p)[MAXC] = lp.p;
 #pragma unroll
     for (int d = 0; d < MAXC; d++) p[d] = kzero();
     p[0] = kone();
-    int deg = 0;
+    lp.deg = 0;
     if (A.kind == TSK_BITNESS) {
       // chi(x) * sum_c w_c b_c(x) (b_c(x) - 1), b_c(x) = b0 + db x
+      uint64_t pair;
+      uint32_t i0, i1;
+      if (!ts_pair_at(A, i, pair, i0, i1)) continue;
+      K2 chi0, chi1;
+      ts_chi(A, pair, chi0, chi1);
       K2 q0 = kzero(), q1 = kzero(), q2 = kzero();
       for (int c = 0; c < A.nbits; c++) {
-        K2 b0 = COLV(A.bit0 + c, i0), b1 = COLV(A.bit0 + c, i1);
+        const K2 b0 = ts_col(A, A.bit0 + c, i0), b1 = ts_col(A, A.bit0 + c, i1);
         if (kiszero(b0) && kiszero(b1)) continue;
-        K2 db = ksub(b1, b0), w = weights[c];
-        K2 b0m1 = ksub(b0, kone());
+        const K2 db = ksub(b1, b0), w = weights[c];
+        const K2 b0m1 = ksub(b0, kone());
         q0 = kadd(q0, kmul(w, kmul(b0, b0m1)));
         q1 = kadd(q1, kmul(w, kadd(kmul(b0, db), kmul(db, b0m1))));
         q2 = kadd(q2, kmul(w, kmul(db, db)));
       }
-      K2 dc = ksub(chi1, chi0);
+      const K2 dc = ksub(chi1, chi0);
       p[0] = kmul(chi0, q0);
       p[1] = kadd(kmul(chi0, q1), kmul(dc, q0));
       p[2] = kadd(kmul(chi0, q2), kmul(dc, q1));
       p[3] = kmul(dc, q2);
-    } else {
-      K2 g0 = COLV(0, i0), g1 = COLV(0, i1);
-      if (kiszero(g0) && kiszero(g1)) continue;
-      poly_mul_lin<MAXC>(p, deg, g0, ksub(g1, g0));
-      if (has_chi) poly_mul_lin<MAXC>(p, deg, chi0, ksub(chi1, chi0));
-      K2 f0 = kzero(), f1 = kzero();
-      bool has_f = true;
-      switch (A.kind) {
-        case TSK_SHOUT_VALUE:
-          f0 = COLV(A.col_a, i0);
-          f1 = COLV(A.col_a, i1);
-          break;
-        case TSK_INDEX_ADAPTER:
-          has_f = false;
-          break;
-        case TSK_TWIST_READ: {
-          K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
-          f0 = ksub(vp0, COLV(A.col_a, i0));
-          f1 = ksub(vp1, COLV(A.col_a, i1));
-          break;
-        }
-        case TSK_TWIST_WRITE: {
-          K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
-          f0 = ksub(ksub(COLV(A.col_a, i0), vp0), COLV(A.col_b, i0));
-          f1 = ksub(ksub(COLV(A.col_a, i1), vp1), COLV(A.col_b, i1));
-          break;
-        }
-        case TSK_VAL_EVAL: {
-          f0 = COLV(A.col_b, i0);
-          f1 = COLV(A.col_b, i1);
-          K2 h0 = lt_child(A, pair, 0, r_cycle, chals), h1 = lt_child(A, pair, 1, r_cycle, chals);
-          poly_mul_lin<MAXC>(p, deg, h0, ksub(h1, h0));
-          break;
-        }
-        case TSK_TOTAL_INC:
-          f0 = COLV(A.col_b, i0);
-          f1 = COLV(A.col_b, i1);
-          break;
-        default:
-          break;
-      }
-      if (has_f) poly_mul_lin<MAXC>(p, deg, f0, ksub(f1, f0));
-      for (int b = 0; b < A.nbits; b++) {
-        K2 e0 = eq_bit_affine(COLV(A.bit0 + b, i0), r_addr[b]);
-        K2 e1 = eq_bit_affine(COLV(A.bit0 + b, i1), r_addr[b]);
-        poly_mul_lin<MAXC>(p, deg, e0, ksub(e1, e0));
-      }
+    } else if (!ts_pair_factors(A, i, lp)) {
+      continue;
     }
-#undef COLV
 #pragma unroll
     for (int d = 0; d < MAXC; d++) acc[d] = kadd(acc[d], p[d]);
   }
   block_store<MAXC>(acc, A.ncoef, partials);
+}
+
+// One time-lane round by evaluation at X = blockIdx.y (0 .. ncoef-1): each one-wave block sums
+// its pairs' products of linear factors at X (one multiply-add per factor, no coefficient
+// arrays); partials[blockIdx.x * ncoef + X] receives the block's sum and the host interpolates
+// the ncoef sums to the round's coefficients.
+__global__ __launch_bounds__(64) void k_ts_time_points(TsTimeArgs A, K2* __restrict__ partials) {
+  const K2 X{(uint64_t)blockIdx.y, 0};
+  K2 acc = kzero();
+  for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < A.n; i += gridDim.x * 64) {
+    K2 prod = kone();
+    if (ts_pair_factors(A, i, [&](K2 a, K2 b) { prod = kmul(prod, kadd(a, kmul(b, X))); })) acc = kadd(acc, prod);
+  }
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x * A.ncoef + blockIdx.y] = acc;
 }
 
 // one K value into device memory (a round challenge into an oracle's constants block)
@@ -916,15 +972,16 @@ bool TsOracleGpu::time_round_launch() {
     A.lt_s0 = lt_s_[0].as<K2>();
     A.lt_s1 = lt_s_[1].as<K2>();
   }
-  unsigned nb = nblocks(n_);
-  if (ncoef <= 8)
+  unsigned nb;
+  round_points_ = ncoef > 8;
+  if (!round_points_) {
+    nb = nblocks(n_);
     hipLaunchKernelGGL(k_ts_time_round<8>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
-  else if (ncoef <= 16)
-    hipLaunchKernelGGL(k_ts_time_round<16>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
-  else if (ncoef <= 24)
-    hipLaunchKernelGGL(k_ts_time_round<24>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
-  else
-    hipLaunchKernelGGL(k_ts_time_round<TS_MAX_COEF>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
+  } else {
+    if (kind_ == TSK_BITNESS) throw Error(1, "bitness round: degree above 7");
+    nb = (unsigned)std::min<size_t>((n_ + 63) / 64, TMAXBLK);
+    hipLaunchKernelGGL(k_ts_time_points, dim3(nb, ncoef), dim3(64), 0, st_, A, partials_.as<K2>());
+  }
   hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_dev_.as<K2>());
   NS_CHECK(hipMemcpyAsync(coef_h_, coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
   return true;
@@ -1245,6 +1302,7 @@ bool TsOracleGpu::addr_round_launch(bool final) {
 
 void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
   finish_fold();
+  round_points_ = false;
   round_ncoef_ = 3;
   round_dev_ = addr_round_launch(final);
   finish_round(c);
@@ -1252,6 +1310,7 @@ void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
 
 void TsOracleGpu::launch_round() {
   finish_fold();
+  round_points_ = false;
   switch (kind_) {
     case TSK_PRODUCT:
     case TSK_ADDR_LOOKUP:
@@ -1269,13 +1328,46 @@ void TsOracleGpu::launch_round() {
   }
 }
 
+// Coefficients of the polynomial of degree < n through (x, y[x]), x = 0 .. n-1: Newton forward
+// differences over the falling factorials x (x-1) ... (x-k+1), expanded to monomials.  Exact in
+// the field, so it returns the coefficient vector itself for any polynomial of degree < n.
+static void coeffs_from_points(const K2* y, int n, K2* c) {
+  static const std::vector<uint64_t> inv_fact = [] {
+    std::vector<uint64_t> v(TS_MAX_COEF + 1);
+    uint64_t f = 1;
+    for (int k = 0; k <= TS_MAX_COEF; k++) {
+      if (k) f = fmul(f, (uint64_t)k);
+      v[k] = finv(f);
+    }
+    return v;
+  }();
+  std::vector<K2> d(y, y + n), B(n, kzero());
+  for (int k = 1; k < n; k++)
+    for (int j = n - 1; j >= k; j--) d[j] = ksub(d[j], d[j - 1]);
+  B[0] = kone();
+  for (int j = 0; j < n; j++) c[j] = kzero();
+  for (int k = 0; k < n; k++) {
+    if (k > 0) {
+      const uint64_t m = (uint64_t)(k - 1);  // B <- B * (x - m)
+      for (int j = k; j >= 1; j--) B[j] = ksub(B[j - 1], K2{fmul(B[j].c0, m), fmul(B[j].c1, m)});
+      B[0] = ksub(kzero(), K2{fmul(B[0].c0, m), fmul(B[0].c1, m)});
+    }
+    const K2 w{fmul(d[k].c0, inv_fact[k]), fmul(d[k].c1, inv_fact[k])};
+    for (int j = 0; j <= k; j++) c[j] = kadd(c[j], kmul(w, B[j]));
+  }
+}
+
 void TsOracleGpu::finish_round(std::vector<K2>& c) {
   c.assign(round_ncoef_, kzero());
   if (round_dev_) {
     NS_CHECK(hipStreamSynchronize(st_));
-    std::copy(coef_h_, coef_h_ + round_ncoef_, c.begin());
+    if (round_points_)
+      coeffs_from_points(coef_h_, round_ncoef_, c.data());
+    else
+      std::copy(coef_h_, coef_h_ + round_ncoef_, c.begin());
   }
   round_dev_ = false;
+  round_points_ = false;
 }
 
 void TsOracleGpu::round_coeffs(std::vector<K2>& c) {
