@@ -105,6 +105,8 @@ class TsOracleGpu {
   bool round_dev_ = false;   // the launched round copies them from the device
   bool round_points_ = false;  // ... as values at 0 .. round_ncoef_-1 rather than coefficients
   bool fold_pending_ = false;  // a launched time-lane fold has not reported its support size
+  std::vector<K2> claim_round_;     // round-0 coefficients computed for the claim, reused once
+  bool claim_round_valid_ = false, round_from_claim_ = false;
   size_t degree_bound_ = 0, ell_n_ = 0, ell_addr_ = 0, bit_idx_ = 0;
   K2 prefix_eq_ = K2{1, 0};
   std::vector<K2> r_cycle_, r_addr_, chals_;

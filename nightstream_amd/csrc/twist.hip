@@ -736,6 +736,19 @@ void TsBuf::ensure(size_t n) {
   bytes = cap;
 }
 
+// value of a sparse column at times queried in nondecreasing order (a forward walk instead of a
+// binary search per query)
+struct ColCursor {
+  const SparseCol* c;
+  size_t k = 0;
+  explicit ColCursor(const SparseCol& col) : c(&col) {}
+  K2 at(uint32_t t) {
+    const auto& ix = c->idx;
+    while (k < ix.size() && ix[k] < t) k++;
+    return (k < ix.size() && ix[k] == t) ? c->val[k] : kzero();
+  }
+};
+
 template <class T>
 static void upload(TsBuf& d, const std::vector<T>& h, hipStream_t st, size_t min_elems = 1) {
   size_t n = std::max(h.size(), min_elems);
@@ -744,8 +757,6 @@ static void upload(TsBuf& d, const std::vector<T>& h, hipStream_t st, size_t min
 }
 
 TsOracleGpu::TsOracleGpu() {
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw HipError(hipErrorNoDevice, "no device", __FILE__, __LINE__);
   st_ = TsPool::get().stream();
   coef_h_ = TsPool::get().mailbox();
 }
@@ -811,24 +822,38 @@ void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t 
   }
   for (auto& c : bits) cs.push_back(&c);
   ncol_ = cs.size();
-  std::vector<uint32_t> u;
+  // union support: a byte map over time when the domain is not much larger than the entries,
+  // else sort + unique
+  size_t total = 0;
   for (auto* c : cs) {
     if (c->idx.size() != c->val.size()) throw Error(1, "sparse column: idx/val length mismatch");
-    for (auto t : c->idx) {
-      if (t >= len_) throw Error(1, "sparse column index out of range");
-      u.push_back(t);
+    for (size_t j = 0; j < c->idx.size(); j++) {
+      if (c->idx[j] >= len_) throw Error(1, "sparse column index out of range");
+      if (j && c->idx[j] <= c->idx[j - 1]) throw Error(1, "sparse column indices must be strictly increasing");
     }
+    total += c->idx.size();
   }
-  std::sort(u.begin(), u.end());
-  u.erase(std::unique(u.begin(), u.end()), u.end());
+  std::vector<uint32_t> u;
+  u.reserve(total);
+  if (len_ <= 16 * total + 4096) {
+    std::vector<uint8_t> mark(len_, 0);
+    for (auto* c : cs)
+      for (auto t : c->idx) mark[t] = 1;
+    for (size_t t = 0; t < len_; t++)
+      if (mark[t]) u.push_back((uint32_t)t);
+  } else {
+    for (auto* c : cs) u.insert(u.end(), c->idx.begin(), c->idx.end());
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+  }
   n_ = u.size();
   cap_ = std::max<size_t>(n_, 1);
   std::vector<K2> vals(ncol_ * cap_, kzero());
   for (size_t c = 0; c < ncol_; c++) {
     const SparseCol* col = cs[c];
+    size_t p = 0;  // every index of the column is in u, both ascending
     for (size_t j = 0; j < col->idx.size(); j++) {
-      if (j && col->idx[j] <= col->idx[j - 1]) throw Error(1, "sparse column indices must be strictly increasing");
-      size_t p = std::lower_bound(u.begin(), u.end(), col->idx[j]) - u.begin();
+      while (u[p] < col->idx[j]) p++;
       vals[c * cap_ + p] = col->val[j];
     }
   }
@@ -875,19 +900,18 @@ void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t 
     const SparseCol& ic = *term_inc;
     const std::vector<SparseCol>& wb = *term_bits;
     if (wb.size() != r_addr_.size()) throw Error(1, "twist: wa_bits length != r_addr length");
-    auto get = [](const SparseCol& c, uint32_t t) {
-      auto it = std::lower_bound(c.idx.begin(), c.idx.end(), t);
-      return (it != c.idx.end() && *it == t) ? c.val[it - c.idx.begin()] : kzero();
-    };
+    ColCursor icc(ic);
+    std::vector<ColCursor> wbc;
+    for (auto& c : wb) wbc.emplace_back(c);
     term_t_h_.clear();
     term_v_h_.clear();
     for (size_t i = 0; i < hw.idx.size(); i++) {
       uint32_t t = hw.idx[i];
-      K2 h = hw.val[i], it = get(ic, t);
+      K2 h = hw.val[i], it = icc.at(t);
       if (kiszero(h) || kiszero(it)) continue;
       K2 eq = kone();
       for (size_t bb = 0; bb < wb.size(); bb++) {
-        K2 bit = get(wb[bb], t), uu = r_addr_[bb];
+        K2 bit = wbc[bb].at(t), uu = r_addr_[bb];
         eq = kmul(eq, kadd(kmul(bit, ksub(kadd(uu, uu), kone())), ksub(kone(), uu)));
       }
       K2 v = kmul(kmul(h, it), eq);
@@ -1098,15 +1122,16 @@ void TsOracleGpu::init_lookup(const std::vector<SparseCol>& addr_bits, const Spa
   // lookups: nonzero gates; addresses from the bit columns (a bit counts when it equals ONE)
   std::vector<uint32_t> ts, addr;
   std::vector<K2> gates;
+  std::vector<ColCursor> bc;
+  for (auto& c : addr_bits) bc.emplace_back(c);
   for (size_t i = 0; i < has_lookup.idx.size(); i++) {
     if (has_lookup.idx[i] >= len) throw Error(1, "sparse column index out of range");
+    if (i && has_lookup.idx[i] <= has_lookup.idx[i - 1])
+      throw Error(1, "sparse column indices must be strictly increasing");
     if (kiszero(has_lookup.val[i])) continue;
     uint32_t t = has_lookup.idx[i], a = 0;
-    for (size_t b = 0; b < ell_addr; b++) {
-      auto& c = addr_bits[b];
-      auto it = std::lower_bound(c.idx.begin(), c.idx.end(), t);
-      if (it != c.idx.end() && *it == t && keq(c.val[it - c.idx.begin()], kone())) a |= 1u << b;
-    }
+    for (size_t b = 0; b < ell_addr; b++)
+      if (keq(bc[b].at(t), kone())) a |= 1u << b;
     ts.push_back(t);
     addr.push_back(a);
     gates.push_back(has_lookup.val[i]);
@@ -1168,22 +1193,37 @@ void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs
   ell_n_ = ell_n;
   const size_t len = (size_t)1 << ell_n, pow2_addr = (size_t)1 << ell_addr;
   const bool rd = kind == TSK_READ_ADDR;
-  auto get = [](const SparseCol& c, uint32_t t) {
-    auto it = std::lower_bound(c.idx.begin(), c.idx.end(), t);
-    return (it != c.idx.end() && *it == t) ? c.val[it - c.idx.begin()] : kzero();
-  };
-  auto addr_at = [&](const std::vector<SparseCol>& bits, uint32_t t) {
-    uint32_t a = 0;
-    for (size_t b = 0; b < bits.size(); b++)
-      if (keq(get(bits[b], t), kone())) a |= 1u << b;
-    return a;
-  };
-  auto nz_times = [len](const SparseCol& c) {
-    std::vector<uint32_t> v;
+  // every column is read at the event times in ascending order
+  auto check_col = [len](const SparseCol& c) {
     for (size_t i = 0; i < c.idx.size(); i++) {
       if (c.idx[i] >= len) throw Error(1, "sparse column index out of range");
-      if (!kiszero(c.val[i])) v.push_back(c.idx[i]);
+      if (i && c.idx[i] <= c.idx[i - 1]) throw Error(1, "sparse column indices must be strictly increasing");
     }
+  };
+  check_col(has_write);
+  check_col(inc);
+  check_col(wv_or_inc);
+  for (auto& c : wa_bits) check_col(c);
+  if (rd) {
+    check_col(*has_read);
+    check_col(*rv);
+    for (auto& c : *ra_bits) check_col(c);
+  }
+  auto cursors = [](const std::vector<SparseCol>& bits) {
+    std::vector<ColCursor> v;
+    for (auto& c : bits) v.emplace_back(c);
+    return v;
+  };
+  auto addr_at = [](std::vector<ColCursor>& bits, uint32_t t) {
+    uint32_t a = 0;
+    for (size_t b = 0; b < bits.size(); b++)
+      if (keq(bits[b].at(t), kone())) a |= 1u << b;
+    return a;
+  };
+  auto nz_times = [](const SparseCol& c) {
+    std::vector<uint32_t> v;
+    for (size_t i = 0; i < c.idx.size(); i++)
+      if (!kiszero(c.val[i])) v.push_back(c.idx[i]);
     return v;
   };
   std::vector<uint32_t> times = nz_times(has_write);
@@ -1196,19 +1236,27 @@ void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs
   nev_ = times.size();
   std::vector<K2> hr(nev_), rvv(nev_), hw(nev_), ic(nev_), one(nev_, kone());
   std::vector<uint32_t> ra(nev_, 0), wa(nev_, 0);
+  ColCursor hwc(has_write), icc(inc), wvc(wv_or_inc);
+  std::vector<ColCursor> wac = cursors(wa_bits), rac;
+  std::unique_ptr<ColCursor> hrc, rvc;
+  if (rd) {
+    rac = cursors(*ra_bits);
+    hrc = std::make_unique<ColCursor>(*has_read);
+    rvc = std::make_unique<ColCursor>(*rv);
+  }
   for (size_t e = 0; e < nev_; e++) {
     uint32_t t = times[e];
-    hw[e] = get(has_write, t);
-    ic[e] = get(inc, t);
+    hw[e] = hwc.at(t);
+    ic[e] = icc.at(t);
     if (rd) {
-      hr[e] = get(*has_read, t);
-      rvv[e] = get(*rv, t);
-      ra[e] = kiszero(hr[e]) ? 0 : addr_at(*ra_bits, t);
-      wa[e] = kiszero(hw[e]) ? 0 : addr_at(wa_bits, t);
+      hr[e] = hrc->at(t);
+      rvv[e] = rvc->at(t);
+      ra[e] = kiszero(hr[e]) ? 0 : addr_at(rac, t);
+      wa[e] = kiszero(hw[e]) ? 0 : addr_at(wac, t);
     } else {
       hr[e] = kzero();
-      rvv[e] = get(wv_or_inc, t);  // wv
-      wa[e] = addr_at(wa_bits, t);
+      rvv[e] = wvc.at(t);  // wv
+      wa[e] = addr_at(wac, t);
     }
   }
   const size_t ne = std::max<size_t>(nev_, 1);
@@ -1311,6 +1359,10 @@ void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
 void TsOracleGpu::launch_round() {
   finish_fold();
   round_points_ = false;
+  if (claim_round_valid_) {  // the claim already computed this round
+    round_from_claim_ = true;
+    return;
+  }
   switch (kind_) {
     case TSK_PRODUCT:
     case TSK_ADDR_LOOKUP:
@@ -1358,6 +1410,11 @@ static void coeffs_from_points(const K2* y, int n, K2* c) {
 }
 
 void TsOracleGpu::finish_round(std::vector<K2>& c) {
+  if (round_from_claim_) {
+    c = claim_round_;
+    round_from_claim_ = false;
+    return;
+  }
   c.assign(round_ncoef_, kzero());
   if (round_dev_) {
     NS_CHECK(hipStreamSynchronize(st_));
@@ -1415,6 +1472,8 @@ void TsOracleGpu::evals(const K2* xs, size_t n, K2* out) {
 K2 TsOracleGpu::claim() {
   std::vector<K2> c;
   round_coeffs(c);
+  claim_round_ = c;
+  claim_round_valid_ = true;
   // p(0) + p(1) = 2 c0 + c1 + ... + c_d
   K2 s = c.empty() ? kzero() : c[0];
   for (auto& x : c) s = kadd(s, x);
@@ -1430,6 +1489,7 @@ void TsOracleGpu::fold(K2 r) {
 // host state (the support size) to collect
 void TsOracleGpu::launch_fold(K2 r) {
   finish_fold();
+  claim_round_valid_ = false;
   if (num_rounds() == 0) return;
   switch (kind_) {
     case TSK_PRODUCT:
