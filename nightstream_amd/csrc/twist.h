@@ -92,7 +92,7 @@ class TsOracleGpu {
 
  private:
   bool time_round_launch();
-  void time_fold_launch(K2 r);
+  void time_fold_launch(K2 r, K2* chal_dst);
   bool addr_round_launch(bool final);
   void addr_round(std::vector<K2>& c, bool final);
   bool product_round_launch();
@@ -119,7 +119,8 @@ class TsOracleGpu {
   // Twist val_pre terms (sorted by time) and their exclusive prefix sums
   K2 init_at_ = K2{0, 0};
   size_t nterms_ = 0;
-  TsBuf term_t_, term_v_, term_p_;
+  TsBuf term_t_, term_v_, term_p_, term_rank_;
+  bool has_rank_ = false;
   TsBuf lt_c_[2], lt_s_[2], lt_tmp_;  // per-round LT_low contributions and their prefix sums
   std::vector<uint32_t> term_t_h_;
   std::vector<K2> term_v_h_;

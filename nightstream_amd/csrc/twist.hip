@@ -75,6 +75,40 @@ __device__ void block_store(K2 (&acc)[MAXC], int ncoef, K2* partials) {
   }
 }
 
+// ---------------- one-workgroup paths for supports of at most ONEBLK_N entries ----------------
+constexpr int ONEBLK_T = 1024, ONEBLK_PER = 8, ONEBLK_N = ONEBLK_T * ONEBLK_PER;
+
+__device__ __forceinline__ uint32_t shfl_up_u32(uint32_t v, int off) { return __shfl_up(v, off, 64); }
+__device__ __forceinline__ K2 shfl_up_k2(K2 v, int off) {
+  return K2{__shfl_up(v.c0, off, 64), __shfl_up(v.c1, off, 64)};
+}
+__device__ __forceinline__ uint32_t add_u32(uint32_t a, uint32_t b) { return a + b; }
+__device__ __forceinline__ K2 add_k2(K2 a, K2 b) { return kadd(a, b); }
+
+// exclusive prefix of v over a ONEBLK_T-thread block, in thread order; `wsum` holds
+// ONEBLK_T / 64 values of scratch LDS.  Ends with a barrier, so wsum can be reused.
+template <class T, T (*SHFL)(T, int), T (*ADD)(T, T)>
+__device__ __forceinline__ T block_exclusive(T v, T zero, T* wsum, T& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const T y = SHFL(x, off);
+    if (lane >= off) x = ADD(x, y);
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  T before = zero, tot = zero;
+  for (int j = 0; j < ONEBLK_T / 64; j++) {
+    if (j < w) before = ADD(before, wsum[j]);
+    tot = ADD(tot, wsum[j]);
+  }
+  total = tot;
+  T ex = SHFL(x, 1);
+  if (lane == 0) ex = zero;
+  __syncthreads();
+  return ADD(before, ex);
+}
+
 __global__ void k_ts_sum(const K2* __restrict__ partials, int nblk, int ncoef, K2* __restrict__ out) {
   const int d = blockIdx.x;
   K2 v = kzero();
@@ -100,6 +134,7 @@ struct TsTimeArgs {
   const uint32_t* term_t;
   const K2* term_v;
   const K2* term_p;
+  const uint32_t* term_rank;  // [len + 1]: number of terms before each time, or null (search)
   const K2* lt_s0;  // inclusive prefix sums over the terms of term_v * lt_low(term_t, bit_idx, c)
   const K2* lt_s1;
   uint32_t nterms;
@@ -137,20 +172,32 @@ __device__ K2 lt_low(uint32_t u, int b, int c, const K2* chals) {
   return acc;
 }
 
-// val_pre at the child time point (twist_oracle.rs:313-361): every term before the pair's time
+// val_pre at both child time points (twist_oracle.rs:313-361): every term before the pair's time
 // block contributes its full value (exclusive prefix term_p), the terms inside the block their
-// LT_low-weighted value, summed as a difference of the per-term prefix sums lt_s0 / lt_s1.
-__device__ K2 val_pre_child(const TsTimeArgs& A, uint64_t pair, int c) {
+// LT_low-weighted value, summed as a difference of the per-term prefix sums lt_s0 / lt_s1.  The
+// block's term range comes from the rank table when there is one, else from two binary searches.
+__device__ __forceinline__ void val_pre_pair(const TsTimeArgs& A, uint64_t pair, K2& vp0, K2& vp1) {
   const int b = A.bit_idx;
-  uint64_t lo_key = pair << (b + 1), hi_key = (pair + 1) << (b + 1);
-  uint32_t lo = lower_bound_u32(A.term_t, 0, A.nterms, lo_key);
-  uint32_t hi = lower_bound_u32(A.term_t, lo, A.nterms, hi_key);
-  K2 v = kadd(A.init_at, A.term_p[lo]);
-  if (hi > lo) {
-    const K2* S = c ? A.lt_s1 : A.lt_s0;
-    v = kadd(v, lo > 0 ? ksub(S[hi - 1], S[lo - 1]) : S[hi - 1]);
+  const uint64_t lo_key = pair << (b + 1), hi_key = (pair + 1) << (b + 1);
+  uint32_t lo, hi;
+  if (A.term_rank) {
+    lo = A.term_rank[lo_key];
+    hi = A.term_rank[hi_key];
+  } else {
+    lo = lower_bound_u32(A.term_t, 0, A.nterms, lo_key);
+    hi = lower_bound_u32(A.term_t, lo, A.nterms, hi_key);
   }
-  return v;
+  const K2 v = kadd(A.init_at, A.term_p[lo]);
+  vp0 = vp1 = v;
+  if (hi > lo) {
+    if (lo > 0) {
+      vp0 = kadd(v, ksub(A.lt_s0[hi - 1], A.lt_s0[lo - 1]));
+      vp1 = kadd(v, ksub(A.lt_s1[hi - 1], A.lt_s1[lo - 1]));
+    } else {
+      vp0 = kadd(v, A.lt_s0[hi - 1]);
+      vp1 = kadd(v, A.lt_s1[hi - 1]);
+    }
+  }
 }
 
 // per-term LT_low contributions of one round, for both children (one thread per term)
@@ -161,6 +208,30 @@ __global__ void k_lt_contrib(const uint32_t* __restrict__ term_t, const K2* __re
   const K2 v = term_v[k];
   out0[k] = kmul(v, lt_low(term_t[k], b, 0, chals));
   out1[k] = kmul(v, lt_low(term_t[k], b, 1, chals));
+}
+
+// inclusive prefix sums of two K arrays of n <= ONEBLK_N entries in one launch (thread t owns
+// entries [t*PER, t*PER+PER))
+__global__ __launch_bounds__(ONEBLK_T) void k_scan2_1blk(const K2* __restrict__ in0, const K2* __restrict__ in1,
+                                                         uint32_t n, K2* __restrict__ out0, K2* __restrict__ out1) {
+  __shared__ K2 wsum[ONEBLK_T / 64];
+  const uint32_t k0 = threadIdx.x * ONEBLK_PER;
+  for (int c = 0; c < 2; c++) {
+    const K2* in = c ? in1 : in0;
+    K2 loc[ONEBLK_PER];
+    K2 run = kzero();
+#pragma unroll
+    for (int j = 0; j < ONEBLK_PER; j++) {
+      if (k0 + j < n) run = kadd(run, in[k0 + j]);
+      loc[j] = run;
+    }
+    K2 total;
+    const K2 base = block_exclusive<K2, shfl_up_k2, add_k2>(run, kzero(), wsum, total);
+    K2* out = c ? out1 : out0;
+#pragma unroll
+    for (int j = 0; j < ONEBLK_PER; j++)
+      if (k0 + j < n) out[k0 + j] = kadd(base, loc[j]);
+  }
 }
 
 // lt_eval (mle.rs:7-24) at the child time point (chals | c | bits of pair) against r_time
@@ -241,13 +312,15 @@ __device__ __forceinline__ bool ts_pair_factors(const TsTimeArgs& A, uint32_t i,
       has_f = false;
       break;
     case TSK_TWIST_READ: {
-      const K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
+      K2 vp0, vp1;
+      val_pre_pair(A, pair, vp0, vp1);
       f0 = ksub(vp0, ts_col(A, A.col_a, i0));
       f1 = ksub(vp1, ts_col(A, A.col_a, i1));
       break;
     }
     case TSK_TWIST_WRITE: {
-      const K2 vp0 = val_pre_child(A, pair, 0), vp1 = val_pre_child(A, pair, 1);
+      K2 vp0, vp1;
+      val_pre_pair(A, pair, vp0, vp1);
       f0 = ksub(ksub(ts_col(A, A.col_a, i0), vp0), ts_col(A, A.col_b, i0));
       f1 = ksub(ksub(ts_col(A, A.col_a, i1), vp1), ts_col(A, A.col_b, i1));
       break;
@@ -355,13 +428,10 @@ __global__ void k_ts_heads(const uint32_t* __restrict__ idx, uint32_t n, uint32_
   flags[i] = (i == 0 || (idx[i - 1] >> 1) != (idx[i] >> 1)) ? 1u : 0u;
 }
 
-__global__ void k_ts_fold(const uint32_t* __restrict__ idx, const K2* __restrict__ cols, uint32_t n, uint32_t stride,
-                          int ncol, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, K2 r,
-                          uint32_t* __restrict__ nidx, K2* __restrict__ ncols, uint32_t* __restrict__ nout) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (i == n - 1) nout[0] = pos[i] + flags[i];
-  if (!flags[i]) return;
+// parent p of the pair headed by entry i: its time index and every column folded at r
+__device__ __forceinline__ void ts_fold_entry(const uint32_t* __restrict__ idx, const K2* __restrict__ cols,
+                                              uint32_t n, uint32_t stride, int ncol, uint32_t i, uint32_t p, K2 r,
+                                              uint32_t* __restrict__ nidx, K2* __restrict__ ncols) {
   const uint32_t id = idx[i];
   uint32_t i0 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu;
   if ((id & 1) == 0) {
@@ -370,12 +440,51 @@ __global__ void k_ts_fold(const uint32_t* __restrict__ idx, const K2* __restrict
   } else {
     i1 = i;
   }
-  const uint32_t p = pos[i];
   nidx[p] = id >> 1;
   for (int c = 0; c < ncol; c++) {
     K2 v0 = i0 != 0xFFFFFFFFu ? cols[(size_t)c * stride + i0] : kzero();
     K2 v1 = i1 != 0xFFFFFFFFu ? cols[(size_t)c * stride + i1] : kzero();
     ncols[(size_t)c * stride + p] = kinterp(v0, v1, r);
+  }
+}
+
+__global__ void k_ts_fold(const uint32_t* __restrict__ idx, const K2* __restrict__ cols, uint32_t n, uint32_t stride,
+                          int ncol, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, K2 r,
+                          uint32_t* __restrict__ nidx, K2* __restrict__ ncols, uint32_t* __restrict__ nout) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i == n - 1) nout[0] = pos[i] + flags[i];
+  if (!flags[i]) return;
+  ts_fold_entry(idx, cols, n, stride, ncol, i, pos[i], r, nidx, ncols);
+}
+
+// pair heads of a time-lane fold and their exclusive scan in one launch (n <= ONEBLK_N; thread t
+// owns entries [t*PER, t*PER+PER)): flags[i], pos[i] and the folded support size pos[n]; thread
+// 0 also stores the round challenge into the constants block when chal_dst is set
+__global__ __launch_bounds__(ONEBLK_T) void k_ts_heads_1blk(const uint32_t* __restrict__ idx, uint32_t n,
+                                                            uint32_t* __restrict__ flags, uint32_t* __restrict__ pos,
+                                                            K2* __restrict__ chal_dst, K2 r) {
+  __shared__ uint32_t wsum[ONEBLK_T / 64];
+  const uint32_t i0 = threadIdx.x * ONEBLK_PER;
+  uint32_t f[ONEBLK_PER], cnt = 0;
+#pragma unroll
+  for (int k = 0; k < ONEBLK_PER; k++) {
+    const uint32_t i = i0 + k;
+    f[k] = (i < n && (i == 0 || (idx[i - 1] >> 1) != (idx[i] >> 1))) ? 1u : 0u;
+    cnt += f[k];
+  }
+  uint32_t total;
+  uint32_t p = block_exclusive<uint32_t, shfl_up_u32, add_u32>(cnt, 0u, wsum, total);
+#pragma unroll
+  for (int k = 0; k < ONEBLK_PER; k++)
+    if (i0 + k < n) {
+      flags[i0 + k] = f[k];
+      pos[i0 + k] = p;
+      p += f[k];
+    }
+  if (threadIdx.x == 0) {
+    pos[n] = total;
+    if (chal_dst) *chal_dst = r;
   }
 }
 
@@ -926,6 +1035,17 @@ void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t 
     upload(term_t_, term_t_h_, st_);
     upload(term_v_, term_v_h_, st_);
     upload(term_p_, pre, st_);
+    // rank table: terms before each time 0..len (4 (len + 1) bytes), for domains up to 2^20
+    has_rank_ = ell_n <= 20;
+    if (has_rank_) {
+      std::vector<uint32_t> rank(len_ + 1);
+      size_t k = 0;
+      for (size_t t = 0; t <= len_; t++) {
+        while (k < nterms_ && term_t_h_[k] < t) k++;
+        rank[t] = (uint32_t)k;
+      }
+      upload(term_rank_, rank, st_);
+    }
   }
   NS_CHECK(hipStreamSynchronize(st_));
 }
@@ -974,6 +1094,7 @@ bool TsOracleGpu::time_round_launch() {
   A.term_t = term_t_.as<uint32_t>();
   A.term_v = term_v_.as<K2>();
   A.term_p = term_p_.as<K2>();
+  A.term_rank = has_rank_ ? term_rank_.as<uint32_t>() : nullptr;
   A.nterms = (uint32_t)nterms_;
   A.ncoef = ncoef;
   A.lt_s0 = A.lt_s1 = nullptr;
@@ -986,13 +1107,18 @@ bool TsOracleGpu::time_round_launch() {
     hipLaunchKernelGGL(k_lt_contrib, dim3((nt + 255) / 256), dim3(256), 0, st_, term_t_.as<uint32_t>(),
                        term_v_.as<K2>(), nt, (int)bit_idx_, consts_.as<K2>() + ell_n_, lt_c_[0].as<K2>(),
                        lt_c_[1].as<K2>());
-    size_t tb = 0;
-    NS_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, tb, lt_c_[0].as<K2>(), lt_s_[0].as<K2>(), KAddOp(), (int)nt,
-                                               st_));
-    lt_tmp_.ensure(tb + 16);
-    for (int c = 0; c < 2; c++)
-      NS_CHECK(hipcub::DeviceScan::InclusiveScan(lt_tmp_.p, tb, lt_c_[c].as<K2>(), lt_s_[c].as<K2>(), KAddOp(),
+    if (nt <= (uint32_t)ONEBLK_N) {
+      hipLaunchKernelGGL(k_scan2_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, lt_c_[0].as<K2>(), lt_c_[1].as<K2>(), nt,
+                         lt_s_[0].as<K2>(), lt_s_[1].as<K2>());
+    } else {
+      size_t tb = 0;
+      NS_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, tb, lt_c_[0].as<K2>(), lt_s_[0].as<K2>(), KAddOp(),
                                                  (int)nt, st_));
+      lt_tmp_.ensure(tb + 16);
+      for (int c = 0; c < 2; c++)
+        NS_CHECK(hipcub::DeviceScan::InclusiveScan(lt_tmp_.p, tb, lt_c_[c].as<K2>(), lt_s_[c].as<K2>(), KAddOp(),
+                                                   (int)nt, st_));
+    }
     A.lt_s0 = lt_s_[0].as<K2>();
     A.lt_s1 = lt_s_[1].as<K2>();
   }
@@ -1011,8 +1137,24 @@ bool TsOracleGpu::time_round_launch() {
   return true;
 }
 
-// enqueue a time-lane fold; the folded support size arrives in the mailbox (finish_fold)
-void TsOracleGpu::time_fold_launch(K2 r) {
+// enqueue a time-lane fold; the folded support size arrives in the mailbox (finish_fold).
+// chal_dst: where the challenge goes in the constants block (null: not stored)
+void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
+  if (n_ > 0 && n_ <= (size_t)ONEBLK_N) {
+    const uint32_t n = (uint32_t)n_;
+    uint32_t* nout = pos_.as<uint32_t>() + n;
+    hipLaunchKernelGGL(k_ts_heads_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, idx_[cur_].as<uint32_t>(), n,
+                       flags_.as<uint32_t>(), pos_.as<uint32_t>(), chal_dst, r);
+    hipLaunchKernelGGL(k_ts_fold, dim3((n + 255) / 256), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(),
+                       cols_[cur_].as<K2>(), n, (uint32_t)cap_, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(),
+                       r, idx_[cur_ ^ 1].as<uint32_t>(), cols_[cur_ ^ 1].as<K2>(), nout);
+    NS_CHECK(hipMemcpyAsync(coef_h_ + TS_MAX_COEF, nout, 4, hipMemcpyDeviceToHost, st_));
+    fold_pending_ = true;
+    cur_ ^= 1;
+    len_ >>= 1;
+    return;
+  }
+  if (chal_dst) hipLaunchKernelGGL(k_put_k2, dim3(1), dim3(64), 0, st_, chal_dst, r);
   if (n_ > 0) {
     const uint32_t n = (uint32_t)n_;
     const uint32_t stride = (uint32_t)cap_;
@@ -1521,7 +1663,7 @@ void TsOracleGpu::launch_fold(K2 r) {
       return;
     }
     case TSK_TOTAL_INC:
-      time_fold_launch(r);
+      time_fold_launch(r, nullptr);
       return;
     default:
       break;
@@ -1530,9 +1672,8 @@ void TsOracleGpu::launch_fold(K2 r) {
     K2 rc = r_cycle_[bit_idx_];
     prefix_eq_ = kmul(prefix_eq_, kadd(kmul(ksub(kone(), r), ksub(kone(), rc)), kmul(r, rc)));
   }
-  time_fold_launch(r);
   // constants block: r_cycle (ell_n) | challenges (ell_n slots); at most ell_n folds
-  hipLaunchKernelGGL(k_put_k2, dim3(1), dim3(64), 0, st_, consts_.as<K2>() + ell_n_ + chals_.size(), r);
+  time_fold_launch(r, consts_.as<K2>() + ell_n_ + chals_.size());
   chals_.push_back(r);
   bit_idx_++;
 }

@@ -41,7 +41,11 @@ def both(twist, oracle, kind, *args):
     return twist.TsOracle(kind, *args), oracle.TsOracle(kind, *args)
 
 
-@pytest.mark.parametrize("ell_addr,ell_n,p,wf,zipf", [(5, 10, 1.0, 0.34, 0.0), (16, 10, 0.35, 0.4, 1.1)])
+# 2^14 cycles, every one an access: a support of 16384 entries, above the one-workgroup limit of
+# 8192, so the first rounds take the multi-kernel fold (writes are kept rare: the oracle's
+# val_pre costs pairs x increment terms)
+@pytest.mark.parametrize("ell_addr,ell_n,p,wf,zipf", [(5, 10, 1.0, 0.34, 0.0), (16, 10, 0.35, 0.4, 1.1),
+                                                     (5, 14, 1.0, 0.01, 0.0)])
 def test_twist_time_lanes(twist, oracle, ell_addr, ell_n, p, wf, zipf):
     tr = gen_memory_trace(ell_n, ell_addr, p, wf, seed=7 + ell_addr, zipf_s=zipf)
     rng = np.random.default_rng(ell_addr)
@@ -63,6 +67,48 @@ def test_twist_time_lanes(twist, oracle, ell_addr, ell_n, p, wf, zipf):
     w = [rand_k(rng) for _ in tr["ra_bits"]]
     g, c = both(twist, oracle, "bitness", r_cycle, tr["ra_bits"], w)
     drive(g, c, rng, "bitness")
+
+
+def poly_at(vals, x):
+    """Value at x of the polynomial through (j, vals[j]), j = 0..d (Lagrange, exact in K)."""
+    from kmath import kadd, kinv, kmul, ksub
+    acc = (0, 0)
+    for j in range(len(vals)):
+        num, den = (1, 0), (1, 0)
+        for m in range(len(vals)):
+            if m != j:
+                num = kmul(num, ksub(x, (m, 0)))
+                den = kmul(den, ksub((j, 0), (m, 0)))
+        acc = kadd(acc, kmul(vals[j], kmul(num, kinv(den))))
+    return acc
+
+
+def test_twist_sumcheck_chain_large(twist):
+    """2^15 cycles with ~11k increment terms, above the one-workgroup limit of 8192 for the fold
+    and the val_pre prefix sums (the oracle restatement is quadratic there): every round must
+    satisfy p(0) + p(1) = previous p(r), and the value after the last round (a separate kernel,
+    the full val_pre sum) must equal the last p(r) -- the verifier's checks."""
+    from kmath import kadd
+    tr = gen_memory_trace(15, 5, 1.0, 0.34, seed=12)
+    rng = np.random.default_rng(90)
+    r_cycle = [rand_k(rng) for _ in range(15)]
+    r_addr = [rand_k(rng) for _ in range(5)]
+    init_at = rand_k(rng)
+    for kind, args in (("twist_read", (r_cycle, tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["inc"],
+                                       tr["wa_bits"], r_addr, init_at)),
+                       ("twist_write", (r_cycle, tr["has_write"], tr["wv"], tr["inc"], tr["wa_bits"], r_addr,
+                                        init_at))):
+        print(f"{kind} 2^15 sum-check chain", flush=True)
+        g = twist.TsOracle(kind, *args)
+        claim = None
+        for rd in range(g.num_rounds()):
+            vals = g.evals([(v, 0) for v in range(g.degree_bound() + 1)])
+            if claim is not None:
+                assert kadd(vals[0], vals[1]) == claim, f"{kind}: round {rd}"
+            r = rand_k(rng)
+            claim = poly_at(vals, r)
+            g.fold(r)
+        assert g.evals([rand_k(rng)])[0] == claim, f"{kind}: final value"
 
 
 @pytest.mark.parametrize("ell_addr,ell_n", [(5, 12), (16, 12)])
