@@ -788,9 +788,11 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
 }
 
 // ---------------- fused fold + round on the K tables (rounds >= 3) ----------------
-// Work item (pair t, row group g) folds its rows' entries 4t..4t+3 of the previous tables by the
-// previous challenge r into entries 2t, 2t+1 of the new tables (written for the next round) and
-// evaluates the round polynomial on them; the f group also folds the scalar tables.  Each block
+// Work item (pair t, row group g < G) folds its rows' entries 4t..4t+3 of the previous tables by
+// the previous challenge r into entries 2t, 2t+1 of the new tables (written for the next round)
+// and evaluates the round polynomial on them; item g = G evaluates f(Mz) and the Eval term from
+// the scalar tables (folded in registers), and items g = G + 1 + q fold scalar table q for the
+// next round, so no thread waits on more than one round of loads per table.  Each block
 // leaves its 5 partial coefficients for k_sum_partials (the kernel boundary orders them; a
 // device-scope fence per block would write back L2).  Same values as launch_fold_k followed by
 // launch_round_eval.
@@ -810,7 +812,7 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
 #pragma unroll
   for (int d = 0; d < 5; d++) c[d] = kzero();
   K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
-  const size_t total = pairs * ((size_t)G + 1);
+  const size_t total = pairs * ((size_t)G + 1 + A.nsc);
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const size_t t = idx % pairs;
     const int g = (int)(idx / pairs);
@@ -820,25 +822,43 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
     K2 in[5];
 #pragma unroll
     for (int d = 0; d < 5; d++) in[d] = kzero();
-    if (g == G) {
-      // fold the scalar tables at 2t, 2t+1, then evaluate f(Mz)(X) and the Eval term on the
-      // folded values (recomputed where needed: no runtime-indexed register arrays)
-      auto folded = [&](int q, K2& lo, K2& hi) {
+    if (g > G) {
+      // scalar table q = g - G - 1 folded at 2t, 2t+1 for the next round (no round contribution)
+      const int q = g - G - 1;
+      K2 lo = elo, hi = ehi;
+      if (q > 0) {
         const K2* src = A.sc_src + (size_t)q * L2 + 4 * t;
         lo = fold2(src[0], src[1], r, r7);
         hi = fold2(src[2], src[3], r, r7);
-      };
-      for (int q = 0; q < A.nsc; q++) {
-        K2 lo, hi;
-        if (q == 0) {
-          lo = elo;
-          hi = ehi;
-        } else {
-          folded(q, lo, hi);
+      }
+      K2* dst = A.sc_dst + (size_t)q * A.len + 2 * t;
+      dst[0] = lo;
+      dst[1] = hi;
+      continue;
+    }
+    if (g == G) {
+      // f(Mz)(X) and the Eval term on the folded f-variable and Eval tables: every load is issued
+      // at once, and a variable is picked by a register select (a runtime-indexed array would
+      // live in scratch memory)
+      constexpr int NFV = FUSED_MAX_SC - 1;  // nsc = 1 + nf (+ 2 Eval tables) <= FUSED_MAX_SC
+      K2 flo[NFV], fhi[NFV];
+#pragma unroll
+      for (int q = 0; q < NFV; q++) {
+        flo[q] = fhi[q] = kzero();
+        if (q < A.nf) {
+          const K2* src = A.sc_src + (size_t)(1 + q) * L2 + 4 * t;
+          flo[q] = fold2(src[0], src[1], r, r7);
+          fhi[q] = fold2(src[2], src[3], r, r7);
         }
-        K2* dst = A.sc_dst + (size_t)q * A.len + 2 * t;
-        dst[0] = lo;
-        dst[1] = hi;
+      }
+      K2 rl = kzero(), rh = kzero(), vl = kzero(), vh = kzero();
+      if (A.has_eval) {
+        const K2* sr = A.sc_src + (size_t)(1 + A.nf) * L2 + 4 * t;
+        const K2* sv = sr + L2;
+        rl = fold2(sr[0], sr[1], r, r7);
+        rh = fold2(sr[2], sr[3], r, r7);
+        vl = fold2(sv[0], sv[1], r, r7);
+        vh = fold2(sv[2], sv[3], r, r7);
       }
       for (int tt = 0; tt < A.f.nterms; tt++) {
         K2 p[5];
@@ -846,8 +866,14 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
 #pragma unroll
         for (int d = 1; d < 5; d++) p[d] = kzero();
         for (int v = 0; v < A.f.nv[tt]; v++) {
-          K2 lo, hi;
-          folded(1 + A.f.var[tt][v], lo, hi);
+          const int q = A.f.var[tt][v];
+          K2 lo = flo[0], hi = fhi[0];
+#pragma unroll
+          for (int qq = 1; qq < NFV; qq++)
+            if (q == qq) {
+              lo = flo[qq];
+              hi = fhi[qq];
+            }
           const K2 a = lo, b = ksub(hi, lo);
           for (int e = 0; e < A.f.exp[tt][v]; e++) {
 #pragma unroll
@@ -859,9 +885,6 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
         for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
       }
       if (A.has_eval) {
-        K2 rl, rh, vl, vh;
-        folded(1 + A.nf, rl, rh);
-        folded(2 + A.nf, vl, vh);
         const K2 r0_ = rl, r1_ = ksub(rh, rl);
         const K2 v0 = vl, v1 = ksub(vh, vl);
         ev0 = kadd(ev0, kmul(r0_, v0));
@@ -932,7 +955,7 @@ void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hip
   while (G < nrow && pairs * G < target) G *= 2;
   if (G > nrow) G = nrow;
   G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
-  size_t nblk = (pairs * (G + 1) + RB - 1) / RB;
+  size_t nblk = (pairs * (G + 1 + (size_t)A.nsc) + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
   K2* partials = (K2*)ws;
