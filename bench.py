@@ -122,6 +122,7 @@ def main():
                     help="shard: one 2^16-row shard (default); batch: 128 shards of 2^10..2^16 rows, plain path vs "
                          "a pool of concurrent prover contexts; serve: Poisson arrivals with continuous admission")
     ap.add_argument("--workers", type=int, default=16, help="concurrent prover contexts (batch / serve)")
+    ap.add_argument("--build-threads", type=int, default=8, help="twist: host threads building a chunk's oracles")
     ap.add_argument("--shards", type=int, default=128, help="batch: number of shards")
     ap.add_argument("--rate-duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--rates", type=str, default="", help="serve: comma-separated arrival rates (jobs/s)")
@@ -435,8 +436,9 @@ def bench_twist(args):
     of a chunk is constructed, the oracles are grouped by number of rounds (12 time-lane rounds,
     5 or 16 address-lane rounds) and each group is driven through all its rounds with shared
     seeded challenges (no transcript), on the GPU with the batched driver, timed per chunk.  The
-    same chunks are also timed one oracle at a time on the GPU; the CPU restatement runs chunk 0
-    as the baseline, and chunk 0's values must be identical on all three."""
+    same chunks are also timed one oracle at a time on the GPU (and built on one thread); the CPU
+    restatement runs chunk 0 as the baseline, and chunk 0's values must be identical on all three.
+    With the batched driver a chunk's oracles are built on --build-threads host threads."""
     import nightstream_amd as ns
     from nightstream_amd import twist
     from oracle import pyoracle
@@ -445,11 +447,15 @@ def bench_twist(args):
     chals = [[(int(a) * 2 % Q_FIELD, int(b) * 2 % Q_FIELD) for a, b in rng_ch.integers(0, 1 << 63, size=(40, 2))]
              for _ in range(64)]
 
-    def run(cls, c, batch=None):
+    def run(cls, c, batch=None, pool=None):
         twists, shouts = chunks[c]
         rng = np.random.default_rng(1000 + c)
         t0 = time.perf_counter()
-        made = [(label, make()) for label, make in twist_chunk_oracles(cls, twists, shouts, rng)]
+        makers = twist_chunk_oracles(cls, twists, shouts, rng)
+        if pool is not None:  # the constructors release the GIL; each oracle owns its stream
+            made = list(pool.map(lambda lm: (lm[0], lm[1]()), makers))
+        else:
+            made = [(label, make()) for label, make in makers]
         per = {"construct": time.perf_counter() - t0}
         groups = {}
         for i, (_, o) in enumerate(made):
@@ -462,14 +468,14 @@ def bench_twist(args):
             per[f"{nr} rounds x {len(idxs)} oracles"] = time.perf_counter() - t1
         return vals, per, len(made)
 
-    def timed(batch):
+    def timed(batch, pool=None):
         for c in range(min(args.warmup, len(chunks))):
-            run(twist.TsOracle, c, batch)
+            run(twist.TsOracle, c, batch, pool)
         ns.device_synchronize()
         t0 = time.perf_counter()
         per_all, vals0, n_or = {}, None, 0
         for c in range(len(chunks)):
-            v, per, n_or = run(twist.TsOracle, c, batch)
+            v, per, n_or = run(twist.TsOracle, c, batch, pool)
             if c == 0:
                 vals0 = v
             for k, x in per.items():
@@ -477,7 +483,9 @@ def bench_twist(args):
         ns.device_synchronize()
         return time.perf_counter() - t0, per_all, vals0, n_or
 
-    t_gpu, per_all, gpu_vals0, n_or = timed(twist)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=args.build_threads) as pool:
+        t_gpu, per_all, gpu_vals0, n_or = timed(twist, pool)
     t_one, _, one_vals0, _ = timed(None)
     pyoracle.build()
     t1 = time.perf_counter()
@@ -494,6 +502,7 @@ def bench_twist(args):
                    "oracles_per_chunk": n_or, "chunks": len(chunks)},
         "ms_per_chunk": round(t_gpu / len(chunks) * 1e3, 3),
         "ms_per_chunk_one_oracle_at_a_time": round(t_one / len(chunks) * 1e3, 3),
+        "construct_threads": args.build_threads,
         "ms_per_phase_per_chunk": {k: round(v / len(chunks) * 1e3, 3) for k, v in per_all.items()},
         "cpu_baseline": {"value": round(1.0 / t_cpu, 4), "unit": "chunks/s", "cores": 1, "kind": "port",
                          "sample": "CPU restatement (oracle/twist.c) of chunk 0, every oracle, single thread"},

@@ -194,6 +194,9 @@ int ns_ts_addr_lookup_new(const ns_sparse_col* addr_bits, size_t ell_addr, ns_sp
                           uint64_t claim[2], ns_ts** out);
 int ns_ts_evals(ns_ts* o, const uint64_t* xs, size_t n, uint64_t* out);
 int ns_ts_fold(ns_ts* o, const uint64_t r[2]);
+/* Threads: the ns_ts_*_new constructors may run concurrently on several host threads (each
+ * oracle gets its own stream; the library's device-block cache is locked); one oracle must not be
+ * used from two threads at once, as with the reference's &mut self. */
 /* One round of the batched sum-check with shared challenges
  * (run_batched_sumcheck_prover, neo-reductions/src/sumcheck.rs:350-456, the evals_at loop at
  * :390-407): every oracle's round is enqueued on its own stream before the host waits for any.

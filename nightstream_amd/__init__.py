@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -42,13 +43,22 @@ class NsError(RuntimeError):
 
 
 _lib = None
+_lib_lock = threading.Lock()
 
 
 def lib():
-    """Load libnightstream.so (raises if it has not been built)."""
+    """Load libnightstream.so (raises if it has not been built).  Thread-safe: the first caller
+    loads and annotates the one library object every later call returns."""
     global _lib
     if _lib is not None:
         return _lib
+    with _lib_lock:
+        if _lib is None:
+            _lib = _load()
+    return _lib
+
+
+def _load():
     if not os.path.exists(LIB_PATH):
         raise NsError(-10, f"{LIB_PATH} is missing: run `python -m nightstream_amd.build`")
     L = C.CDLL(LIB_PATH)
@@ -97,7 +107,6 @@ def lib():
     L.ns_prover_phase_perms.argtypes = [vp, vp, sz]
     L.ns_poseidon2_permute.argtypes = [vp, sz, C.c_int]
     L.ns_params_goldilocks_auto_r1cs_ccs.argtypes = [sz, vp]
-    _lib = L
     return L
 
 

@@ -10,6 +10,7 @@ reference constructors (neo-memory/src/twist_oracle.rs) in the same order.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 
 import numpy as np
 
@@ -51,13 +52,23 @@ class _Keep:
 
 
 _BOUND = False
+_BIND_LOCK = threading.Lock()
 
 
 def _bind():
-    global _BOUND
+    """The library with the ns_ts_* signatures set (once, under a lock: oracles may be built on
+    several threads)."""
     L = lib()
     if _BOUND:
         return L
+    with _BIND_LOCK:
+        if not _BOUND:
+            _set_argtypes(L)
+    return L
+
+
+def _set_argtypes(L):
+    global _BOUND
     vp, sz, S = C.c_void_p, C.c_size_t, SparseCol
     out = C.POINTER(vp)
     L.ns_ts_product_new.argtypes = [vp, sz, sz, sz, out]
@@ -82,7 +93,6 @@ def _bind():
     L.ns_ts_free.argtypes = [vp]
     L.ns_ts_free.restype = None
     _BOUND = True
-    return L
 
 
 class TsOracle:

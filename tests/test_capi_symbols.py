@@ -37,3 +37,29 @@ def test_python_wrapper_binds_header_symbols():
     import nightstream_amd
     assert set(nightstream_amd.EXPORTS) <= set(header_symbols())
     assert set(header_symbols()) <= set(nightstream_amd.EXPORTS)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libnightstream.so not built (run __graft_entry__.build())")
+def test_library_loads_once_under_threads():
+    """16 threads loading and binding the library at once (as a caller building oracles on a
+    thread pool does) all get the one library object, with the Twist/Shout signatures set."""
+    import subprocess
+    import sys
+    code = (
+        "import threading, ctypes\n"
+        "import nightstream_amd as ns\n"
+        "from nightstream_amd import twist\n"
+        "got = []\n"
+        "barrier = threading.Barrier(16)\n"
+        "def go():\n"
+        "    barrier.wait()\n"
+        "    got.append(twist._bind())\n"
+        "ts = [threading.Thread(target=go) for _ in range(16)]\n"
+        "[t.start() for t in ts]\n"
+        "[t.join() for t in ts]\n"
+        "assert len(got) == 16 and all(g is got[0] for g in got) and got[0] is ns.lib()\n"
+        "assert got[0].ns_ts_batch_round.argtypes is not None\n"
+        "assert got[0].ns_ts_num_rounds.restype is ctypes.c_size_t\n"
+        "print('ok')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr

@@ -196,6 +196,45 @@ def test_batched_sumcheck(twist, oracle):
         twist.batch_round(mixed)
 
 
+def test_concurrent_construction(twist):
+    """Oracles built on 8 host threads at once (the constructors release the GIL) give the same
+    round values, claims and final values as the same oracles built one after another."""
+    from concurrent.futures import ThreadPoolExecutor
+    ell_n = 12
+    tr = gen_memory_trace(ell_n, 16, 0.35, 0.4, seed=41, zipf_s=1.1)
+    lanes = gen_lookup_trace(ell_n, seed=42)
+    rng = np.random.default_rng(95)
+    rk = lambda n: [rand_k(rng) for _ in range(n)]  # noqa: E731
+    r_cycle, r_addr, init_at, r_time = rk(ell_n), rk(16), rand_k(rng), rk(ell_n)
+    specs = [("twist_read", r_cycle, tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["inc"],
+              tr["wa_bits"], r_addr, init_at),
+             ("twist_write", r_cycle, tr["has_write"], tr["wv"], tr["inc"], tr["wa_bits"], r_addr, init_at),
+             ("val_eval", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], r_addr, r_time),
+             ("total_inc", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], r_addr)]
+    for ln in lanes:
+        specs += [("shout_value", r_cycle, ln["has_lookup"], ln["val"]),
+                  ("index_adapter", r_cycle, ln["has_lookup"], ln["addr_bits"], rk(16))]
+    specs = specs * 2
+    chals = rk(ell_n)
+
+    def run(oracles):
+        out = [[o.claim] for o in oracles]
+        for rd in range(ell_n):
+            for i, y in enumerate(twist.batch_round(oracles)):
+                out[i].append(y)
+            twist.batch_fold(oracles, chals[rd])
+        x = [rand_k(np.random.default_rng(7))]
+        for i, o in enumerate(oracles):
+            out[i].append(o.evals(x))
+        return out
+
+    print(f"concurrent construction: {len(specs)} oracles on 8 threads", flush=True)
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        par = list(pool.map(lambda a: twist.TsOracle(*a), specs))
+    seq = [twist.TsOracle(*a) for a in specs]
+    assert run(par) == run(seq)
+
+
 def test_product_oracle(twist, oracle):
     rng = np.random.default_rng(70)
     facs = [[rand_k(rng) for _ in range(1 << 10)] for _ in range(3)]
