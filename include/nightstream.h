@@ -80,6 +80,9 @@ int ns_decomp_b(const uint64_t* z, size_t m, uint32_t b, int8_t* Z_out);
  *      neo-ajtai/src/s_module.rs:335-364 -> commit_precomp_ct_row_major, commit.rs:679-744).
  * Commits n_wit host digit matrices in one batched launch; out[w*kappa*D + col*D + row]. */
 int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit, uint64_t* out);
+/* The same commitments for digit matrices already resident in HBM: Z_dev[w] is a device pointer
+ * to the D x m int8 matrix of witness w (from ns_device_alloc); only the commitments cross PCIe. */
+int ns_ajtai_commit_batch_device(const ns_pp* pp, const int8_t* const* Z_dev, size_t n_wit, uint64_t* out);
 
 /* ---- RoundOracle for Pi_CCS (neo-reductions/src/sumcheck.rs:17-34; OptimizedOracle,
  *      optimized_engine/oracle.rs:1597-1639 new_with_sparse, 2075-2104 trait impl).

@@ -23,7 +23,7 @@ EXPORTS = [
     "ns_last_error", "ns_device_count", "ns_free", "ns_device_alloc", "ns_device_free", "ns_memcpy_h2d",
     "ns_memcpy_d2h", "ns_device_synchronize", "ns_params_goldilocks_auto_r1cs_ccs", "ns_ccs_from_r1cs",
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
-    "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
+    "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
     "ns_dec_split", "ns_prover_new", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
@@ -86,6 +86,7 @@ def _load():
     L.ns_pp_coeffs.argtypes = [vp, vp]
     L.ns_decomp_b.argtypes = [vp, sz, C.c_uint32, vp]
     L.ns_ajtai_commit_batch.argtypes = [vp, vp, sz, vp]
+    L.ns_ajtai_commit_batch_device.argtypes = [vp, vp, sz, vp]
     L.ns_ccs_oracle_new.argtypes = [vp, vp, sz, vp, vp, vp, vp, sz, sz, vp, C.POINTER(vp)]
     L.ns_oracle_evals.argtypes = [vp, vp, sz, vp]
     L.ns_oracle_fold.argtypes = [vp, vp]
@@ -230,6 +231,13 @@ class PublicParams:
         out = np.zeros(len(Zs) * KAPPA * D, np.uint64)
         _check(lib().ns_ajtai_commit_batch(self.h, _ptr_array(Zs), len(Zs), _ptr(out)))
         return out.reshape(len(Zs), KAPPA * D)
+
+    def commit_device(self, bufs) -> np.ndarray:
+        """commit for D x m digit matrices already in HBM (DeviceBuffer objects) -> [n, kappa*D]."""
+        ptrs = (C.c_void_p * max(1, len(bufs)))(*[b.ptr for b in bufs])
+        out = np.zeros(max(1, len(bufs)) * KAPPA * D, np.uint64)
+        _check(lib().ns_ajtai_commit_batch_device(self.h, ptrs, len(bufs), _ptr(out)))
+        return out[:len(bufs) * KAPPA * D].reshape(len(bufs), KAPPA * D)
 
     def __del__(self):
         if getattr(self, "h", None):

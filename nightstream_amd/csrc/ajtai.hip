@@ -138,10 +138,15 @@ static size_t aj_splits(size_t m, size_t nw) {
   return best;
 }
 
+// Row-major digit rows a K step apart sit m bytes apart: at m >= 2^18 one K step of a 64-witness
+// batch touches over a thousand 2 MB pages, so the digits are first copied to K-step-major order.
+static bool aj_kmajor(size_t m) { return m >= ((size_t)1 << 18); }
+
 size_t ajtai_workspace_bytes(size_t m, size_t nw) {
   size_t npad = aj_groups(nw) * AJ_NBLK;
   size_t sp = aj_splits(m, nw);
-  return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
+  const size_t bt = aj_kmajor(m) ? nw * D * m + 256 : 0;
+  return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512 + bt;
 }
 
 // Workgroups go to the 8 XCDs round-robin (hardware block b runs on XCD b % 8).  The remap gives
@@ -153,9 +158,12 @@ __device__ inline unsigned xcd_remap(unsigned b, unsigned nb) {
   return x < tall ? x * per + l : tall * per + (x - tall) * (per - 1) + l;
 }
 
-// 1-D grid of 36 x groups x S blocks; split z covers K steps [z nsteps / S, (z + 1) nsteps / S)
+// 1-D grid of 36 x groups x S blocks; split z covers K steps [z nsteps / S, (z + 1) nsteps / S).
+// Digits come row-major (Zs[w] + t m, a K step 64 bytes further) or, when Bt is set, K-step-major
+// (Bt[(step * nw * 54 + w * 54 + t) * 64], see k_digits_kmajor).
 __global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restrict__ limbs, size_t nsteps,
-                                                          const int8_t* const* __restrict__ Zs, size_t nw, size_t m,
+                                                          const int8_t* const* __restrict__ Zs,
+                                                          const int8_t* __restrict__ Bt, size_t nw, size_t m,
                                                           size_t npad, int groups, int S,
                                                           int32_t* __restrict__ part) {
   typedef const __attribute__((address_space(1))) v4i gv4i;
@@ -196,14 +204,18 @@ __global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restr
     const int row = inb ? crow(chunk) : 0, col = (chunk & 3) * 16;
     const size_t nc = n0 + row, w = nc / D, t = nc % D;
     bok[c] = inb && w < nw;
-    bsrc[c] = bok[c] ? Zs[w] + t * m + col : Zs[0] + col;
+    if (Bt)
+      bsrc[c] = Bt + (bok[c] ? nc * 64 : 0) + col;
+    else
+      bsrc[c] = bok[c] ? Zs[w] + t * m + col : Zs[0] + col;
     bdst[c] = inb ? row * AJ_LDP + col : -1;
   }
+  const size_t bstep = Bt ? nw * D * 64 : AJ_BK;  // bytes from one K step of a digit row to the next
   v4i ra0[AJ_ASL], rb0[AJ_BSL], ra1[AJ_ASL], rb1[AJ_BSL];
 #define AJ_GLOAD(RA, RB, st)                                                                \
   {                                                                                         \
     _Pragma("unroll") for (int c = 0; c < AJ_ASL; c++) RA[c] = *(gv4i*)(asrc[c] + (size_t)(st) * ASTEP); \
-    _Pragma("unroll") for (int c = 0; c < AJ_BSL; c++) RB[c] = *(gv4i*)(bsrc[c] + (size_t)(st) * AJ_BK); \
+    _Pragma("unroll") for (int c = 0; c < AJ_BSL; c++) RB[c] = *(gv4i*)(bsrc[c] + (size_t)(st) * bstep); \
   }
 #define AJ_LSTORE(RA, RB, buf)                                                              \
   {                                                                                         \
@@ -278,6 +290,23 @@ __global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restr
       }
 }
 
+// Digits to K-step-major order for large m: Bt[(step * R + r) * 64 + b] = Z_w[t][step * 64 + b]
+// with r = w * 54 + t and R = nw * 54 rows.  Block (row tile of 64, run of KM_STEPS steps): each
+// lane moves 16 bytes per step; per step the block writes 64 consecutive rows (4 KB contiguous)
+// and reads 64 bytes of each row, whose 128-byte line the next step uses.
+constexpr int KM_STEPS = 32;
+__global__ __launch_bounds__(256) void k_digits_kmajor(const int8_t* const* __restrict__ Zs, size_t nw, size_t m,
+                                                       int8_t* __restrict__ Bt) {
+  const size_t R = nw * D, nsteps = m / 64;
+  const size_t r = (size_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int q = (threadIdx.x & 3) * 16;
+  if (r >= R) return;
+  const int8_t* src = Zs[r / D] + (r % D) * m + q;
+  const size_t s0 = (size_t)blockIdx.y * KM_STEPS, s1 = min(s0 + KM_STEPS, nsteps);
+  for (size_t st = s0; st < s1; st++)
+    *(v4i*)(Bt + (st * R + r) * 64 + q) = *(const v4i*)(src + st * 64);
+}
+
 __device__ __forceinline__ int sum4_i8a(int x) {
   return (x << 24 >> 24) + (x << 16 >> 24) + (x << 8 >> 24) + (x >> 24);
 }
@@ -349,8 +378,16 @@ void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* con
   int32_t* part = (int32_t*)workspace;
   int32_t* S = part + splits * (size_t)AJ_MROWS * npad;
   hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
+  int8_t* Bt = nullptr;
+  if (aj_kmajor(m)) {
+    // after the split-sum scratch (which follows S), 256-byte aligned
+    const size_t used = (splits + (splits > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
+    Bt = (int8_t*)(((uintptr_t)workspace + used + 255) & ~(uintptr_t)255);
+    hipLaunchKernelGGL(k_digits_kmajor, dim3((unsigned)((nw * D + 63) / 64), (unsigned)((nsteps + KM_STEPS - 1) / KM_STEPS)),
+                       dim3(256), 0, s, Zs_dev, nw, m, Bt);
+  }
   NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)(AJ_MBLK * groups * splits)), dim3(AJ_THREADS), 0, s,
-                  limbs, nsteps, Zs_dev, nw, m, npad, (int)groups, (int)splits, part);
+                  limbs, nsteps, Zs_dev, Bt, nw, m, npad, (int)groups, (int)splits, part);
   const int32_t* G = part;
   if (splits > 1) {
     // reduce the K splits with coalesced loads before the anti-diagonal epilogue

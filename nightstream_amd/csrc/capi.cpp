@@ -185,6 +185,24 @@ int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit,
   });
 }
 
+int ns_ajtai_commit_batch_device(const ns_pp* pp, const int8_t* const* Z_dev, size_t n_wit, uint64_t* out) {
+  return guard([&] {
+    require_device();
+    if (!pp) throw Error(1, "null PP");
+    if (n_wit && (!Z_dev || !out)) throw Error(1, "null witness array or output");
+    for (size_t i = 0; i < n_wit; i++)
+      if (!Z_dev[i]) throw Error(1, "null device witness");
+    const size_t m = pp->pp->m;
+    hipStream_t s = default_stream();
+    DevBuf ptrs_d(n_wit * sizeof(void*)), outd(n_wit * KAPPA * D * 8), ws(ajtai_workspace_bytes(m, n_wit));
+    NS_CHECK(hipMemcpyAsync(ptrs_d.p, Z_dev, n_wit * sizeof(void*), hipMemcpyHostToDevice, s));
+    ajtai_commit(pp->pp->limbs.as<int8_t>(), KAPPA, m, ptrs_d.as<const int8_t*>(), n_wit, outd.as<uint64_t>(), ws.p,
+                 s);
+    NS_CHECK(hipMemcpyAsync(out, outd.p, n_wit * KAPPA * D * 8, hipMemcpyDeviceToHost, s));
+    NS_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 static std::vector<K2> kvec(const uint64_t* p, size_t n) {
   std::vector<K2> v(n);
   for (size_t i = 0; i < n; i++) v[i] = K2{p[2 * i], p[2 * i + 1]};

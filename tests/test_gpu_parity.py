@@ -50,6 +50,31 @@ def test_commit_matches_oracle(ns, oracle, m, nw):
         assert np.array_equal(got[w], oracle.ajtai_commit(pp_host, 16, m, Zs[w])), f"witness {w}"
 
 
+@pytest.mark.parametrize("logm", [16, 18])
+def test_commit_device_resident(ns, oracle, logm):
+    """ns_ajtai_commit_batch_device on digit matrices already in HBM equals the host-input entry
+    point and, with PP sampled on the GPU, the oracle on a dense witness; a witness whose digits
+    sit in 64 random columns equals the oracle on those PP columns (a commitment is a sum over
+    columns).  2^18 takes the K-step-major digit path."""
+    rng = np.random.default_rng(31 + logm)
+    m = 1 << logm
+    print(f"device-resident commit m=2^{logm} nw=5", flush=True)
+    pp = ns.PublicParams(m, seed=bytes([3] * 32), keep_coeffs=True)
+    Zs = [rng.integers(-1, 2, size=(D, m), dtype=np.int8) for _ in range(5)]
+    bufs = [ns.DeviceBuffer.from_array(z.reshape(-1)) for z in Zs]
+    got5 = pp.commit_device(bufs)
+    assert np.array_equal(got5, pp.commit(Zs))
+    ppc = pp.coeffs()
+    assert np.array_equal(got5[3], oracle.ajtai_commit(ppc, 16, m, Zs[3]))
+    cols = np.sort(rng.choice(m, size=64, replace=False))
+    zsub = rng.integers(-1, 2, size=(D, 64), dtype=np.int8)
+    z = np.zeros((D, m), np.int8)
+    z[:, cols] = zsub
+    got = pp.commit_device([ns.DeviceBuffer.from_array(z.reshape(-1))])[0]
+    ppc = ppc.reshape(16, m, D)
+    assert np.array_equal(got, oracle.ajtai_commit(np.ascontiguousarray(ppc[:, cols, :]).reshape(-1), 16, 64, zsub))
+
+
 def test_commit_extreme_digits(ns, oracle):
     m = 1024
     seed = bytes([9] * 32)
