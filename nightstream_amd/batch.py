@@ -3,13 +3,14 @@
 A shard's fold steps are sequential (each step's accumulator feeds the next), and each step's
 Fiat-Shamir transcript is host work of roughly constant size (about 17.8k Poseidon2
 permutations, independent of n).  Throughput across shards therefore comes from running many
-sessions at once: every worker thread owns one prover context (its own HIP stream and device
-buffers, reused from job to job through `rebind`), drives its shard's transcript on its own core,
-and its kernels overlap with the other workers' on the GPU.  ctypes releases the GIL for the
+sessions at once: a running job holds one prover context (its own HIP stream and device buffers,
+kept in a pool and reused from job to job through `rebind`), drives its shard's transcript on its
+own core, and its kernels overlap with the other jobs' on the GPU.  ctypes releases the GIL for the
 whole `prove` call, so the Python layer only dispatches.
 
   prove_sequential(shards)          the plain path: one shard at a time, one prover context
   prove_batched(shards, workers)    the same shards through a pool of `workers` contexts
+  ProverPool(workers).warm(shards)  contexts created and sized up front, reusable across calls
   serve(jobs, rate, ...)            open-loop Poisson arrivals with continuous admission by slot
                                     and by estimated HBM footprint; per-job latency
 Every proof is the same byte string whichever path produced it (each shard has its own
@@ -116,33 +117,57 @@ class ShardFactory:
 
 
 class ProverPool:
-    """`workers` prover contexts, each driven by its own thread."""
+    """Up to `workers` prover contexts kept across jobs and calls: a job takes an idle context (or
+    creates one while fewer than `workers` exist) and returns it when done, so a warmed pool proves
+    without creating HIP streams or growing device buffers (both serialize in the HIP runtime)."""
 
     def __init__(self, workers: int):
         self.workers = workers
-        self._local = threading.local()
-        self._all = []
+        self._idle = []
         self._lock = threading.Lock()
+        self.created = 0
 
-    def session(self, shard: Shard):
+    def _acquire(self, shard: Shard):
         import nightstream_amd as ns
-        s = getattr(self._local, "s", None)
+        with self._lock:
+            s = self._idle.pop() if self._idle else None
+            if s is None:
+                self.created += 1
         if s is None:
-            s = ns.FoldingSession(shard.ccs, shard.pp)
-            self._local.s = s
-            with self._lock:
-                self._all.append(s)
-        else:
-            s.rebind(shard.ccs, shard.pp)
+            return ns.FoldingSession(shard.ccs, shard.pp)
+        s.rebind(shard.ccs, shard.pp)
         return s
 
     def prove(self, shard: Shard) -> bytes:
-        return self.session(shard).prove_device(shard.z_ptrs())
+        s = self._acquire(shard)
+        try:
+            return s.prove_device(shard.z_ptrs())
+        finally:
+            with self._lock:
+                self._idle.append(s)
+
+    def warm(self, shards):
+        """Create every context and grow its buffers for the largest shard of `shards` (most
+        rows, then most steps): `workers` concurrent proofs of it, so each takes its own context."""
+        big = max(shards, key=lambda sh: (sh.n, sh.steps))
+        steps = max(sh.steps for sh in shards if sh.n == big.n)
+        warm = Shard(big.n, steps, big.seed, big.ccs, big.pp, big.z_dev)
+        sessions = [self._acquire(warm) for _ in range(self.workers - len(self._idle))]
+        with self._lock:
+            sessions += self._idle
+            self._idle = []
+        threads = [threading.Thread(target=s.prove_device, args=(warm.z_ptrs(),)) for s in sessions]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        with self._lock:
+            self._idle.extend(sessions)
 
 
-def prove_sequential(shards):
+def prove_sequential(shards, pool: ProverPool | None = None):
     """The plain per-shard path: one context, one shard at a time."""
-    pool = ProverPool(1)
+    pool = pool or ProverPool(1)
     out = []
     t0 = time.perf_counter()
     for sh in shards:
@@ -150,10 +175,10 @@ def prove_sequential(shards):
     return out, time.perf_counter() - t0
 
 
-def prove_batched(shards, workers: int):
+def prove_batched(shards, workers: int, pool: ProverPool | None = None):
     """All shards through `workers` concurrent contexts (largest first to balance the tail)."""
     import concurrent.futures as cf
-    pool = ProverPool(workers)
+    pool = pool or ProverPool(workers)
     order = sorted(range(len(shards)), key=lambda i: -shards[i].n * shards[i].steps)
     out = [None] * len(shards)
     t0 = time.perf_counter()
@@ -179,7 +204,8 @@ class JobResult:
         return self.finish - self.arrival
 
 
-def serve(jobs, rate: float, duration: float, workers: int, hbm_cap: float, seed: int = 0, pp_resident: float = 0.0):
+def serve(jobs, rate: float, duration: float, workers: int, hbm_cap: float, seed: int = 0, pp_resident: float = 0.0,
+          pool: ProverPool | None = None):
     """Open-loop Poisson arrivals at `rate` jobs/s for `duration` seconds, drawing jobs in order
     from `jobs` (a list of Shard, cycled).  A job starts when a worker is free and the summed
     footprint of running jobs plus its own stays within `hbm_cap` bytes (continuous admission:
@@ -193,7 +219,7 @@ def serve(jobs, rate: float, duration: float, workers: int, hbm_cap: float, seed
         if t > duration:
             break
         arrivals.append(t)
-    pool = ProverPool(workers)
+    pool = pool or ProverPool(workers)
     pending = queue.Queue()
     results = []
     res_lock = threading.Lock()

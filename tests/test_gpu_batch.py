@@ -29,6 +29,21 @@ def test_batched_equals_plain(shards):
     assert len(set(seq)) == len(seq)
 
 
+def test_warm_pool_reused(shards):
+    """A warmed pool (contexts created and sized up front) serves two batched runs without
+    creating more contexts, and its proofs equal the plain path's."""
+    from nightstream_amd.batch import ProverPool, prove_batched, prove_sequential
+    print("warmed pool of three contexts, two runs", flush=True)
+    seq, _ = prove_sequential(shards)
+    pool = ProverPool(3)
+    pool.warm(shards)
+    assert pool.created == 3
+    for _ in range(2):
+        bat, _ = prove_batched(shards, 3, pool)
+        assert bat == seq
+    assert pool.created == 3
+
+
 def test_rebind_reuses_buffers(shards):
     import nightstream_amd as ns
     a, b = shards[1], shards[0]
