@@ -13,6 +13,8 @@
 // so the NC part of a pair needs only signed selections of wg (no products).
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "kernels.h"
 
 namespace ns {
@@ -485,9 +487,10 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
   // round 1 splits each (pair, row group) item into two halves of the 20 digit monomials
   const int H = MODE == 1 ? 2 : 1;
   const size_t total = pairs * ((size_t)G * H + 1);
+  const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-    const size_t t = idx % pairs;
-    const int gh = (int)(idx / pairs);
+    const size_t t = idx & (pairs - 1);
+    const int gh = (int)(idx >> lp);
     const int g = gh / H, h = gh % H;
     const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
     const K2 e0 = A.eqb[2 * t];
@@ -748,6 +751,7 @@ size_t round_ws_bytes() { return (size_t)RMAXBLK * 5 * sizeof(K2); }
 
 void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool round0, K2* out, void* ws,
                        hipStream_t s) {
+  if (T.len < 2 || (T.len & (T.len - 1))) throw std::invalid_argument("round: table length must be a power of two >= 2");
   RoundArgs A;
   A.len = T.len;
   A.eqb = T.eqb;
@@ -813,9 +817,10 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
   for (int d = 0; d < 5; d++) c[d] = kzero();
   K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
   const size_t total = pairs * ((size_t)G + 1 + A.nsc);
+  const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-    const size_t t = idx % pairs;
-    const int g = (int)(idx / pairs);
+    const size_t t = idx & (pairs - 1);
+    const int g = (int)(idx >> lp);
     const K2* eq = A.sc_src + 4 * t;
     const K2 elo = fold2(eq[0], eq[1], r, r7), ehi = fold2(eq[2], eq[3], r, r7);
     const K2 e0 = elo, e1 = ksub(ehi, elo);
@@ -950,6 +955,7 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
 
 void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s) {
   const size_t pairs = A.len / 2, nrow = (size_t)A.k * D;
+  if (pairs == 0 || (pairs & (pairs - 1))) throw std::invalid_argument("fused round: table length must be a power of two >= 2");
   const size_t target = (size_t)RB * 512;
   size_t G = 1;
   while (G < nrow && pairs * G < target) G *= 2;

@@ -155,6 +155,23 @@ static inline uint64_t red(u128 x) {
   return t2 + ((uint64_t)0 - (uint64_t)c & EPS);
 }
 
+// A congruent u64 of a 128-bit sum whose high word is below 2^31 (sums of a few u64 values):
+// lo + hi (2^32 - 1), one carry correction (the wrapped value is below 2^63, so adding 2^32 - 1
+// cannot carry again).
+static inline uint64_t red_small(u128 x) {
+  const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+  uint64_t s;
+  const bool c = __builtin_add_overflow(lo, (hi << 32) - hi, &s);
+  return s + ((uint64_t)0 - (uint64_t)c & EPS);
+}
+
+// a + rc for a < 2^64 and rc < q: a + rc < 2^65 - 2^32, so one correction gives a congruent u64
+static inline uint64_t add_rc(uint64_t a, uint64_t rc) {
+  uint64_t s;
+  const bool c = __builtin_add_overflow(a, rc, &s);
+  return s + ((uint64_t)0 - (uint64_t)c & EPS);
+}
+
 static inline uint64_t sbox7(uint64_t x) {
   uint64_t x2 = red((u128)x * x);
   uint64_t x3 = red((u128)x2 * x);
@@ -180,19 +197,19 @@ static inline void ext_layer(uint64_t* st) {
     sums[k] = 0;
     for (int c = 0; c < W; c += 4) sums[k] += y[c + k];
   }
-  for (int i = 0; i < W; i++) st[i] = red(y[i] + sums[i & 3]);
+  for (int i = 0; i < W; i++) st[i] = red_small(y[i] + sums[i & 3]);  // below 35 * 2^64
 }
 
 template <int W>
 static inline void permute_w(const Poseidon2& P, uint64_t* st) {
   ext_layer<W>(st);
   for (int r = 0; r < 4; r++) {
-    for (int i = 0; i < W; i++) st[i] = sbox7(red((u128)st[i] + P.ext_init[r][i]));
+    for (int i = 0; i < W; i++) st[i] = sbox7(add_rc(st[i], P.ext_init[r][i]));
     ext_layer<W>(st);
   }
   // Internal rounds: s0' = x7 d0 + sum = x7 (d0 + 1) + rest with rest = sum_{i>0} s_i, so lane 0
   // does not wait for sum; x7 (d0 + 1) = (x^2 * x (d0 + 1)) * x^4 keeps it three multiplies deep.
-  uint64_t x = red((u128)st[0] + P.internal_rc[0]);
+  uint64_t x = add_rc(st[0], P.internal_rc[0]);
   for (int r = 0; r < 22; r++) {
     u128 rest = 0;
     for (int i = 1; i < W; i++) rest += st[i];
@@ -204,13 +221,13 @@ static inline void permute_w(const Poseidon2& P, uint64_t* st) {
     const uint64_t x7 = red((u128)x3 * x4);
     u128 nx = (u128)red((u128)x3c * x4) + rest;
     if (r < 21) nx += P.internal_rc[r + 1];
-    const uint64_t sum = red(rest + x7);
+    const uint64_t sum = red_small(rest + x7);  // below 16 * 2^64
     for (int i = 1; i < W; i++) st[i] = red((u128)st[i] * P.diag[i] + sum);
-    x = red(nx);
+    x = red_small(nx);  // below 18 * 2^64
   }
   st[0] = x;
   for (int r = 0; r < 4; r++) {
-    for (int i = 0; i < W; i++) st[i] = sbox7(red((u128)st[i] + P.ext_term[r][i]));
+    for (int i = 0; i < W; i++) st[i] = sbox7(add_rc(st[i], P.ext_term[r][i]));
     ext_layer<W>(st);
   }
   for (int i = 0; i < W; i++) st[i] = st[i] >= Q ? st[i] - Q : st[i];
