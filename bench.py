@@ -200,7 +200,8 @@ def main():
     roofline = {
         "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
         "frac": round(achieved / peak, 5), "traffic": traffic,
-        "kernel": KERNEL_NAMES[cls], "launches": int(counts[dom]), "avg_ms": round(avg_ms, 5),
+        "kernel": "+".join((KERNEL_NAMES[cls],) if isinstance(KERNEL_NAMES[cls], str) else KERNEL_NAMES[cls]),
+        "launches": int(counts[dom]), "avg_ms": round(avg_ms, 5),
         "per_launch_algorithmic": per_launch,
         "kernel_ms": {KT[i]: round(ms[i], 4) for i in range(len(KT))},
     }

@@ -802,6 +802,117 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
 // launch_round_eval.
 __device__ __forceinline__ K2 fold2(K2 lo, K2 hi, K2 r, uint64_t r7) { return kadd(lo, kmul_w7(r, r7, ksub(hi, lo))); }
 
+// scalar table q of pair t folded at 2t, 2t+1 for the next round (q = 0 is the eq table, already
+// folded into elo / ehi by the caller)
+__device__ __forceinline__ void fused_fold_scalar(const FusedArgs& A, size_t t, int q, K2 elo, K2 ehi) {
+  const size_t L2 = 2 * A.len;
+  K2 lo = elo, hi = ehi;
+  if (q > 0) {
+    const K2* src = A.sc_src + (size_t)q * L2 + 4 * t;
+    lo = fold2(src[0], src[1], A.r, A.r7);
+    hi = fold2(src[2], src[3], A.r, A.r7);
+  }
+  K2* dst = A.sc_dst + (size_t)q * A.len + 2 * t;
+  dst[0] = lo;
+  dst[1] = hi;
+}
+
+// f(Mz)(X) of pair t added to in[0..4] and its Eval term to ev0..ev2, on the f-variable and Eval
+// tables folded in registers: every load is issued at once, and a variable is picked by a
+// register select (a runtime-indexed array would live in scratch memory)
+__device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&in)[5], K2& ev0, K2& ev1, K2& ev2) {
+  const size_t L2 = 2 * A.len;
+  const K2 r = A.r;
+  const uint64_t r7 = A.r7;
+  constexpr int NFV = FUSED_MAX_SC - 1;  // nsc = 1 + nf (+ 2 Eval tables) <= FUSED_MAX_SC
+  K2 flo[NFV], fhi[NFV];
+#pragma unroll
+  for (int q = 0; q < NFV; q++) {
+    flo[q] = fhi[q] = kzero();
+    if (q < A.nf) {
+      const K2* src = A.sc_src + (size_t)(1 + q) * L2 + 4 * t;
+      flo[q] = fold2(src[0], src[1], r, r7);
+      fhi[q] = fold2(src[2], src[3], r, r7);
+    }
+  }
+  K2 rl = kzero(), rh = kzero(), vl = kzero(), vh = kzero();
+  if (A.has_eval) {
+    const K2* sr = A.sc_src + (size_t)(1 + A.nf) * L2 + 4 * t;
+    const K2* sv = sr + L2;
+    rl = fold2(sr[0], sr[1], r, r7);
+    rh = fold2(sr[2], sr[3], r, r7);
+    vl = fold2(sv[0], sv[1], r, r7);
+    vh = fold2(sv[2], sv[3], r, r7);
+  }
+  for (int tt = 0; tt < A.f.nterms; tt++) {
+    K2 p[5];
+    p[0] = K2{A.f.coeff[tt], 0};
+#pragma unroll
+    for (int d = 1; d < 5; d++) p[d] = kzero();
+    for (int v = 0; v < A.f.nv[tt]; v++) {
+      const int q = A.f.var[tt][v];
+      K2 lo = flo[0], hi = fhi[0];
+#pragma unroll
+      for (int qq = 1; qq < NFV; qq++)
+        if (q == qq) {
+          lo = flo[qq];
+          hi = fhi[qq];
+        }
+      const K2 a = lo, b = ksub(hi, lo);
+      for (int e = 0; e < A.f.exp[tt][v]; e++) {
+#pragma unroll
+        for (int d = 4; d >= 1; d--) p[d] = kadd(kmul(a, p[d]), kmul(b, p[d - 1]));
+        p[0] = kmul(a, p[0]);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
+  }
+  if (A.has_eval) {
+    const K2 r0_ = rl, r1_ = ksub(rh, rl);
+    const K2 v0 = vl, v1 = ksub(vh, vl);
+    ev0 = kadd(ev0, kmul(r0_, v0));
+    ev1 = kadd(ev1, kadd(kmul(r0_, v1), kmul(r1_, v0)));
+    ev2 = kadd(ev2, kmul(r1_, v1));
+  }
+}
+
+// NC rows r_lo, r_lo + step, ... < r_hi of pair t: folded into the next tables, and the four round
+// terms of their sum written to in[0..3] (lazy accumulators, one reduction per term)
+__device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __restrict__ wg, size_t t, int r_lo,
+                                              int r_hi, int step, K2 (&in)[5]) {
+  const size_t L2 = 2 * A.len;
+  const K2 r = A.r;
+  const uint64_t r7 = A.r7;
+  KS2 z1 = ks2_zero(), z3 = ks2_zero(), z5 = ks2_zero(), z6 = ks2_zero();
+  KS1 z2 = ks1_zero(), z4 = ks1_zero();
+  for (int row = r_lo; row < r_hi; row += step) {
+    const K2* src = A.nc_src + (size_t)row * L2 + 4 * t;
+    const K2 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
+    const K2 a = fold2(u0, u1, r, r7), hv = fold2(u2, u3, r, r7);
+    K2* dst = A.nc_dst + (size_t)row * A.len + 2 * t;
+    dst[0] = a;
+    dst[1] = hv;
+    const K2 b = ksub(hv, a);
+    const K2 wv = wg[row];
+    const uint64_t w7 = fmul7(wv.c1);
+    const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
+    const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
+    ks2_mac(z1, wa, a2);
+    ks1_add(z2, wa);
+    ks2_mac(z3, wb, a2);
+    ks1_add(z4, wb);
+    ks2_mac(z5, wa, b2);
+    ks2_mac(z6, wb, b2);
+  }
+  const K2 s1 = ks2_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2_reduce(z3), s4 = ks1_reduce(z4),
+           s5 = ks2_reduce(z5), s6 = ks2_reduce(z6);
+  in[0] = ksub(s1, s2);
+  in[1] = ksub(kadd(kadd(s3, s3), s3), s4);
+  in[2] = kadd(kadd(s5, s5), s5);
+  in[3] = s6;
+}
+
 __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
                                                      int G) {
   // row weights come from global memory (11 KB, cache-resident): a short round's block needs a
@@ -810,7 +921,7 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
   const int nrow = A.k * D;
   const K2 r = A.r;
   const uint64_t r7 = A.r7;
-  const size_t pairs = A.len / 2, L2 = 2 * A.len;
+  const size_t pairs = A.len / 2;
   const int R = (nrow + G - 1) / G;
   K2 c[5];
 #pragma unroll
@@ -828,103 +939,14 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
 #pragma unroll
     for (int d = 0; d < 5; d++) in[d] = kzero();
     if (g > G) {
-      // scalar table q = g - G - 1 folded at 2t, 2t+1 for the next round (no round contribution)
-      const int q = g - G - 1;
-      K2 lo = elo, hi = ehi;
-      if (q > 0) {
-        const K2* src = A.sc_src + (size_t)q * L2 + 4 * t;
-        lo = fold2(src[0], src[1], r, r7);
-        hi = fold2(src[2], src[3], r, r7);
-      }
-      K2* dst = A.sc_dst + (size_t)q * A.len + 2 * t;
-      dst[0] = lo;
-      dst[1] = hi;
+      fused_fold_scalar(A, t, g - G - 1, elo, ehi);  // no round contribution
       continue;
     }
     if (g == G) {
-      // f(Mz)(X) and the Eval term on the folded f-variable and Eval tables: every load is issued
-      // at once, and a variable is picked by a register select (a runtime-indexed array would
-      // live in scratch memory)
-      constexpr int NFV = FUSED_MAX_SC - 1;  // nsc = 1 + nf (+ 2 Eval tables) <= FUSED_MAX_SC
-      K2 flo[NFV], fhi[NFV];
-#pragma unroll
-      for (int q = 0; q < NFV; q++) {
-        flo[q] = fhi[q] = kzero();
-        if (q < A.nf) {
-          const K2* src = A.sc_src + (size_t)(1 + q) * L2 + 4 * t;
-          flo[q] = fold2(src[0], src[1], r, r7);
-          fhi[q] = fold2(src[2], src[3], r, r7);
-        }
-      }
-      K2 rl = kzero(), rh = kzero(), vl = kzero(), vh = kzero();
-      if (A.has_eval) {
-        const K2* sr = A.sc_src + (size_t)(1 + A.nf) * L2 + 4 * t;
-        const K2* sv = sr + L2;
-        rl = fold2(sr[0], sr[1], r, r7);
-        rh = fold2(sr[2], sr[3], r, r7);
-        vl = fold2(sv[0], sv[1], r, r7);
-        vh = fold2(sv[2], sv[3], r, r7);
-      }
-      for (int tt = 0; tt < A.f.nterms; tt++) {
-        K2 p[5];
-        p[0] = K2{A.f.coeff[tt], 0};
-#pragma unroll
-        for (int d = 1; d < 5; d++) p[d] = kzero();
-        for (int v = 0; v < A.f.nv[tt]; v++) {
-          const int q = A.f.var[tt][v];
-          K2 lo = flo[0], hi = fhi[0];
-#pragma unroll
-          for (int qq = 1; qq < NFV; qq++)
-            if (q == qq) {
-              lo = flo[qq];
-              hi = fhi[qq];
-            }
-          const K2 a = lo, b = ksub(hi, lo);
-          for (int e = 0; e < A.f.exp[tt][v]; e++) {
-#pragma unroll
-            for (int d = 4; d >= 1; d--) p[d] = kadd(kmul(a, p[d]), kmul(b, p[d - 1]));
-            p[0] = kmul(a, p[0]);
-          }
-        }
-#pragma unroll
-        for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
-      }
-      if (A.has_eval) {
-        const K2 r0_ = rl, r1_ = ksub(rh, rl);
-        const K2 v0 = vl, v1 = ksub(vh, vl);
-        ev0 = kadd(ev0, kmul(r0_, v0));
-        ev1 = kadd(ev1, kadd(kmul(r0_, v1), kmul(r1_, v0)));
-        ev2 = kadd(ev2, kmul(r1_, v1));
-      }
+      fused_f_item(A, t, in, ev0, ev1, ev2);
     } else {
       const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
-      KS2 z1 = ks2_zero(), z3 = ks2_zero(), z5 = ks2_zero(), z6 = ks2_zero();
-      KS1 z2 = ks1_zero(), z4 = ks1_zero();
-      for (int row = r_lo; row < r_hi; row++) {
-        const K2* src = A.nc_src + (size_t)row * L2 + 4 * t;
-        const K2 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
-        const K2 a = fold2(u0, u1, r, r7), hv = fold2(u2, u3, r, r7);
-        K2* dst = A.nc_dst + (size_t)row * A.len + 2 * t;
-        dst[0] = a;
-        dst[1] = hv;
-        const K2 b = ksub(hv, a);
-        const K2 wv = wg[row];
-        const uint64_t w7 = fmul7(wv.c1);
-        const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
-        const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
-        ks2_mac(z1, wa, a2);
-        ks1_add(z2, wa);
-        ks2_mac(z3, wb, a2);
-        ks1_add(z4, wb);
-        ks2_mac(z5, wa, b2);
-        ks2_mac(z6, wb, b2);
-      }
-      const K2 s1 = ks2_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2_reduce(z3), s4 = ks1_reduce(z4),
-               s5 = ks2_reduce(z5), s6 = ks2_reduce(z6);
-      in[0] = ksub(s1, s2);
-      in[1] = ksub(kadd(kadd(s3, s3), s3), s4);
-      in[2] = kadd(kadd(s5, s5), s5);
-      in[3] = s6;
+      fused_nc_rows(A, wg, t, r_lo, r_hi, 1, in);
     }
     const uint64_t e07 = fmul7(e0.c1), e17 = fmul7(e1.c1);
     c[0] = kadd(c[0], kmul_w7(e0, e07, in[0]));
@@ -961,10 +983,10 @@ void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hip
   while (G < nrow && pairs * G < target) G *= 2;
   if (G > nrow) G = nrow;
   G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
+  K2* partials = (K2*)ws;
   size_t nblk = (pairs * (G + 1 + (size_t)A.nsc) + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
-  K2* partials = (K2*)ws;
   NS_LAUNCH_TIMED(KT_FUSED, k_round_fused, dim3((unsigned)nblk), dim3(RB), 0, s, A, wg, partials, (int)G);
   hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
 }

@@ -75,20 +75,27 @@ def traffic_table(fetch: str, write: str, copy_fetch: str, copy_write: str, copy
     return "\n".join(out) + "\n"
 
 
-def load_traffic(md_path: str, kernel: str):
-    """Traffic bytes per launch of `kernel` from a table written by traffic_table, or None."""
+def load_traffic(md_path: str, kernels):
+    """Traffic bytes per launch of a kernel class from a table written by traffic_table, or None.
+    `kernels` is one kernel name or several (a class with more than one kernel): the launch-
+    weighted mean over the rows found."""
+    if isinstance(kernels, str):
+        kernels = (kernels,)
     try:
         text = open(md_path).read()
     except OSError:
         return None
+    launches = total = 0.0
     for line in text.splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == kernel:
+        if len(cells) == 5 and cells[0] in kernels:
             try:
-                return float(cells[4])
+                n, b = float(cells[1]), float(cells[4])
             except ValueError:
                 return None
-    return None
+            launches += n
+            total += n * b
+    return total / launches if launches else None
 
 
 def main(argv=None):
