@@ -1132,8 +1132,8 @@ bool TsOracleGpu::time_round_launch() {
     nb = (unsigned)std::min<size_t>((n_ + 63) / 64, TMAXBLK);
     hipLaunchKernelGGL(k_ts_time_points, dim3(nb, ncoef), dim3(64), 0, st_, A, partials_.as<K2>());
   }
-  hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_dev_.as<K2>());
-  NS_CHECK(hipMemcpyAsync(coef_h_, coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  // the sums go straight to the pinned mailbox (read after the stream synchronizes)
+  hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_h_);
   return true;
 }
 
@@ -1142,13 +1142,12 @@ bool TsOracleGpu::time_round_launch() {
 void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
   if (n_ > 0 && n_ <= (size_t)ONEBLK_N) {
     const uint32_t n = (uint32_t)n_;
-    uint32_t* nout = pos_.as<uint32_t>() + n;
+    uint32_t* nout = (uint32_t*)(coef_h_ + TS_MAX_COEF);  // pinned mailbox, read by finish_fold
     hipLaunchKernelGGL(k_ts_heads_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, idx_[cur_].as<uint32_t>(), n,
                        flags_.as<uint32_t>(), pos_.as<uint32_t>(), chal_dst, r);
     hipLaunchKernelGGL(k_ts_fold, dim3((n + 255) / 256), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(),
                        cols_[cur_].as<K2>(), n, (uint32_t)cap_, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(),
                        r, idx_[cur_ ^ 1].as<uint32_t>(), cols_[cur_ ^ 1].as<K2>(), nout);
-    NS_CHECK(hipMemcpyAsync(coef_h_ + TS_MAX_COEF, nout, 4, hipMemcpyDeviceToHost, st_));
     fold_pending_ = true;
     cur_ ^= 1;
     len_ >>= 1;
@@ -1164,11 +1163,10 @@ void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
     NS_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, flags_.as<uint32_t>(), pos_.as<uint32_t>(), (int)n, st_));
     if (tmp > scan_tmp_.bytes) throw Error(1, "scan workspace too small");
     NS_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp_.p, tmp, flags_.as<uint32_t>(), pos_.as<uint32_t>(), (int)n, st_));
-    uint32_t* nout = pos_.as<uint32_t>() + n;
+    uint32_t* nout = (uint32_t*)(coef_h_ + TS_MAX_COEF);  // pinned mailbox, read by finish_fold
     hipLaunchKernelGGL(k_ts_fold, dim3(g), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(), cols_[cur_].as<K2>(), n,
                        stride, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(), r, idx_[cur_ ^ 1].as<uint32_t>(),
                        cols_[cur_ ^ 1].as<K2>(), nout);
-    NS_CHECK(hipMemcpyAsync(coef_h_ + TS_MAX_COEF, nout, 4, hipMemcpyDeviceToHost, st_));
     fold_pending_ = true;
     cur_ ^= 1;
   }
@@ -1250,8 +1248,8 @@ bool TsOracleGpu::product_round_launch() {
   else
     hipLaunchKernelGGL(k_prod_round<TS_MAX_COEF>, dim3(nb), dim3(TB), 0, st_, fac_[cur_].as<K2>(), (int)nf_, flen_, half,
                        partials_.as<K2>());
-  hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_dev_.as<K2>());
-  NS_CHECK(hipMemcpyAsync(coef_h_, coef_dev_.p, ncoef * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  // the sums go straight to the pinned mailbox (read after the stream synchronizes)
+  hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_h_);
   return true;
 }
 
@@ -1485,8 +1483,7 @@ bool TsOracleGpu::addr_round_launch(bool final) {
   A.it = it_[cur_].as<K2>();
   unsigned nb = nblocks(n);
   hipLaunchKernelGGL(k_addr_round, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
-  hipLaunchKernelGGL(k_ts_sum, dim3(3), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, 3, coef_dev_.as<K2>());
-  NS_CHECK(hipMemcpyAsync(coef_h_, coef_dev_.p, 3 * sizeof(K2), hipMemcpyDeviceToHost, st_));
+  hipLaunchKernelGGL(k_ts_sum, dim3(3), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, 3, coef_h_);
   return true;
 }
 
