@@ -13,6 +13,11 @@
 // Phi_81 = X^54 + X^27 + 1 using X^81 = 1 and X^54 = -X^27 - 1.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "kernels.h"
 
 namespace ns {
@@ -142,9 +147,13 @@ static size_t aj_splits(size_t m, size_t nw) {
 // batch touches over a thousand 2 MB pages, so the digits are first copied to K-step-major order.
 static bool aj_kmajor(size_t m) { return m >= ((size_t)1 << 18); }
 
+// Largest K-split count the tuner tries: at most 16, with at least 16 K steps per split.
+static size_t aj_max_splits(size_t m) { return std::max<size_t>(1, std::min<size_t>(16, (m / AJ_BK) / 16)); }
+
+// sized for the largest split count, so any tuned choice fits
 size_t ajtai_workspace_bytes(size_t m, size_t nw) {
   size_t npad = aj_groups(nw) * AJ_NBLK;
-  size_t sp = aj_splits(m, nw);
+  size_t sp = aj_max_splits(m);
   const size_t bt = aj_kmajor(m) ? nw * D * m + 256 : 0;
   return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512 + bt;
 }
@@ -370,28 +379,75 @@ __global__ __launch_bounds__(512) void k_ajtai_epilogue(const int32_t* __restric
   }
 }
 
+// K-split autotuner: the first commit of each (m, nw) in the process times every split count from
+// 1 to aj_max_splits(m) on its own inputs -- the GEMM and the reduction of its split planes, which
+// grows with the count (twice each, HIP events, the faster kept) -- and caches the fastest; later
+// commits use the cached count.  Every split count gives the same
+// commitments (exact int32 partial sums, at most 2^20 * 128 in magnitude), so the choice only
+// moves time.  A single lock serializes the first measurement of each shape.
+static size_t aj_tuned_splits(const int8_t* limbs, size_t m, const int8_t* const* Zs_dev, const int8_t* Bt, size_t nw,
+                              int32_t* part, int32_t* red, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<std::pair<size_t, size_t>, size_t> chosen;
+  std::lock_guard<std::mutex> g(mu);
+  const auto key = std::make_pair(m, nw);
+  auto it = chosen.find(key);
+  if (it != chosen.end()) return it->second;
+  const size_t groups = aj_groups(nw), npad = groups * AJ_NBLK, nsteps = m / AJ_BK;
+  hipEvent_t e0, e1;
+  NS_CHECK(hipEventCreate(&e0));
+  NS_CHECK(hipEventCreate(&e1));
+  size_t best = aj_splits(m, nw);
+  float best_ms = 1e30f;
+  for (size_t sp = 1; sp <= aj_max_splits(m); sp++) {
+    float ms = 1e30f;
+    for (int rep = 0; rep < 2; rep++) {
+      NS_CHECK(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_ajtai_gemm, dim3((unsigned)(AJ_MBLK * groups * sp)), dim3(AJ_THREADS), 0, s, limbs, nsteps,
+                         Zs_dev, Bt, nw, m, npad, (int)groups, (int)sp, part);
+      if (sp > 1) launch_sum_splits(part, sp, (size_t)AJ_MROWS * npad, red, s);
+      NS_CHECK(hipEventRecord(e1, s));
+      NS_CHECK(hipEventSynchronize(e1));
+      float t = 0;
+      NS_CHECK(hipEventElapsedTime(&t, e0, e1));
+      ms = std::min(ms, t);
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = sp;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  chosen[key] = best;
+  return best;
+}
+
 void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* const* Zs_dev, size_t nw,
                   uint64_t* out, void* workspace, hipStream_t s) {
   if (nw == 0) return;
-  const size_t splits = aj_splits(m, nw), groups = aj_groups(nw), npad = groups * AJ_NBLK;
+  const size_t groups = aj_groups(nw), npad = groups * AJ_NBLK, spmax = aj_max_splits(m);
   const size_t nsteps = m / AJ_BK;
+  // workspace: partial-sum planes for up to spmax splits | S | the split sums | K-major digits
   int32_t* part = (int32_t*)workspace;
-  int32_t* S = part + splits * (size_t)AJ_MROWS * npad;
+  int32_t* S = part + spmax * (size_t)AJ_MROWS * npad;
   hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
   int8_t* Bt = nullptr;
   if (aj_kmajor(m)) {
     // after the split-sum scratch (which follows S), 256-byte aligned
-    const size_t used = (splits + (splits > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
+    const size_t used = (spmax + (spmax > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
     Bt = (int8_t*)(((uintptr_t)workspace + used + 255) & ~(uintptr_t)255);
     hipLaunchKernelGGL(k_digits_kmajor, dim3((unsigned)((nw * D + 63) / 64), (unsigned)((nsteps + KM_STEPS - 1) / KM_STEPS)),
                        dim3(256), 0, s, Zs_dev, nw, m, Bt);
   }
+  // split-sum plane after S, 256-byte aligned
+  int32_t* red = (int32_t*)(((uintptr_t)(S + nw * D) + 255) & ~(uintptr_t)255);
+  const size_t splits = aj_tuned_splits(limbs, m, Zs_dev, Bt, nw, part, red, s);
   NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)(AJ_MBLK * groups * splits)), dim3(AJ_THREADS), 0, s,
                   limbs, nsteps, Zs_dev, Bt, nw, m, npad, (int)groups, (int)splits, part);
   const int32_t* G = part;
   if (splits > 1) {
     // reduce the K splits with coalesced loads before the anti-diagonal epilogue
-    int32_t* red = (int32_t*)(((uintptr_t)(S + nw * D) + 255) & ~(uintptr_t)255);
     launch_sum_splits(part, splits, (size_t)AJ_MROWS * npad, red, s);
     G = red;
   }
