@@ -585,6 +585,56 @@ __global__ void k_addr_gather(const uint32_t* __restrict__ svals, const K2* __re
   out[j] = dval[svals[j]];
 }
 
+// k_addr_keys + the stable sort by bucket + k_addr_gather + the K scan in one workgroup, for at
+// most ONEBLK_N events and ell_addr <= 18: the sort key is (bucket << 13) | event, unique, so the
+// order is (bucket, event) whatever the sort's stability; padding keys are all ones (above every
+// real key, whose bucket is at most 2^ell_addr).  Thread t ends with sorted ranks t*8 .. t*8+7.
+constexpr int AP_EBITS = 13;  // event index bits (ONEBLK_N = 8192)
+__global__ __launch_bounds__(ONEBLK_T) void k_addr_prep_1blk(const K2* __restrict__ hw, const K2* __restrict__ inc,
+                                                             const K2* __restrict__ ww, const uint32_t* __restrict__ wa,
+                                                             uint32_t nev, int bit, uint32_t sentinel, int kbits,
+                                                             uint32_t* __restrict__ skeys, uint32_t* __restrict__ svals,
+                                                             K2* __restrict__ scan) {
+  using Sort = hipcub::BlockRadixSort<uint32_t, ONEBLK_T, ONEBLK_PER>;
+  __shared__ typename Sort::TempStorage tmp;
+  __shared__ K2 wsum[ONEBLK_T / 64];
+  const int nbits = AP_EBITS + kbits;
+  const uint32_t pad = nbits >= 32 ? 0xFFFFFFFFu : ((1u << nbits) - 1), emask = (1u << AP_EBITS) - 1;
+  uint32_t key[ONEBLK_PER];
+#pragma unroll
+  for (int j = 0; j < ONEBLK_PER; j++) {
+    const uint32_t e = threadIdx.x * ONEBLK_PER + j;
+    if (e < nev) {
+      const uint32_t b = !kiszero(hw[e]) ? (bit >= 0 ? (wa[e] >> bit) : 0u) : sentinel;
+      key[j] = (b << AP_EBITS) | e;
+    } else {
+      key[j] = pad;
+    }
+  }
+  Sort(tmp).Sort(key, 0, nbits);
+  K2 loc[ONEBLK_PER];
+  K2 run = kzero();
+#pragma unroll
+  for (int j = 0; j < ONEBLK_PER; j++) {
+    if (threadIdx.x * ONEBLK_PER + j < nev) {
+      const uint32_t e = key[j] & emask;
+      if (!kiszero(hw[e])) run = kadd(run, kmul(kmul(inc[e], hw[e]), ww[e]));
+    }
+    loc[j] = run;
+  }
+  K2 total;
+  const K2 base = block_exclusive<K2, shfl_up_k2, add_k2>(run, kzero(), wsum, total);
+#pragma unroll
+  for (int j = 0; j < ONEBLK_PER; j++) {
+    const uint32_t i = threadIdx.x * ONEBLK_PER + j;
+    if (i < nev) {
+      skeys[i] = key[j] >> AP_EBITS;
+      svals[i] = key[j] & emask;
+      scan[i] = kadd(base, loc[j]);
+    }
+  }
+}
+
 // mem value of bucket k just before event e: it[k] + sum of bucket-k write deltas of earlier events
 __device__ K2 bucket_before(const uint32_t* skeys, const uint32_t* svals, const K2* scan, uint32_t nev, uint32_t k,
                             uint32_t e, const K2* it) {
@@ -1453,16 +1503,22 @@ bool TsOracleGpu::addr_round_launch(bool final) {
   const int bit = final ? -1 : (int)bit_idx_;
   const uint32_t sentinel = final ? 1u : (uint32_t)((ninit_tab_ >> bit_idx_));
   unsigned g = (n + 255) / 256;
-  K2* dval = dsort_.as<K2>() + n;
-  hipLaunchKernelGGL(k_addr_keys, dim3(g), dim3(256), 0, st_, ev_hw_.as<K2>(), ev_inc_.as<K2>(), ev_ww_.as<K2>(),
-                     ev_wa_.as<uint32_t>(), n, bit, sentinel, keys_[0].as<uint32_t>(), vals_[0].as<uint32_t>(), dval);
-  size_t tb = sort_tmp_bytes_;
-  NS_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp_.p, tb, keys_[0].as<uint32_t>(), keys_[1].as<uint32_t>(),
-                                              vals_[0].as<uint32_t>(), vals_[1].as<uint32_t>(), (int)n, 0,
-                                              (int)ell_addr_ + 1, st_));
-  hipLaunchKernelGGL(k_addr_gather, dim3(g), dim3(256), 0, st_, vals_[1].as<uint32_t>(), dval, n, dsort_.as<K2>());
-  tb = sort_tmp_bytes_;
-  NS_CHECK(hipcub::DeviceScan::InclusiveScan(sort_tmp_.p, tb, dsort_.as<K2>(), dscan_.as<K2>(), KAddOp(), (int)n, st_));
+  if (n <= (uint32_t)ONEBLK_N && ell_addr_ + 1 + AP_EBITS <= 32) {
+    hipLaunchKernelGGL(k_addr_prep_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, ev_hw_.as<K2>(), ev_inc_.as<K2>(),
+                       ev_ww_.as<K2>(), ev_wa_.as<uint32_t>(), n, bit, sentinel, (int)ell_addr_ + 1,
+                       keys_[1].as<uint32_t>(), vals_[1].as<uint32_t>(), dscan_.as<K2>());
+  } else {
+    K2* dval = dsort_.as<K2>() + n;
+    hipLaunchKernelGGL(k_addr_keys, dim3(g), dim3(256), 0, st_, ev_hw_.as<K2>(), ev_inc_.as<K2>(), ev_ww_.as<K2>(),
+                       ev_wa_.as<uint32_t>(), n, bit, sentinel, keys_[0].as<uint32_t>(), vals_[0].as<uint32_t>(), dval);
+    size_t tb = sort_tmp_bytes_;
+    NS_CHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp_.p, tb, keys_[0].as<uint32_t>(), keys_[1].as<uint32_t>(),
+                                                vals_[0].as<uint32_t>(), vals_[1].as<uint32_t>(), (int)n, 0,
+                                                (int)ell_addr_ + 1, st_));
+    hipLaunchKernelGGL(k_addr_gather, dim3(g), dim3(256), 0, st_, vals_[1].as<uint32_t>(), dval, n, dsort_.as<K2>());
+    tb = sort_tmp_bytes_;
+    NS_CHECK(hipcub::DeviceScan::InclusiveScan(sort_tmp_.p, tb, dsort_.as<K2>(), dscan_.as<K2>(), KAddOp(), (int)n, st_));
+  }
   AddrArgs A;
   A.nev = n;
   A.is_read = kind_ == TSK_READ_ADDR;
