@@ -111,9 +111,10 @@ def test_twist_sumcheck_chain_large(twist):
         assert g.evals([rand_k(rng)])[0] == claim, f"{kind}: final value"
 
 
-@pytest.mark.parametrize("ell_addr,ell_n", [(5, 12), (16, 12)])
+# (5, 14): 16384 events, above the one-workgroup sort limit of 8192, so the device sort and scan run
+@pytest.mark.parametrize("ell_addr,ell_n", [(5, 12), (16, 12), (5, 14)])
 def test_twist_address_lanes(twist, oracle, ell_addr, ell_n):
-    tr = gen_memory_trace(ell_n, ell_addr, 1.0 if ell_addr == 5 else 0.35, 0.4, seed=3 + ell_addr,
+    tr = gen_memory_trace(ell_n, ell_addr, 1.0 if ell_addr == 5 else 0.35, 0.4, seed=3 + ell_addr + ell_n,
                           zipf_s=0.0 if ell_addr == 5 else 1.1)
     rng = np.random.default_rng(50 + ell_addr)
     r_cycle = [rand_k(rng) for _ in range(ell_n)]
