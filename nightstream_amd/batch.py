@@ -147,10 +147,12 @@ class ProverPool:
                 self._idle.append(s)
 
     def warm(self, shards):
-        """Create every context and grow its buffers for the largest shard of `shards` (most
-        rows, then most steps): `workers` concurrent proofs of it, so each takes its own context."""
+        """Create every context and grow its buffers for the largest shape in `shards`: the most
+        rows and, independently, the most steps (the batched step commitment's workspace grows
+        with the step count, not with the rows), proved `workers` times at once so that each
+        proof takes its own context."""
         big = max(shards, key=lambda sh: (sh.n, sh.steps))
-        steps = max(sh.steps for sh in shards if sh.n == big.n)
+        steps = max(sh.steps for sh in shards)
         warm = Shard(big.n, steps, big.seed, big.ccs, big.pp, big.z_dev)
         sessions = [self._acquire(warm) for _ in range(self.workers - len(self._idle))]
         with self._lock:
