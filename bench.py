@@ -122,9 +122,17 @@ def main():
                     help="shard: one 2^16-row shard (default); batch: 128 shards of 2^10..2^16 rows, plain path vs "
                          "a pool of concurrent prover contexts; serve: Poisson arrivals with continuous admission")
     ap.add_argument("--workers", type=int, default=16, help="concurrent prover contexts (batch / serve)")
+    ap.add_argument("--procs", type=int, default=8,
+                    help="batch: prover processes sharing the GPU, --workers contexts split evenly (1: one process)")
+    ap.add_argument("--proc-hw-queues", type=int, default=1,
+                    help="batch: HIP hardware queues per prover process (0: the runtime's default of 4)")
     ap.add_argument("--build-threads", type=int, default=8, help="twist: host threads building a chunk's oracles")
     ap.add_argument("--ajtai-max-log", type=int, default=20, help="ajtai: largest m = 2^k of the sweep")
     ap.add_argument("--shards", type=int, default=128, help="batch: number of shards")
+    ap.add_argument("--batch-memory", type=int, default=1, choices=[0, 1],
+                    help="batch: every other shard carries the Twist/Shout memory sidecar (1) or none does (0)")
+    ap.add_argument("--serve-memory-frac", type=float, default=0.3,
+                    help="serve: fraction of jobs carrying the Twist/Shout memory sidecar")
     ap.add_argument("--rate-duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--rates", type=str, default="", help="serve: comma-separated arrival rates (jobs/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -250,53 +258,69 @@ def _stderr(msg):
 
 
 def bench_batch(args):
-    """128 independent shards (rows 2^10..2^16 log-uniform, 4-16 fold steps): the plain path (one
-    shard at a time in one prover context) against --workers concurrent contexts; every proof of
-    the batched run must equal the plain run's."""
+    """128 independent shards (rows 2^10..2^16 log-uniform, 4-16 fold steps; every other one
+    carries the memory sidecar of a RISC-V-like trace of its chunk size, proved at every step):
+    the plain path (one shard at a time in one prover context, the sidecar's oracles one at a
+    time) against --workers concurrent contexts with grouped sidecar rounds, spread over --procs
+    prover processes; every proof of the batched run must equal the plain run's."""
     import nightstream_amd as ns
-    from nightstream_amd.batch import ShardFactory, prove_batched, prove_sequential
+    from nightstream_amd.batch import ProcessProver, ProverPool, ShardFactory, build_shards, prove_batched, \
+        prove_sequential
 
     rng = np.random.default_rng(args.seed)
-    fac = ShardFactory(PP_SEED)
+    shape = [(int(rng.integers(10, 17)), int(rng.integers(4, 17))) for _ in range(args.shards)]
+    # up to 4 distinct R1CS instances per size; every other shard carries a memory trace
+    specs = [(1 << e, steps, 1000 + e * 16 + (i % 4), bool(args.batch_memory and i % 2 == 0))
+             for i, (e, steps) in enumerate(shape)]
+    order = sorted(range(len(specs)), key=lambda i: -specs[i][0] * specs[i][1])
+    procs = max(1, args.procs)
     t0 = time.perf_counter()
-    specs = [(int(rng.integers(10, 17)), int(rng.integers(4, 17))) for _ in range(args.shards)]
-    cache = {}
-    shards = []
-    for i, (e, steps) in enumerate(specs):
-        key = (e, i % 4)  # up to 4 distinct R1CS instances per size
-        if key not in cache:
-            cache[key] = fac.make(1 << e, 1, 1000 + e * 16 + (i % 4))
-        base = cache[key]
-        from nightstream_amd.batch import Shard
-        shards.append(Shard(base.n, steps, base.seed, base.ccs, base.pp, base.z_dev))
+    pp = None
+    if procs > 1:  # started before this process touches the GPU; each builds and warms its own
+        pp = ProcessProver(PP_SEED, specs, procs, max(1, args.workers // procs), args.proc_hw_queues)
+    t_procs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    shards = build_shards(ShardFactory(PP_SEED), specs)
     ns.device_synchronize()
     setup = time.perf_counter() - t0
     total_steps = sum(s.steps for s in shards)
-    _stderr(f"batch: {len(shards)} shards, {total_steps} fold steps, {len(cache)} instances, setup {setup:.1f} s")
+    mem_steps = sum(s.steps for s in shards if s.mem is not None)
+    _stderr(f"batch: {len(shards)} shards, {total_steps} fold steps ({mem_steps} with memory), "
+            f"setup {setup:.1f} s, {procs} prover processes ready in {t_procs:.1f} s")
     # both paths keep their contexts (as a running prover does): created and sized for the
     # largest shard before the timed region
-    from nightstream_amd.batch import ProverPool
-    pool1, pooln = ProverPool(1), ProverPool(args.workers)
+    pool1 = ProverPool(1, batched_rounds=False)
     t0 = time.perf_counter()
     pool1.warm(shards)
-    pooln.warm(shards)
+    pooln = None
+    if pp is None:
+        pooln = ProverPool(args.workers)
+        pooln.warm(shards)
     t_warm = time.perf_counter() - t0
     seq, t_seq = prove_sequential(shards, pool1)
     _stderr(f"batch: plain path {t_seq:.2f} s")
-    bat, t_bat = prove_batched(shards, args.workers, pooln)
-    _stderr(f"batch: {args.workers} contexts {t_bat:.2f} s")
+    if pp is None:
+        bat, t_bat = prove_batched(shards, args.workers, pooln)
+        created = pooln.created - args.workers
+    else:
+        bat, t_bat = pp.prove_all(order)
+        created = 0
+        pp.close()
+    _stderr(f"batch: {args.workers} contexts in {procs} processes {t_bat:.2f} s")
     identical = all(a == b for a, b in zip(seq, bat))
     line = {
-        "metric": "fold-steps/s, 128-shard batch (2^10..2^16 rows, 4-16 steps)",
+        "metric": "fold-steps/s, 128-shard batch (2^10..2^16 rows, 4-16 steps, half with Twist/Shout)",
         "value": round(total_steps / t_bat, 3), "unit": "fold-steps/s", "n_gpus": 1,
         "higher_is_better": True, "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "batch", "shards": len(shards), "fold_steps": total_steps, "workers": args.workers,
-                   "sizes_log2": [e for e, _ in specs], "steps": [s for _, s in specs]},
+        "config": {"workload": "batch", "shards": len(shards), "fold_steps": total_steps,
+                   "fold_steps_with_memory_sidecar": mem_steps, "workers": args.workers, "processes": procs,
+                   "hw_queues_per_process": args.proc_hw_queues if procs > 1 else None,
+                   "sizes_log2": [e for e, _ in shape], "steps": [st for _, st in shape]},
         "plain_path_fold_steps_per_s": round(total_steps / t_seq, 3),
         "speedup_vs_plain": round(t_seq / t_bat, 3),
         "seconds": {"plain": round(t_seq, 3), "batched": round(t_bat, 3), "setup": round(setup, 1),
-                    "context_warmup": round(t_warm, 2)},
-        "contexts_created_in_timed_runs": pool1.created + pooln.created - 1 - args.workers,
+                    "context_warmup": round(t_warm, 2), "process_start": round(t_procs, 1)},
+        "contexts_created_in_timed_runs": pool1.created - 1 + created,
         "proofs_bit_identical": identical,
     }
     print(json.dumps(line), flush=True)
@@ -319,12 +343,14 @@ def bench_serve(args):
     for e in range(12, 19):
         for v in range(2):
             base[(e, v)] = fac.make(1 << e, 1, 2000 + e * 4 + v)
+        if args.serve_memory_frac > 0:
+            base[(e, "mem")] = fac.make(1 << e, 1, 2000 + e * 4 + 2, memory=True)
     ns.device_synchronize()
     jobs = []
     for i in range(4096):
         e = int(rng.integers(12, 19))
-        b = base[(e, i % 2)]
-        jobs.append(Shard(b.n, int(rng.integers(1, 33)), b.seed, b.ccs, b.pp, b.z_dev))
+        b = base[(e, "mem")] if rng.random() < args.serve_memory_frac else base[(e, i % 2)]
+        jobs.append(Shard(b.n, int(rng.integers(1, 33)), b.seed, b.ccs, b.pp, b.z_dev, b.mem, b.mem_ell))
     pp_res = float(sum(pp_bytes(1 << e) for e in range(12, 19)))
     _stderr(f"serve: pool of {len(base)} instances ready in {time.perf_counter() - t0:.1f} s")
     # reference proofs (alone) for the first jobs, to compare against the mixed runs
@@ -334,7 +360,7 @@ def bench_serve(args):
     # each server keeps its contexts across rates (as a running server does), created and sized
     # for the largest job before the first rate
     from nightstream_amd.batch import ProverPool
-    pools = {"pooled": ProverPool(args.workers), "plain": ProverPool(1)}
+    pools = {"pooled": ProverPool(args.workers), "plain": ProverPool(1, batched_rounds=False)}
     for p in pools.values():
         p.warm(jobs)
     plain_saturated = False  # once the single-context server's p99 passes 10 s its queue only grows
@@ -370,91 +396,19 @@ def bench_serve(args):
     alone = ProverPool(1)
     same = all(alone.prove(jobs[j]) == p for j, p in solo.items())
     line = {
-        "metric": "served fold-steps/s and p50/p99 job latency (Poisson arrivals, 2^12..2^18 rows, 1-32 steps)",
+        "metric": "served fold-steps/s and p50/p99 job latency (Poisson arrivals, 2^12..2^18 rows, 1-32 steps, "
+                  f"{args.serve_memory_frac:.0%} with Twist/Shout)",
         "value": max((r["pooled"]["fold_steps_per_s"] for r in out if "pooled" in r), default=None),
         "unit": "fold-steps/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": "serve", "workers": args.workers, "rate_duration_s": args.rate_duration,
-                   "hbm_cap_bytes": 240e9},
+                   "hbm_cap_bytes": 240e9, "memory_sidecar_frac": args.serve_memory_frac},
         "rates": out, "mixed_equals_alone": same,
     }
     print(json.dumps(line), flush=True)
 
 
 Q_FIELD = 0xFFFFFFFF00000001
-
-
-def twist_chunk_lanes(seed: int, ell_n: int = 12):
-    """The memory lanes of one config-3 chunk of 2^ell_n cycles: two register-file Twist
-    instances (32 words, one read port each, writes on the first), the RAM Twist (2^16 words,
-    35% of cycles, Zipf(1.1) addresses, 40% writes) and the three Shout lookup lanes
-    (8-bit x 8-bit ADD / XOR / AND tables of 2^16 entries)."""
-    from nightstream_amd.workload import gen_lookup_trace, gen_memory_trace
-    regs_a = gen_memory_trace(ell_n, 5, 1.0, 0.5, seed=seed * 8 + 1)
-    regs_b = gen_memory_trace(ell_n, 5, 1.0, 0.0, seed=seed * 8 + 2)
-    ram = gen_memory_trace(ell_n, 16, 0.35, 0.4, seed=seed * 8 + 3, zipf_s=1.1)
-    return [regs_a, regs_b, ram], gen_lookup_trace(ell_n, seed=seed * 8 + 4)
-
-
-def twist_chunk_oracles(cls, twists, shouts, rng, ell_n: int = 12):
-    """Every Twist/Shout oracle of one chunk (time lanes, address lanes, lookups) as (label,
-    constructor) pairs over class `cls` (the GPU TsOracle or the CPU restatement's)."""
-    def rand_kk(r):
-        return (int(r.integers(0, 1 << 63)) * 2 % Q_FIELD, int(r.integers(0, 1 << 63)) * 2 % Q_FIELD)
-    out = []
-    for i, tr in enumerate(twists):
-        ea = tr["ell_addr"]
-        r_cycle = [rand_kk(rng) for _ in range(ell_n)]
-        r_addr = [rand_kk(rng) for _ in range(ea)]
-        init_at = rand_kk(rng)
-        r_time = [rand_kk(rng) for _ in range(ell_n)]
-        w = [rand_kk(rng) for _ in tr["ra_bits"]]
-        out += [
-            (f"twist{i}/read", lambda tr=tr, a=(r_cycle, r_addr, init_at): cls(
-                "twist_read", a[0], tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["inc"], tr["wa_bits"],
-                a[1], a[2])),
-            (f"twist{i}/write", lambda tr=tr, a=(r_cycle, r_addr, init_at): cls(
-                "twist_write", a[0], tr["has_write"], tr["wv"], tr["inc"], tr["wa_bits"], a[1], a[2])),
-            (f"twist{i}/val_eval", lambda tr=tr, a=(r_addr, r_time): cls(
-                "val_eval", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], a[0], a[1])),
-            (f"twist{i}/total_inc", lambda tr=tr, a=r_addr: cls(
-                "total_inc", ell_n, tr["wa_bits"], tr["has_write"], tr["inc"], a)),
-            (f"twist{i}/bitness", lambda tr=tr, a=(r_cycle, w): cls("bitness", a[0], tr["ra_bits"], a[1])),
-            (f"twist{i}/read_addr", lambda tr=tr, a=r_cycle: cls(
-                "read_addr", tr["init"], a, tr["has_read"], tr["rv"], tr["ra_bits"], tr["has_write"], tr["wa_bits"],
-                tr["inc"])),
-            (f"twist{i}/write_addr", lambda tr=tr, a=r_cycle: cls(
-                "write_addr", tr["init"], a, tr["has_write"], tr["wv"], tr["wa_bits"], tr["inc"])),
-        ]
-    for j, ln in enumerate(shouts):
-        r_cycle = [rand_kk(rng) for _ in range(ell_n)]
-        r_addr = [rand_kk(rng) for _ in range(16)]
-        out += [
-            (f"shout{j}/value", lambda ln=ln, a=r_cycle: cls("shout_value", a, ln["has_lookup"], ln["val"])),
-            (f"shout{j}/index_adapter", lambda ln=ln, a=(r_cycle, r_addr): cls(
-                "index_adapter", a[0], ln["has_lookup"], ln["addr_bits"], a[1])),
-            (f"shout{j}/addr_lookup", lambda ln=ln, a=r_cycle: cls(
-                "addr_lookup", ln["addr_bits"], ln["has_lookup"], ln["table"], a)),
-        ]
-    return out
-
-
-def drive_group(os, chals, batch=None):
-    """All rounds of oracles with the same number of rounds and shared challenges (the round loop
-    of run_batched_sumcheck_prover): with `batch` (the GPU module) every round of the group is one
-    batch_round / batch_fold; without it each oracle is evaluated at 0..deg and folded in turn.
-    Returns per oracle the list of per-round values."""
-    vals = [[] for _ in os]
-    for rd in range(os[0].num_rounds() if os else 0):
-        if batch is not None:
-            for i, y in enumerate(batch.batch_round(os)):
-                vals[i].append(y)
-            batch.batch_fold(os, chals[rd])
-        else:
-            for i, o in enumerate(os):
-                vals[i].append(o.evals([(v, 0) for v in range(o.degree_bound() + 1)]))
-                o.fold(chals[rd])
-    return vals
 
 
 def bench_twist(args):
@@ -467,9 +421,10 @@ def bench_twist(args):
     With the batched driver a chunk's oracles are built on --build-threads host threads."""
     import nightstream_amd as ns
     from nightstream_amd import twist
+    from nightstream_amd.sidecar import chunk_lanes, chunk_oracles, drive_group
     from oracle import pyoracle
     rng_ch = np.random.default_rng(args.seed)
-    chunks = [twist_chunk_lanes(c) for c in range(args.steps)]
+    chunks = [chunk_lanes(c) for c in range(args.steps)]
     chals = [[(int(a) * 2 % Q_FIELD, int(b) * 2 % Q_FIELD) for a, b in rng_ch.integers(0, 1 << 63, size=(40, 2))]
              for _ in range(64)]
 
@@ -477,7 +432,7 @@ def bench_twist(args):
         twists, shouts = chunks[c]
         rng = np.random.default_rng(1000 + c)
         t0 = time.perf_counter()
-        makers = twist_chunk_oracles(cls, twists, shouts, rng)
+        makers = chunk_oracles(cls, twists, shouts, rng)
         if pool is not None:  # the constructors release the GIL; each oracle owns its stream
             made = list(pool.map(lambda lm: (lm[0], lm[1]()), makers))
         else:

@@ -80,6 +80,15 @@ def pp_bytes(m: int) -> int:
     return 6912 * m
 
 
+MEM_MAX_ELL = 16  # memory-sidecar chunks are at most 2^16 cycles
+
+
+def mem_footprint_bytes(ell: int) -> int:
+    """HBM of one chunk's 30 Twist/Shout oracles: at most 4 KiB per cycle (columns, their folded
+    copies, the sorted address entries and the scan scratch) plus the three 2^16-entry tables."""
+    return (4096 << ell) + 3 * 16 * (1 << 16) if ell else 0
+
+
 @dataclasses.dataclass
 class Shard:
     n: int
@@ -88,6 +97,8 @@ class Shard:
     ccs: object = None
     pp: object = None
     z_dev: object = None  # DeviceBuffer holding z (u64[m])
+    mem: object = None  # memory lanes of one chunk (sidecar.chunk_lanes), proved at every step
+    mem_ell: int = 0  # log2 of the chunk's cycles
 
     def z_ptrs(self):
         return [self.z_dev.data_ptr()] * self.steps
@@ -107,22 +118,33 @@ class ShardFactory:
             self.pps[m] = self.ns.PublicParams(m, seed=self.pp_seed)
         return self.pps[m]
 
-    def make(self, n: int, steps: int, seed: int) -> Shard:
+    def make(self, n: int, steps: int, seed: int, memory: bool = False) -> Shard:
+        """A shard of `steps` fold steps over one n-row R1CS chunk; with `memory`, every step also
+        carries the memory sidecar of a RISC-V-like trace of min(n, 2^16) cycles."""
         inst, z = gen_r1cs_fast(n, seed)
         sh = Shard(n, steps, seed)
         sh.ccs = self.ns.Ccs(inst)
         sh.pp = self.pp(n)
         sh.z_dev = self.ns.DeviceBuffer.from_array(z)
+        if memory:
+            from .sidecar import chunk_lanes
+            sh.mem_ell = min(n.bit_length() - 1, MEM_MAX_ELL)
+            sh.mem = chunk_lanes(seed, sh.mem_ell)
         return sh
 
 
 class ProverPool:
     """Up to `workers` prover contexts kept across jobs and calls: a job takes an idle context (or
     creates one while fewer than `workers` exist) and returns it when done, so a warmed pool proves
-    without creating HIP streams or growing device buffers (both serialize in the HIP runtime)."""
+    without creating HIP streams or growing device buffers (both serialize in the HIP runtime).
 
-    def __init__(self, workers: int):
+    A shard with a memory trace proves its sidecar after its CCS steps: one chunk per step, with
+    its randomness bound to the CCS proof; `batched_rounds` picks the grouped round driver (one
+    batch_round / batch_fold per group round) over the plain one (one oracle at a time)."""
+
+    def __init__(self, workers: int, batched_rounds: bool = True):
         self.workers = workers
+        self.batched_rounds = batched_rounds
         self._idle = []
         self._lock = threading.Lock()
         self.created = 0
@@ -141,10 +163,17 @@ class ProverPool:
     def prove(self, shard: Shard) -> bytes:
         s = self._acquire(shard)
         try:
-            return s.prove_device(shard.z_ptrs())
+            proof = s.prove_device(shard.z_ptrs())
         finally:
             with self._lock:
                 self._idle.append(s)
+        if shard.mem is None:
+            return proof
+        from .sidecar import prove_chunk, step_seed
+        parts = [proof]
+        for st in range(shard.steps):
+            parts.append(prove_chunk(shard.mem, shard.mem_ell, step_seed(proof, st), self.batched_rounds))
+        return b"".join(parts)
 
     def warm(self, shards):
         """Create every context and grow its buffers for the largest shape in `shards`: the most
@@ -158,7 +187,15 @@ class ProverPool:
         with self._lock:
             sessions += self._idle
             self._idle = []
-        threads = [threading.Thread(target=s.prove_device, args=(warm.z_ptrs(),)) for s in sessions]
+        mem = max((sh for sh in shards if sh.mem is not None), key=lambda sh: sh.mem_ell, default=None)
+
+        def run(s):
+            s.prove_device(warm.z_ptrs())
+            if mem is not None:  # fills the oracles' block, stream and mailbox caches
+                from .sidecar import prove_chunk
+                prove_chunk(mem.mem, mem.mem_ell, 0, self.batched_rounds)
+
+        threads = [threading.Thread(target=run, args=(s,)) for s in sessions]
         for th in threads:
             th.start()
         for th in threads:
@@ -189,6 +226,110 @@ def prove_batched(shards, workers: int, pool: ProverPool | None = None):
         for f in cf.as_completed(futs):
             out[futs[f]] = f.result()
     return out, time.perf_counter() - t0
+
+
+def build_shards(fac: ShardFactory, specs):
+    """Shards from (n, steps, seed, memory) specs; shards with the same (n, seed, memory) share one
+    CCS, witness and memory trace on the device."""
+    cache, out = {}, []
+    for n, steps, seed, memory in specs:
+        key = (n, seed, bool(memory))
+        if key not in cache:
+            cache[key] = fac.make(n, 1, seed, memory=bool(memory))
+        b = cache[key]
+        out.append(Shard(b.n, steps, b.seed, b.ccs, b.pp, b.z_dev, b.mem, b.mem_ell))
+    return out
+
+
+def _proc_worker(pp_seed, specs, threads, tasks, results):
+    """One prover process: builds the shards of `specs` on its own HIP runtime, warms `threads`
+    contexts, reports ready, then proves shard indices from `tasks` on `threads` threads until a
+    None per thread arrives, putting (index, proof) on `results`."""
+    fac = ShardFactory(pp_seed)
+    shards = build_shards(fac, specs)
+    pool = ProverPool(threads)
+    pool.warm(shards)
+    results.put(("ready", pool.created))
+
+    def loop():
+        while True:
+            i = tasks.get()
+            if i is None:
+                return
+            results.put((i, pool.prove(shards[i])))
+
+    ths = [threading.Thread(target=loop) for _ in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+
+
+class ProcessProver:
+    """`procs` prover processes of `threads` contexts each over the same shard specs.  Each HIP
+    call takes a lock of its process's runtime, so one process with many contexts is bound by the
+    rate of runtime calls (launches, copies, waits) rather than by the GPU; separate processes
+    have separate runtimes and share only the device.  Processes are started with `spawn` (a
+    fresh interpreter: nothing of the parent's HIP state is inherited) and pull shard indices from
+    one queue, so the load balances itself."""
+
+    def __init__(self, pp_seed: bytes, specs, procs: int, threads: int, hw_queues: int = 0):
+        import multiprocessing as mp
+        import os
+        ctx = mp.get_context("spawn")
+        self.n = len(specs)
+        self.procs, self.threads = procs, threads
+        self.tasks, self.results = ctx.Queue(), ctx.Queue()
+        self.ps = [ctx.Process(target=_proc_worker, args=(pp_seed, specs, threads, self.tasks, self.results),
+                               daemon=True) for _ in range(procs)]
+        old = os.environ.get("GPU_MAX_HW_QUEUES")
+        if hw_queues:  # hardware queues per worker process (the children read it at HIP init)
+            os.environ["GPU_MAX_HW_QUEUES"] = str(hw_queues)
+        try:
+            for p in self.ps:
+                p.start()
+        finally:
+            if hw_queues:
+                if old is None:
+                    os.environ.pop("GPU_MAX_HW_QUEUES", None)
+                else:
+                    os.environ["GPU_MAX_HW_QUEUES"] = old
+        self.created = 0
+        for _ in range(procs):
+            tag, created = self._get()
+            if tag != "ready":
+                raise RuntimeError("prover process failed to start")
+            self.created += created
+
+    def _get(self):
+        import queue as q
+        while True:
+            try:
+                return self.results.get(timeout=5.0)
+            except q.Empty:
+                dead = [p.exitcode for p in self.ps if p.exitcode not in (None, 0)]
+                if dead:
+                    raise RuntimeError(f"prover process exited with status {dead[0]}")
+
+    def prove_all(self, order):
+        """Prove every shard (in `order` of submission); returns (proofs by index, seconds)."""
+        out = [None] * self.n
+        t0 = time.perf_counter()
+        for i in order:
+            self.tasks.put(i)
+        for _ in range(len(order)):
+            i, proof = self._get()
+            out[i] = proof
+        return out, time.perf_counter() - t0
+
+    def close(self):
+        for _ in range(self.procs * self.threads):
+            self.tasks.put(None)
+        for p in self.ps:
+            p.join(timeout=60)
+        bad = [p.exitcode for p in self.ps if p.exitcode != 0]
+        if bad:
+            raise RuntimeError(f"prover process exited with status {bad[0]}")
 
 
 @dataclasses.dataclass
@@ -253,7 +394,7 @@ def serve(jobs, rate: float, duration: float, workers: int, hbm_cap: float, seed
         if arr > now:
             time.sleep(arr - now)
         sh = jobs[i % len(jobs)]
-        fp = footprint_bytes(sh.n)
+        fp = footprint_bytes(sh.n) + mem_footprint_bytes(sh.mem_ell if sh.mem is not None else 0)
         with cond:
             # FIFO admission: wait for a slot and for HBM headroom
             while state["free"] == 0 or state["mem"] + fp > hbm_cap:
