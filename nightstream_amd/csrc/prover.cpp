@@ -36,37 +36,32 @@ template <class T>
 static void h2d(void* dst, const T* src, size_t n, hipStream_t s) {
   NS_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
-// Device-to-host copies land in a per-thread pinned staging buffer (a direct DMA, no runtime
-// staging through pageable memory) and are then copied to the caller's memory.
-namespace {
-struct PinnedStage {
-  void* p = nullptr;
-  size_t bytes = 0;
-  void* get(size_t n) {
-    if (n > bytes) {
-      if (p) (void)hipHostFree(p);
-      p = nullptr;
-      size_t want = n < ((size_t)1 << 20) ? ((size_t)1 << 20) : n;
-      NS_CHECK(hipHostMalloc(&p, want, hipHostMallocDefault));
-      bytes = want;
-    }
-    return p;
-  }
-  ~PinnedStage() {
+// Device-to-host copies land in a pinned staging buffer (a direct DMA, no runtime staging
+// through pageable memory) and are then copied to the caller's memory.  A prover context owns
+// its stage (grown on first use, so a warmed context allocates no pinned memory while proving).
+void* PinnedStage::get(size_t n) {
+  if (n > bytes) {
     if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t want = n < ((size_t)1 << 20) ? ((size_t)1 << 20) : n;
+    NS_CHECK(hipHostMalloc(&p, want, hipHostMallocDefault));
+    bytes = want;
   }
-};
-thread_local PinnedStage g_stage;
-}  // namespace
+  return p;
+}
+PinnedStage::~PinnedStage() {
+  if (p) (void)hipHostFree(p);
+}
 
 template <class T>
-static void d2h(T* dst, const void* src, size_t n, hipStream_t s) {
+static void d2h(PinnedStage& stage, T* dst, const void* src, size_t n, hipStream_t s) {
   const size_t bytes = n * sizeof(T);
   if (bytes == 0) return;
-  void* stage = g_stage.get(bytes);
-  NS_CHECK(hipMemcpyAsync(stage, src, bytes, hipMemcpyDeviceToHost, s));
+  void* h = stage.get(bytes);
+  NS_CHECK(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, s));
   NS_CHECK(hipStreamSynchronize(s));
-  memcpy(dst, stage, bytes);
+  memcpy(dst, h, bytes);
 }
 
 // ---------------- kernel timers ----------------
@@ -348,13 +343,14 @@ std::unique_ptr<Pp> Pp::setup_seeded(const uint8_t seed[32], size_t m, bool keep
   hipLaunchKernelGGL(k_count_rejects, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, words.as<uint64_t>(), N,
                      cnt.as<unsigned long long>());
   unsigned long long rej = 0;
-  d2h(&rej, cnt.p, 1, s);
+  PinnedStage stage;
+  d2h(stage, &rej, cnt.p, 1, s);
   if (rej == 0) {
     pp->coeffs = std::move(words);
   } else {
     std::vector<uint64_t> x(draws), out;
     out.reserve(N);
-    d2h(x.data(), words.p, draws, s);
+    d2h(stage, x.data(), words.p, draws, s);
     size_t used = 0;
     while (out.size() < N) {
       if (used == x.size()) {
@@ -807,7 +803,7 @@ std::vector<K2> CcsOracle::y_eval(const std::vector<K2>& r, DevBuf& ws, DevBuf& 
   w_->ydev.ensure(k_ * t * D * sizeof(K2));
   launch_ydigits(w_->Zptrs.as<const int8_t*>(), k_, vjs.as<K2>(), t, m, w_->ydev.as<K2>(), ws.p, st_);
   std::vector<K2> out(k_ * t * D);
-  d2h(out.data(), w_->ydev.p, out.size(), st_);
+  d2h(w_->stage, out.data(), w_->ydev.p, out.size(), st_);
   return out;
 }
 
@@ -870,7 +866,7 @@ void Prover::commit_device(const int8_t* const* Zs_dev, size_t nw, uint64_t* hos
   h2d(ptrs_.p, (const void* const*)Zs_dev, nw, stream_);
   ajtai_commit(pp_->limbs.as<int8_t>(), KAPPA, s_->m, ptrs_.as<const int8_t*>(), nw, cout_.as<uint64_t>(),
                ws_ajtai_.p, stream_);
-  d2h(host_out, cout_.p, nw * KAPPA * D, stream_);
+  d2h(ows_.stage, host_out, cout_.p, nw * KAPPA * D, stream_);
 }
 
 static size_t log2_pow2(size_t n) {
@@ -936,7 +932,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
         x_host[i].assign(z_steps[i], z_steps[i] + m_in);
       } else {
         x_host[i].resize(m_in);
-        if (m_in) d2h(x_host[i].data(), z_steps[i], m_in, st);
+        if (m_in) d2h(ows_.stage, x_host[i].data(), z_steps[i], m_in, st);
       }
       launch_decomp_b2(zsrc, m, Zmcs[i].as<int8_t>(), st);
       if (!z_on_device) NS_CHECK(hipStreamSynchronize(st));
@@ -1160,7 +1156,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       head_.ensure(np * D * m_in);
       launch_head_cols(pl, (int)np, m, (int)m_in, head_.as<int8_t>(), st);
       std::vector<int8_t> tmp(np * D * m_in);
-      d2h(tmp.data(), head_.p, tmp.size(), st);
+      d2h(ows_.stage, tmp.data(), head_.p, tmp.size(), st);
       for (size_t p = 0; p < np; p++)
         for (size_t i = 0; i < (size_t)D * m_in; i++) (*Xs[p])[i] = from_i64(tmp[p * D * m_in + i]);
     };
@@ -1334,7 +1330,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     NS_CHECK(hipMemsetAsync(flags.p, 0, (k_dec + 1) * 4, st));
     launch_dec_split(zmix.as<int32_t>(), (size_t)D * m, (int)k_dec, (int8_t* const*)pptr_dev.p, flags.as<int>(), st);
     std::vector<int> fl(k_dec + 1);
-    d2h(fl.data(), flags.p, k_dec + 1, st);
+    d2h(ows_.stage, fl.data(), flags.p, k_dec + 1, st);
     if (fl[k_dec]) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
     std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
     commit_device(pptr.data(), k_dec, child_c.data());
@@ -1344,7 +1340,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       ws_yd.ensure(ydigits_ws_bytes(k_dec, t, m));
       ydev_.ensure(y_child.size() * sizeof(K2));
       launch_ydigits(pptr_dev.as<const int8_t*>(), k_dec, vjs.as<K2>(), t, m, ydev_.as<K2>(), ws_yd.p, st);
-      d2h(y_child.data(), ydev_.p, y_child.size(), st);
+      d2h(ows_.stage, y_child.data(), ydev_.p, y_child.size(), st);
     }
     std::vector<MeHost> children(k_dec);
     {

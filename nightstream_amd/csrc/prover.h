@@ -118,9 +118,21 @@ struct HostMailbox {
   }
 };
 
+// Pinned staging for device-to-host copies, grown on demand (at least 1 MiB).
+struct PinnedStage {
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinnedStage() = default;
+  PinnedStage(const PinnedStage&) = delete;
+  PinnedStage& operator=(const PinnedStage&) = delete;
+  ~PinnedStage();
+  void* get(size_t n);
+};
+
 struct OracleWs {
   DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc;
-  HostMailbox mbox;  // 8 K2: the round polynomial's coefficients
+  HostMailbox mbox;   // 8 K2: the round polynomial's coefficients
+  PinnedStage stage;  // the context's device-to-host staging
 };
 
 struct StepTiming {
