@@ -87,7 +87,7 @@ class TsOracleGpu {
   void launch_round();
   void finish_round(std::vector<K2>& c);
   void round_coeffs(std::vector<K2>& c);
-  void launch_fold(K2 r);
+  void launch_fold(K2 r, bool defer = false);
   void finish_fold();
 
  private:
@@ -105,6 +105,8 @@ class TsOracleGpu {
   bool round_dev_ = false;   // the launched round copies them from the device
   bool round_points_ = false;  // ... as values at 0 .. round_ncoef_-1 rather than coefficients
   bool fold_pending_ = false;  // a launched time-lane fold has not reported its support size
+  bool defer_fold_ = false;    // ... and the next round may read that size on the device
+  bool n_on_device_ = false;   // the launched round reads the support size from nslot_
   std::vector<K2> claim_round_;     // round-0 coefficients computed for the claim, reused once
   bool claim_round_valid_ = false, round_from_claim_ = false;
   size_t degree_bound_ = 0, ell_n_ = 0, ell_addr_ = 0, bit_idx_ = 0;
@@ -114,6 +116,15 @@ class TsOracleGpu {
   size_t n_ = 0, cap_ = 1, len_ = 0, ncol_ = 0, nbits_ = 0;
   int col_a_ = -1, col_b_ = -1;
   TsBuf idx_[2], cols_[2], flags_, pos_, scan_tmp_, partials_, coef_dev_, consts_;
+  TsBuf nslot_;  // the last time-lane fold's support size (u32), for a round enqueued behind it
+  // pinned blocks holding the constructors' uploads until the stream synchronizes
+  std::vector<std::pair<void*, size_t>> stage_;
+  size_t stage_off_ = 0;
+  void* stage(size_t bytes);
+  void stage_release();
+  void h2d_staged(void* dst, const void* src, size_t bytes);
+  template <class T>
+  void upload(TsBuf& d, const std::vector<T>& h, size_t min_elems = 1);
   int cur_ = 0;
   size_t scan_tmp_bytes_ = 0;
   // Twist val_pre terms (sorted by time) and their exclusive prefix sums

@@ -139,6 +139,7 @@ struct TsTimeArgs {
   const K2* lt_s1;
   uint32_t nterms;
   int ncoef;
+  const uint32_t* n_dev;  // when set, the support size is read here (n is then an upper bound)
 };
 
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint64_t key) {
@@ -360,6 +361,7 @@ struct LinProd {
 // products; bitness, whose summand is not a product of linear factors, runs only here.
 template <int MAXC>
 __global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restrict__ partials) {
+  if (A.n_dev) A.n = *A.n_dev;
   const K2* weights = A.consts + 2 * A.ell + A.nbits;
   K2 acc[MAXC];
 #pragma unroll
@@ -407,6 +409,7 @@ __global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restri
 // arrays); partials[blockIdx.x * ncoef + X] receives the block's sum and the host interpolates
 // the ncoef sums to the round's coefficients.
 __global__ __launch_bounds__(64) void k_ts_time_points(TsTimeArgs A, K2* __restrict__ partials) {
+  if (A.n_dev) A.n = *A.n_dev;
   const K2 X{(uint64_t)blockIdx.y, 0};
   K2 acc = kzero();
   for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < A.n; i += gridDim.x * 64) {
@@ -450,10 +453,11 @@ __device__ __forceinline__ void ts_fold_entry(const uint32_t* __restrict__ idx, 
 
 __global__ void k_ts_fold(const uint32_t* __restrict__ idx, const K2* __restrict__ cols, uint32_t n, uint32_t stride,
                           int ncol, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, K2 r,
-                          uint32_t* __restrict__ nidx, K2* __restrict__ ncols, uint32_t* __restrict__ nout) {
+                          uint32_t* __restrict__ nidx, K2* __restrict__ ncols, uint32_t* __restrict__ nout,
+                          uint32_t* __restrict__ ndev) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (i == n - 1) nout[0] = pos[i] + flags[i];
+  if (i == n - 1) nout[0] = ndev[0] = pos[i] + flags[i];
   if (!flags[i]) return;
   ts_fold_entry(idx, cols, n, stride, ncol, i, pos[i], r, nidx, ncols);
 }
@@ -866,12 +870,40 @@ class TsPool {
     std::lock_guard<std::mutex> g(mu_);
     mboxes_.push_back(m);
   }
+  // pinned host blocks for the constructors' uploads (size classes as for device blocks)
+  void* pinned_take(size_t cap) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = pinned_.find(cap);
+      if (it != pinned_.end()) {
+        void* p = it->second;
+        pinned_.erase(it);
+        pinned_cached_ -= cap;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    NS_CHECK(hipHostMalloc(&p, cap, hipHostMallocDefault));
+    return p;
+  }
+  void pinned_give(void* p, size_t cap) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (pinned_cached_ + cap <= kMaxPinned) {
+        pinned_.emplace(cap, p);
+        pinned_cached_ += cap;
+        return;
+      }
+    }
+    (void)hipHostFree(p);
+  }
 
  private:
   static constexpr size_t kMaxCached = (size_t)4 << 30;
+  static constexpr size_t kMaxPinned = (size_t)1 << 30;
   std::mutex mu_;
-  std::multimap<size_t, void*> blocks_;
-  size_t cached_ = 0;
+  std::multimap<size_t, void*> blocks_, pinned_;
+  size_t cached_ = 0, pinned_cached_ = 0;
   std::vector<hipStream_t> streams_;
   std::vector<K2*> mboxes_;
 };
@@ -908,11 +940,39 @@ struct ColCursor {
   }
 };
 
+// Host-to-device copies of the constructors go through pinned blocks (a DMA copy, no runtime
+// staging through pageable memory); the blocks stay with the oracle until its stream has
+// synchronized, then return to the cache.
+void* TsOracleGpu::stage(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (stage_.empty() || stage_off_ + bytes > stage_.back().second) {
+    const size_t cap = TsPool::size_class(std::max(bytes, (size_t)1 << 20));
+    stage_.emplace_back(TsPool::get().pinned_take(cap), cap);
+    stage_off_ = 0;
+  }
+  void* p = (char*)stage_.back().first + stage_off_;
+  stage_off_ += bytes;
+  return p;
+}
+
+void TsOracleGpu::stage_release() {
+  for (auto& b : stage_) TsPool::get().pinned_give(b.first, b.second);
+  stage_.clear();
+  stage_off_ = 0;
+}
+
+void TsOracleGpu::h2d_staged(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  void* h = stage(bytes);
+  memcpy(h, src, bytes);
+  NS_CHECK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, st_));
+}
+
 template <class T>
-static void upload(TsBuf& d, const std::vector<T>& h, hipStream_t st, size_t min_elems = 1) {
+void TsOracleGpu::upload(TsBuf& d, const std::vector<T>& h, size_t min_elems) {
   size_t n = std::max(h.size(), min_elems);
   d.ensure(n * sizeof(T));
-  if (!h.empty()) NS_CHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  h2d_staged(d.p, h.data(), h.size() * sizeof(T));
 }
 
 TsOracleGpu::TsOracleGpu() {
@@ -926,6 +986,7 @@ TsOracleGpu::~TsOracleGpu() {
     (void)hipStreamSynchronize(st_);
     TsPool::get().put_stream(st_);
   }
+  stage_release();
   if (coef_h_) TsPool::get().put_mailbox(coef_h_);
 }
 
@@ -942,7 +1003,7 @@ void TsOracleGpu::init_product(const std::vector<std::vector<K2>>& factors, size
     if (factors[f].size() != flen_) throw Error(1, "product oracle: all factors must have the same length");
     std::copy(factors[f].begin(), factors[f].end(), flat.begin() + f * flen_);
   }
-  upload(fac_[0], flat, st_);
+  upload(fac_[0], flat);
   fac_[1].ensure(flat.size() * sizeof(K2));
   cur_ = 0;
   degree_bound_ = degree_bound;
@@ -1016,8 +1077,8 @@ void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t 
       vals[c * cap_ + p] = col->val[j];
     }
   }
-  upload(idx_[0], u, st_);
-  upload(cols_[0], vals, st_);
+  upload(idx_[0], u);
+  upload(cols_[0], vals);
   idx_[1].ensure(std::max<size_t>(n_, 1) * 4);
   cols_[1].ensure(vals.size() * sizeof(K2));
   flags_.ensure(std::max<size_t>(n_, 1) * 4 + 8);
@@ -1037,7 +1098,7 @@ void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t 
   } else {
     cst.insert(cst.end(), r_addr_.begin(), r_addr_.end());
   }
-  NS_CHECK(hipMemcpyAsync(consts_.p, cst.data(), cst.size() * sizeof(K2), hipMemcpyHostToDevice, st_));
+  h2d_staged(consts_.p, cst.data(), cst.size() * sizeof(K2));
   switch (kind) {
     case TSK_SHOUT_VALUE: degree_bound_ = 3; break;
     case TSK_INDEX_ADAPTER: degree_bound_ = 2 + nbits_; break;
@@ -1082,9 +1143,9 @@ void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t 
     nterms_ = term_t_h_.size();
     std::vector<K2> pre(nterms_ + 1, kzero());
     for (size_t k = 0; k < nterms_; k++) pre[k + 1] = kadd(pre[k], term_v_h_[k]);
-    upload(term_t_, term_t_h_, st_);
-    upload(term_v_, term_v_h_, st_);
-    upload(term_p_, pre, st_);
+    upload(term_t_, term_t_h_);
+    upload(term_v_, term_v_h_);
+    upload(term_p_, pre);
     // rank table: terms before each time 0..len (4 (len + 1) bytes), for domains up to 2^20
     has_rank_ = ell_n <= 20;
     if (has_rank_) {
@@ -1094,10 +1155,11 @@ void TsOracleGpu::init_time(TsKind kind, const std::vector<K2>& r_cycle, size_t 
         while (k < nterms_ && term_t_h_[k] < t) k++;
         rank[t] = (uint32_t)k;
       }
-      upload(term_rank_, rank, st_);
+      upload(term_rank_, rank);
     }
   }
   NS_CHECK(hipStreamSynchronize(st_));
+  stage_release();
 }
 
 size_t TsOracleGpu::num_rounds() const {
@@ -1147,6 +1209,7 @@ bool TsOracleGpu::time_round_launch() {
   A.term_rank = has_rank_ ? term_rank_.as<uint32_t>() : nullptr;
   A.nterms = (uint32_t)nterms_;
   A.ncoef = ncoef;
+  A.n_dev = n_on_device_ ? nslot_.as<uint32_t>() : nullptr;
   A.lt_s0 = A.lt_s1 = nullptr;
   if ((kind_ == TSK_TWIST_READ || kind_ == TSK_TWIST_WRITE) && nterms_ > 0) {
     const uint32_t nt = (uint32_t)nterms_;
@@ -1190,6 +1253,7 @@ bool TsOracleGpu::time_round_launch() {
 // enqueue a time-lane fold; the folded support size arrives in the mailbox (finish_fold).
 // chal_dst: where the challenge goes in the constants block (null: not stored)
 void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
+  nslot_.ensure(sizeof(uint32_t));
   if (n_ > 0 && n_ <= (size_t)ONEBLK_N) {
     const uint32_t n = (uint32_t)n_;
     uint32_t* nout = (uint32_t*)(coef_h_ + TS_MAX_COEF);  // pinned mailbox, read by finish_fold
@@ -1197,7 +1261,7 @@ void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
                        flags_.as<uint32_t>(), pos_.as<uint32_t>(), chal_dst, r);
     hipLaunchKernelGGL(k_ts_fold, dim3((n + 255) / 256), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(),
                        cols_[cur_].as<K2>(), n, (uint32_t)cap_, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(),
-                       r, idx_[cur_ ^ 1].as<uint32_t>(), cols_[cur_ ^ 1].as<K2>(), nout);
+                       r, idx_[cur_ ^ 1].as<uint32_t>(), cols_[cur_ ^ 1].as<K2>(), nout, nslot_.as<uint32_t>());
     fold_pending_ = true;
     cur_ ^= 1;
     len_ >>= 1;
@@ -1216,7 +1280,7 @@ void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
     uint32_t* nout = (uint32_t*)(coef_h_ + TS_MAX_COEF);  // pinned mailbox, read by finish_fold
     hipLaunchKernelGGL(k_ts_fold, dim3(g), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(), cols_[cur_].as<K2>(), n,
                        stride, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(), r, idx_[cur_ ^ 1].as<uint32_t>(),
-                       cols_[cur_ ^ 1].as<K2>(), nout);
+                       cols_[cur_ ^ 1].as<K2>(), nout, nslot_.as<uint32_t>());
     fold_pending_ = true;
     cur_ ^= 1;
   }
@@ -1333,25 +1397,25 @@ void TsOracleGpu::init_lookup(const std::vector<SparseCol>& addr_bits, const Spa
   flen_ = pow2_addr;
   fac_[0].ensure(2 * pow2_addr * sizeof(K2));
   fac_[1].ensure(2 * pow2_addr * sizeof(K2));
-  NS_CHECK(hipMemcpyAsync(fac_[0].p, tab.data(), pow2_addr * sizeof(K2), hipMemcpyHostToDevice, st_));
+  h2d_staged(fac_[0].p, tab.data(), pow2_addr * sizeof(K2));
   K2* W = fac_[0].as<K2>() + pow2_addr;
   if (n == 0) {
     NS_CHECK(hipMemsetAsync(W, 0, pow2_addr * sizeof(K2), st_));
   } else {
     TsBuf &dts = scratch_[0], &dg = scratch_[1], &dw = scratch_[2], &da = scratch_[3], &dv = scratch_[4],
           &sa = scratch_[5], &sv = scratch_[6], &rc = scratch_[7], &tmp = scratch_[8];
-    upload(dts, ts, st_);
-    upload(dg, gates, st_);
-    upload(da, addr, st_);
+    upload(dts, ts);
+    upload(dg, gates);
+    upload(da, addr);
     std::vector<K2> rcy(r_cycle);
     rcy.resize(std::max<size_t>(ell_n, 1), kzero());
-    upload(rc, rcy, st_);
+    upload(rc, rcy);
     dw.ensure(n * sizeof(K2));
     hipLaunchKernelGGL(k_ts_chi_at, dim3((n + 255) / 256), dim3(256), 0, st_, dts.as<uint32_t>(), n, rc.as<K2>(),
                        (int)ell_n, dg.as<K2>(), dw.as<K2>());
     std::vector<uint32_t> iota(n);
     for (uint32_t i = 0; i < n; i++) iota[i] = i;
-    upload(dv, iota, st_);
+    upload(dv, iota);
     sa.ensure(n * 4);
     sv.ensure(n * 4);
     size_t tb = 0;
@@ -1365,6 +1429,7 @@ void TsOracleGpu::init_lookup(const std::vector<SparseCol>& addr_bits, const Spa
     NS_CHECK(hipStreamSynchronize(st_));
   }
   NS_CHECK(hipStreamSynchronize(st_));
+  stage_release();
   kind_ = TSK_ADDR_LOOKUP;
   degree_bound_ = 2;
   cur_ = 0;
@@ -1450,20 +1515,20 @@ void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs
     }
   }
   const size_t ne = std::max<size_t>(nev_, 1);
-  upload(ev_hr_, hr, st_, 1);
-  upload(ev_rv_, rvv, st_, 1);
-  upload(ev_hw_, hw, st_, 1);
-  upload(ev_inc_, ic, st_, 1);
-  upload(ev_ra_, ra, st_, 1);
-  upload(ev_wa_, wa, st_, 1);
-  upload(ev_rw_, one, st_, 1);
-  upload(ev_ww_, one, st_, 1);
+  upload(ev_hr_, hr, 1);
+  upload(ev_rv_, rvv, 1);
+  upload(ev_hw_, hw, 1);
+  upload(ev_inc_, ic, 1);
+  upload(ev_ra_, ra, 1);
+  upload(ev_wa_, wa, 1);
+  upload(ev_rw_, one, 1);
+  upload(ev_ww_, one, 1);
   ev_eq_.ensure(ne * sizeof(K2));
   TsBuf &dts = scratch_[0], &rc = scratch_[1];
-  upload(dts, times, st_, 1);
+  upload(dts, times, 1);
   std::vector<K2> rcy(r_cycle);
   rcy.resize(std::max<size_t>(ell_n, 1), kzero());
-  upload(rc, rcy, st_);
+  upload(rc, rcy);
   if (nev_)
     hipLaunchKernelGGL(k_ts_chi_at, dim3((unsigned)((nev_ + 255) / 256)), dim3(256), 0, st_, dts.as<uint32_t>(),
                        (uint32_t)nev_, rc.as<K2>(), (int)ell_n, (const K2*)nullptr, ev_eq_.as<K2>());
@@ -1473,7 +1538,7 @@ void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs
     if (init_addrs[i] >= pow2_addr) throw Error(1, "init address out of range");
     it[init_addrs[i]] = kadd(it[init_addrs[i]], init_vals[i]);
   }
-  upload(it_[0], it, st_);
+  upload(it_[0], it);
   it_[1].ensure(pow2_addr * sizeof(K2));
   ninit_tab_ = pow2_addr;
   for (int i = 0; i < 2; i++) {
@@ -1495,6 +1560,7 @@ void TsOracleGpu::init_addr(TsKind kind, const std::vector<uint32_t>& init_addrs
   bit_idx_ = 0;
   degree_bound_ = 2;
   NS_CHECK(hipStreamSynchronize(st_));
+  stage_release();
 }
 
 bool TsOracleGpu::addr_round_launch(bool final) {
@@ -1552,6 +1618,13 @@ void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
 }
 
 void TsOracleGpu::launch_round() {
+  // After a batched fold the round is enqueued without waiting for it: its kernels read the
+  // folded support size on the device (the host's n_ is an upper bound until finish_round).
+  if (fold_pending_ && defer_fold_) {
+    fold_pending_ = false;
+    n_on_device_ = true;
+  }
+  defer_fold_ = false;
   finish_fold();
   round_points_ = false;
   if (claim_round_valid_) {  // the claim already computed this round
@@ -1611,8 +1684,14 @@ void TsOracleGpu::finish_round(std::vector<K2>& c) {
     return;
   }
   c.assign(round_ncoef_, kzero());
-  if (round_dev_) {
+  if (round_dev_ || n_on_device_) {
     NS_CHECK(hipStreamSynchronize(st_));
+    if (n_on_device_) {  // the fold before this round is complete
+      n_ = *(const uint32_t*)(coef_h_ + TS_MAX_COEF);
+      n_on_device_ = false;
+    }
+  }
+  if (round_dev_) {
     if (round_points_)
       coeffs_from_points(coef_h_, round_ncoef_, c.data());
     else
@@ -1682,8 +1761,9 @@ void TsOracleGpu::fold(K2 r) {
 
 // every fold is stream-ordered before the next round's kernels; only a time-lane fold leaves
 // host state (the support size) to collect
-void TsOracleGpu::launch_fold(K2 r) {
+void TsOracleGpu::launch_fold(K2 r, bool defer) {
   finish_fold();
+  defer_fold_ = defer;
   claim_round_valid_ = false;
   if (num_rounds() == 0) return;
   switch (kind_) {
@@ -1747,8 +1827,8 @@ void ts_batch_round(TsOracleGpu* const* os, size_t n, std::vector<std::vector<K2
 void ts_batch_fold(TsOracleGpu* const* os, size_t n, K2 r) {
   for (size_t i = 0; i < n; i++)
     if (!os[i]) throw Error(1, "batched fold: null oracle");
-  for (size_t i = 0; i < n; i++) os[i]->launch_fold(r);
-  for (size_t i = 0; i < n; i++) os[i]->finish_fold();
+  // the folds' support sizes are collected by the next batched round (or by any other use)
+  for (size_t i = 0; i < n; i++) os[i]->launch_fold(r, true);
 }
 
 // ---------------- shared CPU bus (host) ----------------
