@@ -19,6 +19,10 @@ import sys
 
 def short_name(name: str) -> str:
     n = name.strip().strip('"')
+    if "rocprim" in n:  # library kernels: the algorithm's name is enough
+        m = re.search(r"(radix_sort_block_sort|radix_sort|merge_sort_block_merge|merge_sort|scan|transform|"
+                      r"init_lookback)", n)
+        return "rocprim " + (m.group(1) if m else "kernel")
     n = re.sub(r"^void\s+", "", n)
     n = n.replace("ns::", "")
     depth = 0
