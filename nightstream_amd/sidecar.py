@@ -104,25 +104,34 @@ def step_seed(tag: bytes, step: int) -> int:
     return int.from_bytes(hashlib.sha256(tag + step.to_bytes(4, "little")).digest()[:8], "little")
 
 
-def chunk_values(cls, lanes, ell_n: int, seed: int, batch=None) -> bytes:
-    """Build a chunk's 30 oracles over class `cls` (random points from `seed`), drive every group
-    of equal round count through its rounds with challenges from `seed` (grouped rounds through
-    module `batch`, or one oracle at a time when it is None), and return each oracle's claim and
-    every round's values (evaluations at 0..deg, K as two u64 words) as little-endian bytes in
-    oracle order."""
+def chunk_rounds(cls, lanes, ell_n: int, seed: int, batch=None):
+    """Build a chunk's 30 oracles over class `cls` (random points from `seed`) and drive every
+    group of equal round count through its rounds with challenges from `seed` (grouped rounds
+    through module `batch`, or one oracle at a time when it is None).  Returns per oracle
+    (label, claim, [values at 0..deg of each round], [the challenge of each round]); the claim
+    is the constructor's (zero for the Twist read and write checks, whose constructors return
+    none)."""
     twists, shouts = lanes
     rng = np.random.default_rng(seed)
-    made = [mk() for _, mk in chunk_oracles(cls, twists, shouts, rng, ell_n)]
+    labeled = [(lbl, mk()) for lbl, mk in chunk_oracles(cls, twists, shouts, rng, ell_n)]
+    made = [o for _, o in labeled]
     groups = {}
     for i, o in enumerate(made):
         groups.setdefault(o.num_rounds(), []).append(i)
-    vals = [None] * len(made)
+    out = [None] * len(made)
     for nr in sorted(groups):
         idxs = groups[nr]
         chals = [_rand_k(rng) for _ in range(nr)]
         for i, v in zip(idxs, drive_group([made[i] for i in idxs], chals, batch)):
-            vals[i] = v
-    words = [w for o, per in zip(made, vals) for k in [o.claim] + [k for rd in per for k in rd] for w in k]
+            out[i] = (labeled[i][0], made[i].claim, v, chals)
+    return out
+
+
+def chunk_values(cls, lanes, ell_n: int, seed: int, batch=None) -> bytes:
+    """`chunk_rounds` as bytes: each oracle's claim and every round's values (K as two u64
+    words, little-endian) in oracle order."""
+    words = [w for _, claim, per, _ in chunk_rounds(cls, lanes, ell_n, seed, batch)
+             for k in [claim] + [k for rd in per for k in rd] for w in k]
     return np.asarray(words, dtype=np.uint64).tobytes()
 
 
