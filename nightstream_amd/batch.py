@@ -266,12 +266,13 @@ def _proc_worker(pp_seed, specs, threads, tasks, results):
 
 
 class ProcessProver:
-    """`procs` prover processes of `threads` contexts each over the same shard specs.  Each HIP
-    call takes a lock of its process's runtime, so one process with many contexts is bound by the
-    rate of runtime calls (launches, copies, waits) rather than by the GPU; separate processes
-    have separate runtimes and share only the device.  Processes are started with `spawn` (a
-    fresh interpreter: nothing of the parent's HIP state is inherited) and pull shard indices from
-    one queue, so the load balances itself."""
+    """`procs` prover processes of `threads` contexts each over the same shard specs.  One process
+    with many contexts tops out at about 100k GPU dispatches per second (launches and copies of
+    short kernels); separate processes have their own HIP runtime and hardware queues and share
+    only the device.  Give each few hardware queues (`hw_queues`, set in the children's
+    environment): every mapped queue costs the GPU's queue scheduling time even when idle.
+    Processes are started with `spawn` (a fresh interpreter: nothing of the parent's HIP state is
+    inherited) and pull shard indices from one queue, so the load balances itself."""
 
     def __init__(self, pp_seed: bytes, specs, procs: int, threads: int, hw_queues: int = 0):
         import multiprocessing as mp

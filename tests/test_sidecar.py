@@ -2,10 +2,10 @@
 sum-check chain under the driver's grouping and challenge order.
 
 For each oracle, p_0(0) + p_0(1) equals its claim (where its constructor returns one), and
-p_i(0) + p_i(1) equals p_{i-1} at the challenge of round i-1.  The round polynomials are given by their values at 0..deg, so p(r) is
-the Lagrange interpolation over those integer nodes, in K = F_q[u]/(u^2 - 7) with Python
-integers (neo-math/src/field.rs; the batched driver of neo-reductions/src/sumcheck.rs:350-456
-folds every oracle of a group with the same challenge)."""
+p_i(0) + p_i(1) equals p_{i-1} at the challenge of round i-1.  The round polynomials are given
+by their values at 0..deg, so p(r) is the Lagrange interpolation over those integer nodes, in
+K = F_q[u]/(u^2 - 7) with Python integers (neo-math/src/field.rs; the batched driver of
+neo-reductions/src/sumcheck.rs:350-456 folds every oracle of a group with the same challenge)."""
 import pytest
 
 Q = 0xFFFFFFFF00000001
