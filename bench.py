@@ -412,7 +412,7 @@ Q_FIELD = 0xFFFFFFFF00000001
 
 
 def bench_twist(args):
-    """Config-3 memory sidecar, Twist/Shout round oracles only: 2^12-cycle chunks; every oracle
+    """Memory sidecar of a RISC-V-like trace, Twist/Shout round oracles only: 2^12-cycle chunks; every oracle
     of a chunk is constructed, the oracles are grouped by number of rounds (12 time-lane rounds,
     5 or 16 address-lane rounds) and each group is driven through all its rounds with shared
     seeded challenges (no transcript), on the GPU with the batched driver, timed per chunk.  The
@@ -474,7 +474,7 @@ def bench_twist(args):
     t_cpu = time.perf_counter() - t1
     identical = cpu_vals0 == gpu_vals0 == one_vals0
     line = {
-        "metric": "Twist/Shout round oracles of a config-3 chunk (2^12 cycles), chunks/s",
+        "metric": "Twist/Shout round oracles of a RISC-V-like trace chunk (2^12 cycles), chunks/s",
         "value": round(len(chunks) / t_gpu, 3), "unit": "chunks/s", "n_gpus": 1, "steps": len(chunks),
         "higher_is_better": True, "vs_baseline": None, "dtype": "u64 (Goldilocks F_q^2)", "data": "synthetic",
         "config": {"workload": "twist: 2 register-file Twists (32 words), RAM Twist (2^16 words, 35%, Zipf 1.1, "
@@ -498,7 +498,7 @@ AJTAI_MACS_PER_COLUMN = 16 * 54 * 54 * 8  # kappa * D^2 * 8 byte limbs: int8 MAC
 
 
 def bench_ajtai(args):
-    """Config-2 Ajtai sweep: 64 witnesses of m = 2^16 .. 2^20 columns (balanced base-2 digits
+    """Ajtai commitment sweep: 64 witnesses of m = 2^16 .. 2^20 columns (balanced base-2 digits
     uniform in {-1, 0, 1}, D = 54 rows) committed in one batched launch against PP sampled from
     the seed on the GPU, with the digit matrices already resident in HBM; timed per m.  Parity
     at every m: a sparse witness (digits in 64 random columns) against the oracle on those PP
@@ -554,7 +554,7 @@ def bench_ajtai(args):
     top = f"2^{max(logms)}"
     tops = per_m[top]["int8_TOPS"]
     line = {
-        "metric": f"Ajtai commitments of 64 witnesses sharing PP (config-2 sweep), witnesses/s at m = {top}",
+        "metric": f"Ajtai commitments of 64 witnesses sharing PP (2^16..2^20 sweep), witnesses/s at m = {top}",
         "value": per_m[top]["witnesses_per_s"], "unit": "witnesses/s", "n_gpus": 1, "steps": len(logms),
         "higher_is_better": True, "vs_baseline": None,
         "dtype": "int8 MFMA, int32 accumulation, exact Goldilocks reduction", "data": "synthetic",
