@@ -1033,8 +1033,14 @@ __global__ void k_fold_digits2(const int8_t* const* __restrict__ digits, size_t 
   if (i >= quarter) return;
   const int8_t* z = digits[row / D] + (row % D) * m;
   int d[4];
+  if ((m & 3) == 0 && 4 * i + 3 < m) {  // rows start 4-byte aligned: one load for the four digits
+    const uint32_t w4 = *reinterpret_cast<const uint32_t*>(z + 4 * i);
 #pragma unroll
-  for (int b = 0; b < 4; b++) d[b] = (4 * i + b < m) ? z[4 * i + b] : 0;
+    for (int b = 0; b < 4; b++) d[b] = (int)(int8_t)(w4 >> (8 * b));
+  } else {
+#pragma unroll
+    for (int b = 0; b < 4; b++) d[b] = (4 * i + b < m) ? z[4 * i + b] : 0;
+  }
   const K2 w[4] = {w00, w10, w01, w11};
   K2 v = kzero();
 #pragma unroll
