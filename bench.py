@@ -88,26 +88,47 @@ def kernel_bound(cls: str):
     return "hbm", "GB/s", HBM_PEAK_GBS
 
 
-def cpu_baseline(inst, z, cpu_steps: int):
+def cpu_baseline(inst, zs, cpu_steps: int):
+    """The CPU restatement on the first `cpu_steps` witnesses of the timed shard, twice:
+    all-cores (OpenMP wherever a loop parallelises) and serial-where-serial (the Pi_RLC Z_mix and
+    precompute_for_r loops on one thread, as the reference runs them; Ajtai columns and DEC
+    children keep every thread, as the reference's rayon loops do).  Returns the reported baseline
+    (serial-where-serial), the all-cores figure and the oracle's proof bytes of those steps."""
     from oracle import pyoracle
 
     pyoracle.build()
-    prover = pyoracle.Prover(inst, PP_SEED)
-    t0 = time.perf_counter()
-    prover.prove([z] * cpu_steps)
-    wall = time.perf_counter() - t0
-    secs = prover.step_seconds
+    runs = {}
+    proof = None
+    for mode in ("serial_where_serial", "all_cores"):
+        pyoracle.set_serial_where_serial(mode == "serial_where_serial")
+        prover = pyoracle.Prover(inst, PP_SEED)
+        t0 = time.perf_counter()
+        pr, _ = prover.prove(zs[:cpu_steps])
+        wall = time.perf_counter() - t0
+        if proof is None:
+            proof = pr
+        elif pr != proof:
+            raise SystemExit("CPU restatement: the two threading modes gave different proofs")
+        runs[mode] = (prover.step_seconds, wall)
+    pyoracle.set_serial_where_serial(False)
     # steady state: step 1 folds k = 13 claims like every later step; step 0 folds k = 1
-    sample_idx = 1 if cpu_steps >= 2 else 0
-    return {
-        "value": 1.0 / secs[sample_idx],
+    idx = 1 if cpu_steps >= 2 else 0
+    secs, wall = runs["serial_where_serial"]
+    asecs, awall = runs["all_cores"]
+    base = {
+        "value": 1.0 / secs[idx],
         "unit": "fold-steps/s",
         "cores": pyoracle.num_threads(),
         "kind": "port",
-        "sample": (f"CPU restatement (oracle/) of the same 2^{inst.n.bit_length() - 1}-row shard: fold step "
-                   f"{sample_idx} (k = {13 if sample_idx else 1} claims), {secs[sample_idx]:.2f} s; "
-                   f"{cpu_steps} steps + MCS commits took {wall:.1f} s wall"),
+        "threading": "serial where the reference is serial (Z_mix, precompute_for_r, the row rounds); "
+                     "Ajtai columns and DEC children on all cores",
+        "sample": (f"CPU restatement (oracle/) of the timed 2^{inst.n.bit_length() - 1}-row shard's first "
+                   f"{cpu_steps} steps: fold step {idx} (k = {13 if idx else 1} claims) took {secs[idx]:.2f} s; "
+                   f"{cpu_steps} steps + MCS commits {wall:.1f} s wall"),
+        "all_cores": {"value": 1.0 / asecs[idx], "cores": pyoracle.num_threads(),
+                      "sample": f"same steps with every loop on all cores: step {idx} {asecs[idx]:.2f} s"},
     }
+    return base, proof
 
 
 def main():
@@ -164,22 +185,24 @@ def main():
         return bench_ajtai(args)
 
     import nightstream_amd as ns
-    from nightstream_amd.workload import gen_r1cs
+    from nightstream_amd.workload import gen_r1cs_steps
 
     if ns.device_count() < 1:
         sys.exit("no GPU visible")
     t_setup = time.perf_counter()
-    inst, zs = gen_r1cs(args.n, args.seed, 1)
-    z = zs[0]
+    # one CCS, a distinct witness per fold step (free columns resampled per step)
+    nz = max(args.steps, args.warmup, args.cpu_steps, 1)
+    inst, zs = gen_r1cs_steps(args.n, args.seed, nz)
     ccs = ns.Ccs(inst)
     pp = ns.PublicParams(args.n, seed=PP_SEED)
     sess = ns.FoldingSession(ccs, pp)
-    z_dev = ns.DeviceBuffer.from_array(z)
+    z_dev = [ns.DeviceBuffer.from_array(z) for z in zs]
+    ptrs = [b.data_ptr() for b in z_dev]
     ns.device_synchronize()
     setup_s = time.perf_counter() - t_setup
 
     if args.warmup > 0:
-        sess.prove_device([z_dev.data_ptr()] * args.warmup)
+        sess.prove_device(ptrs[:args.warmup])
     # The commitment autotuner times its K-split candidates on the first commit of each
     # (m, witnesses) shape, as a long-running prover does on its first proofs: the warmup settled
     # the 12-child shape; settle the batched commitment of the timed run's steps here (zero
@@ -190,7 +213,7 @@ def main():
     ns.device_synchronize()
     sess.set_timing(True)
     t0 = time.perf_counter()
-    proof = sess.prove_device([z_dev.data_ptr()] * args.steps)
+    proof = sess.prove_device(ptrs[:args.steps])
     ns.device_synchronize()
     t1 = time.perf_counter()
     sess.set_timing(False)
@@ -206,10 +229,11 @@ def main():
     avg_ms = ms[dom] / max(counts[dom], 1)
     per_launch = work[cls] / max(counts[dom], 1)
     achieved = (per_launch / (avg_ms * 1e-3)) / (1e9 if unit == "GB/s" else 1e12)
-    # HBM bytes per launch of this kernel, from the FETCH_SIZE / WRITE_SIZE summary of the default
-    # bench command (copy-kernel calibrated); only meaningful for the workload it was taken on
+    # HBM bytes per launch of this kernel from the committed FETCH_SIZE / WRITE_SIZE summary
+    # (copy-kernel calibrated), taken on the 2^16-row, 16-step shard; per-launch traffic of a
+    # round kernel depends on n, not on the step count
     traffic = None
-    if args.n == 1 << 16 and args.steps == 16:
+    if args.n == 1 << 16:
         from nightstream_amd.profiling import load_traffic
         traffic = load_traffic(os.path.join(ROOT, "profiles", "traffic_plain_shard_2e16.md"), KERNEL_NAMES[cls])
     roofline = {
@@ -222,8 +246,12 @@ def main():
     }
 
     cpu = None
+    parity = None
     if args.cpu_steps > 0:
-        cpu = cpu_baseline(inst, z, args.cpu_steps)
+        cpu, cpu_proof = cpu_baseline(inst, zs, args.cpu_steps)
+        # the GPU proves the same first steps as a shard of their own (outside the timed region)
+        gpu_proof = ns.FoldingSession(ccs, pp).prove_device(ptrs[:args.cpu_steps])
+        parity = {"steps": args.cpu_steps, "proof_bytes": len(cpu_proof), "bit_identical": gpu_proof == cpu_proof}
 
     value = args.steps / elapsed
     line = {
@@ -240,17 +268,21 @@ def main():
         "dtype": "u64 (Goldilocks F_q / F_q^2; int8 MFMA with int32 accumulation for exact contractions)",
         "data": "synthetic",
         "config": {
-            "workload": f"plain shard: R1CS n = m = 2^{args.n.bit_length() - 1}, {args.steps} fold steps, "
-                        "default Goldilocks params (D=54, kappa=16, b=2, k_rho=12), CCS-only",
+            "workload": f"plain shard: R1CS n = m = 2^{args.n.bit_length() - 1}, {args.steps} fold steps with "
+                        "distinct witnesses, default Goldilocks params (D=54, kappa=16, b=2, k_rho=12), CCS-only",
             "n": args.n, "m": args.n, "fold_steps": args.steps, "m_in": 1,
             "proof_bytes": len(proof), "setup_s": round(setup_s, 2),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity_vs_cpu": parity,
+        "gpu_kernel_ms_per_step": round(sum(ms) / args.steps, 4),
         "host_phases_s": phases,
         "host_phase_perms": perms,
     }
     print(json.dumps(line), flush=True)
+    if parity is not None and not parity["bit_identical"]:
+        sys.exit("GPU proof differs from the CPU restatement's on the sampled steps")
 
 
 def _stderr(msg):

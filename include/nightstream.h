@@ -78,10 +78,14 @@ int ns_decomp_b(const uint64_t* z, size_t m, uint32_t b, int8_t* Z_out);
 
 /* ---- SModuleHomomorphism::commit (neo-ccs/src/traits.rs:8-14; AjtaiSModule::commit,
  *      neo-ajtai/src/s_module.rs:335-364 -> commit_precomp_ct_row_major, commit.rs:679-744).
- * Commits n_wit host digit matrices in one batched launch; out[w*kappa*D + col*D + row]. */
+ * Commits n_wit host digit matrices in one batched launch; out[w*kappa*D + col*D + row].
+ * Digits must lie in {-1, 0, 1} (balanced b = 2): the int8 GEMM accumulates per-K-split int32
+ * partial sums that are exact only in that range; other digits return NS_EINVAL. */
 int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit, uint64_t* out);
 /* The same commitments for digit matrices already resident in HBM: Z_dev[w] is a device pointer
- * to the D x m int8 matrix of witness w (from ns_device_alloc); only the commitments cross PCIe. */
+ * to the D x m int8 matrix of witness w (from ns_device_alloc); only the commitments cross PCIe.
+ * Same {-1, 0, 1} digit contract; device digits are not scanned (the caller's decomposition,
+ * ns_decomp_b or the prover's own, produces only those). */
 int ns_ajtai_commit_batch_device(const ns_pp* pp, const int8_t* const* Z_dev, size_t n_wit, uint64_t* out);
 
 /* ---- RoundOracle for Pi_CCS (neo-reductions/src/sumcheck.rs:17-34; OptimizedOracle,

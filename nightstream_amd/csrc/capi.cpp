@@ -171,7 +171,18 @@ static std::vector<DevBuf> upload_digits(const int8_t* const* Z, size_t n, size_
 int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit, uint64_t* out) {
   return guard([&] {
     require_device();
+    if (!pp) throw Error(1, "null PP");
+    if (n_wit && (!Z || !out)) throw Error(1, "null witness array or output");
     const size_t m = pp->pp->m;
+    // The GEMM's int32 K-split partial sums are exact for digits in {-1, 0, 1} (at most
+    // 128 * m <= 2^27 per split for m <= 2^20); wider digits are refused, not wrapped.
+    for (size_t w = 0; w < n_wit; w++) {
+      if (!Z[w]) throw Error(1, "null witness");
+      const uint8_t* z = (const uint8_t*)Z[w];
+      uint8_t bad = 0;
+      for (size_t i = 0; i < (size_t)D * m; i++) bad |= (uint8_t)((uint8_t)(z[i] + 1) > 2);
+      if (bad) throw Error(1, "witness " + std::to_string(w) + " has a digit outside {-1, 0, 1}");
+    }
     hipStream_t s = default_stream();
     auto bufs = upload_digits(Z, n_wit, m, s);
     std::vector<const int8_t*> ptrs;

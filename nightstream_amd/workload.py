@@ -102,6 +102,61 @@ def gen_r1cs(n: int, seed: int, nsteps: int = 1) -> tuple[R1CS, list[np.ndarray]
     return inst, [z.copy() for _ in range(nsteps)]
 
 
+def gen_r1cs_steps(n: int, seed: int, nsteps: int, free_frac: float = 0.25) -> tuple[R1CS, list[np.ndarray]]:
+    """One R1CS instance satisfied by `nsteps` DISTINCT witnesses (one per fold step).
+
+    The generator's distribution per row (A, B: 3 distinct uniform columns with uniform nonzero
+    coefficients; C: one column j != 0 with coefficient (Az)_i (Bz)_i / z_j) is drawn over a
+    "bound" set of columns that holds z[0] and a random (1 - free_frac) share of the others.  The
+    remaining "free" columns appear in no constraint, so every step's witness keeps the bound
+    entries and draws its free entries afresh (uniform in |z| <= 2^54 - 1): the steps' digit
+    matrices, MCS commitments and accumulators all differ, while one CCS structure holds for the
+    whole shard, as in fold_shard_prove (neo-fold/src/shard.rs:1725-1745).  Vectorised, so 2^16
+    rows take about a second.
+    """
+    m = n
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(np.arange(1, m))
+    nfree = int((m - 1) * free_frac)
+    free = np.sort(perm[:nfree])
+    bound = np.concatenate([[0], np.sort(perm[nfree:])]).astype(np.int64)
+    z = rand_balanced_z(rng, m)
+    zo = z.astype(object)
+    nb = len(bound)
+
+    def rows3():
+        idx = rng.integers(0, nb, size=(n, 3))
+        while True:
+            dup = (idx[:, 0] == idx[:, 1]) | (idx[:, 0] == idx[:, 2]) | (idx[:, 1] == idx[:, 2])
+            if not dup.any():
+                break
+            idx[dup] = rng.integers(0, nb, size=(int(dup.sum()), 3))
+        cols = np.sort(bound[idx], axis=1)
+        vals = rand_field(rng, 3 * n).reshape(n, 3)
+        return cols.astype(np.uint32), vals
+
+    a_cols, a_vals = rows3()
+    b_cols, b_vals = rows3()
+    az = (a_vals.astype(object) * zo[a_cols.astype(np.int64)]).sum(axis=1) % Q
+    bz = (b_vals.astype(object) * zo[b_cols.astype(np.int64)]).sum(axis=1) % Q
+    nonzero_bound = bound[1:][z[bound[1:]] != 0]
+    jj = nonzero_bound[rng.integers(0, len(nonzero_bound), size=n)]
+    inv = np.array([pow(int(v), Q - 2, Q) for v in z[jj]], dtype=object)
+    c_vals = np.array((az * bz % Q) * inv % Q, dtype=np.uint64)
+    rp3 = np.arange(0, 3 * n + 1, 3, dtype=np.uint64)
+    rp1 = np.arange(0, n + 1, dtype=np.uint64)
+    a = (rp3, a_cols.reshape(-1).copy(), a_vals.reshape(-1).copy())
+    b = (rp3, b_cols.reshape(-1).copy(), b_vals.reshape(-1).copy())
+    c = (rp1, jj.astype(np.uint32), c_vals)
+    inst = R1CS(n, m, a, b, c, z)
+    zs = [z.copy()]
+    for _ in range(1, nsteps):
+        zi = z.copy()
+        zi[free] = rand_balanced_z(rng, len(free) + 1)[1:]
+        zs.append(zi)
+    return inst, zs
+
+
 def check_r1cs(inst: R1CS, z: np.ndarray) -> bool:
     zi = [int(v) for v in z]
 

@@ -154,6 +154,10 @@ size_t or_rowstate_len(const or_rowstate* st);
 size_t or_rowstate_export(const or_rowstate* st, int which, size_t idx, kx* out, size_t cap);
 
 /* precompute_for_r (oracle.rs:1659-1783): y_eval[i][j][rho] for all k_total witnesses */
+void or_precompute_y_par(const or_ccs* s, const int8_t* const* Zs, size_t k_total, const kx* r, size_t ell_n,
+                         kx* y_eval, int parallel);
+/* 1: the loops the reference runs serially (Z_mix, precompute_for_r) run on one thread */
+void or_set_serial_where_serial(int on);
 void or_precompute_y(const or_ccs* s, const int8_t* const* Zs, size_t k_total, const kx* r, size_t ell_n,
                      kx* y_eval /* k_total*t*54 */);
 void or_vjs(const or_ccs* s, const kx* r, size_t ell_n, kx* vjs /* t*m */);

@@ -360,10 +360,17 @@ void or_vjs(const or_ccs* s, const kx* r, size_t ell_n, kx* vjs) {
 
 void or_precompute_y(const or_ccs* s, const int8_t* const* Zs, size_t k_total, const kx* r, size_t ell_n,
                      kx* y_eval) {
+  or_precompute_y_par(s, Zs, k_total, r, ell_n, y_eval, 1);
+}
+
+/* parallel = 0 runs on one thread, as the reference's precompute_for_r does (oracle.rs:1674-1677);
+ * the DEC children's y products are rayon-parallel there (common.rs:1547) and pass 1. */
+void or_precompute_y_par(const or_ccs* s, const int8_t* const* Zs, size_t k_total, const kx* r, size_t ell_n,
+                         kx* y_eval, int parallel) {
   size_t m = s->m, t = s->t, D = NEO_D;
   kx* vjs = (kx*)malloc(t * m * sizeof(kx));
   or_vjs(s, r, ell_n, vjs);
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if (parallel)
   for (long i = 0; i < (long)k_total; i++)
     for (long j = 0; j < (long)t; j++) {
       const kx* v = vjs + (size_t)j * m;

@@ -91,6 +91,7 @@ def lib():
         L.or_last_step_seconds.argtypes = [C.c_size_t]
         L.or_num_threads.restype = C.c_int
         L.or_tr_permutation_count.restype = C.c_uint64
+        L.or_set_serial_where_serial.argtypes = [C.c_int]
         L.or_params_auto_r1cs.restype = C.c_int
         L.or_extension_check.restype = C.c_int
         _lib = L
@@ -132,6 +133,12 @@ class Ccs:
 
 def num_threads() -> int:
     return lib().or_num_threads()
+
+
+def set_serial_where_serial(on: bool) -> None:
+    """Run the loops the reference runs on one thread (Pi_RLC Z_mix, precompute_for_r) on one
+    thread; the rest keeps every OpenMP thread (where the reference uses rayon)."""
+    lib().or_set_serial_where_serial(1 if on else 0)
 
 
 def p2_permute(state) -> np.ndarray:

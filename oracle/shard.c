@@ -342,9 +342,15 @@ void or_rot_from_coeffs(const int8_t* a, fq* rho) {
   }
 }
 
+/* Serial-where-serial mode: the loops the reference runs on one thread (the Pi_RLC Z_mix triple
+ * loop, common.rs:1329-1363, and precompute_for_r, oracle.rs:1674-1677) then run on one thread
+ * here too; the rayon-parallel parts (Ajtai columns, DEC children) keep every thread. */
+static int g_serial_where_serial = 0;
+void or_set_serial_where_serial(int on) { g_serial_where_serial = on; }
+
 void or_rlc_zmix(const fq* const* rhos, const int8_t* const* Zs, size_t k, size_t m, int32_t* zmix) {
   memset(zmix, 0, D * m * sizeof(int32_t));
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (!g_serial_where_serial)
   for (long rr = 0; rr < D; rr++) {
     for (size_t i = 0; i < k; i++)
       for (size_t kk = 0; kk < D; kk++) {
@@ -622,7 +628,7 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
 
     /* Ajtai rounds */
     kx* y_eval = (kx*)malloc(kt * t * D * sizeof(kx));
-    or_precompute_y(s, Zs, kt, r_time, ell_n, y_eval);
+    or_precompute_y_par(s, Zs, kt, r_time, ell_n, y_eval, !g_serial_where_serial);
     ajtai_ctx ac;
     memset(&ac, 0, sizeof(ac));
     ac.k_total = kt;

@@ -6,7 +6,7 @@ round polynomials, proof bytes) on the same seeded inputs.
 import numpy as np
 import pytest
 
-from nightstream_amd.workload import Q, gen_r1cs, rand_balanced_z, rand_field
+from nightstream_amd.workload import Q, gen_r1cs, gen_r1cs_steps, rand_balanced_z, rand_field
 
 pytestmark = pytest.mark.gpu
 D = 54
@@ -73,6 +73,19 @@ def test_commit_device_resident(ns, oracle, logm):
     got = pp.commit_device([ns.DeviceBuffer.from_array(z.reshape(-1))])[0]
     ppc = ppc.reshape(16, m, D)
     assert np.array_equal(got, oracle.ajtai_commit(np.ascontiguousarray(ppc[:, cols, :]).reshape(-1), 16, 64, zsub))
+
+
+def test_commit_rejects_wide_digits(ns):
+    """Digits outside {-1, 0, 1} are refused (the int32 K-split partials are exact only there)."""
+    m = 1024
+    pp = ns.PublicParams(m, seed=bytes([9] * 32))
+    z = np.zeros((D, m), np.int8)
+    z[7, 100] = 2
+    with pytest.raises(ns.NsError):
+        pp.commit([z])
+    z[7, 100] = -128
+    with pytest.raises(ns.NsError):
+        pp.commit([np.zeros((D, m), np.int8), z])
 
 
 def test_commit_extreme_digits(ns, oracle):
@@ -154,3 +167,76 @@ def test_prove_shard_bit_identical(ns, oracle, n, steps):
     proof_cpu, _ = oracle.Prover(inst, seed).prove(zs)
     assert len(proof_gpu) == len(proof_cpu)
     assert proof_gpu == proof_cpu
+
+
+@pytest.mark.parametrize("n,steps", [(1 << 10, 4)])
+def test_prove_shard_distinct_witnesses(ns, oracle, n, steps):
+    """Every step folds its own witness (one CCS satisfied by all of them): a prover that read
+    another step's digits or commitment would differ from the oracle here."""
+    inst, zs = gen_r1cs_steps(n, 5, steps)
+    seed = bytes([42] * 32)
+    sess = ns.FoldingSession(ns.Ccs(inst), ns.PublicParams(n, seed=seed))
+    for z in zs:
+        sess.add_step_z(z)
+    proof_gpu = sess.fold_and_prove()
+    proof_cpu, _ = oracle.Prover(inst, seed).prove(zs)
+    assert proof_gpu == proof_cpu
+
+
+def test_prove_shard_2e16_device_resident(ns, oracle):
+    """The timed workload's shape: n = m = 2^16, witnesses resident in HBM (ns_prove_shard_device),
+    three fold steps so the steady state k = 13 is reached twice, distinct per-step witnesses.
+    The size-dependent launch shapes (row groups and grids of the round kernels, multi-block
+    scans, the autotuned commitment K-split at m = 2^16) are the ones the timed run takes.  The
+    oracle needs ~30 s on 16 host cores."""
+    n, steps = 1 << 16, 3
+    print(f"2^16 shard, {steps} steps, distinct witnesses", flush=True)
+    inst, zs = gen_r1cs_steps(n, 7, steps)
+    seed = bytes([0x4E, 0x53] + [0x07] * 30)
+    sess = ns.FoldingSession(ns.Ccs(inst), ns.PublicParams(n, seed=seed))
+    bufs = [ns.DeviceBuffer.from_array(z) for z in zs]
+    proof_gpu = sess.prove_device([b.data_ptr() for b in bufs])
+    proof_host = ns.FoldingSession(ns.Ccs(inst), ns.PublicParams(n, seed=seed))
+    for z in zs:
+        proof_host.add_step_z(z)
+    assert proof_host.fold_and_prove() == proof_gpu
+    proof_cpu, _ = oracle.Prover(inst, seed).prove(zs)
+    assert len(proof_gpu) == len(proof_cpu)
+    assert proof_gpu == proof_cpu
+
+
+@pytest.mark.parametrize("logm", [16, 20])
+def test_commit_sweep_64_witnesses(ns, oracle, logm):
+    """The Ajtai sweep's shape: 64 witnesses in one batched call on device-resident digits.
+    At 2^16 every witness is dense and checked against the oracle (a commitment over all m
+    columns).  At 2^20 (K-step-major digit path) witness 0 is dense and checked in full; each
+    other witness has its digits in 64 random columns of its own and is checked against the
+    oracle on those PP columns (a commitment is a sum over columns), so all 64 output slots and
+    every K step of the GEMM are covered."""
+    rng = np.random.default_rng(77 + logm)
+    m, nw = 1 << logm, 64
+    print(f"64-witness commit sweep m=2^{logm}", flush=True)
+    pp = ns.PublicParams(m, seed=bytes([5] * 32), keep_coeffs=True)
+    ppc = pp.coeffs()
+    Zs, cols = [], []
+    for w in range(nw):
+        if logm == 16 or w == 0:
+            Zs.append(rng.integers(-1, 2, size=(D, m), dtype=np.int8))
+            cols.append(None)
+        else:
+            c = np.sort(rng.choice(m, size=64, replace=False))
+            z = np.zeros((D, m), np.int8)
+            z[:, c] = rng.integers(-1, 2, size=(D, 64), dtype=np.int8)
+            Zs.append(z)
+            cols.append(c)
+    bufs = [ns.DeviceBuffer.from_array(z.reshape(-1)) for z in Zs]
+    got = pp.commit_device(bufs)
+    del bufs
+    ppc3 = ppc.reshape(16, m, D)
+    for w in range(nw):
+        if cols[w] is None:
+            want = oracle.ajtai_commit(ppc, 16, m, Zs[w])
+        else:
+            sub = np.ascontiguousarray(ppc3[:, cols[w], :]).reshape(-1)
+            want = oracle.ajtai_commit(sub, 16, 64, np.ascontiguousarray(Zs[w][:, cols[w]]))
+        assert np.array_equal(got[w], want), f"witness {w}"
