@@ -342,20 +342,18 @@ int ns_rlc_zmix(const int8_t* rho_coeffs, const int8_t* const* Z, size_t k, size
 int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* outs, int* nonzero) {
   return guard([&] {
     require_device();
-    if (k_dec == 0 || k_dec > 30) throw Error(1, "k_dec out of range");
+    if (k_dec == 0 || k_dec > (size_t)DEC_MAX_PLANES) throw Error(1, "k_dec out of range");
     hipStream_t s = default_stream();
     const size_t len = (size_t)D * m;
-    DevBuf zd(len * 4), flags((k_dec + 1) * 4), pd(k_dec * sizeof(void*));
+    DevBuf zd(len * 4), flags((k_dec + 1) * 4);
     std::vector<DevBuf> planes;
-    std::vector<int8_t*> ptrs;
+    DecPlanes pl{};
     for (size_t i = 0; i < k_dec; i++) {
       planes.emplace_back(len);
-      ptrs.push_back(planes.back().as<int8_t>());
+      pl.p[i] = planes.back().as<int8_t>();
     }
     NS_CHECK(hipMemcpyAsync(zd.p, zmix, len * 4, hipMemcpyHostToDevice, s));
-    NS_CHECK(hipMemcpyAsync(pd.p, ptrs.data(), k_dec * sizeof(void*), hipMemcpyHostToDevice, s));
-    NS_CHECK(hipMemsetAsync(flags.p, 0, (k_dec + 1) * 4, s));
-    launch_dec_split(zd.as<int32_t>(), len, (int)k_dec, pd.as<int8_t* const>(), flags.as<int>(), s);
+    launch_dec_split(zd.as<int32_t>(), len, (int)k_dec, pl, flags.as<int>(), s);
     std::vector<int> fl(k_dec + 1);
     NS_CHECK(hipMemcpyAsync(fl.data(), flags.p, (k_dec + 1) * 4, hipMemcpyDeviceToHost, s));
     NS_CHECK(hipStreamSynchronize(s));

@@ -13,6 +13,7 @@
 // so the NC part of a pair needs only signed selections of wg (no products).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdexcept>
 
 #include "kernels.h"
@@ -1105,49 +1106,89 @@ void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t 
                   m, zmix);
 }
 
+// Balanced base-2 digits (neo-reductions/src/common.rs:75-268): repeatedly r = v % 2 (the sign
+// of v when v is odd), v = (v - r) / 2.  Both steps keep the sign of v and halve |v|, so digit i
+// is sign(v) * bit i of |v|, and v is in range exactly when |v| < 2^k_dec.  Each thread takes 16
+// consecutive entries (four 16-byte loads) and writes one 16-byte word per plane; plane i is
+// non-zero exactly when bit i of the OR of all |v| is set, so one OR word per block carries every
+// flag.  vec = 0 (an unaligned pointer from a caller, or len not a multiple of 16) takes byte
+// stores instead.
 __global__ __launch_bounds__(256) void k_dec_split(const int32_t* __restrict__ zmix, size_t len, int k_dec,
-                                                   int8_t* const* __restrict__ outs, int* __restrict__ flags) {
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned nz = 0, bad = 0;
-  if (idx < len) {
-    int v = zmix[idx];
-    int B = 1 << k_dec;
-    bad = (v >= B || v <= -B) ? 1u : 0u;
-    for (int i = 0; i < k_dec; i++) {
-      int r = v % 2;  // balanced base-2 digit: sign of v when v is odd
-      outs[i][idx] = (int8_t)r;
-      nz |= (r != 0 ? 1u : 0u) << i;
-      v = (v - r) / 2;
+                                                   DecPlanes outs, int* __restrict__ flags, int vec) {
+  const size_t base = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+  uint32_t orv = 0;
+  if (base < len) {
+    const bool full = vec && base + 16 <= len;
+    uint32_t a[16], negw[4] = {0, 0, 0, 0};
+    if (full) {
+      const int4* src = reinterpret_cast<const int4*>(zmix + base);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int4 x = src[q];
+        const int32_t v4[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int32_t v = v4[j];
+          a[4 * q + j] = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+          negw[q] |= (v < 0 ? 0xFFu : 0u) << (8 * j);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int32_t v = base + e < len ? zmix[base + e] : 0;
+        a[e] = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+        negw[e >> 2] |= (v < 0 ? 0xFFu : 0u) << (8 * (e & 3));
+      }
     }
-    if (v != 0) bad = 1;
+#pragma unroll
+    for (int e = 0; e < 16; e++) orv |= a[e];
+    for (int i = 0; i < k_dec; i++) {
+      uint32_t w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        // one byte per entry: 0x01 for a set bit, widened to 0xFF where the entry is negative
+        const uint32_t bits = ((a[4 * q] >> i) & 1u) | (((a[4 * q + 1] >> i) & 1u) << 8) |
+                              (((a[4 * q + 2] >> i) & 1u) << 16) | (((a[4 * q + 3] >> i) & 1u) << 24);
+        w[q] = bits | ((bits * 0xFFu) & negw[q]);
+      }
+      int8_t* dst = outs.p[i] + base;
+      if (full) {
+        *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+      } else {
+        for (int e = 0; e < 16 && base + e < len; e++) dst[e] = (int8_t)(w[e >> 2] >> (8 * (e & 3)));
+      }
+    }
   }
-  // OR across the block, then one atomic per flag word (integer atomics are exact)
-  for (int off = 32; off > 0; off >>= 1) {
-    nz |= __shfl_down(nz, off, 64);
-    bad |= __shfl_down(bad, off, 64);
-  }
-  __shared__ unsigned red[8];
-  if ((threadIdx.x & 63) == 0) {
-    red[(threadIdx.x >> 6) * 2] = nz;
-    red[(threadIdx.x >> 6) * 2 + 1] = bad;
-  }
+  for (int off = 32; off > 0; off >>= 1) orv |= __shfl_down(orv, off, 64);
+  __shared__ uint32_t red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = orv;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned a = 0, b = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
-      a |= red[2 * w];
-      b |= red[2 * w + 1];
-    }
+    uint32_t o = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) o |= red[w];
     // most blocks find the flags already raised: skip the contended atomic then
     for (int i = 0; i < k_dec; i++)
-      if (((a >> i) & 1) && __atomic_load_n(&flags[i], __ATOMIC_RELAXED) == 0) atomicOr(&flags[i], 1);
-    if (b) atomicOr(&flags[k_dec], 1);
+      if (((o >> i) & 1u) && __atomic_load_n(&flags[i], __ATOMIC_RELAXED) == 0) atomicOr(&flags[i], 1);
+    if (o >> k_dec) atomicOr(&flags[k_dec], 1);
   }
 }
 
-void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const* outs_dev, int* flags, hipStream_t s) {
-  NS_LAUNCH_TIMED(KT_SPLIT, k_dec_split, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, zmix, len, k_dec,
-                  outs_dev, flags);
+__global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, const DecPlanes& outs, int* flags, hipStream_t s) {
+  if (k_dec < 1 || k_dec > DEC_MAX_PLANES) throw std::invalid_argument("DEC split: 1..30 planes");
+  // flags start at zero: a fill kernel, not a memset node (callers may capture this sequence)
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, flags, k_dec + 1, 0);
+  if (len == 0) return;
+  bool aligned = (((uintptr_t)zmix) & 15) == 0 && (len & 15) == 0;
+  for (int i = 0; i < k_dec; i++) aligned = aligned && (((uintptr_t)outs.p[i]) & 15) == 0;
+  const size_t threads = (len + 15) / 16;
+  NS_LAUNCH_TIMED(KT_SPLIT, k_dec_split, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, zmix, len, k_dec,
+                  outs, flags, aligned ? 1 : 0);
 }
 
 // ---------------- head columns (ME public inputs X) ----------------

@@ -159,7 +159,6 @@ void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t
 // rho_dev: the k rotation matrices as int8 [k][64][64] (row r, column kk; zero beyond 54)
 void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t k, size_t m, int32_t* zmix,
                      hipStream_t s);
-// balanced base-2 digits of zmix into k_dec planes; flags[i] |= plane i nonzero; flags[k_dec] = range error
 // out[e] = sum over K splits of part[sp * n + e] (int32)
 void launch_sum_splits(const int32_t* part, size_t splits, size_t n, int32_t* out, hipStream_t s);
 // Gather the first m_in columns of up to 16 digit planes [54][m]: out[(p*54 + rho)*m_in + c].
@@ -167,7 +166,12 @@ struct PlanePtrs {
   const int8_t* p[16];
 };
 void launch_head_cols(const PlanePtrs& planes, int np, size_t m, int m_in, int8_t* out, hipStream_t s);
-void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, int8_t* const* outs_dev, int* flags,
-                      hipStream_t s);
+// balanced base-2 digits of zmix into k_dec planes (by value: no pointer table in device memory);
+// flags[0..k_dec] are cleared by the launch, then flags[i] = plane i nonzero, flags[k_dec] = range error
+constexpr int DEC_MAX_PLANES = 30;
+struct DecPlanes {
+  int8_t* p[DEC_MAX_PLANES];
+};
+void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, const DecPlanes& outs, int* flags, hipStream_t s);
 
 }  // namespace ns

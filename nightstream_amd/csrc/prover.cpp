@@ -1327,8 +1327,9 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     flags_.ensure((k_dec + 1) * 4);
     DevBuf &pptr_dev = pptr_dev_, &flags = flags_;
     h2d(pptr_dev.p, (const void* const*)pptr.data(), k_dec, st);
-    NS_CHECK(hipMemsetAsync(flags.p, 0, (k_dec + 1) * 4, st));
-    launch_dec_split(zmix.as<int32_t>(), (size_t)D * m, (int)k_dec, (int8_t* const*)pptr_dev.p, flags.as<int>(), st);
+    DecPlanes dec_pl{};
+    for (size_t i = 0; i < k_dec; i++) dec_pl.p[i] = const_cast<int8_t*>(pptr[i]);
+    launch_dec_split(zmix.as<int32_t>(), (size_t)D * m, (int)k_dec, dec_pl, flags.as<int>(), st);
     std::vector<int> fl(k_dec + 1);
     d2h(ows_.stage, fl.data(), flags.p, k_dec + 1, st);
     if (fl[k_dec]) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
