@@ -110,6 +110,15 @@ int ns_rlc_zmix(const int8_t* rho_coeffs, const int8_t* const* Z, size_t k, size
 /* ---- Pi_DEC split (neo-reductions/src/common.rs:75-268): balanced base-2 digits of zmix
  *      into k_dec planes; nonzero[i] = plane i has a nonzero digit.  NS_ERANGE if |v| >= 2^k_dec. */
 int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* outs, int* nonzero);
+/* ---- no reference counterpart: the Pi_DEC split for callers that cannot allocate device memory
+ *      (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * zmix_dev: device int32 [54][m]; outs_dev[i] (host array of k_dec device pointers): device int8
+ * [54][m] planes; nonzero: host int[k_dec].  Same digits, flags and NS_ERANGE as ns_dec_split.
+ * Allocates nothing: its counters live in a static per-GPU scratch array of 64 ints (so
+ * 1 <= k_dec <= 30), and every call on one GPU shares that array -- calls that overlap on one
+ * GPU race. */
+int ns_dec_split_device_noalloc(const int32_t* zmix_dev, size_t m, size_t k_dec, int8_t* const* outs_dev,
+                                int* nonzero);
 
 /* ---- shard prover: fold_shard_prove (neo-fold/src/shard.rs:2285-2316) driven as
  *      FoldingSession::{add_step_io, fold_and_prove} (neo-fold/src/session.rs:1018-1311).

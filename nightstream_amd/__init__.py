@@ -25,7 +25,7 @@ EXPORTS = [
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
-    "ns_dec_split", "ns_prover_new", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
+    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
     "ns_poseidon2_permute",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
@@ -93,6 +93,7 @@ def _load():
     L.ns_oracle_y_eval.argtypes = [vp, vp, vp]
     L.ns_rlc_zmix.argtypes = [vp, vp, sz, sz, vp]
     L.ns_dec_split.argtypes = [vp, sz, sz, vp, vp]
+    L.ns_dec_split_device_noalloc.argtypes = [vp, sz, sz, vp, vp]
     L.ns_prover_new.argtypes = [vp, vp, C.POINTER(vp)]
     L.ns_prover_rebind.argtypes = [vp, vp, vp]
     L.ns_device_alloc.argtypes = [sz, C.POINTER(vp)]

@@ -365,6 +365,27 @@ int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* out
   });
 }
 
+int ns_dec_split_device_noalloc(const int32_t* zmix_dev, size_t m, size_t k_dec, int8_t* const* outs_dev,
+                                int* nonzero) {
+  return guard([&] {
+    require_device();
+    if (k_dec == 0 || k_dec > (size_t)DEC_MAX_PLANES) throw Error(1, "k_dec out of range (1..30)");
+    if (!nonzero || (m && !zmix_dev) || !outs_dev) throw Error(1, "null argument");
+    for (size_t i = 0; i < k_dec; i++)
+      if (!outs_dev[i]) throw Error(1, "null output plane");
+    hipStream_t s = default_stream();
+    int* flags = dec_noalloc_flags();  // this device's copy of the static scratch (no allocation)
+    DecPlanes pl{};
+    for (size_t i = 0; i < k_dec; i++) pl.p[i] = outs_dev[i];
+    launch_dec_split(zmix_dev, (size_t)D * m, (int)k_dec, pl, flags, s);
+    int fl[DEC_MAX_PLANES + 1];
+    NS_CHECK(hipMemcpyAsync(fl, flags, (k_dec + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    NS_CHECK(hipStreamSynchronize(s));
+    if (fl[k_dec]) throw Error(6, "DEC split: value out of range for k_rho digits");
+    for (size_t i = 0; i < k_dec; i++) nonzero[i] = fl[i] ? 1 : 0;
+  });
+}
+
 int ns_prover_new(const ns_ccs* s, const ns_pp* pp, ns_prover** out) {
   return guard([&] {
     require_device();

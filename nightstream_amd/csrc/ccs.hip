@@ -1191,6 +1191,29 @@ void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, const DecPlane
                   outs, flags, aligned ? 1 : 0);
 }
 
+// Static scratch of the no-allocation DEC entry point: the plane non-zero counters and the range
+// counter (DEC_MAX_PLANES + 1 ints used of 64).  Each GPU holds its own copy for the life of the
+// process; every call on one GPU shares it, so calls that overlap on one GPU race.
+static __device__ __align__(256) int scratch[64];
+
+// This device's address of `scratch` (the host-side name is only a stand-in, never a device
+// address), looked up once per device with that device current and kept; later calls reuse it.
+int* dec_noalloc_flags() {
+  static std::mutex mu;
+  static int* per_dev[256] = {nullptr};
+  int dev = 0;
+  NS_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 256) throw std::invalid_argument("device ordinal out of range");
+  std::lock_guard<std::mutex> g(mu);
+  if (!per_dev[dev]) {
+    void* ptr = nullptr;
+    NS_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    per_dev[dev] = (int*)ptr;
+  }
+  return per_dev[dev];
+}
+static_assert(DEC_MAX_PLANES + 1 <= 64, "DEC scratch holds the flags of every plane");
+
 // ---------------- head columns (ME public inputs X) ----------------
 __global__ void k_head_cols(PlanePtrs planes, int np, size_t m, int m_in, int8_t* __restrict__ out) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;

@@ -173,5 +173,8 @@ struct DecPlanes {
   int8_t* p[DEC_MAX_PLANES];
 };
 void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, const DecPlanes& outs, int* flags, hipStream_t s);
+// The current device's address of the DEC split's static scratch (counters), for the
+// no-allocation entry point.
+int* dec_noalloc_flags();
 
 }  // namespace ns
