@@ -135,6 +135,34 @@ int ns_prove_shard(ns_prover* p, const uint64_t* const* z_steps, size_t n_steps,
 /* Same, with z_steps_dev[s] device pointers (inputs already resident in HBM). */
 int ns_prove_shard_device(ns_prover* p, const uint64_t* const* z_steps_dev, size_t n_steps, size_t m_in,
                           uint8_t** proof, size_t* proof_len, double* seconds);
+/* ---- the same with Twist/Shout memory over the shared CPU bus (Route A): fold_shard_prove with
+ *      StepWitnessBundle lut_instances / mem_instances (neo-memory/src/witness.rs:23-205,
+ *      neo-fold/src/shard.rs:1725-2283, memory_sidecar/memory.rs).  mem[s] describes step s's
+ *      instances (metadata-only: their columns are the bus tail of z, layout
+ *      neo-memory/src/cpu/bus_layout.rs:98-191, which the CCS must pad and bind as
+ *      memory_sidecar/cpu_bus.rs:171-181 requires; violations return NS_EINVAL).  The proof adds
+ *      each step's MemSidecarProof, the batched time claims of the memory lanes and the val-lane
+ *      RLC/DEC, and ends with both obligation lanes (ShardObligations{main, val}).
+ *      z_on_device: z_steps are device pointers.  mem == NULL proves CCS-only steps. */
+typedef struct {
+  size_t k, d, n_side, steps, ell;
+  const uint64_t* table;  /* k = 2^(d*ell) F_q entries (LutInstance with no table_spec) */
+} ns_lut_inst;
+typedef struct {
+  size_t k, d, n_side, steps, ell;
+  int init_sparse;        /* 0: MemInit::Zero; 1: MemInit::Sparse(init_addrs[i], init_vals[i]) */
+  size_t ninit;
+  const uint64_t* init_addrs; /* strictly increasing, < k */
+  const uint64_t* init_vals;  /* F_q */
+} ns_mem_inst;
+typedef struct {
+  size_t n_lut;
+  const ns_lut_inst* luts;
+  size_t n_mem;
+  const ns_mem_inst* mems;
+} ns_step_mem;
+int ns_prove_shard_mem(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps, size_t m_in,
+                       const ns_step_mem* mem, uint8_t** proof, size_t* proof_len, double* seconds);
 /* Per-kernel timing of the next proves (HIP events on the prover's stream); classes:
  * 0 round-0 eval, 1 round-2 eval, 2 table fold, 3 Ajtai GEMM, 4 digit x K GEMM, 5 RLC Z_mix,
  * 6 DEC split, 7 round-1 eval, 8 fused fold + round (rounds >= 3).  ms[i] is the summed

@@ -25,7 +25,7 @@ EXPORTS = [
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
-    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
+    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
     "ns_poseidon2_permute",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
@@ -103,6 +103,8 @@ def _load():
     L.ns_prove_shard.argtypes = [vp, vp, sz, sz, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
                                  C.POINTER(C.c_double)]
     L.ns_prove_shard_device.argtypes = L.ns_prove_shard.argtypes
+    L.ns_prove_shard_mem.argtypes = [vp, vp, C.c_int, sz, sz, vp, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
+                                     C.POINTER(C.c_double)]
     L.ns_prover_set_timing.argtypes = [vp, C.c_int]
     L.ns_prover_kernel_stats.argtypes = [vp, vp, vp, sz]
     L.ns_prover_phase_stats.argtypes = [vp, vp, sz]
@@ -316,11 +318,55 @@ def dec_split(zmix: np.ndarray, k_dec: int):
     return [o.reshape(D, m) for o in outs], [bool(x) for x in nz]
 
 
-class FoldingSession:
-    """Mirror of neo-fold FoldingSession (session.rs:668-1953) for CCS-only shards.
+class _LutInst(C.Structure):
+    _fields_ = [("k", C.c_size_t), ("d", C.c_size_t), ("n_side", C.c_size_t), ("steps", C.c_size_t),
+                ("ell", C.c_size_t), ("table", C.c_void_p)]
 
-    new_ajtai_seeded(ccs, seed) -> add_step_io(x, w) per step -> fold_and_prove() returns the
-    canonical proof encoding; the accumulator and witnesses stay in HBM during the fold.
+
+class _MemInst(C.Structure):
+    _fields_ = [("k", C.c_size_t), ("d", C.c_size_t), ("n_side", C.c_size_t), ("steps", C.c_size_t),
+                ("ell", C.c_size_t), ("init_sparse", C.c_int), ("ninit", C.c_size_t), ("init_addrs", C.c_void_p),
+                ("init_vals", C.c_void_p)]
+
+
+class _StepMem(C.Structure):
+    _fields_ = [("n_lut", C.c_size_t), ("luts", C.c_void_p), ("n_mem", C.c_size_t), ("mems", C.c_void_p)]
+
+
+def _step_mem_array(mems, keep: list):
+    """ns_step_mem[] for per-step memory instances (objects with .luts / .mems whose items carry
+    k, d, n_side, steps, ell and .table (LUT) or .init (MEM: None = MemInit::Zero, else sorted
+    (addr, value) pairs)); the arrays they point into are appended to `keep`."""
+    arr = (_StepMem * len(mems))()
+    for s, sm in enumerate(mems):
+        luts = (_LutInst * max(1, len(sm.luts)))()
+        for i, lt in enumerate(sm.luts):
+            tab = np.ascontiguousarray(lt.table, np.uint64)
+            keep.append(tab)
+            luts[i] = _LutInst(lt.k, lt.d, lt.n_side, lt.steps, lt.ell, tab.ctypes.data)
+        ms = (_MemInst * max(1, len(sm.mems)))()
+        for i, mi in enumerate(sm.mems):
+            if mi.init is None:
+                ms[i] = _MemInst(mi.k, mi.d, mi.n_side, mi.steps, mi.ell, 0, 0, None, None)
+            else:
+                a = np.array([x for x, _ in mi.init] or [0], np.uint64)
+                v = np.array([y for _, y in mi.init] or [0], np.uint64)
+                keep += [a, v]
+                ms[i] = _MemInst(mi.k, mi.d, mi.n_side, mi.steps, mi.ell, 1, len(mi.init), a.ctypes.data,
+                                 v.ctypes.data)
+        keep += [luts, ms]
+        arr[s] = _StepMem(len(sm.luts), C.cast(luts, C.c_void_p), len(sm.mems), C.cast(ms, C.c_void_p))
+    keep.append(arr)
+    return arr
+
+
+class FoldingSession:
+    """Mirror of neo-fold FoldingSession (session.rs:668-1953).
+
+    new_ajtai_seeded(ccs, seed) -> add_step_io(x, w[, mem]) per step -> fold_and_prove() returns
+    the canonical proof encoding; the accumulator and witnesses stay in HBM during the fold.  A
+    step's `mem` carries its Twist/Shout instances over the shared CPU bus (Route A); every step
+    of a shard either has one or none does.
     """
 
     def __init__(self, ccs: Ccs, pp: PublicParams):
@@ -329,6 +375,7 @@ class FoldingSession:
         _check(lib().ns_prover_new(ccs.h, pp.h, C.byref(h)))
         self.h = h
         self.steps: list[np.ndarray] = []
+        self.mem: list = []
         self.m_in: int | None = None
         self.last_seconds = 0.0
 
@@ -338,7 +385,7 @@ class FoldingSession:
         pp = PublicParams(inst.m, seed=seed)
         return cls(ccs, pp)
 
-    def add_step_io(self, public_input, witness):
+    def add_step_io(self, public_input, witness, mem=None):
         x = np.asarray(public_input, np.uint64)
         w = np.asarray(witness, np.uint64)
         if self.m_in is None:
@@ -349,29 +396,45 @@ class FoldingSession:
         if len(z) != self.ccs.m:
             raise NsError(-1, f"len(x) + len(w) = {len(z)} but CCS.m = {self.ccs.m}")
         self.steps.append(np.ascontiguousarray(z))
+        self.mem.append(mem)
 
-    def add_step_z(self, z, m_in=1):
-        self.add_step_io(z[:m_in], z[m_in:])
+    def add_step_z(self, z, m_in=1, mem=None):
+        self.add_step_io(z[:m_in], z[m_in:], mem)
 
     def fold_and_prove(self) -> bytes:
         out = C.POINTER(C.c_uint8)()
         n = C.c_size_t()
         sec = C.c_double()
-        _check(lib().ns_prove_shard(self.h, _ptr_array(self.steps), len(self.steps), self.m_in or 0, C.byref(out),
-                                    C.byref(n), C.byref(sec)))
+        if any(x is not None for x in self.mem):
+            if any(x is None for x in self.mem):
+                raise NsError(-1, "every step needs its memory instances once one step has them")
+            keep = []
+            arr = _step_mem_array(self.mem, keep)
+            _check(lib().ns_prove_shard_mem(self.h, _ptr_array(self.steps), 0, len(self.steps), self.m_in or 0,
+                                            C.cast(arr, C.c_void_p), C.byref(out), C.byref(n), C.byref(sec)))
+        else:
+            _check(lib().ns_prove_shard(self.h, _ptr_array(self.steps), len(self.steps), self.m_in or 0,
+                                        C.byref(out), C.byref(n), C.byref(sec)))
         data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)
         lib().ns_free(C.cast(out, C.c_void_p))
         self.last_seconds = sec.value
         return data
 
-    def prove_device(self, z_ptrs, m_in: int = 1) -> bytes:
-        """fold_and_prove over step witnesses already resident in HBM (device pointers to u64[m])."""
+    def prove_device(self, z_ptrs, m_in: int = 1, mem=None) -> bytes:
+        """fold_and_prove over step witnesses already resident in HBM (device pointers to u64[m]);
+        mem: per-step memory instances (Route A) or None."""
         arr = (C.c_void_p * len(z_ptrs))(*z_ptrs)
         out = C.POINTER(C.c_uint8)()
         n = C.c_size_t()
         sec = C.c_double()
-        _check(lib().ns_prove_shard_device(self.h, arr, len(z_ptrs), m_in, C.byref(out), C.byref(n),
-                                           C.byref(sec)))
+        if mem is not None:
+            keep = []
+            marr = _step_mem_array(mem, keep)
+            _check(lib().ns_prove_shard_mem(self.h, arr, 1, len(z_ptrs), m_in, C.cast(marr, C.c_void_p), C.byref(out),
+                                            C.byref(n), C.byref(sec)))
+        else:
+            _check(lib().ns_prove_shard_device(self.h, arr, len(z_ptrs), m_in, C.byref(out), C.byref(n),
+                                               C.byref(sec)))
         data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)
         lib().ns_free(C.cast(out, C.c_void_p))
         self.last_seconds = sec.value
@@ -381,7 +444,7 @@ class FoldingSession:
         """Reuse this session's device buffers for another CCS / PP (same semantics as a new session)."""
         _check(lib().ns_prover_rebind(self.h, ccs.h, pp.h))
         self.ccs, self.pp = ccs, pp
-        self.steps, self.m_in = [], None
+        self.steps, self.mem, self.m_in = [], [], None
 
     def set_timing(self, on: bool):
         _check(lib().ns_prover_set_timing(self.h, int(on)))

@@ -8,6 +8,7 @@
 #include "../../include/nightstream.h"
 #include "prover.h"
 #include "twist.h"
+#include "route_a.h"
 
 using namespace ns;
 
@@ -420,6 +421,42 @@ int ns_prove_shard_device(ns_prover* p, const uint64_t* const* z_steps_dev, size
   return guard([&] {
     if (n_steps == 0) throw Error(1, "need at least one step");
     std::vector<uint64_t> enc = p->p->prove_shard(z_steps_dev, n_steps, m_in, true);
+    uint8_t* buf = (uint8_t*)malloc(enc.size() * 8);
+    if (!buf) throw Error(1, "out of host memory");
+    memcpy(buf, enc.data(), enc.size() * 8);
+    *proof = buf;
+    *proof_len = enc.size() * 8;
+    if (seconds) *seconds = p->p->last_seconds;
+  });
+}
+
+int ns_prove_shard_mem(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps, size_t m_in,
+                       const ns_step_mem* mem, uint8_t** proof, size_t* proof_len, double* seconds) {
+  return guard([&] {
+    if (!p || !z_steps || !proof || !proof_len) throw Error(1, "null argument");
+    if (n_steps == 0) throw Error(1, "need at least one step");
+    // the per-step instances as the prover's host structs (the arrays stay the caller's)
+    std::vector<std::vector<LutInstH>> luts(n_steps);
+    std::vector<std::vector<MemInstH>> mems(n_steps);
+    std::vector<StepMemH> steps(n_steps);
+    for (size_t s = 0; mem && s < n_steps; s++) {
+      for (size_t i = 0; i < mem[s].n_lut; i++) {
+        const ns_lut_inst& a = mem[s].luts[i];
+        if (!a.table && a.k) throw Error(1, "LUT instance without a table");
+        if (a.d * a.ell > 32 || a.k != ((size_t)1 << (a.d * a.ell))) throw Error(1, "LUT table size must be 2^(d*ell)");
+        luts[s].push_back(LutInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.table});
+      }
+      for (size_t i = 0; i < mem[s].n_mem; i++) {
+        const ns_mem_inst& a = mem[s].mems[i];
+        if (a.d * a.ell > 32) throw Error(1, "Twist address width above 32 bits");
+        for (size_t q = 1; a.init_sparse && q < a.ninit; q++)
+          if (a.init_addrs[q] <= a.init_addrs[q - 1]) throw Error(1, "MemInit::Sparse must be strictly increasing by address");
+        mems[s].push_back(MemInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.init_sparse != 0,
+                                   a.init_sparse ? a.ninit : 0, a.init_addrs, a.init_vals});
+      }
+      steps[s] = StepMemH{luts[s].size(), luts[s].data(), mems[s].size(), mems[s].data()};
+    }
+    std::vector<uint64_t> enc = p->p->prove_shard(z_steps, n_steps, m_in, z_on_device != 0, mem ? steps.data() : nullptr);
     uint8_t* buf = (uint8_t*)malloc(enc.size() * 8);
     if (!buf) throw Error(1, "out of host memory");
     memcpy(buf, enc.data(), enc.size() * 8);

@@ -166,6 +166,18 @@ struct PlanePtrs {
   const int8_t* p[16];
 };
 void launch_head_cols(const PlanePtrs& planes, int np, size_t m, int m_in, int8_t* out, hipStream_t s);
+
+// ---- shared CPU bus (bus.hip) ----
+// w[j] = chi_r(m_in + j) for j < chunk (r: ell K values on the device)
+void launch_bus_weights(uint32_t m_in, uint32_t chunk, const K2* r_dev, int ell, K2* w, hipStream_t s);
+// out[i] = decoded z[base + i] from the digit planes Z [54][m]
+void launch_bus_decode_range(const int8_t* Z, size_t m, size_t base, size_t len, uint64_t* out, hipStream_t s);
+// bus openings of np digit planes: y[(p*bus_cols + col)*64 + rho], ys[p*bus_cols + col]
+void launch_bus_open_planes(const PlanePtrs& planes, int np, size_t m, size_t bus_base, uint32_t chunk,
+                            uint32_t bus_cols, const K2* w, K2* y, K2* ys, hipStream_t s);
+// the same for one int32 matrix [54][m] (Z_mix)
+void launch_bus_open_i32(const int32_t* Z, size_t m, size_t bus_base, uint32_t chunk, uint32_t bus_cols, const K2* w,
+                         K2* y, K2* ys, hipStream_t s);
 // balanced base-2 digits of zmix into k_dec planes (by value: no pointer table in device memory);
 // flags[0..k_dec] are cleared by the launch, then flags[i] = plane i nonzero, flags[k_dec] = range error
 constexpr int DEC_MAX_PLANES = 30;

@@ -11,6 +11,7 @@
 #include <array>
 
 #include "prover.h"
+#include "route_a.h"
 
 #include <string.h>
 
@@ -875,8 +876,344 @@ static size_t log2_pow2(size_t n) {
   return l < 1 ? 1 : l;
 }
 
+// ---------------- shared-bus openings on the device ----------------
+void Prover::bus_weights(const std::vector<K2>& r, const BusLayout& L) {
+  rbits_.ensure(std::max<size_t>(r.size(), 1) * sizeof(K2));
+  busw_.ensure(L.chunk * sizeof(K2));
+  h2d(rbits_.p, r.data(), r.size(), stream_);
+  launch_bus_weights((uint32_t)L.m_in, (uint32_t)L.chunk, rbits_.as<K2>(), (int)r.size(), busw_.as<K2>(), stream_);
+}
+
+// append_bus_openings_to_me_instance (cpu_bus.rs:195-320) for every (plane, ME) pair at the
+// point whose weights bus_weights() left in busw_
+void Prover::open_bus_planes(const std::vector<const int8_t*>& planes, const BusLayout& L,
+                             const std::vector<MeHost*>& outs, size_t t_core) {
+  const size_t bc = L.bus_cols;
+  if (!bc) return;
+  for (size_t p0 = 0; p0 < planes.size(); p0 += 16) {
+    const size_t np = std::min<size_t>(16, planes.size() - p0);
+    PlanePtrs pl{};
+    for (size_t q = 0; q < np; q++) pl.p[q] = planes[p0 + q];
+    busy_.ensure(np * bc * 64 * sizeof(K2));
+    busys_.ensure(np * bc * sizeof(K2));
+    launch_bus_open_planes(pl, (int)np, s_->m, L.bus_base, (uint32_t)L.chunk, (uint32_t)bc, busw_.as<K2>(),
+                           busy_.as<K2>(), busys_.as<K2>(), stream_);
+    std::vector<K2> y(np * bc * 64), ys(np * bc);
+    d2h(ows_.stage, y.data(), busy_.p, y.size(), stream_);
+    d2h(ows_.stage, ys.data(), busys_.p, ys.size(), stream_);
+    for (size_t q = 0; q < np; q++) {
+      MeHost& e = *outs[p0 + q];
+      std::copy(y.begin() + q * bc * 64, y.begin() + (q + 1) * bc * 64, e.y.begin() + t_core * DPAD);
+      std::copy(ys.begin() + q * bc, ys.begin() + (q + 1) * bc, e.y_scalars.begin() + t_core);
+    }
+  }
+}
+
+void Prover::open_bus_i32(const int32_t* zmix, const BusLayout& L, MeHost& out, size_t t_core) {
+  const size_t bc = L.bus_cols;
+  if (!bc) return;
+  busy_.ensure(bc * 64 * sizeof(K2));
+  busys_.ensure(bc * sizeof(K2));
+  launch_bus_open_i32(zmix, s_->m, L.bus_base, (uint32_t)L.chunk, (uint32_t)bc, busw_.as<K2>(), busy_.as<K2>(),
+                      busys_.as<K2>(), stream_);
+  d2h(ows_.stage, out.y.data() + t_core * DPAD, busy_.p, bc * 64, stream_);
+  d2h(ows_.stage, out.y_scalars.data() + t_core, busys_.p, bc, stream_);
+}
+
+// v_j = M_j^T chi_r for every matrix (the vectors precompute_for_r contracts the digits with)
+void Prover::vjs_at(const std::vector<K2>& r, DevBuf& vjs) {
+  const Ccs& s = *s_;
+  const size_t ell = r.size();
+  rbits_.ensure(std::max<size_t>(ell, 1) * sizeof(K2));
+  rchi_.ensure(((size_t)1 << ell) * sizeof(K2));
+  h2d(rbits_.p, r.data(), ell, stream_);
+  launch_chi(rbits_.as<K2>(), (int)ell, rchi_.as<K2>(), stream_);
+  vjs.ensure(s.t * s.m * sizeof(K2));
+  const size_t n_eff = std::min(s.n, (size_t)1 << ell);
+  for (size_t j = 0; j < s.t; j++) launch_spmv_t(s.dev[j], rchi_.as<K2>(), n_eff, vjs.as<K2>() + j * s.m, stream_);
+}
+
+// y[w][j][rho] = sum_c Z_w[rho][c] v_j[c] for device digit planes (int8 MFMA contraction)
+std::vector<K2> Prover::y_products(const std::vector<const int8_t*>& planes, const DevBuf& vjs) {
+  const Ccs& s = *s_;
+  const size_t nw = planes.size();
+  yptr_.ensure(nw * sizeof(void*));
+  h2d(yptr_.p, (const void* const*)planes.data(), nw, stream_);
+  ws_yd_.ensure(ydigits_ws_bytes(nw, s.t, s.m));
+  ydev_.ensure(nw * s.t * D * sizeof(K2));
+  launch_ydigits(yptr_.as<const int8_t*>(), nw, vjs.as<K2>(), s.t, s.m, ydev_.as<K2>(), ws_yd_.p, stream_);
+  std::vector<K2> out(nw * s.t * D);
+  d2h(ows_.stage, out.data(), ydev_.p, out.size(), stream_);
+  return out;
+}
+
+void Prover::head_cols(const int8_t* const* planes, size_t np, size_t m_in, std::vector<std::vector<uint64_t>*> Xs) {
+  for (size_t p = 0; p < np; p++) Xs[p]->assign((size_t)D * m_in, 0);
+  if (!m_in || !np) return;
+  for (size_t p0 = 0; p0 < np; p0 += 16) {
+    const size_t cnt = std::min<size_t>(16, np - p0);
+    PlanePtrs pl{};
+    for (size_t p = 0; p < cnt; p++) pl.p[p] = planes[p0 + p];
+    head_.ensure(cnt * D * m_in);
+    launch_head_cols(pl, (int)cnt, s_->m, (int)m_in, head_.as<int8_t>(), stream_);
+    std::vector<int8_t> tmp(cnt * D * m_in);
+    d2h(ows_.stage, tmp.data(), head_.p, tmp.size(), stream_);
+    for (size_t p = 0; p < cnt; p++)
+      for (size_t i = 0; i < (size_t)D * m_in; i++) (*Xs[p0 + p])[i] = from_i64(tmp[p * D * m_in + i]);
+  }
+}
+
+// bind_rlc_inputs (shard.rs:1125-1168) + sample_rot_rhos_n + Pi_RLC + Pi_DEC of one lane
+// (prove_rlc_dec_lane, shard.rs:1170-1331).  ins' witnesses are Zs (device planes); the children's
+// digit planes go to `bank` (16 planes); vjs: M_j^T chi_r at the lane's point r = ins[0].r.
+void Prover::rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const std::vector<MeHost>& ins,
+                          const std::vector<const int8_t*>& Zs, DevBuf* bank, const DevBuf& vjs, const BusLayout& L,
+                          LaneOut& out) {
+  const Ccs& s = *s_;
+  const size_t m = s.m, t = s.t, k_dec = params_.k_rho, kt = ins.size(), m_in = ins[0].m_in;
+  const size_t t_tot = t + L.bus_cols;
+  hipStream_t st = stream_;
+  tr.append_message("fold/rlc_inputs/v1", lane);
+  {
+    uint64_t v = idx;
+    tr.append_u64s("step_idx", &v, 1);
+    v = kt;
+    tr.append_u64s("me_count", &v, 1);
+    for (auto& e : ins) {
+      tr.append_fields("c_data", e.c, KAPPA * D);
+      v = e.m_in;
+      tr.append_u64s("m_in", &v, 1);
+      tr.append_message("me_fold_digest", e.fold_digest, 32);
+      for (auto& x : e.r) tr.append_k("r_limb", x);
+      tr.append_fields("X", e.X.data(), e.X.size());
+      for (auto& x : e.y) tr.append_k("y_elem", x);
+      for (auto& x : e.y_scalars) tr.append_k("y_scalar", x);
+      v = 0;
+      tr.append_u64s("c_step_coords_len", &v, 1);
+      tr.append_fields("c_step_coords", nullptr, 0);
+      tr.append_u64s("u_offset", &v, 1);
+      tr.append_u64s("u_len", &v, 1);
+    }
+  }
+  if ((uint64_t)kt * 216u * (params_.b - 1) >= ((uint64_t)1 << params_.k_rho))
+    throw Error(5, "RLC norm bound violated");
+  std::vector<int8_t> rho_a(kt * D);
+  out.rhos.assign(kt * D * D, 0);
+  std::vector<int8_t> rho_i8(kt * 64 * 64, 0);  // [i][row][col] padded to 64 x 64
+  for (size_t i = 0; i < kt; i++) {
+    tr.append_msg_u64("rlc/rot/index", i);
+    // draw_alphabet_vector (neo-reductions/src/common.rs:410-439)
+    static const int8_t A5[5] = {-2, -1, 0, 1, 2};
+    const uint32_t bucket = (65536u / 5u) * 5u;
+    size_t got = 0;
+    uint64_t ctr = i;
+    while (got < (size_t)D) {
+      tr.append_msg_u64("rlc/rot/chunk", ctr);
+      uint8_t dig[32];
+      tr.digest32(dig);
+      for (int w = 0; w < 16 && got < (size_t)D; w++) {
+        uint32_t x = (uint32_t)dig[2 * w] | ((uint32_t)dig[2 * w + 1] << 8);
+        if (x < bucket) rho_a[i * D + got++] = A5[x % 5];
+      }
+      ctr++;
+    }
+    uint64_t* R = &out.rhos[i * D * D];
+    if (kt == 1) {
+      for (int q = 0; q < D; q++) R[q * D + q] = 1;
+    } else {
+      // rot_from_coeffs: column j = X^j * a mod Phi_81
+      uint64_t col[D], nxt[D];
+      for (int r = 0; r < D; r++) col[r] = from_i64(rho_a[i * D + r]);
+      for (int j = 0; j < D; j++) {
+        for (int r = 0; r < D; r++) R[r * D + j] = col[r];
+        uint64_t last = col[D - 1];
+        nxt[0] = fneg(last);
+        for (int r = 1; r < D; r++) nxt[r] = col[r - 1];
+        nxt[27] = fsub(nxt[27], last);
+        memcpy(col, nxt, sizeof(col));
+      }
+    }
+    for (int q = 0; q < D * D; q++) {
+      int64_t v = to_balanced(R[q]);
+      if (v < -127 || v > 127) throw Error(5, "rotation entry out of int8 range");
+      rho_i8[(i * 64 + q / D) * 64 + q % D] = (int8_t)v;
+    }
+  }
+  std::vector<K2> bpow(D);
+  bpow[0] = kone();
+  for (int i = 1; i < D; i++) bpow[i] = kadd(bpow[i - 1], bpow[i - 1]);
+  MeHost& parent = out.parent;
+  parent.m_in = m_in;
+  parent.t = kt == 1 ? ins[0].t : t_tot;
+  parent.r = ins[0].r;
+  memcpy(parent.fold_digest, ins[0].fold_digest, 32);
+  parent.X.assign((size_t)D * m_in, 0);
+  parent.y.assign(parent.t * DPAD, kzero());
+  parent.y_scalars.assign(parent.t, kzero());
+  // Z_mix on the device (the identity rotation when kt = 1: Z_mix = Z_0)
+  zmix_.ensure((size_t)D * m * 4);
+  rho_dev_.ensure(rho_i8.size());
+  zptr_.ensure(kt * sizeof(void*));
+  h2d(rho_dev_.p, rho_i8.data(), rho_i8.size(), st);
+  h2d(zptr_.p, (const void* const*)Zs.data(), kt, st);
+  launch_rlc_zmix(rho_dev_.as<int8_t>(), zptr_.as<const int8_t*>(), kt, m, zmix_.as<int32_t>(), st);
+  const int32_t* zsrc = zmix_.as<int32_t>();
+  if (kt == 1) {
+    memcpy(parent.c, ins[0].c, sizeof(parent.c));
+    parent.X = ins[0].X;
+    parent.y = ins[0].y;
+    for (size_t j = 0; j < parent.t; j++) {
+      K2 sc = kzero();
+      for (int rho = 0; rho < D; rho++) sc = kadd(sc, kmul(parent.y[j * DPAD + rho], bpow[rho]));
+      parent.y_scalars[j] = sc;
+    }
+  } else {
+    for (size_t i = 0; i < kt; i++) {
+      const uint64_t* R = &out.rhos[i * D * D];
+      for (int rr = 0; rr < D; rr++)
+        for (int kk = 0; kk < D; kk++) {
+          uint64_t co = R[rr * D + kk];
+          if (!co) continue;
+          for (size_t c = 0; c < m_in; c++)
+            parent.X[rr * m_in + c] = fadd(parent.X[rr * m_in + c], fmul(co, ins[i].X[kk * m_in + c]));
+        }
+    }
+    // y_j = sum_i rot(a_i) y_ij and c = sum_i rot(a_i) c_i: rot(a) v is a(X) v(X) mod Phi_81
+    // with the small coefficients a_i in [-2, 2] (mix_small_coeffs); core rows only
+    std::vector<const uint64_t*> vp(kt);
+    std::vector<uint64_t> vbuf(kt * D * 2), outc(D);
+    for (size_t j = 0; j < t; j++) {
+      for (int comp = 0; comp < 2; comp++) {
+        for (size_t i = 0; i < kt; i++) {
+          const K2* y = &ins[i].y[j * DPAD];
+          for (int v = 0; v < D; v++) vbuf[i * D + v] = comp ? y[v].c1 : y[v].c0;
+          vp[i] = &vbuf[i * D];
+        }
+        mix_small_coeffs(rho_a.data(), vp.data(), kt, outc.data());
+        for (int q = 0; q < D; q++) (comp ? parent.y[j * DPAD + q].c1 : parent.y[j * DPAD + q].c0) = outc[q];
+      }
+    }
+    for (int col = 0; col < KAPPA; col++) {
+      for (size_t i = 0; i < kt; i++) vp[i] = &ins[i].c[col * D];
+      mix_small_coeffs(rho_a.data(), vp.data(), kt, &parent.c[col * D]);
+    }
+    for (size_t j = 0; j < t; j++) {
+      K2 sc = kzero();
+      for (int rho = 0; rho < D; rho++) sc = kadd(sc, kmul(parent.y[j * DPAD + rho], bpow[rho]));
+      parent.y_scalars[j] = sc;
+    }
+    if (L.bus_cols) {
+      bus_weights(parent.r, L);
+      open_bus_i32(zsrc, L, parent, t);
+    }
+  }
+
+  // ---- Pi_DEC ----
+  if (k_dec > 16) throw Error(1, "k_rho > 16 is not supported");
+  std::vector<const int8_t*> pptr(k_dec);
+  DecPlanes dec_pl{};
+  for (size_t i = 0; i < k_dec; i++) {
+    bank[i].ensure((size_t)D * m);
+    pptr[i] = bank[i].as<int8_t>();
+    dec_pl.p[i] = bank[i].as<int8_t>();
+  }
+  flags_.ensure((k_dec + 1) * 4);
+  launch_dec_split(zsrc, (size_t)D * m, (int)k_dec, dec_pl, flags_.as<int>(), st);
+  std::vector<int> fl(k_dec + 1);
+  d2h(ows_.stage, fl.data(), flags_.p, k_dec + 1, st);
+  if (fl[k_dec]) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
+  std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
+  commit_device(pptr.data(), k_dec, child_c.data());
+  std::vector<K2> y_child = y_products(pptr, vjs);
+  out.children.assign(k_dec, MeHost{});
+  {
+    std::vector<std::vector<uint64_t>*> xs;
+    for (auto& c : out.children) xs.push_back(&c.X);
+    head_cols(pptr.data(), k_dec, m_in, xs);
+  }
+  for (size_t i = 0; i < k_dec; i++) {
+    MeHost& e = out.children[i];
+    if (fl[i]) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
+    else memset(e.c, 0, sizeof(e.c));
+    e.m_in = m_in;
+    e.t = t_tot;
+    e.r = parent.r;
+    e.y.assign(t_tot * DPAD, kzero());
+    e.y_scalars.assign(t_tot, kzero());
+    for (size_t j = 0; j < t; j++) {
+      K2 sc = kzero();
+      for (int rho = 0; rho < D; rho++) {
+        K2 v = y_child[(i * t + j) * D + rho];
+        e.y[j * DPAD + rho] = v;
+        sc = kadd(sc, kmul(v, bpow[rho]));
+      }
+      e.y_scalars[j] = sc;
+    }
+    memcpy(e.fold_digest, parent.fold_digest, 32);
+  }
+  if (L.bus_cols) {
+    bus_weights(parent.r, L);
+    std::vector<MeHost*> outs;
+    for (auto& c : out.children) outs.push_back(&c);
+    open_bus_planes(pptr, L, outs, t);
+  }
+  {
+    bool ok_y = true, ok_x = true, ok_c = true;
+    for (size_t j = 0; j < t && ok_y; j++)
+      for (int q = 0; q < DPAD; q++) {
+        K2 lhs = kzero();
+        uint64_t pw = 1;
+        for (size_t i = 0; i < k_dec; i++) {
+          lhs = kadd(lhs, kscale(out.children[i].y[j * DPAD + q], pw));
+          pw = fadd(pw, pw);
+        }
+        if (!keq(lhs, parent.y[j * DPAD + q])) {
+          ok_y = false;
+          break;
+        }
+      }
+    for (size_t q = 0; q < (size_t)D * m_in; q++) {
+      uint64_t lhs = 0, pw = 1;
+      for (size_t i = 0; i < k_dec; i++) {
+        lhs = fadd(lhs, fmul(out.children[i].X[q], pw));
+        pw = fadd(pw, pw);
+      }
+      if (lhs != parent.X[q]) ok_x = false;
+    }
+    for (int q = 0; q < KAPPA * D; q++) {
+      uint64_t lhs = out.children[0].c[q], pw = 2;
+      for (size_t i = 1; i < k_dec; i++) {
+        lhs = fadd(lhs, fmul(out.children[i].c[q], pw));
+        pw = fadd(pw, pw);
+      }
+      if (lhs != parent.c[q]) ok_c = false;
+    }
+    if (!(ok_y && ok_x && ok_c))
+      throw Error(7, std::string(strcmp(lane, "val") ? "DEC" : "DEC(val)") + " public check failed at step " +
+                         std::to_string(idx) + " (y=" + std::to_string(ok_y) + ", X=" + std::to_string(ok_x) +
+                         ", c=" + std::to_string(ok_c) + ")");
+  }
+  out.planes = pptr;
+}
+
+static void enc_rounds(Enc& pb, const std::vector<K2>& polys, size_t rounds, size_t np) {
+  pb.u(rounds);
+  for (size_t rd = 0; rd < rounds; rd++) {
+    pb.u(np);
+    for (size_t i = 0; i < np; i++) pb.k(polys[rd * np + i]);
+  }
+}
+
+static void enc_lane(Enc& pb, const LaneOut& o, size_t k_dec) {
+  pb.u(o.rhos.size() / (D * D));
+  for (auto x : o.rhos) pb.u(x);
+  pb.me(o.parent);
+  pb.u(k_dec);
+  for (auto& e : o.children) pb.me(e);
+}
+
 std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t nsteps, size_t m_in,
-                                          bool z_on_device) {
+                                          bool z_on_device, const StepMemH* mem) {
   using clk = std::chrono::steady_clock;
   std::unique_ptr<KTimers> kt;
   if (timing) {
@@ -915,6 +1252,15 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   };
   phase_mark(10);  // step intake
 
+  const StepMemH no_mem{};
+  BusLayout L;
+  if (mem) {
+    L = infer_bus_layout(mem, nsteps, s.n, m, m_in);
+  } else {
+    L.m = m;
+    L.m_in = m_in;
+  }
+  const size_t t_tot = t + L.bus_cols;
 
   // ---- step intake: digits and MCS commitments for all steps ----
   if (zmcs_.size() < nsteps) zmcs_.resize(nsteps);
@@ -939,6 +1285,7 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     }
     NS_CHECK(hipStreamSynchronize(st));
   }
+  if (mem) check_bus_guardrails(s, L, x_host);
   std::vector<uint64_t> cmcs(nsteps * KAPPA * D);
   {
     std::vector<const int8_t*> ptrs(nsteps);
@@ -952,8 +1299,9 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   Transcript tr("neo.fold/session");
   trp = &tr;
   pmark = tr.permutations();
+  DigestCache digests;
 
-  std::vector<MeHost> acc;
+  std::vector<MeHost> acc, val_obl;
   std::vector<const int8_t*> acc_wit;  // digit planes of the accumulator (planes_[cur_planes])
   int cur_planes = 0;
   DevBuf& vjs = vjs_;
@@ -964,14 +1312,14 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   std::vector<K2> bpow(D);
   bpow[0] = kone();
   for (int i = 1; i < D; i++) bpow[i] = kadd(bpow[i - 1], bpow[i - 1]);
+  std::vector<TwistCols> prev_tw;
+  bool have_prev = false;
+  std::vector<uint64_t> zbus;
 
   for (size_t idx = 0; idx < nsteps; idx++) {
+    const StepMemH& sm = mem ? mem[idx] : no_mem;
     const size_t nacc = acc.size(), kt = nacc + 1;
-    // absorb_step_memory (no LUT / MEM instances)
-    tr.append_message("step/absorb_memory_start", (const uint8_t*)nullptr, 0);
-    tr.append_msg_u64("step/lut_count", 0);
-    tr.append_msg_u64("step/mem_count", 0);
-    tr.append_message("step/absorb_memory_done", (const uint8_t*)nullptr, 0);
+    absorb_step_memory(tr, sm, digests);
     phase_mark(0);
     // bind_header_and_instances_with_digest
     tr.append_message("phase/pi-ccs", (const uint8_t*)nullptr, 0);
@@ -1068,10 +1416,10 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       T = kmul(gk, inner);
     }
     tr.append_k("sumcheck/initial_sum", T);
+    std::vector<K2> r_cycle(ell_n);
     for (size_t i = 0; i < ell_n; i++) {
       tr.append_msg_u64("route_a/r_cycle", i);
-      (void)tr.challenge_field("route_a/cycle/0");
-      (void)tr.challenge_field("route_a/cycle/1");
+      r_cycle[i] = tr.challenge_k("route_a/cycle/0", "route_a/cycle/1");
     }
     phase_mark(1);
     // device oracle
@@ -1080,41 +1428,36 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     for (size_t a = 0; a < nacc; a++) Zs[1 + a] = acc_wit[a];
     CcsOracle oracle(s, Zs.data(), kt, alpha.data(), beta_a, beta_r, gamma, ell_d, ell_n,
                      nacc ? acc[0].r.data() : nullptr, st, &ows_);
+    // Route A memory: the bus region decoded from the step's digit planes, then the address-lane
+    // sum-checks (Shout first) and the time-lane oracles (memory.rs:232-1025)
+    ShoutPre sp;
+    std::vector<TwistPre> tp;
+    std::vector<std::unique_ptr<TsOracleGpu>> mem_owned;
+    std::vector<RaClaim> claims(1);
+    claims[0].ccs = &oracle;
+    claims[0].deg = d_sc;
+    claims[0].label = "ccs/time";
+    claims[0].dyn = true;
+    claims[0].sum = T;
+    if (sm.n_lut || sm.n_mem) {
+      const size_t blen = L.bus_cols * L.chunk;
+      zbus.resize(blen);
+      busz_.ensure(blen * 8);
+      launch_bus_decode_range(Zmcs[idx].as<int8_t>(), m, L.bus_base, blen, busz_.as<uint64_t>(), st);
+      d2h(ows_.stage, zbus.data(), busz_.p, blen, st);
+      shout_addr_pre(tr, sm, L, zbus.data(), ell_n, r_cycle, idx, sp);
+      twist_addr_pre(tr, sm, L, zbus.data(), ell_n, r_cycle, tp);
+      build_time_claims(sm, ell_n, r_cycle, sp, tp, mem_owned, claims);
+    }
     phase_mark(2);
-    // batched time sum-check with the single CCS claim
-    tr.append_msg_u64("batched/dynamic_bind/len", 1);
-    tr.append_message("batched/dynamic_bind/claim_label", "ccs/time");
-    tr.append_msg_u64("batched/dynamic_bind/claim_idx", 0);
-    tr.append_msg_u64("batched/dynamic_bind/degree_bound", d_sc);
-    {
-      uint8_t one = 1;
-      tr.append_message("batched/dynamic_bind/is_dynamic", &one, 1);
-    }
-    tr.append_k("batched/dynamic_bind/claimed_sum", T);
-    tr.append_message("sc/domain", "shard/batched_time");
-    tr.append_msg_u64("sc/inst_idx", idx);
-    K2 running = T;
-    std::vector<K2> time_coeffs(ell_n * xs_n), r_time(ell_n);
-    for (size_t rd = 0; rd < ell_n; rd++) {
-      tr.append_msg_u64("batched/round_idx", rd);
-      std::vector<K2> c = oracle.round_coeffs();
-      if (c.size() > xs_n) throw Error(7, "round polynomial exceeds the degree bound");
-      // the reference interpolates the values at 0..d_sc; the polynomial of degree < d_sc + 1
-      // through them is c itself, padded with zeros
-      K2* co = &time_coeffs[rd * xs_n];
-      for (size_t i = 0; i < xs_n; i++) co[i] = i < c.size() ? c[i] : kzero();
-      if (!keq(kadd(poly_eval(co, xs_n, xs[0]), poly_eval(co, xs_n, xs[1])), running))
-        throw Error(4, "step " + std::to_string(idx) + ": batched round " + std::to_string(rd) + " invariant failed");
-      tr.append_message("batched/claim_label", "ccs/time");
-      tr.append_msg_u64("batched/claim_idx", 0);
-      for (size_t i = 0; i < xs_n; i++) tr.append_k("batched/round/coeff", co[i]);
-      K2 r = tr.challenge_k("batched/challenge/0", "batched/challenge/1");
-      r_time[rd] = r;
-      running = poly_eval(co, xs_n, r);
-      oracle.fold(r);
-    }
-    tr.append_message("sc/domain_end", "shard/batched_time");
-    tr.append_msg_u64("sc/inst_idx_end", idx);
+    // the batched time sum-check: the CCS claim and the memory time lanes (route_a_time.rs:28-152)
+    bind_dynamic_claims(tr, claims);
+    std::vector<K2> r_time;
+    run_batched(tr, "shard/batched_time", idx, claims, ell_n, r_time);
+    K2 running = claims[0].running;
+    const std::vector<K2>& time_coeffs = claims[0].polys;
+    for (auto& c : claims) c.ts = nullptr;
+    mem_owned.clear();
 
     phase_mark(3);
     // precompute_for_r on the device
@@ -1145,37 +1488,23 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     tr.digest32(fold_digest);
 
     phase_mark(5);
-    // ME outputs
-    // X = the first m_in columns of each digit plane, gathered on the device in one launch
-    auto head_cols = [&](const int8_t* const* planes, size_t np, std::vector<std::vector<uint64_t>*> Xs) {
-      for (size_t p = 0; p < np; p++) Xs[p]->assign((size_t)D * m_in, 0);
-      if (!m_in || !np) return;
-      if (np > 16) throw Error(1, "at most 16 planes per head-column gather");
-      PlanePtrs pl{};
-      for (size_t p = 0; p < np; p++) pl.p[p] = planes[p];
-      head_.ensure(np * D * m_in);
-      launch_head_cols(pl, (int)np, m, (int)m_in, head_.as<int8_t>(), st);
-      std::vector<int8_t> tmp(np * D * m_in);
-      d2h(ows_.stage, tmp.data(), head_.p, tmp.size(), st);
-      for (size_t p = 0; p < np; p++)
-        for (size_t i = 0; i < (size_t)D * m_in; i++) (*Xs[p])[i] = from_i64(tmp[p * D * m_in + i]);
-    };
+    // ME outputs, with the shared-bus openings appended (shard.rs:2082-2120)
     std::vector<MeHost> ccs_out(kt);
     for (size_t i = 0; i < kt; i++) {
       MeHost& e = ccs_out[i];
       e.m_in = m_in;
-      e.t = t;
+      e.t = t_tot;
       if (i == 0) {
         memcpy(e.c, &cmcs[idx * KAPPA * D], sizeof(e.c));
         const int8_t* pl0 = Zmcs[idx].as<int8_t>();
-        head_cols(&pl0, 1, {&e.X});
+        head_cols(&pl0, 1, m_in, {&e.X});
       } else {
         memcpy(e.c, acc[i - 1].c, sizeof(e.c));
         e.X = acc[i - 1].X;
       }
       e.r = r_time;
-      e.y.assign(t * DPAD, kzero());
-      e.y_scalars.assign(t, kzero());
+      e.y.assign(t_tot * DPAD, kzero());
+      e.y_scalars.assign(t_tot, kzero());
       for (size_t j = 0; j < t; j++) {
         K2 sc = kzero();
         for (int rho = 0; rho < D; rho++) {
@@ -1187,222 +1516,65 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       }
       memcpy(e.fold_digest, fold_digest, 32);
     }
+    if (L.bus_cols) {
+      bus_weights(r_time, L);
+      std::vector<MeHost*> outs;
+      for (auto& e : ccs_out) outs.push_back(&e);
+      open_bus_planes(Zs, L, outs, t);
+    }
+    // finalize_route_a_memory_prover: val-eval batch, CPU ME claims at r_val (memory.rs:1284-1670)
+    ValPre vp;
+    std::vector<MeHost> vclaims;
+    std::vector<const int8_t*> vZ;
+    if (sm.n_mem) {
+      val_eval_batch(tr, sm, tp, (idx > 0 && have_prev) ? &prev_tw : nullptr, ell_n, r_time, idx, vp);
+      Transcript fork = tr;
+      fork.append_message("fork", "cpu_bus/me_digest_val");
+      uint8_t vdig[32];
+      fork.digest32(vdig);
+      const size_t nvc = idx > 0 ? 2 : 1;
+      vjs_at(vp.r_val, vjs_val_);
+      for (size_t q = 0; q < nvc; q++) vZ.push_back(Zmcs[idx - q].as<int8_t>());
+      std::vector<K2> yv = y_products(vZ, vjs_val_);
+      vclaims.resize(nvc);
+      for (size_t q = 0; q < nvc; q++) {
+        MeHost& e = vclaims[q];
+        const size_t stp = idx - q;
+        memcpy(e.c, &cmcs[stp * KAPPA * D], sizeof(e.c));
+        e.m_in = m_in;
+        e.t = t_tot;
+        head_cols(&vZ[q], 1, m_in, {&e.X});
+        e.r = vp.r_val;
+        e.y.assign(t_tot * DPAD, kzero());
+        e.y_scalars.assign(t_tot, kzero());
+        for (size_t j = 0; j < t; j++) {
+          K2 sc = kzero();
+          for (int rho = 0; rho < D; rho++) {
+            K2 v = yv[(q * t + j) * D + rho];
+            e.y[j * DPAD + rho] = v;
+            sc = kadd(sc, kmul(v, bpow[rho]));
+          }
+          e.y_scalars[j] = sc;
+        }
+        memcpy(e.fold_digest, vdig, 32);
+      }
+      if (L.bus_cols) {
+        bus_weights(vp.r_val, L);
+        std::vector<MeHost*> outs;
+        for (auto& e : vclaims) outs.push_back(&e);
+        open_bus_planes(vZ, L, outs, t);
+      }
+    }
 
     phase_mark(6);
-    // ---- Pi_RLC ----
-    tr.append_message("fold/rlc_inputs/v1", "main");
-    {
-      uint64_t v = idx;
-      tr.append_u64s("step_idx", &v, 1);
-      v = kt;
-      tr.append_u64s("me_count", &v, 1);
-      for (auto& e : ccs_out) {
-        tr.append_fields("c_data", e.c, KAPPA * D);
-        v = e.m_in;
-        tr.append_u64s("m_in", &v, 1);
-        tr.append_message("me_fold_digest", e.fold_digest, 32);
-        for (auto& x : e.r) tr.append_k("r_limb", x);
-        tr.append_fields("X", e.X.data(), e.X.size());
-        for (auto& x : e.y) tr.append_k("y_elem", x);
-        for (auto& x : e.y_scalars) tr.append_k("y_scalar", x);
-        v = 0;
-        tr.append_u64s("c_step_coords_len", &v, 1);
-        tr.append_fields("c_step_coords", nullptr, 0);
-        tr.append_u64s("u_offset", &v, 1);
-        tr.append_u64s("u_len", &v, 1);
-      }
-    }
-    if ((uint64_t)kt * 216u * (params_.b - 1) >= ((uint64_t)1 << params_.k_rho))
-      throw Error(5, "RLC norm bound violated");
-    std::vector<int8_t> rho_a(kt * D);
-    std::vector<uint64_t> rhos(kt * D * D, 0);
-    std::vector<int8_t> rho_i8(kt * 64 * 64, 0);  // [i][row][col] padded to 64 x 64
-    for (size_t i = 0; i < kt; i++) {
-      tr.append_msg_u64("rlc/rot/index", i);
-      // draw_alphabet_vector (neo-reductions/src/common.rs:410-439)
-      static const int8_t A5[5] = {-2, -1, 0, 1, 2};
-      const uint32_t bucket = (65536u / 5u) * 5u;
-      size_t got = 0;
-      uint64_t ctr = i;
-      while (got < (size_t)D) {
-        tr.append_msg_u64("rlc/rot/chunk", ctr);
-        uint8_t dig[32];
-        tr.digest32(dig);
-        for (int w = 0; w < 16 && got < (size_t)D; w++) {
-          uint32_t x = (uint32_t)dig[2 * w] | ((uint32_t)dig[2 * w + 1] << 8);
-          if (x < bucket) rho_a[i * D + got++] = A5[x % 5];
-        }
-        ctr++;
-      }
-      uint64_t* R = &rhos[i * D * D];
-      if (kt == 1) {
-        for (int q = 0; q < D; q++) R[q * D + q] = 1;
-      } else {
-        // rot_from_coeffs: column j = X^j * a mod Phi_81
-        uint64_t col[D], nxt[D];
-        for (int r = 0; r < D; r++) col[r] = from_i64(rho_a[i * D + r]);
-        for (int j = 0; j < D; j++) {
-          for (int r = 0; r < D; r++) R[r * D + j] = col[r];
-          uint64_t last = col[D - 1];
-          nxt[0] = fneg(last);
-          for (int r = 1; r < D; r++) nxt[r] = col[r - 1];
-          nxt[27] = fsub(nxt[27], last);
-          memcpy(col, nxt, sizeof(col));
-        }
-      }
-      for (int q = 0; q < D * D; q++) {
-        int64_t v = to_balanced(R[q]);
-        if (v < -127 || v > 127) throw Error(5, "rotation entry out of int8 range");
-        rho_i8[(i * 64 + q / D) * 64 + q % D] = (int8_t)v;
-      }
-    }
-    MeHost parent;
-    parent.m_in = m_in;
-    parent.t = t;
-    parent.r = r_time;
-    memcpy(parent.fold_digest, fold_digest, 32);
-    parent.X.assign((size_t)D * m_in, 0);
-    parent.y.assign(t * DPAD, kzero());
-    parent.y_scalars.assign(t, kzero());
-    if (kt == 1) {
-      memcpy(parent.c, ccs_out[0].c, sizeof(parent.c));
-      parent.X = ccs_out[0].X;
-      parent.y = ccs_out[0].y;
-    } else {
-      for (size_t i = 0; i < kt; i++) {
-        const uint64_t* R = &rhos[i * D * D];
-        for (int rr = 0; rr < D; rr++)
-          for (int kk = 0; kk < D; kk++) {
-            uint64_t co = R[rr * D + kk];
-            if (!co) continue;
-            for (size_t c = 0; c < m_in; c++)
-              parent.X[rr * m_in + c] = fadd(parent.X[rr * m_in + c], fmul(co, ccs_out[i].X[kk * m_in + c]));
-          }
-      }
-      // y_j = sum_i rot(a_i) y_ij and c = sum_i rot(a_i) c_i: rot(a) v is a(X) v(X) mod Phi_81
-      // with the small coefficients a_i in [-2, 2] (mix_small_coeffs)
-      std::vector<const uint64_t*> vp(kt);
-      std::vector<uint64_t> vbuf(kt * D * 2), outc(D);
-      for (size_t j = 0; j < t; j++) {
-        for (int comp = 0; comp < 2; comp++) {
-          for (size_t i = 0; i < kt; i++) {
-            const K2* y = &ccs_out[i].y[j * DPAD];
-            for (int v = 0; v < D; v++) vbuf[i * D + v] = comp ? y[v].c1 : y[v].c0;
-            vp[i] = &vbuf[i * D];
-          }
-          mix_small_coeffs(rho_a.data(), vp.data(), kt, outc.data());
-          for (int q = 0; q < D; q++) (comp ? parent.y[j * DPAD + q].c1 : parent.y[j * DPAD + q].c0) = outc[q];
-        }
-      }
-      for (int col = 0; col < KAPPA; col++) {
-        for (size_t i = 0; i < kt; i++) vp[i] = &ccs_out[i].c[col * D];
-        mix_small_coeffs(rho_a.data(), vp.data(), kt, &parent.c[col * D]);
-      }
-    }
-    for (size_t j = 0; j < t; j++) {
-      K2 sc = kzero();
-      for (int rho = 0; rho < D; rho++) sc = kadd(sc, kmul(parent.y[j * DPAD + rho], bpow[rho]));
-      parent.y_scalars[j] = sc;
-    }
-    phase_mark(7);
-    // Z_mix on the device
-    zmix_.ensure((size_t)D * m * 4);
-    rho_dev_.ensure(rho_i8.size());
-    zptr_.ensure(kt * sizeof(void*));
-    DevBuf &zmix = zmix_, &rho_dev = rho_dev_, &zptr = zptr_;
-    h2d(rho_dev.p, rho_i8.data(), rho_i8.size(), st);
-    h2d(zptr.p, (const void* const*)Zs.data(), kt, st);
-    launch_rlc_zmix(rho_dev.as<int8_t>(), zptr.as<const int8_t*>(), kt, m, zmix.as<int32_t>(), st);
-
-    phase_mark(8);
-    // ---- Pi_DEC ----
-    if (k_dec > 16) throw Error(1, "k_rho > 16 is not supported");
+    // ---- main lane, then the val lane (shard.rs:2171-2254) ----
     const int nxt_planes = cur_planes ^ 1;
-    std::vector<const int8_t*> pptr(k_dec);
-    for (size_t i = 0; i < k_dec; i++) {
-      planes_[nxt_planes][i].ensure((size_t)D * m);
-      pptr[i] = planes_[nxt_planes][i].as<int8_t>();
-    }
-    pptr_dev_.ensure(k_dec * sizeof(void*));
-    flags_.ensure((k_dec + 1) * 4);
-    DevBuf &pptr_dev = pptr_dev_, &flags = flags_;
-    h2d(pptr_dev.p, (const void* const*)pptr.data(), k_dec, st);
-    DecPlanes dec_pl{};
-    for (size_t i = 0; i < k_dec; i++) dec_pl.p[i] = const_cast<int8_t*>(pptr[i]);
-    launch_dec_split(zmix.as<int32_t>(), (size_t)D * m, (int)k_dec, dec_pl, flags.as<int>(), st);
-    std::vector<int> fl(k_dec + 1);
-    d2h(ows_.stage, fl.data(), flags.p, k_dec + 1, st);
-    if (fl[k_dec]) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
-    std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
-    commit_device(pptr.data(), k_dec, child_c.data());
-    // child y = Z_i v_j with the v_j of the shared point r_time (already in vjs)
-    std::vector<K2> y_child(k_dec * t * D);
-    {
-      ws_yd.ensure(ydigits_ws_bytes(k_dec, t, m));
-      ydev_.ensure(y_child.size() * sizeof(K2));
-      launch_ydigits(pptr_dev.as<const int8_t*>(), k_dec, vjs.as<K2>(), t, m, ydev_.as<K2>(), ws_yd.p, st);
-      d2h(ows_.stage, y_child.data(), ydev_.p, y_child.size(), st);
-    }
-    std::vector<MeHost> children(k_dec);
-    {
-      std::vector<std::vector<uint64_t>*> xs;
-      for (auto& c : children) xs.push_back(&c.X);
-      head_cols(pptr.data(), k_dec, xs);
-    }
-    for (size_t i = 0; i < k_dec; i++) {
-      MeHost& e = children[i];
-      if (fl[i]) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
-      else memset(e.c, 0, sizeof(e.c));
-      e.m_in = m_in;
-      e.t = t;
-      e.r = parent.r;
-      e.y.assign(t * DPAD, kzero());
-      e.y_scalars.assign(t, kzero());
-      for (size_t j = 0; j < t; j++) {
-        K2 sc = kzero();
-        for (int rho = 0; rho < D; rho++) {
-          K2 v = y_child[(i * t + j) * D + rho];
-          e.y[j * DPAD + rho] = v;
-          sc = kadd(sc, kmul(v, bpow[rho]));
-        }
-        e.y_scalars[j] = sc;
-      }
-      memcpy(e.fold_digest, parent.fold_digest, 32);
-    }
-    {
-      bool ok_y = true, ok_x = true, ok_c = true;
-      for (size_t j = 0; j < t && ok_y; j++)
-        for (int q = 0; q < DPAD; q++) {
-          K2 lhs = kzero();
-          uint64_t pw = 1;
-          for (size_t i = 0; i < k_dec; i++) {
-            lhs = kadd(lhs, kscale(children[i].y[j * DPAD + q], pw));
-            pw = fadd(pw, pw);
-          }
-          if (!keq(lhs, parent.y[j * DPAD + q])) {
-            ok_y = false;
-            break;
-          }
-        }
-      for (size_t q = 0; q < (size_t)D * m_in; q++) {
-        uint64_t lhs = 0, pw = 1;
-        for (size_t i = 0; i < k_dec; i++) {
-          lhs = fadd(lhs, fmul(children[i].X[q], pw));
-          pw = fadd(pw, pw);
-        }
-        if (lhs != parent.X[q]) ok_x = false;
-      }
-      for (int q = 0; q < KAPPA * D; q++) {
-        uint64_t lhs = children[0].c[q], pw = 2;
-        for (size_t i = 1; i < k_dec; i++) {
-          lhs = fadd(lhs, fmul(children[i].c[q], pw));
-          pw = fadd(pw, pw);
-        }
-        if (lhs != parent.c[q]) ok_c = false;
-      }
-      if (!(ok_y && ok_x && ok_c))
-        throw Error(7, "DEC public check failed at step " + std::to_string(idx) + " (y=" + std::to_string(ok_y) +
-                           ", X=" + std::to_string(ok_x) + ", c=" + std::to_string(ok_c) + ")");
+    LaneOut mainl, vall;
+    rlc_dec_lane(tr, "main", idx, ccs_out, Zs, planes_[nxt_planes], vjs, L, mainl);
+    phase_mark(8);
+    if (!vclaims.empty()) {
+      tr.append_msg_u64("fold/val_lane_start", idx);
+      rlc_dec_lane(tr, "val", idx, vclaims, vZ, valplanes_, vjs_val_, L, vall);
     }
     phase_mark(9);
     tr.append_msg_u64("fold/step_done", idx);
@@ -1432,35 +1604,84 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     pb.k(gamma);
     pb.k(running);
     pb.digest(fold_digest);
-    pb.u(kt);
-    for (auto x : rhos) pb.u(x);
-    pb.me(parent);
-    pb.u(k_dec);
-    for (auto& e : children) pb.me(e);
-    for (int i = 0; i < 6; i++) pb.u(0);
-    pb.u(1);
-    pb.k(T);
-    pb.u(1);
-    pb.u(d_sc);
-    pb.u(1);
-    pb.u(8);
-    for (const char* c = "ccs/time"; *c; c++) pb.u((uint8_t)*c);
-    pb.u(1);
-    pb.u(ell_n);
-    for (size_t rd = 0; rd < ell_n; rd++) {
-      pb.u(xs_n);
-      for (size_t i = 0; i < xs_n; i++) pb.k(time_coeffs[rd * xs_n + i]);
+    enc_lane(pb, mainl, k_dec);
+    // mem: MemSidecarProof { cpu_me_claims_val, shout_addr_pre, proofs }
+    pb.u(vclaims.size());
+    for (auto& e : vclaims) pb.me(e);
+    pb.u(sm.n_lut);
+    for (auto& x : sp.claimed) pb.k(x);
+    pb.u(sp.active_mask);
+    pb.u(sp.rounds.size());
+    for (size_t i = 0; i < sp.rounds.size(); i++) enc_rounds(pb, sp.rounds[i], sp.ell_addr, sp.degs[i] + 1);
+    pb.u(sp.r_addr.size());
+    for (auto& x : sp.r_addr) pb.k(x);
+    pb.u(sm.n_lut + sm.n_mem);
+    for (size_t i = 0; i < sm.n_lut; i++) {
+      pb.u(1);  // MemOrLutProof::Shout(ShoutProof::default())
+      pb.u(0);
+      pb.u(0);
+      pb.u(0);
     }
-    pb.u(0);
+    for (size_t i = 0; i < sm.n_mem; i++) {
+      const TwistPre& P = tp[i];
+      pb.u(0);  // MemOrLutProof::Twist
+      pb.u(2);
+      pb.k(kzero());
+      pb.k(kzero());
+      pb.u(2);
+      for (int c = 0; c < 2; c++) enc_rounds(pb, P.rounds[c], P.ell_addr, P.degs[c] + 1);
+      pb.u(P.r_addr.size());
+      for (auto& x : P.r_addr) pb.k(x);
+      const RaClaim* vc = &vp.claims[vp.per * i];
+      pb.u(1);
+      pb.k(vc[0].sum);
+      enc_rounds(pb, vc[0].polys, ell_n, vc[0].deg + 1);
+      pb.k(vc[1].sum);
+      enc_rounds(pb, vc[1].polys, ell_n, vc[1].deg + 1);
+      if (vp.per == 3) {
+        pb.u(1);
+        pb.k(vc[2].sum);
+        pb.u(1);
+        enc_rounds(pb, vc[2].polys, ell_n, vc[2].deg + 1);
+      } else {
+        pb.u(0);
+        pb.u(0);
+      }
+    }
+    // batched_time: claimed_sums, degree_bounds, labels, round_polys
+    pb.u(claims.size());
+    for (auto& c : claims) pb.k(c.sum);
+    pb.u(claims.size());
+    for (auto& c : claims) pb.u(c.deg);
+    pb.u(claims.size());
+    for (auto& c : claims) {
+      const size_t ln = strlen(c.label);
+      pb.u(ln);
+      for (size_t q = 0; q < ln; q++) pb.u((uint8_t)c.label[q]);
+    }
+    pb.u(claims.size());
+    for (auto& c : claims) enc_rounds(pb, c.polys, ell_n, c.deg + 1);
+    if (!vclaims.empty()) {
+      pb.u(1);
+      enc_lane(pb, vall, k_dec);
+      for (auto& e : vall.children) val_obl.push_back(e);
+    } else {
+      pb.u(0);
+    }
 
-    acc = std::move(children);
-    acc_wit = pptr;
+    acc = std::move(mainl.children);
+    acc_wit = mainl.planes;
     cur_planes = nxt_planes;
+    prev_tw.clear();
+    for (auto& P : tp) prev_tw.push_back(std::move(P.cols));
+    have_prev = sm.n_mem > 0;
     phase_mark(11);
   }
   phase_mark(-1);
   pb.u(acc.size());
   for (auto& e : acc) pb.me(e);
+  pb.u(val_obl.size());
+  for (auto& e : val_obl) pb.me(e);
   NS_CHECK(hipStreamSynchronize(st));
   last_seconds = std::chrono::duration<double>(clk::now() - t0).count();
   last_permutations = tr.permutations();

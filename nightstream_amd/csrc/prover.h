@@ -135,6 +135,17 @@ struct OracleWs {
   PinnedStage stage;  // the context's device-to-host staging
 };
 
+struct StepMemH;
+struct BusLayout;
+
+// One RLC -> DEC lane's output: the rotations, the parent, the k_rho children and their planes.
+struct LaneOut {
+  std::vector<uint64_t> rhos;  // k x D x D
+  MeHost parent;
+  std::vector<MeHost> children;
+  std::vector<const int8_t*> planes;
+};
+
 struct StepTiming {
   double transcript_s = 0, gpu_wait_s = 0;
 };
@@ -146,8 +157,10 @@ class Prover {
   // Fold `nsteps` steps given their full z vectors (host, length m each).  Returns the
   // canonical proof encoding (u64 little-endian words; same layout as the CPU restatement).
   // z_on_device: z_steps are device pointers (inputs already resident in HBM).
+  // mem (optional, one entry per step): the step's Twist/Shout instances over the shared CPU bus
+  // (Route A memory sidecar, neo-fold/src/memory_sidecar/memory.rs).
   std::vector<uint64_t> prove_shard(const uint64_t* const* z_steps, size_t nsteps, size_t m_in,
-                                    bool z_on_device = false);
+                                    bool z_on_device = false, const StepMemH* mem = nullptr);
   double last_seconds = 0;
   // Per-kernel-class timing of the last prove (HIP events on the prover stream).
   bool timing = false;
@@ -179,6 +192,18 @@ class Prover {
   OracleWs ows_;
   DevBuf planes_[2][16], zmix_, rho_dev_, zptr_, pptr_dev_, flags_, ydev_, ptrs_, cout_, zbuf_, vjs_, head_;
   std::vector<DevBuf> zmcs_;
+  // Route A: val-lane child planes, M_j^T chi at r_val, bus weights / openings / decoded bus region
+  DevBuf valplanes_[16], vjs_val_, rbits_, rchi_, busw_, busy_, busys_, busz_, yptr_;
+  void bus_weights(const std::vector<K2>& r, const BusLayout& L);
+  void open_bus_planes(const std::vector<const int8_t*>& planes, const BusLayout& L, const std::vector<MeHost*>& outs,
+                       size_t t_core);
+  void open_bus_i32(const int32_t* zmix, const BusLayout& L, MeHost& out, size_t t_core);
+  void vjs_at(const std::vector<K2>& r, DevBuf& vjs);
+  std::vector<K2> y_products(const std::vector<const int8_t*>& planes, const DevBuf& vjs);
+  void head_cols(const int8_t* const* planes, size_t np, size_t m_in, std::vector<std::vector<uint64_t>*> Xs);
+  void rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const std::vector<MeHost>& ins,
+                    const std::vector<const int8_t*>& Zs, DevBuf* bank, const DevBuf& vjs, const BusLayout& L,
+                    LaneOut& out);
 };
 
 // Ajtai-bit rounds of Pi_CCS on the host (optimized_engine/oracle.rs:1812-1951), from the

@@ -11,11 +11,18 @@ stream, field-for-field in the order of the Rust structs, so proofs compare bit-
         sumcheck_challenges: count, K*; challenges_public: alpha, beta_a, beta_r (count, K*), gamma K;
         sumcheck_final: K; header_digest: 4 words (32 bytes)
       FoldStep.rlc_rhos: count, 54*54 F_q each; rlc_parent: MeInstance; dec_children: count, MeInstance*
-      mem (MemSidecarProof, empty for CCS-only steps): 6 zero words
+      mem (MemSidecarProof, shard_proof_types.rs:120-125):
+        cpu_me_claims_val: count, MeInstance*
+        shout_addr_pre: claimed_sums (count, K*), active_mask, round_polys (count, rounds*), r_addr (count, K*)
+        proofs: count, then per instance a tag (0 Twist, 1 Shout) and
+          BatchedAddrProof: claimed_sums (count, K*), round_polys (count, rounds*), r_addr (count, K*);
+          Twist only, val_eval: 0 (None) | 1, claimed_inc_sum_lt K, rounds_lt, claimed_inc_sum_total K,
+            rounds_total, claimed_prev_inc_sum_total (0 | 1 K), rounds_prev_total (0 | 1 rounds)
+        (rounds = count, (len, K*)*); a CCS-only step's mem is 6 zero words
       batched_time: claimed_sums (count, K*), degree_bounds (count, u64*), labels (count, (len, bytes)*),
-                    round_polys (count, (count, (len, K*)*)*)
-      val_fold: 0 (None)
-    final obligations (main lane): count, MeInstance*
+                    round_polys (count, rounds*)
+      val_fold: 0 (None) | 1, rlc_rhos, rlc_parent, dec_children (as in FoldStep)
+    ShardObligations: main (count, MeInstance*), val (count, MeInstance*)
   MeInstance (neo-ccs/src/relations.rs): c (kappa*D words, col-major), m_in, X (D*m_in),
     r (count, K*), y (t, (64, K*64)*), y_scalars (t, K*), fold_digest (4 words),
     c_step_coords.len() (0), u_offset (0), u_len (0).
@@ -43,6 +50,25 @@ class MeInstance:
 
 
 @dataclass
+class TwistAddrVal:
+    claimed_sums: list
+    round_polys: list        # [claim][round] -> coefficients
+    r_addr: list
+    val_eval: dict | None    # lt / total / prev_total: (claimed_sum, rounds)
+
+
+@dataclass
+class MemSidecar:
+    cpu_me_claims_val: list
+    shout_claimed_sums: list
+    shout_active_mask: int
+    shout_round_polys: list
+    shout_r_addr: list
+    twist: list              # TwistAddrVal per Twist instance
+    n_shout_proofs: int
+
+
+@dataclass
 class StepProof:
     ccs_out: list
     sumcheck_rounds: list
@@ -61,12 +87,15 @@ class StepProof:
     batched_degree_bounds: list = field(default_factory=list)
     batched_labels: list = field(default_factory=list)
     batched_round_polys: list = field(default_factory=list)
+    mem: MemSidecar | None = None
+    val_fold: tuple | None = None  # (rhos, parent, children)
 
 
 @dataclass
 class ShardProof:
     steps: list
     final_main: list
+    final_val: list = field(default_factory=list)
 
 
 class _Reader:
@@ -94,6 +123,18 @@ class _Reader:
 
     def digest(self) -> bytes:
         return self.words(4).astype("<u8").tobytes()
+
+    def rounds(self) -> list:
+        out = []
+        for _ in range(self.u()):
+            n = self.u()
+            out.append([self.k() for _ in range(n)])
+        return out
+
+    def addr_proof(self):
+        sums = self.ks()
+        polys = [self.rounds() for _ in range(self.u())]
+        return sums, polys, self.ks()
 
     def me(self) -> MeInstance:
         c = self.words(KAPPA * D)
@@ -132,26 +173,46 @@ def parse(data: bytes) -> ShardProof:
         rhos = [R.words(D * D).reshape(D, D) for _ in range(R.u())]
         parent = R.me()
         children = [R.me() for _ in range(R.u())]
-        if any(R.u() for _ in range(6)):
-            raise ValueError("non-empty memory sidecar in a CCS-only proof")
+        vclaims = [R.me() for _ in range(R.u())]
+        sh_sums = R.ks()
+        sh_mask = R.u()
+        sh_polys = [R.rounds() for _ in range(R.u())]
+        sh_r = R.ks()
+        twists, n_shout = [], 0
+        for _ in range(R.u()):
+            tag = R.u()
+            asums, apolys, ar = R.addr_proof()
+            if tag == 1:
+                n_shout += 1
+                continue
+            if tag != 0:
+                raise ValueError("unknown MemOrLutProof variant")
+            ve = None
+            if R.u():
+                ve = {"lt": (R.k(), R.rounds()), "total": (R.k(), R.rounds())}
+                prev_sum = R.k() if R.u() else None
+                prev_rounds = R.rounds() if R.u() else None
+                if (prev_sum is None) != (prev_rounds is None):
+                    raise ValueError("rollover claim without its rounds")
+                ve["prev_total"] = None if prev_sum is None else (prev_sum, prev_rounds)
+            twists.append(TwistAddrVal(asums, apolys, ar, ve))
+        mem = MemSidecar(vclaims, sh_sums, sh_mask, sh_polys, sh_r, twists, n_shout)
         sums = R.ks()
         degs = [R.u() for _ in range(R.u())]
         labels = []
         for _ in range(R.u()):
             n = R.u()
             labels.append(bytes(R.u() for _ in range(n)))
-        polys = []
-        for _ in range(R.u()):
-            per = []
-            for _ in range(R.u()):
-                n = R.u()
-                per.append([R.k() for _ in range(n)])
-            polys.append(per)
-        if R.u() != 0:
-            raise ValueError("val_fold present in a CCS-only proof")
+        polys = [R.rounds() for _ in range(R.u())]
+        val_fold = None
+        if R.u():
+            vrhos = [R.words(D * D).reshape(D, D) for _ in range(R.u())]
+            vparent = R.me()
+            val_fold = (vrhos, vparent, [R.me() for _ in range(R.u())])
         steps.append(StepProof(ccs_out, rounds, init, chals, alpha, beta_a, beta_r, gamma, final, hd, rhos, parent,
-                               children, sums, degs, labels, polys))
+                               children, sums, degs, labels, polys, mem, val_fold))
     final_main = [R.me() for _ in range(R.u())]
+    final_val = [R.me() for _ in range(R.u())]
     if R.i != len(R.w):
         raise ValueError("trailing words after the proof")
-    return ShardProof(steps, final_main)
+    return ShardProof(steps, final_main, final_val)

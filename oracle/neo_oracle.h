@@ -172,6 +172,17 @@ void or_rlc_zmix(const fq* const* rhos, const int8_t* const* Zs, size_t k, size_
 int  or_split_b(const int32_t* zmix, size_t len, size_t k, int8_t* const* outs, int* nonzero);
 
 /* ---------------- shard prover (neo-fold/src/shard.rs:1725-2283) ---------------- */
+/* Memory instances of one step (neo-memory/src/witness.rs:23-96): LUT tables are materialized
+ * (no table_spec); a MEM init is MemInit::Zero (sparse = 0) or MemInit::Sparse pairs. */
+typedef struct { size_t k, d, n_side, steps, ell; const fq* table; } or_lut_inst;
+typedef struct {
+  size_t k, d, n_side, steps, ell;
+  int sparse;
+  size_t ninit;
+  const uint64_t* init_addrs;
+  const fq* init_vals;
+} or_mem_inst;
+typedef struct { size_t n_lut; const or_lut_inst* luts; size_t n_mem; const or_mem_inst* mems; } or_step_mem;
 typedef struct or_prover or_prover;
 or_prover* or_prover_new(const or_ccs* s, const uint8_t pp_seed[32], int n_threads);
 void or_prover_free(or_prover* p);
@@ -179,6 +190,10 @@ void or_prover_free(or_prover* p);
  * Writes the canonical proof serialization into *out (malloc'd) and its length. */
 int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t m_in,
                    uint8_t** out, size_t* out_len, double* seconds_per_step);
+/* The same with each step's Twist/Shout instances over the shared CPU bus (Route A); mem may
+ * be NULL (no instances in any step). */
+int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem* mem, size_t nsteps, size_t m_in,
+                       uint8_t** out, size_t* out_len, double* seconds_per_step);
 void or_free(void* ptr);
 /* Wall time of fold step i of the last or_prove_shard call (the MCS commits of all steps are
  * done before step 0 and counted in neither). */

@@ -86,6 +86,10 @@ def lib():
                                      C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t),
                                      C.POINTER(C.c_double)]
         L.or_free.argtypes = [C.c_void_p]
+        L.or_prove_shard_mem.restype = C.c_int
+        L.or_prove_shard_mem.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                         C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t),
+                                         C.POINTER(C.c_double)]
         L.or_last_error.restype = C.c_char_p
         L.or_last_step_seconds.restype = C.c_double
         L.or_last_step_seconds.argtypes = [C.c_size_t]
@@ -258,6 +262,46 @@ def split_b(zmix: np.ndarray, k: int):
     return outs, [bool(x) for x in nz]
 
 
+class LutInstC(C.Structure):
+    _fields_ = [("k", C.c_size_t), ("d", C.c_size_t), ("n_side", C.c_size_t), ("steps", C.c_size_t),
+                ("ell", C.c_size_t), ("table", C.c_void_p)]
+
+
+class MemInstC(C.Structure):
+    _fields_ = [("k", C.c_size_t), ("d", C.c_size_t), ("n_side", C.c_size_t), ("steps", C.c_size_t),
+                ("ell", C.c_size_t), ("sparse", C.c_int), ("ninit", C.c_size_t), ("init_addrs", C.c_void_p),
+                ("init_vals", C.c_void_p)]
+
+
+class StepMemC(C.Structure):
+    _fields_ = [("n_lut", C.c_size_t), ("luts", C.c_void_p), ("n_mem", C.c_size_t), ("mems", C.c_void_p)]
+
+
+def step_mem_structs(mems, keep: list):
+    """ctypes mirrors of per-step memory instances (objects with .luts / .mems as in
+    nightstream_amd.bus_workload.StepMem); arrays they point into are appended to `keep`."""
+    arr = (StepMemC * len(mems))()
+    for s, sm in enumerate(mems):
+        luts = (LutInstC * max(1, len(sm.luts)))()
+        for i, lt in enumerate(sm.luts):
+            tab = np.ascontiguousarray(lt.table, np.uint64)
+            keep.append(tab)
+            luts[i] = LutInstC(lt.k, lt.d, lt.n_side, lt.steps, lt.ell, tab.ctypes.data)
+        ms = (MemInstC * max(1, len(sm.mems)))()
+        for i, mi in enumerate(sm.mems):
+            if mi.init is None:
+                ms[i] = MemInstC(mi.k, mi.d, mi.n_side, mi.steps, mi.ell, 0, 0, None, None)
+            else:
+                a = np.array([x for x, _ in mi.init] or [0], np.uint64)
+                v = np.array([y for _, y in mi.init] or [0], np.uint64)
+                keep += [a, v]
+                ms[i] = MemInstC(mi.k, mi.d, mi.n_side, mi.steps, mi.ell, 1, len(mi.init), a.ctypes.data, v.ctypes.data)
+        keep += [luts, ms]
+        arr[s] = StepMemC(len(sm.luts), C.cast(luts, C.c_void_p), len(sm.mems), C.cast(ms, C.c_void_p))
+    keep.append(arr)
+    return arr
+
+
 class Prover:
     """CPU shard prover (fold loop restatement); proves with the canonical proof encoding."""
 
@@ -267,12 +311,16 @@ class Prover:
         if not self.ptr:
             raise RuntimeError(lib().or_last_error().decode())
 
-    def prove(self, zs, m_in=1):
+    def prove(self, zs, m_in=1, mem=None):
+        """mem: per-step memory instances (Route A over the shared CPU bus), or None."""
         zs = [np.ascontiguousarray(z, np.uint64) for z in zs]
         out = C.POINTER(C.c_uint8)()
         n = C.c_size_t()
         sec = C.c_double()
-        rc = lib().or_prove_shard(self.ptr, _ptrs(zs), len(zs), m_in, C.byref(out), C.byref(n), C.byref(sec))
+        keep = []
+        marr = step_mem_structs(mem, keep) if mem is not None else None
+        rc = lib().or_prove_shard_mem(self.ptr, _ptrs(zs), None if marr is None else C.cast(marr, C.c_void_p),
+                                      len(zs), m_in, C.byref(out), C.byref(n), C.byref(sec))
         if rc != 0:
             raise RuntimeError(f"oracle prove failed ({rc}): {lib().or_last_error().decode()}")
         data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)

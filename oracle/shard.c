@@ -33,6 +33,7 @@
 #include <string.h>
 #include <time.h>
 #include "neo_oracle.h"
+#include "route_a.h"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -403,8 +404,233 @@ static int dims(const or_prover* p, size_t* ell_d, size_t* ell_n, size_t* ell, s
   return 0;
 }
 
+/* ---------------- one RLC -> DEC lane (shard.rs:1125-1331) ---------------- */
+typedef struct {
+  size_t k;
+  fq** rhos;        /* k rotation matrices, D x D */
+  me_t parent;
+  me_t* children;   /* k_dec */
+  int8_t** zsplit;  /* k_dec digit planes */
+} lane_out;
+
+static void lane_free(lane_out* o, size_t k_dec, int free_planes) {
+  for (size_t i = 0; i < o->k; i++) free(o->rhos[i]);
+  free(o->rhos);
+  me_free(&o->parent);
+  if (o->children)
+    for (size_t i = 0; i < k_dec; i++) me_free(&o->children[i]);
+  free(o->children);
+  if (free_planes && o->zsplit)
+    for (size_t i = 0; i < k_dec; i++) free(o->zsplit[i]);
+  if (free_planes) free(o->zsplit);
+  memset(o, 0, sizeof(*o));
+}
+
+static void bind_rlc_inputs(or_tr* tr, const char* lane, size_t idx, const me_t* ins, size_t k) {
+  or_tr_append_message(tr, "fold/rlc_inputs/v1", (const uint8_t*)lane, strlen(lane));
+  uint64_t v = idx;
+  or_tr_append_u64s(tr, "step_idx", &v, 1);
+  v = k;
+  or_tr_append_u64s(tr, "me_count", &v, 1);
+  for (size_t i = 0; i < k; i++) {
+    const me_t* e = &ins[i];
+    or_tr_append_fields(tr, "c_data", e->c, KAPPA * D);
+    v = e->m_in;
+    or_tr_append_u64s(tr, "m_in", &v, 1);
+    or_tr_append_message(tr, "me_fold_digest", e->fold_digest, 32);
+    for (size_t q = 0; q < e->ell_n; q++) tr_k(tr, "r_limb", e->r[q]);
+    or_tr_append_fields(tr, "X", e->X, D * e->m_in);
+    for (size_t q = 0; q < e->t * DPAD; q++) tr_k(tr, "y_elem", e->y[q]);
+    for (size_t q = 0; q < e->t; q++) tr_k(tr, "y_scalar", e->y_scalars[q]);
+    v = 0;
+    or_tr_append_u64s(tr, "c_step_coords_len", &v, 1);
+    or_tr_append_fields(tr, "c_step_coords", NULL, 0);
+    or_tr_append_u64s(tr, "u_offset", &v, 1);
+    or_tr_append_u64s(tr, "u_len", &v, 1);
+  }
+}
+
+/* bind_rlc_inputs + sample_rot_rhos_n + rlc_reduction_optimized (common.rs:1221-1379) + DEC
+ * split, child commitments and child y (common.rs:1445-1628), with the shared-bus openings of
+ * the parent (from Z_mix) and of every child (from its digit plane) appended after the core rows. */
+static int rlc_dec_lane(or_prover* p, or_tr* tr, const char* lane, size_t idx, const me_t* ins, size_t k,
+                        const int8_t* const* Zs, const ra_layout* L, size_t ell_n, lane_out* o) {
+  const or_ccs* s = p->s;
+  const size_t m = s->m, t_core = s->t, bus = L->bus_cols, t_tot = t_core + bus, k_dec = p->params.k_rho;
+  const size_t m_in = ins[0].m_in;
+  memset(o, 0, sizeof(*o));
+  bind_rlc_inputs(tr, lane, idx, ins, k);
+  if ((uint64_t)k * 216u * (p->params.b - 1) >= ((uint64_t)1 << p->params.k_rho)) {
+    snprintf(g_err, sizeof(g_err), "RLC norm bound violated for count=%zu", k);
+    return -4;
+  }
+  o->k = k;
+  o->rhos = (fq**)calloc(k, sizeof(fq*));
+  int8_t(*rho_a)[D] = malloc(k * sizeof(*rho_a));
+  for (size_t i = 0; i < k; i++) {
+    tr_msg_u64(tr, "rlc/rot/index", i);
+    draw_alphabet(tr, D, rho_a[i], i);
+    o->rhos[i] = (fq*)calloc(D * D, sizeof(fq));
+    if (k == 1) {
+      for (int q = 0; q < D; q++) o->rhos[i][q * D + q] = 1;
+    } else {
+      or_rot_from_coeffs(rho_a[i], o->rhos[i]);
+    }
+  }
+  free(rho_a);
+  fq bpow[D];
+  bpow[0] = 1;
+  for (int i = 1; i < D; i++) bpow[i] = or_fq_add(bpow[i - 1], bpow[i - 1]);
+  me_t* parent = &o->parent;
+  me_init(parent, m_in, ell_n, k == 1 ? ins[0].t : t_tot);
+  memcpy(parent->r, ins[0].r, ell_n * sizeof(kx));
+  memcpy(parent->fold_digest, ins[0].fold_digest, 32);
+  int32_t* zmix = (int32_t*)malloc(D * m * sizeof(int32_t));
+  if (k == 1) {
+    memcpy(parent->c, ins[0].c, sizeof(parent->c));
+    memcpy(parent->X, ins[0].X, D * m_in * sizeof(fq));
+    memcpy(parent->y, ins[0].y, ins[0].t * DPAD * sizeof(kx));
+    for (size_t i = 0; i < D * m; i++) zmix[i] = Zs[0][i];
+    for (size_t j = 0; j < parent->t; j++) {
+      kx sc = K0;
+      for (size_t rho = 0; rho < D; rho++) sc = or_k_add(sc, or_k_scale(parent->y[j * DPAD + rho], bpow[rho]));
+      parent->y_scalars[j] = sc;
+    }
+  } else {
+    for (size_t i = 0; i < k; i++)
+      for (size_t rr = 0; rr < D; rr++)
+        for (size_t kk = 0; kk < D; kk++) {
+          fq co = o->rhos[i][rr * D + kk];
+          if (!co) continue;
+          for (size_t c = 0; c < m_in; c++)
+            parent->X[rr * m_in + c] = or_fq_add(parent->X[rr * m_in + c], or_fq_mul(co, ins[i].X[kk * m_in + c]));
+        }
+    for (size_t j = 0; j < t_core; j++)
+      for (size_t i = 0; i < k; i++)
+        for (size_t rr = 0; rr < D; rr++) {
+          kx a = K0;
+          for (size_t kk = 0; kk < D; kk++) a = or_k_add(a, or_k_scale(ins[i].y[j * DPAD + kk], o->rhos[i][rr * D + kk]));
+          parent->y[j * DPAD + rr] = or_k_add(parent->y[j * DPAD + rr], a);
+        }
+    or_rlc_zmix((const fq* const*)o->rhos, Zs, k, m, zmix);
+    fq term[KAPPA * D], acoef[D];
+    memset(parent->c, 0, sizeof(parent->c));
+    for (size_t i = 0; i < k; i++) {
+      for (int q = 0; q < D; q++) acoef[q] = o->rhos[i][q * D + 0];
+      or_s_mul(acoef, ins[i].c, KAPPA, term);
+      for (int q = 0; q < KAPPA * D; q++) parent->c[q] = or_fq_add(parent->c[q], term[q]);
+    }
+    for (size_t j = 0; j < t_core; j++) {
+      kx sc = K0;
+      for (size_t rho = 0; rho < D; rho++) sc = or_k_add(sc, or_k_scale(parent->y[j * DPAD + rho], bpow[rho]));
+      parent->y_scalars[j] = sc;
+    }
+    if (bus && ra_bus_openings_i32(zmix, L, parent->r, ell_n, parent->y + t_core * DPAD, parent->y_scalars + t_core)) {
+      free(zmix);
+      snprintf(g_err, sizeof(g_err), "bus openings of Z_mix failed");
+      return -8;
+    }
+  }
+  /* Pi_DEC */
+  o->zsplit = (int8_t**)calloc(k_dec, sizeof(int8_t*));
+  int nonzero[64];
+  for (size_t i = 0; i < k_dec; i++) o->zsplit[i] = (int8_t*)malloc(D * m);
+  int bad = or_split_b(zmix, D * m, k_dec, o->zsplit, nonzero);
+  free(zmix);
+  if (bad) {
+    snprintf(g_err, sizeof(g_err), "step %zu: DEC split out of range (%s lane)", idx, lane);
+    return -5;
+  }
+  o->children = (me_t*)calloc(k_dec, sizeof(me_t));
+  kx* y_child = (kx*)malloc(k_dec * t_core * D * sizeof(kx));
+  or_precompute_y(s, (const int8_t* const*)o->zsplit, k_dec, parent->r, ell_n, y_child);
+  for (size_t i = 0; i < k_dec; i++) {
+    me_t* ch = &o->children[i];
+    me_init(ch, m_in, ell_n, t_tot);
+    if (nonzero[i]) or_ajtai_commit(p->pp, KAPPA, m, o->zsplit[i], ch->c);
+    for (size_t r = 0; r < D; r++)
+      for (size_t c = 0; c < m_in; c++) ch->X[r * m_in + c] = or_fq_from_i64(o->zsplit[i][r * m + c]);
+    memcpy(ch->r, parent->r, ell_n * sizeof(kx));
+    for (size_t j = 0; j < t_core; j++) {
+      kx sc = K0;
+      for (size_t rho = 0; rho < D; rho++) {
+        kx v = y_child[(i * t_core + j) * D + rho];
+        ch->y[j * DPAD + rho] = v;
+        sc = or_k_add(sc, or_k_scale(v, bpow[rho]));
+      }
+      ch->y_scalars[j] = sc;
+    }
+    if (bus && or_bus_openings(o->zsplit[i], m, m_in, L->bus_base, L->chunk, bus, parent->r, ell_n,
+                               ch->y + t_core * DPAD, ch->y_scalars + t_core)) {
+      free(y_child);
+      snprintf(g_err, sizeof(g_err), "bus openings of a DEC child failed");
+      return -8;
+    }
+    memcpy(ch->fold_digest, parent->fold_digest, 32);
+  }
+  free(y_child);
+  int ok_y = 1, ok_x = 1, ok_c = 1;
+  for (size_t j = 0; j < t_core && ok_y; j++)
+    for (size_t q = 0; q < DPAD; q++) {
+      kx lhs = K0;
+      fq pw = 1;
+      for (size_t i = 0; i < k_dec; i++) {
+        lhs = or_k_add(lhs, or_k_scale(o->children[i].y[j * DPAD + q], pw));
+        pw = or_fq_add(pw, pw);
+      }
+      if (!or_k_eq(lhs, parent->y[j * DPAD + q])) {
+        ok_y = 0;
+        break;
+      }
+    }
+  for (size_t q = 0; q < D * m_in; q++) {
+    fq lhs = 0, pw = 1;
+    for (size_t i = 0; i < k_dec; i++) {
+      lhs = or_fq_add(lhs, or_fq_mul(o->children[i].X[q], pw));
+      pw = or_fq_add(pw, pw);
+    }
+    if (lhs != parent->X[q]) ok_x = 0;
+  }
+  for (size_t q = 0; q < KAPPA * D; q++) {
+    fq lhs = o->children[0].c[q], pw = 2;
+    for (size_t i = 1; i < k_dec; i++) {
+      lhs = or_fq_add(lhs, or_fq_mul(o->children[i].c[q], pw));
+      pw = or_fq_add(pw, pw);
+    }
+    if (lhs != parent->c[q]) ok_c = 0;
+  }
+  if (!(ok_y && ok_x && ok_c)) {
+    snprintf(g_err, sizeof(g_err), "%s public check failed at step %zu (y=%d, X=%d, c=%d)",
+             strcmp(lane, "val") ? "DEC" : "DEC(val)", idx, ok_y, ok_x, ok_c);
+    return -6;
+  }
+  return 0;
+}
+
+static void ser_rounds(u64buf* b, const kx* polys, size_t rounds, size_t np) {
+  ub_push(b, rounds);
+  for (size_t rd = 0; rd < rounds; rd++) {
+    ub_push(b, np);
+    for (size_t i = 0; i < np; i++) ub_k(b, polys[rd * np + i]);
+  }
+}
+
+static void ser_lane(u64buf* b, const lane_out* o, size_t k_dec) {
+  ub_push(b, o->k);
+  for (size_t i = 0; i < o->k; i++)
+    for (int q = 0; q < D * D; q++) ub_push(b, o->rhos[i][q]);
+  ser_me(b, &o->parent);
+  ub_push(b, k_dec);
+  for (size_t i = 0; i < k_dec; i++) ser_me(b, &o->children[i]);
+}
+
 int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t m_in, uint8_t** out,
                    size_t* out_len, double* seconds_per_step) {
+  return or_prove_shard_mem(p, z_steps, NULL, nsteps, m_in, out, out_len, seconds_per_step);
+}
+
+int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem* mem, size_t nsteps, size_t m_in,
+                       uint8_t** out, size_t* out_len, double* seconds_per_step) {
   const or_ccs* s = p->s;
   size_t m = s->m, t = s->t, k_dec = p->params.k_rho;
   size_t ell_d, ell_n, ell, d_sc;
@@ -415,6 +641,17 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
     snprintf(g_err, sizeof(g_err), "extension policy failed");
     return -1;
   }
+  static const or_step_mem no_mem = {0, NULL, 0, NULL};
+  ra_layout L;
+  if (mem) {
+    if (ra_infer_layout(mem, nsteps, s->n, m, m_in, &L, g_err, sizeof(g_err))) return -1;
+  } else {
+    memset(&L, 0, sizeof(L));
+    L.m = m;
+    L.m_in = m_in;
+    L.chunk = 1;
+  }
+  const size_t t_core = t, t_tot = t + L.bus_cols;
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
 
@@ -434,9 +671,11 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
   or_tr tr;
   or_tr_new(&tr, (const uint8_t*)"neo.fold/session", 16);
 
-  size_t nacc = 0;
+  size_t nacc = 0, nval = 0, prev_nmem = 0;
   me_t* acc = NULL;
+  me_t* val_obl = NULL;
   int8_t** acc_wit = NULL;
+  ra_twist_cols* prev_tw = NULL;
   const size_t xs_n = d_sc + 1;
   kx xs[16];
   for (size_t i = 0; i < xs_n; i++) xs[i] = or_k_from(i);
@@ -446,11 +685,8 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
   struct timespec ts_step;
   clock_gettime(CLOCK_MONOTONIC, &ts_step);
   for (size_t idx = 0; idx < nsteps; idx++) {
-    /* absorb_step_memory with no LUT / MEM instances */
-    tr_msg_empty(&tr, "step/absorb_memory_start");
-    tr_msg_u64(&tr, "step/lut_count", 0);
-    tr_msg_u64(&tr, "step/mem_count", 0);
-    tr_msg_empty(&tr, "step/absorb_memory_done");
+    const or_step_mem* sm = mem ? &mem[idx] : &no_mem;
+    ra_absorb_step_memory(&tr, sm);
 
     size_t k = nacc + 1;
     /* bind_header_and_instances_with_digest */
@@ -574,57 +810,62 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
       }
     }
     tr_k(&tr, "sumcheck/initial_sum", T);
-    /* r_cycle (sampled, unused by CCS-only steps) */
+    /* Route A memory checks use a separate transcript-derived cycle point r_cycle */
+    kx r_cycle[64];
     for (size_t i = 0; i < ell_n; i++) {
       tr_msg_u64(&tr, "route_a/r_cycle", i);
-      (void)or_tr_challenge_field(&tr, "route_a/cycle/0");
-      (void)or_tr_challenge_field(&tr, "route_a/cycle/1");
+      r_cycle[i].c0 = or_tr_challenge_field(&tr, "route_a/cycle/0");
+      r_cycle[i].c1 = or_tr_challenge_field(&tr, "route_a/cycle/1");
     }
-    /* oracle */
+    /* CCS oracle */
     size_t kt = k;
     const int8_t** Zs = (const int8_t**)malloc(kt * sizeof(int8_t*));
     Zs[0] = Zmcs[idx];
     for (size_t a = 0; a < nacc; a++) Zs[1 + a] = acc_wit[a];
     or_rowstate* rs = or_rowstate_build(s, Zs, kt, alpha, beta_a, beta_r, gamma, ell_d, ell_n, nacc ? acc[0].r : NULL);
 
-    /* batched time sum-check (one CCS claim) */
-    tr_msg_u64(&tr, "batched/dynamic_bind/len", 1);
-    or_tr_append_message(&tr, "batched/dynamic_bind/claim_label", (const uint8_t*)"ccs/time", 8);
-    tr_msg_u64(&tr, "batched/dynamic_bind/claim_idx", 0);
-    tr_msg_u64(&tr, "batched/dynamic_bind/degree_bound", d_sc);
-    {
-      uint8_t one = 1;
-      or_tr_append_message(&tr, "batched/dynamic_bind/is_dynamic", &one, 1);
+    /* addr-pre sum-checks (memory.rs:232-621): Shout first, then Twist */
+    fq* zdec = NULL;
+    if (sm->n_lut || sm->n_mem) {
+      zdec = (fq*)malloc(m * sizeof(fq));
+      or_bus_decode(Zmcs[idx], m, zdec);
     }
-    tr_k(&tr, "batched/dynamic_bind/claimed_sum", T);
-    or_tr_append_message(&tr, "sc/domain", (const uint8_t*)"shard/batched_time", 18);
-    tr_msg_u64(&tr, "sc/inst_idx", idx);
-    kx running = T;
-    kx* time_coeffs = (kx*)malloc(ell_n * xs_n * sizeof(kx));
+    ra_shout_pre sp;
+    ra_twist_pre* tp = (ra_twist_pre*)calloc(sm->n_mem + 1, sizeof(ra_twist_pre));
+    if (ra_shout_addr_pre(&tr, sm, &L, zdec, ell_n, r_cycle, idx, &sp, g_err, sizeof(g_err)) ||
+        ra_twist_addr_pre(&tr, sm, &L, zdec, ell_n, r_cycle, tp, g_err, sizeof(g_err))) {
+      rc = -7;
+      break;
+    }
+    free(zdec);
+    /* the batched time sum-check: CCS claim, then Shout and Twist time lanes (route_a_time.rs:28-152) */
+    const size_t nmem_cl = 3 * (sm->n_lut + sm->n_mem);
+    ra_claim* cl = (ra_claim*)calloc(1 + nmem_cl, sizeof(ra_claim));
+    cl[0].kind = RA_ROWSTATE;
+    cl[0].o = rs;
+    cl[0].deg = d_sc;
+    cl[0].label = "ccs/time";
+    cl[0].dynamic = 1;
+    size_t ncl_mem = 0;
+    if (ra_build_time_claims(sm, ell_n, r_cycle, &sp, tp, cl + 1, &ncl_mem, g_err, sizeof(g_err))) {
+      rc = -7;
+      break;
+    }
+    const size_t ncl = 1 + ncl_mem;
+    cl[0].claimed_sum = T;
+    ra_bind_dynamic_claims(&tr, cl, ncl);
     kx* r_time = (kx*)malloc(ell_n * sizeof(kx));
-    for (size_t rd = 0; rd < ell_n; rd++) {
-      tr_msg_u64(&tr, "batched/round_idx", rd);
-      kx ys[16];
-      or_rowstate_evals(rs, xs, xs_n, ys);
-      if (!or_k_eq(or_k_add(ys[0], ys[1]), running)) {
-        snprintf(g_err, sizeof(g_err), "step %zu: batched round %zu invariant failed", idx, rd);
-        rc = -2;
-        break;
-      }
-      kx* co = time_coeffs + rd * xs_n;
-      or_interpolate(xs, ys, xs_n, co);
-      or_tr_append_message(&tr, "batched/claim_label", (const uint8_t*)"ccs/time", 8);
-      tr_msg_u64(&tr, "batched/claim_idx", 0);
-      for (size_t i = 0; i < xs_n; i++) tr_k(&tr, "batched/round/coeff", co[i]);
-      kx r = tr_chal_k2(&tr, "batched/challenge/0", "batched/challenge/1");
-      r_time[rd] = r;
-      running = or_poly_eval(co, xs_n, r);
-      or_rowstate_fold(rs, r);
+    if (ra_batched_sumcheck(&tr, "shard/batched_time", idx, cl, ncl, ell_n, r_time, g_err, sizeof(g_err))) {
+      rc = -2;
+      break;
+    }
+    kx running = cl[0].running;
+    const kx* time_coeffs = cl[0].polys;
+    for (size_t i = 1; i < ncl; i++) {
+      or_ts_free((or_ts*)cl[i].o);
+      cl[i].o = NULL;
     }
     or_rowstate_free(rs);
-    if (rc) break;
-    or_tr_append_message(&tr, "sc/domain_end", (const uint8_t*)"shard/batched_time", 18);
-    tr_msg_u64(&tr, "sc/inst_idx_end", idx);
 
     /* Ajtai rounds */
     kx* y_eval = (kx*)malloc(kt * t * D * sizeof(kx));
@@ -677,13 +918,13 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
     uint8_t fold_digest[32];
     or_tr_digest32(&tr, fold_digest);
 
-    /* ME outputs */
+    /* ME outputs, with the shared-bus openings appended (shard.rs:2082-2120) */
     me_t* ccs_out = (me_t*)calloc(kt, sizeof(me_t));
     fq bpow[D];
     bpow[0] = 1;
     for (int i = 1; i < D; i++) bpow[i] = or_fq_add(bpow[i - 1], bpow[i - 1]);
     for (size_t i = 0; i < kt; i++) {
-      me_init(&ccs_out[i], m_in, ell_n, t);
+      me_init(&ccs_out[i], m_in, ell_n, t_tot);
       if (i == 0) {
         memcpy(ccs_out[i].c, cmcs[idx], sizeof(ccs_out[i].c));
         for (size_t r = 0; r < D; r++)
@@ -693,171 +934,81 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
         memcpy(ccs_out[i].X, acc[i - 1].X, D * m_in * sizeof(fq));
       }
       memcpy(ccs_out[i].r, r_time, ell_n * sizeof(kx));
-      for (size_t j = 0; j < t; j++) {
+      for (size_t j = 0; j < t_core; j++) {
         kx sc = K0;
         for (size_t rho = 0; rho < D; rho++) {
-          kx v = y_eval[(i * t + j) * D + rho];
+          kx v = y_eval[(i * t_core + j) * D + rho];
           ccs_out[i].y[j * DPAD + rho] = v;
           sc = or_k_add(sc, or_k_scale(v, bpow[rho]));
         }
         ccs_out[i].y_scalars[j] = sc;
       }
-      memcpy(ccs_out[i].fold_digest, fold_digest, 32);
-    }
-
-
-    /* ---- Pi_RLC (main lane): bind_rlc_inputs (shard.rs:1125-1168) ---- */
-    or_tr_append_message(&tr, "fold/rlc_inputs/v1", (const uint8_t*)"main", 4);
-    {
-      uint64_t v = idx;
-      or_tr_append_u64s(&tr, "step_idx", &v, 1);
-      v = kt;
-      or_tr_append_u64s(&tr, "me_count", &v, 1);
-      for (size_t i = 0; i < kt; i++) {
-        const me_t* e = &ccs_out[i];
-        or_tr_append_fields(&tr, "c_data", e->c, KAPPA * D);
-        v = e->m_in;
-        or_tr_append_u64s(&tr, "m_in", &v, 1);
-        or_tr_append_message(&tr, "me_fold_digest", e->fold_digest, 32);
-        for (size_t q = 0; q < e->ell_n; q++) tr_k(&tr, "r_limb", e->r[q]);
-        or_tr_append_fields(&tr, "X", e->X, D * e->m_in);
-        for (size_t q = 0; q < e->t * DPAD; q++) tr_k(&tr, "y_elem", e->y[q]);
-        for (size_t q = 0; q < e->t; q++) tr_k(&tr, "y_scalar", e->y_scalars[q]);
-        v = 0;
-        or_tr_append_u64s(&tr, "c_step_coords_len", &v, 1);
-        or_tr_append_fields(&tr, "c_step_coords", NULL, 0);
-        or_tr_append_u64s(&tr, "u_offset", &v, 1);
-        or_tr_append_u64s(&tr, "u_len", &v, 1);
-      }
-    }
-    /* sample_rot_rhos_n (common.rs:467-573) */
-    if ((uint64_t)kt * 216u * (p->params.b - 1) >= ((uint64_t)1 << p->params.k_rho)) {
-      snprintf(g_err, sizeof(g_err), "RLC norm bound violated for count=%zu", kt);
-      rc = -4;
-      break;
-    }
-    fq** rhos = (fq**)malloc(kt * sizeof(fq*));
-    int8_t(*rho_a)[D] = malloc(kt * sizeof(*rho_a));
-    for (size_t i = 0; i < kt; i++) {
-      tr_msg_u64(&tr, "rlc/rot/index", i);
-      draw_alphabet(&tr, D, rho_a[i], i);
-      rhos[i] = (fq*)calloc(D * D, sizeof(fq));
-      if (kt == 1) {
-        for (int q = 0; q < D; q++) rhos[i][q * D + q] = 1;
-      } else {
-        or_rot_from_coeffs(rho_a[i], rhos[i]);
-      }
-    }
-    /* rlc_reduction_optimized / k = 1 fast path */
-    me_t parent;
-    me_init(&parent, m_in, ell_n, t);
-    memcpy(parent.r, ccs_out[0].r, ell_n * sizeof(kx));
-    memcpy(parent.fold_digest, ccs_out[0].fold_digest, 32);
-    int32_t* zmix = (int32_t*)malloc(D * m * sizeof(int32_t));
-    if (kt == 1) {
-      memcpy(parent.c, ccs_out[0].c, sizeof(parent.c));
-      memcpy(parent.X, ccs_out[0].X, D * m_in * sizeof(fq));
-      memcpy(parent.y, ccs_out[0].y, t * DPAD * sizeof(kx));
-      for (size_t i = 0; i < D * m; i++) zmix[i] = Zs[0][i];
-    } else {
-      for (size_t i = 0; i < kt; i++)
-        for (size_t rr = 0; rr < D; rr++)
-          for (size_t kk = 0; kk < D; kk++) {
-            fq co = rhos[i][rr * D + kk];
-            if (!co) continue;
-            for (size_t c = 0; c < m_in; c++)
-              parent.X[rr * m_in + c] = or_fq_add(parent.X[rr * m_in + c], or_fq_mul(co, ccs_out[i].X[kk * m_in + c]));
-          }
-      for (size_t j = 0; j < t; j++)
-        for (size_t i = 0; i < kt; i++)
-          for (size_t rr = 0; rr < D; rr++) {
-            kx a = K0;
-            for (size_t kk = 0; kk < D; kk++)
-              a = or_k_add(a, or_k_scale(ccs_out[i].y[j * DPAD + kk], rhos[i][rr * D + kk]));
-            parent.y[j * DPAD + rr] = or_k_add(parent.y[j * DPAD + rr], a);
-          }
-      or_rlc_zmix((const fq* const*)rhos, Zs, kt, m, zmix);
-      /* mix_rhos_commits: s_lincomb(rot_matrix_to_rq(rho_i), c_i) */
-      fq term[KAPPA * D], acoef[D];
-      memset(parent.c, 0, sizeof(parent.c));
-      for (size_t i = 0; i < kt; i++) {
-        for (int q = 0; q < D; q++) acoef[q] = rhos[i][q * D + 0];
-        or_s_mul(acoef, ccs_out[i].c, KAPPA, term);
-        for (int q = 0; q < KAPPA * D; q++) parent.c[q] = or_fq_add(parent.c[q], term[q]);
-      }
-    }
-    for (size_t j = 0; j < t; j++) {
-      kx sc = K0;
-      for (size_t rho = 0; rho < D; rho++) sc = or_k_add(sc, or_k_scale(parent.y[j * DPAD + rho], bpow[rho]));
-      parent.y_scalars[j] = sc;
-    }
-
-    /* ---- Pi_DEC ---- */
-    int8_t** zsplit = (int8_t**)malloc(k_dec * sizeof(int8_t*));
-    int nonzero[64];
-    for (size_t i = 0; i < k_dec; i++) zsplit[i] = (int8_t*)malloc(D * m);
-    if (or_split_b(zmix, D * m, k_dec, zsplit, nonzero) != 0) {
-      snprintf(g_err, sizeof(g_err), "step %zu: DEC split out of range", idx);
-      rc = -5;
-      break;
-    }
-    me_t* children = (me_t*)calloc(k_dec, sizeof(me_t));
-    kx* y_child = (kx*)malloc(k_dec * t * D * sizeof(kx));
-    or_precompute_y(s, (const int8_t* const*)zsplit, k_dec, parent.r, ell_n, y_child);
-    for (size_t i = 0; i < k_dec; i++) {
-      me_init(&children[i], m_in, ell_n, t);
-      if (nonzero[i]) or_ajtai_commit(p->pp, KAPPA, m, zsplit[i], children[i].c);
-      for (size_t r = 0; r < D; r++)
-        for (size_t c = 0; c < m_in; c++) children[i].X[r * m_in + c] = or_fq_from_i64(zsplit[i][r * m + c]);
-      memcpy(children[i].r, parent.r, ell_n * sizeof(kx));
-      for (size_t j = 0; j < t; j++) {
-        kx sc = K0;
-        for (size_t rho = 0; rho < D; rho++) {
-          kx v = y_child[(i * t + j) * D + rho];
-          children[i].y[j * DPAD + rho] = v;
-          sc = or_k_add(sc, or_k_scale(v, bpow[rho]));
-        }
-        children[i].y_scalars[j] = sc;
-      }
-      memcpy(children[i].fold_digest, parent.fold_digest, 32);
-    }
-    /* ok_y, ok_X, ok_c */
-    {
-      int ok_y = 1, ok_x = 1, ok_c = 1;
-      for (size_t j = 0; j < t && ok_y; j++)
-        for (size_t q = 0; q < DPAD; q++) {
-          kx lhs = K0;
-          fq pw = 1;
-          for (size_t i = 0; i < k_dec; i++) {
-            lhs = or_k_add(lhs, or_k_scale(children[i].y[j * DPAD + q], pw));
-            pw = or_fq_add(pw, pw);
-          }
-          if (!or_k_eq(lhs, parent.y[j * DPAD + q])) {
-            ok_y = 0;
-            break;
-          }
-        }
-      for (size_t q = 0; q < D * m_in; q++) {
-        fq lhs = 0, pw = 1;
-        for (size_t i = 0; i < k_dec; i++) {
-          lhs = or_fq_add(lhs, or_fq_mul(children[i].X[q], pw));
-          pw = or_fq_add(pw, pw);
-        }
-        if (lhs != parent.X[q]) ok_x = 0;
-      }
-      for (size_t q = 0; q < KAPPA * D; q++) {
-        fq lhs = children[0].c[q], pw = 2;
-        for (size_t i = 1; i < k_dec; i++) {
-          lhs = or_fq_add(lhs, or_fq_mul(children[i].c[q], pw));
-          pw = or_fq_add(pw, pw);
-        }
-        if (lhs != parent.c[q]) ok_c = 0;
-      }
-      if (!(ok_y && ok_x && ok_c)) {
-        snprintf(g_err, sizeof(g_err), "DEC public check failed at step %zu (y=%d, X=%d, c=%d)", idx, ok_y, ok_x, ok_c);
-        rc = -6;
+      if (L.bus_cols && or_bus_openings(Zs[i], m, m_in, L.bus_base, L.chunk, L.bus_cols, r_time, ell_n,
+                                        ccs_out[i].y + t_core * DPAD, ccs_out[i].y_scalars + t_core)) {
+        snprintf(g_err, sizeof(g_err), "bus openings failed");
+        rc = -8;
         break;
       }
+      memcpy(ccs_out[i].fold_digest, fold_digest, 32);
+    }
+    if (rc) break;
+
+    /* finalize_route_a_memory_prover: val-eval batch, then CPU ME claims at r_val */
+    ra_val_pre vp;
+    if (ra_val_eval(&tr, sm, tp, (idx > 0 && prev_nmem == sm->n_mem) ? prev_tw : NULL, ell_n, r_time, idx, &vp, g_err,
+                    sizeof(g_err))) {
+      rc = -7;
+      break;
+    }
+    size_t nvc = 0;
+    me_t vclaims[2];
+    const int8_t* vZ[2];
+    if (sm->n_mem) {
+      nvc = idx > 0 ? 2 : 1;
+      or_tr fork = tr;
+      or_tr_append_message(&fork, "fork", (const uint8_t*)"cpu_bus/me_digest_val", 21);
+      uint8_t vdig[32];
+      or_tr_digest32(&fork, vdig);
+      for (size_t q = 0; q < nvc; q++) {
+        const size_t st = idx - q;
+        me_t* e = &vclaims[q];
+        me_init(e, m_in, ell_n, t_tot);
+        memcpy(e->c, cmcs[st], sizeof(e->c));
+        for (size_t r = 0; r < D; r++)
+          for (size_t c = 0; c < m_in; c++) e->X[r * m_in + c] = or_fq_from_i64(Zmcs[st][r * m + c]);
+        memcpy(e->r, vp.r_val, ell_n * sizeof(kx));
+        kx yv[8 * D];
+        const int8_t* zz = Zmcs[st];
+        or_precompute_y(s, &zz, 1, vp.r_val, ell_n, yv);
+        for (size_t j = 0; j < t_core; j++) {
+          kx sc = K0;
+          for (size_t rho = 0; rho < D; rho++) {
+            e->y[j * DPAD + rho] = yv[j * D + rho];
+            sc = or_k_add(sc, or_k_scale(yv[j * D + rho], bpow[rho]));
+          }
+          e->y_scalars[j] = sc;
+        }
+        if (L.bus_cols && or_bus_openings(zz, m, m_in, L.bus_base, L.chunk, L.bus_cols, vp.r_val, ell_n,
+                                          e->y + t_core * DPAD, e->y_scalars + t_core)) {
+          snprintf(g_err, sizeof(g_err), "bus openings failed");
+          rc = -8;
+          break;
+        }
+        memcpy(e->fold_digest, vdig, 32);
+        vZ[q] = zz;
+      }
+      if (rc) break;
+    }
+
+    /* main lane, then the val lane (shard.rs:2171-2254) */
+    lane_out mainl, vall;
+    memset(&vall, 0, sizeof(vall));
+    rc = rlc_dec_lane(p, &tr, "main", idx, ccs_out, kt, Zs, &L, ell_n, &mainl);
+    if (rc) break;
+    if (nvc) {
+      tr_msg_u64(&tr, "fold/val_lane_start", idx);
+      rc = rlc_dec_lane(p, &tr, "val", idx, vclaims, nvc, vZ, &L, ell_n, &vall);
+      if (rc) break;
     }
     tr_msg_u64(&tr, "fold/step_done", idx);
 
@@ -888,35 +1039,71 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
     ub_k(&pb, running);
     ub_digest(&pb, fold_digest);
     /* rlc_rhos, rlc_parent, dec_children */
-    ub_push(&pb, kt);
-    for (size_t i = 0; i < kt; i++)
-      for (int q = 0; q < D * D; q++) ub_push(&pb, rhos[i][q]);
-    ser_me(&pb, &parent);
-    ub_push(&pb, k_dec);
-    for (size_t i = 0; i < k_dec; i++) ser_me(&pb, &children[i]);
-    /* mem sidecar (empty): cpu_me_claims_val, shout_addr_pre{claimed_sums, active_mask, round_polys, r_addr}, proofs */
-    ub_push(&pb, 0);
-    ub_push(&pb, 0);
-    ub_push(&pb, 0);
-    ub_push(&pb, 0);
-    ub_push(&pb, 0);
-    ub_push(&pb, 0);
-    /* batched_time: claimed_sums, degree_bounds, labels, round_polys */
-    ub_push(&pb, 1);
-    ub_k(&pb, T);
-    ub_push(&pb, 1);
-    ub_push(&pb, d_sc);
-    ub_push(&pb, 1);
-    ub_push(&pb, 8);
-    for (int i = 0; i < 8; i++) ub_push(&pb, (uint8_t)"ccs/time"[i]);
-    ub_push(&pb, 1);
-    ub_push(&pb, ell_n);
-    for (size_t rd = 0; rd < ell_n; rd++) {
-      ub_push(&pb, xs_n);
-      for (size_t i = 0; i < xs_n; i++) ub_k(&pb, time_coeffs[rd * xs_n + i]);
+    ser_lane(&pb, &mainl, k_dec);
+    /* mem: MemSidecarProof { cpu_me_claims_val, shout_addr_pre, proofs } */
+    ub_push(&pb, nvc);
+    for (size_t q = 0; q < nvc; q++) ser_me(&pb, &vclaims[q]);
+    ub_push(&pb, sm->n_lut);
+    for (size_t i = 0; i < sm->n_lut; i++) ub_k(&pb, sp.claimed[i]);
+    ub_push(&pb, sp.active_mask);
+    ub_push(&pb, sp.n_active);
+    for (size_t i = 0; i < sp.n_active; i++) ser_rounds(&pb, sp.rounds[i], sp.ell_addr, sp.degs[i] + 1);
+    ub_push(&pb, sm->n_lut ? sp.ell_addr : 0);
+    for (size_t i = 0; sm->n_lut && i < sp.ell_addr; i++) ub_k(&pb, sp.r_addr[i]);
+    ub_push(&pb, sm->n_lut + sm->n_mem);
+    for (size_t i = 0; i < sm->n_lut; i++) {
+      ub_push(&pb, 1); /* MemOrLutProof::Shout(ShoutProof::default()) */
+      ub_push(&pb, 0);
+      ub_push(&pb, 0);
+      ub_push(&pb, 0);
     }
-    /* val_fold: None */
-    ub_push(&pb, 0);
+    for (size_t i = 0; i < sm->n_mem; i++) {
+      const ra_twist_pre* P = &tp[i];
+      ub_push(&pb, 0); /* MemOrLutProof::Twist */
+      ub_push(&pb, 2);
+      ub_k(&pb, K0);
+      ub_k(&pb, K0);
+      ub_push(&pb, 2);
+      for (int c = 0; c < 2; c++) ser_rounds(&pb, P->rounds[c], P->ell_addr, P->degs[c] + 1);
+      ub_push(&pb, P->ell_addr);
+      for (size_t b = 0; b < P->ell_addr; b++) ub_k(&pb, P->r_addr[b]);
+      /* val_eval: Some(TwistValEvalProof) */
+      const ra_claim* vc = &vp.claims[vp.per * i];
+      ub_push(&pb, 1);
+      ub_k(&pb, vc[0].claimed_sum);
+      ser_rounds(&pb, vc[0].polys, ell_n, vc[0].deg + 1);
+      ub_k(&pb, vc[1].claimed_sum);
+      ser_rounds(&pb, vc[1].polys, ell_n, vc[1].deg + 1);
+      if (vp.per == 3) {
+        ub_push(&pb, 1);
+        ub_k(&pb, vc[2].claimed_sum);
+        ub_push(&pb, 1);
+        ser_rounds(&pb, vc[2].polys, ell_n, vc[2].deg + 1);
+      } else {
+        ub_push(&pb, 0);
+        ub_push(&pb, 0);
+      }
+    }
+    /* batched_time: claimed_sums, degree_bounds, labels, round_polys */
+    ub_push(&pb, ncl);
+    for (size_t i = 0; i < ncl; i++) ub_k(&pb, cl[i].claimed_sum);
+    ub_push(&pb, ncl);
+    for (size_t i = 0; i < ncl; i++) ub_push(&pb, cl[i].deg);
+    ub_push(&pb, ncl);
+    for (size_t i = 0; i < ncl; i++) {
+      size_t ln = strlen(cl[i].label);
+      ub_push(&pb, ln);
+      for (size_t q = 0; q < ln; q++) ub_push(&pb, (uint8_t)cl[i].label[q]);
+    }
+    ub_push(&pb, ncl);
+    for (size_t i = 0; i < ncl; i++) ser_rounds(&pb, cl[i].polys, ell_n, cl[i].deg + 1);
+    /* val_fold */
+    if (nvc) {
+      ub_push(&pb, 1);
+      ser_lane(&pb, &vall, k_dec);
+    } else {
+      ub_push(&pb, 0);
+    }
 
     {
       struct timespec now;
@@ -926,7 +1113,7 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
             (double)(now.tv_sec - ts_step.tv_sec) + 1e-9 * (double)(now.tv_nsec - ts_step.tv_nsec);
       ts_step = now;
     }
-    /* ---- advance the accumulator ---- */
+    /* ---- advance the accumulator and collect the val-lane obligations ---- */
     for (size_t a = 0; a < nacc; a++) {
       me_free(&acc[a]);
       free(acc_wit[a]);
@@ -934,34 +1121,63 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
     free(acc);
     free(acc_wit);
     nacc = k_dec;
-    acc = children;
-    acc_wit = zsplit;
+    acc = mainl.children;
+    acc_wit = mainl.zsplit;
+    mainl.children = NULL;
+    mainl.zsplit = NULL;
+    lane_free(&mainl, k_dec, 0);
+    if (nvc) {
+      val_obl = (me_t*)realloc(val_obl, (nval + k_dec) * sizeof(me_t));
+      memcpy(val_obl + nval, vall.children, k_dec * sizeof(me_t));
+      nval += k_dec;
+      free(vall.children);
+      vall.children = NULL;
+      lane_free(&vall, k_dec, 1);
+      for (size_t q = 0; q < nvc; q++) me_free(&vclaims[q]);
+    }
+    if (prev_tw) {
+      for (size_t i = 0; i < prev_nmem; i++) ra_twist_cols_free(&prev_tw[i]);
+      free(prev_tw);
+    }
+    prev_nmem = sm->n_mem;
+    prev_tw = (ra_twist_cols*)calloc(prev_nmem + 1, sizeof(ra_twist_cols));
+    for (size_t i = 0; i < prev_nmem; i++) {
+      prev_tw[i] = tp[i].cols;
+      memset(&tp[i].cols, 0, sizeof(tp[i].cols));
+      ra_twist_pre_free(&tp[i]);
+    }
+    free(tp);
+    ra_shout_pre_free(&sp);
+    ra_val_pre_free(&vp);
+    for (size_t i = 0; i < ncl; i++) ra_claim_free(&cl[i]);
+    free(cl);
     for (size_t i = 0; i < kt; i++) me_free(&ccs_out[i]);
     free(ccs_out);
-    me_free(&parent);
-    for (size_t i = 0; i < kt; i++) free(rhos[i]);
-    free(rhos);
-    free(rho_a);
-    free(zmix);
-    free(y_child);
     free(y_eval);
     free(aj_coeffs);
-    free(time_coeffs);
     free(r_time);
     free(Zs);
   }
   clock_gettime(CLOCK_MONOTONIC, &t1);
   if (seconds_per_step)
     *seconds_per_step = ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) / (double)nsteps;
-  /* output obligations (main) */
+  /* ShardObligations { main, val } */
   ub_push(&pb, nacc);
   for (size_t a = 0; a < nacc; a++) ser_me(&pb, &acc[a]);
+  ub_push(&pb, nval);
+  for (size_t a = 0; a < nval; a++) ser_me(&pb, &val_obl[a]);
   for (size_t a = 0; a < nacc; a++) {
     me_free(&acc[a]);
     free(acc_wit[a]);
   }
   free(acc);
   free(acc_wit);
+  for (size_t a = 0; a < nval; a++) me_free(&val_obl[a]);
+  free(val_obl);
+  if (prev_tw) {
+    for (size_t i = 0; i < prev_nmem; i++) ra_twist_cols_free(&prev_tw[i]);
+    free(prev_tw);
+  }
   for (size_t st = 0; st < nsteps; st++) free(Zmcs[st]);
   free(Zmcs);
   free(cmcs);
