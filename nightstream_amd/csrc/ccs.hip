@@ -83,17 +83,44 @@ void launch_spmv(const DevSparse& M, const K2* x, K2* y, K2 coeff, bool accumula
                      accumulate ? 1 : 0);
 }
 
+// v = M^T chi over the CSC arrays: one thread per column; columns with more than SPMV_HEAVY
+// entries are left to k_spmv_t_heavy (a thread walking 500k entries would take ~0.2 s)
 __global__ void k_spmv_t(const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ row,
                          const uint64_t* __restrict__ val, uint32_t m, const K2* __restrict__ chi, uint32_t n_eff,
                          K2* __restrict__ v) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= m) return;
+  const uint32_t lo = ptr[c], hi = ptr[c + 1];
+  if (hi - lo > SPMV_HEAVY) return;
   K2 acc = kzero();
-  for (uint32_t k = ptr[c]; k < ptr[c + 1]; k++) {
+  for (uint32_t k = lo; k < hi; k++) {
     uint32_t r = row[k];
     if (r < n_eff) acc = kadd(acc, kscale(chi[r], val[k]));
   }
   v[c] = acc;
+}
+
+// one 1024-thread workgroup per heavy column: strided partial sums, then a tree in LDS
+__global__ __launch_bounds__(1024) void k_spmv_t_heavy(const uint32_t* __restrict__ ptr,
+                                                       const uint32_t* __restrict__ row,
+                                                       const uint64_t* __restrict__ val,
+                                                       const uint32_t* __restrict__ cols, const K2* __restrict__ chi,
+                                                       uint32_t n_eff, K2* __restrict__ v) {
+  __shared__ K2 red[1024];
+  const uint32_t c = cols[blockIdx.x];
+  const uint32_t lo = ptr[c], hi = ptr[c + 1];
+  K2 acc = kzero();
+  for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+    const uint32_t r = row[k];
+    if (r < n_eff) acc = kadd(acc, kscale(chi[r], val[k]));
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = kadd(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) v[c] = red[0];
 }
 
 __global__ void k_copy_prefix(const K2* __restrict__ x, size_t cap, size_t m, K2* __restrict__ v) {
@@ -110,6 +137,9 @@ void launch_spmv_t(const DevSparse& M, const K2* chi, size_t n_eff, K2* v, hipSt
     return;
   }
   hipLaunchKernelGGL(k_spmv_t, g, dim3(256), 0, s, M.csc_ptr, M.csc_row, M.csc_val, M.m, chi, (uint32_t)n_eff, v);
+  if (M.n_heavy)
+    hipLaunchKernelGGL(k_spmv_t_heavy, dim3(M.n_heavy), dim3(1024), 0, s, M.csc_ptr, M.csc_row, M.csc_val,
+                       M.heavy_cols, chi, (uint32_t)n_eff, v);
 }
 
 // ---------------- Eval column sums ----------------

@@ -68,7 +68,12 @@ struct DevSparse {
   uint32_t* csc_ptr = nullptr;
   uint32_t* csc_row = nullptr;
   uint64_t* csc_val = nullptr;
+  // columns with more than SPMV_HEAVY entries (a constant-one column of padding rows holds
+  // hundreds of thousands): M^T chi reduces each of them with a whole workgroup
+  uint32_t n_heavy = 0;
+  uint32_t* heavy_cols = nullptr;
 };
+constexpr uint32_t SPMV_HEAVY = 1024;
 
 // Compiled CCS polynomial f for the round kernels: sum_t coeff_t * prod_v x_{var}^{exp}.
 struct FPoly {

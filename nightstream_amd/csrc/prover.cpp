@@ -308,6 +308,11 @@ std::unique_ptr<Ccs> Ccs::from_r1cs(size_t n, size_t m, const HostCsr& a, const 
     D_.csc_ptr = (uint32_t*)up(cp.data(), cp.size() * 4);
     D_.csc_row = (uint32_t*)up(rows.data(), std::max<size_t>(nnz, 1) * 4);
     D_.csc_val = (uint64_t*)up(cv.data(), std::max<size_t>(nnz, 1) * 8);
+    std::vector<uint32_t> heavy;
+    for (size_t cc = 0; cc < m; cc++)
+      if (cp[cc + 1] - cp[cc] > SPMV_HEAVY) heavy.push_back((uint32_t)cc);
+    D_.n_heavy = (uint32_t)heavy.size();
+    if (!heavy.empty()) D_.heavy_cols = (uint32_t*)up(heavy.data(), heavy.size() * 4);
   }
   ccs_digest(*s, s->digest);
   return s;
