@@ -56,6 +56,7 @@ int ns_device_synchronize(void);
 int ns_params_goldilocks_auto_r1cs_ccs(size_t n_rows, uint64_t out[11]);
 
 /* ---- CCS structure: r1cs_to_ccs (neo-ccs/src/r1cs.rs:13-43), identity-first, n == m ----
+ * (host only: the matrices reach HBM when a prover or an oracle first uses the structure)
  * CSR per matrix: row_ptr[n+1], col[nnz], val[nnz].  The digest is digest_ccs_matrices
  * (neo-reductions/src/engines/utils.rs:214-321). */
 int ns_ccs_from_r1cs(size_t n, size_t m, const uint64_t* a_ptr, const uint32_t* a_col, const uint64_t* a_val,
@@ -163,6 +164,16 @@ typedef struct {
 } ns_step_mem;
 int ns_prove_shard_mem(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps, size_t m_in,
                        const ns_step_mem* mem, uint8_t** proof, size_t* proof_len, double* seconds);
+/* ---- fold_shard_verify (neo-fold/src/shard.rs:2482-2994): replays the Fiat-Shamir transcript
+ *      of a canonical proof and re-derives every challenge (alpha, beta, gamma, r_cycle, every
+ *      round challenge, the RLC rotations, the digests), then checks the sum-check chains, the
+ *      Pi_CCS terminal identity, the Route A claim links, Pi_RLC / Pi_DEC of both lanes and both
+ *      obligation lanes.  x_steps[s]: step s's public input (m_in values); mem as for
+ *      ns_prove_shard_mem (NULL for CCS-only shards); mcs_commitments: n_steps x kappa*D expected
+ *      MCS commitments, or NULL to take them from the proof.  Host only (no device needed).
+ *      Returns 0 when the proof is accepted, NS_EPROTOCOL with the reason otherwise. */
+int ns_shard_verify(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_steps, size_t m_in,
+                    const ns_step_mem* mem, const uint64_t* mcs_commitments, const uint8_t* proof, size_t proof_len);
 /* Per-kernel timing of the next proves (HIP events on the prover's stream); classes:
  * 0 round-0 eval, 1 round-2 eval, 2 table fold, 3 Ajtai GEMM, 4 digit x K GEMM, 5 RLC Z_mix,
  * 6 DEC split, 7 round-1 eval, 8 fused fold + round (rounds >= 3).  ms[i] is the summed

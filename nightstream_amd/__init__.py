@@ -25,7 +25,7 @@ EXPORTS = [
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
-    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
+    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_shard_verify", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
     "ns_poseidon2_permute",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
@@ -105,6 +105,7 @@ def _load():
     L.ns_prove_shard_device.argtypes = L.ns_prove_shard.argtypes
     L.ns_prove_shard_mem.argtypes = [vp, vp, C.c_int, sz, sz, vp, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
                                      C.POINTER(C.c_double)]
+    L.ns_shard_verify.argtypes = [vp, vp, sz, sz, vp, vp, vp, sz]
     L.ns_prover_set_timing.argtypes = [vp, C.c_int]
     L.ns_prover_kernel_stats.argtypes = [vp, vp, vp, sz]
     L.ns_prover_phase_stats.argtypes = [vp, vp, sz]
@@ -358,6 +359,22 @@ def _step_mem_array(mems, keep: list):
         arr[s] = _StepMem(len(sm.luts), C.cast(luts, C.c_void_p), len(sm.mems), C.cast(ms, C.c_void_p))
     keep.append(arr)
     return arr
+
+
+def shard_verify(ccs: "Ccs", xs, proof: bytes, mem=None, mcs_commitments=None, m_in: int = 1) -> None:
+    """ns_shard_verify: replay the transcript of a canonical shard proof and check it (host only).
+    xs: each step's public input (m_in values); mem: per-step memory instances or None.  Raises
+    NsError with the reason when the proof is rejected."""
+    xs = [np.ascontiguousarray(np.asarray(x, np.uint64)[:m_in]) for x in xs]
+    keep = []
+    marr = _step_mem_array(mem, keep) if mem is not None else None
+    mc = None
+    if mcs_commitments is not None:
+        mc = np.ascontiguousarray(np.asarray(mcs_commitments, np.uint64).reshape(-1))
+    buf = (C.c_uint8 * len(proof)).from_buffer_copy(proof)
+    _check(lib().ns_shard_verify(ccs.h, _ptr_array(xs), len(xs), m_in,
+                                 None if marr is None else C.cast(marr, C.c_void_p),
+                                 None if mc is None else mc.ctypes.data, buf, len(proof)))
 
 
 class FoldingSession:

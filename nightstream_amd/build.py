@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnightstream.so")
-SOURCES = ["ajtai.hip", "ccs.hip", "twist.hip", "bus.hip", "transcript.cpp", "prover.cpp", "route_a.cpp", "capi.cpp"]
+SOURCES = ["ajtai.hip", "ccs.hip", "twist.hip", "bus.hip", "transcript.cpp", "prover.cpp", "route_a.cpp", "verify.cpp", "capi.cpp"]
 ARCH = os.environ.get("NS_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

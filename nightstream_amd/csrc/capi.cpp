@@ -100,7 +100,6 @@ int ns_ccs_from_r1cs(size_t n, size_t m, const uint64_t* a_ptr, const uint32_t* 
                      const uint64_t* b_ptr, const uint32_t* b_col, const uint64_t* b_val, const uint64_t* c_ptr,
                      const uint32_t* c_col, const uint64_t* c_val, ns_ccs** out) {
   return guard([&] {
-    require_device();
     auto mk = [&](const uint64_t* p, const uint32_t* c, const uint64_t* v) {
       HostCsr h;
       h.ptr.assign(p, p + n + 1);
@@ -463,6 +462,29 @@ int ns_prove_shard_mem(ns_prover* p, const uint64_t* const* z_steps, int z_on_de
     *proof = buf;
     *proof_len = enc.size() * 8;
     if (seconds) *seconds = p->p->last_seconds;
+  });
+}
+
+int ns_shard_verify(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_steps, size_t m_in,
+                    const ns_step_mem* mem, const uint64_t* mcs_commitments, const uint8_t* proof, size_t proof_len) {
+  return guard([&] {
+    if (!s || !proof || (n_steps && !x_steps)) throw Error(1, "null argument");
+    std::vector<std::vector<LutInstH>> luts(n_steps);
+    std::vector<std::vector<MemInstH>> mems(n_steps);
+    std::vector<StepMemH> steps(n_steps);
+    for (size_t st = 0; mem && st < n_steps; st++) {
+      for (size_t i = 0; i < mem[st].n_lut; i++) {
+        const ns_lut_inst& a = mem[st].luts[i];
+        luts[st].push_back(LutInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.table});
+      }
+      for (size_t i = 0; i < mem[st].n_mem; i++) {
+        const ns_mem_inst& a = mem[st].mems[i];
+        mems[st].push_back(MemInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.init_sparse != 0,
+                                    a.init_sparse ? a.ninit : 0, a.init_addrs, a.init_vals});
+      }
+      steps[st] = StepMemH{luts[st].size(), luts[st].data(), mems[st].size(), mems[st].data()};
+    }
+    verify_shard(*s->s, x_steps, n_steps, m_in, mem ? steps.data() : nullptr, mcs_commitments, proof, proof_len);
   });
 }
 

@@ -268,22 +268,32 @@ std::unique_ptr<Ccs> Ccs::from_r1cs(size_t n, size_t m, const HostCsr& a, const 
   }
   f.deg_max = (int)std::max<size_t>(4, s->f_degree() + 1);
   if (f.deg_max != 4) throw Error(1, "f degree > 3 is not supported by the round kernels");
-  // device matrices
-  s->dev.resize(4);
-  s->dev[0].identity = true;
-  s->dev[0].n = (uint32_t)n;
-  s->dev[0].m = (uint32_t)m;
   for (size_t j = 1; j < 4; j++) {
     const HostCsr& H = s->host[j];
     if (H.ptr.size() != n + 1) throw Error(1, "CSR row pointer has wrong length");
+    for (size_t k = 0; k < H.ptr[n]; k++)
+      if (H.col[k] >= m) throw Error(1, "CSR column index out of range");
+  }
+  ccs_digest(*s, s->digest);
+  return s;
+}
+
+// The matrices in HBM (CSR and CSC, heavy-column lists), uploaded on first use by a prover or an
+// oracle: a verifier needs only the host side, so constructing a CCS needs no device.
+void Ccs::ensure_device() {
+  std::lock_guard<std::mutex> g(dev_mu);
+  if (on_device) return;
+  dev.assign(4, DevSparse{});
+  dev[0].identity = true;
+  dev[0].n = (uint32_t)n;
+  dev[0].m = (uint32_t)m;
+  for (size_t j = 1; j < 4; j++) {
+    const HostCsr& H = host[j];
     size_t nnz = H.ptr[n];
     std::vector<uint32_t> rp(n + 1), cp(m + 1, 0), rows(nnz);
     std::vector<uint64_t> cv(nnz);
     for (size_t r = 0; r <= n; r++) rp[r] = (uint32_t)H.ptr[r];
-    for (size_t k = 0; k < nnz; k++) {
-      if (H.col[k] >= m) throw Error(1, "CSR column index out of range");
-      cp[H.col[k] + 1]++;
-    }
+    for (size_t k = 0; k < nnz; k++) cp[H.col[k] + 1]++;
     for (size_t cc = 0; cc < m; cc++) cp[cc + 1] += cp[cc];
     std::vector<uint32_t> pos2(cp.begin(), cp.end() - 1);
     for (size_t r = 0; r < n; r++)
@@ -293,14 +303,14 @@ std::unique_ptr<Ccs> Ccs::from_r1cs(size_t n, size_t m, const HostCsr& a, const 
         cv[pos2[cc]] = H.val[k];
         pos2[cc]++;
       }
-    DevSparse& D_ = s->dev[j];
+    DevSparse& D_ = dev[j];
     D_.n = (uint32_t)n;
     D_.m = (uint32_t)m;
     D_.nnz = (uint32_t)nnz;
     auto up = [&](const void* src, size_t bytes) {
-      s->bufs.emplace_back(bytes);
-      NS_CHECK(hipMemcpy(s->bufs.back().p, src, bytes, hipMemcpyHostToDevice));
-      return s->bufs.back().p;
+      bufs.emplace_back(bytes);
+      NS_CHECK(hipMemcpy(bufs.back().p, src, bytes, hipMemcpyHostToDevice));
+      return bufs.back().p;
     };
     D_.csr_ptr = (uint32_t*)up(rp.data(), rp.size() * 4);
     D_.csr_col = (uint32_t*)up(H.col.data(), std::max<size_t>(nnz, 1) * 4);
@@ -314,8 +324,7 @@ std::unique_ptr<Ccs> Ccs::from_r1cs(size_t n, size_t m, const HostCsr& a, const 
     D_.n_heavy = (uint32_t)heavy.size();
     if (!heavy.empty()) D_.heavy_cols = (uint32_t*)up(heavy.data(), heavy.size() * 4);
   }
-  ccs_digest(*s, s->digest);
-  return s;
+  on_device = true;
 }
 
 // ---------------- PP ----------------
@@ -605,6 +614,7 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
                      const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st,
                      OracleWs* ws)
     : s_(s), st_(st), k_(k), ell_d_(ell_d), ell_n_(ell_n) {
+  const_cast<Ccs&>(s).ensure_device();
   const size_t n_pad = (size_t)1 << ell_n, m = s.m, n = s.n;
   if (n_pad != n) throw Error(1, "row domain must equal n (power of two)");
   if (!ws) {
@@ -850,12 +860,14 @@ struct Enc {
 
 // ---------------- prover ----------------
 Prover::Prover(const Ccs& s, const Pp& pp) : s_(&s), pp_(&pp) {
+  const_cast<Ccs&>(s).ensure_device();
   if (pp.m != s.m) throw Error(1, "PP width does not match CCS m");
   params_ = Params::goldilocks_auto_r1cs_ccs(s.n);
   NS_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 }
 
 void Prover::rebind(const Ccs& s, const Pp& pp) {
+  const_cast<Ccs&>(s).ensure_device();
   if (pp.m != s.m) throw Error(1, "PP width does not match CCS m");
   NS_CHECK(hipStreamSynchronize(stream_));
   s_ = &s;
@@ -879,6 +891,161 @@ static size_t log2_pow2(size_t n) {
   size_t l = 0;
   while (((size_t)1 << l) < n) l++;
   return l < 1 ? 1 : l;
+}
+
+// ---------------- transcript steps shared by the prover and the verifier ----------------
+// bind_header_and_instances_with_digest + bind_me_inputs (neo-reductions/src/engines/utils.rs:83-181)
+void bind_step_header(Transcript& tr, const Ccs& s, const Params& params, size_t ell, size_t d_sc, int32_t slack,
+                      const uint64_t* x, size_t m_in, const uint64_t* c_mcs, const std::vector<MeHost>& acc) {
+  tr.append_message("phase/pi-ccs", (const uint8_t*)nullptr, 0);
+  tr.append_message("neo/ccs/header/v1", (const uint8_t*)nullptr, 0);
+  uint64_t hdr[6] = {64, 2, params.lambda, ell, d_sc, (uint64_t)(slack < 0 ? -slack : slack)};
+  tr.append_u64s("ccs/header", hdr, 6);
+  uint8_t sign = slack >= 0 ? 1 : 0;
+  tr.append_message("ccs/slack_sign", &sign, 1);
+  tr.append_message("neo/ccs/instances", (const uint8_t*)nullptr, 0);
+  uint64_t dd[3] = {s.n, s.m, s.t};
+  tr.append_u64s("dims", dd, 3);
+  for (int i = 0; i < 4; i++) tr.append_fields("mat_digest", &s.digest[i], 1);
+  tr.append_message("neo/ccs/poly", (const uint8_t*)nullptr, 0);
+  uint64_t ar = s.t, tl = s.term_coeff.size();
+  tr.append_u64s("arity", &ar, 1);
+  tr.append_u64s("terms_len", &tl, 1);
+  std::vector<size_t> order(tl);
+  for (size_t i = 0; i < tl; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return s.term_exps[a] < s.term_exps[b]; });
+  for (size_t i : order) {
+    tr.append_fields("coeff", &s.term_coeff[i], 1);
+    std::vector<uint64_t> ex(s.term_exps[i].begin(), s.term_exps[i].end());
+    tr.append_u64s("exps", ex.data(), ex.size());
+  }
+  tr.append_fields("x", x, m_in);
+  uint64_t mi = m_in;
+  tr.append_u64s("m_in", &mi, 1);
+  tr.append_fields("c_data", c_mcs, KAPPA * D);
+  tr.append_message("neo/ccs/me_inputs/v2", (const uint8_t*)nullptr, 0);
+  uint64_t cnt = acc.size();
+  tr.append_u64s("me_count", &cnt, 1);
+  std::vector<uint64_t> buf;
+  for (auto& e : acc) {
+    tr.append_fields("c_data_in", e.c, KAPPA * D);
+    mi = e.m_in;
+    tr.append_u64s("m_in_in", &mi, 1);
+    buf.clear();
+    for (auto& v : e.r) {
+      buf.push_back(v.c0);
+      buf.push_back(v.c1);
+    }
+    tr.append_fields("r_in", buf.data(), buf.size());
+    buf.clear();
+    for (auto& v : e.y) {
+      buf.push_back(v.c0);
+      buf.push_back(v.c1);
+    }
+    tr.append_fields("y_elem", buf.data(), buf.size());
+  }
+}
+
+// sample_challenges (utils.rs:184-212): alpha (ell_d), beta (ell), gamma, each K = two challenges
+StepChallenges sample_challenges(Transcript& tr, size_t ell_d, size_t ell) {
+  StepChallenges ch;
+  tr.append_message("neo/ccs/chals/v1", (const uint8_t*)nullptr, 0);
+  auto draw = [&]() {
+    uint64_t c2[2];
+    tr.challenge_fields("chal/k", 2, c2);
+    return K2{c2[0], c2[1]};
+  };
+  for (size_t i = 0; i < ell_d; i++) ch.alpha.push_back(draw());
+  for (size_t i = 0; i < ell; i++) ch.beta.push_back(draw());
+  ch.gamma = draw();
+  return ch;
+}
+
+// claimed initial sum T from the ME inputs (optimized_engine/common.rs:839-947), core rows only
+K2 claimed_initial_sum(const Ccs& s, const StepChallenges& ch, const std::vector<MeHost>& acc, size_t ell_d) {
+  const size_t nacc = acc.size(), kt = nacc + 1;
+  if (kt < 2) return kzero();
+  std::vector<K2> chi_a = chi_host(ch.alpha.data(), ell_d);
+  K2 gk = kone();
+  for (size_t i = 0; i < kt; i++) gk = kmul(gk, ch.gamma);
+  K2 inner = kzero();
+  for (size_t j = 0; j < s.t; j++) {
+    K2 gkj = kone();
+    for (size_t q = 0; q < j; q++) gkj = kmul(gkj, gk);
+    K2 gi = kone();
+    for (size_t a = 0; a < nacc; a++) {
+      gi = kmul(gi, ch.gamma);  // gamma^{i-1}, i = a + 2
+      K2 ye = kzero();
+      for (int rho = 0; rho < DPAD; rho++) ye = kadd(ye, kmul(acc[a].y[j * DPAD + rho], chi_a[rho]));
+      inner = kadd(inner, kmul(kmul(gi, gkj), ye));
+    }
+  }
+  return kmul(gk, inner);
+}
+
+// bind_rlc_inputs (neo-fold/src/shard.rs:1125-1168)
+void bind_rlc_inputs(Transcript& tr, const char* lane, size_t idx, const std::vector<MeHost>& ins) {
+  tr.append_message("fold/rlc_inputs/v1", lane);
+  uint64_t v = idx;
+  tr.append_u64s("step_idx", &v, 1);
+  v = ins.size();
+  tr.append_u64s("me_count", &v, 1);
+  for (auto& e : ins) {
+    tr.append_fields("c_data", e.c, KAPPA * D);
+    v = e.m_in;
+    tr.append_u64s("m_in", &v, 1);
+    tr.append_message("me_fold_digest", e.fold_digest, 32);
+    for (auto& x : e.r) tr.append_k("r_limb", x);
+    tr.append_fields("X", e.X.data(), e.X.size());
+    for (auto& x : e.y) tr.append_k("y_elem", x);
+    for (auto& x : e.y_scalars) tr.append_k("y_scalar", x);
+    v = 0;
+    tr.append_u64s("c_step_coords_len", &v, 1);
+    tr.append_fields("c_step_coords", nullptr, 0);
+    tr.append_u64s("u_offset", &v, 1);
+    tr.append_u64s("u_len", &v, 1);
+  }
+}
+
+// sample_rot_rhos_n (neo-reductions/src/common.rs:410-573): coefficient vectors a_i in [-2, 2]
+// drawn from transcript digests, rho_i = rot(a_i) (the identity when there is one input)
+void sample_rot_rhos(Transcript& tr, size_t kt, std::vector<int8_t>& rho_a, std::vector<uint64_t>& rhos) {
+  rho_a.assign(kt * D, 0);
+  rhos.assign(kt * D * D, 0);
+  for (size_t i = 0; i < kt; i++) {
+    tr.append_msg_u64("rlc/rot/index", i);
+    // draw_alphabet_vector (neo-reductions/src/common.rs:410-439)
+    static const int8_t A5[5] = {-2, -1, 0, 1, 2};
+    const uint32_t bucket = (65536u / 5u) * 5u;
+    size_t got = 0;
+    uint64_t ctr = i;
+    while (got < (size_t)D) {
+      tr.append_msg_u64("rlc/rot/chunk", ctr);
+      uint8_t dig[32];
+      tr.digest32(dig);
+      for (int w = 0; w < 16 && got < (size_t)D; w++) {
+        uint32_t x = (uint32_t)dig[2 * w] | ((uint32_t)dig[2 * w + 1] << 8);
+        if (x < bucket) rho_a[i * D + got++] = A5[x % 5];
+      }
+      ctr++;
+    }
+    uint64_t* R = &rhos[i * D * D];
+    if (kt == 1) {
+      for (int q = 0; q < D; q++) R[q * D + q] = 1;
+    } else {
+      // rot_from_coeffs: column j = X^j * a mod Phi_81
+      uint64_t col[D], nxt[D];
+      for (int r = 0; r < D; r++) col[r] = from_i64(rho_a[i * D + r]);
+      for (int j = 0; j < D; j++) {
+        for (int r = 0; r < D; r++) R[r * D + j] = col[r];
+        uint64_t last = col[D - 1];
+        nxt[0] = fneg(last);
+        for (int r = 1; r < D; r++) nxt[r] = col[r - 1];
+        nxt[27] = fsub(nxt[27], last);
+        memcpy(col, nxt, sizeof(col));
+      }
+    }
+  }
 }
 
 // ---------------- shared-bus openings on the device ----------------
@@ -978,72 +1145,18 @@ void Prover::rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const st
   const size_t m = s.m, t = s.t, k_dec = params_.k_rho, kt = ins.size(), m_in = ins[0].m_in;
   const size_t t_tot = t + L.bus_cols;
   hipStream_t st = stream_;
-  tr.append_message("fold/rlc_inputs/v1", lane);
-  {
-    uint64_t v = idx;
-    tr.append_u64s("step_idx", &v, 1);
-    v = kt;
-    tr.append_u64s("me_count", &v, 1);
-    for (auto& e : ins) {
-      tr.append_fields("c_data", e.c, KAPPA * D);
-      v = e.m_in;
-      tr.append_u64s("m_in", &v, 1);
-      tr.append_message("me_fold_digest", e.fold_digest, 32);
-      for (auto& x : e.r) tr.append_k("r_limb", x);
-      tr.append_fields("X", e.X.data(), e.X.size());
-      for (auto& x : e.y) tr.append_k("y_elem", x);
-      for (auto& x : e.y_scalars) tr.append_k("y_scalar", x);
-      v = 0;
-      tr.append_u64s("c_step_coords_len", &v, 1);
-      tr.append_fields("c_step_coords", nullptr, 0);
-      tr.append_u64s("u_offset", &v, 1);
-      tr.append_u64s("u_len", &v, 1);
-    }
-  }
+  bind_rlc_inputs(tr, lane, idx, ins);
   if ((uint64_t)kt * 216u * (params_.b - 1) >= ((uint64_t)1 << params_.k_rho))
     throw Error(5, "RLC norm bound violated");
-  std::vector<int8_t> rho_a(kt * D);
-  out.rhos.assign(kt * D * D, 0);
+  std::vector<int8_t> rho_a;
+  sample_rot_rhos(tr, kt, rho_a, out.rhos);
   std::vector<int8_t> rho_i8(kt * 64 * 64, 0);  // [i][row][col] padded to 64 x 64
-  for (size_t i = 0; i < kt; i++) {
-    tr.append_msg_u64("rlc/rot/index", i);
-    // draw_alphabet_vector (neo-reductions/src/common.rs:410-439)
-    static const int8_t A5[5] = {-2, -1, 0, 1, 2};
-    const uint32_t bucket = (65536u / 5u) * 5u;
-    size_t got = 0;
-    uint64_t ctr = i;
-    while (got < (size_t)D) {
-      tr.append_msg_u64("rlc/rot/chunk", ctr);
-      uint8_t dig[32];
-      tr.digest32(dig);
-      for (int w = 0; w < 16 && got < (size_t)D; w++) {
-        uint32_t x = (uint32_t)dig[2 * w] | ((uint32_t)dig[2 * w + 1] << 8);
-        if (x < bucket) rho_a[i * D + got++] = A5[x % 5];
-      }
-      ctr++;
-    }
-    uint64_t* R = &out.rhos[i * D * D];
-    if (kt == 1) {
-      for (int q = 0; q < D; q++) R[q * D + q] = 1;
-    } else {
-      // rot_from_coeffs: column j = X^j * a mod Phi_81
-      uint64_t col[D], nxt[D];
-      for (int r = 0; r < D; r++) col[r] = from_i64(rho_a[i * D + r]);
-      for (int j = 0; j < D; j++) {
-        for (int r = 0; r < D; r++) R[r * D + j] = col[r];
-        uint64_t last = col[D - 1];
-        nxt[0] = fneg(last);
-        for (int r = 1; r < D; r++) nxt[r] = col[r - 1];
-        nxt[27] = fsub(nxt[27], last);
-        memcpy(col, nxt, sizeof(col));
-      }
-    }
+  for (size_t i = 0; i < kt; i++)
     for (int q = 0; q < D * D; q++) {
-      int64_t v = to_balanced(R[q]);
+      const int64_t v = to_balanced(out.rhos[i * D * D + q]);
       if (v < -127 || v > 127) throw Error(5, "rotation entry out of int8 range");
       rho_i8[(i * 64 + q / D) * 64 + q % D] = (int8_t)v;
     }
-  }
   std::vector<K2> bpow(D);
   bpow[0] = kone();
   for (int i = 1; i < D; i++) bpow[i] = kadd(bpow[i - 1], bpow[i - 1]);
@@ -1326,100 +1439,13 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     const size_t nacc = acc.size(), kt = nacc + 1;
     absorb_step_memory(tr, sm, digests);
     phase_mark(0);
-    // bind_header_and_instances_with_digest
-    tr.append_message("phase/pi-ccs", (const uint8_t*)nullptr, 0);
-    tr.append_message("neo/ccs/header/v1", (const uint8_t*)nullptr, 0);
-    {
-      uint64_t hdr[6] = {64, 2, params_.lambda, ell, d_sc, (uint64_t)(slack < 0 ? -slack : slack)};
-      tr.append_u64s("ccs/header", hdr, 6);
-      uint8_t sign = slack >= 0 ? 1 : 0;
-      tr.append_message("ccs/slack_sign", &sign, 1);
-      tr.append_message("neo/ccs/instances", (const uint8_t*)nullptr, 0);
-      uint64_t dd[3] = {s.n, s.m, s.t};
-      tr.append_u64s("dims", dd, 3);
-      for (int i = 0; i < 4; i++) tr.append_fields("mat_digest", &s.digest[i], 1);
-      tr.append_message("neo/ccs/poly", (const uint8_t*)nullptr, 0);
-      uint64_t ar = s.t, tl = s.term_coeff.size();
-      tr.append_u64s("arity", &ar, 1);
-      tr.append_u64s("terms_len", &tl, 1);
-      std::vector<size_t> order(tl);
-      for (size_t i = 0; i < tl; i++) order[i] = i;
-      std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return s.term_exps[a] < s.term_exps[b]; });
-      for (size_t i : order) {
-        tr.append_fields("coeff", &s.term_coeff[i], 1);
-        std::vector<uint64_t> ex(s.term_exps[i].begin(), s.term_exps[i].end());
-        tr.append_u64s("exps", ex.data(), ex.size());
-      }
-      tr.append_fields("x", x_host[idx].data(), m_in);
-      uint64_t mi = m_in;
-      tr.append_u64s("m_in", &mi, 1);
-      tr.append_fields("c_data", &cmcs[idx * KAPPA * D], KAPPA * D);
-    }
-    // bind_me_inputs
-    tr.append_message("neo/ccs/me_inputs/v2", (const uint8_t*)nullptr, 0);
-    {
-      uint64_t cnt = nacc;
-      tr.append_u64s("me_count", &cnt, 1);
-      std::vector<uint64_t> buf;
-      for (auto& e : acc) {
-        tr.append_fields("c_data_in", e.c, KAPPA * D);
-        uint64_t mi = e.m_in;
-        tr.append_u64s("m_in_in", &mi, 1);
-        buf.clear();
-        for (auto& x : e.r) {
-          buf.push_back(x.c0);
-          buf.push_back(x.c1);
-        }
-        tr.append_fields("r_in", buf.data(), buf.size());
-        buf.clear();
-        for (auto& x : e.y) {
-          buf.push_back(x.c0);
-          buf.push_back(x.c1);
-        }
-        tr.append_fields("y_elem", buf.data(), buf.size());
-      }
-    }
-    // sample_challenges
-    tr.append_message("neo/ccs/chals/v1", (const uint8_t*)nullptr, 0);
-    std::vector<K2> alpha(ell_d), beta(ell);
-    for (size_t i = 0; i < ell_d; i++) {
-      uint64_t c2[2];
-      tr.challenge_fields("chal/k", 2, c2);
-      alpha[i] = K2{c2[0], c2[1]};
-    }
-    for (size_t i = 0; i < ell; i++) {
-      uint64_t c2[2];
-      tr.challenge_fields("chal/k", 2, c2);
-      beta[i] = K2{c2[0], c2[1]};
-    }
-    K2 gamma;
-    {
-      uint64_t c2[2];
-      tr.challenge_fields("chal/k", 2, c2);
-      gamma = K2{c2[0], c2[1]};
-    }
-    const K2* beta_a = beta.data();
-    const K2* beta_r = beta.data() + ell_d;
-    // claimed initial sum T (paper_exact.rs:825-944)
-    K2 T = kzero();
-    if (kt >= 2) {
-      std::vector<K2> chi_a = chi_host(alpha.data(), ell_d);
-      K2 gk = kone();
-      for (size_t i = 0; i < kt; i++) gk = kmul(gk, gamma);
-      K2 inner = kzero();
-      for (size_t j = 0; j < t; j++) {
-        K2 gkj = kone();
-        for (size_t q = 0; q < j; q++) gkj = kmul(gkj, gk);
-        K2 gi = kone();
-        for (size_t a = 0; a < nacc; a++) {
-          gi = kmul(gi, gamma);  // gamma^{i-1}, i = a + 2
-          K2 ye = kzero();
-          for (int rho = 0; rho < DPAD; rho++) ye = kadd(ye, kmul(acc[a].y[j * DPAD + rho], chi_a[rho]));
-          inner = kadd(inner, kmul(kmul(gi, gkj), ye));
-        }
-      }
-      T = kmul(gk, inner);
-    }
+    bind_step_header(tr, s, params_, ell, d_sc, slack, x_host[idx].data(), m_in, &cmcs[idx * KAPPA * D], acc);
+    const StepChallenges ch = sample_challenges(tr, ell_d, ell);
+    const std::vector<K2>& alpha = ch.alpha;
+    const K2* beta_a = ch.beta.data();
+    const K2* beta_r = ch.beta.data() + ell_d;
+    const K2 gamma = ch.gamma;
+    const K2 T = claimed_initial_sum(s, ch, acc, ell_d);
     tr.append_k("sumcheck/initial_sum", T);
     std::vector<K2> r_cycle(ell_n);
     for (size_t i = 0; i < ell_n; i++) {

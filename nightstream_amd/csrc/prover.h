@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -76,7 +77,11 @@ struct Ccs {
   FPoly fpoly;                   // compiled over used variables
   std::vector<int> fvar_mat;     // CCS matrix index of each f-var table
   uint64_t digest[4];
+  std::mutex dev_mu;
+  bool on_device = false;
+  // host structure and digest only; the matrices reach HBM on the first ensure_device()
   static std::unique_ptr<Ccs> from_r1cs(size_t n, size_t m, const HostCsr& a, const HostCsr& b, const HostCsr& c);
+  void ensure_device();
   size_t f_degree() const;
 };
 
@@ -137,6 +142,19 @@ struct OracleWs {
 
 struct StepMemH;
 struct BusLayout;
+
+// Fiat-Shamir steps shared by the prover and the verifier (same labels and absorb order)
+struct StepChallenges {
+  std::vector<K2> alpha, beta;  // beta = beta_a (ell_d) | beta_r (ell_n)
+  K2 gamma{0, 0};
+};
+struct Ccs;
+void bind_step_header(Transcript& tr, const Ccs& s, const Params& params, size_t ell, size_t d_sc, int32_t slack,
+                      const uint64_t* x, size_t m_in, const uint64_t* c_mcs, const std::vector<MeHost>& acc);
+StepChallenges sample_challenges(Transcript& tr, size_t ell_d, size_t ell);
+K2 claimed_initial_sum(const Ccs& s, const StepChallenges& ch, const std::vector<MeHost>& acc, size_t ell_d);
+void bind_rlc_inputs(Transcript& tr, const char* lane, size_t idx, const std::vector<MeHost>& ins);
+void sample_rot_rhos(Transcript& tr, size_t kt, std::vector<int8_t>& rho_a, std::vector<uint64_t>& rhos);
 
 // One RLC -> DEC lane's output: the rotations, the parent, the k_rho children and their planes.
 struct LaneOut {

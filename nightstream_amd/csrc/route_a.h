@@ -122,4 +122,9 @@ void val_eval_batch(Transcript& tr, const StepMemH& sm, const std::vector<TwistP
                     const std::vector<TwistCols>* prev, size_t ell_n, const std::vector<K2>& r_time, size_t step_idx,
                     ValPre& out);
 
+// fold_shard_verify over the canonical encoding (verify.cpp): returns when the proof is accepted,
+// throws Error(7, reason) otherwise.  mcs_c: expected MCS commitments (n_steps x kappa*D) or null.
+void verify_shard(const Ccs& s, const uint64_t* const* x_steps, size_t nsteps, size_t m_in, const StepMemH* mem,
+                  const uint64_t* mcs_c, const uint8_t* proof, size_t proof_len);
+
 }  // namespace ns
