@@ -121,6 +121,34 @@ int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* out
 int ns_dec_split_device_noalloc(const int32_t* zmix_dev, size_t m, size_t k_dec, int8_t* const* outs_dev,
                                 int* nonzero);
 
+/* ---- lane-level Pi_RLC / Pi_DEC returning ME instances (neo-reductions/src/api.rs:131-324:
+ *      rlc_with_commit, dec_children_with_commit), on a prover context's stream and buffers.
+ * An MeInstance (neo-ccs/src/relations.rs) is passed with caller-owned arrays: X [54][m_in] F_q,
+ * r [ell_n] K, y [t][64] K (core matrices first, then one opening per shared-bus column), y_scalars
+ * [t] K.  Outputs use the same shapes as the inputs (the caller sizes their arrays). */
+typedef struct {
+  uint64_t c[16 * 54];
+  size_t m_in;
+  uint64_t* X;
+  size_t ell_n;
+  uint64_t* r;
+  size_t t;
+  uint64_t* y;
+  uint64_t* y_scalars;
+  uint8_t fold_digest[32];
+} ns_me;
+/* parent = sum_i rot(a_i) ins[i] with a_i = rho_coeffs[i*54 ..] in [-2, 2] (sample_rot_rhos_n's
+ * vectors; one input: the identity fast path), and Z_mix = sum_i rot(a_i) Z_i into zmix_dev
+ * (device int32 [54][m]); Z_dev[i]: device digit planes of the inputs. */
+int ns_rlc(ns_prover* p, const ns_me* ins, size_t k, const int8_t* rho_coeffs, const int8_t* const* Z_dev,
+           ns_me* parent, int32_t* zmix_dev);
+/* The k_dec children of parent: balanced base-2 planes of zmix_dev into planes_dev[i] (device
+ * int8 [54][m]), child commitments, X and y (rows past the CCS's t are bus openings of a shared
+ * bus of bus_chunk cycles per column; 0 when there is none).  ok[0..2] = the public checks on y,
+ * X and c (dec_children_with_commit's ok_y, ok_X, ok_c).  NS_ERANGE when Z_mix does not fit. */
+int ns_dec(ns_prover* p, const ns_me* parent, const int32_t* zmix_dev, size_t k_dec, int8_t* const* planes_dev,
+           size_t bus_chunk, ns_me* children, int* ok);
+
 /* ---- shard prover: fold_shard_prove (neo-fold/src/shard.rs:2285-2316) driven as
  *      FoldingSession::{add_step_io, fold_and_prove} (neo-fold/src/session.rs:1018-1311).
  * z_steps[s] is step s's full z = [x | w] (length m); x = z[0..m_in].  On success *proof

@@ -25,7 +25,7 @@ EXPORTS = [
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
-    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_shard_verify", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
+    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_shard_verify", "ns_rlc", "ns_dec", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
     "ns_poseidon2_permute",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
@@ -106,6 +106,8 @@ def _load():
     L.ns_prove_shard_mem.argtypes = [vp, vp, C.c_int, sz, sz, vp, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
                                      C.POINTER(C.c_double)]
     L.ns_shard_verify.argtypes = [vp, vp, sz, sz, vp, vp, vp, sz]
+    L.ns_rlc.argtypes = [vp, vp, sz, vp, vp, vp, vp]
+    L.ns_dec.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp]
     L.ns_prover_set_timing.argtypes = [vp, C.c_int]
     L.ns_prover_kernel_stats.argtypes = [vp, vp, vp, sz]
     L.ns_prover_phase_stats.argtypes = [vp, vp, sz]
@@ -359,6 +361,71 @@ def _step_mem_array(mems, keep: list):
         arr[s] = _StepMem(len(sm.luts), C.cast(luts, C.c_void_p), len(sm.mems), C.cast(ms, C.c_void_p))
     keep.append(arr)
     return arr
+
+
+class _Me(C.Structure):
+    _fields_ = [("c", C.c_uint64 * (KAPPA * D)), ("m_in", C.c_size_t), ("X", C.c_void_p), ("ell_n", C.c_size_t),
+                ("r", C.c_void_p), ("t", C.c_size_t), ("y", C.c_void_p), ("y_scalars", C.c_void_p),
+                ("fold_digest", C.c_uint8 * 32)]
+
+
+def _me_c(me, keep: list):
+    """ns_me for a proof.MeInstance (input), its arrays kept alive in `keep`."""
+    X = np.ascontiguousarray(np.asarray(me.X, np.uint64).reshape(-1))
+    r = np.array([w for k in me.r for w in k] or [0], np.uint64)
+    y = np.array([w for row in me.y for k in row for w in k] or [0], np.uint64)
+    ys = np.array([w for k in me.y_scalars for w in k] or [0], np.uint64)
+    keep += [X, r, y, ys]
+    e = _Me()
+    e.c[:] = [int(v) for v in me.c]
+    e.m_in, e.X, e.ell_n, e.r = me.m_in, X.ctypes.data, len(me.r), r.ctypes.data
+    e.t, e.y, e.y_scalars = len(me.y), y.ctypes.data, ys.ctypes.data
+    e.fold_digest[:] = list(me.fold_digest)
+    return e
+
+
+def _me_out(m_in: int, ell_n: int, t: int, keep: list):
+    X, r = np.zeros(max(1, D * m_in), np.uint64), np.zeros(max(1, 2 * ell_n), np.uint64)
+    y, ys = np.zeros(max(1, 2 * 64 * t), np.uint64), np.zeros(max(1, 2 * t), np.uint64)
+    keep.append((X, r, y, ys))
+    e = _Me()
+    e.m_in, e.X, e.ell_n, e.r, e.t, e.y, e.y_scalars = m_in, X.ctypes.data, ell_n, r.ctypes.data, t, y.ctypes.data, \
+        ys.ctypes.data
+    return e, (X, r, y, ys)
+
+
+def _me_py(e, arrs):
+    from .proof import MeInstance
+    X, r, y, ys = arrs
+    k = lambda a, i: (int(a[2 * i]), int(a[2 * i + 1]))  # noqa: E731
+    return MeInstance(np.array(e.c[:], np.uint64), e.m_in, X[:D * e.m_in].reshape(D, e.m_in),
+                      [k(r, i) for i in range(e.ell_n)], [[k(y, j * 64 + q) for q in range(64)] for j in range(e.t)],
+                      [k(ys, j) for j in range(e.t)], bytes(e.fold_digest))
+
+
+def rlc(session: "FoldingSession", mes, rho_coeffs, z_bufs, zmix_buf: "DeviceBuffer"):
+    """ns_rlc: Pi_RLC of ME instances (proof.MeInstance) with rotation vectors rho_coeffs (k x 54 in
+    [-2, 2]) over device digit planes; Z_mix lands in zmix_buf.  Returns the parent."""
+    keep = []
+    arr = (_Me * len(mes))(*[_me_c(m, keep) for m in mes])
+    a = np.ascontiguousarray(np.asarray(rho_coeffs, np.int8).reshape(-1))
+    out, arrs = _me_out(mes[0].m_in, len(mes[0].r), len(mes[0].y), keep)
+    zp = (C.c_void_p * len(z_bufs))(*[b.data_ptr() for b in z_bufs])
+    _check(lib().ns_rlc(session.h, arr, len(mes), a.ctypes.data, zp, C.byref(out), zmix_buf.data_ptr()))
+    return _me_py(out, arrs)
+
+
+def dec(session: "FoldingSession", parent, zmix_buf: "DeviceBuffer", k_dec: int, plane_bufs, bus_chunk: int = 0):
+    """ns_dec: the k_dec children of a parent from Z_mix (planes written to plane_bufs).  Returns
+    (children, (ok_y, ok_X, ok_c))."""
+    keep = []
+    pe = _me_c(parent, keep)
+    outs = [_me_out(parent.m_in, len(parent.r), len(parent.y), keep) for _ in range(k_dec)]
+    arr = (_Me * k_dec)(*[o[0] for o in outs])
+    pp = (C.c_void_p * k_dec)(*[b.data_ptr() for b in plane_bufs])
+    ok = (C.c_int * 3)()
+    _check(lib().ns_dec(session.h, C.byref(pe), zmix_buf.data_ptr(), k_dec, pp, bus_chunk, arr, ok))
+    return [_me_py(arr[i], outs[i][1]) for i in range(k_dec)], tuple(bool(x) for x in ok)
 
 
 def shard_verify(ccs: "Ccs", xs, proof: bytes, mem=None, mcs_commitments=None, m_in: int = 1) -> None:

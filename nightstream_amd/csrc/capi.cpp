@@ -488,6 +488,58 @@ int ns_shard_verify(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_st
   });
 }
 
+static MeHost me_in(const ns_me& a) {
+  MeHost e;
+  if ((a.m_in && !a.X) || (a.ell_n && !a.r) || (a.t && (!a.y || !a.y_scalars))) throw Error(1, "ME arrays missing");
+  memcpy(e.c, a.c, sizeof(e.c));
+  e.m_in = a.m_in;
+  e.X.assign(a.X, a.X + (size_t)D * a.m_in);
+  for (size_t i = 0; i < a.ell_n; i++) e.r.push_back(K2{a.r[2 * i], a.r[2 * i + 1]});
+  e.t = a.t;
+  for (size_t i = 0; i < a.t * DPAD; i++) e.y.push_back(K2{a.y[2 * i], a.y[2 * i + 1]});
+  for (size_t i = 0; i < a.t; i++) e.y_scalars.push_back(K2{a.y_scalars[2 * i], a.y_scalars[2 * i + 1]});
+  memcpy(e.fold_digest, a.fold_digest, 32);
+  return e;
+}
+
+static void me_out(const MeHost& e, ns_me& a) {
+  if (a.m_in != e.m_in || a.ell_n != e.r.size() || a.t != e.t) throw Error(1, "output ME arrays have other shapes");
+  if ((a.m_in && !a.X) || (a.ell_n && !a.r) || (a.t && (!a.y || !a.y_scalars))) throw Error(1, "ME arrays missing");
+  memcpy(a.c, e.c, sizeof(a.c));
+  std::copy(e.X.begin(), e.X.end(), a.X);
+  for (size_t i = 0; i < e.r.size(); i++) a.r[2 * i] = e.r[i].c0, a.r[2 * i + 1] = e.r[i].c1;
+  for (size_t i = 0; i < e.y.size(); i++) a.y[2 * i] = e.y[i].c0, a.y[2 * i + 1] = e.y[i].c1;
+  for (size_t i = 0; i < e.t; i++) a.y_scalars[2 * i] = e.y_scalars[i].c0, a.y_scalars[2 * i + 1] = e.y_scalars[i].c1;
+  memcpy(a.fold_digest, e.fold_digest, 32);
+}
+
+int ns_rlc(ns_prover* p, const ns_me* ins, size_t k, const int8_t* rho_coeffs, const int8_t* const* Z_dev,
+           ns_me* parent, int32_t* zmix_dev) {
+  return guard([&] {
+    require_device();
+    if (!p || !ins || !k || !Z_dev || !parent || !zmix_dev || (k > 1 && !rho_coeffs)) throw Error(1, "null argument");
+    if (k > 64) throw Error(1, "at most 64 RLC inputs");
+    std::vector<MeHost> v;
+    for (size_t i = 0; i < k; i++) v.push_back(me_in(ins[i]));
+    MeHost par;
+    p->p->rlc_api(v, rho_coeffs, Z_dev, par, zmix_dev);
+    me_out(par, *parent);
+  });
+}
+
+int ns_dec(ns_prover* p, const ns_me* parent, const int32_t* zmix_dev, size_t k_dec, int8_t* const* planes_dev,
+           size_t bus_chunk, ns_me* children, int* ok) {
+  return guard([&] {
+    require_device();
+    if (!p || !parent || !zmix_dev || !planes_dev || !children || !ok) throw Error(1, "null argument");
+    std::vector<MeHost> ch;
+    bool okb[3];
+    p->p->dec_api(me_in(*parent), zmix_dev, k_dec, planes_dev, bus_chunk, ch, okb);
+    for (size_t i = 0; i < k_dec; i++) me_out(ch[i], children[i]);
+    for (int i = 0; i < 3; i++) ok[i] = okb[i] ? 1 : 0;
+  });
+}
+
 int ns_prover_set_timing(ns_prover* p, int on) {
   return guard([&] { p->p->timing = on != 0; });
 }

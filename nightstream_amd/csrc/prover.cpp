@@ -1314,6 +1314,187 @@ void Prover::rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const st
   out.planes = pptr;
 }
 
+// ---- lane-level Pi_RLC / Pi_DEC (rlc_with_commit, dec_children_with_commit; api.rs:131-324) ----
+// parent = sum_i rot(a_i) * ins[i] (every y row, so bus openings mix with the core rows), commitment
+// mixed as mix_rhos_commits (session.rs:311-319); Z_mix = sum_i rot(a_i) Z_i into zmix_dev.
+// One input is the reference's fast path: the parent is the input (y_scalars recomputed).
+void Prover::rlc_api(const std::vector<MeHost>& ins, const int8_t* rho_coeffs, const int8_t* const* Z_dev,
+                     MeHost& parent, int32_t* zmix_dev) {
+  const size_t kt = ins.size(), m = s_->m;
+  if (kt == 0) throw Error(1, "RLC needs at least one input");
+  const size_t t = ins[0].t, m_in = ins[0].m_in;
+  for (auto& e : ins)
+    if (e.t != t || e.m_in != m_in || e.r.size() != ins[0].r.size())
+      throw Error(1, "RLC inputs must share t, m_in and the point r");
+  std::vector<int8_t> rho_i8(kt * 64 * 64, 0);
+  std::vector<int8_t> a(kt * D, 0);
+  for (size_t i = 0; i < kt; i++) {
+    for (int q = 0; q < D; q++) {
+      a[i * D + q] = kt == 1 ? (q == 0 ? 1 : 0) : rho_coeffs[i * D + q];
+      if (a[i * D + q] < -2 || a[i * D + q] > 2) throw Error(1, "rotation coefficients must lie in [-2, 2]");
+    }
+    // rot(a_i): column j = X^j * a_i mod Phi_81 (small integers: |entry| <= 4)
+    int64_t col[D], nxt[D];
+    for (int r = 0; r < D; r++) col[r] = a[i * D + r];
+    for (int j = 0; j < D; j++) {
+      for (int r = 0; r < D; r++) rho_i8[(i * 64 + r) * 64 + j] = (int8_t)col[r];
+      const int64_t last = col[D - 1];
+      nxt[0] = -last;
+      for (int r = 1; r < D; r++) nxt[r] = col[r - 1];
+      nxt[27] -= last;
+      memcpy(col, nxt, sizeof(col));
+    }
+  }
+  std::vector<K2> bpow(D);
+  bpow[0] = kone();
+  for (int i = 1; i < D; i++) bpow[i] = kadd(bpow[i - 1], bpow[i - 1]);
+  parent = MeHost{};
+  parent.m_in = m_in;
+  parent.t = t;
+  parent.r = ins[0].r;
+  memcpy(parent.fold_digest, ins[0].fold_digest, 32);
+  parent.X.assign((size_t)D * m_in, 0);
+  parent.y.assign(t * DPAD, kzero());
+  parent.y_scalars.assign(t, kzero());
+  if (kt == 1) {
+    memcpy(parent.c, ins[0].c, sizeof(parent.c));
+    parent.X = ins[0].X;
+    parent.y = ins[0].y;
+  } else {
+    std::vector<const uint64_t*> vp(kt);
+    std::vector<uint64_t> vbuf(kt * D), outc(D);
+    // X: rot(a_i) applied to every column of X_i (a column is a ring element)
+    for (size_t c = 0; c < m_in; c++) {
+      for (size_t i = 0; i < kt; i++) {
+        for (int q = 0; q < D; q++) vbuf[i * D + q] = ins[i].X[q * m_in + c];
+        vp[i] = &vbuf[i * D];
+      }
+      mix_small_coeffs(a.data(), vp.data(), kt, outc.data());
+      for (int q = 0; q < D; q++) parent.X[q * m_in + c] = outc[q];
+    }
+    for (size_t j = 0; j < t; j++)
+      for (int comp = 0; comp < 2; comp++) {
+        for (size_t i = 0; i < kt; i++) {
+          const K2* y = &ins[i].y[j * DPAD];
+          for (int v = 0; v < D; v++) vbuf[i * D + v] = comp ? y[v].c1 : y[v].c0;
+          vp[i] = &vbuf[i * D];
+        }
+        mix_small_coeffs(a.data(), vp.data(), kt, outc.data());
+        for (int q = 0; q < D; q++) (comp ? parent.y[j * DPAD + q].c1 : parent.y[j * DPAD + q].c0) = outc[q];
+      }
+    for (int col = 0; col < KAPPA; col++) {
+      for (size_t i = 0; i < kt; i++) vp[i] = &ins[i].c[col * D];
+      mix_small_coeffs(a.data(), vp.data(), kt, &parent.c[col * D]);
+    }
+  }
+  for (size_t j = 0; j < t; j++) {
+    K2 sc = kzero();
+    for (int rho = 0; rho < D; rho++) sc = kadd(sc, kmul(parent.y[j * DPAD + rho], bpow[rho]));
+    parent.y_scalars[j] = sc;
+  }
+  rho_dev_.ensure(rho_i8.size());
+  zptr_.ensure(kt * sizeof(void*));
+  h2d(rho_dev_.p, rho_i8.data(), rho_i8.size(), stream_);
+  h2d(zptr_.p, (const void* const*)Z_dev, kt, stream_);
+  launch_rlc_zmix(rho_dev_.as<int8_t>(), zptr_.as<const int8_t*>(), kt, m, zmix_dev, stream_);
+  NS_CHECK(hipStreamSynchronize(stream_));
+}
+
+// children of a parent: balanced base-2 split of Z_mix into planes_dev, child commitments
+// (zero for an all-zero plane), child X and y (core rows from v_j = M_j^T chi_r; the rows past
+// the CCS's t are bus openings of each plane when bus_chunk > 0), and the public checks.
+void Prover::dec_api(const MeHost& parent, const int32_t* zmix_dev, size_t k_dec, int8_t* const* planes_dev,
+                     size_t bus_chunk, std::vector<MeHost>& children, bool ok[3]) {
+  const Ccs& s = *s_;
+  const size_t m = s.m, t = s.t, m_in = parent.m_in;
+  if (k_dec == 0 || k_dec > 16) throw Error(1, "k_dec must be 1..16");
+  if (parent.t < t) throw Error(1, "parent has fewer y rows than the CCS has matrices");
+  BusLayout L;
+  L.m = m;
+  L.m_in = m_in;
+  L.bus_cols = parent.t - t;
+  if (L.bus_cols) {
+    if (!bus_chunk || L.bus_cols * bus_chunk > m) throw Error(1, "bus rows need a chunk size that fits the witness");
+    L.chunk = bus_chunk;
+    L.bus_base = m - L.bus_cols * bus_chunk;
+  }
+  DecPlanes dec_pl{};
+  std::vector<const int8_t*> pptr(k_dec);
+  for (size_t i = 0; i < k_dec; i++) {
+    dec_pl.p[i] = planes_dev[i];
+    pptr[i] = planes_dev[i];
+  }
+  flags_.ensure((k_dec + 1) * 4);
+  launch_dec_split(zmix_dev, (size_t)D * m, (int)k_dec, dec_pl, flags_.as<int>(), stream_);
+  std::vector<int> fl(k_dec + 1);
+  d2h(ows_.stage, fl.data(), flags_.p, k_dec + 1, stream_);
+  if (fl[k_dec]) throw Error(6, "DEC split out of range");
+  std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
+  commit_device(pptr.data(), k_dec, child_c.data());
+  vjs_at(parent.r, vjs_val_);
+  std::vector<K2> yc = y_products(pptr, vjs_val_);
+  std::vector<K2> bpow(D);
+  bpow[0] = kone();
+  for (int i = 1; i < D; i++) bpow[i] = kadd(bpow[i - 1], bpow[i - 1]);
+  children.assign(k_dec, MeHost{});
+  {
+    std::vector<std::vector<uint64_t>*> xs;
+    for (auto& c : children) xs.push_back(&c.X);
+    head_cols(pptr.data(), k_dec, m_in, xs);
+  }
+  for (size_t i = 0; i < k_dec; i++) {
+    MeHost& e = children[i];
+    if (fl[i]) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
+    e.m_in = m_in;
+    e.t = parent.t;
+    e.r = parent.r;
+    e.y.assign(e.t * DPAD, kzero());
+    e.y_scalars.assign(e.t, kzero());
+    for (size_t j = 0; j < t; j++) {
+      K2 sc = kzero();
+      for (int rho = 0; rho < D; rho++) {
+        const K2 v = yc[(i * t + j) * D + rho];
+        e.y[j * DPAD + rho] = v;
+        sc = kadd(sc, kmul(v, bpow[rho]));
+      }
+      e.y_scalars[j] = sc;
+    }
+    memcpy(e.fold_digest, parent.fold_digest, 32);
+  }
+  if (L.bus_cols) {
+    bus_weights(parent.r, L);
+    std::vector<MeHost*> outs;
+    for (auto& c : children) outs.push_back(&c);
+    open_bus_planes(pptr, L, outs, t);
+  }
+  ok[0] = ok[1] = ok[2] = true;
+  for (size_t q = 0; q < t * DPAD && ok[0]; q++) {
+    K2 lhs = kzero();
+    uint64_t pw = 1;
+    for (size_t i = 0; i < k_dec; i++) {
+      lhs = kadd(lhs, kscale(children[i].y[q], pw));
+      pw = fadd(pw, pw);
+    }
+    ok[0] = keq(lhs, parent.y[q]);
+  }
+  for (size_t q = 0; q < (size_t)D * m_in; q++) {
+    uint64_t lhs = 0, pw = 1;
+    for (size_t i = 0; i < k_dec; i++) {
+      lhs = fadd(lhs, fmul(children[i].X[q], pw));
+      pw = fadd(pw, pw);
+    }
+    if (lhs != parent.X[q]) ok[1] = false;
+  }
+  for (int q = 0; q < KAPPA * D; q++) {
+    uint64_t lhs = 0, pw = 1;
+    for (size_t i = 0; i < k_dec; i++) {
+      lhs = fadd(lhs, fmul(children[i].c[q], pw));
+      pw = fadd(pw, pw);
+    }
+    if (lhs != parent.c[q]) ok[2] = false;
+  }
+}
+
 static void enc_rounds(Enc& pb, const std::vector<K2>& polys, size_t rounds, size_t np) {
   pb.u(rounds);
   for (size_t rd = 0; rd < rounds; rd++) {

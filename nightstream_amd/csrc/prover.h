@@ -196,6 +196,11 @@ class Prover {
   hipStream_t stream() const { return stream_; }
   const Ccs& ccs() const { return *s_; }
   const Pp& pp() const { return *pp_; }
+  // lane-level Pi_RLC / Pi_DEC on this context's stream and buffers (the C ABI's ns_rlc / ns_dec)
+  void rlc_api(const std::vector<MeHost>& ins, const int8_t* rho_coeffs, const int8_t* const* Z_dev, MeHost& parent,
+               int32_t* zmix_dev);
+  void dec_api(const MeHost& parent, const int32_t* zmix_dev, size_t k_dec, int8_t* const* planes_dev,
+               size_t bus_chunk, std::vector<MeHost>& children, bool ok[3]);
   // Reuse this prover's device buffers (grown on demand) for another CCS / PP pair.
   void rebind(const Ccs& s, const Pp& pp);
   DevBuf& ws_ajtai() { return ws_ajtai_; }
