@@ -19,6 +19,7 @@
 #include <map>
 #include <mutex>
 
+#include "tspool.h"
 #include "twist.h"
 
 namespace ns {
@@ -786,126 +787,31 @@ static unsigned nblocks(size_t n, size_t cap = TMAXBLK) {
 
 // ---------------- block / stream / mailbox cache (one device per process) ----------------
 namespace {
-class TsPool {
- public:
-  static TsPool& get() {
-    static TsPool* pool = new TsPool;  // never destroyed: no HIP calls during process exit
-    return *pool;
+struct HipPoolBackend {
+  using Stream = hipStream_t;
+  static bool dev_alloc(void** p, size_t n) { return hipMalloc(p, n) == hipSuccess; }
+  static void alloc_failed(size_t n) {
+    (void)hipGetLastError();
+    throw Error(1, "out of device memory for a " + std::to_string(n) + "-byte oracle block");
   }
-  static size_t size_class(size_t n) {
-    size_t c = 256;
-    while (c < n) c <<= 1;
-    return c;
-  }
-  void* take(size_t cap) {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      auto it = blocks_.find(cap);
-      if (it != blocks_.end()) {
-        void* p = it->second;
-        blocks_.erase(it);
-        cached_ -= cap;
-        return p;
-      }
-    }
-    void* p = nullptr;
-    if (hipMalloc(&p, cap) != hipSuccess) {
-      (void)hipGetLastError();
-      flush();
-      NS_CHECK(hipMalloc(&p, cap));
-    }
-    return p;
-  }
-  void give(void* p, size_t cap) {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      if (cached_ + cap <= kMaxCached) {
-        blocks_.emplace(cap, p);
-        cached_ += cap;
-        return;
-      }
-    }
-    (void)hipFree(p);
-  }
-  void flush() {
-    std::multimap<size_t, void*> all;
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      all.swap(blocks_);
-      cached_ = 0;
-    }
-    for (auto& b : all) (void)hipFree(b.second);
-  }
-  hipStream_t stream() {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      if (!streams_.empty()) {
-        hipStream_t s = streams_.back();
-        streams_.pop_back();
-        return s;
-      }
-    }
+  static void dev_free(void* p) { (void)hipFree(p); }
+  static void host_alloc(void** p, size_t n) { NS_CHECK(hipHostMalloc(p, n, hipHostMallocDefault)); }
+  static void host_free(void* p) { (void)hipHostFree(p); }
+  static hipStream_t stream_create() {
     hipStream_t s = nullptr;
     NS_CHECK(hipStreamCreate(&s));
     return s;
   }
-  void put_stream(hipStream_t s) {
-    std::lock_guard<std::mutex> g(mu_);
-    streams_.push_back(s);
+  static void clear_error() { (void)hipGetLastError(); }
+};
+struct TsPool {
+  using Pool = BlockPool<HipPoolBackend, K2>;
+  // never destroyed: no HIP calls during process exit
+  static Pool& get() {
+    static Pool* pool = new Pool(TS_MAX_COEF + 1, (size_t)4 << 30, (size_t)1 << 30);
+    return *pool;
   }
-  K2* mailbox() {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      if (!mboxes_.empty()) {
-        K2* m = mboxes_.back();
-        mboxes_.pop_back();
-        return m;
-      }
-    }
-    void* p = nullptr;
-    NS_CHECK(hipHostMalloc(&p, (TS_MAX_COEF + 1) * sizeof(K2), hipHostMallocDefault));
-    return (K2*)p;
-  }
-  void put_mailbox(K2* m) {
-    std::lock_guard<std::mutex> g(mu_);
-    mboxes_.push_back(m);
-  }
-  // pinned host blocks for the constructors' uploads (size classes as for device blocks)
-  void* pinned_take(size_t cap) {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      auto it = pinned_.find(cap);
-      if (it != pinned_.end()) {
-        void* p = it->second;
-        pinned_.erase(it);
-        pinned_cached_ -= cap;
-        return p;
-      }
-    }
-    void* p = nullptr;
-    NS_CHECK(hipHostMalloc(&p, cap, hipHostMallocDefault));
-    return p;
-  }
-  void pinned_give(void* p, size_t cap) {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      if (pinned_cached_ + cap <= kMaxPinned) {
-        pinned_.emplace(cap, p);
-        pinned_cached_ += cap;
-        return;
-      }
-    }
-    (void)hipHostFree(p);
-  }
-
- private:
-  static constexpr size_t kMaxCached = (size_t)4 << 30;
-  static constexpr size_t kMaxPinned = (size_t)1 << 30;
-  std::mutex mu_;
-  std::multimap<size_t, void*> blocks_, pinned_;
-  size_t cached_ = 0, pinned_cached_ = 0;
-  std::vector<hipStream_t> streams_;
-  std::vector<K2*> mboxes_;
+  static size_t size_class(size_t n) { return Pool::size_class(n); }
 };
 }  // namespace
 
