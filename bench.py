@@ -152,9 +152,12 @@ def main():
     ap.add_argument("--ajtai-max-log", type=int, default=20, help="ajtai: largest m = 2^k of the sweep")
     ap.add_argument("--shards", type=int, default=128, help="batch: number of shards")
     ap.add_argument("--batch-memory", type=int, default=1, choices=[0, 1],
-                    help="batch: every other shard carries the Twist/Shout memory sidecar (1) or none does (0)")
+                    help="batch: every other shard is a Twist/Shout shard (1) or none is (0)")
     ap.add_argument("--serve-memory-frac", type=float, default=0.3,
-                    help="serve: fraction of jobs carrying the Twist/Shout memory sidecar")
+                    help="serve: fraction of jobs that are Twist/Shout shards")
+    ap.add_argument("--mix", type=int, default=1, choices=[0, 1],
+                    help="shard: also run a short in-process batch (16 shards, plain vs pooled contexts) and a "
+                         "short Poisson serve over the same shards, reported under 'mix'")
     ap.add_argument("--rate-duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--rates", type=str, default="", help="serve: comma-separated arrival rates (jobs/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -283,36 +286,42 @@ def main():
         "host_phases_s": phases,
         "host_phase_perms": perms,
     }
+    if args.mix:
+        del sess, z_dev
+        line["mix"] = mix_measurement(args.workers, args.seed)
     print(json.dumps(line), flush=True)
     if parity is not None and not parity["bit_identical"]:
         sys.exit("GPU proof differs from the CPU restatement's on the sampled steps")
+    if args.mix and not (line["mix"]["proofs_bit_identical"] and line["mix"]["serve"]["served_equal_plain"]):
+        sys.exit("pooled or served proofs differ from the plain path")
 
 
 def _stderr(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def bench_batch(args):
-    """128 independent shards (rows 2^10..2^16 log-uniform, 4-16 fold steps; every other one
-    carries the memory sidecar of a RISC-V-like trace of its chunk size, proved at every step):
-    the plain path (one shard at a time in one prover context, the sidecar's oracles one at a
-    time) against --workers concurrent contexts with grouped sidecar rounds, spread over --procs
-    prover processes; every proof of the batched run must equal the plain run's."""
+def batch_specs(rng, shards: int, lo: int, hi: int, smin: int, smax: int, memory: bool, variants: int = 4):
+    """(n, steps, seed, memory) specs: rows 2^lo..2^hi log-uniform, smin..smax fold steps, up to
+    `variants` distinct instances per size; with `memory`, every other shard is a Twist/Shout
+    shard (a RISC-V-like trace of n / 128 cycles per step)."""
+    shape = [(int(rng.integers(lo, hi + 1)), int(rng.integers(smin, smax + 1))) for _ in range(shards)]
+    return [(1 << e, steps, 1000 + e * 16 + (i % variants), bool(memory and i % 2 == 0))
+            for i, (e, steps) in enumerate(shape)]
+
+
+def run_batch(specs, workers: int, procs: int = 1, hw_queues: int = 1):
+    """The plain path (one shard at a time in one prover context) against `workers` concurrent
+    contexts (in `procs` prover processes); both keep contexts warmed before their timed run.
+    Returns (record, shards, proofs)."""
     import nightstream_amd as ns
     from nightstream_amd.batch import ProcessProver, ProverPool, ShardFactory, build_shards, prove_batched, \
         prove_sequential
 
-    rng = np.random.default_rng(args.seed)
-    shape = [(int(rng.integers(10, 17)), int(rng.integers(4, 17))) for _ in range(args.shards)]
-    # up to 4 distinct R1CS instances per size; every other shard carries a memory trace
-    specs = [(1 << e, steps, 1000 + e * 16 + (i % 4), bool(args.batch_memory and i % 2 == 0))
-             for i, (e, steps) in enumerate(shape)]
     order = sorted(range(len(specs)), key=lambda i: -specs[i][0] * specs[i][1])
-    procs = max(1, args.procs)
     t0 = time.perf_counter()
     pp = None
     if procs > 1:  # started before this process touches the GPU; each builds and warms its own
-        pp = ProcessProver(PP_SEED, specs, procs, max(1, args.workers // procs), args.proc_hw_queues)
+        pp = ProcessProver(PP_SEED, specs, procs, max(1, workers // procs), hw_queues)
     t_procs = time.perf_counter() - t0
     t0 = time.perf_counter()
     shards = build_shards(ShardFactory(PP_SEED), specs)
@@ -320,56 +329,88 @@ def bench_batch(args):
     setup = time.perf_counter() - t0
     total_steps = sum(s.steps for s in shards)
     mem_steps = sum(s.steps for s in shards if s.mem is not None)
-    _stderr(f"batch: {len(shards)} shards, {total_steps} fold steps ({mem_steps} with memory), "
+    _stderr(f"batch: {len(shards)} shards, {total_steps} fold steps ({mem_steps} with Twist/Shout memory), "
             f"setup {setup:.1f} s, {procs} prover processes ready in {t_procs:.1f} s")
-    # both paths keep their contexts (as a running prover does): created and sized for the
-    # largest shard before the timed region
-    pool1 = ProverPool(1, batched_rounds=False)
+    pool1 = ProverPool(1)
     t0 = time.perf_counter()
     pool1.warm(shards)
     pooln = None
     if pp is None:
-        pooln = ProverPool(args.workers)
+        pooln = ProverPool(workers)
         pooln.warm(shards)
     t_warm = time.perf_counter() - t0
     seq, t_seq = prove_sequential(shards, pool1)
     _stderr(f"batch: plain path {t_seq:.2f} s")
     if pp is None:
-        bat, t_bat = prove_batched(shards, args.workers, pooln)
-        created = pooln.created - args.workers
+        bat, t_bat = prove_batched(shards, workers, pooln)
+        created = pooln.created - workers
     else:
         bat, t_bat = pp.prove_all(order)
         created = 0
         pp.close()
-    _stderr(f"batch: {args.workers} contexts in {procs} processes {t_bat:.2f} s")
-    identical = all(a == b for a, b in zip(seq, bat))
-    line = {
-        "metric": "fold-steps/s, 128-shard batch (2^10..2^16 rows, 4-16 steps, half with Twist/Shout)",
-        "value": round(total_steps / t_bat, 3), "unit": "fold-steps/s", "n_gpus": 1,
-        "higher_is_better": True, "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "batch", "shards": len(shards), "fold_steps": total_steps,
-                   "fold_steps_with_memory_sidecar": mem_steps, "workers": args.workers, "processes": procs,
-                   "hw_queues_per_process": args.proc_hw_queues if procs > 1 else None,
-                   "sizes_log2": [e for e, _ in shape], "steps": [st for _, st in shape]},
-        "plain_path_fold_steps_per_s": round(total_steps / t_seq, 3),
+    _stderr(f"batch: {workers} contexts in {procs} processes {t_bat:.2f} s")
+    rec = {
+        "shards": len(shards), "fold_steps": total_steps, "fold_steps_with_memory": mem_steps,
+        "workers": workers, "processes": procs,
+        "plain_fold_steps_per_s": round(total_steps / t_seq, 3),
+        "batched_fold_steps_per_s": round(total_steps / t_bat, 3),
         "speedup_vs_plain": round(t_seq / t_bat, 3),
         "seconds": {"plain": round(t_seq, 3), "batched": round(t_bat, 3), "setup": round(setup, 1),
                     "context_warmup": round(t_warm, 2), "process_start": round(t_procs, 1)},
         "contexts_created_in_timed_runs": pool1.created - 1 + created,
-        "proofs_bit_identical": identical,
+        "proofs_bit_identical": all(a == b for a, b in zip(seq, bat)),
+    }
+    return rec, shards, seq, (pooln or pool1)
+
+
+def bench_batch(args):
+    """128 independent shards (rows 2^10..2^16 log-uniform, 4-16 fold steps, a distinct witness
+    per step; every other one a Twist/Shout shard whose steps carry Route A memory over the shared
+    CPU bus): the plain path (one shard at a time in one prover context) against --workers
+    concurrent contexts spread over --procs prover processes; every proof of the batched run must
+    equal the plain run's."""
+    rng = np.random.default_rng(args.seed)
+    specs = batch_specs(rng, args.shards, 10, 16, 4, 16, bool(args.batch_memory))
+    rec, _, _, _ = run_batch(specs, args.workers, max(1, args.procs), args.proc_hw_queues)
+    line = {
+        "metric": "fold-steps/s, 128-shard batch (2^10..2^16 rows, 4-16 steps, half with Twist/Shout memory)",
+        "value": rec["batched_fold_steps_per_s"], "unit": "fold-steps/s", "n_gpus": 1,
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "batch", "shards": rec["shards"], "fold_steps": rec["fold_steps"],
+                   "fold_steps_with_memory": rec["fold_steps_with_memory"], "workers": args.workers,
+                   "processes": rec["processes"],
+                   "hw_queues_per_process": args.proc_hw_queues if rec["processes"] > 1 else None,
+                   "specs": [[n.bit_length() - 1, st, int(m)] for n, st, _, m in specs]},
+        **{k: v for k, v in rec.items() if k not in ("shards", "fold_steps", "fold_steps_with_memory", "workers",
+                                                     "processes")},
     }
     print(json.dumps(line), flush=True)
-    if not identical:
+    if not rec["proofs_bit_identical"]:
         sys.exit("batched proofs differ from the plain path")
 
 
+def serve_rate(jobs, rate, duration, workers, pool, pp_res, seed):
+    """One open-loop Poisson run; returns its record and results."""
+    from nightstream_amd.batch import serve
+    res = serve(jobs, rate, duration, workers, 240e9, seed=seed, pp_resident=pp_res, pool=pool)
+    if not res:
+        return None, res
+    lat = np.array([r.latency for r in res])
+    span = max(r.finish for r in res) - min(r.arrival for r in res)
+    steps = sum(r.steps for r in res)
+    return {"jobs": len(res), "fold_steps_per_s": round(steps / span, 3),
+            "p50_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_s": round(float(np.percentile(lat, 99)), 3)}, res
+
+
 def bench_serve(args):
-    """Open-loop Poisson arrivals; jobs of 2^12..2^18 rows (log-uniform) with 1-32 fold steps;
-    admission by free worker and by estimated HBM footprint under a 240 GB cap.  For each rate:
-    fold-steps/s and p50/p99 latency, for the pooled server and for the plain single-context
-    server; the sweep stops once the pooled p99 exceeds 5 s."""
+    """Open-loop Poisson arrivals; jobs of 2^12..2^18 rows (log-uniform) with 1-32 fold steps
+    (a distinct witness per step), --serve-memory-frac of them Twist/Shout shards; admission by
+    free worker and by estimated HBM footprint under a 240 GB cap.  For each rate: fold-steps/s
+    and p50/p99 latency, for the pooled server and for the plain single-context server; the sweep
+    stops once the pooled p99 exceeds 5 s."""
     import nightstream_amd as ns
-    from nightstream_amd.batch import Shard, ShardFactory, pp_bytes, serve
+    from nightstream_amd.batch import ProverPool, ShardFactory, pp_bytes
 
     rng = np.random.default_rng(args.seed)
     fac = ShardFactory(PP_SEED)
@@ -377,25 +418,23 @@ def bench_serve(args):
     base = {}
     for e in range(12, 19):
         for v in range(2):
-            base[(e, v)] = fac.make(1 << e, 1, 2000 + e * 4 + v)
+            base[(e, v)] = fac.make(1 << e, 32, 2000 + e * 4 + v)
         if args.serve_memory_frac > 0:
-            base[(e, "mem")] = fac.make(1 << e, 1, 2000 + e * 4 + 2, memory=True)
+            base[(e, "mem")] = fac.make(1 << e, 32, 2000 + e * 4 + 2, memory=True)
     ns.device_synchronize()
     jobs = []
     for i in range(4096):
         e = int(rng.integers(12, 19))
         b = base[(e, "mem")] if rng.random() < args.serve_memory_frac else base[(e, i % 2)]
-        jobs.append(Shard(b.n, int(rng.integers(1, 33)), b.seed, b.ccs, b.pp, b.z_dev, b.mem, b.mem_ell))
+        jobs.append(b.with_steps(int(rng.integers(1, 33))))
     pp_res = float(sum(pp_bytes(1 << e) for e in range(12, 19)))
     _stderr(f"serve: pool of {len(base)} instances ready in {time.perf_counter() - t0:.1f} s")
-    # reference proofs (alone) for the first jobs, to compare against the mixed runs
     solo = {}
     rates = [float(r) for r in args.rates.split(",")] if args.rates else [5.0, 10.0, 20.0, 40.0, 80.0]
     out = []
     # each server keeps its contexts across rates (as a running server does), created and sized
     # for the largest job before the first rate
-    from nightstream_amd.batch import ProverPool
-    pools = {"pooled": ProverPool(args.workers), "plain": ProverPool(1, batched_rounds=False)}
+    pools = {"pooled": ProverPool(args.workers), "plain": ProverPool(1)}
     for p in pools.values():
         p.warm(jobs)
     plain_saturated = False  # once the single-context server's p99 passes 10 s its queue only grows
@@ -405,42 +444,54 @@ def bench_serve(args):
             if label == "plain" and plain_saturated:
                 rec["plain"] = "saturated at a lower rate"
                 continue
-            res = serve(jobs, rate, args.rate_duration, workers, 240e9, seed=int(rate * 1000), pp_resident=pp_res,
-                        pool=pools[label])
-            if not res:
+            r, res = serve_rate(jobs, rate, args.rate_duration, workers, pools[label], pp_res, int(rate * 1000))
+            if r is None:
                 continue
-            lat = np.array([r.latency for r in res])
-            span = max(r.finish for r in res) - min(r.arrival for r in res)
-            steps = sum(r.steps for r in res)
-            rec[label] = {"jobs": len(res), "fold_steps_per_s": round(steps / span, 3),
-                          "p50_s": round(float(np.percentile(lat, 50)), 3),
-                          "p99_s": round(float(np.percentile(lat, 99)), 3)}
+            rec[label] = r
             if label == "pooled":
-                for r in res[:8]:
-                    solo.setdefault(r.job, r.proof)
-            _stderr(f"serve: rate {rate} {label}: {rec[label]}")
-            if label == "plain" and rec[label]["p99_s"] > 10.0:
+                for x in res[:8]:
+                    solo.setdefault(x.job, x.proof)
+            _stderr(f"serve: rate {rate} {label}: {r}")
+            if label == "plain" and r["p99_s"] > 10.0:
                 plain_saturated = True
-            if label == "pooled" and rec[label]["p99_s"] > 5.0:
+            if label == "pooled" and r["p99_s"] > 5.0:
                 break
         out.append(rec)
         if rec.get("pooled", {}).get("p99_s", 0) > 5.0:
             break
     # each mixed result equals the same job proved alone
-    from nightstream_amd.batch import ProverPool
     alone = ProverPool(1)
     same = all(alone.prove(jobs[j]) == p for j, p in solo.items())
     line = {
         "metric": "served fold-steps/s and p50/p99 job latency (Poisson arrivals, 2^12..2^18 rows, 1-32 steps, "
-                  f"{args.serve_memory_frac:.0%} with Twist/Shout)",
+                  f"{args.serve_memory_frac:.0%} with Twist/Shout memory)",
         "value": max((r["pooled"]["fold_steps_per_s"] for r in out if "pooled" in r), default=None),
         "unit": "fold-steps/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": "serve", "workers": args.workers, "rate_duration_s": args.rate_duration,
-                   "hbm_cap_bytes": 240e9, "memory_sidecar_frac": args.serve_memory_frac},
+                   "hbm_cap_bytes": 240e9, "memory_frac": args.serve_memory_frac},
         "rates": out, "mixed_equals_alone": same,
     }
     print(json.dumps(line), flush=True)
+
+
+def mix_measurement(workers: int, seed: int):
+    """Short in-process evidence for the batch and serve claims beside the shard line: 16 shards
+    (2^10..2^13 rows, 3-8 fold steps, every other one a Twist/Shout shard) proved on the plain
+    path and through `workers` pooled contexts (proofs compared), then ~8 s of Poisson arrivals
+    over the same shards at about 60% of the pooled throughput, with p50/p99 job latency."""
+    from nightstream_amd.batch import pp_bytes
+    rng = np.random.default_rng(seed)
+    specs = batch_specs(rng, 16, 10, 13, 3, 8, True, variants=2)
+    rec, shards, seq, pool = run_batch(specs, workers)
+    mean_steps = rec["fold_steps"] / rec["shards"]
+    rate = round(0.6 * rec["batched_fold_steps_per_s"] / mean_steps, 2)
+    pp_res = float(sum(pp_bytes(1 << e) for e in range(10, 14)))
+    srv, res = serve_rate(shards, rate, 8.0, workers, pool, pp_res, seed)
+    same = all(r.proof == seq[r.job % len(shards)] for r in res)
+    rec["serve"] = {"rate_jobs_per_s": rate, "duration_s": 8.0, **(srv or {}), "served_equal_plain": same}
+    rec["specs"] = [[n.bit_length() - 1, st, int(m)] for n, st, _, m in specs]
+    return rec
 
 
 Q_FIELD = 0xFFFFFFFF00000001

@@ -336,6 +336,18 @@ class _StepMem(C.Structure):
     _fields_ = [("n_lut", C.c_size_t), ("luts", C.c_void_p), ("n_mem", C.c_size_t), ("mems", C.c_void_p)]
 
 
+class StepMemArray:
+    """Per-step memory instances converted once to an ns_step_mem array (reusable across proofs)."""
+
+    def __init__(self, mems):
+        self.keep = []
+        self.arr = _step_mem_array(mems, self.keep)
+        self.n = len(mems)
+
+    def __len__(self):
+        return self.n
+
+
 def _step_mem_array(mems, keep: list):
     """ns_step_mem[] for per-step memory instances (objects with .luts / .mems whose items carry
     k, d, n_side, steps, ell and .table (LUT) or .init (MEM: None = MemInit::Zero, else sorted
@@ -506,16 +518,18 @@ class FoldingSession:
 
     def prove_device(self, z_ptrs, m_in: int = 1, mem=None) -> bytes:
         """fold_and_prove over step witnesses already resident in HBM (device pointers to u64[m]);
-        mem: per-step memory instances (Route A) or None."""
+        mem: per-step memory instances (Route A; a list or a StepMemArray covering at least the
+        proved steps) or None."""
         arr = (C.c_void_p * len(z_ptrs))(*z_ptrs)
         out = C.POINTER(C.c_uint8)()
         n = C.c_size_t()
         sec = C.c_double()
         if mem is not None:
-            keep = []
-            marr = _step_mem_array(mem, keep)
-            _check(lib().ns_prove_shard_mem(self.h, arr, 1, len(z_ptrs), m_in, C.cast(marr, C.c_void_p), C.byref(out),
-                                            C.byref(n), C.byref(sec)))
+            sm = mem if isinstance(mem, StepMemArray) else StepMemArray(mem)
+            if len(sm) < len(z_ptrs):
+                raise ValueError("fewer memory steps than witnesses")
+            _check(lib().ns_prove_shard_mem(self.h, arr, 1, len(z_ptrs), m_in, C.cast(sm.arr, C.c_void_p),
+                                            C.byref(out), C.byref(n), C.byref(sec)))
         else:
             _check(lib().ns_prove_shard_device(self.h, arr, len(z_ptrs), m_in, C.byref(out), C.byref(n),
                                                C.byref(sec)))

@@ -2,7 +2,8 @@
 
 A shard's fold steps are sequential (each step's accumulator feeds the next), and each step's
 Fiat-Shamir transcript is host work of roughly constant size (about 17.8k Poseidon2
-permutations, independent of n).  Throughput across shards therefore comes from running many
+permutations for a CCS-only step, independent of n; a Twist/Shout step binds every ME's 121 bus
+openings too, about 20x more).  Throughput across shards therefore comes from running many
 sessions at once: a running job holds one prover context (its own HIP stream and device buffers,
 kept in a pool and reused from job to job through `rebind`), drives its shard's transcript on its
 own core, and its kernels overlap with the other jobs' on the GPU.  ctypes releases the GIL for the
@@ -80,28 +81,35 @@ def pp_bytes(m: int) -> int:
     return 6912 * m
 
 
-MEM_MAX_ELL = 16  # memory-sidecar chunks are at most 2^16 cycles
-
-
-def mem_footprint_bytes(ell: int) -> int:
-    """HBM of one chunk's 30 Twist/Shout oracles: at most 4 KiB per cycle (columns, their folded
-    copies, the sorted address entries and the scan scratch) plus the three 2^16-entry tables."""
-    return (4096 << ell) + 3 * 16 * (1 << 16) if ell else 0
+def mem_footprint_bytes(n: int) -> int:
+    """HBM that Twist/Shout memory adds to a job of n = 128 c rows (c cycles per step): the val
+    lane's 16 digit planes and its Z_mix, the decoded bus region and the bus weights, and the
+    memory oracles (at most 4 KiB per cycle) with the three 2^16-entry lookup tables."""
+    c = n // 128
+    return int(16 * D * n + 4 * D * n + 8 * n + 16 * 121 * c + 4096 * c + 3 * 16 * (1 << 16))
 
 
 @dataclasses.dataclass
 class Shard:
+    """One shard: `steps` fold steps, each over its own witness (z = [x | w], u64[m]) already in
+    HBM; with `mem`, a Twist/Shout shard whose per-step memory instances ride the shared CPU bus
+    of its witnesses (nightstream_amd/bus_workload.py).  z_devs and mem may hold more steps than
+    `steps` (shards of one trace share them); the first `steps` are proved."""
     n: int
     steps: int
     seed: int
     ccs: object = None
     pp: object = None
-    z_dev: object = None  # DeviceBuffer holding z (u64[m])
-    mem: object = None  # memory lanes of one chunk (sidecar.chunk_lanes), proved at every step
-    mem_ell: int = 0  # log2 of the chunk's cycles
+    z_devs: list = None  # DeviceBuffer per step
+    mem: object = None  # StepMemArray of the per-step memory instances, or None
 
     def z_ptrs(self):
-        return [self.z_dev.data_ptr()] * self.steps
+        return [b.data_ptr() for b in self.z_devs[:self.steps]]
+
+    def with_steps(self, steps: int) -> "Shard":
+        if steps > len(self.z_devs):
+            raise ValueError("shard has fewer witnesses than steps")
+        return Shard(self.n, steps, self.seed, self.ccs, self.pp, self.z_devs, self.mem)
 
 
 class ShardFactory:
@@ -118,33 +126,40 @@ class ShardFactory:
             self.pps[m] = self.ns.PublicParams(m, seed=self.pp_seed)
         return self.pps[m]
 
+    @staticmethod
+    def instance(n: int, steps: int, seed: int, memory: bool = False):
+        """The host instance of a shard: (R1CS, [z per step], [StepMem per step] or None).  Without
+        memory, an n-row R1CS whose steps differ in their free witness columns; with memory, a
+        RISC-V-like trace of n / 128 cycles per step over Twist/Shout memory (n >= 2^8)."""
+        if memory:
+            from .bus_workload import gen_bus_shard
+            e = n.bit_length() - 1
+            if n != 1 << e or e < 8:
+                raise ValueError("a Twist/Shout shard needs n = 128 * 2^k >= 256 rows")
+            inst, zs, mems, _ = gen_bus_shard(e - 7, steps, seed)
+            return inst, zs, mems
+        from .workload import gen_r1cs_steps
+        inst, zs = gen_r1cs_steps(n, seed, steps)
+        return inst, zs, None
+
     def make(self, n: int, steps: int, seed: int, memory: bool = False) -> Shard:
-        """A shard of `steps` fold steps over one n-row R1CS chunk; with `memory`, every step also
-        carries the memory sidecar of a RISC-V-like trace of min(n, 2^16) cycles."""
-        inst, z = gen_r1cs_fast(n, seed)
+        inst, zs, mems = self.instance(n, steps, seed, memory)
         sh = Shard(n, steps, seed)
         sh.ccs = self.ns.Ccs(inst)
         sh.pp = self.pp(n)
-        sh.z_dev = self.ns.DeviceBuffer.from_array(z)
-        if memory:
-            from .sidecar import chunk_lanes
-            sh.mem_ell = min(n.bit_length() - 1, MEM_MAX_ELL)
-            sh.mem = chunk_lanes(seed, sh.mem_ell)
+        sh.z_devs = [self.ns.DeviceBuffer.from_array(z) for z in zs]
+        if mems is not None:
+            sh.mem = self.ns.StepMemArray(mems)
         return sh
 
 
 class ProverPool:
     """Up to `workers` prover contexts kept across jobs and calls: a job takes an idle context (or
     creates one while fewer than `workers` exist) and returns it when done, so a warmed pool proves
-    without creating HIP streams or growing device buffers (both serialize in the HIP runtime).
+    without creating HIP streams or growing device buffers (both serialize in the HIP runtime)."""
 
-    A shard with a memory trace proves its sidecar after its CCS steps: one chunk per step, with
-    its randomness bound to the CCS proof; `batched_rounds` picks the grouped round driver (one
-    batch_round / batch_fold per group round) over the plain one (one oracle at a time)."""
-
-    def __init__(self, workers: int, batched_rounds: bool = True):
+    def __init__(self, workers: int):
         self.workers = workers
-        self.batched_rounds = batched_rounds
         self._idle = []
         self._lock = threading.Lock()
         self.created = 0
@@ -163,37 +178,35 @@ class ProverPool:
     def prove(self, shard: Shard) -> bytes:
         s = self._acquire(shard)
         try:
-            proof = s.prove_device(shard.z_ptrs())
+            return s.prove_device(shard.z_ptrs(), mem=shard.mem)
         finally:
             with self._lock:
                 self._idle.append(s)
-        if shard.mem is None:
-            return proof
-        from .sidecar import prove_chunk, step_seed
-        parts = [proof]
-        for st in range(shard.steps):
-            parts.append(prove_chunk(shard.mem, shard.mem_ell, step_seed(proof, st), self.batched_rounds))
-        return b"".join(parts)
 
     def warm(self, shards):
-        """Create every context and grow its buffers for the largest shape in `shards`: the most
+        """Create every context and grow its buffers for the largest shapes in `shards`: the most
         rows and, independently, the most steps (the batched step commitment's workspace grows
-        with the step count, not with the rows), proved `workers` times at once so that each
-        proof takes its own context."""
+        with the step count, not with the rows), and the largest Twist/Shout shard; proved
+        `workers` times at once so that each proof takes its own context."""
         big = max(shards, key=lambda sh: (sh.n, sh.steps))
         steps = max(sh.steps for sh in shards)
-        warm = Shard(big.n, steps, big.seed, big.ccs, big.pp, big.z_dev)
-        sessions = [self._acquire(warm) for _ in range(self.workers - len(self._idle))]
+        by_steps = max(shards, key=lambda sh: (len(sh.z_devs), sh.n))
+        warm = [big.with_steps(min(len(big.z_devs), steps))]
+        if len(by_steps.z_devs) >= steps and by_steps is not big:
+            warm.append(by_steps.with_steps(steps))
+        mem = [sh for sh in shards if sh.mem is not None]
+        if mem:
+            bm = max(mem, key=lambda sh: (sh.n, len(sh.z_devs)))
+            warm.append(bm.with_steps(len(bm.z_devs)))
+        sessions = [self._acquire(warm[0]) for _ in range(self.workers - len(self._idle))]
         with self._lock:
             sessions += self._idle
             self._idle = []
-        mem = max((sh for sh in shards if sh.mem is not None), key=lambda sh: sh.mem_ell, default=None)
 
         def run(s):
-            s.prove_device(warm.z_ptrs())
-            if mem is not None:  # fills the oracles' block, stream and mailbox caches
-                from .sidecar import prove_chunk
-                prove_chunk(mem.mem, mem.mem_ell, 0, self.batched_rounds)
+            for w in warm:
+                s.rebind(w.ccs, w.pp)
+                s.prove_device(w.z_ptrs(), mem=w.mem)
 
         threads = [threading.Thread(target=run, args=(s,)) for s in sessions]
         for th in threads:
@@ -230,15 +243,14 @@ def prove_batched(shards, workers: int, pool: ProverPool | None = None):
 
 def build_shards(fac: ShardFactory, specs):
     """Shards from (n, steps, seed, memory) specs; shards with the same (n, seed, memory) share one
-    CCS, witness and memory trace on the device."""
-    cache, out = {}, []
+    instance (its CCS, its per-step witnesses and memory on the device), built for the most steps
+    any of them proves."""
+    most = {}
     for n, steps, seed, memory in specs:
         key = (n, seed, bool(memory))
-        if key not in cache:
-            cache[key] = fac.make(n, 1, seed, memory=bool(memory))
-        b = cache[key]
-        out.append(Shard(b.n, steps, b.seed, b.ccs, b.pp, b.z_dev, b.mem, b.mem_ell))
-    return out
+        most[key] = max(most.get(key, 0), steps)
+    cache = {key: fac.make(key[0], st, key[1], memory=key[2]) for key, st in most.items()}
+    return [cache[(n, seed, bool(memory))].with_steps(steps) for n, steps, seed, memory in specs]
 
 
 def _proc_worker(pp_seed, specs, threads, tasks, results):
@@ -395,7 +407,7 @@ def serve(jobs, rate: float, duration: float, workers: int, hbm_cap: float, seed
         if arr > now:
             time.sleep(arr - now)
         sh = jobs[i % len(jobs)]
-        fp = footprint_bytes(sh.n) + mem_footprint_bytes(sh.mem_ell if sh.mem is not None else 0)
+        fp = footprint_bytes(sh.n) + (mem_footprint_bytes(sh.n) if sh.mem is not None else 0)
         with cond:
             # FIFO admission: wait for a slot and for HBM headroom
             while state["free"] == 0 or state["mem"] + fp > hbm_cap:

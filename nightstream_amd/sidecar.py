@@ -14,10 +14,10 @@ of a group is one `batch_round` / `batch_fold` pair (every oracle's kernels enqu
 host waits once per oracle); the plain driver evaluates and folds one oracle at a time.  Both
 produce the same values.
 
-The challenges and the oracles' random points are drawn from a seeded generator: the shard
-prover's transcript does not interleave the memory lanes into its batched time sum-check
-(`neo-fold/src/shard.rs:2006-2156`), so these lanes carry the step's memory-checking work at its
-real shape without the reference's transcript order.
+This module drives the round oracles alone, at their real shape, for the oracle-level
+benchmark and tests: its challenges and random points come from a seeded generator, so its
+values bind nothing.  Proofs of shards with memory take the Route A path of the shard prover
+(`ns_prove_shard_mem`), whose memory lanes draw every challenge from the step transcript.
 """
 from __future__ import annotations
 

@@ -1,9 +1,9 @@
 """Batched and served proving: every proof is the same bytes alone and in the mix.
 
-Six shards of 2^10 / 2^11 rows, two of them with the Twist/Shout memory sidecar, are proved one
-at a time in one prover context (sidecar oracles one at a time), then through three concurrent
-contexts (grouped sidecar rounds), then through the Poisson server; all three must agree, and
-the CCS proof and a sidecar chunk must equal the CPU restatement's."""
+Six shards of 2^10 / 2^11 rows with a distinct witness per step, two of them Twist/Shout shards
+(Route A memory over the shared CPU bus), are proved one at a time in one prover context, then
+through three concurrent contexts, then through the Poisson server; all three must agree, and a
+CCS-only and a Twist/Shout shard from the batched path must equal the CPU restatement's bytes."""
 import numpy as np
 import pytest
 
@@ -68,33 +68,18 @@ def test_served_equals_alone(shards):
         assert r.finish >= r.start >= r.arrival - 1e-3
 
 
-def test_batched_shard_matches_oracle(shards, oracle):
-    from nightstream_amd.batch import gen_r1cs_fast, prove_batched
-    sh = shards[0]
-    inst, z = gen_r1cs_fast(sh.n, sh.seed)
-    data, _ = oracle.Prover(inst, SEED).prove([z] * sh.steps)
+@pytest.mark.parametrize("idx", [0, 1])
+def test_batched_shard_matches_oracle(shards, oracle, idx):
+    """Shard 0 (CCS-only) and shard 1 (Twist/Shout, 2^4-cycle chunks) from the batched path
+    equal the CPU restatement's proofs of the same instances."""
+    from nightstream_amd.batch import ShardFactory, prove_batched
+    sh = shards[idx]
+    assert (sh.mem is not None) == (idx == 1)
+    inst, zs, mems = ShardFactory.instance(sh.n, sh.steps, sh.seed, sh.mem is not None)
+    print(f"CPU restatement of shard {idx}", flush=True)
+    data, _ = oracle.Prover(inst, SEED).prove(zs, mem=mems)
     bat, _ = prove_batched(shards, 3)
-    assert bat[0] == data
-
-
-def test_sidecar_chunk_matches_oracle(shards, oracle):
-    """A memory shard's proof is its CCS proof followed by one sidecar chunk per step; the first
-    chunk from the batched path equals the CPU restatement's oracles driven one at a time with
-    the same points and challenges."""
-    import nightstream_amd as ns
-    from oracle import pyoracle
-    from nightstream_amd.batch import prove_batched
-    from nightstream_amd.sidecar import chunk_values, step_seed
-    sh = shards[1]
-    assert sh.mem is not None and sh.mem_ell == 11
-    bat, _ = prove_batched(shards, 3)
-    ccs = ns.FoldingSession(sh.ccs, sh.pp).prove_device(sh.z_ptrs())
-    assert bat[1].startswith(ccs)
-    pyoracle.build()
-    print("CPU restatement of one chunk", flush=True)
-    want = chunk_values(pyoracle.TsOracle, sh.mem, sh.mem_ell, step_seed(ccs, 0), None)
-    assert len(bat[1]) == len(ccs) + sh.steps * len(want)
-    assert bat[1][len(ccs):len(ccs) + len(want)] == want
+    assert bat[idx] == data
 
 
 def test_process_prover_equals_plain(shards):
