@@ -767,10 +767,11 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
 }
 
 // one block per coefficient: strided partial sums, wave reductions, then the 4 wave results
-__global__ __launch_bounds__(256) void k_sum_partials(const K2* __restrict__ partials, int nblk, K2* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sum_partials(const K2* __restrict__ partials, int nblk, int ncoef,
+                                                      K2* __restrict__ out) {
   const int d = blockIdx.x;
   K2 v = kzero();
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = kadd(v, partials[b * 5 + d]);
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = kadd(v, partials[b * ncoef + d]);
   __shared__ K2 red[4];
   v = wave_reduce(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(const K2* __restrict__ par
   if (threadIdx.x == 0) out[d] = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
 }
 
-size_t round_ws_bytes() { return (size_t)RMAXBLK * 5 * sizeof(K2); }
+size_t round_ws_bytes() { return (size_t)RMAXBLK * 7 * sizeof(K2); }
 
 void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool round0, K2* out, void* ws,
                        hipStream_t s) {
@@ -819,39 +820,43 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
     NS_LAUNCH_TIMED(KT_ROUND1, k_round_eval<1>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
   else
     NS_LAUNCH_TIMED(KT_ROUND, k_round_eval<2>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
-  hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, 5, out);
 }
 
 // ---------------- fused fold + round on the K tables (rounds >= 3) ----------------
+// The eq tables are products: after folding the variables before j, eq_beta[2t + x] =
+// C_j eq(beta_j, x) g_j(t) with g_j = chi of beta_{j+1..} over the pairs t and C_j the product of
+// eq(beta_i, r_i) over the folded variables (likewise eq_{r_in}).  So
+//   sum_t eq_beta(t; X) (f_t + NC_t)(X) = C_j eq(beta_j, X) sum_t g_j(t) (f_t + NC_t)(X)
+// and the device computes P(X) = sum_t g_j(t) (f_t + NC_t)(X) (degree <= 4) and
+// Q(X) = sum_t g'_j(t) Eval_t(X) (degree 1) from the g tables; the host multiplies by the linear
+// eq factors (C_j, gamma^k included).  The eq tables are neither read nor folded here.
+//
 // Work item (pair t, row group g < G) folds its rows' entries 4t..4t+3 of the previous tables by
 // the previous challenge r into entries 2t, 2t+1 of the new tables (written for the next round)
-// and evaluates the round polynomial on them; item g = G evaluates f(Mz) and the Eval term from
-// the scalar tables (folded in registers), and items g = G + 1 + q fold scalar table q for the
-// next round, so no thread waits on more than one round of loads per table.  Each block
-// leaves its 5 partial coefficients for k_sum_partials (the kernel boundary orders them; a
-// device-scope fence per block would write back L2).  Same values as launch_fold_k followed by
-// launch_round_eval.
+// and evaluates its part of P on them; item g = G evaluates f(Mz) and the Eval term from the
+// scalar tables (folded in registers), and items g = G + 1 + i fold the i-th f-variable table
+// (or the Eval table) for the next round, so no thread waits on more than one round of loads per
+// table.  Each block leaves its 7 partial coefficients (P0..P4, Q0, Q1) for k_sum_partials (the
+// kernel boundary orders them; a device-scope fence per block would write back L2).
 __device__ __forceinline__ K2 fold2(K2 lo, K2 hi, K2 r, uint64_t r7) { return kadd(lo, kmul_w7(r, r7, ksub(hi, lo))); }
 
-// scalar table q of pair t folded at 2t, 2t+1 for the next round (q = 0 is the eq table, already
-// folded into elo / ehi by the caller)
-__device__ __forceinline__ void fused_fold_scalar(const FusedArgs& A, size_t t, int q, K2 elo, K2 ehi) {
+constexpr int FUSED_NCOEF = 7;
+
+// scalar table q of pair t folded at 2t, 2t+1 for the next round
+__device__ __forceinline__ void fused_fold_scalar(const FusedArgs& A, size_t t, int q) {
   const size_t L2 = 2 * A.len;
-  K2 lo = elo, hi = ehi;
-  if (q > 0) {
-    const K2* src = A.sc_src + (size_t)q * L2 + 4 * t;
-    lo = fold2(src[0], src[1], A.r, A.r7);
-    hi = fold2(src[2], src[3], A.r, A.r7);
-  }
+  const K2* src = A.sc_src + (size_t)q * L2 + 4 * t;
+  const K2 lo = fold2(src[0], src[1], A.r, A.r7), hi = fold2(src[2], src[3], A.r, A.r7);
   K2* dst = A.sc_dst + (size_t)q * A.len + 2 * t;
   dst[0] = lo;
   dst[1] = hi;
 }
 
-// f(Mz)(X) of pair t added to in[0..4] and its Eval term to ev0..ev2, on the f-variable and Eval
-// tables folded in registers: every load is issued at once, and a variable is picked by a
-// register select (a runtime-indexed array would live in scratch memory)
-__device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&in)[5], K2& ev0, K2& ev1, K2& ev2) {
+// f(Mz)(X) of pair t into in[0..4] and the Eval table's line of pair t into q0 + q1 X, on the
+// f-variable and Eval tables folded in registers: every load is issued at once, and a variable
+// is picked by a register select (a runtime-indexed array would live in scratch memory)
+__device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&in)[5], K2& q0, K2& q1) {
   const size_t L2 = 2 * A.len;
   const K2 r = A.r;
   const uint64_t r7 = A.r7;
@@ -866,15 +871,15 @@ __device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&
       fhi[q] = fold2(src[2], src[3], r, r7);
     }
   }
-  K2 rl = kzero(), rh = kzero(), vl = kzero(), vh = kzero();
+  q0 = q1 = kzero();
   if (A.has_eval) {
-    const K2* sr = A.sc_src + (size_t)(1 + A.nf) * L2 + 4 * t;
-    const K2* sv = sr + L2;
-    rl = fold2(sr[0], sr[1], r, r7);
-    rh = fold2(sr[2], sr[3], r, r7);
-    vl = fold2(sv[0], sv[1], r, r7);
-    vh = fold2(sv[2], sv[3], r, r7);
+    const K2* sv = A.sc_src + (size_t)(2 + A.nf) * L2 + 4 * t;
+    const K2 vl = fold2(sv[0], sv[1], r, r7), vh = fold2(sv[2], sv[3], r, r7);
+    q0 = vl;
+    q1 = ksub(vh, vl);
   }
+#pragma unroll
+  for (int d = 0; d < 5; d++) in[d] = kzero();
   for (int tt = 0; tt < A.f.nterms; tt++) {
     K2 p[5];
     p[0] = K2{A.f.coeff[tt], 0};
@@ -899,25 +904,21 @@ __device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&
 #pragma unroll
     for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
   }
-  if (A.has_eval) {
-    const K2 r0_ = rl, r1_ = ksub(rh, rl);
-    const K2 v0 = vl, v1 = ksub(vh, vl);
-    ev0 = kadd(ev0, kmul(r0_, v0));
-    ev1 = kadd(ev1, kadd(kmul(r0_, v1), kmul(r1_, v0)));
-    ev2 = kadd(ev2, kmul(r1_, v1));
-  }
 }
 
-// NC rows r_lo, r_lo + step, ... < r_hi of pair t: folded into the next tables, and the four round
-// terms of their sum written to in[0..3] (lazy accumulators, one reduction per term)
+// NC rows r_lo .. r_hi - 1 of pair t: folded into the next tables, and the four coefficients of
+// sum_row w_row N(y_row(X)) written to in[0..3] (lazy accumulators, one reduction per term).
+// SCALE: the weights are taken times gt (7 gt.c1 = g7), for items of one or a few rows where
+// scaling each weight is cheaper than scaling the four sums.
+template <bool SCALE>
 __device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __restrict__ wg, size_t t, int r_lo,
-                                              int r_hi, int step, K2 (&in)[5]) {
+                                              int r_hi, K2 gt, uint64_t g7, K2 (&in)[4]) {
   const size_t L2 = 2 * A.len;
   const K2 r = A.r;
   const uint64_t r7 = A.r7;
   KS2 z1 = ks2_zero(), z3 = ks2_zero(), z5 = ks2_zero(), z6 = ks2_zero();
   KS1 z2 = ks1_zero(), z4 = ks1_zero();
-  for (int row = r_lo; row < r_hi; row += step) {
+  for (int row = r_lo; row < r_hi; row++) {
     const K2* src = A.nc_src + (size_t)row * L2 + 4 * t;
     const K2 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
     const K2 a = fold2(u0, u1, r, r7), hv = fold2(u2, u3, r, r7);
@@ -925,7 +926,8 @@ __device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __re
     dst[0] = a;
     dst[1] = hv;
     const K2 b = ksub(hv, a);
-    const K2 wv = wg[row];
+    K2 wv = wg[row];
+    if (SCALE) wv = kmul_w7(gt, g7, wv);
     const uint64_t w7 = fmul7(wv.c1);
     const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
     const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
@@ -948,78 +950,82 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
                                                      int G) {
   // row weights come from global memory (11 KB, cache-resident): a short round's block needs a
   // few of them, so staging all k*54 in LDS would cost more than it saves
-  __shared__ K2 red[4 * 5];
+  __shared__ K2 red[4 * FUSED_NCOEF];
   const int nrow = A.k * D;
-  const K2 r = A.r;
-  const uint64_t r7 = A.r7;
   const size_t pairs = A.len / 2;
   const int R = (nrow + G - 1) / G;
-  K2 c[5];
+  const int nfold = A.nf + (A.has_eval ? 1 : 0);
+  K2 c[FUSED_NCOEF];
 #pragma unroll
-  for (int d = 0; d < 5; d++) c[d] = kzero();
-  K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
-  const size_t total = pairs * ((size_t)G + 1 + A.nsc);
+  for (int d = 0; d < FUSED_NCOEF; d++) c[d] = kzero();
+  const size_t total = pairs * ((size_t)G + 1 + nfold);
   const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const size_t t = idx & (pairs - 1);
     const int g = (int)(idx >> lp);
-    const K2* eq = A.sc_src + 4 * t;
-    const K2 elo = fold2(eq[0], eq[1], r, r7), ehi = fold2(eq[2], eq[3], r, r7);
-    const K2 e0 = elo, e1 = ksub(ehi, elo);
-    K2 in[5];
-#pragma unroll
-    for (int d = 0; d < 5; d++) in[d] = kzero();
-    if (g > G) {
-      fused_fold_scalar(A, t, g - G - 1, elo, ehi);  // no round contribution
+    if (g > G) {  // fold items: f-variable tables 1..nf, then the Eval table 2 + nf
+      const int i = g - G - 1;
+      fused_fold_scalar(A, t, i < A.nf ? 1 + i : 2 + A.nf);
       continue;
     }
+    const K2 gt = A.gb[t];
+    const uint64_t g7 = fmul7(gt.c1);
     if (g == G) {
-      fused_f_item(A, t, in, ev0, ev1, ev2);
-    } else {
-      const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
-      fused_nc_rows(A, wg, t, r_lo, r_hi, 1, in);
-    }
-    const uint64_t e07 = fmul7(e0.c1), e17 = fmul7(e1.c1);
-    c[0] = kadd(c[0], kmul_w7(e0, e07, in[0]));
+      K2 in[5], q0, q1;
+      fused_f_item(A, t, in, q0, q1);
 #pragma unroll
-    for (int d = 1; d < 4; d++) c[d] = kadd(c[d], kadd(kmul_w7(e0, e07, in[d]), kmul_w7(e1, e17, in[d - 1])));
-    c[4] = kadd(c[4], kmul_w7(e1, e17, in[3]));
-    if (g == G) c[4] = kadd(c[4], kmul_w7(e0, e07, in[4]));
-  }
-  if (A.has_eval) {
-    c[0] = kadd(c[0], kmul(A.gamma_k, ev0));
-    c[1] = kadd(c[1], kmul(A.gamma_k, ev1));
-    c[2] = kadd(c[2], kmul(A.gamma_k, ev2));
+      for (int d = 0; d < 5; d++) c[d] = kadd(c[d], kmul_w7(gt, g7, in[d]));
+      if (A.has_eval) {
+        const K2 gr = A.gr[t];
+        const uint64_t gr7 = fmul7(gr.c1);
+        c[5] = kadd(c[5], kmul_w7(gr, gr7, q0));
+        c[6] = kadd(c[6], kmul_w7(gr, gr7, q1));
+      }
+      continue;
+    }
+    const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
+    K2 in[4];
+    if (R <= 3) {
+      fused_nc_rows<true>(A, wg, t, r_lo, r_hi, gt, g7, in);
+#pragma unroll
+      for (int d = 0; d < 4; d++) c[d] = kadd(c[d], in[d]);
+    } else {
+      fused_nc_rows<false>(A, wg, t, r_lo, r_hi, gt, g7, in);
+#pragma unroll
+      for (int d = 0; d < 4; d++) c[d] = kadd(c[d], kmul_w7(gt, g7, in[d]));
+    }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
 #pragma unroll
-  for (int d = 0; d < 5; d++) {
+  for (int d = 0; d < FUSED_NCOEF; d++) {
     K2 v = wave_reduce(c[d]);
-    if (lane == 0) red[wave * 5 + d] = v;
+    if (lane == 0) red[wave * FUSED_NCOEF + d] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 5) {
+  if (threadIdx.x < FUSED_NCOEF) {
     K2 v = kzero();
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = kadd(v, red[w * 5 + threadIdx.x]);
-    partials[blockIdx.x * 5 + threadIdx.x] = v;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = kadd(v, red[w * FUSED_NCOEF + threadIdx.x]);
+    partials[blockIdx.x * FUSED_NCOEF + threadIdx.x] = v;
   }
 }
 
 void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s) {
   const size_t pairs = A.len / 2, nrow = (size_t)A.k * D;
   if (pairs == 0 || (pairs & (pairs - 1))) throw std::invalid_argument("fused round: table length must be a power of two >= 2");
+  if (A.nf + 1 > FUSED_MAX_SC - (A.has_eval ? 2 : 0)) throw std::invalid_argument("fused round: too many scalar tables");
+  if (!A.gb || (A.has_eval && !A.gr)) throw std::invalid_argument("fused round: g tables missing");
   const size_t target = (size_t)RB * 512;
   size_t G = 1;
   while (G < nrow && pairs * G < target) G *= 2;
   if (G > nrow) G = nrow;
   G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
   K2* partials = (K2*)ws;
-  size_t nblk = (pairs * (G + 1 + (size_t)A.nsc) + RB - 1) / RB;
+  const size_t nfold = (size_t)A.nf + (A.has_eval ? 1 : 0);
+  size_t nblk = (pairs * (G + 1 + nfold) + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
   NS_LAUNCH_TIMED(KT_FUSED, k_round_fused, dim3((unsigned)nblk), dim3(RB), 0, s, A, wg, partials, (int)G);
-  hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(FUSED_NCOEF), dim3(256), 0, s, partials, (int)nblk, FUSED_NCOEF, out);
 }
 
 // ---------------- folds ----------------

@@ -134,9 +134,11 @@ size_t round_ws_bytes();
 // Writes the deg_max+1 coefficients (low->high) of the round polynomial to out (device).
 void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg /* [k*54] device */, bool round0,
                        K2* out, void* ws, hipStream_t s);
-// Rounds >= 3: fold every table (scalar [nsc][2 len] -> [nsc][len], NC [k*54][2 len] ->
-// [k*54][len]) by r and write the round polynomial's coefficients to out (one fused fold+round
-// launch and the partial sum).  nsc <= 8.
+// Rounds >= 3: fold the NC tables ([k*54][2 len] -> [k*54][len]) and the f-variable and Eval
+// scalar tables by r (the eq tables are not read: their factors come from gb / gr and the host)
+// and write P0..P4, Q0, Q1 to out: P(X) = sum_t gb[t] (f_t + NC_t)(X), Q(X) = sum_t gr[t] Eval_t(X)
+// over the pairs t of the folded tables.  One fused fold + evaluate launch and the partial sum.
+// nsc <= 8.
 struct FusedArgs {
   size_t len;  // table length after the fold
   const K2* sc_src;
@@ -149,7 +151,9 @@ struct FusedArgs {
   K2 gamma_k;
   FPoly f;
   K2 r;
-  uint64_t r7;  // 7 r.c1
+  uint64_t r7;            // 7 r.c1
+  const K2* gb = nullptr;  // [len / 2]: chi of beta over the variables after this round's
+  const K2* gr = nullptr;  // [len / 2]: the same for r_in (has_eval)
 };
 constexpr int FUSED_MAX_SC = 8;
 void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s);

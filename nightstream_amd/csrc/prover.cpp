@@ -681,6 +681,22 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
   }
   w_->wg.ensure(wg.size() * sizeof(K2));
   h2d(w_->wg.p, wg.data(), wg.size(), st);
+  // chi tables of the variables after each fused round j >= 3 (2^(ell_n - j - 1) entries)
+  beta_r_.assign(beta_r, beta_r + ell_n);
+  if (has_eval_) rin_.assign(r_inputs, r_inputs + ell_n);
+  g_off_.assign(ell_n + 1, 0);
+  g_total_ = 0;
+  for (size_t j = 3; j < ell_n; j++) {
+    g_off_[j] = g_total_;
+    g_total_ += (size_t)1 << (ell_n - j - 1);
+  }
+  if (g_total_) {
+    w_->gt.ensure(2 * g_total_ * sizeof(K2));
+    for (size_t j = 3; j < ell_n; j++) {
+      launch_chi(bits + j + 1, (int)(ell_n - j - 1), w_->gt.as<K2>() + g_off_[j], st);
+      if (has_eval_) launch_chi(bits + 64 + j + 1, (int)(ell_n - j - 1), w_->gt.as<K2>() + g_total_ + g_off_[j], st);
+    }
+  }
   w_->nc[0].ensure(k * D * (n_pad / 2) * sizeof(K2));
   w_->nc[1].ensure(k * D * std::max<size_t>(n_pad / 4, 1) * sizeof(K2));
   w_->coeffs.ensure(8 * sizeof(K2));
@@ -704,13 +720,35 @@ std::vector<K2> CcsOracle::round_coeffs() {
     F.f = s_.fpoly;
     F.r = pending_r_;
     F.r7 = fmul7(pending_r_.c1);
+    const size_t j = round_;
+    if (j < 3 || j >= ell_n_) throw Error(1, "fused round outside the chi tables");
+    F.gb = w_->gt.as<K2>() + g_off_[j];
+    F.gr = has_eval_ ? w_->gt.as<K2>() + g_total_ + g_off_[j] : nullptr;
     K2* mb = w_->mbox.get<K2>(8);
     launch_round_fused(F, w_->wg.as<K2>(), mb, w_->ws.p, st_);
     cur_sc_ ^= 1;
     cur_nc_ ^= 1;
     pending_ = false;
     NS_CHECK(hipStreamSynchronize(st_));
-    return std::vector<K2>(mb, mb + 5);
+    // eq_beta(t; X) = C eq(beta_j, X) g(t): multiply P by C ((1 - beta_j) + (2 beta_j - 1) X)
+    // (a degree-5 term would need f of degree 4, as in the unfused rounds it is dropped), and Q
+    // by gamma^k C' ((1 - r_in_j) + (2 r_in_j - 1) X)
+    const K2* P = mb;
+    const K2 bj = beta_r_[j];
+    const K2 b0 = kmul(cb_, ksub(kone(), bj)), b1 = kmul(cb_, ksub(kadd(bj, bj), kone()));
+    std::vector<K2> o(5, kzero());
+    for (int d = 0; d < 5; d++) {
+      o[d] = kmul(b0, P[d]);
+      if (d) o[d] = kadd(o[d], kmul(b1, P[d - 1]));
+    }
+    if (has_eval_) {
+      const K2 rj = rin_[j], cg = kmul(gamma_k_, cr_);
+      const K2 a0 = kmul(cg, ksub(kone(), rj)), a1 = kmul(cg, ksub(kadd(rj, rj), kone()));
+      o[0] = kadd(o[0], kmul(a0, mb[5]));
+      o[1] = kadd(o[1], kadd(kmul(a0, mb[6]), kmul(a1, mb[5])));
+      o[2] = kadd(o[2], kmul(a1, mb[6]));
+    }
+    return o;
   }
   RowTables T;
   const K2* sc = w_->sc[cur_sc_].as<K2>();
@@ -775,6 +813,10 @@ void CcsOracle::flush() {
 
 void CcsOracle::fold(K2 r) {
   flush();
+  if (round_ < ell_n_) {
+    cb_ = kmul(cb_, eq_lin(r, beta_r_[round_]));
+    if (has_eval_) cr_ = kmul(cr_, eq_lin(r, rin_[round_]));
+  }
   const size_t half = len_ / 2;
   if (round_ >= 2 && nsc_ <= FUSED_MAX_SC && half >= 1) {
     pending_ = true;

@@ -135,7 +135,7 @@ struct PinnedStage {
 };
 
 struct OracleWs {
-  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc;
+  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc, gt;
   HostMailbox mbox;   // 8 K2: the round polynomial's coefficients
   PinnedStage stage;  // the context's device-to-host staging
 };
@@ -277,6 +277,13 @@ class CcsOracle {
   bool pending_ = false;
   K2 pending_r_ = K2{0, 0};
   void flush();
+  // eq factors of the fused rounds: beta_r and r_in per row variable, the products of
+  // eq(beta_i, r_i) and eq(r_in_i, r_i) over the folded variables, and the offsets of the chi
+  // tables of the later variables (gt: every fused round's beta table, then its r_in table)
+  std::vector<K2> beta_r_, rin_;
+  K2 cb_ = K2{1, 0}, cr_ = K2{1, 0};
+  std::vector<size_t> g_off_;
+  size_t g_total_ = 0;
 };
 
 }  // namespace ns
