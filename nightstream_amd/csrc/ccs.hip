@@ -39,6 +39,33 @@ void launch_chi(const K2* bits, int nbits, K2* out, hipStream_t s) {
   hipLaunchKernelGGL(k_chi, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, bits, nbits, out);
 }
 
+// chi tables of the suffixes bits[j+1 .. ell) for j = j0 .. ell - 1, concatenated (table j holds
+// 2^(ell - j - 1) entries); blockIdx.y = 0 takes bits[0 ..], 1 takes bits[64 ..] into out + total
+__global__ void k_chi_suffix(const K2* __restrict__ bits, int ell, int j0, size_t total, K2* __restrict__ out) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const K2* b = bits + 64 * blockIdx.y;
+  size_t off = 0;
+  int j = j0;
+  while (j < ell - 1 && idx >= off + ((size_t)1 << (ell - j - 1))) {
+    off += (size_t)1 << (ell - j - 1);
+    j++;
+  }
+  const size_t e = idx - off;
+  K2 w = kone();
+  for (int i = 0; i < ell - j - 1; i++) {
+    const K2 r = b[j + 1 + i];
+    w = kmul(w, ((e >> i) & 1) ? r : ksub(kone(), r));
+  }
+  out[blockIdx.y * total + idx] = w;
+}
+
+void launch_chi_suffix(const K2* bits, int ell, int j0, size_t total, int ntab, K2* out, hipStream_t s) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_chi_suffix, dim3((unsigned)((total + 255) / 256), (unsigned)ntab), dim3(256), 0, s, bits, ell, j0,
+                     total, out);
+}
+
 // ---------------- z1 ----------------
 __global__ void k_z1(const int8_t* __restrict__ Z, size_t m, K2* __restrict__ z1) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,15 +491,35 @@ struct RoundArgs {
   const K2* pc;  // round 1: [20][4] coefficient polynomials of the digit monomials
 };
 
-__device__ __forceinline__ K2 wave_reduce(K2 v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    K2 o;
-    o.c0 = __shfl_down(v.c0, off, 64);
-    o.c1 = __shfl_down(v.c1, off, 64);
-    v = kadd(v, o);
-  }
-  return v;
+// Wave sum of K values, valid in lane 63 (DPP_LANE): each component is summed lazily as a 64-bit
+// value plus a carry count with DPP row shifts and row broadcasts (register-to-register moves
+// across lanes, no LDS round trips), then reduced once.  Lanes a DPP step reads from outside the
+// row (or that its row mask leaves out) add zero.
+constexpr int DPP_LANE = 63;
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, true);
 }
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void dpp_lazy_add(uint64_t& v, uint32_t& c) {
+  const uint32_t lo = dpp_u32<CTRL, ROW_MASK>((uint32_t)v), hi = dpp_u32<CTRL, ROW_MASK>((uint32_t)(v >> 32));
+  const uint32_t cc = dpp_u32<CTRL, ROW_MASK>(c);
+  const uint64_t o = ((uint64_t)hi << 32) | lo;
+  const uint64_t t = v + o;
+  c += cc + (uint32_t)(t < o);
+  v = t;
+}
+__device__ __forceinline__ uint64_t wave_sum_fq(uint64_t v) {
+  uint32_t c = 0;
+  dpp_lazy_add<0x111, 0xf>(v, c);  // row_shr:1
+  dpp_lazy_add<0x112, 0xf>(v, c);  // row_shr:2
+  dpp_lazy_add<0x114, 0xf>(v, c);  // row_shr:4
+  dpp_lazy_add<0x118, 0xf>(v, c);  // row_shr:8: lane 15 of each row holds the row's sum
+  dpp_lazy_add<0x142, 0xa>(v, c);  // row_bcast:15 into rows 1 and 3
+  dpp_lazy_add<0x143, 0xc>(v, c);  // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+  return reduce128_dev(v, c);
+}
+__device__ __forceinline__ K2 wave_sum(K2 v) { return K2{wave_sum_fq(v.c0), wave_sum_fq(v.c1)}; }
 
 // coefficient vector of width 5 (deg_max = 4).  Work item = (row pair t, row group g): the NC
 // sum over the k*54 digit rows of a pair is split into G groups so that short tables still fill
@@ -755,8 +802,8 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
   __syncthreads();
 #pragma unroll
   for (int d = 0; d < 5; d++) {
-    K2 v = wave_reduce(c[d]);
-    if (lane == 0) red[wave * 5 + d] = v;
+    K2 v = wave_sum(c[d]);
+    if (lane == DPP_LANE) red[wave * 5 + d] = v;
   }
   __syncthreads();
   if (threadIdx.x < 5) {
@@ -773,8 +820,8 @@ __global__ __launch_bounds__(256) void k_sum_partials(const K2* __restrict__ par
   K2 v = kzero();
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = kadd(v, partials[b * ncoef + d]);
   __shared__ K2 red[4];
-  v = wave_reduce(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == DPP_LANE) red[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) out[d] = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
 }
@@ -853,10 +900,10 @@ __device__ __forceinline__ void fused_fold_scalar(const FusedArgs& A, size_t t, 
   dst[1] = hi;
 }
 
-// f(Mz)(X) of pair t into in[0..4] and the Eval table's line of pair t into q0 + q1 X, on the
-// f-variable and Eval tables folded in registers: every load is issued at once, and a variable
-// is picked by a register select (a runtime-indexed array would live in scratch memory)
-__device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&in)[5], K2& q0, K2& q1) {
+// f(Mz)(X) of pair t into in[0..4], on the f-variable tables folded in registers: every load is
+// issued at once, and a variable is picked by a register select (a runtime-indexed array would
+// live in scratch memory)
+__device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&in)[5]) {
   const size_t L2 = 2 * A.len;
   const K2 r = A.r;
   const uint64_t r7 = A.r7;
@@ -871,20 +918,19 @@ __device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&
       fhi[q] = fold2(src[2], src[3], r, r7);
     }
   }
-  q0 = q1 = kzero();
-  if (A.has_eval) {
-    const K2* sv = A.sc_src + (size_t)(2 + A.nf) * L2 + 4 * t;
-    const K2 vl = fold2(sv[0], sv[1], r, r7), vh = fold2(sv[2], sv[3], r, r7);
-    q0 = vl;
-    q1 = ksub(vh, vl);
-  }
 #pragma unroll
   for (int d = 0; d < 5; d++) in[d] = kzero();
   for (int tt = 0; tt < A.f.nterms; tt++) {
+    // p = coeff * prod (lo + (hi - lo) X)^exp over the term's variables, degree by degree: the
+    // first factor scales the F_q coefficient, later ones touch only coefficients up to deg + 1
+    // (deg is uniform across the grid, so the guards are scalar branches)
+    const uint64_t cf = A.f.coeff[tt];
     K2 p[5];
-    p[0] = K2{A.f.coeff[tt], 0};
+    p[0] = K2{cf, 0};
 #pragma unroll
     for (int d = 1; d < 5; d++) p[d] = kzero();
+    int deg = 0;
+    bool scalar = true;
     for (int v = 0; v < A.f.nv[tt]; v++) {
       const int q = A.f.var[tt][v];
       K2 lo = flo[0], hi = fhi[0];
@@ -896,13 +942,22 @@ __device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&
         }
       const K2 a = lo, b = ksub(hi, lo);
       for (int e = 0; e < A.f.exp[tt][v]; e++) {
+        if (scalar) {
+          p[0] = kscale(a, cf);
+          p[1] = kscale(b, cf);
+          scalar = false;
+        } else {
 #pragma unroll
-        for (int d = 4; d >= 1; d--) p[d] = kadd(kmul(a, p[d]), kmul(b, p[d - 1]));
-        p[0] = kmul(a, p[0]);
+          for (int d = 4; d >= 1; d--)
+            if (d <= deg + 1) p[d] = d <= deg ? kadd(kmul(a, p[d]), kmul(b, p[d - 1])) : kmul(b, p[d - 1]);
+          p[0] = kmul(a, p[0]);
+        }
+        deg = deg < 4 ? deg + 1 : 4;  // as in the full product, terms past X^4 are dropped
       }
     }
 #pragma unroll
-    for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
+    for (int d = 0; d < 5; d++)
+      if (d <= deg) in[d] = kadd(in[d], p[d]);
   }
 }
 
@@ -912,7 +967,7 @@ __device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&
 // scaling each weight is cheaper than scaling the four sums.
 template <bool SCALE>
 __device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __restrict__ wg, size_t t, int r_lo,
-                                              int r_hi, K2 gt, uint64_t g7, K2 (&in)[4]) {
+                                              int r_hi, K2 (&in)[4]) {
   const size_t L2 = 2 * A.len;
   const K2 r = A.r;
   const uint64_t r7 = A.r7;
@@ -927,7 +982,10 @@ __device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __re
     dst[1] = hv;
     const K2 b = ksub(hv, a);
     K2 wv = wg[row];
-    if (SCALE) wv = kmul_w7(gt, g7, wv);
+    if (SCALE) {  // loaded beside the row so that its latency overlaps the row's
+      const K2 gt = A.gb[t];
+      wv = kmul_w7(gt, fmul7(gt.c1), wv);
+    }
     const uint64_t w7 = fmul7(wv.c1);
     const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
     const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
@@ -958,48 +1016,77 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
   K2 c[FUSED_NCOEF];
 #pragma unroll
   for (int d = 0; d < FUSED_NCOEF; d++) c[d] = kzero();
-  const size_t total = pairs * ((size_t)G + 1 + nfold);
+  // NC items first, then (from a wave boundary, so that no wave runs both kinds) the f item and
+  // the Eval item of every pair and the scalar fold items
+  const int ne = A.has_eval ? 1 : 0;
+  const size_t nnc = pairs * (size_t)G, base_s = (nnc + 63) & ~(size_t)63;
+  int has_s = 0;
+  const size_t total = base_s + pairs * (1 + ne + (size_t)nfold);
+  // degree of f (uniform): coefficients above it are zero
+  int fdeg = 0;
+  for (int tt = 0; tt < A.f.nterms; tt++) {
+    int dg = 0;
+    for (int v = 0; v < A.f.nv[tt]; v++) dg += A.f.exp[tt][v];
+    fdeg = max(fdeg, min(dg, 4));
+  }
   const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-    const size_t t = idx & (pairs - 1);
-    const int g = (int)(idx >> lp);
-    if (g > G) {  // fold items: f-variable tables 1..nf, then the Eval table 2 + nf
-      const int i = g - G - 1;
+    if (idx >= nnc && idx < base_s) continue;
+    const bool nc_item = idx < nnc;
+    has_s |= !nc_item;
+    const size_t li = nc_item ? idx : idx - base_s;
+    const size_t t = li & (pairs - 1);
+    const int g = nc_item ? (int)(li >> lp) : G + (int)(li >> lp);
+    if (g > G + ne) {  // fold items: f-variable tables 1..nf, then the Eval table 2 + nf
+      const int i = g - G - 1 - ne;
       fused_fold_scalar(A, t, i < A.nf ? 1 + i : 2 + A.nf);
       continue;
     }
-    const K2 gt = A.gb[t];
-    const uint64_t g7 = fmul7(gt.c1);
     if (g == G) {
-      K2 in[5], q0, q1;
-      fused_f_item(A, t, in, q0, q1);
+      const K2 gt = A.gb[t];
+      K2 in[5];
+      fused_f_item(A, t, in);
+      const uint64_t g7 = fmul7(gt.c1);
 #pragma unroll
-      for (int d = 0; d < 5; d++) c[d] = kadd(c[d], kmul_w7(gt, g7, in[d]));
-      if (A.has_eval) {
-        const K2 gr = A.gr[t];
-        const uint64_t gr7 = fmul7(gr.c1);
-        c[5] = kadd(c[5], kmul_w7(gr, gr7, q0));
-        c[6] = kadd(c[6], kmul_w7(gr, gr7, q1));
-      }
+      for (int d = 0; d < 5; d++)
+        if (d <= fdeg) c[d] = kadd(c[d], kmul_w7(gt, g7, in[d]));
+      continue;
+    }
+    if (g == G + 1) {  // Eval item: Q += g'(t) (vl + (vh - vl) X) on the folded Eval table
+      const size_t L2 = 2 * A.len;
+      const K2* sv = A.sc_src + (size_t)(2 + A.nf) * L2 + 4 * t;
+      const K2 gr = A.gr[t];
+      const K2 vl = fold2(sv[0], sv[1], A.r, A.r7), vh = fold2(sv[2], sv[3], A.r, A.r7);
+      const uint64_t gr7 = fmul7(gr.c1);
+      c[5] = kadd(c[5], kmul_w7(gr, gr7, vl));
+      c[6] = kadd(c[6], kmul_w7(gr, gr7, ksub(vh, vl)));
       continue;
     }
     const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
     K2 in[4];
     if (R <= 3) {
-      fused_nc_rows<true>(A, wg, t, r_lo, r_hi, gt, g7, in);
+      fused_nc_rows<true>(A, wg, t, r_lo, r_hi, in);
 #pragma unroll
       for (int d = 0; d < 4; d++) c[d] = kadd(c[d], in[d]);
     } else {
-      fused_nc_rows<false>(A, wg, t, r_lo, r_hi, gt, g7, in);
+      const K2 gt = A.gb[t];
+      fused_nc_rows<false>(A, wg, t, r_lo, r_hi, in);
+      const uint64_t g7 = fmul7(gt.c1);
 #pragma unroll
       for (int d = 0; d < 4; d++) c[d] = kadd(c[d], kmul_w7(gt, g7, in[d]));
     }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // blocks of NC items only (most of them) leave P4, Q0 and Q1 at zero
+  const bool any_s = __syncthreads_or(has_s);
 #pragma unroll
   for (int d = 0; d < FUSED_NCOEF; d++) {
-    K2 v = wave_reduce(c[d]);
-    if (lane == 0) red[wave * FUSED_NCOEF + d] = v;
+    if (d >= 4 && !any_s) {
+      if (lane == DPP_LANE) red[wave * FUSED_NCOEF + d] = kzero();
+      continue;
+    }
+    K2 v = wave_sum(c[d]);
+    if (lane == DPP_LANE) red[wave * FUSED_NCOEF + d] = v;
   }
   __syncthreads();
   if (threadIdx.x < FUSED_NCOEF) {
@@ -1021,7 +1108,8 @@ void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hip
   G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
   K2* partials = (K2*)ws;
   const size_t nfold = (size_t)A.nf + (A.has_eval ? 1 : 0);
-  size_t nblk = (pairs * (G + 1 + nfold) + RB - 1) / RB;
+  const size_t base_s = (pairs * G + 63) & ~(size_t)63;
+  size_t nblk = (base_s + pairs * (1 + (A.has_eval ? 1 : 0) + nfold) + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
   NS_LAUNCH_TIMED(KT_FUSED, k_round_fused, dim3((unsigned)nblk), dim3(RB), 0, s, A, wg, partials, (int)G);

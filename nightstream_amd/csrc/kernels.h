@@ -100,6 +100,9 @@ void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* con
 
 // ---- chi tables, matrix-vector products ----
 void launch_chi(const K2* bits, int nbits, K2* out, hipStream_t s);  // len 2^nbits
+// chi of bits[j+1 .. ell) for every j in [j0, ell), tables concatenated (2^(ell - j - 1) entries
+// each, total entries); ntab = 2 also builds the tables of bits[64 + ..] at out + total
+void launch_chi_suffix(const K2* bits, int ell, int j0, size_t total, int ntab, K2* out, hipStream_t s);
 void launch_z1(const int8_t* Z1, size_t m, K2* z1, hipStream_t s);
 // y[r] = coeff * sum_k val[k] x[col[k]] (+ y[r] if accumulate), r < n
 void launch_spmv(const DevSparse& M, const K2* x, K2* y, K2 coeff, bool accumulate, hipStream_t s);

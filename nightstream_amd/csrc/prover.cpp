@@ -692,10 +692,7 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
   }
   if (g_total_) {
     w_->gt.ensure(2 * g_total_ * sizeof(K2));
-    for (size_t j = 3; j < ell_n; j++) {
-      launch_chi(bits + j + 1, (int)(ell_n - j - 1), w_->gt.as<K2>() + g_off_[j], st);
-      if (has_eval_) launch_chi(bits + 64 + j + 1, (int)(ell_n - j - 1), w_->gt.as<K2>() + g_total_ + g_off_[j], st);
-    }
+    launch_chi_suffix(bits, (int)ell_n, 3, g_total_, has_eval_ ? 2 : 1, w_->gt.as<K2>(), st);
   }
   w_->nc[0].ensure(k * D * (n_pad / 2) * sizeof(K2));
   w_->nc[1].ensure(k * D * std::max<size_t>(n_pad / 4, 1) * sizeof(K2));

@@ -1,8 +1,13 @@
 """Summaries of rocprofv3 output for the prover's kernels (text tables kept beside the code).
 
-    python -m nightstream_amd.profiling stats  <kernel_stats.csv> --cmd "<command>"
+    python -m nightstream_amd.profiling stats  <kernel_stats.csv> --cmd "<command>" [--trace <kernel_trace.csv>]
     python -m nightstream_amd.profiling traffic <fetch_counters.csv> <write_counters.csv> \\
         <copy_fetch_counters.csv> <copy_write_counters.csv> --copy-bytes N --cmd "<command>"
+    python -m nightstream_amd.profiling sq <results.db> --cmd "<command>"
+
+`stats --trace` adds the median duration of each fused Pi_CCS round (k_round_fused launches in
+order, --rounds-per-step = ell_n - 3 of them per fold step).  `sq` tabulates the SQ counters (waves, wave cycles, issue and
+wait cycles, VALU instructions) of each fused round from a rocpd database.
 
 `traffic` turns the per-dispatch FETCH_SIZE / WRITE_SIZE counters (KiB) into HBM bytes per launch.
 Each counter is scaled by the ratio (bytes a plain element-wise copy kernel moves) / (what the same
@@ -46,6 +51,52 @@ def stats_table(path: str, cmd: str, top: int = 25) -> str:
                    f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
                    f"{float(r['Percentage']):.2f} |")
     out += ["", f"Total kernel time: {total / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches."]
+    return "\n".join(out) + "\n"
+
+
+def fused_rounds_line(trace_csv: str, period: int, name: str = "k_round_fused", first: int = 3) -> str:
+    """Median duration of each fused Pi_CCS round over the fold steps of a kernel trace (the
+    fused launches in start order, `period` = ell_n - 3 of them per fold step)."""
+    import statistics
+    rows = [r for r in csv.DictReader(open(trace_csv)) if short_name(r["Kernel_Name"]) == name]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
+    per = collections.defaultdict(list)
+    for i, v in enumerate(d):
+        per[i % period].append(v)
+    return (f"{name} per round (median over {len(d) // period} steps, us): " +
+            ", ".join(f"round {first + i}: {statistics.median(v):.1f}" for i, v in sorted(per.items())))
+
+
+def sq_table(db: str, cmd: str, name: str = "k_round_fused", first: int = 3) -> str:
+    """SQ counters of each fused round (the last fold step of the run), per wave."""
+    import sqlite3
+    cur = sqlite3.connect(db).cursor()
+    d = collections.defaultdict(dict)
+    dur = {}
+    for disp, kname, cname, val, du in cur.execute(
+            "select dispatch_id, kernel_name, counter_name, value, duration from counters_collection"):
+        if short_name(kname) == name:
+            d[disp][cname] = d[disp].get(cname, 0) + val
+            dur[disp] = du
+    ids = sorted(d)
+    # the launches of one step: from the last launch whose wave count is the maximum onward
+    w = [d[k].get("SQ_WAVES", 0) for k in ids]
+    last = max(i for i in range(len(ids)) if w[i] == max(w))
+    step = ids[last:]
+    out = [f"# SQ counters per fused Pi_CCS round ({name}, one fold step with k = 13 claims)", "",
+           f"Command: `{cmd}`", "",
+           "Cycles are shader clocks (SQ quad-cycles x 4), per wave.", "",
+           "| round | waves | duration us | wave cycles | issuing any / VALU | waiting (any) | waiting (inst) | "
+           "VALU insts |", "|---|---|---|---|---|---|---|---|"]
+    for i, k in enumerate(step):
+        c = d[k]
+        nw = max(c.get("SQ_WAVES", 1), 1)
+        wc = 4 * c.get("SQ_WAVE_CYCLES", 0) / nw
+        act, valu = 4 * c.get("SQ_ACTIVE_INST_ANY", 0) / nw, 4 * c.get("SQ_ACTIVE_INST_VALU", 0) / nw
+        out.append(f"| {first + i} | {nw:.0f} | {dur[k] / 1e3:.1f} | {wc:.0f} | {100 * act / max(wc, 1):.0f}% / "
+                   f"{100 * valu / max(wc, 1):.0f}% | {4 * c.get('SQ_WAIT_ANY', 0) / nw:.0f} | "
+                   f"{4 * c.get('SQ_WAIT_INST_ANY', 0) / nw:.0f} | {c.get('SQ_INSTS_VALU', 0) / nw:.0f} |")
     return "\n".join(out) + "\n"
 
 
@@ -108,6 +159,11 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--cmd", required=True)
+    s.add_argument("--trace", default=None)
+    s.add_argument("--rounds-per-step", type=int, default=13)
+    q = sub.add_parser("sq")
+    q.add_argument("db")
+    q.add_argument("--cmd", required=True)
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -119,6 +175,10 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.mode == "stats":
         sys.stdout.write(stats_table(a.csv, a.cmd))
+        if a.trace:
+            sys.stdout.write("\n" + fused_rounds_line(a.trace, a.rounds_per_step) + "\n")
+    elif a.mode == "sq":
+        sys.stdout.write(sq_table(a.db, a.cmd))
     else:
         sys.stdout.write(traffic_table(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes, a.cmd, a.copy_cmd))
 
