@@ -23,7 +23,7 @@ EXPORTS = [
     "ns_last_error", "ns_device_count", "ns_free", "ns_device_alloc", "ns_device_free", "ns_memcpy_h2d",
     "ns_memcpy_d2h", "ns_device_synchronize", "ns_params_goldilocks_auto_r1cs_ccs", "ns_ccs_from_r1cs",
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
-    "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
+    "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ajtai_force_variant", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
     "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_shard_verify", "ns_rlc", "ns_dec", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
@@ -87,6 +87,7 @@ def _load():
     L.ns_decomp_b.argtypes = [vp, sz, C.c_uint32, vp]
     L.ns_ajtai_commit_batch.argtypes = [vp, vp, sz, vp]
     L.ns_ajtai_commit_batch_device.argtypes = [vp, vp, sz, vp]
+    L.ns_ajtai_force_variant.argtypes = [C.c_int]
     L.ns_ccs_oracle_new.argtypes = [vp, vp, sz, vp, vp, vp, vp, sz, sz, vp, C.POINTER(vp)]
     L.ns_oracle_evals.argtypes = [vp, vp, sz, vp]
     L.ns_oracle_fold.argtypes = [vp, vp]
@@ -373,6 +374,11 @@ def _step_mem_array(mems, keep: list):
         arr[s] = _StepMem(len(sm.luts), C.cast(luts, C.c_void_p), len(sm.mems), C.cast(ms, C.c_void_p))
     keep.append(arr)
     return arr
+
+
+def ajtai_force_variant(variant: int) -> None:
+    """Commitment GEMM tile shape: 0 timed choice (default), 1 16x16x64 tiles, 2 32x32x32 tiles."""
+    _check(lib().ns_ajtai_force_variant(int(variant)))
 
 
 class _Me(C.Structure):

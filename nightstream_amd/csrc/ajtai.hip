@@ -150,9 +150,12 @@ static bool aj_kmajor(size_t m) { return m >= ((size_t)1 << 18); }
 // Largest K-split count the tuner tries: at most 16, with at least 16 K steps per split.
 static size_t aj_max_splits(size_t m) { return std::max<size_t>(1, std::min<size_t>(16, (m / AJ_BK) / 16)); }
 
-// sized for the largest split count, so any tuned choice fits
+static size_t a3_groups_host(size_t nw) { return (nw * D + 255) / 256; }
+static size_t aj_npad_max(size_t nw) { return std::max(aj_groups(nw) * AJ_NBLK, a3_groups_host(nw) * 256); }
+
+// sized for the largest split count and either tile shape, so any tuned choice fits
 size_t ajtai_workspace_bytes(size_t m, size_t nw) {
-  size_t npad = aj_groups(nw) * AJ_NBLK;
+  size_t npad = aj_npad_max(nw);
   size_t sp = aj_max_splits(m);
   const size_t bt = aj_kmajor(m) ? nw * D * m + 256 : 0;
   return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512 + bt;
@@ -299,6 +302,163 @@ __global__ __launch_bounds__(AJ_THREADS) void k_ajtai_gemm(const int8_t* __restr
       }
 }
 
+// ---- 32 x 32 x 32 variant: 4 waves of 128 x 128 (4 x 4 MFMA tiles each), direct-to-LDS loads ----
+// Per 64-wide K step a block stages A (256 limb rows) and B (256 digit rows), 64 bytes per row,
+// with global_load_lds_dwordx4: each load instruction fills 1 KB of LDS, lane j the 16 bytes at
+// +16 j, i.e. row j / 4, slot j % 4.  Row r's 16-byte chunk c goes to slot c ^ swz(r), so the 16
+// rows one fragment read touches per phase land on distinct banks (for 32- and 64-bank phases).
+// Three stages rotate, two steps of loads in flight: at step s every wave waits for its own loads
+// of step s, a bare barrier publishes them (and retires step s - 1's reads), step s + 2 is
+// issued, then step s's 16 fragments are read and its 32 MFMAs issued (the second K half's
+// reads interleaved with the first half's MFMAs).  Per step and
+// wave: 16 ds_read_b128 fragments for 32 MFMAs of 32 x 32 x 32 (1 M MACs per 16 KB of LDS reads,
+// against 0.54 M per 14 KB for the 48 x 176 tile of 16 x 16 x 64).  B rows past the last digit row
+// are never loaded: their slots are zeroed once.  What bounds it (m = 2^20, 64 witnesses): the
+// loads with their barriers alone take 14.3 ms and the fragment reads with the MFMAs alone 15.4 ms,
+// together 23.3 ms (2.1 POP/s; the MFMA pipes alone sustain 4.35 POP/s) -- the step's 32 KB of
+// staging (about 14 TB/s across the GPU, 62% L2 hits) and its LDS traffic overlap the MFMAs only
+// in part.
+typedef int v16i __attribute__((ext_vector_type(16)));
+constexpr int A3_BM = 256, A3_BN = 256, A3_THREADS = 256, A3_STAGES = 3;
+constexpr int A3_ROWS = A3_BM + A3_BN, A3_STAGE_BYTES = A3_ROWS * 64;
+constexpr int A3_MBLK = AJ_MROWS / A3_BM;  // 27
+constexpr int A3_LOADS = A3_ROWS * 4 / 64 / (A3_THREADS / 64);  // load instructions per wave per step: 8
+static size_t a3_groups(size_t nw) { return (nw * D + A3_BN - 1) / A3_BN; }
+__device__ __forceinline__ int a3_swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+__global__ __launch_bounds__(A3_THREADS) void k_ajtai_gemm32(const int8_t* __restrict__ limbs, size_t nsteps,
+                                                            const int8_t* const* __restrict__ Zs,
+                                                            const int8_t* __restrict__ Bt, size_t nw, size_t m,
+                                                            size_t npad, int groups, int S,
+                                                            int32_t* __restrict__ part) {
+  __shared__ __align__(16) int8_t lds[A3_STAGES * A3_STAGE_BYTES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned per_split = (unsigned)A3_MBLK * (unsigned)groups;
+  const unsigned split = lid / per_split, rem = lid % per_split;
+  const size_t m0 = (size_t)(rem / groups) * A3_BM;
+  const size_t n0 = (size_t)(rem % groups) * A3_BN;
+  const size_t s_begin = (size_t)split * nsteps / S, s_end = ((size_t)split + 1) * nsteps / S;
+  const size_t nrows_b = nw * D;
+  // zero the slots of B rows past the last digit row (never loaded) in every stage
+  for (int i = tid; i < A3_STAGES * A3_BN * 4; i += A3_THREADS) {
+    const int st = i / (A3_BN * 4), rb = (i >> 2) % A3_BN, p = i & 3;
+    if (n0 + rb >= nrows_b) *(v4i*)(lds + st * A3_STAGE_BYTES + (A3_BM + rb) * 64 + p * 16) = v4i{0, 0, 0, 0};
+  }
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // published by the first step's barrier
+  // this lane's source for each of the wave's load instructions (instruction q = wave * 8 + i
+  // covers stage rows 16 q .. 16 q + 15)
+  const int8_t* src[A3_LOADS];
+  bool on[A3_LOADS];
+  const size_t astep = (size_t)AJ_MROWS * 64, bstep = Bt ? nrows_b * 64 : 64;
+#pragma unroll
+  for (int i = 0; i < A3_LOADS; i++) {
+    const int row = (wave * A3_LOADS + i) * 16 + (lane >> 2), p = lane & 3;
+    if (row < A3_BM) {
+      const int c = p ^ a3_swz(row);
+      src[i] = limbs + (m0 + row) * 64 + c * 16;
+      on[i] = true;
+    } else {
+      const int rb = row - A3_BM, c = p ^ a3_swz(rb);
+      const size_t n = n0 + rb;
+      on[i] = n < nrows_b;
+      const size_t nn = on[i] ? n : 0;
+      src[i] = Bt ? Bt + nn * 64 + c * 16 : Zs[nn / D] + (nn % D) * m + c * 16;
+    }
+  }
+  auto load_stage = [&](size_t st, int buf) {
+    int8_t* base = lds + buf * A3_STAGE_BYTES;
+#pragma unroll
+    for (int i = 0; i < A3_LOADS; i++)
+      if (on[i])
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src[i] + st * ((wave * A3_LOADS + i) * 16 < A3_BM ? astep : bstep)),
+                                         (__attribute__((address_space(3))) void*)(base + (wave * A3_LOADS + i) * 1024),
+                                         16, 0, 0);
+  };
+  v16i acc[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[mt][nt][v] = 0;
+  const int r32 = lane & 31, khalf = lane >> 5;
+  auto compute = [&](int buf) {
+    const int8_t* A = lds + buf * A3_STAGE_BYTES;
+    const int8_t* B = A + A3_BM * 64;
+    v4i a[2][4], b[2][4];
+#pragma unroll
+    for (int kh = 0; kh < 2; kh++) {
+      const int c = kh * 2 + khalf;
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const int row = wm * 128 + mt * 32 + r32;
+        a[kh][mt] = *(const v4i*)(A + row * 64 + ((c ^ a3_swz(row)) << 4));
+      }
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        const int row = wn * 128 + nt * 32 + r32;
+        b[kh][nt] = *(const v4i*)(B + row * 64 + ((c ^ a3_swz(row)) << 4));
+      }
+    }
+#pragma unroll
+    for (int kh = 0; kh < 2; kh++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+          acc[mt][nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[kh][mt], b[kh][nt], acc[mt][nt], 0, 0, 0);
+    // the first half's 8 reads, then its 16 MFMAs with the second half's 8 reads interleaved (a
+    // read, two MFMAs), then the second half's 16 MFMAs: the second half's LDS latency hides
+    // behind the first half's matrix work
+#pragma unroll
+    for (int i = 0; i < 8; i++) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+  };
+  // load instructions this wave issues per step (one whose 16 rows are all padding is skipped)
+  int nload = 0;
+#pragma unroll
+  for (int i = 0; i < A3_LOADS; i++) nload += __ballot(on[i]) != 0;
+  nload = __builtin_amdgcn_readfirstlane(nload);
+  for (int k = 0; k < A3_STAGES - 1; k++)
+    if (s_begin + k < s_end) load_stage(s_begin + k, k);
+  int buf = 0;
+  for (size_t st = s_begin; st < s_end; st++) {
+    // step st's loads are done once at most the later steps' loads are outstanding
+    const size_t later = std::min<size_t>(A3_STAGES - 2, s_end - 1 - st);
+    switch ((int)later * nload) {
+#define A3_W(n) \
+  case n: asm volatile("s_waitcnt vmcnt(" #n ")" ::: "memory"); break;
+      A3_W(16) A3_W(15) A3_W(14) A3_W(13) A3_W(12) A3_W(11) A3_W(10) A3_W(9) A3_W(8) A3_W(7) A3_W(6) A3_W(5)
+      A3_W(4) A3_W(3) A3_W(2) A3_W(1)
+#undef A3_W
+      default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+    // a bare barrier: __syncthreads() would also wait for the later steps' loads
+    asm volatile("s_barrier" ::: "memory");
+    if (st + A3_STAGES - 1 < s_end) load_stage(st + A3_STAGES - 1, (buf + A3_STAGES - 1) % A3_STAGES);
+    compute(buf);
+    buf = (buf + 1) % A3_STAGES;
+  }
+  // D[i][j]: j = lane % 32, i = 8 (v / 4) + 4 (lane / 32) + v % 4
+  int32_t* base = part + ((size_t)split * AJ_MROWS + m0 + wm * 128) * npad + n0 + wn * 128;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const size_t row = mt * 32 + (v >> 2) * 8 + khalf * 4 + (v & 3);
+        base[row * npad + nt * 32 + r32] = acc[mt][nt][v];
+      }
+}
+
 // Digits to K-step-major order for large m: Bt[(step * R + r) * 64 + b] = Z_w[t][step * 64 + b]
 // with r = w * 54 + t and R = nw * 54 rows.  Block (row tile of 64, run of KM_STEPS steps): each
 // lane moves 16 bytes per step; per step the block writes 64 consecutive rows (4 KB contiguous)
@@ -379,72 +539,114 @@ __global__ __launch_bounds__(512) void k_ajtai_epilogue(const int32_t* __restric
   }
 }
 
-// K-split autotuner: the first commit of each (m, nw) in the process times every split count from
-// 1 to aj_max_splits(m) on its own inputs -- the GEMM and the reduction of its split planes, which
-// grows with the count (twice each, HIP events, the faster kept) -- and caches the fastest; later
-// commits use the cached count.  Every split count gives the same
-// commitments (exact int32 partial sums, at most 2^20 * 128 in magnitude), so the choice only
-// moves time.  A single lock serializes the first measurement of each shape.
-static size_t aj_tuned_splits(const int8_t* limbs, size_t m, const int8_t* const* Zs_dev, const int8_t* Bt, size_t nw,
-                              int32_t* part, int32_t* red, hipStream_t s) {
-  static std::mutex mu;
-  static std::map<std::pair<size_t, size_t>, size_t> chosen;
-  std::lock_guard<std::mutex> g(mu);
+// Tile shape and K-split autotuner: the first commit of each (m, nw) in the process times both
+// tile shapes (16 x 16 x 64 MFMAs in 192 x 352 blocks, 32 x 32 x 32 MFMAs in 256 x 256 blocks) at
+// every split count from 1 to aj_max_splits(m) on its own inputs -- the GEMM and the reduction of
+// its split planes, which grows with the count (twice each, HIP events, the faster kept) -- and
+// caches the fastest; later commits use the cached choice.  Every choice gives the same
+// commitments (exact int32 partial sums, at most 2^20 * 128 in magnitude), so it only moves
+// time.  A single lock serializes the first measurement of each shape.
+struct AjChoice {
+  int variant;  // 1: 16 x 16 x 64 tiles, 2: 32 x 32 x 32 tiles
+  size_t splits;
+};
+static std::mutex g_aj_mu;
+static std::map<std::pair<size_t, size_t>, AjChoice> g_aj_chosen;
+static int g_aj_force = 0;
+
+void ajtai_force_variant(int v) {
+  std::lock_guard<std::mutex> g(g_aj_mu);
+  if (v < 0 || v > 2) throw std::invalid_argument("Ajtai tile variant must be 0 (timed), 1 or 2");
+  g_aj_force = v;
+  g_aj_chosen.clear();
+}
+
+static void aj_launch(int variant, const int8_t* limbs, size_t nsteps, const int8_t* const* Zs_dev, const int8_t* Bt,
+                      size_t nw, size_t m, size_t sp, int32_t* part, hipStream_t s, bool timed) {
+  if (variant == 2) {
+    const size_t groups = a3_groups(nw), npad = groups * A3_BN;
+    const dim3 grid((unsigned)(A3_MBLK * groups * sp));
+    if (timed)
+      NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm32, grid, dim3(A3_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m,
+                      npad, (int)groups, (int)sp, part);
+    else
+      hipLaunchKernelGGL(k_ajtai_gemm32, grid, dim3(A3_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m, npad,
+                         (int)groups, (int)sp, part);
+  } else {
+    const size_t groups = aj_groups(nw), npad = groups * AJ_NBLK;
+    const dim3 grid((unsigned)(AJ_MBLK * groups * sp));
+    if (timed)
+      NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, grid, dim3(AJ_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m, npad,
+                      (int)groups, (int)sp, part);
+    else
+      hipLaunchKernelGGL(k_ajtai_gemm, grid, dim3(AJ_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m, npad,
+                         (int)groups, (int)sp, part);
+  }
+}
+static size_t aj_npad(int variant, size_t nw) {
+  return variant == 2 ? a3_groups(nw) * A3_BN : aj_groups(nw) * AJ_NBLK;
+}
+
+static AjChoice aj_tuned(const int8_t* limbs, size_t m, const int8_t* const* Zs_dev, const int8_t* Bt, size_t nw,
+                         int32_t* part, int32_t* red, hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_aj_mu);
   const auto key = std::make_pair(m, nw);
-  auto it = chosen.find(key);
-  if (it != chosen.end()) return it->second;
-  const size_t groups = aj_groups(nw), npad = groups * AJ_NBLK, nsteps = m / AJ_BK;
+  auto it = g_aj_chosen.find(key);
+  if (it != g_aj_chosen.end()) return it->second;
+  const size_t nsteps = m / AJ_BK;
   hipEvent_t e0, e1;
   NS_CHECK(hipEventCreate(&e0));
   NS_CHECK(hipEventCreate(&e1));
-  size_t best = aj_splits(m, nw);
+  AjChoice best{g_aj_force ? g_aj_force : 1, aj_splits(m, nw)};
   float best_ms = 1e30f;
-  for (size_t sp = 1; sp <= aj_max_splits(m); sp++) {
-    float ms = 1e30f;
-    for (int rep = 0; rep < 2; rep++) {
-      NS_CHECK(hipEventRecord(e0, s));
-      hipLaunchKernelGGL(k_ajtai_gemm, dim3((unsigned)(AJ_MBLK * groups * sp)), dim3(AJ_THREADS), 0, s, limbs, nsteps,
-                         Zs_dev, Bt, nw, m, npad, (int)groups, (int)sp, part);
-      if (sp > 1) launch_sum_splits(part, sp, (size_t)AJ_MROWS * npad, red, s);
-      NS_CHECK(hipEventRecord(e1, s));
-      NS_CHECK(hipEventSynchronize(e1));
-      float t = 0;
-      NS_CHECK(hipEventElapsedTime(&t, e0, e1));
-      ms = std::min(ms, t);
-    }
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = sp;
+  for (int variant = 1; variant <= 2; variant++) {
+    if (g_aj_force && variant != g_aj_force) continue;
+    for (size_t sp = 1; sp <= aj_max_splits(m); sp++) {
+      float ms = 1e30f;
+      for (int rep = 0; rep < 2; rep++) {
+        NS_CHECK(hipEventRecord(e0, s));
+        aj_launch(variant, limbs, nsteps, Zs_dev, Bt, nw, m, sp, part, s, false);
+        if (sp > 1) launch_sum_splits(part, sp, (size_t)AJ_MROWS * aj_npad(variant, nw), red, s);
+        NS_CHECK(hipEventRecord(e1, s));
+        NS_CHECK(hipEventSynchronize(e1));
+        float t = 0;
+        NS_CHECK(hipEventElapsedTime(&t, e0, e1));
+        ms = std::min(ms, t);
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = AjChoice{variant, sp};
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  chosen[key] = best;
+  g_aj_chosen[key] = best;
   return best;
 }
 
 void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* const* Zs_dev, size_t nw,
                   uint64_t* out, void* workspace, hipStream_t s) {
   if (nw == 0) return;
-  const size_t groups = aj_groups(nw), npad = groups * AJ_NBLK, spmax = aj_max_splits(m);
+  const size_t npad_ws = aj_npad_max(nw), spmax = aj_max_splits(m);
   const size_t nsteps = m / AJ_BK;
   // workspace: partial-sum planes for up to spmax splits | S | the split sums | K-major digits
   int32_t* part = (int32_t*)workspace;
-  int32_t* S = part + spmax * (size_t)AJ_MROWS * npad;
+  int32_t* S = part + spmax * (size_t)AJ_MROWS * npad_ws;
   hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
   int8_t* Bt = nullptr;
   if (aj_kmajor(m)) {
     // after the split-sum scratch (which follows S), 256-byte aligned
-    const size_t used = (spmax + (spmax > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512;
+    const size_t used = (spmax + (spmax > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad_ws * 4 + nw * D * 4 + 512;
     Bt = (int8_t*)(((uintptr_t)workspace + used + 255) & ~(uintptr_t)255);
     hipLaunchKernelGGL(k_digits_kmajor, dim3((unsigned)((nw * D + 63) / 64), (unsigned)((nsteps + KM_STEPS - 1) / KM_STEPS)),
                        dim3(256), 0, s, Zs_dev, nw, m, Bt);
   }
   // split-sum plane after S, 256-byte aligned
   int32_t* red = (int32_t*)(((uintptr_t)(S + nw * D) + 255) & ~(uintptr_t)255);
-  const size_t splits = aj_tuned_splits(limbs, m, Zs_dev, Bt, nw, part, red, s);
-  NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, dim3((unsigned)(AJ_MBLK * groups * splits)), dim3(AJ_THREADS), 0, s,
-                  limbs, nsteps, Zs_dev, Bt, nw, m, npad, (int)groups, (int)splits, part);
+  const AjChoice ch = aj_tuned(limbs, m, Zs_dev, Bt, nw, part, red, s);
+  const size_t splits = ch.splits, npad = aj_npad(ch.variant, nw);
+  aj_launch(ch.variant, limbs, nsteps, Zs_dev, Bt, nw, m, splits, part, s, true);
   const int32_t* G = part;
   if (splits > 1) {
     // reduce the K splits with coalesced loads before the anti-diagonal epilogue

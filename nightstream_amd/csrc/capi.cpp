@@ -214,6 +214,10 @@ int ns_ajtai_commit_batch_device(const ns_pp* pp, const int8_t* const* Z_dev, si
   });
 }
 
+int ns_ajtai_force_variant(int variant) {
+  return guard([&] { ajtai_force_variant(variant); });
+}
+
 static std::vector<K2> kvec(const uint64_t* p, size_t n) {
   std::vector<K2> v(n);
   for (size_t i = 0; i < n; i++) v[i] = K2{p[2 * i], p[2 * i + 1]};

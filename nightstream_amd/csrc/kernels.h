@@ -95,6 +95,8 @@ void launch_pp_limbs(const uint64_t* pp, size_t kappa, size_t m, int8_t* limbs, 
 void launch_chacha8_words(const uint32_t key[8], uint64_t first_block, size_t nblocks, uint32_t* out, hipStream_t s);
 size_t ajtai_workspace_bytes(size_t m, size_t nw);
 // Commit nw digit matrices (device pointers in Zs_dev, a device array of nw pointers).
+// 0: timed choice of tile shape and K splits; 1 / 2: force the 16x16x64 / 32x32x32 tiles
+void ajtai_force_variant(int v);
 void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* const* Zs_dev, size_t nw,
                   uint64_t* out /* nw x kappa x 54 */, void* workspace, hipStream_t s);
 
