@@ -208,6 +208,31 @@ int ns_prove_shard_mem(ns_prover* p, const uint64_t* const* z_steps, int z_on_de
  *      Returns 0 when the proof is accepted, NS_EPROTOCOL with the reason otherwise. */
 int ns_shard_verify(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_steps, size_t m_in,
                     const ns_step_mem* mem, const uint64_t* mcs_commitments, const uint8_t* proof, size_t proof_len);
+/* ---- output binding: fold_shard_prove_with_output_binding / fold_shard_verify_with_output_binding
+ *      (neo-fold/src/shard.rs:2352-2385, 3032-3108; OutputBindingConfig output_binding.rs:23-35,
+ *      ProgramIO neo-memory/src/output_check.rs:60-171).  On the last step, right after its memory
+ *      is absorbed, the prover runs the output sum-check over eq(r_addr, .) * io_mask *
+ *      (val_final - val_io) (num_bits rounds of degree 3) and adds the batched time claim
+ *      "output_binding/inc_total" linking the final state to the Twist increments of instance
+ *      mem_idx; the proof carries the round polynomials after the steps.  Requirements (else
+ *      NS_EINVAL): n_steps >= 1 with memory, mem_idx < n_mem of the last step, that instance's
+ *      k == 2^num_bits and d*ell == num_bits, num_bits <= 30, claim addresses strictly increasing
+ *      and < 2^num_bits.  final_state (prover only): 2^num_bits F_q values, the memory after the
+ *      last step; the verifier ignores it. */
+typedef struct {
+  size_t mem_idx;
+  size_t num_bits;
+  size_t n_claims;
+  const uint64_t* addrs;       /* claimed addresses, strictly increasing */
+  const uint64_t* values;      /* claimed values (F_q) */
+  const uint64_t* final_state; /* 2^num_bits F_q values (prover) or NULL (verifier) */
+} ns_output_binding;
+int ns_prove_shard_ob(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps, size_t m_in,
+                      const ns_step_mem* mem, const ns_output_binding* ob, uint8_t** proof, size_t* proof_len,
+                      double* seconds);
+int ns_shard_verify_ob(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_steps, size_t m_in,
+                       const ns_step_mem* mem, const ns_output_binding* ob, const uint64_t* mcs_commitments,
+                       const uint8_t* proof, size_t proof_len);
 /* Per-kernel timing of the next proves (HIP events on the prover's stream); classes:
  * 0 round-0 eval, 1 round-2 eval, 2 table fold, 3 Ajtai GEMM, 4 digit x K GEMM, 5 RLC Z_mix,
  * 6 DEC split, 7 round-1 eval, 8 fused fold + round (rounds >= 3).  ms[i] is the summed

@@ -25,7 +25,7 @@ EXPORTS = [
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
     "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ajtai_force_variant", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
-    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_shard_verify", "ns_rlc", "ns_dec", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
+    "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_shard_verify", "ns_prove_shard_ob", "ns_shard_verify_ob", "ns_rlc", "ns_dec", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
     "ns_poseidon2_permute",
     "ns_ts_product_new", "ns_ts_shout_value_new", "ns_ts_index_adapter_new", "ns_ts_bitness_new",
@@ -107,6 +107,9 @@ def _load():
     L.ns_prove_shard_mem.argtypes = [vp, vp, C.c_int, sz, sz, vp, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
                                      C.POINTER(C.c_double)]
     L.ns_shard_verify.argtypes = [vp, vp, sz, sz, vp, vp, vp, sz]
+    L.ns_prove_shard_ob.argtypes = [vp, vp, C.c_int, sz, sz, vp, vp, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(sz),
+                                    C.POINTER(C.c_double)]
+    L.ns_shard_verify_ob.argtypes = [vp, vp, sz, sz, vp, vp, vp, vp, sz]
     L.ns_rlc.argtypes = [vp, vp, sz, vp, vp, vp, vp]
     L.ns_dec.argtypes = [vp, vp, vp, sz, vp, sz, vp, vp]
     L.ns_prover_set_timing.argtypes = [vp, C.c_int]
@@ -446,10 +449,69 @@ def dec(session: "FoldingSession", parent, zmix_buf: "DeviceBuffer", k_dec: int,
     return [_me_py(arr[i], outs[i][1]) for i in range(k_dec)], tuple(bool(x) for x in ok)
 
 
-def shard_verify(ccs: "Ccs", xs, proof: bytes, mem=None, mcs_commitments=None, m_in: int = 1) -> None:
-    """ns_shard_verify: replay the transcript of a canonical shard proof and check it (host only).
-    xs: each step's public input (m_in values); mem: per-step memory instances or None.  Raises
-    NsError with the reason when the proof is rejected."""
+class ProgramIO:
+    """ProgramIO (neo-memory/src/output_check.rs:92-171): claimed (address, value) pairs, kept in
+    address order; a second claim at an address replaces the first (with_claim), or raises
+    (try_with_claim)."""
+
+    def __init__(self):
+        self.claims: dict[int, int] = {}
+
+    def with_claim(self, addr: int, value: int) -> "ProgramIO":
+        self.claims[int(addr)] = int(value) % Q
+        return self
+
+    with_output = with_claim
+    with_input = with_claim
+
+    def try_with_claim(self, addr: int, value: int) -> "ProgramIO":
+        if int(addr) in self.claims:
+            raise ValueError(f"Duplicate address {addr}")
+        return self.with_claim(addr, value)
+
+    def sorted_claims(self):
+        return sorted(self.claims.items())
+
+
+class OutputBindingConfig:
+    """OutputBindingConfig (neo-fold/src/output_binding.rs:23-60): the address width, the claims
+    and the index of the bound Twist instance among the last step's memory instances."""
+
+    def __init__(self, num_bits: int, program_io: ProgramIO, mem_idx: int = 0):
+        self.num_bits, self.program_io, self.mem_idx = num_bits, program_io, mem_idx
+
+    def with_mem_idx(self, mem_idx: int) -> "OutputBindingConfig":
+        self.mem_idx = mem_idx
+        return self
+
+
+class _ObC(C.Structure):
+    _fields_ = [("mem_idx", C.c_size_t), ("num_bits", C.c_size_t), ("n_claims", C.c_size_t),
+                ("addrs", C.c_void_p), ("values", C.c_void_p), ("final_state", C.c_void_p)]
+
+
+def _ob_c(cfg: OutputBindingConfig, final_state, keep: list) -> _ObC:
+    cl = cfg.program_io.sorted_claims()
+    a = np.array([x for x, _ in cl] or [0], np.uint64)
+    v = np.array([y for _, y in cl] or [0], np.uint64)
+    keep += [a, v]
+    fs = None
+    if final_state is not None:
+        fs = np.ascontiguousarray(np.asarray(final_state, np.uint64))
+        if fs.size != 1 << cfg.num_bits:
+            raise NsError(-1, f"final_memory_state.len()={fs.size} != 2^num_bits={1 << cfg.num_bits}")
+        keep.append(fs)
+    o = _ObC(cfg.mem_idx, cfg.num_bits, len(cl), a.ctypes.data, v.ctypes.data, None if fs is None else fs.ctypes.data)
+    keep.append(o)
+    return o
+
+
+def shard_verify(ccs: "Ccs", xs, proof: bytes, mem=None, mcs_commitments=None, m_in: int = 1,
+                 ob: "OutputBindingConfig | None" = None) -> None:
+    """ns_shard_verify (ns_shard_verify_ob when ob is given): replay the transcript of a canonical
+    shard proof and check it (host only).  xs: each step's public input (m_in values); mem: per-step
+    memory instances or None; ob: the output binding the proof must carry.  Raises NsError with the
+    reason when the proof is rejected."""
     xs = [np.ascontiguousarray(np.asarray(x, np.uint64)[:m_in]) for x in xs]
     keep = []
     marr = _step_mem_array(mem, keep) if mem is not None else None
@@ -457,9 +519,14 @@ def shard_verify(ccs: "Ccs", xs, proof: bytes, mem=None, mcs_commitments=None, m
     if mcs_commitments is not None:
         mc = np.ascontiguousarray(np.asarray(mcs_commitments, np.uint64).reshape(-1))
     buf = (C.c_uint8 * len(proof)).from_buffer_copy(proof)
-    _check(lib().ns_shard_verify(ccs.h, _ptr_array(xs), len(xs), m_in,
-                                 None if marr is None else C.cast(marr, C.c_void_p),
-                                 None if mc is None else mc.ctypes.data, buf, len(proof)))
+    mp = None if marr is None else C.cast(marr, C.c_void_p)
+    mcp = None if mc is None else mc.ctypes.data
+    if ob is None:
+        _check(lib().ns_shard_verify(ccs.h, _ptr_array(xs), len(xs), m_in, mp, mcp, buf, len(proof)))
+    else:
+        obc = _ob_c(ob, None, keep)
+        _check(lib().ns_shard_verify_ob(ccs.h, _ptr_array(xs), len(xs), m_in, mp, C.byref(obc), mcp, buf,
+                                        len(proof)))
 
 
 class FoldingSession:
@@ -504,10 +571,28 @@ class FoldingSession:
         self.add_step_io(z[:m_in], z[m_in:], mem)
 
     def fold_and_prove(self) -> bytes:
+        return self._prove_host(None, None)
+
+    def fold_and_prove_with_output_binding(self, ob_cfg: OutputBindingConfig, final_memory_state) -> bytes:
+        """fold_shard_prove_with_output_binding (shard.rs:2352-2385): the last step also proves that
+        Twist instance ob_cfg.mem_idx ends in final_memory_state (2^num_bits values) and that the
+        claimed outputs match it."""
+        if not self.mem or any(x is None for x in self.mem):
+            raise NsError(-1, "output binding requires memory instances on every step")
+        return self._prove_host(ob_cfg, final_memory_state)
+
+    def _prove_host(self, ob_cfg, final_state) -> bytes:
         out = C.POINTER(C.c_uint8)()
         n = C.c_size_t()
         sec = C.c_double()
-        if any(x is not None for x in self.mem):
+        if ob_cfg is not None:
+            keep = []
+            arr = _step_mem_array(self.mem, keep)
+            obc = _ob_c(ob_cfg, final_state, keep)
+            _check(lib().ns_prove_shard_ob(self.h, _ptr_array(self.steps), 0, len(self.steps), self.m_in or 0,
+                                           C.cast(arr, C.c_void_p), C.byref(obc), C.byref(out), C.byref(n),
+                                           C.byref(sec)))
+        elif any(x is not None for x in self.mem):
             if any(x is None for x in self.mem):
                 raise NsError(-1, "every step needs its memory instances once one step has them")
             keep = []
@@ -522,15 +607,25 @@ class FoldingSession:
         self.last_seconds = sec.value
         return data
 
-    def prove_device(self, z_ptrs, m_in: int = 1, mem=None) -> bytes:
+    def prove_device(self, z_ptrs, m_in: int = 1, mem=None, ob=None, final_state=None) -> bytes:
         """fold_and_prove over step witnesses already resident in HBM (device pointers to u64[m]);
         mem: per-step memory instances (Route A; a list or a StepMemArray covering at least the
-        proved steps) or None."""
+        proved steps) or None; ob / final_state: output binding on the last step, or None."""
         arr = (C.c_void_p * len(z_ptrs))(*z_ptrs)
         out = C.POINTER(C.c_uint8)()
         n = C.c_size_t()
         sec = C.c_double()
-        if mem is not None:
+        if ob is not None:
+            if mem is None:
+                raise NsError(-1, "output binding requires memory instances")
+            sm = mem if isinstance(mem, StepMemArray) else StepMemArray(mem)
+            if len(sm) < len(z_ptrs):
+                raise ValueError("fewer memory steps than witnesses")
+            keep = []
+            obc = _ob_c(ob, final_state, keep)
+            _check(lib().ns_prove_shard_ob(self.h, arr, 1, len(z_ptrs), m_in, C.cast(sm.arr, C.c_void_p),
+                                           C.byref(obc), C.byref(out), C.byref(n), C.byref(sec)))
+        elif mem is not None:
             sm = mem if isinstance(mem, StepMemArray) else StepMemArray(mem)
             if len(sm) < len(z_ptrs):
                 raise ValueError("fewer memory steps than witnesses")

@@ -149,9 +149,11 @@ class _Rows:
         return np.cumsum(ptr).astype(np.uint64), c.astype(np.uint32), v.astype(np.uint64)
 
 
-def gen_bus_shard(log_chunk: int, nsteps: int, seed: int):
+def gen_bus_shard(log_chunk: int, nsteps: int, seed: int, with_final: bool = False):
     """The CCS (an R1CS with n = m = 128 * 2^log_chunk), one witness z per step and the step's
-    memory instances.  Returns (R1CS, [z], [StepMem], BusLayout)."""
+    memory instances.  Returns (R1CS, [z], [StepMem], BusLayout), and with with_final also the
+    memory after the last step per Twist instance ({mem_idx: u64[k]}, the final_memory_state of
+    output binding)."""
     c = 1 << log_chunk
     n = m = 128 * c
     rng = np.random.default_rng(seed)
@@ -318,7 +320,13 @@ def gen_bus_shard(log_chunk: int, nsteps: int, seed: int):
                                    MemInst(32, 1, 32, c, REG_ELL, list(reg_init)),
                                    MemInst(K_RAM, 1, K_RAM, c, RAM_ELL, ram_init)]))
     inst.z = zs[0]
-    return inst, zs, mems, lay
+    if not with_final:
+        return inst, zs, mems, lay
+    reg_final = _fq(regs)
+    ram_final = np.zeros(K_RAM, np.uint64)
+    for a, v in ram.items():
+        ram_final[a] = v % Q
+    return inst, zs, mems, lay, {0: reg_final, 1: reg_final.copy(), 2: ram_final}
 
 
 def r1cs_satisfied(inst: R1CS, z: np.ndarray) -> bool:

@@ -433,62 +433,92 @@ int ns_prove_shard_device(ns_prover* p, const uint64_t* const* z_steps_dev, size
   });
 }
 
-int ns_prove_shard_mem(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps, size_t m_in,
-                       const ns_step_mem* mem, uint8_t** proof, size_t* proof_len, double* seconds) {
-  return guard([&] {
-    if (!p || !z_steps || !proof || !proof_len) throw Error(1, "null argument");
-    if (n_steps == 0) throw Error(1, "need at least one step");
-    // the per-step instances as the prover's host structs (the arrays stay the caller's)
-    std::vector<std::vector<LutInstH>> luts(n_steps);
-    std::vector<std::vector<MemInstH>> mems(n_steps);
-    std::vector<StepMemH> steps(n_steps);
+// the per-step instances as the prover's host structs (the arrays stay the caller's)
+struct StepMemConv {
+  std::vector<std::vector<LutInstH>> luts;
+  std::vector<std::vector<MemInstH>> mems;
+  std::vector<StepMemH> steps;
+  StepMemConv(const ns_step_mem* mem, size_t n_steps, bool check) : luts(n_steps), mems(n_steps), steps(n_steps) {
     for (size_t s = 0; mem && s < n_steps; s++) {
       for (size_t i = 0; i < mem[s].n_lut; i++) {
         const ns_lut_inst& a = mem[s].luts[i];
-        if (!a.table && a.k) throw Error(1, "LUT instance without a table");
-        if (a.d * a.ell > 32 || a.k != ((size_t)1 << (a.d * a.ell))) throw Error(1, "LUT table size must be 2^(d*ell)");
+        if (check && !a.table && a.k) throw Error(1, "LUT instance without a table");
+        if (check && (a.d * a.ell > 32 || a.k != ((size_t)1 << (a.d * a.ell))))
+          throw Error(1, "LUT table size must be 2^(d*ell)");
         luts[s].push_back(LutInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.table});
       }
       for (size_t i = 0; i < mem[s].n_mem; i++) {
         const ns_mem_inst& a = mem[s].mems[i];
-        if (a.d * a.ell > 32) throw Error(1, "Twist address width above 32 bits");
-        for (size_t q = 1; a.init_sparse && q < a.ninit; q++)
-          if (a.init_addrs[q] <= a.init_addrs[q - 1]) throw Error(1, "MemInit::Sparse must be strictly increasing by address");
+        if (check && a.d * a.ell > 32) throw Error(1, "Twist address width above 32 bits");
+        for (size_t q = 1; check && a.init_sparse && q < a.ninit; q++)
+          if (a.init_addrs[q] <= a.init_addrs[q - 1])
+            throw Error(1, "MemInit::Sparse must be strictly increasing by address");
         mems[s].push_back(MemInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.init_sparse != 0,
                                    a.init_sparse ? a.ninit : 0, a.init_addrs, a.init_vals});
       }
       steps[s] = StepMemH{luts[s].size(), luts[s].data(), mems[s].size(), mems[s].data()};
     }
-    std::vector<uint64_t> enc = p->p->prove_shard(z_steps, n_steps, m_in, z_on_device != 0, mem ? steps.data() : nullptr);
-    uint8_t* buf = (uint8_t*)malloc(enc.size() * 8);
-    if (!buf) throw Error(1, "out of host memory");
-    memcpy(buf, enc.data(), enc.size() * 8);
-    *proof = buf;
-    *proof_len = enc.size() * 8;
-    if (seconds) *seconds = p->p->last_seconds;
+  }
+};
+
+static ObCfgH ob_in(const ns_output_binding& a) {
+  ObCfgH o;
+  o.mem_idx = a.mem_idx;
+  o.num_bits = a.num_bits;
+  o.n_claims = a.n_claims;
+  o.addrs = a.addrs;
+  o.values = a.values;
+  o.final_state = a.final_state;
+  return o;
+}
+
+static void prove_shard_impl(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps,
+                             size_t m_in, const ns_step_mem* mem, const ns_output_binding* ob, uint8_t** proof,
+                             size_t* proof_len, double* seconds) {
+  if (!p || !z_steps || !proof || !proof_len) throw Error(1, "null argument");
+  if (n_steps == 0) throw Error(1, "need at least one step");
+  StepMemConv cv(mem, n_steps, true);
+  ObCfgH oc;
+  if (ob) oc = ob_in(*ob);
+  std::vector<uint64_t> enc = p->p->prove_shard(z_steps, n_steps, m_in, z_on_device != 0,
+                                                mem ? cv.steps.data() : nullptr, ob ? &oc : nullptr);
+  uint8_t* buf = (uint8_t*)malloc(enc.size() * 8);
+  if (!buf) throw Error(1, "out of host memory");
+  memcpy(buf, enc.data(), enc.size() * 8);
+  *proof = buf;
+  *proof_len = enc.size() * 8;
+  if (seconds) *seconds = p->p->last_seconds;
+}
+
+int ns_prove_shard_mem(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps, size_t m_in,
+                       const ns_step_mem* mem, uint8_t** proof, size_t* proof_len, double* seconds) {
+  return guard([&] { prove_shard_impl(p, z_steps, z_on_device, n_steps, m_in, mem, nullptr, proof, proof_len, seconds); });
+}
+
+int ns_prove_shard_ob(ns_prover* p, const uint64_t* const* z_steps, int z_on_device, size_t n_steps, size_t m_in,
+                      const ns_step_mem* mem, const ns_output_binding* ob, uint8_t** proof, size_t* proof_len,
+                      double* seconds) {
+  return guard([&] {
+    if (!ob) throw Error(1, "null output binding");
+    prove_shard_impl(p, z_steps, z_on_device, n_steps, m_in, mem, ob, proof, proof_len, seconds);
   });
 }
 
 int ns_shard_verify(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_steps, size_t m_in,
                     const ns_step_mem* mem, const uint64_t* mcs_commitments, const uint8_t* proof, size_t proof_len) {
+  return ns_shard_verify_ob(s, x_steps, n_steps, m_in, mem, nullptr, mcs_commitments, proof, proof_len);
+}
+
+int ns_shard_verify_ob(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_steps, size_t m_in,
+                       const ns_step_mem* mem, const ns_output_binding* ob, const uint64_t* mcs_commitments,
+                       const uint8_t* proof, size_t proof_len) {
   return guard([&] {
     if (!s || !proof || (n_steps && !x_steps)) throw Error(1, "null argument");
-    std::vector<std::vector<LutInstH>> luts(n_steps);
-    std::vector<std::vector<MemInstH>> mems(n_steps);
-    std::vector<StepMemH> steps(n_steps);
-    for (size_t st = 0; mem && st < n_steps; st++) {
-      for (size_t i = 0; i < mem[st].n_lut; i++) {
-        const ns_lut_inst& a = mem[st].luts[i];
-        luts[st].push_back(LutInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.table});
-      }
-      for (size_t i = 0; i < mem[st].n_mem; i++) {
-        const ns_mem_inst& a = mem[st].mems[i];
-        mems[st].push_back(MemInstH{a.k, a.d, a.n_side, a.steps, a.ell, a.init_sparse != 0,
-                                    a.init_sparse ? a.ninit : 0, a.init_addrs, a.init_vals});
-      }
-      steps[st] = StepMemH{luts[st].size(), luts[st].data(), mems[st].size(), mems[st].data()};
-    }
-    verify_shard(*s->s, x_steps, n_steps, m_in, mem ? steps.data() : nullptr, mcs_commitments, proof, proof_len);
+    StepMemConv cv(mem, n_steps, false);
+    ObCfgH oc;
+    if (ob) oc = ob_in(*ob);
+    verify_shard(*s->s, x_steps, n_steps, m_in, mem ? cv.steps.data() : nullptr, mcs_commitments, proof, proof_len,
+                 ob ? &oc : nullptr);
   });
 }
 

@@ -1551,7 +1551,7 @@ static void enc_lane(Enc& pb, const LaneOut& o, size_t k_dec) {
 }
 
 std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t nsteps, size_t m_in,
-                                          bool z_on_device, const StepMemH* mem) {
+                                          bool z_on_device, const StepMemH* mem, const ObCfgH* ob) {
   using clk = std::chrono::steady_clock;
   std::unique_ptr<KTimers> kt;
   if (timing) {
@@ -1570,6 +1570,10 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   if (!params_.extension_check((uint32_t)ell, (uint32_t)d_sc, &smin, &slack) || slack < 0)
     throw Error(2, "extension policy failed");
   if (m_in > m) throw Error(1, "m_in exceeds m");
+  if (ob) {
+    if (nsteps == 0 || !mem) throw Error(1, "output binding requires >= 1 step with memory");
+    ob_validate(*ob, mem[nsteps - 1]);
+  }
   hipStream_t st = stream_;
   auto t0 = clk::now();
   for (int i = 0; i < NPHASE; i++) phase_s[i] = 0, phase_perm[i] = 0;
@@ -1653,11 +1657,22 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
   std::vector<TwistCols> prev_tw;
   bool have_prev = false;
   std::vector<uint64_t> zbus;
+  std::vector<std::vector<K2>> ob_rounds;  // OutputSumcheckProof.round_polys
+  std::vector<K2> ob_r_prime;
 
   for (size_t idx = 0; idx < nsteps; idx++) {
     const StepMemH& sm = mem ? mem[idx] : no_mem;
     const size_t nacc = acc.size(), kt = nacc + 1;
     absorb_step_memory(tr, sm, digests);
+    // output binding on the last step, before r_time is sampled (shard.rs:1803-1862)
+    const bool include_ob = ob && idx + 1 == nsteps;
+    if (include_ob) {
+      tr.append_msg_u64("shard/output_binding_start", idx);
+      const uint64_t mi = ob->mem_idx, nb = ob->num_bits;
+      tr.append_u64s("output_binding/mem_idx", &mi, 1);
+      tr.append_u64s("output_binding/num_bits", &nb, 1);
+      ob_prove(tr, *ob, ob_rounds, ob_r_prime);
+    }
     phase_mark(0);
     bind_step_header(tr, s, params_, ell, d_sc, slack, x_host[idx].data(), m_in, &cmcs[idx * KAPPA * D], acc);
     const StepChallenges ch = sample_challenges(tr, ell_d, ell);
@@ -1699,6 +1714,21 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
       shout_addr_pre(tr, sm, L, zbus.data(), ell_n, r_cycle, idx, sp);
       twist_addr_pre(tr, sm, L, zbus.data(), ell_n, r_cycle, tp);
       build_time_claims(sm, ell_n, r_cycle, sp, tp, mem_owned, claims);
+    }
+    if (include_ob) {
+      // TwistTotalIncOracleSparseTime at r', the last batched time claim (shard.rs:1990-2004)
+      const TwistCols& T = tp[ob->mem_idx].cols;
+      mem_owned.push_back(std::make_unique<TsOracleGpu>());
+      TsOracleGpu* o = mem_owned.back().get();
+      o->init_time(TSK_TOTAL_INC, {}, ell_n, &T.has_write, nullptr, &T.inc, T.wa, ob_r_prime, {}, kzero(), nullptr,
+                   nullptr, nullptr);
+      RaClaim c;
+      c.ts = o;
+      c.label = "output_binding/inc_total";
+      c.dyn = true;
+      c.sum = o->claim();
+      c.deg = o->degree_bound();
+      claims.push_back(c);
     }
     phase_mark(2);
     // the batched time sum-check: the CCS claim and the memory time lanes (route_a_time.rs:28-152)
@@ -1929,6 +1959,17 @@ std::vector<uint64_t> Prover::prove_shard(const uint64_t* const* z_steps, size_t
     phase_mark(11);
   }
   phase_mark(-1);
+  // ShardProof.output_proof (shard_proof_types.rs:165-171)
+  if (ob) {
+    pb.u(1);
+    pb.u(ob_rounds.size());
+    for (auto& c : ob_rounds) {
+      pb.u(c.size());
+      for (auto& x : c) pb.k(x);
+    }
+  } else {
+    pb.u(0);
+  }
   pb.u(acc.size());
   for (auto& e : acc) pb.me(e);
   pb.u(val_obl.size());

@@ -141,6 +141,7 @@ struct OracleWs {
 };
 
 struct StepMemH;
+struct ObCfgH;
 struct BusLayout;
 
 // Fiat-Shamir steps shared by the prover and the verifier (same labels and absorb order)
@@ -177,8 +178,11 @@ class Prover {
   // z_on_device: z_steps are device pointers (inputs already resident in HBM).
   // mem (optional, one entry per step): the step's Twist/Shout instances over the shared CPU bus
   // (Route A memory sidecar, neo-fold/src/memory_sidecar/memory.rs).
+  // ob (optional): output binding on the last step (fold_shard_prove_with_output_binding,
+  // neo-fold/src/shard.rs:2352-2385); the proof then carries the output sum-check.
   std::vector<uint64_t> prove_shard(const uint64_t* const* z_steps, size_t nsteps, size_t m_in,
-                                    bool z_on_device = false, const StepMemH* mem = nullptr);
+                                    bool z_on_device = false, const StepMemH* mem = nullptr,
+                                    const ObCfgH* ob = nullptr);
   double last_seconds = 0;
   // Per-kernel-class timing of the last prove (HIP events on the prover stream).
   bool timing = false;

@@ -219,7 +219,7 @@ static std::vector<K2> bitness_weights(const std::vector<K2>& r_cycle, size_t n,
 }
 
 // eval_init_at_r_addr (neo-memory/src/mem_init.rs:133-165)
-static K2 eval_init(const MemInstH& in, const std::vector<K2>& r_addr) {
+K2 eval_init(const MemInstH& in, const std::vector<K2>& r_addr) {
   if (!in.sparse) return kzero();
   K2 acc = kzero();
   for (size_t q = 0; q < in.ninit; q++) {
@@ -522,6 +522,87 @@ void val_eval_batch(Transcript& tr, const StepMemH& sm, const std::vector<TwistP
   run_batched(tr, "twist/val_eval_batch", step_idx, out.claims, ell_n, out.r_val);
   for (auto& c : out.claims) c.ts = nullptr;
   tr.append_message("twist/val_eval/batch_done", (const uint8_t*)nullptr, 0);
+}
+
+// ---------------- output binding (output_check.rs, shard.rs:1803-1862) ----------------
+void ob_validate(const ObCfgH& ob, const StepMemH& sm) {
+  if (ob.mem_idx >= sm.n_mem) throw Error(1, "output binding mem_idx out of range");
+  if (ob.num_bits > 30) throw Error(1, "output binding: num_bits above 30");
+  const MemInstH& mi = sm.mems[ob.mem_idx];
+  if (mi.k != ((size_t)1 << ob.num_bits)) throw Error(1, "output binding: 2^num_bits differs from the Twist k");
+  if (mi.d * mi.ell != ob.num_bits) throw Error(1, "output binding: num_bits differs from the Twist ell_addr");
+  if (ob.n_claims && (!ob.addrs || !ob.values)) throw Error(1, "output binding: claims missing");
+  for (size_t i = 0; i < ob.n_claims; i++) {
+    if (ob.addrs[i] >> ob.num_bits) throw Error(1, "output binding: claimed address out of range");
+    if (i && ob.addrs[i] <= ob.addrs[i - 1]) throw Error(1, "output binding: claims must be increasing by address");
+    if (ob.values[i] >= Q) throw Error(1, "output binding: non-canonical claimed value");
+  }
+}
+
+// chi(r, a) = prod_i (bit i of a ? r_i : 1 - r_i) for every a < 2^|r| (output_check.rs:179-185)
+static std::vector<K2> chi_table(const std::vector<K2>& r) {
+  std::vector<K2> w((size_t)1 << r.size(), kzero());
+  w[0] = kone();
+  for (size_t i = 0; i < r.size(); i++) {
+    const size_t step = (size_t)1 << i;
+    const K2 om = ksub(kone(), r[i]);
+    for (size_t j = 0; j < step; j++) {
+      const K2 v = w[j];
+      w[j] = kmul(v, om);
+      w[j + step] = kmul(v, r[i]);
+    }
+  }
+  return w;
+}
+
+static K2 ob_challenge(Transcript& tr) { return tr.challenge_k("output_check/chal/re", "output_check/chal/im"); }
+
+std::vector<K2> ob_sample_r_addr(Transcript& tr, const ObCfgH& ob) {
+  tr.append_msg_u64("output_check/num_claims", ob.n_claims);
+  for (size_t i = 0; i < ob.n_claims; i++) {
+    tr.append_msg_u64("output_check/addr", ob.addrs[i]);
+    tr.append_fields("output_check/value", &ob.values[i], 1);
+  }
+  std::vector<K2> r(ob.num_bits);
+  for (size_t i = 0; i < ob.num_bits; i++) {
+    tr.append_msg_u64("output_check/r_addr/idx", i);
+    r[i] = ob_challenge(tr);
+  }
+  return r;
+}
+
+// The round polynomial of eq * mask * (val_final - val_io) equals that of the product of the
+// three multilinear factors eq, mask and d = val_final - val_io (d's extension is the difference
+// of the two extensions), so the four tables of the reference become three device factors.
+void ob_prove(Transcript& tr, const ObCfgH& ob, std::vector<std::vector<K2>>& rounds, std::vector<K2>& r_prime) {
+  if (!ob.final_state) throw Error(1, "output binding: final memory state missing");
+  const std::vector<K2> r_addr = ob_sample_r_addr(tr, ob);
+  const size_t k = (size_t)1 << ob.num_bits;
+  std::vector<std::vector<K2>> fac(3);
+  fac[0] = chi_table(r_addr);
+  fac[1].assign(k, kzero());
+  fac[2].resize(k);
+  for (size_t a = 0; a < k; a++) {
+    if (ob.final_state[a] >= Q) throw Error(1, "output binding: non-canonical final memory value");
+    fac[2][a] = K2{ob.final_state[a], 0};
+  }
+  for (size_t i = 0; i < ob.n_claims; i++) {
+    fac[1][ob.addrs[i]] = kone();
+    fac[2][ob.addrs[i]] = ksub(fac[2][ob.addrs[i]], K2{ob.values[i], 0});
+  }
+  rounds.assign(ob.num_bits, {});
+  r_prime.assign(ob.num_bits, kzero());
+  if (ob.num_bits == 0) return;
+  TsOracleGpu o;
+  o.init_product(fac, 3);
+  for (size_t rd = 0; rd < ob.num_bits; rd++) {
+    std::vector<K2>& c = rounds[rd];
+    o.round_coeffs(c);
+    c.resize(4, kzero());
+    for (auto& x : c) tr.append_k("output_check/round_coeff", x);
+    r_prime[rd] = ob_challenge(tr);
+    if (rd + 1 < ob.num_bits) o.fold(r_prime[rd]);
+  }
 }
 
 }  // namespace ns

@@ -32,6 +32,25 @@ struct StepMemH {
   const MemInstH* mems = nullptr;
 };
 
+// OutputBindingConfig (neo-fold/src/output_binding.rs:23-35) with the ProgramIO claims
+// (neo-memory/src/output_check.rs:60-171) and the prover's final memory state of instance mem_idx.
+struct ObCfgH {
+  size_t mem_idx = 0, num_bits = 0, n_claims = 0;
+  const uint64_t* addrs = nullptr;        // strictly increasing, < 2^num_bits
+  const uint64_t* values = nullptr;       // F_q
+  const uint64_t* final_state = nullptr;  // 2^num_bits F_q values (prover only)
+};
+// eval_init_at_r_addr (neo-memory/src/mem_init.rs:133-165): the initial memory at r_addr
+K2 eval_init(const MemInstH& in, const std::vector<K2>& r_addr);
+// Checks of the binding against the last step's Twist instance and the claims (shard.rs:1820-1846,
+// output_check.rs:146-160); throws Error(1, ...) on a violation.
+void ob_validate(const ObCfgH& ob, const StepMemH& sm);
+// The ProgramIO absorb and r_addr sampling (output_check.rs:162-171, 260-277).
+std::vector<K2> ob_sample_r_addr(Transcript& tr, const ObCfgH& ob);
+// generate_output_sumcheck_proof_and_challenges (output_check.rs:623-634) on the device: the
+// summand eq(r_addr, .) * io_mask * (val_final - val_io) as a three-factor product oracle.
+void ob_prove(Transcript& tr, const ObCfgH& ob, std::vector<std::vector<K2>>& rounds, std::vector<K2>& r_prime);
+
 // BusLayout (neo-memory/src/cpu/bus_layout.rs:29-56)
 struct BusLayout {
   struct Sh {
@@ -124,7 +143,8 @@ void val_eval_batch(Transcript& tr, const StepMemH& sm, const std::vector<TwistP
 
 // fold_shard_verify over the canonical encoding (verify.cpp): returns when the proof is accepted,
 // throws Error(7, reason) otherwise.  mcs_c: expected MCS commitments (n_steps x kappa*D) or null.
+// ob: the output binding the proof must carry (fold_shard_verify_with_output_binding), or null.
 void verify_shard(const Ccs& s, const uint64_t* const* x_steps, size_t nsteps, size_t m_in, const StepMemH* mem,
-                  const uint64_t* mcs_c, const uint8_t* proof, size_t proof_len);
+                  const uint64_t* mcs_c, const uint8_t* proof, size_t proof_len, const ObCfgH* ob = nullptr);
 
 }  // namespace ns

@@ -16,7 +16,11 @@
 //   - Pi_RLC (api.rs:333-453): parent X, y (core and bus rows) and commitment recomputed from the
 //     inputs and the re-derived rotations; Pi_DEC (api.rs:458-587): y, X and c recompose from the
 //     children in base 2, children share r and the digest;
-//   - the obligations: main = the last step's children, val = every step's val-lane children.
+//   - the obligations: main = the last step's children, val = every step's val-lane children;
+//   - output binding, when supplied (shard.rs:2566-2614, 2863-2925): the output sum-check from
+//     claim zero, the inc_total claim's terminal value against the Twist time openings at r_time,
+//     and eq(r_addr, r') * io_mask(r') * (val_init(r') + inc_total - val_io(r')) against the
+//     output sum-check's final value.
 // Not checked: the terminal identities of the memory time lanes (their oracles' final values
 // against the bus openings, memory.rs:1671-2380) and the Ajtai openings of commitments (the
 // verifier holds no witness).
@@ -321,7 +325,11 @@ K2 terminal_rhs(const Ccs& s, const std::vector<MeHost>& out, const K2* r_prime,
 
 // Returns normally when the proof is accepted; throws Error(7, reason) otherwise.
 void verify_shard(const Ccs& s, const uint64_t* const* x_steps, size_t nsteps, size_t m_in, const StepMemH* mem,
-                  const uint64_t* mcs_c, const uint8_t* proof, size_t proof_len) {
+                  const uint64_t* mcs_c, const uint8_t* proof, size_t proof_len, const ObCfgH* ob) {
+  if (ob) {
+    if (nsteps == 0 || !mem) throw Error(1, "output binding requires >= 1 step with memory");
+    ob_validate(*ob, mem[nsteps - 1]);
+  }
   try {
     need(proof_len % 8 == 0, "proof length is not a multiple of 8");
     Reader R{(const uint64_t*)proof, proof_len / 8};
@@ -451,9 +459,41 @@ void verify_shard(const Ccs& s, const uint64_t* const* x_steps, size_t nsteps, s
       MeHost vparent;
       std::vector<MeHost> vchildren;
       if (has_val) read_lane(vrhos, vparent, vchildren);
+      // ShardProof.output_proof follows the last step (shard_proof_types.rs:165-171)
+      const bool include_ob = ob && idx + 1 == nsteps;
+      std::vector<std::vector<K2>> ob_polys;
+      if (idx + 1 == nsteps) {
+        const bool has_ob = R.u() != 0;
+        need(has_ob == (ob != nullptr), has_ob ? "proof has output binding, but no binding was supplied"
+                                               : "output binding supplied, but the proof has none");
+        if (has_ob) ob_polys = R.rounds();
+      }
 
       // ---- replay ----
       absorb_step_memory(tr, sm, digests);
+      // output binding (shard.rs:2566-2614): the output sum-check from claim zero, its state kept
+      // for the final check below
+      std::vector<K2> ob_r_prime, ob_r_addr;
+      K2 ob_final = kzero();
+      if (include_ob) {
+        tr.append_msg_u64("shard/output_binding_start", idx);
+        const uint64_t mi = ob->mem_idx, nb = ob->num_bits;
+        tr.append_u64s("output_binding/mem_idx", &mi, 1);
+        tr.append_u64s("output_binding/num_bits", &nb, 1);
+        ob_r_addr = ob_sample_r_addr(tr, *ob);
+        need(ob_polys.size() == ob->num_bits, at + ": output sum-check round count");
+        K2 run = kzero();
+        for (size_t rd = 0; rd < ob_polys.size(); rd++) {
+          const std::vector<K2>& co = ob_polys[rd];
+          need(co.size() == 4, at + ": output sum-check round degree");
+          need(keq(kadd(peval(co, kzero()), peval(co, kone())), run), at + ": output sum-check round p(0) + p(1)");
+          for (auto& x : co) tr.append_k("output_check/round_coeff", x);
+          const K2 r = tr.challenge_k("output_check/chal/re", "output_check/chal/im");
+          ob_r_prime.push_back(r);
+          run = peval(co, r);
+        }
+        ob_final = run;
+      }
       if (mcs_c) need(!memcmp(ccs_out[0].c, mcs_c + idx * KAPPA * D, sizeof(ccs_out[0].c)), at + ": MCS commitment");
       bind_step_header(tr, s, params, ell, d_sc, slack, x_steps[idx], m_in, ccs_out[0].c, acc);
       const StepChallenges ch = sample_challenges(tr, ell_d, ell);
@@ -542,6 +582,10 @@ void verify_shard(const Ccs& s, const uint64_t* const* x_steps, size_t nsteps, s
         want_lb.insert(want_lb.end(), {"twist/read_check", "twist/write_check", "twist/bitness"});
         want_deg.insert(want_deg.end(), {e + 3, e + 3, 3});
       }
+      if (include_ob) {
+        want_lb.push_back("output_binding/inc_total");
+        want_deg.push_back(2 + ob->num_bits);
+      }
       need(bt_labels == want_lb, at + ": batched time labels drift");
       need(bt_degs == want_deg, at + ": batched time degree bounds drift");
       need(keq(bt_sums[0], T), at + ": CCS time claim");
@@ -596,6 +640,29 @@ void verify_shard(const Ccs& s, const uint64_t* const* x_steps, size_t nsteps, s
         need(!memcmp(e.fold_digest, fd, 32), at + ": ccs_out digest");
         if (i >= 1)
           need(!memcmp(e.c, acc[i - 1].c, sizeof(e.c)) && e.X == acc[i - 1].X, at + ": carried c / X");
+      }
+      if (include_ob) {
+        // inc_total's terminal value from the Twist time openings at r_time, then
+        // val_final(r') = val_init(r') + inc_total and the output sum-check's final value
+        // (shard.rs:2863-2925, output_binding.rs:63-81)
+        const BusLayout::Tw& tw = L.tw[ob->mem_idx];
+        const std::vector<K2>& ys = ccs_out[0].y_scalars;
+        need(ys.size() == t_tot && tw.ell == ob->num_bits, at + ": Twist time openings");
+        K2 inc_term = kmul(ys[t + tw.has_write], ys[t + tw.inc]);
+        for (size_t b = 0; b < tw.ell; b++) inc_term = kmul(inc_term, eq_lin(ys[t + tw.wa0 + b], ob_r_prime[b]));
+        need(keq(tcl.back().running, inc_term), at + ": inc_total terminal mismatch");
+        const size_t nb = ob->num_bits;
+        K2 mask = kzero(), vio = kzero();
+        for (size_t q = 0; q < ob->n_claims; q++) {
+          K2 chi = kone();
+          for (size_t b = 0; b < nb; b++)
+            chi = kmul(chi, ((ob->addrs[q] >> b) & 1) ? ob_r_prime[b] : ksub(kone(), ob_r_prime[b]));
+          mask = kadd(mask, chi);
+          vio = kadd(vio, kscale(chi, ob->values[q]));
+        }
+        const K2 eq_eval = eq_pts(ob_r_addr.data(), ob_r_prime.data(), nb);
+        const K2 val_final = kadd(eval_init(sm.mems[ob->mem_idx], ob_r_prime), bt_sums.back());
+        need(keq(kmul(kmul(eq_eval, mask), ksub(val_final, vio)), ob_final), at + ": output binding final check failed");
       }
       const K2 rhs = terminal_rhs(s, ccs_out, r_time.data(), ell_n, a_prime.data(), ch, ell_d,
                                   acc.empty() ? nullptr : &acc[0].r);
@@ -665,6 +732,7 @@ void verify_shard(const Ccs& s, const uint64_t* const* x_steps, size_t nsteps, s
       tr.append_msg_u64("fold/step_done", idx);
       acc = std::move(children);
     }
+    if (nsteps == 0) need(R.u() == 0, "output binding without steps");
     need(R.u() == acc.size(), "main obligations count");
     for (auto& e : acc) need(me_eq(R.me(), e), "main obligations differ from the last children");
     need(R.u() == val_all.size(), "val obligations count");

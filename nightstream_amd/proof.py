@@ -22,6 +22,7 @@ stream, field-for-field in the order of the Rust structs, so proofs compare bit-
       batched_time: claimed_sums (count, K*), degree_bounds (count, u64*), labels (count, (len, bytes)*),
                     round_polys (count, rounds*)
       val_fold: 0 (None) | 1, rlc_rhos, rlc_parent, dec_children (as in FoldStep)
+    ShardProof.output_proof: 0 (None) | 1, output_sc.round_polys (rounds; 4 coefficients each)
     ShardObligations: main (count, MeInstance*), val (count, MeInstance*)
   MeInstance (neo-ccs/src/relations.rs): c (kappa*D words, col-major), m_in, X (D*m_in),
     r (count, K*), y (t, (64, K*64)*), y_scalars (t, K*), fold_digest (4 words),
@@ -96,6 +97,7 @@ class ShardProof:
     steps: list
     final_main: list
     final_val: list = field(default_factory=list)
+    output_rounds: list | None = None  # OutputSumcheckProof.round_polys, when output binding is on
 
 
 class _Reader:
@@ -211,8 +213,9 @@ def parse(data: bytes) -> ShardProof:
             val_fold = (vrhos, vparent, [R.me() for _ in range(R.u())])
         steps.append(StepProof(ccs_out, rounds, init, chals, alpha, beta_a, beta_r, gamma, final, hd, rhos, parent,
                                children, sums, degs, labels, polys, mem, val_fold))
+    output_rounds = R.rounds() if R.u() else None
     final_main = [R.me() for _ in range(R.u())]
     final_val = [R.me() for _ in range(R.u())]
     if R.i != len(R.w):
         raise ValueError("trailing words after the proof")
-    return ShardProof(steps, final_main, final_val)
+    return ShardProof(steps, final_main, final_val, output_rounds)

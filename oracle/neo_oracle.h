@@ -183,6 +183,15 @@ typedef struct {
   const fq* init_vals;
 } or_mem_inst;
 typedef struct { size_t n_lut; const or_lut_inst* luts; size_t n_mem; const or_mem_inst* mems; } or_step_mem;
+/* output binding of a shard's final memory (neo-fold/src/output_binding.rs:23-95,
+ * neo-memory/src/output_check.rs): claims[i] = (addrs[i], values[i]) over the Twist instance
+ * mem_idx of the last step, whose final state has 2^num_bits entries */
+typedef struct {
+  size_t mem_idx, num_bits, n_claims;
+  const uint64_t* addrs; /* strictly increasing, < 2^num_bits */
+  const fq* values;
+  const fq* final_state;
+} or_ob_config;
 typedef struct or_prover or_prover;
 or_prover* or_prover_new(const or_ccs* s, const uint8_t pp_seed[32], int n_threads);
 void or_prover_free(or_prover* p);
@@ -194,6 +203,11 @@ int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t
  * be NULL (no instances in any step). */
 int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem* mem, size_t nsteps, size_t m_in,
                        uint8_t** out, size_t* out_len, double* seconds_per_step);
+/* The same with output binding on the last step (fold_shard_prove_with_output_binding,
+ * neo-fold/src/shard.rs:2352-2385); ob may be NULL. */
+int or_prove_shard_mem_ob(or_prover* p, const fq* const* z_steps, const or_step_mem* mem, size_t nsteps,
+                          size_t m_in, const or_ob_config* ob, uint8_t** out, size_t* out_len,
+                          double* seconds_per_step);
 void or_free(void* ptr);
 /* Wall time of fold step i of the last or_prove_shard call (the MCS commits of all steps are
  * done before step 0 and counted in neither). */

@@ -90,6 +90,10 @@ def lib():
         L.or_prove_shard_mem.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
                                          C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t),
                                          C.POINTER(C.c_double)]
+        L.or_prove_shard_mem_ob.restype = C.c_int
+        L.or_prove_shard_mem_ob.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                            C.c_void_p, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t),
+                                            C.POINTER(C.c_double)]
         L.or_last_error.restype = C.c_char_p
         L.or_last_step_seconds.restype = C.c_double
         L.or_last_step_seconds.argtypes = [C.c_size_t]
@@ -302,6 +306,26 @@ def step_mem_structs(mems, keep: list):
     return arr
 
 
+class ObConfigC(C.Structure):
+    """or_ob_config: the output-binding claims (ProgramIO) and the final memory state."""
+    _fields_ = [("mem_idx", C.c_size_t), ("num_bits", C.c_size_t), ("n_claims", C.c_size_t),
+                ("addrs", C.c_void_p), ("values", C.c_void_p), ("final_state", C.c_void_p)]
+
+
+def ob_struct(ob, keep):
+    """ob: dict(mem_idx, num_bits, claims=[(addr, value)], final_state=array of 2^num_bits)."""
+    claims = sorted(ob["claims"])
+    a = np.array([x for x, _ in claims] or [0], np.uint64)
+    v = np.array([y for _, y in claims] or [0], np.uint64)
+    fs = np.ascontiguousarray(ob["final_state"], np.uint64)
+    if fs.size != 1 << ob["num_bits"]:
+        raise ValueError("final_state must hold 2^num_bits values")
+    keep += [a, v, fs]
+    c = ObConfigC(ob["mem_idx"], ob["num_bits"], len(claims), a.ctypes.data, v.ctypes.data, fs.ctypes.data)
+    keep.append(c)
+    return c
+
+
 class Prover:
     """CPU shard prover (fold loop restatement); proves with the canonical proof encoding."""
 
@@ -311,16 +335,18 @@ class Prover:
         if not self.ptr:
             raise RuntimeError(lib().or_last_error().decode())
 
-    def prove(self, zs, m_in=1, mem=None):
-        """mem: per-step memory instances (Route A over the shared CPU bus), or None."""
+    def prove(self, zs, m_in=1, mem=None, ob=None):
+        """mem: per-step memory instances (Route A over the shared CPU bus), or None.
+        ob: output binding on the last step (see ob_struct), or None."""
         zs = [np.ascontiguousarray(z, np.uint64) for z in zs]
         out = C.POINTER(C.c_uint8)()
         n = C.c_size_t()
         sec = C.c_double()
         keep = []
         marr = step_mem_structs(mem, keep) if mem is not None else None
-        rc = lib().or_prove_shard_mem(self.ptr, _ptrs(zs), None if marr is None else C.cast(marr, C.c_void_p),
-                                      len(zs), m_in, C.byref(out), C.byref(n), C.byref(sec))
+        obs = C.byref(ob_struct(ob, keep)) if ob is not None else None
+        rc = lib().or_prove_shard_mem_ob(self.ptr, _ptrs(zs), None if marr is None else C.cast(marr, C.c_void_p),
+                                         len(zs), m_in, obs, C.byref(out), C.byref(n), C.byref(sec))
         if rc != 0:
             raise RuntimeError(f"oracle prove failed ({rc}): {lib().or_last_error().decode()}")
         data = bytes(C.cast(out, C.POINTER(C.c_uint8 * n.value)).contents)

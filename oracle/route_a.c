@@ -709,3 +709,90 @@ int ra_bus_openings_i32(const int32_t* Z, const ra_layout* L, const kx* r, size_
   free(w);
   return 0;
 }
+
+/* ---------------- output binding: the output sum-check (output_check.rs) ---------------- */
+static kx ob_challenge(or_tr* tr) {
+  /* sample_k_challenge (output_check.rs:20-25) */
+  const fq re = or_tr_challenge_field(tr, "output_check/chal/re");
+  const fq im = or_tr_challenge_field(tr, "output_check/chal/im");
+  kx k = {re, im};
+  return k;
+}
+
+int ra_output_sumcheck(or_tr* tr, const or_ob_config* ob, kx* rounds, kx* r_prime, char* err, size_t errlen) {
+  const size_t nb = ob->num_bits;
+  if (nb > 30) {
+    snprintf(err, errlen, "output binding: num_bits above 30");
+    return -1;
+  }
+  const size_t k = (size_t)1 << nb;
+  for (size_t i = 0; i < ob->n_claims; i++)
+    if (ob->addrs[i] >= k || (i && ob->addrs[i] <= ob->addrs[i - 1])) {
+      snprintf(err, errlen, "output binding: claimed addresses must be increasing and below 2^num_bits");
+      return -1;
+    }
+  /* ProgramIO::absorb_into_transcript (output_check.rs:162-171): claims in address order */
+  ra_msg_u64(tr, "output_check/num_claims", ob->n_claims);
+  for (size_t i = 0; i < ob->n_claims; i++) {
+    ra_msg_u64(tr, "output_check/addr", ob->addrs[i]);
+    or_tr_append_fields(tr, "output_check/value", &ob->values[i], 1);
+  }
+  kx* r_addr = (kx*)malloc((nb + 1) * sizeof(kx));
+  for (size_t i = 0; i < nb; i++) {
+    ra_msg_u64(tr, "output_check/r_addr/idx", i);
+    r_addr[i] = ob_challenge(tr);
+  }
+  /* tables (output_check.rs:319-357): chi(r_addr), the claim mask, the final state, the claims */
+  kx* eq = (kx*)malloc(k * sizeof(kx));
+  kx* mask = (kx*)calloc(k, sizeof(kx));
+  kx* vf = (kx*)malloc(k * sizeof(kx));
+  kx* vio = (kx*)calloc(k, sizeof(kx));
+  or_chi_table(r_addr, nb, eq);
+  for (size_t a = 0; a < k; a++) vf[a] = or_k_from(ob->final_state[a]);
+  for (size_t i = 0; i < ob->n_claims; i++) {
+    mask[ob->addrs[i]] = or_k_from(1);
+    vio[ob->addrs[i]] = or_k_from(ob->values[i]);
+  }
+  /* run_sumcheck_prover (output_check.rs:600-620): evaluations at 0..3 (LSB-first pairs),
+   * interpolated, each coefficient absorbed, then the challenge */
+  kx xs[4], ys[4];
+  for (int x = 0; x < 4; x++) xs[x] = or_k_from((fq)x);
+  size_t n = k;
+  for (size_t rd = 0; rd < nb; rd++) {
+    const size_t half = n / 2;
+    for (int x = 0; x < 4; x++) {
+      kx acc = or_k_from(0);
+      for (size_t i = 0; i < half; i++) {
+        const size_t a0 = 2 * i, a1 = a0 + 1;
+        const kx X = xs[x];
+        const kx e = or_k_add(eq[a0], or_k_mul(or_k_sub(eq[a1], eq[a0]), X));
+        const kx mk = or_k_add(mask[a0], or_k_mul(or_k_sub(mask[a1], mask[a0]), X));
+        const kx f = or_k_add(vf[a0], or_k_mul(or_k_sub(vf[a1], vf[a0]), X));
+        const kx v = or_k_add(vio[a0], or_k_mul(or_k_sub(vio[a1], vio[a0]), X));
+        acc = or_k_add(acc, or_k_mul(or_k_mul(e, mk), or_k_sub(f, v)));
+      }
+      ys[x] = acc;
+    }
+    kx* co = &rounds[rd * 4];
+    or_interpolate(xs, ys, 4, co);
+    for (int c = 0; c < 4; c++) tr_k(tr, "output_check/round_coeff", co[c]);
+    const kx r = ob_challenge(tr);
+    r_prime[rd] = r;
+    for (size_t i = 0; i < half; i++) {
+      const size_t a0 = 2 * i, a1 = a0 + 1;
+      eq[i] = or_k_add(eq[a0], or_k_mul(or_k_sub(eq[a1], eq[a0]), r));
+      mask[i] = or_k_add(mask[a0], or_k_mul(or_k_sub(mask[a1], mask[a0]), r));
+      vf[i] = or_k_add(vf[a0], or_k_mul(or_k_sub(vf[a1], vf[a0]), r));
+      vio[i] = or_k_add(vio[a0], or_k_mul(or_k_sub(vio[a1], vio[a0]), r));
+    }
+    n = half;
+  }
+  free(r_addr);
+  free(eq);
+  free(mask);
+  free(vf);
+  free(vio);
+  (void)err;
+  (void)errlen;
+  return 0;
+}

@@ -82,5 +82,9 @@ void ra_bind_dynamic_claims(or_tr* tr, const ra_claim* cl, size_t n);
 int ra_val_eval(or_tr* tr, const or_step_mem* sm, const ra_twist_pre* tp, const ra_twist_cols* prev, size_t ell_n,
                 const kx* r_time, size_t step_idx, ra_val_pre* out, char* err, size_t errlen);
 void ra_val_pre_free(ra_val_pre* v);
+/* output sum-check of output binding (output_check.rs:252-640): absorbs the claims, samples
+ * r_addr, proves sum_k eq(r_addr, k) mask(k) (final(k) - claimed(k)) = 0 over num_bits rounds;
+ * rounds: num_bits x 4 coefficients, r_prime: the round challenges */
+int ra_output_sumcheck(or_tr* tr, const or_ob_config* ob, kx* rounds, kx* r_prime, char* err, size_t errlen);
 int ra_bus_openings_i32(const int32_t* Z, const ra_layout* L, const kx* r, size_t ell_n, kx* y, kx* ys);
 #endif

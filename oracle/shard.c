@@ -626,12 +626,27 @@ static void ser_lane(u64buf* b, const lane_out* o, size_t k_dec) {
 
 int or_prove_shard(or_prover* p, const fq* const* z_steps, size_t nsteps, size_t m_in, uint8_t** out,
                    size_t* out_len, double* seconds_per_step) {
-  return or_prove_shard_mem(p, z_steps, NULL, nsteps, m_in, out, out_len, seconds_per_step);
+  return or_prove_shard_mem_ob(p, z_steps, NULL, nsteps, m_in, NULL, out, out_len, seconds_per_step);
 }
 
 int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem* mem, size_t nsteps, size_t m_in,
                        uint8_t** out, size_t* out_len, double* seconds_per_step) {
+  return or_prove_shard_mem_ob(p, z_steps, mem, nsteps, m_in, NULL, out, out_len, seconds_per_step);
+}
+
+/* fold_shard_prove(_with_output_binding) (shard.rs:1725-2283, 2352-2385): the output sum-check
+ * runs on the last step right after its memory is absorbed (shard.rs:1803-1862), and its Twist
+ * link is the extra batched time claim "output_binding/inc_total" (shard.rs:1990-2004). */
+int or_prove_shard_mem_ob(or_prover* p, const fq* const* z_steps, const or_step_mem* mem, size_t nsteps,
+                          size_t m_in, const or_ob_config* ob, uint8_t** out, size_t* out_len,
+                          double* seconds_per_step) {
   const or_ccs* s = p->s;
+  if (ob && (nsteps == 0 || !mem)) {
+    snprintf(g_err, sizeof(g_err), "output binding requires >= 1 step with memory");
+    return -1;
+  }
+  kx* ob_rounds = ob ? (kx*)calloc(ob->num_bits * 4 + 1, sizeof(kx)) : NULL;
+  kx* ob_rprime = ob ? (kx*)calloc(ob->num_bits + 1, sizeof(kx)) : NULL;
   size_t m = s->m, t = s->t, k_dec = p->params.k_rho;
   size_t ell_d, ell_n, ell, d_sc;
   dims(p, &ell_d, &ell_n, &ell, &d_sc);
@@ -687,6 +702,29 @@ int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem
   for (size_t idx = 0; idx < nsteps; idx++) {
     const or_step_mem* sm = mem ? &mem[idx] : &no_mem;
     ra_absorb_step_memory(&tr, sm);
+    const int include_ob = ob && idx + 1 == nsteps;
+    if (include_ob) {
+      if (ob->mem_idx >= sm->n_mem) {
+        snprintf(g_err, sizeof(g_err), "output binding mem_idx out of range");
+        rc = -1;
+        break;
+      }
+      const or_mem_inst* mi = &sm->mems[ob->mem_idx];
+      if (ob->num_bits > 30 || mi->k != ((size_t)1 << ob->num_bits) || mi->d * mi->ell != ob->num_bits) {
+        snprintf(g_err, sizeof(g_err), "output binding: num_bits does not match the Twist instance");
+        rc = -1;
+        break;
+      }
+      ra_msg_u64(&tr, "shard/output_binding_start", idx);
+      uint64_t v = ob->mem_idx;
+      or_tr_append_u64s(&tr, "output_binding/mem_idx", &v, 1);
+      v = ob->num_bits;
+      or_tr_append_u64s(&tr, "output_binding/num_bits", &v, 1);
+      if (ra_output_sumcheck(&tr, ob, ob_rounds, ob_rprime, g_err, sizeof(g_err))) {
+        rc = -1;
+        break;
+      }
+    }
 
     size_t k = nacc + 1;
     /* bind_header_and_instances_with_digest */
@@ -840,7 +878,7 @@ int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem
     free(zdec);
     /* the batched time sum-check: CCS claim, then Shout and Twist time lanes (route_a_time.rs:28-152) */
     const size_t nmem_cl = 3 * (sm->n_lut + sm->n_mem);
-    ra_claim* cl = (ra_claim*)calloc(1 + nmem_cl, sizeof(ra_claim));
+    ra_claim* cl = (ra_claim*)calloc(2 + nmem_cl, sizeof(ra_claim));
     cl[0].kind = RA_ROWSTATE;
     cl[0].o = rs;
     cl[0].deg = d_sc;
@@ -851,7 +889,22 @@ int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem
       rc = -7;
       break;
     }
-    const size_t ncl = 1 + ncl_mem;
+    size_t ncl = 1 + ncl_mem;
+    if (include_ob) {
+      /* TwistTotalIncOracleSparseTime at r' (shard.rs:1990-2004), last in the batch */
+      const ra_twist_pre* P = &tp[ob->mem_idx];
+      or_sparse_in wa[64];
+      for (size_t b = 0; b < P->ell_addr; b++) wa[b] = ra_in(&P->cols.wa[b]);
+      kx s_ob;
+      cl[ncl].o = or_ts_total_inc(ell_n, wa, P->ell_addr, ra_in(&P->cols.has_write), ra_in(&P->cols.inc), ob_rprime,
+                                  &s_ob);
+      cl[ncl].kind = RA_TS;
+      cl[ncl].deg = or_ts_degree_bound((or_ts*)cl[ncl].o);
+      cl[ncl].label = "output_binding/inc_total";
+      cl[ncl].dynamic = 1;
+      cl[ncl].claimed_sum = s_ob;
+      ncl++;
+    }
     cl[0].claimed_sum = T;
     ra_bind_dynamic_claims(&tr, cl, ncl);
     kx* r_time = (kx*)malloc(ell_n * sizeof(kx));
@@ -1161,6 +1214,19 @@ int or_prove_shard_mem(or_prover* p, const fq* const* z_steps, const or_step_mem
   clock_gettime(CLOCK_MONOTONIC, &t1);
   if (seconds_per_step)
     *seconds_per_step = ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) / (double)nsteps;
+  /* ShardProof.output_proof (shard_proof_types.rs:165-171): 0 | 1, round_polys */
+  if (ob) {
+    ub_push(&pb, 1);
+    ub_push(&pb, ob->num_bits);
+    for (size_t rd = 0; rd < ob->num_bits; rd++) {
+      ub_push(&pb, 4);
+      for (int c = 0; c < 4; c++) ub_k(&pb, ob_rounds[rd * 4 + c]);
+    }
+  } else {
+    ub_push(&pb, 0);
+  }
+  free(ob_rounds);
+  free(ob_rprime);
   /* ShardObligations { main, val } */
   ub_push(&pb, nacc);
   for (size_t a = 0; a < nacc; a++) ser_me(&pb, &acc[a]);
