@@ -692,6 +692,17 @@ def bench_ajtai(args):
             c = pp.commit_device(bufs)
         t = (time.perf_counter() - t0) / reps
         repeat_ok = np.array_equal(c, first)
+        # each tile shape alone (its own K-split tuning), then back to the timed choice
+        shape_ms = {}
+        for v, name in ((1, "16x16x64"), (2, "32x32x32")):
+            ns.ajtai_force_variant(v)
+            cv = pp.commit_device(bufs)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                cv = pp.commit_device(bufs)
+            shape_ms[name] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+            repeat_ok &= np.array_equal(cv, first)
+        ns.ajtai_force_variant(0)
         cols = np.sort(rng.choice(m, size=64, replace=False))
         zsub = rng.integers(-1, 2, size=(54, 64), dtype=np.int8)
         zs = np.zeros((54, m), np.int8)
@@ -709,6 +720,7 @@ def bench_ajtai(args):
         ok &= repeat_ok and sparse_ok and dense_ok is not False
         per_m[f"2^{lm}"] = {"ms_per_64": round(t * 1e3, 3), "witnesses_per_s": round(nw / t, 2),
                             "int8_TOPS": round(2 * AJTAI_MACS_PER_COLUMN * m * nw / t / 1e12, 1),
+                            "tile_shape_ms_per_64": shape_ms,
                             "pp_setup_s": round(t_setup, 3), "sparse_parity": sparse_ok,
                             **({"dense_parity": dense_ok} if dense_ok is not None else {})}
         print(f"ajtai m=2^{lm}: {per_m[f'2^{lm}']}", file=sys.stderr, flush=True)

@@ -17,6 +17,10 @@ struct ns_ccs {
 };
 struct ns_pp {
   std::unique_ptr<Pp> pp;
+  // commit-call buffers kept across calls (hipMalloc / hipFree of the GEMM's split planes cost
+  // more than the GEMM itself at small batches); one call at a time per handle
+  std::mutex mu;
+  DevBuf ptrs, outd, ws;
 };
 struct ns_prover {
   std::unique_ptr<Prover> p;
@@ -159,6 +163,20 @@ int ns_decomp_b(const uint64_t* z, size_t m, uint32_t b, int8_t* Z_out) {
   });
 }
 
+// the batched commitment of n_wit device digit matrices with the handle's cached buffers
+static void commit_on(ns_pp* pp, const int8_t* const* Z_dev, size_t n_wit, uint64_t* out, hipStream_t s) {
+  const size_t m = pp->pp->m;
+  std::lock_guard<std::mutex> g(pp->mu);
+  pp->ptrs.ensure(n_wit * sizeof(void*));
+  pp->outd.ensure(n_wit * KAPPA * D * 8);
+  pp->ws.ensure(ajtai_workspace_bytes(m, n_wit));
+  NS_CHECK(hipMemcpyAsync(pp->ptrs.p, Z_dev, n_wit * sizeof(void*), hipMemcpyHostToDevice, s));
+  ajtai_commit(pp->pp->limbs.as<int8_t>(), KAPPA, m, pp->ptrs.as<const int8_t*>(), n_wit, pp->outd.as<uint64_t>(),
+               pp->ws.p, s);
+  NS_CHECK(hipMemcpyAsync(out, pp->outd.p, n_wit * KAPPA * D * 8, hipMemcpyDeviceToHost, s));
+  NS_CHECK(hipStreamSynchronize(s));
+}
+
 static std::vector<DevBuf> upload_digits(const int8_t* const* Z, size_t n, size_t m, hipStream_t s) {
   std::vector<DevBuf> out;
   for (size_t i = 0; i < n; i++) {
@@ -187,12 +205,7 @@ int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit,
     auto bufs = upload_digits(Z, n_wit, m, s);
     std::vector<const int8_t*> ptrs;
     for (auto& b : bufs) ptrs.push_back(b.as<int8_t>());
-    DevBuf ptrs_d(n_wit * sizeof(void*)), outd(n_wit * KAPPA * D * 8), ws(ajtai_workspace_bytes(m, n_wit));
-    NS_CHECK(hipMemcpyAsync(ptrs_d.p, ptrs.data(), n_wit * sizeof(void*), hipMemcpyHostToDevice, s));
-    ajtai_commit(pp->pp->limbs.as<int8_t>(), KAPPA, m, ptrs_d.as<const int8_t*>(), n_wit, outd.as<uint64_t>(), ws.p,
-                 s);
-    NS_CHECK(hipMemcpyAsync(out, outd.p, n_wit * KAPPA * D * 8, hipMemcpyDeviceToHost, s));
-    NS_CHECK(hipStreamSynchronize(s));
+    commit_on(const_cast<ns_pp*>(pp), ptrs.data(), n_wit, out, s);
   });
 }
 
@@ -203,14 +216,7 @@ int ns_ajtai_commit_batch_device(const ns_pp* pp, const int8_t* const* Z_dev, si
     if (n_wit && (!Z_dev || !out)) throw Error(1, "null witness array or output");
     for (size_t i = 0; i < n_wit; i++)
       if (!Z_dev[i]) throw Error(1, "null device witness");
-    const size_t m = pp->pp->m;
-    hipStream_t s = default_stream();
-    DevBuf ptrs_d(n_wit * sizeof(void*)), outd(n_wit * KAPPA * D * 8), ws(ajtai_workspace_bytes(m, n_wit));
-    NS_CHECK(hipMemcpyAsync(ptrs_d.p, Z_dev, n_wit * sizeof(void*), hipMemcpyHostToDevice, s));
-    ajtai_commit(pp->pp->limbs.as<int8_t>(), KAPPA, m, ptrs_d.as<const int8_t*>(), n_wit, outd.as<uint64_t>(), ws.p,
-                 s);
-    NS_CHECK(hipMemcpyAsync(out, outd.p, n_wit * KAPPA * D * 8, hipMemcpyDeviceToHost, s));
-    NS_CHECK(hipStreamSynchronize(s));
+    commit_on(const_cast<ns_pp*>(pp), Z_dev, n_wit, out, default_stream());
   });
 }
 
@@ -349,7 +355,7 @@ int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* out
     if (k_dec == 0 || k_dec > (size_t)DEC_MAX_PLANES) throw Error(1, "k_dec out of range");
     hipStream_t s = default_stream();
     const size_t len = (size_t)D * m;
-    DevBuf zd(len * 4), flags((k_dec + 1) * 4);
+    DevBuf zd(len * 4), flags(4);
     std::vector<DevBuf> planes;
     DecPlanes pl{};
     for (size_t i = 0; i < k_dec; i++) {
@@ -358,13 +364,13 @@ int ns_dec_split(const int32_t* zmix, size_t m, size_t k_dec, int8_t* const* out
     }
     NS_CHECK(hipMemcpyAsync(zd.p, zmix, len * 4, hipMemcpyHostToDevice, s));
     launch_dec_split(zd.as<int32_t>(), len, (int)k_dec, pl, flags.as<int>(), s);
-    std::vector<int> fl(k_dec + 1);
-    NS_CHECK(hipMemcpyAsync(fl.data(), flags.p, (k_dec + 1) * 4, hipMemcpyDeviceToHost, s));
+    uint32_t mask = 0;
+    NS_CHECK(hipMemcpyAsync(&mask, flags.p, 4, hipMemcpyDeviceToHost, s));
     NS_CHECK(hipStreamSynchronize(s));
-    if (fl[k_dec]) throw Error(6, "DEC split: value out of range for k_rho digits");
+    if (dec_out_of_range(mask, (int)k_dec)) throw Error(6, "DEC split: value out of range for k_rho digits");
     for (size_t i = 0; i < k_dec; i++) {
       NS_CHECK(hipMemcpy(outs[i], planes[i].p, len, hipMemcpyDeviceToHost));
-      nonzero[i] = fl[i] ? 1 : 0;
+      nonzero[i] = dec_plane_nonzero(mask, (int)i) ? 1 : 0;
     }
   });
 }
@@ -382,11 +388,11 @@ int ns_dec_split_device_noalloc(const int32_t* zmix_dev, size_t m, size_t k_dec,
     DecPlanes pl{};
     for (size_t i = 0; i < k_dec; i++) pl.p[i] = outs_dev[i];
     launch_dec_split(zmix_dev, (size_t)D * m, (int)k_dec, pl, flags, s);
-    int fl[DEC_MAX_PLANES + 1];
-    NS_CHECK(hipMemcpyAsync(fl, flags, (k_dec + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    uint32_t mask = 0;
+    NS_CHECK(hipMemcpyAsync(&mask, flags, sizeof(mask), hipMemcpyDeviceToHost, s));
     NS_CHECK(hipStreamSynchronize(s));
-    if (fl[k_dec]) throw Error(6, "DEC split: value out of range for k_rho digits");
-    for (size_t i = 0; i < k_dec; i++) nonzero[i] = fl[i] ? 1 : 0;
+    if (dec_out_of_range(mask, (int)k_dec)) throw Error(6, "DEC split: value out of range for k_rho digits");
+    for (size_t i = 0; i < k_dec; i++) nonzero[i] = dec_plane_nonzero(mask, (int)i) ? 1 : 0;
   });
 }
 

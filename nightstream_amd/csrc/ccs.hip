@@ -1232,16 +1232,20 @@ void launch_rlc_zmix(const int8_t* rho_dev, const int8_t* const* Zs_dev, size_t 
 
 // Balanced base-2 digits (neo-reductions/src/common.rs:75-268): repeatedly r = v % 2 (the sign
 // of v when v is odd), v = (v - r) / 2.  Both steps keep the sign of v and halve |v|, so digit i
-// is sign(v) * bit i of |v|, and v is in range exactly when |v| < 2^k_dec.  Each thread takes 16
-// consecutive entries (four 16-byte loads) and writes one 16-byte word per plane; plane i is
+// is sign(v) * bit i of |v|, and v is in range exactly when |v| < 2^k_dec.  Each thread takes two
+// runs of 16 consecutive entries (four 16-byte loads each) and writes one 16-byte word per plane
+// and run (two runs per thread: 65% of HBM peak at 2^16 x 54 entries, against 61% with one); plane i is
 // non-zero exactly when bit i of the OR of all |v| is set, so one OR word per block carries every
 // flag.  vec = 0 (an unaligned pointer from a caller, or len not a multiple of 16) takes byte
 // stores instead.
+constexpr int DEC_ITER = 2;
 __global__ __launch_bounds__(256) void k_dec_split(const int32_t* __restrict__ zmix, size_t len, int k_dec,
                                                    DecPlanes outs, int* __restrict__ flags, int vec) {
-  const size_t base = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
   uint32_t orv = 0;
-  if (base < len) {
+  // DEC_ITER runs of 16 entries per thread, a block's runs one block-width apart
+  for (int it = 0; it < DEC_ITER; it++) {
+    const size_t base = (((size_t)blockIdx.x * DEC_ITER + it) * blockDim.x + threadIdx.x) * 16;
+    if (base >= len) break;
     const bool full = vec && base + 16 <= len;
     uint32_t a[16], negw[4] = {0, 0, 0, 0};
     if (full) {
@@ -1291,10 +1295,10 @@ __global__ __launch_bounds__(256) void k_dec_split(const int32_t* __restrict__ z
   if (threadIdx.x == 0) {
     uint32_t o = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) o |= red[w];
-    // most blocks find the flags already raised: skip the contended atomic then
-    for (int i = 0; i < k_dec; i++)
-      if (((o >> i) & 1u) && __atomic_load_n(&flags[i], __ATOMIC_RELAXED) == 0) atomicOr(&flags[i], 1);
-    if (o >> k_dec) atomicOr(&flags[k_dec], 1);
+    // one word for every plane: most blocks find their bits already set and skip the contended
+    // atomic after a single load (one load per plane made each block's tail a dozen L2 round trips)
+    uint32_t* mask = reinterpret_cast<uint32_t*>(flags);
+    if (o & ~__atomic_load_n(mask, __ATOMIC_RELAXED)) atomicOr(mask, o);
   }
 }
 
@@ -1306,18 +1310,18 @@ __global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
 void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, const DecPlanes& outs, int* flags, hipStream_t s) {
   if (k_dec < 1 || k_dec > DEC_MAX_PLANES) throw std::invalid_argument("DEC split: 1..30 planes");
   // flags start at zero: a fill kernel, not a memset node (callers may capture this sequence)
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, flags, k_dec + 1, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, flags, 1, 0);
   if (len == 0) return;
   bool aligned = (((uintptr_t)zmix) & 15) == 0 && (len & 15) == 0;
   for (int i = 0; i < k_dec; i++) aligned = aligned && (((uintptr_t)outs.p[i]) & 15) == 0;
   const size_t threads = (len + 15) / 16;
-  NS_LAUNCH_TIMED(KT_SPLIT, k_dec_split, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, zmix, len, k_dec,
-                  outs, flags, aligned ? 1 : 0);
+  const unsigned blocks = (unsigned)((threads + 256 * DEC_ITER - 1) / (256 * DEC_ITER));
+  NS_LAUNCH_TIMED(KT_SPLIT, k_dec_split, dim3(blocks), dim3(256), 0, s, zmix, len, k_dec, outs, flags, aligned ? 1 : 0);
 }
 
-// Static scratch of the no-allocation DEC entry point: the plane non-zero counters and the range
-// counter (DEC_MAX_PLANES + 1 ints used of 64).  Each GPU holds its own copy for the life of the
-// process; every call on one GPU shares it, so calls that overlap on one GPU race.
+// Static scratch of the no-allocation DEC entry point: the split's OR-of-entries word (one int used
+// of 64).  Each GPU holds its own copy for the life of the process; every call on one GPU shares
+// it, so calls that overlap on one GPU race.
 static __device__ __align__(256) int scratch[64];
 
 // This device's address of `scratch` (the host-side name is only a stand-in, never a device
@@ -1336,7 +1340,7 @@ int* dec_noalloc_flags() {
   }
   return per_dev[dev];
 }
-static_assert(DEC_MAX_PLANES + 1 <= 64, "DEC scratch holds the flags of every plane");
+static_assert(DEC_MAX_PLANES < 31, "the DEC split's OR word keeps a range bit above every plane");
 
 // ---------------- head columns (ME public inputs X) ----------------
 __global__ void k_head_cols(PlanePtrs planes, int np, size_t m, int m_in, int8_t* __restrict__ out) {

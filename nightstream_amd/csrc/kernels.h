@@ -193,12 +193,15 @@ void launch_bus_open_planes(const PlanePtrs& planes, int np, size_t m, size_t bu
 void launch_bus_open_i32(const int32_t* Z, size_t m, size_t bus_base, uint32_t chunk, uint32_t bus_cols, const K2* w,
                          K2* y, K2* ys, hipStream_t s);
 // balanced base-2 digits of zmix into k_dec planes (by value: no pointer table in device memory);
-// flags[0..k_dec] are cleared by the launch, then flags[i] = plane i nonzero, flags[k_dec] = range error
+// flags[0] is cleared by the launch, then holds the OR of every |entry|: plane i is nonzero iff
+// bit i is set, and a bit at or above k_dec means an entry out of range (dec_out_of_range)
 constexpr int DEC_MAX_PLANES = 30;
 struct DecPlanes {
   int8_t* p[DEC_MAX_PLANES];
 };
 void launch_dec_split(const int32_t* zmix, size_t len, int k_dec, const DecPlanes& outs, int* flags, hipStream_t s);
+inline bool dec_out_of_range(uint32_t mask, int k_dec) { return (mask >> k_dec) != 0; }
+inline bool dec_plane_nonzero(uint32_t mask, int i) { return ((mask >> i) & 1u) != 0; }
 // The current device's address of the DEC split's static scratch (counters), for the
 // no-allocation entry point.
 int* dec_noalloc_flags();

@@ -1274,11 +1274,11 @@ void Prover::rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const st
     pptr[i] = bank[i].as<int8_t>();
     dec_pl.p[i] = bank[i].as<int8_t>();
   }
-  flags_.ensure((k_dec + 1) * 4);
+  flags_.ensure(4);
   launch_dec_split(zsrc, (size_t)D * m, (int)k_dec, dec_pl, flags_.as<int>(), st);
-  std::vector<int> fl(k_dec + 1);
-  d2h(ows_.stage, fl.data(), flags_.p, k_dec + 1, st);
-  if (fl[k_dec]) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
+  uint32_t dmask = 0;
+  d2h(ows_.stage, &dmask, flags_.p, 1, st);
+  if (dec_out_of_range(dmask, (int)k_dec)) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
   std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
   commit_device(pptr.data(), k_dec, child_c.data());
   std::vector<K2> y_child = y_products(pptr, vjs);
@@ -1290,7 +1290,7 @@ void Prover::rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const st
   }
   for (size_t i = 0; i < k_dec; i++) {
     MeHost& e = out.children[i];
-    if (fl[i]) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
+    if (dec_plane_nonzero(dmask, (int)i)) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
     else memset(e.c, 0, sizeof(e.c));
     e.m_in = m_in;
     e.t = t_tot;
@@ -1463,11 +1463,11 @@ void Prover::dec_api(const MeHost& parent, const int32_t* zmix_dev, size_t k_dec
     dec_pl.p[i] = planes_dev[i];
     pptr[i] = planes_dev[i];
   }
-  flags_.ensure((k_dec + 1) * 4);
+  flags_.ensure(4);
   launch_dec_split(zmix_dev, (size_t)D * m, (int)k_dec, dec_pl, flags_.as<int>(), stream_);
-  std::vector<int> fl(k_dec + 1);
-  d2h(ows_.stage, fl.data(), flags_.p, k_dec + 1, stream_);
-  if (fl[k_dec]) throw Error(6, "DEC split out of range");
+  uint32_t dmask = 0;
+  d2h(ows_.stage, &dmask, flags_.p, 1, stream_);
+  if (dec_out_of_range(dmask, (int)k_dec)) throw Error(6, "DEC split out of range");
   std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
   commit_device(pptr.data(), k_dec, child_c.data());
   vjs_at(parent.r, vjs_val_);
@@ -1483,7 +1483,7 @@ void Prover::dec_api(const MeHost& parent, const int32_t* zmix_dev, size_t k_dec
   }
   for (size_t i = 0; i < k_dec; i++) {
     MeHost& e = children[i];
-    if (fl[i]) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
+    if (dec_plane_nonzero(dmask, (int)i)) memcpy(e.c, &child_c[i * KAPPA * D], sizeof(e.c));
     e.m_in = m_in;
     e.t = parent.t;
     e.r = parent.r;

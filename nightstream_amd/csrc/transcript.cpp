@@ -2,6 +2,9 @@
 // See transcript.h for the construction and its parity status.
 #include "transcript.h"
 
+#include <algorithm>
+#include <chrono>
+
 #include <immintrin.h>
 #include <string.h>
 
@@ -122,6 +125,7 @@ void Poseidon2::init(int w, ChaCha8& rng) {
         vec_rc[r][i] = ext_init[r][i];
         vec_rc[4 + r][i] = ext_term[r][i];
       }
+
     use_avx512 = false;
     if (__builtin_cpu_supports("avx512f")) {
       // enable the vector path only if it reproduces the scalar permutation on a set of states
@@ -135,6 +139,25 @@ void Poseidon2::init(int w, ChaCha8& rng) {
         permute_scalar(a);
         permute8_avx512(*this, b);
         if (memcmp(a, b, sizeof a) != 0) use_avx512 = false;
+      }
+      // and only where it is the faster one: on some hosts (Zen 5) the scalar chain wins by ~5%
+      if (use_avx512) {
+        auto best_ns = [&](bool vec) {
+          double best = 1e30;
+          uint64_t t[8] = {1, 2, 3, 4, 5, 6, 7, 8};
+          for (int trial = 0; trial < 3; trial++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int k = 0; k < 2000; k++) {
+              if (vec) permute8_avx512(*this, t);
+              else permute_scalar(t);
+            }
+            best = std::min(best, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+          }
+          return best;
+        };
+        best_ns(true);
+        const double tv = best_ns(true), ts = best_ns(false);
+        use_avx512 = tv < 0.98 * ts;
       }
     }
   }
@@ -219,11 +242,13 @@ static inline void permute_w(const Poseidon2& P, uint64_t* st) {
     const uint64_t x3c = red((u128)x2 * xc);
     const uint64_t x4 = red((u128)x2 * x2);
     const uint64_t x7 = red((u128)x3 * x4);
-    u128 nx = (u128)red((u128)x3c * x4) + rest;
+    // a product of two u64 plus two more u64 stays below 2^128, so lane 0 takes one reduction
+    // after its last multiply (the rest is reduced beside the chain)
+    u128 nx = (u128)x3c * x4 + red_small(rest);
     if (r < 21) nx += P.internal_rc[r + 1];
     const uint64_t sum = red_small(rest + x7);  // below 16 * 2^64
     for (int i = 1; i < W; i++) st[i] = red((u128)st[i] * P.diag[i] + sum);
-    x = red_small(nx);  // below 18 * 2^64
+    x = red(nx);
   }
   st[0] = x;
   for (int r = 0; r < 4; r++) {
@@ -316,8 +341,8 @@ NS_AVX512 static void permute8_avx512(const Poseidon2& P, uint64_t* st) {
   uint64_t s0 = red((u128)st0 + P.internal_rc[0]);
   uint64_t restr = red(v_hsum(v));
   for (int r = 0; r < 22; r++) {
-    __m512i mv = v_mul(mu, v);
-    u128 W = v_hsum(mv);
+    const __m512i mv = v_mul(mu, v);
+    const u128 W = v_hsum(mv);
     // s0 -> s0^7 (diag0 + 1) in three dependent multiplies: s0 (diag0 + 1) is formed beside s0^2
     // and carried through s0^3 (diag0 + 1) = s0^2 * it; s0^7 itself comes off the chain
     const uint64_t xc = red((u128)s0 * P.diag0p1);
@@ -326,13 +351,13 @@ NS_AVX512 static void permute8_avx512(const Poseidon2& P, uint64_t* st) {
     const uint64_t x3c = red((u128)x2 * xc);
     const uint64_t x4 = red((u128)x2 * x2);
     const uint64_t x7 = red((u128)x3 * x4);
-    u128 acc = (u128)x3c * x4;
-    uint64_t sum = red((u128)restr + x7);
-    v = _mm512_mask_blend_epi64(1, v_addq(mv, _mm512_set1_epi64((long long)sum)), _mm512_setzero_si512());
-    acc = (u128)red(acc) + restr;
+    // x3c * x4 + restr + rc < (2^64 - 1)^2 + 2 (2^64 - 1) = 2^128 - 1: one reduction on the chain
+    u128 acc = (u128)x3c * x4 + restr;
+    const uint64_t sum = red((u128)restr + x7);
     if (r < 21) acc += P.internal_rc[r + 1];
     s0 = red(acc);
     restr = red(W + (u128)sum * 7);
+    v = _mm512_mask_blend_epi64(1, v_addq(mv, _mm512_set1_epi64((long long)sum)), _mm512_setzero_si512());
   }
   x = _mm512_mask_blend_epi64(1, v, _mm512_set1_epi64((long long)s0));
   for (int r = 0; r < 4; r++)
