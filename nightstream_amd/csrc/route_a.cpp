@@ -266,10 +266,12 @@ void run_batched(Transcript& tr, const char* domain, size_t inst, std::vector<Ra
       throw Error(7, std::string(domain) + ": claim " + c.label + " has fewer rounds than the batch");
   }
   std::vector<K2> co;
+  std::vector<TsOracleGpu*> tso;  // the device oracles step as one batch (shared launches)
+  for (auto& c : cl)
+    if (c.ts) tso.push_back(c.ts);
   for (size_t rd = 0; rd < rounds; rd++) {
     tr.append_msg_u64("batched/round_idx", rd);
-    for (auto& c : cl)
-      if (c.ts) c.ts->launch_round();
+    if (!tso.empty()) ts_batch_launch_round(tso.data(), tso.size());
     for (size_t i = 0; i < cl.size(); i++) {
       RaClaim& c = cl[i];
       if (c.ts) c.ts->finish_round(co);
@@ -294,9 +296,9 @@ void run_batched(Transcript& tr, const char* domain, size_t inst, std::vector<Ra
     chals[rd] = r;
     for (auto& c : cl) {
       c.running = poly_at(c.polys, rd * (c.deg + 1), c.deg + 1, r);
-      if (c.ts) c.ts->launch_fold(r, true);
-      else c.ccs->fold(r);
+      if (!c.ts) c.ccs->fold(r);
     }
+    if (!tso.empty()) ts_batch_fold(tso.data(), tso.size(), r);
   }
   for (auto& c : cl)
     if (c.ts) c.ts->finish_fold();

@@ -8,6 +8,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <memory>
 #include <vector>
 
 #include "prover.h"
@@ -34,6 +35,12 @@ enum TsKind {
 };
 
 constexpr int TS_MAX_COEF = 40;  // degree bound 3 + ell_addr <= 39
+constexpr int TS_GROUP = 16;     // time lanes per ragged group launch (arguments by value)
+
+struct TsTimeArgs;
+struct TsFoldArgs;
+struct TsLtArgs;
+struct TsSharedStream;
 
 // Device memory of a Twist/Shout oracle, drawn from a process-wide cache of blocks in
 // power-of-two size classes instead of hipMalloc / hipFree: a chunk builds and drops ~30 oracles,
@@ -91,15 +98,27 @@ class TsOracleGpu {
   void finish_fold();
 
  private:
+  friend void ts_batch_launch_round(TsOracleGpu* const* os, size_t n);
+  friend void ts_batch_fold(TsOracleGpu* const* os, size_t n, K2 r);
+  friend void ts_join(TsOracleGpu* const* os, size_t n);
+  bool time_lane() const;
+  void pre_round();
+  // lt: where the Twist val_pre prefix sums of at most ONEBLK_N terms are described instead of
+  // launched (*lt_described), for a group launch
+  bool time_round_prepare(TsTimeArgs& A, unsigned& nb, TsLtArgs* lt, bool* lt_described);
   bool time_round_launch();
-  void time_fold_launch(K2 r, K2* chal_dst);
+  // grp: where a fold of at most ONEBLK_N entries is described instead of launched (group path)
+  bool time_fold_launch(K2 r, K2* chal_dst, TsFoldArgs* grp = nullptr);
+  bool fold_begin(K2 r, bool defer, TsFoldArgs* grp);
   bool addr_round_launch(bool final);
   void addr_round(std::vector<K2>& c, bool final);
   bool product_round_launch();
   K2 time_final();
 
   TsKind kind_ = TSK_PRODUCT;
-  hipStream_t st_ = nullptr;  // from the same cache as the blocks, like the pinned mailbox
+  hipStream_t st_ = nullptr;      // where the oracle's work goes: its own stream or its group's
+  hipStream_t own_st_ = nullptr;  // from the same cache as the blocks, like the pinned mailbox
+  std::shared_ptr<TsSharedStream> group_;  // the batched sum-check this oracle was last stepped in
   K2* coef_h_ = nullptr;      // pinned: TS_MAX_COEF coefficients, then the folded support size
   int round_ncoef_ = 0;      // coefficients of the launched round
   bool round_dev_ = false;   // the launched round copies them from the device
@@ -148,10 +167,17 @@ class TsOracleGpu {
 
 // One round of the batched sum-check (neo-reductions/src/sumcheck.rs:388-407): every oracle's
 // round is enqueued before the host waits for any; coeffs[i] receives oracle i's coefficients.
-// All oracles must have the same number of rounds, at least one.
+// All oracles must have the same number of rounds, at least one.  The time lanes of a batch
+// share one stream from their first batched round on (ts_join) and share launches: ragged groups
+// of up to TS_GROUP members per round kernel, per sum and per fold.  Address lanes, products and
+// lookups keep their own streams and launches.
 void ts_batch_round(TsOracleGpu* const* os, size_t n, std::vector<std::vector<K2>>& coeffs);
+// the enqueueing half of ts_batch_round; each oracle's finish_round collects its coefficients
+void ts_batch_launch_round(TsOracleGpu* const* os, size_t n);
 // Fold every oracle with the shared challenge (sumcheck.rs:436-442).
 void ts_batch_fold(TsOracleGpu* const* os, size_t n, K2 r);
+// Puts the oracles on one stream (after their own work drained) unless they already share one.
+void ts_join(TsOracleGpu* const* os, size_t n);
 
 // Shared CPU bus helpers (neo-fold/src/memory_sidecar/cpu_bus.rs): Z is the host digit matrix
 // [54][m] of the CPU witness.

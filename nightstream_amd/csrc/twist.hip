@@ -56,9 +56,9 @@ __device__ __forceinline__ K2 wave_sum(K2 v) {
   return v;
 }
 
-// Block-reduce `ncoef` coefficients into partials[blockIdx.x * ncoef + d].
+// Block-reduce `ncoef` coefficients into partials[blk * ncoef + d].
 template <int MAXC>
-__device__ void block_store(K2 (&acc)[MAXC], int ncoef, K2* partials) {
+__device__ void block_store_at(K2 (&acc)[MAXC], int ncoef, K2* partials, uint32_t blk) {
   __shared__ K2 red[TB / 64][MAXC];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -72,8 +72,12 @@ __device__ void block_store(K2 (&acc)[MAXC], int ncoef, K2* partials) {
   if (threadIdx.x < ncoef) {
     K2 v = kzero();
     for (int w = 0; w < TB / 64; w++) v = kadd(v, red[w][threadIdx.x]);
-    partials[blockIdx.x * ncoef + threadIdx.x] = v;
+    partials[blk * ncoef + threadIdx.x] = v;
   }
+}
+template <int MAXC>
+__device__ void block_store(K2 (&acc)[MAXC], int ncoef, K2* partials) {
+  block_store_at<MAXC>(acc, ncoef, partials, blockIdx.x);
 }
 
 // ---------------- one-workgroup paths for supports of at most ONEBLK_N entries ----------------
@@ -203,19 +207,23 @@ __device__ __forceinline__ void val_pre_pair(const TsTimeArgs& A, uint64_t pair,
 }
 
 // per-term LT_low contributions of one round, for both children (one thread per term)
-__global__ void k_lt_contrib(const uint32_t* __restrict__ term_t, const K2* __restrict__ term_v, uint32_t nterms,
-                             int b, const K2* __restrict__ chals, K2* __restrict__ out0, K2* __restrict__ out1) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void lt_contrib_at(const uint32_t* __restrict__ term_t, const K2* __restrict__ term_v,
+                                              uint32_t nterms, int b, const K2* __restrict__ chals,
+                                              K2* __restrict__ out0, K2* __restrict__ out1, uint32_t k) {
   if (k >= nterms) return;
   const K2 v = term_v[k];
   out0[k] = kmul(v, lt_low(term_t[k], b, 0, chals));
   out1[k] = kmul(v, lt_low(term_t[k], b, 1, chals));
 }
+__global__ void k_lt_contrib(const uint32_t* __restrict__ term_t, const K2* __restrict__ term_v, uint32_t nterms,
+                             int b, const K2* __restrict__ chals, K2* __restrict__ out0, K2* __restrict__ out1) {
+  lt_contrib_at(term_t, term_v, nterms, b, chals, out0, out1, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // inclusive prefix sums of two K arrays of n <= ONEBLK_N entries in one launch (thread t owns
 // entries [t*PER, t*PER+PER))
-__global__ __launch_bounds__(ONEBLK_T) void k_scan2_1blk(const K2* __restrict__ in0, const K2* __restrict__ in1,
-                                                         uint32_t n, K2* __restrict__ out0, K2* __restrict__ out1) {
+__device__ __forceinline__ void scan2_1blk_body(const K2* __restrict__ in0, const K2* __restrict__ in1, uint32_t n,
+                                                K2* __restrict__ out0, K2* __restrict__ out1) {
   __shared__ K2 wsum[ONEBLK_T / 64];
   const uint32_t k0 = threadIdx.x * ONEBLK_PER;
   for (int c = 0; c < 2; c++) {
@@ -234,6 +242,10 @@ __global__ __launch_bounds__(ONEBLK_T) void k_scan2_1blk(const K2* __restrict__ 
     for (int j = 0; j < ONEBLK_PER; j++)
       if (k0 + j < n) out[k0 + j] = kadd(base, loc[j]);
   }
+}
+__global__ __launch_bounds__(ONEBLK_T) void k_scan2_1blk(const K2* __restrict__ in0, const K2* __restrict__ in1,
+                                                         uint32_t n, K2* __restrict__ out0, K2* __restrict__ out1) {
+  scan2_1blk_body(in0, in1, n, out0, out1);
 }
 
 // lt_eval (mle.rs:7-24) at the child time point (chals | c | bits of pair) against r_time
@@ -359,15 +371,16 @@ struct LinProd {
 };
 
 // One time-lane round as coefficients (degree < MAXC): every thread multiplies out its pairs'
-// products; bitness, whose summand is not a product of linear factors, runs only here.
+// products; bitness, whose summand is not a product of linear factors, runs only here.  The
+// work of block blk of a grid of nblk blocks.
 template <int MAXC>
-__global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restrict__ partials) {
+__device__ __forceinline__ void ts_time_round_body(TsTimeArgs& A, uint32_t blk, uint32_t nblk, K2* partials) {
   if (A.n_dev) A.n = *A.n_dev;
   const K2* weights = A.consts + 2 * A.ell + A.nbits;
   K2 acc[MAXC];
 #pragma unroll
   for (int d = 0; d < MAXC; d++) acc[d] = kzero();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = blk * blockDim.x + threadIdx.x; i < A.n; i += nblk * blockDim.x) {
     LinProd<MAXC> lp;
     K2(&p)[MAXC] = lp.p;
 #pragma unroll
@@ -402,23 +415,108 @@ __global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restri
 #pragma unroll
     for (int d = 0; d < MAXC; d++) acc[d] = kadd(acc[d], p[d]);
   }
-  block_store<MAXC>(acc, A.ncoef, partials);
+  block_store_at<MAXC>(acc, A.ncoef, partials, blk);
+}
+template <int MAXC>
+__global__ __launch_bounds__(TB) void k_ts_time_round(TsTimeArgs A, K2* __restrict__ partials) {
+  ts_time_round_body<MAXC>(A, blockIdx.x, gridDim.x, partials);
 }
 
 // One time-lane round by evaluation at X = blockIdx.y (0 .. ncoef-1): each one-wave block sums
 // its pairs' products of linear factors at X (one multiply-add per factor, no coefficient
 // arrays); partials[blockIdx.x * ncoef + X] receives the block's sum and the host interpolates
 // the ncoef sums to the round's coefficients.
-__global__ __launch_bounds__(64) void k_ts_time_points(TsTimeArgs A, K2* __restrict__ partials) {
+__device__ __forceinline__ void ts_time_points_body(TsTimeArgs& A, uint32_t blk, uint32_t nblk, uint32_t x,
+                                                    K2* partials) {
   if (A.n_dev) A.n = *A.n_dev;
-  const K2 X{(uint64_t)blockIdx.y, 0};
+  const K2 X{(uint64_t)x, 0};
   K2 acc = kzero();
-  for (uint32_t i = blockIdx.x * 64 + threadIdx.x; i < A.n; i += gridDim.x * 64) {
+  for (uint32_t i = blk * 64 + threadIdx.x; i < A.n; i += nblk * 64) {
     K2 prod = kone();
     if (ts_pair_factors(A, i, [&](K2 a, K2 b) { prod = kmul(prod, kadd(a, kmul(b, X))); })) acc = kadd(acc, prod);
   }
   acc = wave_sum(acc);
-  if (threadIdx.x == 0) partials[blockIdx.x * A.ncoef + blockIdx.y] = acc;
+  if (threadIdx.x == 0) partials[blk * A.ncoef + x] = acc;
+}
+__global__ __launch_bounds__(64) void k_ts_time_points(TsTimeArgs A, K2* __restrict__ partials) {
+  ts_time_points_body(A, blockIdx.x, gridDim.x, blockIdx.y, partials);
+}
+
+// ---------------- ragged groups of time lanes ----------------
+// The time lanes of one batched sum-check share a launch per kernel: up to TS_GROUP oracles,
+// arguments by value, block b belonging to member o with blk0[o] <= b < blk0[o + 1] and running
+// as block b - blk0[o] of a grid of blk0[o + 1] - blk0[o] blocks, as in the member's own launch.
+struct TsRoundGroup {
+  TsTimeArgs a[TS_GROUP];
+  K2* partials[TS_GROUP];
+  uint32_t blk0[TS_GROUP + 1];
+  int count;
+};
+static_assert(sizeof(TsRoundGroup) <= 4096, "kernel arguments of a round group");
+__device__ __forceinline__ int group_member(const uint32_t* blk0, int count, uint32_t b) {
+  int o = 0;
+  while (o + 1 < count && blk0[o + 1] <= b) o++;
+  return o;
+}
+template <int MAXC>
+__global__ __launch_bounds__(TB) void k_ts_time_round_group(TsRoundGroup G) {
+  const int o = group_member(G.blk0, G.count, blockIdx.x);
+  TsTimeArgs A = G.a[o];
+  ts_time_round_body<MAXC>(A, blockIdx.x - G.blk0[o], G.blk0[o + 1] - G.blk0[o], G.partials[o]);
+}
+// grid.y = the largest member's coefficient count; rows past a member's own count have no work
+__global__ __launch_bounds__(64) void k_ts_time_points_group(TsRoundGroup G) {
+  const int o = group_member(G.blk0, G.count, blockIdx.x);
+  TsTimeArgs A = G.a[o];
+  if ((int)blockIdx.y >= A.ncoef) return;
+  ts_time_points_body(A, blockIdx.x - G.blk0[o], G.blk0[o + 1] - G.blk0[o], blockIdx.y, G.partials[o]);
+}
+struct TsSumGroup {
+  const K2* partials[TS_GROUP];
+  K2* out[TS_GROUP];
+  int nblk[TS_GROUP], ncoef[TS_GROUP];
+  int count;
+};
+// k_ts_sum of every member: block (d, o) sums coefficient d of member o into its mailbox
+__global__ void k_ts_sum_group(TsSumGroup S) {
+  const int o = blockIdx.y, d = blockIdx.x;
+  if (o >= S.count || d >= S.ncoef[o]) return;
+  const int nblk = S.nblk[o], ncoef = S.ncoef[o];
+  const K2* partials = S.partials[o];
+  K2 v = kzero();
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = kadd(v, partials[b * ncoef + d]);
+  __shared__ K2 red[4];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) S.out[o][d] = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
+}
+// The Twist val_pre terms' LT contributions of a round and their prefix sums (at most ONEBLK_N
+// terms per member): ragged blocks of 256 for the contributions, one block per member for the
+// two scans.
+struct TsLtArgs {
+  const uint32_t* term_t;
+  const K2* term_v;
+  uint32_t nterms;
+  int bit;
+  const K2* chals;
+  K2 *c0, *c1, *s0, *s1;
+};
+struct TsLtGroup {
+  TsLtArgs m[TS_GROUP];
+  uint32_t blk0[TS_GROUP + 1];
+  int count;
+};
+static_assert(sizeof(TsLtGroup) <= 4096, "kernel arguments of an LT group");
+__global__ void k_lt_contrib_group(TsLtGroup G) {
+  const int o = group_member(G.blk0, G.count, blockIdx.x);
+  const TsLtArgs& L = G.m[o];
+  lt_contrib_at(L.term_t, L.term_v, L.nterms, L.bit, L.chals, L.c0, L.c1,
+                (blockIdx.x - G.blk0[o]) * blockDim.x + threadIdx.x);
+}
+__global__ __launch_bounds__(ONEBLK_T) void k_scan2_1blk_group(TsLtGroup G) {
+  const TsLtArgs& L = G.m[blockIdx.x];
+  scan2_1blk_body(L.c0, L.c1, L.nterms, L.s0, L.s1);
 }
 
 // one K value into device memory (a round challenge into an oracle's constants block)
@@ -452,23 +550,29 @@ __device__ __forceinline__ void ts_fold_entry(const uint32_t* __restrict__ idx, 
   }
 }
 
-__global__ void k_ts_fold(const uint32_t* __restrict__ idx, const K2* __restrict__ cols, uint32_t n, uint32_t stride,
-                          int ncol, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, K2 r,
-                          uint32_t* __restrict__ nidx, K2* __restrict__ ncols, uint32_t* __restrict__ nout,
-                          uint32_t* __restrict__ ndev) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ts_fold_body(const uint32_t* __restrict__ idx, const K2* __restrict__ cols, uint32_t n,
+                                             uint32_t stride, int ncol, const uint32_t* __restrict__ flags,
+                                             const uint32_t* __restrict__ pos, K2 r, uint32_t* __restrict__ nidx,
+                                             K2* __restrict__ ncols, uint32_t* __restrict__ nout,
+                                             uint32_t* __restrict__ ndev, uint32_t i) {
   if (i >= n) return;
   if (i == n - 1) nout[0] = ndev[0] = pos[i] + flags[i];
   if (!flags[i]) return;
   ts_fold_entry(idx, cols, n, stride, ncol, i, pos[i], r, nidx, ncols);
 }
+__global__ void k_ts_fold(const uint32_t* __restrict__ idx, const K2* __restrict__ cols, uint32_t n, uint32_t stride,
+                          int ncol, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, K2 r,
+                          uint32_t* __restrict__ nidx, K2* __restrict__ ncols, uint32_t* __restrict__ nout,
+                          uint32_t* __restrict__ ndev) {
+  ts_fold_body(idx, cols, n, stride, ncol, flags, pos, r, nidx, ncols, nout, ndev, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // pair heads of a time-lane fold and their exclusive scan in one launch (n <= ONEBLK_N; thread t
 // owns entries [t*PER, t*PER+PER)): flags[i], pos[i] and the folded support size pos[n]; thread
 // 0 also stores the round challenge into the constants block when chal_dst is set
-__global__ __launch_bounds__(ONEBLK_T) void k_ts_heads_1blk(const uint32_t* __restrict__ idx, uint32_t n,
-                                                            uint32_t* __restrict__ flags, uint32_t* __restrict__ pos,
-                                                            K2* __restrict__ chal_dst, K2 r) {
+__device__ __forceinline__ void ts_heads_1blk_body(const uint32_t* __restrict__ idx, uint32_t n,
+                                                   uint32_t* __restrict__ flags, uint32_t* __restrict__ pos,
+                                                   K2* __restrict__ chal_dst, K2 r) {
   __shared__ uint32_t wsum[ONEBLK_T / 64];
   const uint32_t i0 = threadIdx.x * ONEBLK_PER;
   uint32_t f[ONEBLK_PER], cnt = 0;
@@ -491,6 +595,46 @@ __global__ __launch_bounds__(ONEBLK_T) void k_ts_heads_1blk(const uint32_t* __re
     pos[n] = total;
     if (chal_dst) *chal_dst = r;
   }
+}
+__global__ __launch_bounds__(ONEBLK_T) void k_ts_heads_1blk(const uint32_t* __restrict__ idx, uint32_t n,
+                                                            uint32_t* __restrict__ flags, uint32_t* __restrict__ pos,
+                                                            K2* __restrict__ chal_dst, K2 r) {
+  ts_heads_1blk_body(idx, n, flags, pos, chal_dst, r);
+}
+
+// A time-lane fold of at most ONEBLK_N entries (the one-workgroup heads scan, then the
+// compaction), for the members of a group: one heads block per member, then the compactions of
+// all members as ragged blocks of 256 threads.
+struct TsFoldArgs {
+  const uint32_t* idx;
+  const K2* cols;
+  uint32_t n, stride;
+  int ncol;
+  uint32_t* flags;
+  uint32_t* pos;
+  uint32_t* nidx;
+  K2* ncols;
+  uint32_t* nout;  // pinned mailbox word (the folded support size, for the host)
+  uint32_t* ndev;  // device slot (the same, for a round enqueued behind the fold)
+  K2* chal_dst;    // the challenge's slot in the constants block, or null
+};
+constexpr int TS_FOLD_GROUP = 32;  // folds per group launch (smaller arguments than a round's)
+struct TsFoldGroup {
+  TsFoldArgs f[TS_FOLD_GROUP];
+  uint32_t blk0[TS_FOLD_GROUP + 1];
+  int count;
+  K2 r;
+};
+static_assert(sizeof(TsFoldGroup) <= 4096, "kernel arguments of a fold group");
+__global__ __launch_bounds__(ONEBLK_T) void k_ts_heads_1blk_group(TsFoldGroup G) {
+  const TsFoldArgs& F = G.f[blockIdx.x];
+  ts_heads_1blk_body(F.idx, F.n, F.flags, F.pos, F.chal_dst, G.r);
+}
+__global__ __launch_bounds__(256) void k_ts_fold_group(TsFoldGroup G) {
+  const int o = group_member(G.blk0, G.count, blockIdx.x);
+  const TsFoldArgs& F = G.f[o];
+  ts_fold_body(F.idx, F.cols, F.n, F.stride, F.ncol, F.flags, F.pos, G.r, F.nidx, F.ncols, F.nout, F.ndev,
+               (blockIdx.x - G.blk0[o]) * 256 + threadIdx.x);
 }
 
 // sum_k term_v[k] * lt_eval_at_bool_index(term_t[k], chals) (the fully bound Val_pre)
@@ -882,18 +1026,42 @@ void TsOracleGpu::upload(TsBuf& d, const std::vector<T>& h, size_t min_elems) {
 }
 
 TsOracleGpu::TsOracleGpu() {
-  st_ = TsPool::get().stream();
+  own_st_ = st_ = TsPool::get().stream();
   coef_h_ = TsPool::get().mailbox();
 }
 
-// the stream drains before the members' blocks go back to the cache
-TsOracleGpu::~TsOracleGpu() {
-  if (st_) {
-    (void)hipStreamSynchronize(st_);
-    TsPool::get().put_stream(st_);
+// The stream of a batch: every member's work goes there from the batch's first round on, and
+// the stream returns to the cache (drained) with its last member.
+struct TsSharedStream {
+  hipStream_t s = nullptr;
+  TsSharedStream() : s(TsPool::get().stream()) {}
+  ~TsSharedStream() {
+    (void)hipStreamSynchronize(s);
+    TsPool::get().put_stream(s);
   }
+};
+
+// the stream the oracle's work went to drains before its own stream, the mailbox and (as
+// members, after the body) the device blocks go back to the cache
+TsOracleGpu::~TsOracleGpu() {
+  if (st_) (void)hipStreamSynchronize(st_);
+  if (own_st_) TsPool::get().put_stream(own_st_);
   stage_release();
   if (coef_h_) TsPool::get().put_mailbox(coef_h_);
+}
+
+void ts_join(TsOracleGpu* const* os, size_t n) {
+  if (n == 0) return;
+  bool same = os[0]->group_ != nullptr;
+  for (size_t i = 1; i < n && same; i++) same = os[i]->group_ == os[0]->group_;
+  if (same) return;
+  auto g = std::make_shared<TsSharedStream>();
+  for (size_t i = 0; i < n; i++) {
+    // whatever the oracle enqueued so far (its constructor's uploads, an earlier batch) is done
+    NS_CHECK(hipStreamSynchronize(os[i]->st_));
+    os[i]->group_ = g;
+    os[i]->st_ = g->s;
+  }
 }
 
 static SparseCol empty_col() { return SparseCol{}; }
@@ -1091,10 +1259,14 @@ size_t TsOracleGpu::num_rounds() const {
 
 // enqueue one round of a time-lane oracle (the challenges so far are already in the constants
 // block: launch_fold stores each one); false when the support is empty and the round is zero
-bool TsOracleGpu::time_round_launch() {
+// The round's arguments and grid (the Twist val_pre prefix sums are enqueued here); false when
+// the support is empty (a zero round).  Sets round_points_: by evaluation (k_ts_time_points,
+// blocks of 64) above degree 7, else as coefficients (k_ts_time_round<8>, blocks of TB).
+bool TsOracleGpu::time_round_prepare(TsTimeArgs& A, unsigned& nb, TsLtArgs* lt, bool* lt_described) {
+  if (lt_described) *lt_described = false;
   const int ncoef = (int)degree_bound_ + 1;
   if (n_ == 0) return false;
-  TsTimeArgs A;
+  partials_.ensure((size_t)TMAXBLK * TS_MAX_COEF * sizeof(K2));
   A.kind = kind_;
   A.n = (uint32_t)n_;
   A.stride = (uint32_t)cap_;
@@ -1123,55 +1295,78 @@ bool TsOracleGpu::time_round_launch() {
       lt_c_[c].ensure(nterms_ * sizeof(K2));
       lt_s_[c].ensure(nterms_ * sizeof(K2));
     }
-    hipLaunchKernelGGL(k_lt_contrib, dim3((nt + 255) / 256), dim3(256), 0, st_, term_t_.as<uint32_t>(),
-                       term_v_.as<K2>(), nt, (int)bit_idx_, consts_.as<K2>() + ell_n_, lt_c_[0].as<K2>(),
-                       lt_c_[1].as<K2>());
-    if (nt <= (uint32_t)ONEBLK_N) {
-      hipLaunchKernelGGL(k_scan2_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, lt_c_[0].as<K2>(), lt_c_[1].as<K2>(), nt,
-                         lt_s_[0].as<K2>(), lt_s_[1].as<K2>());
+    if (lt && nt <= (uint32_t)ONEBLK_N) {
+      // described for the group's LT launches, which run before its round kernels
+      *lt = TsLtArgs{term_t_.as<uint32_t>(), term_v_.as<K2>(), nt, (int)bit_idx_, consts_.as<K2>() + ell_n_,
+                     lt_c_[0].as<K2>(), lt_c_[1].as<K2>(), lt_s_[0].as<K2>(), lt_s_[1].as<K2>()};
+      *lt_described = true;
     } else {
-      size_t tb = 0;
-      NS_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, tb, lt_c_[0].as<K2>(), lt_s_[0].as<K2>(), KAddOp(),
-                                                 (int)nt, st_));
-      lt_tmp_.ensure(tb + 16);
-      for (int c = 0; c < 2; c++)
-        NS_CHECK(hipcub::DeviceScan::InclusiveScan(lt_tmp_.p, tb, lt_c_[c].as<K2>(), lt_s_[c].as<K2>(), KAddOp(),
+      hipLaunchKernelGGL(k_lt_contrib, dim3((nt + 255) / 256), dim3(256), 0, st_, term_t_.as<uint32_t>(),
+                         term_v_.as<K2>(), nt, (int)bit_idx_, consts_.as<K2>() + ell_n_, lt_c_[0].as<K2>(),
+                         lt_c_[1].as<K2>());
+      if (nt <= (uint32_t)ONEBLK_N) {
+        hipLaunchKernelGGL(k_scan2_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, lt_c_[0].as<K2>(), lt_c_[1].as<K2>(), nt,
+                           lt_s_[0].as<K2>(), lt_s_[1].as<K2>());
+      } else {
+        size_t tb = 0;
+        NS_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, tb, lt_c_[0].as<K2>(), lt_s_[0].as<K2>(), KAddOp(),
                                                    (int)nt, st_));
+        lt_tmp_.ensure(tb + 16);
+        for (int c = 0; c < 2; c++)
+          NS_CHECK(hipcub::DeviceScan::InclusiveScan(lt_tmp_.p, tb, lt_c_[c].as<K2>(), lt_s_[c].as<K2>(), KAddOp(),
+                                                     (int)nt, st_));
+      }
     }
     A.lt_s0 = lt_s_[0].as<K2>();
     A.lt_s1 = lt_s_[1].as<K2>();
   }
-  unsigned nb;
   round_points_ = ncoef > 8;
   if (!round_points_) {
     nb = nblocks(n_);
-    hipLaunchKernelGGL(k_ts_time_round<8>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
   } else {
     if (kind_ == TSK_BITNESS) throw Error(1, "bitness round: degree above 7");
     nb = (unsigned)std::min<size_t>((n_ + 63) / 64, TMAXBLK);
-    hipLaunchKernelGGL(k_ts_time_points, dim3(nb, ncoef), dim3(64), 0, st_, A, partials_.as<K2>());
   }
+  return true;
+}
+
+bool TsOracleGpu::time_round_launch() {
+  TsTimeArgs A;
+  unsigned nb = 0;
+  if (!time_round_prepare(A, nb, nullptr, nullptr)) return false;
+  if (!round_points_)
+    hipLaunchKernelGGL(k_ts_time_round<8>, dim3(nb), dim3(TB), 0, st_, A, partials_.as<K2>());
+  else
+    hipLaunchKernelGGL(k_ts_time_points, dim3(nb, A.ncoef), dim3(64), 0, st_, A, partials_.as<K2>());
   // the sums go straight to the pinned mailbox (read after the stream synchronizes)
-  hipLaunchKernelGGL(k_ts_sum, dim3(ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, ncoef, coef_h_);
+  hipLaunchKernelGGL(k_ts_sum, dim3(A.ncoef), dim3(256), 0, st_, partials_.as<K2>(), (int)nb, A.ncoef, coef_h_);
   return true;
 }
 
 // enqueue a time-lane fold; the folded support size arrives in the mailbox (finish_fold).
-// chal_dst: where the challenge goes in the constants block (null: not stored)
-void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
+// chal_dst: where the challenge goes in the constants block (null: not stored).  With grp set, a
+// fold of at most ONEBLK_N entries is described there for a group launch instead (true).
+bool TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst, TsFoldArgs* grp) {
   nslot_.ensure(sizeof(uint32_t));
   if (n_ > 0 && n_ <= (size_t)ONEBLK_N) {
     const uint32_t n = (uint32_t)n_;
     uint32_t* nout = (uint32_t*)(coef_h_ + TS_MAX_COEF);  // pinned mailbox, read by finish_fold
-    hipLaunchKernelGGL(k_ts_heads_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, idx_[cur_].as<uint32_t>(), n,
-                       flags_.as<uint32_t>(), pos_.as<uint32_t>(), chal_dst, r);
-    hipLaunchKernelGGL(k_ts_fold, dim3((n + 255) / 256), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(),
-                       cols_[cur_].as<K2>(), n, (uint32_t)cap_, (int)ncol_, flags_.as<uint32_t>(), pos_.as<uint32_t>(),
-                       r, idx_[cur_ ^ 1].as<uint32_t>(), cols_[cur_ ^ 1].as<K2>(), nout, nslot_.as<uint32_t>());
+    if (grp) {
+      *grp = TsFoldArgs{idx_[cur_].as<uint32_t>(), cols_[cur_].as<K2>(), n, (uint32_t)cap_, (int)ncol_,
+                        flags_.as<uint32_t>(), pos_.as<uint32_t>(), idx_[cur_ ^ 1].as<uint32_t>(),
+                        cols_[cur_ ^ 1].as<K2>(), nout, nslot_.as<uint32_t>(), chal_dst};
+    } else {
+      hipLaunchKernelGGL(k_ts_heads_1blk, dim3(1), dim3(ONEBLK_T), 0, st_, idx_[cur_].as<uint32_t>(), n,
+                         flags_.as<uint32_t>(), pos_.as<uint32_t>(), chal_dst, r);
+      hipLaunchKernelGGL(k_ts_fold, dim3((n + 255) / 256), dim3(256), 0, st_, idx_[cur_].as<uint32_t>(),
+                         cols_[cur_].as<K2>(), n, (uint32_t)cap_, (int)ncol_, flags_.as<uint32_t>(),
+                         pos_.as<uint32_t>(), r, idx_[cur_ ^ 1].as<uint32_t>(), cols_[cur_ ^ 1].as<K2>(), nout,
+                         nslot_.as<uint32_t>());
+    }
     fold_pending_ = true;
     cur_ ^= 1;
     len_ >>= 1;
-    return;
+    return grp != nullptr;
   }
   if (chal_dst) hipLaunchKernelGGL(k_put_k2, dim3(1), dim3(64), 0, st_, chal_dst, r);
   if (n_ > 0) {
@@ -1191,6 +1386,7 @@ void TsOracleGpu::time_fold_launch(K2 r, K2* chal_dst) {
     cur_ ^= 1;
   }
   len_ >>= 1;
+  return false;
 }
 
 // value of a fully folded time-lane oracle (twist_oracle.rs singleton branches)
@@ -1523,9 +1719,14 @@ void TsOracleGpu::addr_round(std::vector<K2>& c, bool final) {
   finish_round(c);
 }
 
-void TsOracleGpu::launch_round() {
-  // After a batched fold the round is enqueued without waiting for it: its kernels read the
-  // folded support size on the device (the host's n_ is an upper bound until finish_round).
+bool TsOracleGpu::time_lane() const {
+  return kind_ != TSK_PRODUCT && kind_ != TSK_ADDR_LOOKUP && kind_ != TSK_READ_ADDR && kind_ != TSK_WRITE_ADDR;
+}
+
+// What every round does before its kernels.  After a batched fold the round is enqueued without
+// waiting for it: its kernels read the folded support size on the device (the host's n_ is an
+// upper bound until finish_round).  A round the claim already computed is reused.
+void TsOracleGpu::pre_round() {
   if (fold_pending_ && defer_fold_) {
     fold_pending_ = false;
     n_on_device_ = true;
@@ -1533,10 +1734,12 @@ void TsOracleGpu::launch_round() {
   defer_fold_ = false;
   finish_fold();
   round_points_ = false;
-  if (claim_round_valid_) {  // the claim already computed this round
-    round_from_claim_ = true;
-    return;
-  }
+  if (claim_round_valid_) round_from_claim_ = true;
+}
+
+void TsOracleGpu::launch_round() {
+  pre_round();
+  if (round_from_claim_) return;
   switch (kind_) {
     case TSK_PRODUCT:
     case TSK_ADDR_LOOKUP:
@@ -1667,11 +1870,14 @@ void TsOracleGpu::fold(K2 r) {
 
 // every fold is stream-ordered before the next round's kernels; only a time-lane fold leaves
 // host state (the support size) to collect
-void TsOracleGpu::launch_fold(K2 r, bool defer) {
+void TsOracleGpu::launch_fold(K2 r, bool defer) { (void)fold_begin(r, defer, nullptr); }
+
+// launch_fold; with grp set, a small time-lane fold is described there for a group launch (true)
+bool TsOracleGpu::fold_begin(K2 r, bool defer, TsFoldArgs* grp) {
   finish_fold();
   defer_fold_ = defer;
   claim_round_valid_ = false;
-  if (num_rounds() == 0) return;
+  if (num_rounds() == 0) return false;
   switch (kind_) {
     case TSK_PRODUCT:
     case TSK_ADDR_LOOKUP: {
@@ -1681,7 +1887,7 @@ void TsOracleGpu::launch_fold(K2 r, bool defer) {
       cur_ ^= 1;
       len_ = half;
       chals_.push_back(r);
-      return;
+      return false;
     }
     case TSK_READ_ADDR:
     case TSK_WRITE_ADDR: {
@@ -1699,11 +1905,10 @@ void TsOracleGpu::launch_fold(K2 r, bool defer) {
                          it_[cur_ ^ 1].as<K2>(), 1, (size_t)0, half, r);
       cur_ ^= 1;
       bit_idx_++;
-      return;
+      return false;
     }
     case TSK_TOTAL_INC:
-      time_fold_launch(r, nullptr);
-      return;
+      return time_fold_launch(r, nullptr, grp);
     default:
       break;
   }
@@ -1712,12 +1917,13 @@ void TsOracleGpu::launch_fold(K2 r, bool defer) {
     prefix_eq_ = kmul(prefix_eq_, kadd(kmul(ksub(kone(), r), ksub(kone(), rc)), kmul(r, rc)));
   }
   // constants block: r_cycle (ell_n) | challenges (ell_n slots); at most ell_n folds
-  time_fold_launch(r, consts_.as<K2>() + ell_n_ + chals_.size());
+  const bool described = time_fold_launch(r, consts_.as<K2>() + ell_n_ + chals_.size(), grp);
   chals_.push_back(r);
   bit_idx_++;
+  return described;
 }
 
-void ts_batch_round(TsOracleGpu* const* os, size_t n, std::vector<std::vector<K2>>& coeffs) {
+static void batch_check(TsOracleGpu* const* os, size_t n) {
   if (n == 0) throw Error(1, "batched round: no oracles");
   for (size_t i = 0; i < n; i++)
     if (!os[i]) throw Error(1, "batched round: null oracle");
@@ -1725,16 +1931,142 @@ void ts_batch_round(TsOracleGpu* const* os, size_t n, std::vector<std::vector<K2
   for (size_t i = 0; i < n; i++)
     if (os[i]->num_rounds() != rounds) throw Error(1, "batched round: every oracle must have the same number of rounds");
   if (rounds == 0) throw Error(1, "batched round: no rounds left");
+}
+
+// One group launch of up to TS_GROUP prepared time-lane rounds of one family, then their sums.
+static void launch_round_group(hipStream_t st, const std::vector<TsTimeArgs>& args, const std::vector<unsigned>& nbs,
+                               const std::vector<K2*>& partials, const std::vector<K2*>& outs, size_t first,
+                               size_t count, bool points) {
+  TsRoundGroup G{};
+  TsSumGroup S{};
+  G.count = S.count = (int)count;
+  uint32_t b = 0;
+  int maxc = 1;
+  for (size_t j = 0; j < count; j++) {
+    const TsTimeArgs& A = args[first + j];
+    G.a[j] = A;
+    G.partials[j] = partials[first + j];
+    G.blk0[j] = b;
+    b += nbs[first + j];
+    S.partials[j] = partials[first + j];
+    S.out[j] = outs[first + j];
+    S.nblk[j] = (int)nbs[first + j];
+    S.ncoef[j] = A.ncoef;
+    maxc = std::max(maxc, A.ncoef);
+  }
+  G.blk0[count] = b;
+  (void)hipGetLastError();  // launch errors below are this group's
+  if (points)
+    hipLaunchKernelGGL(k_ts_time_points_group, dim3(b, (unsigned)maxc), dim3(64), 0, st, G);
+  else
+    hipLaunchKernelGGL(k_ts_time_round_group<8>, dim3(b), dim3(TB), 0, st, G);
+  NS_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_ts_sum_group, dim3((unsigned)maxc, (unsigned)count), dim3(256), 0, st, S);
+  NS_CHECK(hipGetLastError());
+}
+
+void ts_batch_launch_round(TsOracleGpu* const* os, size_t n) {
+  if (n == 0) throw Error(1, "batched round: no oracles");
+  for (size_t i = 0; i < n; i++)
+    if (!os[i] || os[i]->num_rounds() == 0) throw Error(1, "batched round: null oracle or no rounds left");
+  // the time lanes share a stream and their launches; the other oracles keep their own streams
+  std::vector<TsOracleGpu*> tl;
+  for (size_t i = 0; i < n; i++)
+    if (os[i]->time_lane()) tl.push_back(os[i]);
+  ts_join(tl.data(), tl.size());
+  const hipStream_t st = tl.empty() ? nullptr : tl[0]->st_;
+  // the time lanes' rounds by family (coefficients, points); every other oracle launches its own
+  std::vector<TsTimeArgs> args[2];
+  std::vector<unsigned> nbs[2];
+  std::vector<K2*> partials[2], outs[2];
+  std::vector<TsLtArgs> lts;
+  for (size_t i = 0; i < n; i++) {
+    TsOracleGpu* o = os[i];
+    if (!o->time_lane()) {
+      o->launch_round();
+      continue;
+    }
+    o->pre_round();
+    if (o->round_from_claim_) continue;
+    o->round_ncoef_ = (int)o->degree_bound_ + 1;
+    TsTimeArgs A;
+    unsigned nb = 0;
+    TsLtArgs lt;
+    bool lt_used = false;
+    o->round_dev_ = o->time_round_prepare(A, nb, &lt, &lt_used);
+    if (lt_used) lts.push_back(lt);
+    if (!o->round_dev_) continue;
+    const int f = o->round_points_ ? 1 : 0;
+    args[f].push_back(A);
+    nbs[f].push_back(nb);
+    partials[f].push_back(o->partials_.as<K2>());
+    outs[f].push_back(o->coef_h_);
+  }
+  // the Twist val_pre prefix sums first (the round kernels read them)
+  for (size_t j = 0; j < lts.size(); j += TS_GROUP) {
+    const size_t count = std::min<size_t>(TS_GROUP, lts.size() - j);
+    TsLtGroup G{};
+    G.count = (int)count;
+    uint32_t b = 0;
+    for (size_t q = 0; q < count; q++) {
+      G.m[q] = lts[j + q];
+      G.blk0[q] = b;
+      b += (lts[j + q].nterms + 255) / 256;
+    }
+    G.blk0[count] = b;
+    (void)hipGetLastError();  // launch errors below are this group's
+    hipLaunchKernelGGL(k_lt_contrib_group, dim3(b), dim3(256), 0, st, G);
+    NS_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_scan2_1blk_group, dim3((unsigned)count), dim3(ONEBLK_T), 0, st, G);
+    NS_CHECK(hipGetLastError());
+  }
+  for (int f = 0; f < 2; f++)
+    for (size_t j = 0; j < args[f].size(); j += TS_GROUP)
+      launch_round_group(st, args[f], nbs[f], partials[f], outs[f], j,
+                         std::min<size_t>(TS_GROUP, args[f].size() - j), f == 1);
+}
+
+void ts_batch_round(TsOracleGpu* const* os, size_t n, std::vector<std::vector<K2>>& coeffs) {
+  batch_check(os, n);
+  ts_batch_launch_round(os, n);
   coeffs.resize(n);
-  for (size_t i = 0; i < n; i++) os[i]->launch_round();
   for (size_t i = 0; i < n; i++) os[i]->finish_round(coeffs[i]);
 }
 
 void ts_batch_fold(TsOracleGpu* const* os, size_t n, K2 r) {
   for (size_t i = 0; i < n; i++)
     if (!os[i]) throw Error(1, "batched fold: null oracle");
-  // the folds' support sizes are collected by the next batched round (or by any other use)
-  for (size_t i = 0; i < n; i++) os[i]->launch_fold(r, true);
+  if (n == 0) return;
+  std::vector<TsOracleGpu*> tl;
+  for (size_t i = 0; i < n; i++)
+    if (os[i]->time_lane()) tl.push_back(os[i]);
+  ts_join(tl.data(), tl.size());
+  const hipStream_t st = tl.empty() ? nullptr : tl[0]->st_;
+  // the folds' support sizes are collected by the next batched round (or by any other use); the
+  // small time-lane folds of the batch share one heads launch and one compaction launch per group
+  std::vector<TsFoldArgs> fa;
+  for (size_t i = 0; i < n; i++) {
+    TsFoldArgs F;
+    if (os[i]->fold_begin(r, true, &F)) fa.push_back(F);
+  }
+  for (size_t j = 0; j < fa.size(); j += TS_FOLD_GROUP) {
+    const size_t count = std::min<size_t>(TS_FOLD_GROUP, fa.size() - j);
+    TsFoldGroup G{};
+    G.count = (int)count;
+    G.r = r;
+    uint32_t b = 0;
+    for (size_t q = 0; q < count; q++) {
+      G.f[q] = fa[j + q];
+      G.blk0[q] = b;
+      b += (fa[j + q].n + 255) / 256;
+    }
+    G.blk0[count] = b;
+    (void)hipGetLastError();  // launch errors below are this group's
+    hipLaunchKernelGGL(k_ts_heads_1blk_group, dim3((unsigned)count), dim3(ONEBLK_T), 0, st, G);
+    NS_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_ts_fold_group, dim3(b), dim3(256), 0, st, G);
+    NS_CHECK(hipGetLastError());
+  }
 }
 
 // ---------------- shared CPU bus (host) ----------------

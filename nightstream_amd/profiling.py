@@ -68,6 +68,19 @@ def fused_rounds_line(trace_csv: str, period: int, name: str = "k_round_fused", 
             ", ".join(f"round {first + i}: {statistics.median(v):.1f}" for i, v in sorted(per.items())))
 
 
+def tail_line(trace_csv: str, prefix: str, count: int) -> str:
+    """Average duration of the last `count` dispatches of the kernels whose short name starts with
+    `prefix`: the timed region of a run whose first commitments of each shape are the tuner's
+    tile-shape and K-split trials."""
+    rows = [r for r in csv.DictReader(open(trace_csv)) if short_name(r["Kernel_Name"]).startswith(prefix)]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    last = rows[-count:]
+    d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in last]
+    names = collections.Counter(short_name(r["Kernel_Name"]) for r in last)
+    return (f"{prefix}* over the last {len(d)} dispatches (the timed region): avg {sum(d) / max(1, len(d)):.1f} us "
+            f"(min {min(d):.1f}, max {max(d):.1f}); " + ", ".join(f"{k} x{v}" for k, v in sorted(names.items())))
+
+
 def sq_table(db: str, cmd: str, name: str = "k_round_fused", first: int = 3) -> str:
     """SQ counters of each fused round (the last fold step of the run), per wave."""
     import sqlite3
@@ -161,6 +174,8 @@ def main(argv=None):
     s.add_argument("--cmd", required=True)
     s.add_argument("--trace", default=None)
     s.add_argument("--rounds-per-step", type=int, default=13)
+    s.add_argument("--tail-prefix", default=None)
+    s.add_argument("--tail-count", type=int, default=17)
     q = sub.add_parser("sq")
     q.add_argument("db")
     q.add_argument("--cmd", required=True)
@@ -177,6 +192,8 @@ def main(argv=None):
         sys.stdout.write(stats_table(a.csv, a.cmd))
         if a.trace:
             sys.stdout.write("\n" + fused_rounds_line(a.trace, a.rounds_per_step) + "\n")
+            if a.tail_prefix:
+                sys.stdout.write("\n" + tail_line(a.trace, a.tail_prefix, a.tail_count) + "\n")
     elif a.mode == "sq":
         sys.stdout.write(sq_table(a.db, a.cmd))
     else:
