@@ -145,6 +145,7 @@ struct TsTimeArgs {
   uint32_t nterms;
   int ncoef;
   const uint32_t* n_dev;  // when set, the support size is read here (n is then an upper bound)
+  int cgroups;            // bitness: threads per pair, each summing every cgroups-th bit column
 };
 
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint64_t key) {
@@ -380,7 +381,9 @@ __device__ __forceinline__ void ts_time_round_body(TsTimeArgs& A, uint32_t blk, 
   K2 acc[MAXC];
 #pragma unroll
   for (int d = 0; d < MAXC; d++) acc[d] = kzero();
-  for (uint32_t i = blk * blockDim.x + threadIdx.x; i < A.n; i += nblk * blockDim.x) {
+  const uint32_t cg = (uint32_t)A.cgroups;
+  for (uint32_t t = blk * blockDim.x + threadIdx.x; t < A.n * cg; t += nblk * blockDim.x) {
+    const uint32_t i = t / cg;
     LinProd<MAXC> lp;
     K2(&p)[MAXC] = lp.p;
 #pragma unroll
@@ -395,7 +398,9 @@ __device__ __forceinline__ void ts_time_round_body(TsTimeArgs& A, uint32_t blk, 
       K2 chi0, chi1;
       ts_chi(A, pair, chi0, chi1);
       K2 q0 = kzero(), q1 = kzero(), q2 = kzero();
-      for (int c = 0; c < A.nbits; c++) {
+      // this thread's share of the columns (a pair's cgroups threads sum disjoint column sets;
+      // the summand is linear in the column sum)
+      for (int c = (int)(t % cg); c < A.nbits; c += (int)cg) {
         const K2 b0 = ts_col(A, A.bit0 + c, i0), b1 = ts_col(A, A.bit0 + c, i1);
         if (kiszero(b0) && kiszero(b1)) continue;
         const K2 db = ksub(b1, b0), w = weights[c];
@@ -1320,9 +1325,11 @@ bool TsOracleGpu::time_round_prepare(TsTimeArgs& A, unsigned& nb, TsLtArgs* lt, 
     A.lt_s0 = lt_s_[0].as<K2>();
     A.lt_s1 = lt_s_[1].as<K2>();
   }
+  A.cgroups = 1;
+  if (kind_ == TSK_BITNESS && nbits_ > 4) A.cgroups = (int)std::min<size_t>(8, (nbits_ + 3) / 4);
   round_points_ = ncoef > 8;
   if (!round_points_) {
-    nb = nblocks(n_);
+    nb = nblocks(n_ * (size_t)A.cgroups);
   } else {
     if (kind_ == TSK_BITNESS) throw Error(1, "bitness round: degree above 7");
     nb = (unsigned)std::min<size_t>((n_ + 63) / 64, TMAXBLK);
