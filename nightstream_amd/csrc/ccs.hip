@@ -826,10 +826,40 @@ __global__ __launch_bounds__(256) void k_sum_partials(const K2* __restrict__ par
   if (threadIdx.x == 0) out[d] = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
 }
 
+// The same sum in one block (wave d sums coefficient d), published to pinned host memory with a
+// ready word: lane 0 of wave 0 stores the coefficients, then `seq` into *flag with a system-scope
+// release, so a host that reads the flag with acquire sees the coefficients.  The host polls the
+// word instead of waiting on the stream (the stream wait's wake-up cost ~30 us per round).
+__global__ __launch_bounds__(512) void k_sum_partials_sig(const K2* __restrict__ partials, int nblk, int ncoef,
+                                                          K2* __restrict__ out, uint64_t* flag, uint64_t seq) {
+  __shared__ K2 res[8];
+  const int d = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (d < ncoef) {
+    K2 v = kzero();
+    for (int b = lane; b < nblk; b += 64) v = kadd(v, partials[b * ncoef + d]);
+    v = wave_sum(v);
+    if (lane == DPP_LANE) res[d] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < ncoef; i++) out[i] = res[i];
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+static void launch_sum_partials(const K2* partials, size_t nblk, int ncoef, K2* out, const RoundSignal* sig,
+                                hipStream_t s) {
+  if (sig && sig->flag)
+    hipLaunchKernelGGL(k_sum_partials_sig, dim3(1), dim3(512), 0, s, partials, (int)nblk, ncoef, out, sig->flag,
+                       sig->seq);
+  else
+    hipLaunchKernelGGL(k_sum_partials, dim3(ncoef), dim3(256), 0, s, partials, (int)nblk, ncoef, out);
+}
+
 size_t round_ws_bytes() { return (size_t)RMAXBLK * 7 * sizeof(K2); }
 
 void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool round0, K2* out, void* ws,
-                       hipStream_t s) {
+                       hipStream_t s, const RoundSignal* sig) {
   if (T.len < 2 || (T.len & (T.len - 1))) throw std::invalid_argument("round: table length must be a power of two >= 2");
   RoundArgs A;
   A.len = T.len;
@@ -867,7 +897,7 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
     NS_LAUNCH_TIMED(KT_ROUND1, k_round_eval<1>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
   else
     NS_LAUNCH_TIMED(KT_ROUND, k_round_eval<2>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
-  hipLaunchKernelGGL(k_sum_partials, dim3(5), dim3(256), 0, s, partials, (int)nblk, 5, out);
+  launch_sum_partials(partials, nblk, 5, out, sig, s);
 }
 
 // ---------------- fused fold + round on the K tables (rounds >= 3) ----------------
@@ -1096,7 +1126,8 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
   }
 }
 
-void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s) {
+void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s,
+                        const RoundSignal* sig) {
   const size_t pairs = A.len / 2, nrow = (size_t)A.k * D;
   if (pairs == 0 || (pairs & (pairs - 1))) throw std::invalid_argument("fused round: table length must be a power of two >= 2");
   if (A.nf + 1 > FUSED_MAX_SC - (A.has_eval ? 2 : 0)) throw std::invalid_argument("fused round: too many scalar tables");
@@ -1113,7 +1144,7 @@ void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hip
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
   NS_LAUNCH_TIMED(KT_FUSED, k_round_fused, dim3((unsigned)nblk), dim3(RB), 0, s, A, wg, partials, (int)G);
-  hipLaunchKernelGGL(k_sum_partials, dim3(FUSED_NCOEF), dim3(256), 0, s, partials, (int)nblk, FUSED_NCOEF, out);
+  launch_sum_partials(partials, nblk, FUSED_NCOEF, out, sig, s);
 }
 
 // ---------------- folds ----------------

@@ -136,9 +136,15 @@ struct RowTables {
   const K2* pc = nullptr;     // [20][4] monomial coefficient polynomials for digit_round1
 };
 size_t round_ws_bytes();
+// Optional ready word for a round's coefficients in coherent pinned host memory: the partial-sum
+// kernel stores the coefficients, then seq into *flag (system-scope release).
+struct RoundSignal {
+  uint64_t* flag = nullptr;
+  uint64_t seq = 0;
+};
 // Writes the deg_max+1 coefficients (low->high) of the round polynomial to out (device).
 void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg /* [k*54] device */, bool round0,
-                       K2* out, void* ws, hipStream_t s);
+                       K2* out, void* ws, hipStream_t s, const RoundSignal* sig = nullptr);
 // Rounds >= 3: fold the NC tables ([k*54][2 len] -> [k*54][len]) and the f-variable and Eval
 // scalar tables by r (the eq tables are not read: their factors come from gb / gr and the host)
 // and write P0..P4, Q0, Q1 to out: P(X) = sum_t gb[t] (f_t + NC_t)(X), Q(X) = sum_t gr[t] Eval_t(X)
@@ -161,7 +167,8 @@ struct FusedArgs {
   const K2* gr = nullptr;  // [len / 2]: the same for r_in (has_eval)
 };
 constexpr int FUSED_MAX_SC = 8;
-void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s);
+void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s,
+                        const RoundSignal* sig = nullptr);
 // Fold every table with challenge r: dst[i] = src[2i] + r (src[2i+1] - src[2i]).
 void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStream_t s);
 void launch_fold_digits2(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1,

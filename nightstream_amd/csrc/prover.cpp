@@ -701,6 +701,27 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
   NS_CHECK(hipStreamSynchronize(st));  // host vectors above go out of scope
 }
 
+// Waits for a round's ready word (k_sum_partials_sig) by polling the coherent mailbox.  After
+// 50 ms (a very long round, or a failed launch) the stream wait takes over and reports errors.
+static void wait_round(const RoundSignal& sg, hipStream_t st) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  for (uint32_t spin = 1;; spin++) {
+    if (__atomic_load_n(sg.flag, __ATOMIC_ACQUIRE) == sg.seq) return;
+    __builtin_ia32_pause();
+    if ((spin & 1023) == 0 && clk::now() - t0 > std::chrono::milliseconds(50)) break;
+  }
+  NS_CHECK(hipStreamSynchronize(st));
+  if (__atomic_load_n(sg.flag, __ATOMIC_ACQUIRE) != sg.seq) throw Error(1, "round coefficients not published");
+}
+
+RoundSignal CcsOracle::next_signal() {
+  RoundSignal sg;
+  sg.flag = (uint64_t*)((char*)w_->mbox.get<K2>(8) + 192);
+  sg.seq = ++w_->round_seq;
+  return sg;
+}
+
 std::vector<K2> CcsOracle::round_coeffs() {
   if (pending_) {
     FusedArgs F;
@@ -722,11 +743,12 @@ std::vector<K2> CcsOracle::round_coeffs() {
     F.gb = w_->gt.as<K2>() + g_off_[j];
     F.gr = has_eval_ ? w_->gt.as<K2>() + g_total_ + g_off_[j] : nullptr;
     K2* mb = w_->mbox.get<K2>(8);
-    launch_round_fused(F, w_->wg.as<K2>(), mb, w_->ws.p, st_);
+    const RoundSignal sg = next_signal();
+    launch_round_fused(F, w_->wg.as<K2>(), mb, w_->ws.p, st_, &sg);
     cur_sc_ ^= 1;
     cur_nc_ ^= 1;
     pending_ = false;
-    NS_CHECK(hipStreamSynchronize(st_));
+    wait_round(sg, st_);
     // eq_beta(t; X) = C eq(beta_j, X) g(t): multiply P by C ((1 - beta_j) + (2 beta_j - 1) X)
     // (a degree-5 term would need f of degree 4, as in the unfused rounds it is dropped), and Q
     // by gamma^k C' ((1 - r_in_j) + (2 r_in_j - 1) X)
@@ -793,8 +815,9 @@ std::vector<K2> CcsOracle::round_coeffs() {
     T.pc = w_->pc.as<K2>();
   }
   K2* mb = w_->mbox.get<K2>(8);
-  launch_round_eval(T, s_.fpoly, w_->wg.as<K2>(), round_ == 0, mb, w_->ws.p, st_);
-  NS_CHECK(hipStreamSynchronize(st_));
+  const RoundSignal sg = next_signal();
+  launch_round_eval(T, s_.fpoly, w_->wg.as<K2>(), round_ == 0, mb, w_->ws.p, st_, &sg);
+  wait_round(sg, st_);
   return std::vector<K2>(mb, mb + 5);
 }
 

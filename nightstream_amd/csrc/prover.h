@@ -118,7 +118,8 @@ struct HostMailbox {
   }
   template <class T>
   T* get(size_t n) {
-    if (!p) NS_CHECK(hipHostMalloc(&p, std::max<size_t>(n * sizeof(T), 256), hipHostMallocDefault));
+    // coherent: kernels publish round coefficients and a ready word here while they run
+    if (!p) NS_CHECK(hipHostMalloc(&p, std::max<size_t>(n * sizeof(T), 256), hipHostMallocCoherent));
     return (T*)p;
   }
 };
@@ -136,7 +137,8 @@ struct PinnedStage {
 
 struct OracleWs {
   DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc, gt;
-  HostMailbox mbox;   // 8 K2: the round polynomial's coefficients
+  HostMailbox mbox;   // 8 K2: the round polynomial's coefficients; the ready word at byte 192
+  uint64_t round_seq = 0;  // last ready-word value handed to a round launch
   PinnedStage stage;  // the context's device-to-host staging
 };
 
@@ -281,6 +283,7 @@ class CcsOracle {
   bool pending_ = false;
   K2 pending_r_ = K2{0, 0};
   void flush();
+  RoundSignal next_signal();  // the mailbox's ready word and the value the next round stores
   // eq factors of the fused rounds: beta_r and r_in per row variable, the products of
   // eq(beta_i, r_i) and eq(r_in_i, r_i) over the folded variables, and the offsets of the chi
   // tables of the later variables (gt: every fused round's beta table, then its r_in table)
