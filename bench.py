@@ -35,7 +35,7 @@ KERNEL_NAMES = {
     "round1_eval": "k_round_eval<1>",
     "round_eval": "k_round_eval<2>",
     "table_fold": "k_fold_k",
-    "ajtai_gemm": "k_ajtai_gemm",
+    "ajtai_gemm": ("k_ajtai_gemm32p", "k_ajtai_gemm32", "k_ajtai_gemm"),
     "ydigits_gemm": "k_ydigits_gemm<4>",
     "rlc_zmix": "k_rowcontract<4>",
     "dec_split": "k_dec_split",
@@ -262,6 +262,20 @@ def main():
         "kernel_ms": {KT[i]: round(ms[i], 4) for i in range(len(KT))},
     }
 
+    # the same figures for every kernel class with measured time, largest summed time first (the
+    # roofline object above is the first of them)
+    by_kernel = []
+    tot_ms = sum(ms)
+    for i in sorted(range(len(KT)), key=lambda i: -ms[i]):
+        if counts[i] == 0:
+            continue
+        b_i, u_i, p_i = kernel_bound(KT[i])
+        avg_i = ms[i] / counts[i]
+        ach_i = (work[KT[i]] / counts[i] / (avg_i * 1e-3)) / (1e9 if u_i == "GB/s" else 1e12)
+        by_kernel.append({"kernel": KT[i], "bound": b_i, "unit": u_i, "achieved": round(ach_i, 1),
+                          "frac": round(ach_i / p_i, 4), "avg_ms": round(avg_i, 5), "launches": int(counts[i]),
+                          "share_of_kernel_time": round(ms[i] / tot_ms, 3)})
+
     cpu = None
     parity = None
     if args.cpu_steps > 0:
@@ -291,6 +305,7 @@ def main():
             "proof_bytes": len(proof), "setup_s": round(setup_s, 2),
         },
         "roofline": roofline,
+        "roofline_by_kernel": by_kernel,
         "cpu_baseline": cpu,
         "parity_vs_cpu": parity,
         "gpu_kernel_ms_per_step": round(sum(ms) / args.steps, 4),
@@ -703,7 +718,7 @@ def bench_ajtai(args):
         repeat_ok = np.array_equal(c, first)
         # each tile shape alone (its own K-split tuning), then back to the timed choice
         shape_ms = {}
-        for v, name in ((1, "16x16x64"), (2, "32x32x32")):
+        for v, name in ((1, "16x16x64"), (2, "32x32x32"), (3, "32x32x32 packed digits")):
             ns.ajtai_force_variant(v)
             cv = pp.commit_device(bufs)
             t0 = time.perf_counter()

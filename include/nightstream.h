@@ -88,10 +88,11 @@ int ns_ajtai_commit_batch(const ns_pp* pp, const int8_t* const* Z, size_t n_wit,
  * Same {-1, 0, 1} digit contract; device digits are not scanned (the caller's decomposition,
  * ns_decomp_b or the prover's own, produces only those). */
 int ns_ajtai_commit_batch_device(const ns_pp* pp, const int8_t* const* Z_dev, size_t n_wit, uint64_t* out);
-/* Tile shape of the commitment GEMM: 0 (default) times both shapes (16 x 16 x 64 MFMAs in
- * 192 x 352 blocks, 32 x 32 x 32 MFMAs in 256 x 256 blocks) and every K-split count on the first
- * commit of each (m, n_wit) and keeps the fastest; 1 or 2 forces that shape.  Clears the cached
- * choices.  The commitments do not depend on it.  No counterpart in the reference: a tuning and
+/* Tile shape of the commitment GEMM: 0 (default) times the three shapes (16 x 16 x 64 MFMAs in
+ * 192 x 352 blocks; 32 x 32 x 32 MFMAs in 256 x 256 blocks on int8 digits; the same blocks on
+ * digits packed to 2-bit codes, which relies on the {-1, 0, 1} digit contract) and every K-split
+ * count on the first commit of each (m, n_wit) and keeps the fastest; 1, 2 or 3 forces that
+ * shape.  Clears the cached choices.  The commitments do not depend on it.  No counterpart in the reference: a tuning and
  * testing hook for callers that want a fixed kernel (reproducible timing, per-shape tests). */
 int ns_ajtai_force_variant(int variant);
 

@@ -380,7 +380,8 @@ def _step_mem_array(mems, keep: list):
 
 
 def ajtai_force_variant(variant: int) -> None:
-    """Commitment GEMM tile shape: 0 timed choice (default), 1 16x16x64 tiles, 2 32x32x32 tiles."""
+    """Commitment GEMM tile shape: 0 timed choice (default), 1 16x16x64 tiles, 2 32x32x32 tiles on int8
+    digits, 3 32x32x32 tiles on 2-bit packed digits."""
     _check(lib().ns_ajtai_force_variant(int(variant)))
 
 

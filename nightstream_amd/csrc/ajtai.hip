@@ -153,13 +153,25 @@ static size_t aj_max_splits(size_t m) { return std::max<size_t>(1, std::min<size
 static size_t a3_groups_host(size_t nw) { return (nw * D + 255) / 256; }
 static size_t aj_npad_max(size_t nw) { return std::max(aj_groups(nw) * AJ_NBLK, a3_groups_host(nw) * 256); }
 
-// sized for the largest split count and either tile shape, so any tuned choice fits
-size_t ajtai_workspace_bytes(size_t m, size_t nw) {
-  size_t npad = aj_npad_max(nw);
-  size_t sp = aj_max_splits(m);
-  const size_t bt = aj_kmajor(m) ? nw * D * m + 256 : 0;
-  return (sp + (sp > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad * 4 + nw * D * 4 + 512 + bt;
+// Workspace: partial-sum planes for up to aj_max_splits(m) splits | S (digit row sums) | the
+// split-sum plane | K-step-major int8 digits (m >= 2^18) | packed 2-bit digits; every region
+// 256-byte aligned and sized for either tile shape, so any tuned choice fits.
+struct AjWs {
+  size_t part, S, red, bt, bp, total;
+};
+static size_t aj_up256(size_t x) { return (x + 255) & ~(size_t)255; }
+static AjWs aj_ws_layout(size_t m, size_t nw) {
+  const size_t npad = aj_npad_max(nw), sp = aj_max_splits(m), plane = (size_t)AJ_MROWS * npad * 4;
+  AjWs w;
+  w.part = 0;
+  w.S = aj_up256(sp * plane);
+  w.red = aj_up256(w.S + nw * D * 4);
+  w.bt = aj_up256(w.red + (sp > 1 ? plane : 0));
+  w.bp = aj_up256(w.bt + (aj_kmajor(m) ? nw * D * m : 0));
+  w.total = w.bp + nw * D * (m / 4) + 256;
+  return w;
 }
+size_t ajtai_workspace_bytes(size_t m, size_t nw) { return aj_ws_layout(m, nw).total; }
 
 // Workgroups go to the 8 XCDs round-robin (hardware block b runs on XCD b % 8).  The remap gives
 // each XCD a contiguous run of logical tiles, ordered (K split, M tile, N group), so the blocks
@@ -497,6 +509,213 @@ __global__ __launch_bounds__(256) void k_digit_rowsum(const int8_t* const* __res
   if (threadIdx.x == 0) S[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// ---- 32 x 32 x 32 variant with packed digits ----
+// Digits carry two bits of information, so the B operand travels as 2-bit codes (d & 3: 0, 1, 3)
+// and is expanded to int8 fragments in registers: per 64-wide K step and digit row, 16 bytes =
+// four 32-bit words, word g holding digits 16 g .. 16 g + 15 with digit 16 g + 4 i + j at bit
+// 8 j + 2 i, so (word >> 2 i) & 0x03030303 holds the codes of digits 16 g + 4 i .. + 3 in its
+// bytes and one byte permute maps codes to int8 (0 -> 0, 1 -> 1, 3 -> -1).  Words are stored in
+// the order g = 0, 2, 1, 3: lane half h of an MFMA B fragment needs words h (K half 0) and 2 + h
+// (K half 1), one 8-byte read.  Per K step a block stages 16 KB of limbs and 4 KB of packed
+// digits (against 32 KB for the int8 layout), and a wave reads 8 KB of limb fragments and 2 KB
+// of digit words from LDS (against 16 KB).
+constexpr int A4_STAGES = 4, A4_ABYTES = A3_BM * 64, A4_BBYTES = A3_BN * 16;
+constexpr int A4_STAGE_BYTES = A4_ABYTES + A4_BBYTES;  // 20 KB
+constexpr int A4_ALOADS = A4_ABYTES / 1024 / 4;         // limb load instructions per wave per step: 4
+constexpr int A4_VM_PER_STEP = A4_ALOADS + 1;           // + one 1 KB load of 64 digit records
+
+__global__ __launch_bounds__(256) void k_digits_pack(const int8_t* const* __restrict__ Zs, size_t nw, size_t m,
+                                                     uint32_t* __restrict__ Bp, int32_t* __restrict__ S) {
+  const size_t R = nw * D, nsteps = m / 64;
+  const size_t r = (size_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int q = threadIdx.x & 3;  // 16-digit quarter of the step = word g
+  if (r >= R) return;             // a row's four lanes leave together
+  const int8_t* src = Zs[r / D] + (r % D) * m + q * 16;
+  const int pos = (q & 1) * 2 + (q >> 1);
+  const size_t s0 = (size_t)blockIdx.y * KM_STEPS, s1 = min(s0 + KM_STEPS, nsteps);
+  int sum = 0;
+  for (size_t st = s0; st < s1; st++) {
+    const v4i v = *(const v4i*)(src + st * 64);  // v[i] byte j = digit 4 i + j of the quarter
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      w |= ((uint32_t)v[i] & 0x03030303u) << (2 * i);
+      sum += sum4_i8a(v[i]);
+    }
+    Bp[(st * R + r) * 4 + pos] = w;
+  }
+  sum += __shfl_xor(sum, 1, 64);
+  sum += __shfl_xor(sum, 2, 64);
+  if (q == 0) atomicAdd(&S[r], sum);
+}
+
+__global__ void k_zero_i32(int32_t* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// 16 digits (one packed word) to an int8 MFMA fragment
+__device__ __forceinline__ v4i a4_unpack(uint32_t w) {
+  constexpr uint32_t TAB = 0xFF000100u;  // selector 0 -> 0x00, 1 -> 0x01, 3 -> 0xFF
+  v4i f;
+  f[0] = (int)__builtin_amdgcn_perm(0u, TAB, w & 0x03030303u);
+  f[1] = (int)__builtin_amdgcn_perm(0u, TAB, (w >> 2) & 0x03030303u);
+  f[2] = (int)__builtin_amdgcn_perm(0u, TAB, (w >> 4) & 0x03030303u);
+  f[3] = (int)__builtin_amdgcn_perm(0u, TAB, (w >> 6) & 0x03030303u);
+  return f;
+}
+
+// Same grid, tiles and split layout as k_ajtai_gemm32 (256 x 256 blocks of four 128 x 128 waves).
+// NST stages rotate (four are as fast as seven: the loads are not latency-bound).  Every step
+// issues its 5 loads per wave unconditionally (steps past the end re-load the last step into a
+// stage nobody reads again), so the vmcnt waits are exact.  The LDS reads are software-pipelined
+// across steps: while the 16 MFMAs of step st's first K half run, the fragments of its second
+// half are read; then each wave waits for its own loads of st + 1 (later steps' loads stay in
+// flight), a bare barrier publishes every wave's slices, the loads of st + NST - 1 go into the
+// stage read at st - 1, and the first-half fragments and digit records of st + 1 are read behind
+// the 16 MFMAs of st's second half.  Digit rows past the last real row read whatever was loaded
+// there and are masked to zero codes.  Measured at m = 2^18, 64 witnesses, the MFMAs alone run
+// at 84% of peak, and each of the LDS reads, the unpacking and the loads adds its own time unless
+// the schedule interleaves it with the MFMAs (one wave per SIMD): the interleaved schedule below
+// took the kernel from 2.42 to 2.71 POP/s there.
+template <int NST>
+__global__ __launch_bounds__(256) void k_ajtai_gemm32p(const int8_t* __restrict__ limbs, size_t nsteps,
+                                                      const uint32_t* __restrict__ Bp, size_t nrows_b, size_t npad,
+                                                      int groups, int S, int32_t* __restrict__ part) {
+  __shared__ __align__(16) int8_t lds[NST * A4_STAGE_BYTES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const unsigned lid = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned per_split = (unsigned)A3_MBLK * (unsigned)groups;
+  const unsigned split = lid / per_split, rem = lid % per_split;
+  const size_t m0 = (size_t)(rem / groups) * A3_BM;
+  const size_t n0 = (size_t)(rem % groups) * A3_BN;
+  const size_t s_begin = (size_t)split * nsteps / S, s_end = ((size_t)split + 1) * nsteps / S;
+  const size_t astep = (size_t)AJ_MROWS * 64, bstep = nrows_b * 16;  // bytes per K step
+  const int8_t* asrc[A4_ALOADS];
+#pragma unroll
+  for (int i = 0; i < A4_ALOADS; i++) {
+    const int row = (wave * A4_ALOADS + i) * 16 + (lane >> 2), c = (lane & 3) ^ a3_swz(row);
+    asrc[i] = limbs + (m0 + row) * 64 + c * 16;
+  }
+  // digit records n0 + 64 wave + lane (row 0 for rows past the end: loaded, masked when read)
+  const size_t nb = n0 + (size_t)wave * 64 + lane;
+  const int8_t* bsrc = (const int8_t*)Bp + (nb < nrows_b ? nb : 0) * 16;
+  uint32_t bmask[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) bmask[nt] = n0 + wn * 128 + nt * 32 + (lane & 31) < nrows_b ? 0xFFFFFFFFu : 0u;
+  auto load_step = [&](size_t st, int buf) {
+    int8_t* base = lds + buf * A4_STAGE_BYTES;
+#pragma unroll
+    for (int i = 0; i < A4_ALOADS; i++)
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(asrc[i] + st * astep),
+                                       (__attribute__((address_space(3))) void*)(base + (wave * A4_ALOADS + i) * 1024),
+                                       16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(bsrc + st * bstep),
+                                     (__attribute__((address_space(3))) void*)(base + A4_ABYTES + wave * 1024), 16, 0, 0);
+  };
+  v16i acc[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[mt][nt][v] = 0;
+  const int r32 = lane & 31, khalf = lane >> 5;
+  // limb fragments of K half kh from stage buf
+  auto read_a = [&](int buf, int kh, v4i (&a)[4]) {
+    const int8_t* A = lds + buf * A4_STAGE_BYTES;
+    const int c = kh * 2 + khalf;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      const int row = wm * 128 + mt * 32 + r32;
+      a[mt] = *(const v4i*)(A + row * 64 + ((c ^ a3_swz(row)) << 4));
+    }
+  };
+  // this lane's digit records from stage buf (16 bytes per 32-row tile; words picked at use)
+  auto read_b = [&](int buf, v4i (&rec)[4]) {
+    const int8_t* B = lds + buf * A4_STAGE_BYTES + A4_ABYTES;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) rec[nt] = *(const v4i*)(B + (wn * 128 + nt * 32 + r32) * 16);
+  };
+  // digit fragments of K half kh from the records, and the 16 MFMAs of a K half (N tile outer,
+  // so the first four need only the first fragment)
+  auto unpack = [&](const v4i (&rec)[4], int kh, v4i (&b)[4]) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      const uint32_t w = (uint32_t)(khalf ? rec[nt][2 + kh] : rec[nt][kh]) & bmask[nt];
+      b[nt] = a4_unpack(w);
+    }
+  };
+  auto mfmas = [&](const v4i (&a)[4], const v4i (&b)[4]) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++)
+        acc[mt][nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+  };
+  static_assert(A4_VM_PER_STEP == 5 && NST >= 4 && 5 * (NST - 2) <= 63, "vmcnt immediates");
+  const size_t last = s_end - 1;
+#pragma unroll
+  for (int k = 0; k < NST - 1; k++) load_step(min(s_begin + k, last), k);
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(5 * (NST - 2)) : "memory");
+  asm volatile("s_barrier" ::: "memory");
+  v4i a0[4], a1[4], rc[4], rn[4];
+  read_a(0, 0, a0);
+  read_b(0, rc);
+  int buf = 0;
+  for (size_t st = s_begin; st < s_end; st++) {
+    // first K half of st: the first digit fragment, then each MFMA with a slice of the unpacking
+    // of both halves, the second half's 4 limb fragments read beside the first MFMAs (a wave of
+    // a SIMD to itself does not overlap its VALU work with its MFMAs on its own: the schedule
+    // has to interleave them)
+    v4i b0[4], b1[4];
+    read_a(buf, 1, a1);
+    unpack(rc, 0, b0);
+    mfmas(a0, b0);
+    unpack(rc, 1, b1);
+    __builtin_amdgcn_sched_group_barrier(0x002, 13, 0);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      if (i < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const int nbuf = buf + 1 == NST ? 0 : buf + 1;
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(5 * (NST - 3)) : "memory");
+    asm volatile("s_barrier" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    // second K half of st: the loads of st + NST - 1 and the reads of st + 1's first half and
+    // digit records beside its MFMAs
+    load_step(min(st + NST - 1, last), buf == 0 ? NST - 1 : buf - 1);
+    read_a(nbuf, 0, a0);
+    read_b(nbuf, rn);
+    mfmas(a1, b1);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      if (i < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      else if (i < 13) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) rc[nt] = rn[nt];
+    buf = nbuf;
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the re-loads of the last steps
+  int32_t* base = part + ((size_t)split * AJ_MROWS + m0 + wm * 128) * npad + n0 + wn * 128;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const size_t row = mt * 32 + (v >> 2) * 8 + khalf * 4 + (v & 3);
+        base[row * npad + nt * 32 + r32] = acc[mt][nt][v];
+      }
+}
+
 // block = (w, i), 8 waves: wave l sums limb l's anti-diagonals G_l[s] (int64) for s < 107;
 // then 107 threads combine the limbs in F_q and 54 threads reduce mod Phi_81.
 __global__ __launch_bounds__(512) void k_ajtai_epilogue(const int32_t* __restrict__ part, size_t splits,
@@ -547,7 +766,7 @@ __global__ __launch_bounds__(512) void k_ajtai_epilogue(const int32_t* __restric
 // commitments (exact int32 partial sums, at most 2^20 * 128 in magnitude), so it only moves
 // time.  A single lock serializes the first measurement of each shape.
 struct AjChoice {
-  int variant;  // 1: 16 x 16 x 64 tiles, 2: 32 x 32 x 32 tiles
+  int variant;  // 1: 16 x 16 x 64 tiles, 2: 32 x 32 x 32 tiles, 3: 32 x 32 x 32 tiles on packed digits
   size_t splits;
 };
 static std::mutex g_aj_mu;
@@ -556,56 +775,99 @@ static int g_aj_force = 0;
 
 void ajtai_force_variant(int v) {
   std::lock_guard<std::mutex> g(g_aj_mu);
-  if (v < 0 || v > 2) throw std::invalid_argument("Ajtai tile variant must be 0 (timed), 1 or 2");
+  if (v < 0 || v > 3) throw std::invalid_argument("Ajtai tile variant must be 0 (timed), 1, 2 or 3");
   g_aj_force = v;
   g_aj_chosen.clear();
 }
 
-static void aj_launch(int variant, const int8_t* limbs, size_t nsteps, const int8_t* const* Zs_dev, const int8_t* Bt,
-                      size_t nw, size_t m, size_t sp, int32_t* part, hipStream_t s, bool timed) {
-  if (variant == 2) {
+// the digit operand of each tile shape: int8 rows (Zs, or Bt when K-step-major) or packed words
+struct AjDigits {
+  const int8_t* const* Zs;
+  const int8_t* Bt;
+  const uint32_t* Bp;
+};
+
+static void aj_launch(int variant, const int8_t* limbs, size_t nsteps, const AjDigits& dg, size_t nw, size_t m,
+                      size_t sp, int32_t* part, hipStream_t s, bool timed) {
+  if (variant == 3) {
     const size_t groups = a3_groups(nw), npad = groups * A3_BN;
     const dim3 grid((unsigned)(A3_MBLK * groups * sp));
     if (timed)
-      NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm32, grid, dim3(A3_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m,
+      NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm32p<A4_STAGES>, grid, dim3(256), 0, s, limbs, nsteps, dg.Bp, nw * D, npad,
+                      (int)groups, (int)sp, part);
+    else
+      hipLaunchKernelGGL(k_ajtai_gemm32p<A4_STAGES>, grid, dim3(256), 0, s, limbs, nsteps, dg.Bp, nw * D, npad, (int)groups,
+                         (int)sp, part);
+  } else if (variant == 2) {
+    const size_t groups = a3_groups(nw), npad = groups * A3_BN;
+    const dim3 grid((unsigned)(A3_MBLK * groups * sp));
+    if (timed)
+      NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm32, grid, dim3(A3_THREADS), 0, s, limbs, nsteps, dg.Zs, dg.Bt, nw, m,
                       npad, (int)groups, (int)sp, part);
     else
-      hipLaunchKernelGGL(k_ajtai_gemm32, grid, dim3(A3_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m, npad,
+      hipLaunchKernelGGL(k_ajtai_gemm32, grid, dim3(A3_THREADS), 0, s, limbs, nsteps, dg.Zs, dg.Bt, nw, m, npad,
                          (int)groups, (int)sp, part);
   } else {
     const size_t groups = aj_groups(nw), npad = groups * AJ_NBLK;
     const dim3 grid((unsigned)(AJ_MBLK * groups * sp));
     if (timed)
-      NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, grid, dim3(AJ_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m, npad,
-                      (int)groups, (int)sp, part);
+      NS_LAUNCH_TIMED(KT_AJTAI_GEMM, k_ajtai_gemm, grid, dim3(AJ_THREADS), 0, s, limbs, nsteps, dg.Zs, dg.Bt, nw, m,
+                      npad, (int)groups, (int)sp, part);
     else
-      hipLaunchKernelGGL(k_ajtai_gemm, grid, dim3(AJ_THREADS), 0, s, limbs, nsteps, Zs_dev, Bt, nw, m, npad,
+      hipLaunchKernelGGL(k_ajtai_gemm, grid, dim3(AJ_THREADS), 0, s, limbs, nsteps, dg.Zs, dg.Bt, nw, m, npad,
                          (int)groups, (int)sp, part);
   }
 }
 static size_t aj_npad(int variant, size_t nw) {
-  return variant == 2 ? a3_groups(nw) * A3_BN : aj_groups(nw) * AJ_NBLK;
+  return variant >= 2 ? a3_groups(nw) * A3_BN : aj_groups(nw) * AJ_NBLK;
 }
 
-static AjChoice aj_tuned(const int8_t* limbs, size_t m, const int8_t* const* Zs_dev, const int8_t* Bt, size_t nw,
-                         int32_t* part, int32_t* red, hipStream_t s) {
+// Digit preparation of a tile shape: S (row sums) and the layout its GEMM reads.  The packed
+// layout's kernel accumulates S itself (zeroed first); the int8 shapes take k_digit_rowsum and,
+// for m >= 2^18, the K-step-major copy.
+static void aj_prepare(int variant, const int8_t* const* Zs_dev, size_t nw, size_t m, const AjDigits& dg, int32_t* S,
+                       hipStream_t s) {
+  const size_t nsteps = m / AJ_BK;
+  const dim3 tiles((unsigned)((nw * D + 63) / 64), (unsigned)((nsteps + KM_STEPS - 1) / KM_STEPS));
+  if (variant >= 3) {
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((nw * D + 255) / 256)), dim3(256), 0, s, S, nw * D);
+    hipLaunchKernelGGL(k_digits_pack, tiles, dim3(256), 0, s, Zs_dev, nw, m, const_cast<uint32_t*>(dg.Bp), S);
+  } else {
+    hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
+    if (dg.Bt) hipLaunchKernelGGL(k_digits_kmajor, tiles, dim3(256), 0, s, Zs_dev, nw, m, const_cast<int8_t*>(dg.Bt));
+  }
+}
+
+// Tile shape and K-split autotuner: the first commit of each (m, nw) in the process prepares the
+// digits in every layout, then times the three tile shapes (16 x 16 x 64 MFMAs in 192 x 352 blocks,
+// 32 x 32 x 32 MFMAs in 256 x 256 blocks from int8 or from packed digits) at every split count
+// from 1 to aj_max_splits(m) on its own inputs -- the GEMM and the reduction of its split planes,
+// which grows with the count (twice each, HIP events, the faster kept) -- and caches the fastest;
+// later commits prepare and use only the cached choice.  Every choice gives the same commitments
+// (exact int32 partial sums, at most 2^20 * 128 in magnitude), so it only moves time.  A single
+// lock serializes the first measurement of each shape.
+static AjChoice aj_tuned(const int8_t* limbs, size_t m, const int8_t* const* Zs_dev, const AjDigits& dg, size_t nw,
+                         int32_t* part, int32_t* S, int32_t* red, hipStream_t s, bool* prepared) {
   std::lock_guard<std::mutex> g(g_aj_mu);
   const auto key = std::make_pair(m, nw);
   auto it = g_aj_chosen.find(key);
+  *prepared = false;
   if (it != g_aj_chosen.end()) return it->second;
+  *prepared = true;
   const size_t nsteps = m / AJ_BK;
   hipEvent_t e0, e1;
   NS_CHECK(hipEventCreate(&e0));
   NS_CHECK(hipEventCreate(&e1));
+  for (int variant = 3; variant >= 1; variant -= 2) aj_prepare(variant, Zs_dev, nw, m, dg, S, s);
   AjChoice best{g_aj_force ? g_aj_force : 1, aj_splits(m, nw)};
   float best_ms = 1e30f;
-  for (int variant = 1; variant <= 2; variant++) {
+  for (int variant = 1; variant <= 3; variant++) {
     if (g_aj_force && variant != g_aj_force) continue;
     for (size_t sp = 1; sp <= aj_max_splits(m); sp++) {
       float ms = 1e30f;
       for (int rep = 0; rep < 2; rep++) {
         NS_CHECK(hipEventRecord(e0, s));
-        aj_launch(variant, limbs, nsteps, Zs_dev, Bt, nw, m, sp, part, s, false);
+        aj_launch(variant, limbs, nsteps, dg, nw, m, sp, part, s, false);
         if (sp > 1) launch_sum_splits(part, sp, (size_t)AJ_MROWS * aj_npad(variant, nw), red, s);
         NS_CHECK(hipEventRecord(e1, s));
         NS_CHECK(hipEventSynchronize(e1));
@@ -628,25 +890,19 @@ static AjChoice aj_tuned(const int8_t* limbs, size_t m, const int8_t* const* Zs_
 void ajtai_commit(const int8_t* limbs, size_t kappa, size_t m, const int8_t* const* Zs_dev, size_t nw,
                   uint64_t* out, void* workspace, hipStream_t s) {
   if (nw == 0) return;
-  const size_t npad_ws = aj_npad_max(nw), spmax = aj_max_splits(m);
   const size_t nsteps = m / AJ_BK;
-  // workspace: partial-sum planes for up to spmax splits | S | the split sums | K-major digits
-  int32_t* part = (int32_t*)workspace;
-  int32_t* S = part + spmax * (size_t)AJ_MROWS * npad_ws;
-  hipLaunchKernelGGL(k_digit_rowsum, dim3((unsigned)(nw * D)), dim3(256), 0, s, Zs_dev, m, S);
-  int8_t* Bt = nullptr;
-  if (aj_kmajor(m)) {
-    // after the split-sum scratch (which follows S), 256-byte aligned
-    const size_t used = (spmax + (spmax > 1 ? 1 : 0)) * (size_t)AJ_MROWS * npad_ws * 4 + nw * D * 4 + 512;
-    Bt = (int8_t*)(((uintptr_t)workspace + used + 255) & ~(uintptr_t)255);
-    hipLaunchKernelGGL(k_digits_kmajor, dim3((unsigned)((nw * D + 63) / 64), (unsigned)((nsteps + KM_STEPS - 1) / KM_STEPS)),
-                       dim3(256), 0, s, Zs_dev, nw, m, Bt);
-  }
-  // split-sum plane after S, 256-byte aligned
-  int32_t* red = (int32_t*)(((uintptr_t)(S + nw * D) + 255) & ~(uintptr_t)255);
-  const AjChoice ch = aj_tuned(limbs, m, Zs_dev, Bt, nw, part, red, s);
+  const AjWs L = aj_ws_layout(m, nw);
+  char* ws = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+  if (ws != (char*)workspace) throw std::invalid_argument("Ajtai workspace must be 256-byte aligned");
+  int32_t* part = (int32_t*)(ws + L.part);
+  int32_t* S = (int32_t*)(ws + L.S);
+  int32_t* red = (int32_t*)(ws + L.red);
+  AjDigits dg{Zs_dev, aj_kmajor(m) ? (const int8_t*)(ws + L.bt) : nullptr, (const uint32_t*)(ws + L.bp)};
+  bool prepared = false;  // a first commit of this shape prepares every layout while tuning
+  const AjChoice ch = aj_tuned(limbs, m, Zs_dev, dg, nw, part, S, red, s, &prepared);
+  if (!prepared) aj_prepare(ch.variant, Zs_dev, nw, m, dg, S, s);
   const size_t splits = ch.splits, npad = aj_npad(ch.variant, nw);
-  aj_launch(ch.variant, limbs, nsteps, Zs_dev, Bt, nw, m, splits, part, s, true);
+  aj_launch(ch.variant, limbs, nsteps, dg, nw, m, splits, part, s, true);
   const int32_t* G = part;
   if (splits > 1) {
     // reduce the K splits with coalesced loads before the anti-diagonal epilogue

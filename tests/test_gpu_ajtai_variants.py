@@ -1,5 +1,6 @@
-"""Both tile shapes of the commitment GEMM (16 x 16 x 64 MFMAs in 192 x 352 blocks, 32 x 32 x 32
-MFMAs in 256 x 256 blocks with direct-to-LDS staging) give the same commitments at every shape the
+"""The tile shapes of the commitment GEMM (16 x 16 x 64 MFMAs in 192 x 352 blocks, 32 x 32 x 32
+MFMAs in 256 x 256 blocks with direct-to-LDS staging of int8 digits, and the same blocks on digits
+packed to 2-bit codes) give the same commitments at every shape the
 prover uses, each checked against the oracle: the shard's DEC children (12 at 2^16), a padded
 N tile (13 and 5 witnesses), the sweep's 64 witnesses, a short m, and the K-step-major digit path
 (2^18).  Witnesses are dense where the oracle is cheap, else digits in 64 random columns (a
@@ -38,10 +39,11 @@ def test_tile_variants_agree_with_oracle(ns, oracle, forced, logm, nw):
             cols.append(c)
     bufs = [ns.DeviceBuffer.from_array(z.reshape(-1)) for z in Zs]
     got = {}
-    for v in (1, 2):
+    for v in (1, 2, 3):
         forced(v)
         got[v] = pp.commit_device(bufs)
-    assert np.array_equal(got[1], got[2])
+    for v in (2, 3):
+        assert np.array_equal(got[1], got[v]), f"tile shape {v}"
     check = [0, nw - 1] + list(rng.choice(nw, size=min(4, nw), replace=False))
     for w in sorted(set(int(x) for x in check)):
         if cols[w] is None:
