@@ -344,7 +344,9 @@ def run_batch(specs, workers: int, procs: int = 1, hw_queues: int = 1):
     from nightstream_amd.batch import ProcessProver, ProverPool, ShardFactory, build_shards, prove_batched, \
         prove_sequential
 
-    order = sorted(range(len(specs)), key=lambda i: -specs[i][0] * specs[i][1])
+    # longest estimated time first (a Twist/Shout step ~0.2 s of host transcript at any size)
+    order = sorted(range(len(specs)), key=lambda i: -specs[i][1] * (0.2 if specs[i][3] else
+                                                                    0.012 * specs[i][0] / 65536 + 0.002))
     t0 = time.perf_counter()
     pp = None
     if procs > 1:  # started before this process touches the GPU; each builds and warms its own
@@ -503,13 +505,14 @@ def bench_serve(args):
 
 
 def mix_measurement(workers: int, seed: int):
-    """Short in-process evidence for the batch and serve claims beside the shard line: 16 shards
-    (2^10..2^13 rows, 3-8 fold steps, every other one a Twist/Shout shard) proved on the plain
-    path and through `workers` pooled contexts (proofs compared), then ~8 s of Poisson arrivals
-    over the same shards at about 60% of the pooled throughput, with p50/p99 job latency."""
+    """Short in-process evidence for the batch and serve claims beside the shard line: 48 shards
+    (2^10..2^13 rows, 3-8 fold steps, every other one a Twist/Shout shard; three per pooled
+    context, so that the batched time is not one shard's serial chain) proved on the plain path
+    and through `workers` pooled contexts (proofs compared), then ~8 s of Poisson arrivals over the
+    same shards at about 60% of the pooled throughput, with p50/p99 job latency."""
     from nightstream_amd.batch import pp_bytes
     rng = np.random.default_rng(seed)
-    specs = batch_specs(rng, 16, 10, 13, 3, 8, True, variants=2)
+    specs = batch_specs(rng, 48, 10, 13, 3, 8, True, variants=2)
     rec, shards, seq, pool = run_batch(specs, workers)
     mean_steps = rec["fold_steps"] / rec["shards"]
     rate = round(0.6 * rec["batched_fold_steps_per_s"] / mean_steps, 2)

@@ -227,11 +227,21 @@ def prove_sequential(shards, pool: ProverPool | None = None):
     return out, time.perf_counter() - t0
 
 
+def est_seconds(shard) -> float:
+    """Rough proving time of a shard for longest-first scheduling: a Twist/Shout step costs
+    ~0.2 s of host transcript whatever its size (every ME binds its bus openings into Pi_RLC); a
+    CCS-only step ~12 ms at 2^16 rows on the MI355X, scaling with the rows, plus ~2 ms of fixed
+    host work."""
+    per = 0.2 if shard.mem is not None else 0.012 * shard.n / 65536 + 0.002
+    return shard.steps * per
+
+
 def prove_batched(shards, workers: int, pool: ProverPool | None = None):
-    """All shards through `workers` concurrent contexts (largest first to balance the tail)."""
+    """All shards through `workers` concurrent contexts (longest estimated time first, so the tail
+    is short)."""
     import concurrent.futures as cf
     pool = pool or ProverPool(workers)
-    order = sorted(range(len(shards)), key=lambda i: -shards[i].n * shards[i].steps)
+    order = sorted(range(len(shards)), key=lambda i: -est_seconds(shards[i]))
     out = [None] * len(shards)
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
