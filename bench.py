@@ -146,6 +146,8 @@ def main():
                     help="shard: one 2^16-row shard (default); batch: 128 shards of 2^10..2^16 rows, plain path vs "
                          "a pool of concurrent prover contexts; serve: Poisson arrivals with continuous admission")
     ap.add_argument("--workers", type=int, default=16, help="concurrent prover contexts (batch / serve)")
+    ap.add_argument("--shared-rounds", type=int, default=0, choices=[0, 1],
+                    help="batch / serve: pooled contexts share fused-round launches (1) or not (0)")
     ap.add_argument("--procs", type=int, default=8,
                     help="batch: prover processes sharing the GPU, --workers contexts split evenly (1: one process)")
     ap.add_argument("--proc-hw-queues", type=int, default=1,
@@ -182,6 +184,8 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
+    from nightstream_amd import batch as _batch
+    _batch.SHARE_ROUNDS = bool(args.shared_rounds)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "twist":
@@ -370,6 +374,7 @@ def run_batch(specs, workers: int, procs: int = 1, hw_queues: int = 1):
     t_warm = time.perf_counter() - t0
     seq, t_seq = prove_sequential(shards, pool1)
     _stderr(f"batch: plain path {t_seq:.2f} s")
+    hub0 = ns.shared_round_stats()
     if pp is None:
         bat, t_bat = prove_batched(shards, workers, pooln)
         created = pooln.created - workers
@@ -389,6 +394,11 @@ def run_batch(specs, workers: int, procs: int = 1, hw_queues: int = 1):
         "contexts_created_in_timed_runs": pool1.created - 1 + created,
         "proofs_bit_identical": all(a == b for a, b in zip(seq, bat)),
     }
+    hub1 = ns.shared_round_stats()
+    if pp is None and hub1[1] > hub0[1]:
+        # fused Pi_CCS rounds of concurrent contexts issued together (this process only)
+        rec["shared_round_launches"] = {"rounds": hub1[1] - hub0[1], "launches": hub1[0] - hub0[0],
+                                        "rounds_per_launch": round((hub1[1] - hub0[1]) / max(1, hub1[0] - hub0[0]), 2)}
     return rec, shards, seq, (pooln or pool1)
 
 

@@ -234,6 +234,15 @@ int ns_prove_shard_ob(ns_prover* p, const uint64_t* const* z_steps, int z_on_dev
 int ns_shard_verify_ob(const ns_ccs* s, const uint64_t* const* x_steps, size_t n_steps, size_t m_in,
                        const ns_step_mem* mem, const ns_output_binding* ob, const uint64_t* mcs_commitments,
                        const uint8_t* proof, size_t proof_len);
+/* Shared launches for the fused Pi_CCS rounds (rounds >= 3) of concurrent provers on one GPU
+ * (process-wide; off by default): a prover queues its round, and whichever prover finds no launch
+ * being issued issues every queued round -- one alone as its own launch, several as one ragged
+ * launch with a job array -- on a shared stream.  Proofs do not depend on it.  The counters
+ * report launches issued and rounds carried since the process started.  No counterpart in the
+ * reference (its sessions do not share a device): for servers and batch provers running many
+ * sessions at once. */
+int ns_set_shared_rounds(int on);
+int ns_shared_round_stats(uint64_t* launches, uint64_t* rounds);
 /* Per-kernel timing of the next proves (HIP events on the prover's stream); classes:
  * 0 round-0 eval, 1 round-2 eval, 2 table fold, 3 Ajtai GEMM, 4 digit x K GEMM, 5 RLC Z_mix,
  * 6 DEC split, 7 round-1 eval, 8 fused fold + round (rounds >= 3).  ms[i] is the summed

@@ -23,7 +23,8 @@ EXPORTS = [
     "ns_last_error", "ns_device_count", "ns_free", "ns_device_alloc", "ns_device_free", "ns_memcpy_h2d",
     "ns_memcpy_d2h", "ns_device_synchronize", "ns_params_goldilocks_auto_r1cs_ccs", "ns_ccs_from_r1cs",
     "ns_ccs_digest", "ns_ccs_free", "ns_pp_setup_seeded", "ns_pp_from_coeffs", "ns_pp_coeffs", "ns_pp_free",
-    "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ajtai_force_variant", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
+    "ns_decomp_b", "ns_ajtai_commit_batch", "ns_ajtai_commit_batch_device", "ns_ajtai_force_variant",
+    "ns_set_shared_rounds", "ns_shared_round_stats", "ns_ccs_oracle_new", "ns_oracle_evals", "ns_oracle_fold",
     "ns_oracle_num_rounds", "ns_oracle_degree_bound", "ns_oracle_y_eval", "ns_oracle_free", "ns_rlc_zmix",
     "ns_dec_split", "ns_dec_split_device_noalloc", "ns_prover_new", "ns_prove_shard_mem", "ns_shard_verify", "ns_prove_shard_ob", "ns_shard_verify_ob", "ns_rlc", "ns_dec", "ns_prover_rebind", "ns_prove_shard", "ns_prove_shard_device", "ns_prover_set_timing",
     "ns_prover_kernel_stats", "ns_prover_phase_stats", "ns_prover_phase_perms", "ns_prover_free",
@@ -88,6 +89,8 @@ def _load():
     L.ns_ajtai_commit_batch.argtypes = [vp, vp, sz, vp]
     L.ns_ajtai_commit_batch_device.argtypes = [vp, vp, sz, vp]
     L.ns_ajtai_force_variant.argtypes = [C.c_int]
+    L.ns_set_shared_rounds.argtypes = [C.c_int]
+    L.ns_shared_round_stats.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.ns_ccs_oracle_new.argtypes = [vp, vp, sz, vp, vp, vp, vp, sz, sz, vp, C.POINTER(vp)]
     L.ns_oracle_evals.argtypes = [vp, vp, sz, vp]
     L.ns_oracle_fold.argtypes = [vp, vp]
@@ -377,6 +380,28 @@ def _step_mem_array(mems, keep: list):
         arr[s] = _StepMem(len(sm.luts), C.cast(luts, C.c_void_p), len(sm.mems), C.cast(ms, C.c_void_p))
     keep.append(arr)
     return arr
+
+
+def set_shared_rounds(on: bool) -> None:
+    """Shared launches for the fused rounds of concurrent provers on this GPU (process-wide)."""
+    global _shared_on
+    _check(lib().ns_set_shared_rounds(1 if on else 0))
+    _shared_on = bool(on)
+
+
+_shared_on = False
+
+
+def shared_rounds_enabled() -> bool:
+    """Whether the last set_shared_rounds call of this process turned shared rounds on."""
+    return _shared_on
+
+
+def shared_round_stats():
+    """(launches issued, rounds carried) by the shared-round launcher since the process started."""
+    a, b = C.c_uint64(0), C.c_uint64(0)
+    _check(lib().ns_shared_round_stats(C.byref(a), C.byref(b)))
+    return a.value, b.value
 
 
 def ajtai_force_variant(variant: int) -> None:

@@ -19,6 +19,7 @@ transcript and the kernels are deterministic).
 """
 from __future__ import annotations
 
+import contextlib
 import dataclasses
 import queue
 import threading
@@ -227,6 +228,25 @@ def prove_sequential(shards, pool: ProverPool | None = None):
     return out, time.perf_counter() - t0
 
 
+# Pools can share fused-round launches among their concurrent contexts (proofs are the same
+# either way).  Off by default: on the MI355X the rounds of 16 contexts coincide rarely (1.02
+# rounds per shared launch on a 64-shard CCS-only batch), and issuing them on one shared stream
+# gives up the overlap of the contexts' own streams (1,026 against 1,285 fold-steps/s).
+SHARE_ROUNDS = False
+
+
+@contextlib.contextmanager
+def shared_rounds(on: bool):
+    """Shared launches for the fused rounds of the contexts proving at once while the block runs
+    (the plain path keeps one launch pair per round)."""
+    import nightstream_amd as ns
+    ns.set_shared_rounds(on and SHARE_ROUNDS)
+    try:
+        yield
+    finally:
+        ns.set_shared_rounds(False)
+
+
 def est_seconds(shard) -> float:
     """Rough proving time of a shard for longest-first scheduling: a Twist/Shout step costs
     ~0.2 s of host transcript whatever its size (every ME binds its bus openings into Pi_RLC); a
@@ -244,7 +264,7 @@ def prove_batched(shards, workers: int, pool: ProverPool | None = None):
     order = sorted(range(len(shards)), key=lambda i: -est_seconds(shards[i]))
     out = [None] * len(shards)
     t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+    with shared_rounds(workers > 1), cf.ThreadPoolExecutor(max_workers=workers) as ex:
         futs = {ex.submit(pool.prove, shards[i]): i for i in order}
         for f in cf.as_completed(futs):
             out[futs[f]] = f.result()
@@ -263,7 +283,7 @@ def build_shards(fac: ShardFactory, specs):
     return [cache[(n, seed, bool(memory))].with_steps(steps) for n, steps, seed, memory in specs]
 
 
-def _proc_worker(pp_seed, specs, threads, tasks, results):
+def _proc_worker(pp_seed, specs, threads, tasks, results, share_rounds=True):
     """One prover process: builds the shards of `specs` on its own HIP runtime, warms `threads`
     contexts, reports ready, then proves shard indices from `tasks` on `threads` threads until a
     None per thread arrives, putting (index, proof) on `results`."""
@@ -271,6 +291,9 @@ def _proc_worker(pp_seed, specs, threads, tasks, results):
     shards = build_shards(fac, specs)
     pool = ProverPool(threads)
     pool.warm(shards)
+    if threads > 1 and share_rounds:
+        import nightstream_amd as ns
+        ns.set_shared_rounds(True)
     results.put(("ready", pool.created))
 
     def loop():
@@ -303,7 +326,7 @@ class ProcessProver:
         self.n = len(specs)
         self.procs, self.threads = procs, threads
         self.tasks, self.results = ctx.Queue(), ctx.Queue()
-        self.ps = [ctx.Process(target=_proc_worker, args=(pp_seed, specs, threads, self.tasks, self.results),
+        self.ps = [ctx.Process(target=_proc_worker, args=(pp_seed, specs, threads, self.tasks, self.results, SHARE_ROUNDS),
                                daemon=True) for _ in range(procs)]
         old = os.environ.get("GPU_MAX_HW_QUEUES")
         if hw_queues:  # hardware queues per worker process (the children read it at HIP init)
@@ -410,24 +433,25 @@ def serve(jobs, rate: float, duration: float, workers: int, hbm_cap: float, seed
                 cond.notify_all()
 
     threads = [threading.Thread(target=worker, daemon=True) for _ in range(workers)]
-    for th in threads:
-        th.start()
-    for i, arr in enumerate(arrivals):
-        now = time.perf_counter() - t0
-        if arr > now:
-            time.sleep(arr - now)
-        sh = jobs[i % len(jobs)]
-        fp = footprint_bytes(sh.n) + (mem_footprint_bytes(sh.n) if sh.mem is not None else 0)
-        with cond:
-            # FIFO admission: wait for a slot and for HBM headroom
-            while state["free"] == 0 or state["mem"] + fp > hbm_cap:
-                cond.wait()
-            state["free"] -= 1
-            state["mem"] += fp
-        pending.put((i, sh, arr, fp))
-    for _ in threads:
-        pending.put(None)
-    for th in threads:
-        th.join()
+    with shared_rounds(workers > 1):
+        for th in threads:
+            th.start()
+        for i, arr in enumerate(arrivals):
+            now = time.perf_counter() - t0
+            if arr > now:
+                time.sleep(arr - now)
+            sh = jobs[i % len(jobs)]
+            fp = footprint_bytes(sh.n) + (mem_footprint_bytes(sh.n) if sh.mem is not None else 0)
+            with cond:
+                # FIFO admission: wait for a slot and for HBM headroom
+                while state["free"] == 0 or state["mem"] + fp > hbm_cap:
+                    cond.wait()
+                state["free"] -= 1
+                state["mem"] += fp
+            pending.put((i, sh, arr, fp))
+        for _ in threads:
+            pending.put(None)
+        for th in threads:
+            th.join()
     results.sort(key=lambda r: r.job)
     return results

@@ -224,6 +224,19 @@ int ns_ajtai_force_variant(int variant) {
   return guard([&] { ajtai_force_variant(variant); });
 }
 
+int ns_set_shared_rounds(int on) {
+  return guard([&] { set_shared_rounds(on != 0); });
+}
+
+int ns_shared_round_stats(uint64_t* launches, uint64_t* rounds) {
+  return guard([&] {
+    if (!launches || !rounds) throw Error(1, "null output");
+    const FusedHubStats st = fused_hub_stats();
+    *launches = st.launches;
+    *rounds = st.jobs;
+  });
+}
+
 static std::vector<K2> kvec(const uint64_t* p, size_t n) {
   std::vector<K2> v(n);
   for (size_t i = 0; i < n; i++) v[i] = K2{p[2 * i], p[2 * i + 1]};

@@ -474,6 +474,7 @@ void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t 
 // ---------------- round polynomial ----------------
 constexpr int RB = 256;        // threads per block
 constexpr int RMAXBLK = 2048;  // partial slots
+constexpr int FUSED_NCOEF = 7;  // P0..P4, Q0, Q1 of a fused round
 
 struct RoundArgs {
   size_t len;
@@ -847,6 +848,24 @@ __global__ __launch_bounds__(512) void k_sum_partials_sig(const K2* __restrict__
   }
 }
 
+// Block j: the sum of job j's partial coefficients, published with its ready word.
+__global__ __launch_bounds__(512) void k_sum_partials_multi(const FusedJob* __restrict__ jobs) {
+  __shared__ K2 res[8];
+  const FusedJob& J = jobs[blockIdx.x];
+  const int d = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (d < FUSED_NCOEF) {
+    K2 v = kzero();
+    for (int b = lane; b < J.nblk; b += 64) v = kadd(v, J.partials[b * FUSED_NCOEF + d]);
+    v = wave_sum(v);
+    if (lane == DPP_LANE) res[d] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < FUSED_NCOEF; i++) J.out[i] = res[i];
+    __hip_atomic_store(J.flag, J.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 static void launch_sum_partials(const K2* partials, size_t nblk, int ncoef, K2* out, const RoundSignal* sig,
                                 hipStream_t s) {
   if (sig && sig->flag)
@@ -918,7 +937,6 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
 // kernel boundary orders them; a device-scope fence per block would write back L2).
 __device__ __forceinline__ K2 fold2(K2 lo, K2 hi, K2 r, uint64_t r7) { return kadd(lo, kmul_w7(r, r7, ksub(hi, lo))); }
 
-constexpr int FUSED_NCOEF = 7;
 
 // scalar table q of pair t folded at 2t, 2t+1 for the next round
 __device__ __forceinline__ void fused_fold_scalar(const FusedArgs& A, size_t t, int q) {
@@ -1034,11 +1052,10 @@ __device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __re
   in[3] = s6;
 }
 
-__global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
-                                                     int G) {
-  // row weights come from global memory (11 KB, cache-resident): a short round's block needs a
-  // few of them, so staging all k*54 in LDS would cost more than it saves
-  __shared__ K2 red[4 * FUSED_NCOEF];
+// One block's share of a fused round: block bid of nb blocks over the work items of A.  Shared by
+// the single-session launch and the ragged launch over several sessions' rounds.
+__device__ __forceinline__ void fused_block(const FusedArgs& A, const K2* __restrict__ wg, K2* __restrict__ partials,
+                                           int G, unsigned bid, unsigned nb, K2* red) {
   const int nrow = A.k * D;
   const size_t pairs = A.len / 2;
   const int R = (nrow + G - 1) / G;
@@ -1060,7 +1077,7 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
     fdeg = max(fdeg, min(dg, 4));
   }
   const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+  for (size_t idx = (size_t)bid * blockDim.x + threadIdx.x; idx < total; idx += (size_t)nb * blockDim.x) {
     if (idx >= nnc && idx < base_s) continue;
     const bool nc_item = idx < nnc;
     has_s |= !nc_item;
@@ -1122,12 +1139,30 @@ __global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __res
   if (threadIdx.x < FUSED_NCOEF) {
     K2 v = kzero();
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = kadd(v, red[w * FUSED_NCOEF + threadIdx.x]);
-    partials[blockIdx.x * FUSED_NCOEF + threadIdx.x] = v;
+    partials[bid * FUSED_NCOEF + threadIdx.x] = v;
   }
 }
 
-void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s,
-                        const RoundSignal* sig) {
+__global__ __launch_bounds__(RB) void k_round_fused(FusedArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
+                                                     int G) {
+  // row weights come from global memory (11 KB, cache-resident): a short round's block needs a
+  // few of them, so staging all k*54 in LDS would cost more than it saves
+  __shared__ K2 red[4 * FUSED_NCOEF];
+  fused_block(A, wg, partials, G, blockIdx.x, gridDim.x, red);
+}
+
+// Several sessions' fused rounds in one launch: job j owns blocks [off_j, off_j + nblk_j) in the
+// order of the array (at most FUSED_MAX_JOBS jobs, found by a scan of the block counts).
+__global__ __launch_bounds__(RB) void k_round_fused_multi(const FusedJob* __restrict__ jobs, int njobs) {
+  __shared__ K2 red[4 * FUSED_NCOEF];
+  unsigned b = blockIdx.x;
+  int j = 0;
+  while (j + 1 < njobs && b >= (unsigned)jobs[j].nblk) b -= (unsigned)jobs[j++].nblk;
+  const FusedJob& J = jobs[j];
+  fused_block(J.A, J.wg, J.partials, J.G, b, (unsigned)J.nblk, red);
+}
+
+FusedJob fused_job(const FusedArgs& A, const K2* wg, K2* out, void* ws, const RoundSignal& sig) {
   const size_t pairs = A.len / 2, nrow = (size_t)A.k * D;
   if (pairs == 0 || (pairs & (pairs - 1))) throw std::invalid_argument("fused round: table length must be a power of two >= 2");
   if (A.nf + 1 > FUSED_MAX_SC - (A.has_eval ? 2 : 0)) throw std::invalid_argument("fused round: too many scalar tables");
@@ -1137,14 +1172,41 @@ void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hip
   while (G < nrow && pairs * G < target) G *= 2;
   if (G > nrow) G = nrow;
   G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
-  K2* partials = (K2*)ws;
   const size_t nfold = (size_t)A.nf + (A.has_eval ? 1 : 0);
   const size_t base_s = (pairs * G + 63) & ~(size_t)63;
   size_t nblk = (base_s + pairs * (1 + (A.has_eval ? 1 : 0) + nfold) + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
-  NS_LAUNCH_TIMED(KT_FUSED, k_round_fused, dim3((unsigned)nblk), dim3(RB), 0, s, A, wg, partials, (int)G);
-  launch_sum_partials(partials, nblk, FUSED_NCOEF, out, sig, s);
+  FusedJob J;
+  J.A = A;
+  J.wg = wg;
+  J.partials = (K2*)ws;
+  J.out = out;
+  J.flag = sig.flag;
+  J.seq = sig.seq;
+  J.G = (int)G;
+  J.nblk = (int)nblk;
+  return J;
+}
+
+void launch_fused_job(const FusedJob& J, hipStream_t s) {
+  NS_LAUNCH_TIMED(KT_FUSED, k_round_fused, dim3((unsigned)J.nblk), dim3(RB), 0, s, J.A, J.wg, J.partials, J.G);
+  RoundSignal sig;
+  sig.flag = J.flag;
+  sig.seq = J.seq;
+  launch_sum_partials(J.partials, (size_t)J.nblk, FUSED_NCOEF, J.out, J.flag ? &sig : nullptr, s);
+}
+
+void launch_fused_jobs(const FusedJob* jobs_dev, int njobs, size_t total_blocks, hipStream_t s) {
+  if (njobs < 1 || njobs > FUSED_MAX_JOBS) throw std::invalid_argument("fused jobs: 1..FUSED_MAX_JOBS per launch");
+  hipLaunchKernelGGL(k_round_fused_multi, dim3((unsigned)total_blocks), dim3(RB), 0, s, jobs_dev, njobs);
+  hipLaunchKernelGGL(k_sum_partials_multi, dim3((unsigned)njobs), dim3(512), 0, s, jobs_dev);
+}
+
+void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s,
+                        const RoundSignal* sig) {
+  RoundSignal none;
+  launch_fused_job(fused_job(A, wg, out, ws, sig ? *sig : none), s);
 }
 
 // ---------------- folds ----------------

@@ -169,6 +169,24 @@ struct FusedArgs {
 constexpr int FUSED_MAX_SC = 8;
 void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s,
                         const RoundSignal* sig = nullptr);
+// One session's fused round as a self-contained job (its arguments, weights, partial-sum scratch,
+// coefficient mailbox and ready word), so that the rounds of concurrent sessions can share a
+// launch: launch_fused_jobs runs up to FUSED_MAX_JOBS jobs (the array in device memory) in one
+// fused launch and one partial-sum launch; every job's coefficients and ready word land as in
+// its own launch.
+struct FusedJob {
+  FusedArgs A;
+  const K2* wg;
+  K2* partials;
+  K2* out;
+  uint64_t* flag;
+  uint64_t seq;
+  int G, nblk;
+};
+constexpr int FUSED_MAX_JOBS = 32;
+FusedJob fused_job(const FusedArgs& A, const K2* wg, K2* out, void* ws, const RoundSignal& sig);
+void launch_fused_job(const FusedJob& J, hipStream_t s);
+void launch_fused_jobs(const FusedJob* jobs_dev, int njobs, size_t total_blocks, hipStream_t s);
 // Fold every table with challenge r: dst[i] = src[2i] + r (src[2i+1] - src[2i]).
 void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStream_t s);
 void launch_fold_digits2(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1,

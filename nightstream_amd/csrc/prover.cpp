@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 
 namespace ns {
@@ -715,6 +716,116 @@ static void wait_round(const RoundSignal& sg, hipStream_t st) {
   if (__atomic_load_n(sg.flag, __ATOMIC_ACQUIRE) != sg.seq) throw Error(1, "round coefficients not published");
 }
 
+// ---------------- shared fused rounds (FusedHub) ----------------
+namespace {
+std::atomic<bool> g_shared_rounds{false};
+std::atomic<uint64_t> g_hub_launches{0}, g_hub_jobs{0};
+
+class FusedHub {
+ public:
+  static FusedHub& current() {
+    static std::mutex mu;
+    static std::vector<std::unique_ptr<FusedHub>> hubs;
+    int dev = 0;
+    NS_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    if ((size_t)dev >= hubs.size()) hubs.resize(dev + 1);
+    if (!hubs[dev]) hubs[dev].reset(new FusedHub());
+    return *hubs[dev];
+  }
+  hipStream_t stream() const { return st_; }
+  // queue a round; returns once it has been issued (by this thread or the combiner at work)
+  void submit(const FusedJob& J, hipEvent_t wait_first) {
+    std::unique_lock<std::mutex> lk(mu_);
+    pending_.push_back({J, wait_first});
+    if (combining_) return;
+    combining_ = true;
+    while (!pending_.empty()) {
+      std::vector<Queued> batch;
+      batch.swap(pending_);
+      lk.unlock();
+      try {
+        issue(batch);
+      } catch (...) {
+        lk.lock();
+        combining_ = false;
+        throw;
+      }
+      lk.lock();
+    }
+    combining_ = false;
+  }
+
+ private:
+  struct Queued {
+    FusedJob job;
+    hipEvent_t wait_first;
+  };
+  static constexpr int SLOTS = 8;
+  std::mutex mu_;
+  std::vector<Queued> pending_;
+  bool combining_ = false;
+  hipStream_t st_ = nullptr;
+  FusedJob* host_[SLOTS] = {};
+  FusedJob* dev_[SLOTS] = {};
+  hipEvent_t ev_[SLOTS] = {};
+  bool live_[SLOTS] = {};
+  int next_ = 0;
+
+  FusedHub() {
+    NS_CHECK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    for (int i = 0; i < SLOTS; i++) {
+      NS_CHECK(hipHostMalloc((void**)&host_[i], FUSED_MAX_JOBS * sizeof(FusedJob), hipHostMallocDefault));
+      NS_CHECK(hipMalloc((void**)&dev_[i], FUSED_MAX_JOBS * sizeof(FusedJob)));
+      NS_CHECK(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+    }
+  }
+  // only the combiner calls this (combining_ is held), so slots need no lock
+  void issue(std::vector<Queued>& batch) {
+    for (auto& q : batch)
+      if (q.wait_first) NS_CHECK(hipStreamWaitEvent(st_, q.wait_first, 0));
+    for (size_t at = 0; at < batch.size(); at += FUSED_MAX_JOBS) {
+      const size_t nj = std::min(batch.size() - at, (size_t)FUSED_MAX_JOBS);
+      g_hub_launches++;
+      g_hub_jobs += nj;
+      if (nj == 1) {
+        launch_fused_job(batch[at].job, st_);
+        continue;
+      }
+      const int s = next_;
+      next_ = (next_ + 1) % SLOTS;
+      if (live_[s]) NS_CHECK(hipEventSynchronize(ev_[s]));  // the slot's last launch has read its jobs
+      size_t blocks = 0;
+      for (size_t i = 0; i < nj; i++) {
+        host_[s][i] = batch[at + i].job;
+        blocks += (size_t)batch[at + i].job.nblk;
+      }
+      NS_CHECK(hipMemcpyAsync(dev_[s], host_[s], nj * sizeof(FusedJob), hipMemcpyHostToDevice, st_));
+      launch_fused_jobs(dev_[s], (int)nj, blocks, st_);
+      NS_CHECK(hipEventRecord(ev_[s], st_));
+      live_[s] = true;
+    }
+  }
+};
+}  // namespace
+
+void set_shared_rounds(bool on) { g_shared_rounds = on; }
+bool shared_rounds() { return g_shared_rounds; }
+FusedHubStats fused_hub_stats() {
+  FusedHubStats s;
+  s.launches = g_hub_launches;
+  s.jobs = g_hub_jobs;
+  return s;
+}
+
+void CcsOracle::join_hub() {
+  if (!hub_used_) return;
+  hub_used_ = false;
+  if (!w_->hub_out) NS_CHECK(hipEventCreateWithFlags(&w_->hub_out, hipEventDisableTiming));
+  NS_CHECK(hipEventRecord(w_->hub_out, FusedHub::current().stream()));
+  NS_CHECK(hipStreamWaitEvent(st_, w_->hub_out, 0));
+}
+
 RoundSignal CcsOracle::next_signal() {
   RoundSignal sg;
   sg.flag = (uint64_t*)((char*)w_->mbox.get<K2>(8) + 192);
@@ -744,11 +855,26 @@ std::vector<K2> CcsOracle::round_coeffs() {
     F.gr = has_eval_ ? w_->gt.as<K2>() + g_total_ + g_off_[j] : nullptr;
     K2* mb = w_->mbox.get<K2>(8);
     const RoundSignal sg = next_signal();
-    launch_round_fused(F, w_->wg.as<K2>(), mb, w_->ws.p, st_, &sg);
+    hipStream_t wst = st_;
+    if (shared_rounds()) {
+      // the first shared round waits for this stream's work; later ones follow on the hub stream
+      hipEvent_t first = nullptr;
+      if (!hub_used_) {
+        if (!w_->hub_in) NS_CHECK(hipEventCreateWithFlags(&w_->hub_in, hipEventDisableTiming));
+        NS_CHECK(hipEventRecord(w_->hub_in, st_));
+        first = w_->hub_in;
+      }
+      FusedHub& hub = FusedHub::current();
+      hub_used_ = true;
+      hub.submit(fused_job(F, w_->wg.as<K2>(), mb, w_->ws.p, sg), first);
+      wst = hub.stream();
+    } else {
+      launch_round_fused(F, w_->wg.as<K2>(), mb, w_->ws.p, st_, &sg);
+    }
     cur_sc_ ^= 1;
     cur_nc_ ^= 1;
     pending_ = false;
-    wait_round(sg, st_);
+    wait_round(sg, wst);
     // eq_beta(t; X) = C eq(beta_j, X) g(t): multiply P by C ((1 - beta_j) + (2 beta_j - 1) X)
     // (a degree-5 term would need f of degree 4, as in the unfused rounds it is dropped), and Q
     // by gamma^k C' ((1 - r_in_j) + (2 r_in_j - 1) X)
@@ -822,6 +948,7 @@ std::vector<K2> CcsOracle::round_coeffs() {
 }
 
 void CcsOracle::flush() {
+  join_hub();
   if (!pending_) return;
   const size_t prev = len_ * 2;
   launch_fold_k(w_->sc[cur_sc_].as<K2>(), (size_t)nsc_, prev, w_->sc[cur_sc_ ^ 1].as<K2>(), pending_r_, st_);

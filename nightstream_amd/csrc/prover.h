@@ -140,7 +140,31 @@ struct OracleWs {
   HostMailbox mbox;   // 8 K2: the round polynomial's coefficients; the ready word at byte 192
   uint64_t round_seq = 0;  // last ready-word value handed to a round launch
   PinnedStage stage;  // the context's device-to-host staging
+  // the stream order between this context's stream and the shared-round stream (see FusedHub)
+  hipEvent_t hub_in = nullptr, hub_out = nullptr;
+  OracleWs() = default;
+  OracleWs(const OracleWs&) = delete;
+  OracleWs& operator=(const OracleWs&) = delete;
+  ~OracleWs() {
+    if (hub_in) (void)hipEventDestroy(hub_in);
+    if (hub_out) (void)hipEventDestroy(hub_out);
+  }
 };
+
+// Shared launches for the fused rounds of concurrent sessions on one GPU (flat combining).  A
+// session queues its round as a FusedJob; the first session to find no combiner at work becomes
+// the combiner and issues every queued round -- a lone job as its own launch pair, several as one
+// ragged fused launch and one ragged partial-sum launch over a device job array -- on the hub's
+// stream, until the queue is empty.  Rounds that arrive while a launch is being issued ride on
+// the next one: an uncontended session pays nothing, contended sessions pay one launch pair per
+// group of rounds instead of one per round.  Every session then waits on its own ready word.
+// Enabled process-wide by set_shared_rounds (prover pools with several contexts turn it on).
+void set_shared_rounds(bool on);
+bool shared_rounds();
+struct FusedHubStats {
+  uint64_t launches = 0, jobs = 0;
+};
+FusedHubStats fused_hub_stats();
 
 struct StepMemH;
 struct ObCfgH;
@@ -284,6 +308,10 @@ class CcsOracle {
   K2 pending_r_ = K2{0, 0};
   void flush();
   RoundSignal next_signal();  // the mailbox's ready word and the value the next round stores
+  // shared rounds: the first fused round of the oracle on the hub stream waits for this stream's
+  // work (hub_pending_in_); before this stream reads what hub rounds wrote, it waits for them
+  bool hub_used_ = false;
+  void join_hub();
   // eq factors of the fused rounds: beta_r and r_in per row variable, the products of
   // eq(beta_i, r_i) and eq(r_in_i, r_i) over the folded variables, and the offsets of the chi
   // tables of the later variables (gt: every fused round's beta table, then its r_in table)

@@ -96,3 +96,27 @@ def test_process_prover_equals_plain(shards):
     assert pp.created == 4
     seq, _ = prove_sequential(shards)
     assert out == seq
+
+
+def test_shared_round_launches(ns, monkeypatch):
+    """Eight CCS-only shards proved by eight concurrent contexts with shared fused-round launches
+    give the plain path's bytes, and the shared launcher carried every fused round (rounds
+    3 .. ell_n - 1 of every step), in no more launches than rounds."""
+    from nightstream_amd import batch
+    from nightstream_amd.batch import ShardFactory, prove_batched, prove_sequential
+    monkeypatch.setattr(batch, "SHARE_ROUNDS", True)
+    fac = ShardFactory(SEED)
+    specs = [(1 << (10 + i % 3), 2 + i % 3, 40 + i) for i in range(8)]
+    shards = [fac.make(n, steps, seed) for n, steps, seed in specs]
+    print("plain path, 8 CCS-only shards", flush=True)
+    seq, _ = prove_sequential(shards)
+    l0, r0 = ns.shared_round_stats()
+    print("eight contexts, shared rounds", flush=True)
+    bat, _ = prove_batched(shards, 8)
+    l1, r1 = ns.shared_round_stats()
+    assert bat == seq
+    want = sum(steps * (n.bit_length() - 1 - 3) for n, steps, _ in specs)
+    assert r1 - r0 == want
+    assert 0 < l1 - l0 <= r1 - r0
+    print(f"shared rounds: {r1 - r0} rounds in {l1 - l0} launches", flush=True)
+    assert not ns.shared_rounds_enabled()
