@@ -13,6 +13,7 @@
 // so the NC part of a pair needs only signed selections of wg (no products).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <stdexcept>
 
@@ -1009,6 +1010,40 @@ __device__ __forceinline__ void fused_f_item(const FusedArgs& A, size_t t, K2 (&
   }
 }
 
+// f = c0 x_a x_b + c1 x_c (the R1CS shape) of pair t, straight-line: the twelve table entries are
+// loaded at once, then P = c0 (a + a' X)(b + b' X) + c1 (c + c' X) on the folded values
+__device__ __forceinline__ void fused_f_r1cs(const FusedArgs& A, size_t t, K2 (&in)[5]) {
+  const size_t L2 = 2 * A.len;
+  const K2* pa = A.sc_src + (size_t)(1 + A.r1cs[0]) * L2 + 4 * t;
+  const K2* pb = A.sc_src + (size_t)(1 + A.r1cs[1]) * L2 + 4 * t;
+  const K2* pc = A.sc_src + (size_t)(1 + A.r1cs[2]) * L2 + 4 * t;
+  const K2 a0 = pa[0], a1 = pa[1], a2 = pa[2], a3 = pa[3];
+  const K2 b0 = pb[0], b1 = pb[1], b2 = pb[2], b3 = pb[3];
+  const K2 e0 = pc[0], e1 = pc[1], e2 = pc[2], e3 = pc[3];
+  const K2 r = A.r;
+  const uint64_t r7 = A.r7;
+  const K2 al = fold2(a0, a1, r, r7), ad = ksub(fold2(a2, a3, r, r7), al);
+  const K2 bl = fold2(b0, b1, r, r7), bd = ksub(fold2(b2, b3, r, r7), bl);
+  const K2 el = fold2(e0, e1, r, r7), ed = ksub(fold2(e2, e3, r, r7), el);
+  const uint64_t al7 = fmul7(al.c1), ad7 = fmul7(ad.c1);
+  K2 p0 = kmul_w7(al, al7, bl), p1 = kadd(kmul_w7(al, al7, bd), kmul_w7(ad, ad7, bl)), p2 = kmul_w7(ad, ad7, bd);
+  const uint64_t c0 = A.r1cs_c[0], c1 = A.r1cs_c[1];
+  if (c0 != 1) {
+    p0 = kscale(p0, c0);
+    p1 = kscale(p1, c0);
+    p2 = kscale(p2, c0);
+  }
+  if (c1 == Q - 1) {
+    in[0] = ksub(p0, el);
+    in[1] = ksub(p1, ed);
+  } else {
+    in[0] = kadd(p0, kscale(el, c1));
+    in[1] = kadd(p1, kscale(ed, c1));
+  }
+  in[2] = p2;
+  in[3] = in[4] = kzero();
+}
+
 // NC rows r_lo .. r_hi - 1 of pair t: folded into the next tables, and the four coefficients of
 // sum_row w_row N(y_row(X)) written to in[0..3] (lazy accumulators, one reduction per term).
 // SCALE: the weights are taken times gt (7 gt.c1 = g7), for items of one or a few rows where
@@ -1069,13 +1104,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& A, const K2* __rest
   const size_t nnc = pairs * (size_t)G, base_s = (nnc + 63) & ~(size_t)63;
   int has_s = 0;
   const size_t total = base_s + pairs * (1 + ne + (size_t)nfold);
-  // degree of f (uniform): coefficients above it are zero
-  int fdeg = 0;
-  for (int tt = 0; tt < A.f.nterms; tt++) {
-    int dg = 0;
-    for (int v = 0; v < A.f.nv[tt]; v++) dg += A.f.exp[tt][v];
-    fdeg = max(fdeg, min(dg, 4));
-  }
+  const int fdeg = A.fdeg;  // degree of f (uniform): coefficients above it are zero
   const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
   for (size_t idx = (size_t)bid * blockDim.x + threadIdx.x; idx < total; idx += (size_t)nb * blockDim.x) {
     if (idx >= nnc && idx < base_s) continue;
@@ -1092,7 +1121,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& A, const K2* __rest
     if (g == G) {
       const K2 gt = A.gb[t];
       K2 in[5];
-      fused_f_item(A, t, in);
+      if (A.r1cs[0] >= 0)
+        fused_f_r1cs(A, t, in);
+      else
+        fused_f_item(A, t, in);
       const uint64_t g7 = fmul7(gt.c1);
 #pragma unroll
       for (int d = 0; d < 5; d++)
@@ -1162,6 +1194,32 @@ __global__ __launch_bounds__(RB) void k_round_fused_multi(const FusedJob* __rest
   fused_block(J.A, J.wg, J.partials, J.G, b, (unsigned)J.nblk, red);
 }
 
+// the degree of f (capped at 4, as the round drops higher terms) and its R1CS shape, if it has one
+static void classify_f(FusedArgs& A) {
+  const FPoly& f = A.f;
+  int deg = 0;
+  for (int tt = 0; tt < f.nterms; tt++) {
+    int d = 0;
+    for (int v = 0; v < f.nv[tt]; v++) d += f.exp[tt][v];
+    deg = std::max(deg, std::min(d, 4));
+  }
+  A.fdeg = deg;
+  A.r1cs[0] = A.r1cs[1] = A.r1cs[2] = -1;
+  if (f.nterms != 2) return;
+  for (int p = 0; p < 2; p++) {
+    const int l = 1 - p;  // p: the product term, l: the linear term
+    if (f.nv[p] != 2 || f.exp[p][0] != 1 || f.exp[p][1] != 1 || f.nv[l] != 1 || f.exp[l][0] != 1) continue;
+    const int v[3] = {f.var[p][0], f.var[p][1], f.var[l][0]};
+    bool ok = true;
+    for (int i = 0; i < 3; i++) ok = ok && v[i] >= 0 && v[i] < A.nf;
+    if (!ok) return;
+    for (int i = 0; i < 3; i++) A.r1cs[i] = v[i];
+    A.r1cs_c[0] = f.coeff[p];
+    A.r1cs_c[1] = f.coeff[l];
+    return;
+  }
+}
+
 FusedJob fused_job(const FusedArgs& A, const K2* wg, K2* out, void* ws, const RoundSignal& sig) {
   const size_t pairs = A.len / 2, nrow = (size_t)A.k * D;
   if (pairs == 0 || (pairs & (pairs - 1))) throw std::invalid_argument("fused round: table length must be a power of two >= 2");
@@ -1179,6 +1237,7 @@ FusedJob fused_job(const FusedArgs& A, const K2* wg, K2* out, void* ws, const Ro
   if (nblk < 1) nblk = 1;
   FusedJob J;
   J.A = A;
+  classify_f(J.A);
   J.wg = wg;
   J.partials = (K2*)ws;
   J.out = out;

@@ -165,6 +165,11 @@ struct FusedArgs {
   uint64_t r7;            // 7 r.c1
   const K2* gb = nullptr;  // [len / 2]: chi of beta over the variables after this round's
   const K2* gr = nullptr;  // [len / 2]: the same for r_in (has_eval)
+  // set from f by fused_job: its degree (at most 4) and, for f = c0 x_a x_b + c1 x_c (the R1CS
+  // shape), the variables a, b, c (r1cs[0] < 0 otherwise) and c0, c1
+  int fdeg = 4;
+  int r1cs[3] = {-1, -1, -1};
+  uint64_t r1cs_c[2] = {0, 0};
 };
 constexpr int FUSED_MAX_SC = 8;
 void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s,

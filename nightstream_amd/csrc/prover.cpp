@@ -614,7 +614,8 @@ std::vector<K2> AjtaiPhase::coeffs() const {
 CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* alpha, const K2* beta_a,
                      const K2* beta_r, K2 gamma, size_t ell_d, size_t ell_n, const K2* r_inputs, hipStream_t st,
                      OracleWs* ws)
-    : s_(s), st_(st), k_(k), ell_d_(ell_d), ell_n_(ell_n) {
+    : s_(s), st_(st), k_(k), ell_n_(ell_n) {
+  (void)ell_d;  // the Ajtai-bit rounds run on the host
   const_cast<Ccs&>(s).ensure_device();
   const size_t n_pad = (size_t)1 << ell_n, m = s.m, n = s.n;
   if (n_pad != n) throw Error(1, "row domain must equal n (power of two)");

@@ -293,7 +293,7 @@ class CcsOracle {
  private:
   const Ccs& s_;
   hipStream_t st_;
-  size_t k_, ell_d_, ell_n_, len_, round_ = 0;
+  size_t k_, ell_n_, len_, round_ = 0;
   int nf_, nsc_;
   bool has_eval_;
   K2 gamma_k_;
