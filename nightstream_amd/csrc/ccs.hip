@@ -491,6 +491,9 @@ struct RoundArgs {
   K2 gamma_k;
   FPoly f;
   const K2* pc;  // round 1: [20][4] coefficient polynomials of the digit monomials
+  const K2* U;   // modes 3, 4: [monomial][pair] weighted digit-monomial sums (k_digit_monomials)
+  int r1cs[3];   // f = c0 x_a x_b + c1 x_c: a, b, c (r1cs[0] < 0 for other shapes), c0, c1
+  uint64_t r1cs_c[2];
 };
 
 // Wave sum of K values, valid in lane 63 (DPP_LANE): each component is summed lazily as a 64-bit
@@ -523,6 +526,159 @@ __device__ __forceinline__ uint64_t wave_sum_fq(uint64_t v) {
 }
 __device__ __forceinline__ K2 wave_sum(K2 v) { return K2{wave_sum_fq(v.c0), wave_sum_fq(v.c1)}; }
 
+// c0 (a + a' X)(b + b' X) + c1 (e + e' X) into in[0..4] (the f polynomial of the R1CS shape on one
+// pair's values)
+__device__ __forceinline__ void r1cs_poly(K2 al, K2 ad, K2 bl, K2 bd, K2 el, K2 ed, uint64_t c0, uint64_t c1,
+                                          K2 (&in)[5]) {
+  const uint64_t al7 = fmul7(al.c1), ad7 = fmul7(ad.c1);
+  K2 p0 = kmul_w7(al, al7, bl), p1 = kadd(kmul_w7(al, al7, bd), kmul_w7(ad, ad7, bl)), p2 = kmul_w7(ad, ad7, bd);
+  if (c0 != 1) {
+    p0 = kscale(p0, c0);
+    p1 = kscale(p1, c0);
+    p2 = kscale(p2, c0);
+  }
+  if (c1 == Q - 1) {
+    in[0] = ksub(p0, el);
+    in[1] = ksub(p1, ed);
+  } else {
+    in[0] = kadd(p0, kscale(el, c1));
+    in[1] = kadd(p1, kscale(ed, c1));
+  }
+  in[2] = p2;
+  in[3] = in[4] = kzero();
+}
+
+// ---- rounds 0 and 1 on int8 MFMA ----
+// Both rounds need, for every digit monomial j and pair t, the weighted sum over the k*54 digit
+// rows U_j[t] = sum_row wg[row] mono_j(row, t), mono_j in {-1, 0, 1}: round 0 has the four
+// monomials lo, hi, |lo| hi, lo |hi| of a digit pair, round 1 the twenty of a digit quad (see
+// k_round_eval).  That is a GEMM of the weights by the monomials: A = the weights' balanced
+// base-256 limbs (16 limb rows, 8 per F_q component, in [-128, 127]; instance w's 54 rows at
+// k = 64 w + rho, the 10 rows after them zero), B = the monomials (k = row, n = pair), on
+// v_mfma_i32_16x16x64_i8 with int32 accumulation (|sum| < 2^18 per limb).  A wave owns 16 pairs
+// (round 1) or 32 (round 0, two n tiles) and all rows; lane (n = lane % 16, k group = lane / 16)
+// loads one digit word per row for rows 16 (lane / 16) .. + 15 of each instance -- 64 contiguous
+// bytes per row across the 16 lanes -- transposes four rows' words into digit-major bytes with
+// byte permutes and forms the monomials on packed bytes: for d in {0x00, 0x01, 0xFF},
+// |d_i| d_j = d_j & mask(d_i) and d_i d_j d_l = (d_i ^ d_j ^ d_l) & masks (the xor of three
+// nonzero signs is their product).  The epilogue turns each pair's 16 limb sums into the K value
+// U_j[t] (limbs 0..3 + 2^32 limbs 4..7 per component, modulo q) and stores it.
+__device__ __forceinline__ uint32_t mono_mask(uint32_t x) {
+  const uint32_t e = x & 0x01010101u;  // |d| per byte
+  return (e << 8) - e;                 // 0xFF where d != 0
+}
+__device__ __forceinline__ uint64_t from_i64_dev(int64_t v) { return v >= 0 ? (uint64_t)v : Q - (uint64_t)(-v); }
+
+template <int MODE>  // 0: round 0 (digit pairs, 32 pairs per wave), 1: round 1 (digit quads, 16 pairs per wave)
+__global__ __launch_bounds__(256) void k_digit_monomials(const int8_t* const* __restrict__ digits, int k, size_t m,
+                                                        const int8_t* __restrict__ wl, size_t pairs,
+                                                        K2* __restrict__ U) {
+  constexpr int NT = MODE == 0 ? 2 : 1;   // n tiles per wave
+  constexpr int J = MODE == 0 ? 4 : 20;   // monomials
+  constexpr int NACC = J * NT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  const size_t tb = ((size_t)blockIdx.x * 4 + wave) * (16 * NT);
+  if (tb >= pairs) return;  // whole waves only: pairs is a multiple of 16 NT
+  // digit byte offset of this lane's word: 4 digits of pair tb + r16 (round 1), or 2 digits of
+  // each of the pairs tb + 2 r16, tb + 2 r16 + 1 (round 0)
+  const size_t off = MODE == 0 ? 2 * (tb + 2 * (size_t)r16) : 4 * (tb + (size_t)r16);
+  const int lim = min(16, D - 16 * kg);  // rows of this lane's group that exist (kg = 3: 6)
+  v4i acc[NACC];
+#pragma unroll
+  for (int a = 0; a < NACC; a++) acc[a] = v4i{0, 0, 0, 0};
+  const size_t ldk = (size_t)64 * k;  // limb row length
+  uint32_t cur[16], nxt[16];
+  auto load = [&](int w, uint32_t (&x)[16]) {
+    const int8_t* base = digits[w] + (size_t)(16 * kg) * m + off;
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = *(const uint32_t*)(base + (size_t)(i < lim ? i : lim - 1) * m);
+  };
+  load(0, cur);
+  for (int w = 0; w < k; w++) {
+    if (w + 1 < k) load(w + 1, nxt);
+    const v4i a = *(const v4i*)(wl + (size_t)r16 * ldk + (size_t)w * 64 + 16 * kg);
+    // digit-major bytes of rows 4q .. 4q+3: T[q][i] byte b = digit i of row 4q + b
+    uint32_t T[4][4], M[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t x0 = cur[4 * q], x1 = cur[4 * q + 1], x2 = cur[4 * q + 2], x3 = cur[4 * q + 3];
+      const uint32_t l01 = __builtin_amdgcn_perm(x1, x0, 0x05010400u), h01 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
+      const uint32_t l23 = __builtin_amdgcn_perm(x3, x2, 0x05010400u), h23 = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+      T[q][0] = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+      T[q][1] = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+      T[q][2] = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+      T[q][3] = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+#pragma unroll
+      for (int i = 0; i < 4; i++) M[q][i] = mono_mask(T[q][i]);
+    }
+#define NS_MONO(slot, expr)                                                   \
+  {                                                                           \
+    v4i b;                                                                    \
+    _Pragma("unroll") for (int q = 0; q < 4; q++) {                           \
+      const uint32_t* t_ = T[q];                                              \
+      const uint32_t* m_ = M[q];                                              \
+      (void)t_;                                                               \
+      (void)m_;                                                               \
+      b[q] = (int)(expr);                                                     \
+    }                                                                         \
+    acc[slot] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[slot], 0, 0, 0); \
+  }
+    if (MODE == 0) {
+      // n tile 0: pairs tb + 2 r16 (digits 0, 1 of the word), tile 1: tb + 2 r16 + 1 (digits 2, 3)
+      NS_MONO(0, t_[0])
+      NS_MONO(1, t_[1])
+      NS_MONO(2, t_[1] & m_[0])
+      NS_MONO(3, t_[0] & m_[1])
+      NS_MONO(4, t_[2])
+      NS_MONO(5, t_[3])
+      NS_MONO(6, t_[3] & m_[2])
+      NS_MONO(7, t_[2] & m_[3])
+    } else {
+      NS_MONO(0, t_[0])
+      NS_MONO(1, t_[1])
+      NS_MONO(2, t_[2])
+      NS_MONO(3, t_[3])
+      NS_MONO(4, t_[1] & m_[0])
+      NS_MONO(5, t_[2] & m_[0])
+      NS_MONO(6, t_[3] & m_[0])
+      NS_MONO(7, t_[0] & m_[1])
+      NS_MONO(8, t_[2] & m_[1])
+      NS_MONO(9, t_[3] & m_[1])
+      NS_MONO(10, t_[0] & m_[2])
+      NS_MONO(11, t_[1] & m_[2])
+      NS_MONO(12, t_[3] & m_[2])
+      NS_MONO(13, t_[0] & m_[3])
+      NS_MONO(14, t_[1] & m_[3])
+      NS_MONO(15, t_[2] & m_[3])
+      NS_MONO(16, (t_[0] ^ t_[1] ^ t_[2]) & m_[0] & m_[1] & m_[2])
+      NS_MONO(17, (t_[0] ^ t_[1] ^ t_[3]) & m_[0] & m_[1] & m_[3])
+      NS_MONO(18, (t_[0] ^ t_[2] ^ t_[3]) & m_[0] & m_[2] & m_[3])
+      NS_MONO(19, (t_[1] ^ t_[2] ^ t_[3]) & m_[1] & m_[2] & m_[3])
+    }
+#undef NS_MONO
+    if (w + 1 < k) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) cur[i] = nxt[i];
+    }
+  }
+  // lane holds limbs 4 kg .. 4 kg + 3 of its pair column: component kg / 2, bytes 4 (kg & 1) ..
+#pragma unroll
+  for (int a = 0; a < NACC; a++) {
+    const int64_t p = (int64_t)acc[a][0] + ((int64_t)acc[a][1] << 8) + ((int64_t)acc[a][2] << 16) +
+                      ((int64_t)acc[a][3] << 24);
+    const int plo = __shfl_xor((int)(uint32_t)p, 16, 64), phi = __shfl_xor((int)(uint32_t)((uint64_t)p >> 32), 16, 64);
+    const int64_t hi = (int64_t)(((uint64_t)(uint32_t)phi << 32) | (uint32_t)plo);
+    if ((kg & 1) == 0) {
+      const uint64_t v = fadd(from_i64_dev(p), fmul(from_i64_dev(hi), (uint64_t)1 << 32));
+      const int j = a % J, tile = a / J;
+      const size_t t = MODE == 0 ? tb + 2 * (size_t)r16 + tile : tb + (size_t)r16;
+      uint64_t* dst = (uint64_t*)(U + (size_t)j * pairs + t);
+      dst[kg >> 1] = v;
+    }
+  }
+}
+
 // coefficient vector of width 5 (deg_max = 4).  Work item = (row pair t, row group g): the NC
 // sum over the k*54 digit rows of a pair is split into G groups so that short tables still fill
 // the GPU; every group multiplies its partial NC polynomial by the pair's eq_beta(X) (the
@@ -549,8 +705,8 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
       nsh[i] = kneg(w);
   }
   __shared__ const int8_t* dptr[32];  // digit planes of the k instances (rounds 0 and 1)
-  if (MODE != 2 && threadIdx.x < A.k && threadIdx.x < 32) dptr[threadIdx.x] = A.digits[threadIdx.x];
-  if (MODE == 1)
+  if ((MODE == 0 || MODE == 1) && threadIdx.x < A.k && threadIdx.x < 32) dptr[threadIdx.x] = A.digits[threadIdx.x];
+  if (MODE == 1 || MODE == 4)
     for (int i = threadIdx.x; i < 80; i += blockDim.x) {
       const K2 p = A.pc[i];
       pcs[i] = p;
@@ -564,21 +720,41 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
 #pragma unroll
   for (int d = 0; d < 5; d++) c[d] = kzero();
   K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
-  // round 1 splits each (pair, row group) item into two halves of the 20 digit monomials
-  const int H = MODE == 1 ? 2 : 1;
-  const size_t total = pairs * ((size_t)G * H + 1);
+  // round 1 splits each (pair, row group) item into two halves of the 20 digit monomials (four
+  // quarters when they come from U)
+  const int H = MODE == 1 ? 2 : (MODE == 4 ? 4 : 1);
+  // items per pair: G * H row items, then the f item and (with r_in) the Eval item
+  const int nfe = A.eqr ? 2 : 1;
+  const size_t total = pairs * ((size_t)G * H + nfe);
   const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
     const size_t t = idx & (pairs - 1);
     const int gh = (int)(idx >> lp);
-    const int g = gh / H, h = gh % H;
+    const bool fe = gh >= G * H;  // f item (gh = G H) or Eval item (gh = G H + 1)
+    if (fe && gh > G * H) {
+      // gamma^k eq_rin(X) Eval(X), summed apart and scaled once
+      const K2 r0 = A.eqr[2 * t], r1 = ksub(A.eqr[2 * t + 1], r0);
+      const K2 v0 = A.ev[2 * t], v1 = ksub(A.ev[2 * t + 1], v0);
+      ev0 = kadd(ev0, kmul(r0, v0));
+      ev1 = kadd(ev1, kadd(kmul(r0, v1), kmul(r1, v0)));
+      ev2 = kadd(ev2, kmul(r1, v1));
+      continue;
+    }
+    const int g = fe ? G : gh / H, h = fe ? 0 : gh % H;
     const int r_lo = min(nrow, g * R), r_hi = min(nrow, r_lo + R);
     const K2 e0 = A.eqb[2 * t];
     const K2 e1 = ksub(A.eqb[2 * t + 1], e0);
     K2 in[5];
 #pragma unroll
     for (int d = 0; d < 5; d++) in[d] = kzero();
-    if (g == G) {
+    if (fe && A.r1cs[0] >= 0) {
+      // f(M z1)(X) of the R1CS shape, straight-line
+      const K2* ta = A.fv + (size_t)A.r1cs[0] * A.len + 2 * t;
+      const K2* tb = A.fv + (size_t)A.r1cs[1] * A.len + 2 * t;
+      const K2* tc = A.fv + (size_t)A.r1cs[2] * A.len + 2 * t;
+      const K2 al = ta[0], ah = ta[1], bl = tb[0], bh = tb[1], el = tc[0], eh = tc[1];
+      r1cs_poly(al, ksub(ah, al), bl, ksub(bh, bl), el, ksub(eh, el), A.r1cs_c[0], A.r1cs_c[1], in);
+    } else if (fe) {
       // f(M z1)(X)
       for (int tt = 0; tt < A.f.nterms; tt++) {
         K2 p[5];
@@ -597,16 +773,6 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
 #pragma unroll
         for (int d = 0; d < 5; d++) in[d] = kadd(in[d], p[d]);
       }
-      if (A.eqr) {
-        K2 r0 = A.eqr[2 * t], r1 = ksub(A.eqr[2 * t + 1], r0);
-        K2 v0 = A.ev[2 * t], v1 = ksub(A.ev[2 * t + 1], v0);
-        ev0 = kadd(ev0, kmul(r0, v0));
-        ev1 = kadd(ev1, kadd(kmul(r0, v1), kmul(r1, v0)));
-        ev2 = kadd(ev2, kmul(r1, v1));
-      }
-    }
-    if (g == G) {
-      // no rows
     } else if (MODE == 0) {
       // signed selections of wg, summed lazily (+w or q - w) and reduced once per work item
       KS1 zlo = ks1_zero(), zhi = ks1_zero(), z3 = ks1_zero(), z4 = ks1_zero();
@@ -752,6 +918,24 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
       }
 #undef NS_UPD
 #undef NS_APPLY
+    } else if (MODE == 3) {
+      // round 0 from U: the four weighted monomial sums of the pair, then the cubics as in mode 0
+      const K2 slo = A.U[t], shi = A.U[pairs + t], s3 = A.U[2 * pairs + t], s4 = A.U[3 * pairs + t];
+      K2 s3x3 = kadd(kadd(s3, s3), s3);
+      K2 s4x3 = kadd(kadd(s4, s4), s4);
+      K2 slo2 = kadd(slo, slo);
+      in[1] = kadd(in[1], ksub(ksub(s3x3, slo2), shi));
+      in[2] = kadd(in[2], ksub(kadd(kadd(slo2, slo), s4x3), kadd(s3x3, s3x3)));
+      in[3] = kadd(in[3], ksub(kadd(shi, s3x3), kadd(slo, s4x3)));
+    } else if (MODE == 4) {
+      // round 1 from U: monomials 5 h .. 5 h + 4 of the pair times their cubics
+#pragma unroll
+      for (int jj = 0; jj < 5; jj++) {
+        const int j = 5 * h + jj;
+        const K2 av = A.U[(size_t)j * pairs + t];
+#pragma unroll
+        for (int d = 0; d < 4; d++) in[d] = kadd(in[d], kmul_w7(pcs[j * 4 + d], pcs7[j * 4 + d], av));
+      }
     } else {
       // w N(a + b X) = w (a^3 - a) + w (3 a^2 b - b) X + 3 w a b^2 X^2 + w b^3 X^3: wa, wb, a^2, b^2
       // reduced per row, the four products and two sums accumulated lazily over the rows
@@ -791,7 +975,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
 #pragma unroll
     for (int d = 1; d < 4; d++) c[d] = kadd(c[d], kadd(kmul_w7(e0, e07, in[d]), kmul_w7(e1, e17, in[d - 1])));
     c[4] = kadd(c[4], kmul_w7(e1, e17, in[3]));
-    if (g == G) c[4] = kadd(c[4], kmul_w7(e0, e07, in[4]));
+    if (fe) c[4] = kadd(c[4], kmul_w7(e0, e07, in[4]));
   }
   if (A.eqr) {
     c[0] = kadd(c[0], kmul(A.gamma_k, ev0));
@@ -878,6 +1062,54 @@ static void launch_sum_partials(const K2* partials, size_t nblk, int ncoef, K2* 
 
 size_t round_ws_bytes() { return (size_t)RMAXBLK * 7 * sizeof(K2); }
 
+std::vector<int8_t> weight_limbs(const K2* wg, size_t k) {
+  const size_t ld = 64 * k;
+  std::vector<int8_t> out(16 * ld, 0);
+  for (size_t w = 0; w < k; w++)
+    for (int rho = 0; rho < D; rho++) {
+      const K2 x = wg[w * D + rho];
+      const uint64_t comp[2] = {x.c0, x.c1};
+      for (int c = 0; c < 2; c++) {
+        __int128 v = comp[c] <= 0x7F7F7F7F7F7F7F7Full ? (__int128)comp[c] : (__int128)comp[c] - (__int128)Q;
+        for (int l = 0; l < 8; l++) {
+          int d = (int)(v & 255);
+          if (d >= 128) d -= 256;
+          out[(size_t)(c * 8 + l) * ld + w * 64 + rho] = (int8_t)d;
+          v = (v - d) >> 8;
+        }
+        if (v != 0) throw std::logic_error("weight_limbs: value outside the balanced range");
+      }
+    }
+  return out;
+}
+
+// The degree of f (capped at 4, as the rounds drop higher terms) and its R1CS shape, if it has one:
+// f = c0 x_a x_b + c1 x_c with a, b, c < nf (r1cs[0] = -1 otherwise).
+static void classify_fpoly(const FPoly& f, int nf, int& fdeg, int (&r1cs)[3], uint64_t (&c)[2]) {
+  int deg = 0;
+  for (int tt = 0; tt < f.nterms; tt++) {
+    int d = 0;
+    for (int v = 0; v < f.nv[tt]; v++) d += f.exp[tt][v];
+    deg = std::max(deg, std::min(d, 4));
+  }
+  fdeg = deg;
+  r1cs[0] = r1cs[1] = r1cs[2] = -1;
+  c[0] = c[1] = 0;
+  if (f.nterms != 2) return;
+  for (int p = 0; p < 2; p++) {
+    const int l = 1 - p;  // p: the product term, l: the linear term
+    if (f.nv[p] != 2 || f.exp[p][0] != 1 || f.exp[p][1] != 1 || f.nv[l] != 1 || f.exp[l][0] != 1) continue;
+    const int v[3] = {f.var[p][0], f.var[p][1], f.var[l][0]};
+    bool ok = true;
+    for (int i = 0; i < 3; i++) ok = ok && v[i] >= 0 && v[i] < nf;
+    if (!ok) return;
+    for (int i = 0; i < 3; i++) r1cs[i] = v[i];
+    c[0] = f.coeff[p];
+    c[1] = f.coeff[l];
+    return;
+  }
+}
+
 void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool round0, K2* out, void* ws,
                        hipStream_t s, const RoundSignal* sig) {
   if (T.len < 2 || (T.len & (T.len - 1))) throw std::invalid_argument("round: table length must be a power of two >= 2");
@@ -894,7 +1126,35 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   A.k = T.k;
   A.gamma_k = T.gamma_k;
   A.f = f;
+  int fdeg = 0;
+  classify_fpoly(f, T.nf, fdeg, A.r1cs, A.r1cs_c);
   size_t pairs = T.len / 2;
+  K2* partials = (K2*)ws;
+  A.pc = T.pc;
+  A.U = nullptr;
+  // rounds 0 and 1 on int8 MFMA (k_digit_monomials) when the tables allow whole 64-pair blocks
+  // and U fits the buffer given for it; then the round kernel reads U in place of the digits
+  const int dmode = round0 ? 0 : (T.digit_round1 ? 1 : -1);
+  const size_t nmono = dmode == 0 ? 4 : 20;
+  if (dmode >= 0 && T.wl && T.ubuf && pairs >= 128 && pairs % 128 == 0 && T.k <= 32 &&
+      T.ubuf_bytes >= nmono * pairs * sizeof(K2)) {
+    const size_t per_block = dmode == 0 ? 128 : 64;
+    const dim3 g1((unsigned)(pairs / per_block));
+    A.U = T.ubuf;
+    const size_t H = dmode == 0 ? 1 : 4;
+    size_t nblk = (pairs * (H + (T.eqr ? 2 : 1)) + RB - 1) / RB;
+    if (nblk > RMAXBLK) nblk = RMAXBLK;
+    const size_t shmem = ((size_t)T.k * D * 2 + 80 + 4 * 5 + 40) * sizeof(K2);
+    if (dmode == 0) {
+      NS_LAUNCH_TIMED(KT_ROUND0, k_digit_monomials<0>, g1, dim3(256), 0, s, T.digits, T.k, T.m, T.wl, pairs, T.ubuf);
+      NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<3>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, 1);
+    } else {
+      NS_LAUNCH_TIMED(KT_ROUND1, k_digit_monomials<1>, g1, dim3(256), 0, s, T.digits, T.k, T.m, T.wl, pairs, T.ubuf);
+      NS_LAUNCH_TIMED(KT_ROUND1, k_round_eval<4>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, 1);
+    }
+    launch_sum_partials(partials, nblk, 5, out, sig, s);
+    return;
+  }
   // Split the k*54 NC rows of a pair across G work items: long tables get long items (each item
   // ends with a few reductions and, in round 1, 40 coefficient products), short tables get as
   // many items as it takes to fill the GPU, down to one row per item.
@@ -905,12 +1165,10 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   if (G > nrow) G = nrow;
   G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
   const size_t H = T.digit_round1 ? 2 : 1;
-  size_t nblk = (pairs * (G * H + 1) + RB - 1) / RB;
+  size_t nblk = (pairs * (G * H + (T.eqr ? 2 : 1)) + RB - 1) / RB;
   if (nblk > RMAXBLK) nblk = RMAXBLK;
   if (nblk < 1) nblk = 1;
   size_t shmem = ((size_t)T.k * D * 2 + 80 + 4 * 5 + 40) * sizeof(K2);
-  K2* partials = (K2*)ws;
-  A.pc = T.pc;
   if (round0)
     NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<0>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, (int)G);
   else if (T.digit_round1)
@@ -1025,23 +1283,7 @@ __device__ __forceinline__ void fused_f_r1cs(const FusedArgs& A, size_t t, K2 (&
   const K2 al = fold2(a0, a1, r, r7), ad = ksub(fold2(a2, a3, r, r7), al);
   const K2 bl = fold2(b0, b1, r, r7), bd = ksub(fold2(b2, b3, r, r7), bl);
   const K2 el = fold2(e0, e1, r, r7), ed = ksub(fold2(e2, e3, r, r7), el);
-  const uint64_t al7 = fmul7(al.c1), ad7 = fmul7(ad.c1);
-  K2 p0 = kmul_w7(al, al7, bl), p1 = kadd(kmul_w7(al, al7, bd), kmul_w7(ad, ad7, bl)), p2 = kmul_w7(ad, ad7, bd);
-  const uint64_t c0 = A.r1cs_c[0], c1 = A.r1cs_c[1];
-  if (c0 != 1) {
-    p0 = kscale(p0, c0);
-    p1 = kscale(p1, c0);
-    p2 = kscale(p2, c0);
-  }
-  if (c1 == Q - 1) {
-    in[0] = ksub(p0, el);
-    in[1] = ksub(p1, ed);
-  } else {
-    in[0] = kadd(p0, kscale(el, c1));
-    in[1] = kadd(p1, kscale(ed, c1));
-  }
-  in[2] = p2;
-  in[3] = in[4] = kzero();
+  r1cs_poly(al, ad, bl, bd, el, ed, A.r1cs_c[0], A.r1cs_c[1], in);
 }
 
 // NC rows r_lo .. r_hi - 1 of pair t: folded into the next tables, and the four coefficients of
@@ -1194,31 +1436,7 @@ __global__ __launch_bounds__(RB) void k_round_fused_multi(const FusedJob* __rest
   fused_block(J.A, J.wg, J.partials, J.G, b, (unsigned)J.nblk, red);
 }
 
-// the degree of f (capped at 4, as the round drops higher terms) and its R1CS shape, if it has one
-static void classify_f(FusedArgs& A) {
-  const FPoly& f = A.f;
-  int deg = 0;
-  for (int tt = 0; tt < f.nterms; tt++) {
-    int d = 0;
-    for (int v = 0; v < f.nv[tt]; v++) d += f.exp[tt][v];
-    deg = std::max(deg, std::min(d, 4));
-  }
-  A.fdeg = deg;
-  A.r1cs[0] = A.r1cs[1] = A.r1cs[2] = -1;
-  if (f.nterms != 2) return;
-  for (int p = 0; p < 2; p++) {
-    const int l = 1 - p;  // p: the product term, l: the linear term
-    if (f.nv[p] != 2 || f.exp[p][0] != 1 || f.exp[p][1] != 1 || f.nv[l] != 1 || f.exp[l][0] != 1) continue;
-    const int v[3] = {f.var[p][0], f.var[p][1], f.var[l][0]};
-    bool ok = true;
-    for (int i = 0; i < 3; i++) ok = ok && v[i] >= 0 && v[i] < A.nf;
-    if (!ok) return;
-    for (int i = 0; i < 3; i++) A.r1cs[i] = v[i];
-    A.r1cs_c[0] = f.coeff[p];
-    A.r1cs_c[1] = f.coeff[l];
-    return;
-  }
-}
+static void classify_f(FusedArgs& A) { classify_fpoly(A.f, A.nf, A.fdeg, A.r1cs, A.r1cs_c); }
 
 FusedJob fused_job(const FusedArgs& A, const K2* wg, K2* out, void* ws, const RoundSignal& sig) {
   const size_t pairs = A.len / 2, nrow = (size_t)A.k * D;

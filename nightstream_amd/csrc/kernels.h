@@ -8,6 +8,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "field.h"
 
 namespace ns {
@@ -134,7 +136,19 @@ struct RowTables {
   K2 gamma_k;
   bool digit_round1 = false;  // round 1 evaluated from the digits (tables folded once, by r0)
   const K2* pc = nullptr;     // [20][4] monomial coefficient polynomials for digit_round1
+  // rounds 0 and 1 on int8 MFMA: the NC weights as balanced base-256 limbs, [16][64 k] int8
+  // (weight_limbs), and device scratch for the weighted digit-monomial sums (4 x len / 2 K
+  // values in round 0, 20 x len / 2 in round 1); without them the rounds run on the vector ALUs
+  const int8_t* wl = nullptr;
+  K2* ubuf = nullptr;
+  size_t ubuf_bytes = 0;
 };
+// Balanced base-256 limbs of the NC weights wg [k*54] for the digit-monomial MFMA: limb row
+// 8 c + l (component c, byte l) of [16][64 k]; weight w*54 + rho sits in column 64 w + rho (the
+// 10 columns after each instance's 54 are zero).  Each component is taken as the representative
+// v of its class mod q with |v| <= 128 (2^64 - 1) / 255, so that v = sum_l 256^l s_l exactly
+// with every s_l in [-128, 127].
+std::vector<int8_t> weight_limbs(const K2* wg, size_t k);
 size_t round_ws_bytes();
 // Optional ready word for a round's coefficients in coherent pinned host memory: the partial-sum
 // kernel stores the coefficients, then seq into *flag (system-scope release).

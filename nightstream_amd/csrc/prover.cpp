@@ -683,6 +683,9 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
   }
   w_->wg.ensure(wg.size() * sizeof(K2));
   h2d(w_->wg.p, wg.data(), wg.size(), st);
+  const std::vector<int8_t> wl = weight_limbs(wg.data(), k);
+  w_->wl.ensure(wl.size());
+  h2d(w_->wl.p, wl.data(), wl.size(), st);
   // chi tables of the variables after each fused round j >= 3 (2^(ell_n - j - 1) entries)
   beta_r_.assign(beta_r, beta_r + ell_n);
   if (has_eval_) rin_.assign(r_inputs, r_inputs + ell_n);
@@ -909,6 +912,11 @@ std::vector<K2> CcsOracle::round_coeffs() {
   T.m = s_.m;
   T.k = (int)k_;
   T.gamma_k = gamma_k_;
+  if (round_ <= 1) {  // the NC tables do not exist yet: their buffer holds the monomial sums
+    T.wl = w_->wl.as<const int8_t>();
+    T.ubuf = w_->nc[0].as<K2>();
+    T.ubuf_bytes = w_->nc[0].bytes;
+  }
   if (round_ == 1) {
     // cubic coefficients of the 20 digit monomials for N((1-X) a + X a') with
     // a = (1-r0) d0 + r0 d1, a' = (1-r0) d2 + r0 d3

@@ -136,7 +136,7 @@ struct PinnedStage {
 };
 
 struct OracleWs {
-  DevBuf Zptrs, sc[2], nc[2], wg, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc, gt;
+  DevBuf Zptrs, sc[2], nc[2], wg, wl, coeffs, ws, z1, bits, Wd, Sa, chi, ydev, cs_ws, pc, gt;
   HostMailbox mbox;   // 8 K2: the round polynomial's coefficients; the ready word at byte 192
   uint64_t round_seq = 0;  // last ready-word value handed to a round launch
   PinnedStage stage;  // the context's device-to-host staging
