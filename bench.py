@@ -51,10 +51,10 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
     (eq_beta, 3 f-var tables, plus eq_rin and Eval once k > 1), 16 B per K entry.  Rounds 0 and 1
     read the int8 digit matrices (1 B per digit, all n digits of a row each) instead of NC K
     entries; the fold after round 1 writes the NC tables of length n/4 from the digits.  Round 2
-    reads its tables; each round r >= 3 is one fused launch that reads the round r-1 NC, f-var
-    and Eval tables (2L entries) and writes the folded ones (L entries), and reads the chi tables
-    of the later variables (L/2 entries, one for eq_beta and one for eq_rin) in place of the eq
-    tables.
+    reads the digits as well (from n = 512; below, its K tables); each round r >= 3 is one fused
+    launch that reads the round r-1 NC, f-var and Eval tables (2L entries) and writes the folded
+    ones (L entries), and reads the chi tables of the later variables (L/2 entries, one for
+    eq_beta and one for eq_rin) in place of the eq tables.
     """
     ell_n = n.bit_length() - 1
     out = {c: 0.0 for c in KT}
@@ -63,7 +63,11 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
         nsc = 4 if s == 0 else 6
         out["round0_eval"] += k * D * n + nsc * 16 * n
         out["round1_eval"] += k * D * n + nsc * 16 * (n // 2)
-        out["round_eval"] += (k * D + nsc) * 16 * (n >> 2)
+        # round 2 from the digits (int8 MFMA, 120 monomials of 8 digits) once n / 8 fills 64-pair blocks
+        if n >= 512:
+            out["round_eval"] += k * D * n + nsc * 16 * (n >> 2)
+        else:
+            out["round_eval"] += (k * D + nsc) * 16 * (n >> 2)
         nfold, ngt = (3, 1) if s == 0 else (4, 2)
         for r in range(3, ell_n):
             L = n >> r

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <utility>
 #include <stdexcept>
 
 #include "kernels.h"
@@ -494,6 +495,7 @@ struct RoundArgs {
   const K2* U;   // modes 3, 4: [monomial][pair] weighted digit-monomial sums (k_digit_monomials)
   int r1cs[3];   // f = c0 x_a x_b + c1 x_c: a, b, c (r1cs[0] < 0 for other shapes), c0, c1
   uint64_t r1cs_c[2];
+  int nb_u, nslice;  // mode 6: E-sum blocks (monomials x pair slices) ahead of the item blocks
 };
 
 // Wave sum of K values, valid in lane 63 (DPP_LANE): each component is summed lazily as a 64-bit
@@ -679,12 +681,147 @@ __global__ __launch_bounds__(256) void k_digit_monomials(const int8_t* const* __
   }
 }
 
+// Round 2 from the digits: each pair's two table entries fold digits 8t .. 8t+3 and 8t+4 .. 8t+7,
+// so N(y) expands into 120 monomials of 8 digits: d_i (8), |d_i| d_j (i != j, 56, i-major) and
+// d_i d_j d_l (i < j < l, 56, lexicographic).  A block owns 16 pairs; its six waves take 20
+// monomials each.  Per instance the block loads the 64 x 16 8-byte digit words once (thread u:
+// pair u % 16, rows 4 (u / 16) .. + 3), transposes them to digit-major bytes with their masks
+// into LDS (double-buffered, 20-dword rows so that the 16-byte reads of a 16-lane group hit
+// distinct banks), and every wave builds its monomials' B fragments from there.
+struct Mono8 {
+  int n, a, b, c;
+};
+__host__ __device__ constexpr Mono8 mono8_of(int j) {
+  if (j < 8) return Mono8{1, j, 0, 0};
+  if (j < 64) {
+    const int p = j - 8, i = p / 7, r = p % 7;
+    return Mono8{2, i, r < i ? r : r + 1, 0};
+  }
+  int idx = j - 64;
+  for (int a = 0; a < 8; a++)
+    for (int b = a + 1; b < 8; b++)
+      for (int c = b + 1; c < 8; c++) {
+        if (idx == 0) return Mono8{3, a, b, c};
+        idx--;
+      }
+  return Mono8{0, 0, 0, 0};
+}
+template <int J8>
+__device__ __forceinline__ uint32_t mono8(const uint32_t* T, const uint32_t* M) {
+  constexpr Mono8 mo = mono8_of(J8);
+  if constexpr (mo.n == 1)
+    return T[mo.a];
+  else if constexpr (mo.n == 2)
+    return T[mo.b] & M[mo.a];
+  else
+    return (T[mo.a] ^ T[mo.b] ^ T[mo.c]) & M[mo.a] & M[mo.b] & M[mo.c];
+}
+constexpr int M8_GROUP = 20, M8_NGROUPS = 6, M8_LDS_ROW = 20;
+// monomials 20 G .. 20 G + 19 of the four 4-row words (T[q][0..7], masks T[q][8..15]), each
+// straight into its MFMA
+template <int G, int... I>
+__device__ __forceinline__ void mono8_mfma(const uint32_t (&T)[4][16], v4i a, v4i (&acc)[M8_GROUP],
+                                           std::integer_sequence<int, I...>) {
+  ((acc[I] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+        a,
+        v4i{(int)mono8<M8_GROUP * G + I>(T[0], T[0] + 8), (int)mono8<M8_GROUP * G + I>(T[1], T[1] + 8),
+            (int)mono8<M8_GROUP * G + I>(T[2], T[2] + 8), (int)mono8<M8_GROUP * G + I>(T[3], T[3] + 8)},
+        acc[I], 0, 0, 0)),
+   ...);
+}
+
+__global__ __launch_bounds__(64 * M8_NGROUPS) void k_digit_monomials8(const int8_t* const* __restrict__ digits, int k,
+                                                                     size_t m, const int8_t* __restrict__ wl,
+                                                                     size_t pairs, K2* __restrict__ U) {
+  __shared__ __align__(16) uint32_t tm[2][16][16][M8_LDS_ROW];  // [buffer][row quad][pair][T 0..7, M 0..7]
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  const size_t tb = (size_t)blockIdx.x * 16;
+  // transpose unit of thread u < 256: pair u % 16, row quad u / 16 (rows past 53 re-read row 53:
+  // their weight limbs are zero)
+  const int u = threadIdx.x, un = u & 15, uq = u >> 4;
+  const bool unit = u < 256;
+  const size_t uoff = 8 * (tb + (size_t)un);
+  uint2 x[4];
+  auto load = [&](int w) {
+    const int8_t* base = digits[w] + uoff;
+#pragma unroll
+    for (int i = 0; i < 4; i++) x[i] = *(const uint2*)(base + (size_t)min(4 * uq + i, D - 1) * m);
+  };
+  if (unit) load(0);
+  v4i acc[M8_GROUP];
+#pragma unroll
+  for (int a = 0; a < M8_GROUP; a++) acc[a] = v4i{0, 0, 0, 0};
+  const size_t ldk = (size_t)64 * k;
+  for (int w = 0; w < k; w++) {
+    const int buf = w & 1;
+    if (unit) {
+      uint32_t T[16];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint32_t x0 = h ? x[0].y : x[0].x, x1 = h ? x[1].y : x[1].x;
+        const uint32_t x2 = h ? x[2].y : x[2].x, x3 = h ? x[3].y : x[3].x;
+        const uint32_t l01 = __builtin_amdgcn_perm(x1, x0, 0x05010400u), h01 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
+        const uint32_t l23 = __builtin_amdgcn_perm(x3, x2, 0x05010400u), h23 = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+        T[4 * h + 0] = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+        T[4 * h + 1] = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+        T[4 * h + 2] = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+        T[4 * h + 3] = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) T[8 + i] = mono_mask(T[i]);
+      uint4* dst = (uint4*)tm[buf][uq][un];
+#pragma unroll
+      for (int v = 0; v < 4; v++) dst[v] = uint4{T[4 * v], T[4 * v + 1], T[4 * v + 2], T[4 * v + 3]};
+      if (w + 1 < k) load(w + 1);  // in flight during this instance's MFMAs
+    }
+    __syncthreads();  // buffer `buf` written; the other buffer's readers (instance w - 1) are done
+    const v4i a = *(const v4i*)(wl + (size_t)r16 * ldk + (size_t)w * 64 + 16 * kg);
+    uint32_t T[4][16];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4* src = (const uint4*)tm[buf][4 * kg + q][r16];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint4 y = src[v];
+        T[q][4 * v] = y.x;
+        T[q][4 * v + 1] = y.y;
+        T[q][4 * v + 2] = y.z;
+        T[q][4 * v + 3] = y.w;
+      }
+    }
+    constexpr auto seq = std::make_integer_sequence<int, M8_GROUP>{};
+    switch (grp) {
+      case 0: mono8_mfma<0>(T, a, acc, seq); break;
+      case 1: mono8_mfma<1>(T, a, acc, seq); break;
+      case 2: mono8_mfma<2>(T, a, acc, seq); break;
+      case 3: mono8_mfma<3>(T, a, acc, seq); break;
+      case 4: mono8_mfma<4>(T, a, acc, seq); break;
+      default: mono8_mfma<5>(T, a, acc, seq); break;
+    }
+  }
+#pragma unroll
+  for (int jj = 0; jj < M8_GROUP; jj++) {
+    const int64_t p = (int64_t)acc[jj][0] + ((int64_t)acc[jj][1] << 8) + ((int64_t)acc[jj][2] << 16) +
+                      ((int64_t)acc[jj][3] << 24);
+    const int plo = __shfl_xor((int)(uint32_t)p, 16, 64), phi = __shfl_xor((int)(uint32_t)((uint64_t)p >> 32), 16, 64);
+    const int64_t hi = (int64_t)(((uint64_t)(uint32_t)phi << 32) | (uint32_t)plo);
+    if ((kg & 1) == 0) {
+      const uint64_t v = fadd(from_i64_dev(p), fmul(from_i64_dev(hi), (uint64_t)1 << 32));
+      uint64_t* dst = (uint64_t*)(U + (size_t)(grp * M8_GROUP + jj) * pairs + tb + r16);
+      dst[kg >> 1] = v;
+    }
+  }
+}
+
+
 // coefficient vector of width 5 (deg_max = 4).  Work item = (row pair t, row group g): the NC
 // sum over the k*54 digit rows of a pair is split into G groups so that short tables still fill
 // the GPU; every group multiplies its partial NC polynomial by the pair's eq_beta(X) (the
 // round polynomial is linear in the NC sum), and group 0 adds the f and Eval terms.
 // MODE 0: round 0 over the int8 digits; MODE 1: round 1 over the int8 digits (tables folded once
-// by r0, see below); MODE 2: K tables.
+// by r0, see below); MODE 2: K tables; MODE 6: no rows (G = 0): the NC part from the weighted
+// monomial sums U of k_digit_monomials(8) in the first A.nb_u blocks, the f and Eval items after.
 template <int MODE>
 __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __restrict__ wg, K2* __restrict__ partials,
                                                    int G) {
@@ -706,7 +843,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
   }
   __shared__ const int8_t* dptr[32];  // digit planes of the k instances (rounds 0 and 1)
   if ((MODE == 0 || MODE == 1) && threadIdx.x < A.k && threadIdx.x < 32) dptr[threadIdx.x] = A.digits[threadIdx.x];
-  if (MODE == 1 || MODE == 4)
+  if (MODE == 1)
     for (int i = threadIdx.x; i < 80; i += blockDim.x) {
       const K2 p = A.pc[i];
       pcs[i] = p;
@@ -714,20 +851,68 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
     }
   __syncthreads();
   const size_t pairs = A.len / 2;
-  const int R = (nrow + G - 1) / G;
+  const int R = G > 0 ? (nrow + G - 1) / G : 0;  // G = 0 (mode 6): the f and Eval items only
   // groups 0..G-1 take R digit rows each; group G takes the f and Eval terms of the pair
   K2 c[5];
 #pragma unroll
   for (int d = 0; d < 5; d++) c[d] = kzero();
   K2 ev0 = kzero(), ev1 = kzero(), ev2 = kzero();
-  // round 1 splits each (pair, row group) item into two halves of the 20 digit monomials (four
-  // quarters when they come from U)
-  const int H = MODE == 1 ? 2 : (MODE == 4 ? 4 : 1);
+  // round 1 splits each (pair, row group) item into two halves of the 20 digit monomials
+  const int H = MODE == 1 ? 2 : 1;
   // items per pair: G * H row items, then the f item and (with r_in) the Eval item
   const int nfe = A.eqr ? 2 : 1;
   const size_t total = pairs * ((size_t)G * H + nfe);
   const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+  // mode 6: blocks below A.nb_u take the NC part from U (E-sums of monomial j over a slice of
+  // pairs, times pc_j), the rest take the f and Eval items
+  unsigned bid = blockIdx.x, nbk = gridDim.x;
+  if (MODE == 6 && blockIdx.x < (unsigned)A.nb_u) {
+    const int j = blockIdx.x / A.nslice, sl = blockIdx.x % A.nslice;
+    const size_t per = pairs / A.nslice, t0 = (size_t)sl * per;
+    KS2 a0 = ks2_zero(), a1 = ks2_zero();
+    for (size_t t = t0 + threadIdx.x; t < t0 + per; t += 2 * blockDim.x) {
+      // two pairs' loads in flight at once (per is a multiple of 2 blockDim or below blockDim)
+      const size_t t2 = t + blockDim.x;
+      const bool two = t2 < t0 + per;
+      const K2 e0 = A.eqb[2 * t], e0h = A.eqb[2 * t + 1], u = A.U[(size_t)j * pairs + t];
+      K2 f0 = kzero(), f0h = kzero(), v = kzero();
+      if (two) {
+        f0 = A.eqb[2 * t2];
+        f0h = A.eqb[2 * t2 + 1];
+        v = A.U[(size_t)j * pairs + t2];
+      }
+      ks2_mac(a0, e0, u);
+      ks2_mac(a1, ksub(e0h, e0), u);
+      ks2_mac(a0, f0, v);
+      ks2_mac(a1, ksub(f0h, f0), v);
+    }
+    // block totals of E0, E1, then coefficient d = pc_j[d] E0 + pc_j[d - 1] E1 by thread d
+    K2* red = sh + nrow + 80;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const K2 s0 = wave_sum(ks2_reduce(a0)), s1 = wave_sum(ks2_reduce(a1));
+    if (lane == DPP_LANE) {
+      red[wave] = s0;
+      red[4 + wave] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      const int d = threadIdx.x;
+      const K2 E0 = kadd(kadd(red[0], red[1]), kadd(red[2], red[3]));
+      const K2 E1 = kadd(kadd(red[4], red[5]), kadd(red[6], red[7]));
+      K2 v = d < 4 ? kmul(A.pc[j * 4 + d], E0) : kzero();
+      if (d >= 1) v = kadd(v, kmul(A.pc[j * 4 + d - 1], E1));
+      partials[(size_t)blockIdx.x * 5 + d] = v;
+    }
+    return;  // block-uniform
+  }
+  bool items = true;
+  if (MODE == 6) {
+    items = blockIdx.x >= (unsigned)A.nb_u;  // the E-sum blocks take no items
+    bid = items ? blockIdx.x - A.nb_u : 0;
+    nbk -= A.nb_u;
+  }
+  for (size_t idx = items ? (size_t)bid * blockDim.x + threadIdx.x : total; idx < total;
+       idx += (size_t)nbk * blockDim.x) {
     const size_t t = idx & (pairs - 1);
     const int gh = (int)(idx >> lp);
     const bool fe = gh >= G * H;  // f item (gh = G H) or Eval item (gh = G H + 1)
@@ -918,25 +1103,7 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
       }
 #undef NS_UPD
 #undef NS_APPLY
-    } else if (MODE == 3) {
-      // round 0 from U: the four weighted monomial sums of the pair, then the cubics as in mode 0
-      const K2 slo = A.U[t], shi = A.U[pairs + t], s3 = A.U[2 * pairs + t], s4 = A.U[3 * pairs + t];
-      K2 s3x3 = kadd(kadd(s3, s3), s3);
-      K2 s4x3 = kadd(kadd(s4, s4), s4);
-      K2 slo2 = kadd(slo, slo);
-      in[1] = kadd(in[1], ksub(ksub(s3x3, slo2), shi));
-      in[2] = kadd(in[2], ksub(kadd(kadd(slo2, slo), s4x3), kadd(s3x3, s3x3)));
-      in[3] = kadd(in[3], ksub(kadd(shi, s3x3), kadd(slo, s4x3)));
-    } else if (MODE == 4) {
-      // round 1 from U: monomials 5 h .. 5 h + 4 of the pair times their cubics
-#pragma unroll
-      for (int jj = 0; jj < 5; jj++) {
-        const int j = 5 * h + jj;
-        const K2 av = A.U[(size_t)j * pairs + t];
-#pragma unroll
-        for (int d = 0; d < 4; d++) in[d] = kadd(in[d], kmul_w7(pcs[j * 4 + d], pcs7[j * 4 + d], av));
-      }
-    } else {
+    } else if (MODE == 2) {
       // w N(a + b X) = w (a^3 - a) + w (3 a^2 b - b) X + 3 w a b^2 X^2 + w b^3 X^3: wa, wb, a^2, b^2
       // reduced per row, the four products and two sums accumulated lazily over the rows
       KS2 z1 = ks2_zero(), z3 = ks2_zero(), z5 = ks2_zero(), z6 = ks2_zero();
@@ -1132,27 +1299,39 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   K2* partials = (K2*)ws;
   A.pc = T.pc;
   A.U = nullptr;
-  // rounds 0 and 1 on int8 MFMA (k_digit_monomials) when the tables allow whole 64-pair blocks
-  // and U fits the buffer given for it; then the round kernel reads U in place of the digits
-  const int dmode = round0 ? 0 : (T.digit_round1 ? 1 : -1);
-  const size_t nmono = dmode == 0 ? 4 : 20;
-  if (dmode >= 0 && T.wl && T.ubuf && pairs >= 128 && pairs % 128 == 0 && T.k <= 32 &&
+  A.nb_u = 0;
+  A.nslice = 1;
+  // rounds 0, 1 and 2 on int8 MFMA when the tables allow whole 128-pair blocks (round 2: 64) and U
+  // fits the buffer given for it: k_digit_monomials(8) -> U, then mode 6 (the NC part from U, the
+  // f and Eval items), one partial sum
+  const int dmode = round0 ? 0 : (T.digit_round1 ? 1 : (T.digit_round2 ? 2 : -1));
+  const size_t nmono = dmode == 0 ? 4 : (dmode == 1 ? 20 : 120);
+  const size_t min_pairs = dmode == 2 ? 64 : 128;
+  if (dmode >= 0 && T.wl && T.ubuf && T.pc && pairs >= min_pairs && pairs % min_pairs == 0 && T.k <= 32 &&
       T.ubuf_bytes >= nmono * pairs * sizeof(K2)) {
-    const size_t per_block = dmode == 0 ? 128 : 64;
-    const dim3 g1((unsigned)(pairs / per_block));
-    A.U = T.ubuf;
-    const size_t H = dmode == 0 ? 1 : 4;
-    size_t nblk = (pairs * (H + (T.eqr ? 2 : 1)) + RB - 1) / RB;
-    if (nblk > RMAXBLK) nblk = RMAXBLK;
-    const size_t shmem = ((size_t)T.k * D * 2 + 80 + 4 * 5 + 40) * sizeof(K2);
-    if (dmode == 0) {
-      NS_LAUNCH_TIMED(KT_ROUND0, k_digit_monomials<0>, g1, dim3(256), 0, s, T.digits, T.k, T.m, T.wl, pairs, T.ubuf);
-      NS_LAUNCH_TIMED(KT_ROUND0, k_round_eval<3>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, 1);
+    const int kt = dmode == 0 ? KT_ROUND0 : (dmode == 1 ? KT_ROUND1 : KT_ROUND);
+    if (dmode == 2) {
+      NS_LAUNCH_TIMED(kt, k_digit_monomials8, dim3((unsigned)(pairs / 16)), dim3(64 * M8_NGROUPS), 0, s, T.digits,
+                      T.k, T.m, T.wl, pairs, T.ubuf);
+    } else if (dmode == 1) {
+      NS_LAUNCH_TIMED(kt, k_digit_monomials<1>, dim3((unsigned)(pairs / 64)), dim3(256), 0, s, T.digits, T.k, T.m,
+                      T.wl, pairs, T.ubuf);
     } else {
-      NS_LAUNCH_TIMED(KT_ROUND1, k_digit_monomials<1>, g1, dim3(256), 0, s, T.digits, T.k, T.m, T.wl, pairs, T.ubuf);
-      NS_LAUNCH_TIMED(KT_ROUND1, k_round_eval<4>, dim3((unsigned)nblk), dim3(RB), shmem, s, A, wg, partials, 1);
+      NS_LAUNCH_TIMED(kt, k_digit_monomials<0>, dim3((unsigned)(pairs / 128)), dim3(256), 0, s, T.digits, T.k, T.m,
+                      T.wl, pairs, T.ubuf);
     }
-    launch_sum_partials(partials, nblk, 5, out, sig, s);
+    size_t nb_fe = (pairs * (T.eqr ? 2 : 1) + RB - 1) / RB;
+    if (nb_fe > RMAXBLK / 4) nb_fe = RMAXBLK / 4;
+    // E-sum slices of 512 pairs (two per thread), as many as the partial slots allow (powers of two)
+    size_t nslice = std::max<size_t>(1, pairs / 512);
+    while (nslice > 1 && nmono * nslice + nb_fe > RMAXBLK) nslice /= 2;
+    const size_t nb_u = nmono * nslice;
+    A.U = T.ubuf;
+    A.nb_u = (int)nb_u;
+    A.nslice = (int)nslice;
+    const size_t shmem = ((size_t)T.k * D * 2 + 80 + 4 * 5 + 40) * sizeof(K2);
+    NS_LAUNCH_TIMED(kt, k_round_eval<6>, dim3((unsigned)(nb_u + nb_fe)), dim3(RB), shmem, s, A, wg, partials, 0);
+    launch_sum_partials(partials, nb_u + nb_fe, 5, out, sig, s);
     return;
   }
   // Split the k*54 NC rows of a pair across G work items: long tables get long items (each item

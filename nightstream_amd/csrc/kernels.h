@@ -135,10 +135,14 @@ struct RowTables {
   int k;
   K2 gamma_k;
   bool digit_round1 = false;  // round 1 evaluated from the digits (tables folded once, by r0)
-  const K2* pc = nullptr;     // [20][4] monomial coefficient polynomials for digit_round1
-  // rounds 0 and 1 on int8 MFMA: the NC weights as balanced base-256 limbs, [16][64 k] int8
-  // (weight_limbs), and device scratch for the weighted digit-monomial sums (4 x len / 2 K
-  // values in round 0, 20 x len / 2 in round 1); without them the rounds run on the vector ALUs
+  bool digit_round2 = false;  // round 2 evaluated from the digits (int8 MFMA path only)
+  // coefficient polynomials (X^0..X^3) of the digit monomials: [4][4] in round 0 (MFMA path),
+  // [20][4] in round 1, [120][4] in round 2
+  const K2* pc = nullptr;
+  // rounds 0-2 on int8 MFMA: the NC weights as balanced base-256 limbs, [16][64 k] int8
+  // (weight_limbs), and device scratch for the weighted digit-monomial sums (4, 20 or 120 x len / 2
+  // K values in rounds 0, 1, 2); without them rounds 0 and 1 run on the vector ALUs and round 2
+  // on the K tables
   const int8_t* wl = nullptr;
   K2* ubuf = nullptr;
   size_t ubuf_bytes = 0;

@@ -700,7 +700,8 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
     launch_chi_suffix(bits, (int)ell_n, 3, g_total_, has_eval_ ? 2 : 1, w_->gt.as<K2>(), st);
   }
   w_->nc[0].ensure(k * D * (n_pad / 2) * sizeof(K2));
-  w_->nc[1].ensure(k * D * std::max<size_t>(n_pad / 4, 1) * sizeof(K2));
+  // nc[1] also holds round 2's 120 x n/8 digit-monomial sums (larger than the tables when k = 1)
+  w_->nc[1].ensure(std::max(k * D * std::max<size_t>(n_pad / 4, 1), 15 * n_pad) * sizeof(K2));
   w_->coeffs.ensure(8 * sizeof(K2));
   w_->ws.ensure(round_ws_bytes());
   NS_CHECK(hipStreamSynchronize(st));  // host vectors above go out of scope
@@ -912,42 +913,66 @@ std::vector<K2> CcsOracle::round_coeffs() {
   T.m = s_.m;
   T.k = (int)k_;
   T.gamma_k = gamma_k_;
-  if (round_ <= 1) {  // the NC tables do not exist yet: their buffer holds the monomial sums
-    T.wl = w_->wl.as<const int8_t>();
-    T.ubuf = w_->nc[0].as<K2>();
-    T.ubuf_bytes = w_->nc[0].bytes;
-  }
-  if (round_ == 1) {
-    // cubic coefficients of the 20 digit monomials for N((1-X) a + X a') with
-    // a = (1-r0) d0 + r0 d1, a' = (1-r0) d2 + r0 d3
-    const K2 al = ksub(kone(), r0_), be = r0_;
-    typedef std::array<K2, 4> P4;
-    auto lin = [](K2 c0, K2 c1) { return P4{c0, c1, kzero(), kzero()}; };
-    auto mul = [](const P4& a, const P4& b) {
-      P4 o{kzero(), kzero(), kzero(), kzero()};
-      for (int i = 0; i < 4; i++)
-        for (int j = 0; i + j < 4; j++) o[i + j] = kadd(o[i + j], kmul(a[i], b[j]));
-      return o;
-    };
-    auto scale = [](const P4& a, uint64_t c) { return P4{kscale(a[0], c), kscale(a[1], c), kscale(a[2], c), kscale(a[3], c)}; };
-    const P4 L[4] = {lin(al, kneg(al)), lin(be, kneg(be)), lin(kzero(), al), lin(kzero(), be)};
-    std::vector<K2> pc;
-    pc.reserve(80);
-    auto push = [&](const P4& p) { pc.insert(pc.end(), p.begin(), p.end()); };
-    for (int i = 0; i < 4; i++) {
+  // rounds 0-2 from the digits: the coefficient polynomials of the digit monomials, for the int8
+  // MFMA path (and round 1's vector path).  With y(X) = sum_i L_i(X) d_i (digits in {-1, 0, 1}, so
+  // d^2 = |d| and d^3 = d), N(y) = y^3 - y = sum_i (L_i^3 - L_i) d_i + 3 sum_{i != j} L_i^2 L_j |d_i| d_j
+  // + 6 sum_{i < j < l} L_i L_j L_l d_i d_j d_l.
+  typedef std::array<K2, 4> P4;
+  auto lin = [](K2 c0, K2 c1) { return P4{c0, c1, kzero(), kzero()}; };
+  auto mul = [](const P4& a, const P4& b) {
+    P4 o{kzero(), kzero(), kzero(), kzero()};
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; i + j < 4; j++) o[i + j] = kadd(o[i + j], kmul(a[i], b[j]));
+    return o;
+  };
+  auto scale = [](const P4& a, uint64_t c) { return P4{kscale(a[0], c), kscale(a[1], c), kscale(a[2], c), kscale(a[3], c)}; };
+  std::vector<K2> pc;
+  auto push = [&](const P4& p) { pc.insert(pc.end(), p.begin(), p.end()); };
+  auto monomial_polys = [&](const std::vector<P4>& L) {
+    const int nd = (int)L.size();
+    for (int i = 0; i < nd; i++) {
       P4 c = mul(mul(L[i], L[i]), L[i]);
       for (int d = 0; d < 4; d++) c[d] = ksub(c[d], L[i][d]);
       push(c);
     }
-    for (int i = 0; i < 4; i++)
-      for (int j = 0; j < 4; j++)
+    for (int i = 0; i < nd; i++)
+      for (int j = 0; j < nd; j++)
         if (i != j) push(scale(mul(mul(L[i], L[i]), L[j]), 3));
-    const int trip[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
-    for (auto& tr : trip) push(scale(mul(mul(L[tr[0]], L[tr[1]]), L[tr[2]]), 6));
-    w_->pc.ensure(80 * sizeof(K2));
-    h2d(w_->pc.p, pc.data(), pc.size(), st_);
+    for (int i = 0; i < nd; i++)
+      for (int j = i + 1; j < nd; j++)
+        for (int l = j + 1; l < nd; l++) push(scale(mul(mul(L[i], L[j]), L[l]), 6));
+  };
+  if (round_ == 0) {
+    // lo, hi, |lo| hi, lo |hi| of a digit pair: N(lo + (hi - lo) X) = lo A + hi B + 3 |lo| hi C + 3 lo |hi| D
+    auto ki = [](int64_t v) { return K2{from_i64(v), 0}; };
+    push(P4{kzero(), ki(-2), ki(3), ki(-1)});
+    push(P4{kzero(), ki(-1), kzero(), ki(1)});
+    push(P4{kzero(), ki(3), ki(-6), ki(3)});
+    push(P4{kzero(), kzero(), ki(3), ki(-3)});
+  } else if (round_ == 1) {
+    // a = (1-r0) d0 + r0 d1, a' = (1-r0) d2 + r0 d3, y = (1-X) a + X a'
+    const K2 al = ksub(kone(), r0_), be = r0_;
+    monomial_polys({lin(al, kneg(al)), lin(be, kneg(be)), lin(kzero(), al), lin(kzero(), be)});
     T.digit_round1 = true;
+  } else if (round_ == 2 && len_ >= 2) {
+    // the pair's entries fold digits 8t..8t+3 and 8t+4..8t+7 by chi_(r0, r1)
+    const K2 a0 = ksub(kone(), r0_), a1 = ksub(kone(), r1_);
+    const K2 w[4] = {kmul(a0, a1), kmul(r0_, a1), kmul(a0, r1_), kmul(r0_, r1_)};
+    std::vector<P4> L;
+    for (int i = 0; i < 4; i++) L.push_back(lin(w[i], kneg(w[i])));
+    for (int i = 0; i < 4; i++) L.push_back(lin(kzero(), w[i]));
+    monomial_polys(L);
+    T.digit_round2 = true;
+  }
+  if (!pc.empty()) {
+    w_->pc.ensure(pc.size() * sizeof(K2));
+    h2d(w_->pc.p, pc.data(), pc.size(), st_);
     T.pc = w_->pc.as<K2>();
+    // the monomial sums go where no live table is: nc[0] before the tables exist, else the other one
+    DevBuf& ub = round_ <= 1 ? w_->nc[0] : w_->nc[cur_nc_ ^ 1];
+    T.wl = w_->wl.as<const int8_t>();
+    T.ubuf = ub.as<K2>();
+    T.ubuf_bytes = ub.bytes;
   }
   K2* mb = w_->mbox.get<K2>(8);
   const RoundSignal sg = next_signal();
@@ -991,6 +1016,7 @@ void CcsOracle::fold(K2 r) {
       cur_nc_ = 0;
     }
   } else if (round_ == 1 && len_ * 2 >= 4) {
+    r1_ = r;
     launch_fold_digits2(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_ * 2, w_->nc[0].as<K2>(), r0_, r, st_);
     cur_nc_ = 0;
   } else {

@@ -302,6 +302,7 @@ class CcsOracle {
   int cur_sc_ = 0, cur_nc_ = 0;
   std::vector<const int8_t*> zs_;
   K2 r0_ = K2{0, 0};  // round-0 challenge: round 1 is evaluated from the digits with it
+  K2 r1_ = K2{0, 0};  // round-1 challenge: with r0, round 2 is evaluated from the digits
   // From round 2 on, fold(r) only records r: the next round_coeffs folds and evaluates in one
   // launch (launch_round_fused), and anything else that reads the tables flushes it first.
   bool pending_ = false;
