@@ -31,9 +31,11 @@ INT8_PEAK_TOPS = 5000.0
 KT = ["round0_eval", "round_eval", "table_fold", "ajtai_gemm", "ydigits_gemm", "rlc_zmix", "dec_split", "round1_eval",
       "round_fused"]
 KERNEL_NAMES = {
-    "round0_eval": "k_round_eval<0>",
-    "round1_eval": "k_round_eval<1>",
-    "round_eval": "k_round_eval<2>",
+    # rounds 0-2: the digit-monomial MFMA kernel, then the E-sum / f / Eval launch (shared by the
+    # three rounds); below the MFMA path's sizes, k_round_eval<0>, <1> and <2>
+    "round0_eval": ("k_digit_monomials<0>", "k_round_eval<6>"),
+    "round1_eval": ("k_digit_monomials<1>", "k_round_eval<6>"),
+    "round_eval": ("k_digit_monomials8", "k_round_eval<6>"),
     "table_fold": "k_fold_k",
     "ajtai_gemm": ("k_ajtai_gemm32p", "k_ajtai_gemm32", "k_ajtai_gemm"),
     "ydigits_gemm": "k_ydigits_gemm<4>",

@@ -380,7 +380,24 @@ __global__ __launch_bounds__(256) void k_ydigits_gemm(const int8_t* const* __res
   v4i acc[NS];
 #pragma unroll
   for (int n = 0; n < NS; n++) acc[n] = v4i{0, 0, 0, 0};
-  for (size_t k = 0; k < kc; k += 64) {
+  // four K steps' loads in flight at once, then their MFMAs (one step's loads alone leave the wave
+  // waiting a memory latency per step)
+  constexpr int UNR = 4;
+  size_t k = 0;
+  for (; k + UNR * 64 <= kc; k += UNR * 64) {
+    v4i a[UNR], b[UNR][NS];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      a[u] = aval ? *(const v4i*)(arow + k + 64 * u) : v4i{0, 0, 0, 0};
+#pragma unroll
+      for (int n = 0; n < NS; n++) b[u][n] = *(const v4i*)(brow[n] + k + 64 * u);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+#pragma unroll
+      for (int n = 0; n < NS; n++) acc[n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u], b[u][n], acc[n], 0, 0, 0);
+  }
+  for (; k < kc; k += 64) {
     v4i a = aval ? *(const v4i*)(arow + k) : v4i{0, 0, 0, 0};
 #pragma unroll
     for (int n = 0; n < NS; n++) {
