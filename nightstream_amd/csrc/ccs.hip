@@ -1525,6 +1525,63 @@ __device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __re
   in[3] = s6;
 }
 
+// x / 2 in F_q per component (x odd: (x + q) / 2)
+__device__ __forceinline__ K2 khalf(K2 x) {
+  return K2{(x.c0 >> 1) + ((x.c0 & 1) ? (Q + 1) / 2 : 0), (x.c1 >> 1) + ((x.c1 & 1) ? (Q + 1) / 2 : 0)};
+}
+
+// Short rounds (few pairs): one NC row of pair t split four ways by evaluation point, so that a
+// lane's chain is one fold, one square and two products.  Lane s evaluates w g N(y(X)) at X = 0
+// (s = 0), 1 (s = 1), -1 (s = 3) and takes the leading coefficient w g b^3 (s = 2), all from the
+// folded line y = a + b X (b, y(-1) = 2a - y(1) fold linearly from the previous entries); with
+// V0, V1, V-1, Vinf those values the cubic's coefficients are P0 = V0, P3 = Vinf,
+// P1 = (V1 - V-1)/2 - Vinf, P2 = (V1 + V-1)/2 - V0, and each lane adds its share.  Lanes 0 and 1
+// write the folded entries for the next round.
+__device__ __forceinline__ void fused_nc_split(const FusedArgs& A, const K2* __restrict__ wg, size_t t, int row, int s,
+                                               K2 (&c)[FUSED_NCOEF]) {
+  const K2* src = A.nc_src + (size_t)row * (2 * A.len) + 4 * t;
+  const K2 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
+  const K2 gt = A.gb[t], wv = wg[row];
+  const K2 lo = s == 0 ? u0 : (s == 1 ? u2 : (s == 2 ? ksub(u2, u0) : ksub(kadd(u0, u0), u2)));
+  const K2 hi = s == 0 ? u1 : (s == 1 ? u3 : (s == 2 ? ksub(u3, u1) : ksub(kadd(u1, u1), u3)));
+  const K2 w = kmul_w7(gt, fmul7(gt.c1), wv);
+  const K2 y = fold2(lo, hi, A.r, A.r7);
+  if (s < 2) A.nc_dst[(size_t)row * A.len + 2 * t + s] = y;
+  const K2 y2 = ksqr_dev(y);
+  const K2 wy = kmul_w7(w, fmul7(w.c1), y);
+  const K2 v = kmul(wy, s == 2 ? y2 : ksub(y2, kone()));
+  const K2 hv = khalf(v), nv = kneg(v), nh = kneg(hv);
+  c[0] = kadd(c[0], s == 0 ? v : kzero());
+  c[1] = kadd(c[1], s == 1 ? hv : (s == 2 ? nv : (s == 3 ? nh : kzero())));
+  c[2] = kadd(c[2], s == 0 ? nv : (s == 2 ? kzero() : hv));
+  c[3] = kadd(c[3], s == 2 ? v : kzero());
+}
+
+// The R1CS f item of a short round split three ways: lane s evaluates g P at X = 0 (s = 0), 1
+// (s = 1) and takes the leading coefficient g c0 a' b' (s = 2); P1 = P(1) - P0 - P2.
+__device__ __forceinline__ void fused_f_split(const FusedArgs& A, size_t t, int s, K2 (&c)[FUSED_NCOEF]) {
+  const size_t L2 = 2 * A.len;
+  const K2* pa = A.sc_src + (size_t)(1 + A.r1cs[0]) * L2 + 4 * t;
+  const K2* pb = A.sc_src + (size_t)(1 + A.r1cs[1]) * L2 + 4 * t;
+  const K2* pc = A.sc_src + (size_t)(1 + A.r1cs[2]) * L2 + 4 * t;
+  const K2 a0 = pa[0], a1 = pa[1], a2 = pa[2], a3 = pa[3];
+  const K2 b0 = pb[0], b1 = pb[1], b2 = pb[2], b3 = pb[3];
+  const K2 e0 = pc[0], e1 = pc[1], e2 = pc[2], e3 = pc[3];
+  const K2 gt = A.gb[t];
+  const K2 alo = s == 0 ? a0 : (s == 1 ? a2 : ksub(a2, a0)), ahi = s == 0 ? a1 : (s == 1 ? a3 : ksub(a3, a1));
+  const K2 blo = s == 0 ? b0 : (s == 1 ? b2 : ksub(b2, b0)), bhi = s == 0 ? b1 : (s == 1 ? b3 : ksub(b3, b1));
+  const K2 elo = s == 0 ? e0 : e2, ehi = s == 0 ? e1 : e3;
+  const K2 ya = fold2(alo, ahi, A.r, A.r7), yb = fold2(blo, bhi, A.r, A.r7), ye = fold2(elo, ehi, A.r, A.r7);
+  K2 p = kmul_w7(ya, fmul7(ya.c1), yb);
+  const uint64_t c0 = A.r1cs_c[0], c1 = A.r1cs_c[1];
+  if (c0 != 1) p = kscale(p, c0);
+  if (s < 2) p = c1 == Q - 1 ? ksub(p, ye) : kadd(p, kscale(ye, c1));
+  const K2 v = kmul_w7(gt, fmul7(gt.c1), p);
+  c[0] = kadd(c[0], s == 0 ? v : kzero());
+  c[1] = kadd(c[1], s == 1 ? v : kneg(v));
+  c[2] = kadd(c[2], s == 2 ? v : kzero());
+}
+
 // One block's share of a fused round: block bid of nb blocks over the work items of A.  Shared by
 // the single-session launch and the ragged launch over several sessions' rounds.
 __device__ __forceinline__ void fused_block(const FusedArgs& A, const K2* __restrict__ wg, K2* __restrict__ partials,
@@ -1537,11 +1594,13 @@ __device__ __forceinline__ void fused_block(const FusedArgs& A, const K2* __rest
 #pragma unroll
   for (int d = 0; d < FUSED_NCOEF; d++) c[d] = kzero();
   // NC items first, then (from a wave boundary, so that no wave runs both kinds) the f item and
-  // the Eval item of every pair and the scalar fold items
+  // the Eval item of every pair and the scalar fold items.  Short rounds (A.split) take one row
+  // per NC item split four ways and the R1CS f item split three ways.
   const int ne = A.has_eval ? 1 : 0;
-  const size_t nnc = pairs * (size_t)G, base_s = (nnc + 63) & ~(size_t)63;
+  const int spl = A.split ? 4 : 1, fs = A.split && A.r1cs[0] >= 0 ? 3 : 1;
+  const size_t nnc = pairs * (size_t)G * spl, base_s = (nnc + 63) & ~(size_t)63;
   int has_s = 0;
-  const size_t total = base_s + pairs * (1 + ne + (size_t)nfold);
+  const size_t total = base_s + pairs * (fs + ne + (size_t)nfold);
   const int fdeg = A.fdeg;  // degree of f (uniform): coefficients above it are zero
   const int lp = __builtin_ctzll(pairs);  // pairs is a power of two (tables are 2^ell_n long)
   for (size_t idx = (size_t)bid * blockDim.x + threadIdx.x; idx < total; idx += (size_t)nb * blockDim.x) {
@@ -1550,7 +1609,17 @@ __device__ __forceinline__ void fused_block(const FusedArgs& A, const K2* __rest
     has_s |= !nc_item;
     const size_t li = nc_item ? idx : idx - base_s;
     const size_t t = li & (pairs - 1);
-    const int g = nc_item ? (int)(li >> lp) : G + (int)(li >> lp);
+    const size_t hi_ = li >> lp;
+    if (nc_item && A.split) {  // one row, sub-item s
+      fused_nc_split(A, wg, t, (int)(hi_ >> 2), (int)(hi_ & 3), c);
+      continue;
+    }
+    if (!nc_item && fs == 3 && hi_ < 3) {
+      fused_f_split(A, t, (int)hi_, c);
+      continue;
+    }
+    // g = G: the f item, G + 1: the Eval item (with r_in), then the fold items
+    const int g = nc_item ? (int)hi_ : G + (int)hi_ - (fs - 1);
     if (g > G + ne) {  // fold items: f-variable tables 1..nf, then the Eval table 2 + nf
       const int i = g - G - 1 - ne;
       fused_fold_scalar(A, t, i < A.nf ? 1 + i : 2 + A.nf);
@@ -1639,19 +1708,24 @@ FusedJob fused_job(const FusedArgs& A, const K2* wg, K2* out, void* ws, const Ro
   if (pairs == 0 || (pairs & (pairs - 1))) throw std::invalid_argument("fused round: table length must be a power of two >= 2");
   if (A.nf + 1 > FUSED_MAX_SC - (A.has_eval ? 2 : 0)) throw std::invalid_argument("fused round: too many scalar tables");
   if (!A.gb || (A.has_eval && !A.gr)) throw std::invalid_argument("fused round: g tables missing");
-  const size_t target = (size_t)RB * 512;
-  size_t G = 1;
-  while (G < nrow && pairs * G < target) G *= 2;
-  if (G > nrow) G = nrow;
-  G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
-  const size_t nfold = (size_t)A.nf + (A.has_eval ? 1 : 0);
-  const size_t base_s = (pairs * G + 63) & ~(size_t)63;
-  size_t nblk = (base_s + pairs * (1 + (A.has_eval ? 1 : 0) + nfold) + RB - 1) / RB;
-  if (nblk > RMAXBLK) nblk = RMAXBLK;
-  if (nblk < 1) nblk = 1;
   FusedJob J;
   J.A = A;
   classify_f(J.A);
+  // rounds of at most 32 pairs leave the GPU mostly idle and wait on one work item's chain of
+  // dependent products: they take one row per NC item, split four ways (and the R1CS f item
+  // three ways)
+  J.A.split = pairs <= 32 ? 1 : 0;
+  const size_t target = (size_t)RB * 512;
+  size_t G = 1;
+  while (G < nrow && pairs * G < target) G *= 2;
+  if (G > nrow || J.A.split) G = nrow;
+  G = (nrow + (nrow + G - 1) / G - 1) / ((nrow + G - 1) / G);  // no empty groups
+  const size_t nfold = (size_t)A.nf + (A.has_eval ? 1 : 0);
+  const size_t spl = J.A.split ? 4 : 1, fs = J.A.split && J.A.r1cs[0] >= 0 ? 3 : 1;
+  const size_t base_s = (pairs * G * spl + 63) & ~(size_t)63;
+  size_t nblk = (base_s + pairs * (fs + (A.has_eval ? 1 : 0) + nfold) + RB - 1) / RB;
+  if (nblk > RMAXBLK) nblk = RMAXBLK;
+  if (nblk < 1) nblk = 1;
   J.wg = wg;
   J.partials = (K2*)ws;
   J.out = out;

@@ -188,6 +188,7 @@ struct FusedArgs {
   int fdeg = 4;
   int r1cs[3] = {-1, -1, -1};
   uint64_t r1cs_c[2] = {0, 0};
+  int split = 0;  // set by fused_job for short rounds: NC rows and the R1CS f item split by evaluation point
 };
 constexpr int FUSED_MAX_SC = 8;
 void launch_round_fused(const FusedArgs& A, const K2* wg, K2* out, void* ws, hipStream_t s,
