@@ -35,8 +35,8 @@ KERNEL_NAMES = {
     # three rounds); below the MFMA path's sizes, k_round_eval<0>, <1> and <2>
     "round0_eval": ("k_digit_monomials<0>", "k_round_eval<6>"),
     "round1_eval": ("k_digit_monomials<1>", "k_round_eval<6>"),
-    "round_eval": ("k_digit_monomials8", "k_round_eval<6>"),
-    "table_fold": "k_fold_k",
+    "round_eval": ("k_digit_monomials8", "k_round_eval<6>", "k_round_eval<2>"),
+    "table_fold": ("k_fold_k", "k_fold_digits3"),
     "ajtai_gemm": ("k_ajtai_gemm32p", "k_ajtai_gemm32", "k_ajtai_gemm"),
     "ydigits_gemm": "k_ydigits_gemm<4>",
     "rlc_zmix": "k_rowcontract<4>",
@@ -50,12 +50,13 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
     """Algorithmic bytes (or int8 ops) per kernel class summed over one shard of `steps`.
 
     Pi_CCS tables per row at round r (length L = n >> r): k*54 NC entries + nsc scalar tables
-    (eq_beta, 3 f-var tables, plus eq_rin and Eval once k > 1), 16 B per K entry.  Rounds 0 and 1
+    (eq_beta, 3 f-var tables, plus eq_rin and Eval once k > 1), 16 B per K entry.  Rounds 0-2
     read the int8 digit matrices (1 B per digit, all n digits of a row each) instead of NC K
-    entries; the fold after round 1 writes the NC tables of length n/4 from the digits.  Round 2
-    reads the digits as well (from n = 512; below, its K tables); each round r >= 3 is one fused
-    launch that reads the round r-1 NC, f-var and Eval tables (2L entries) and writes the folded
-    ones (L entries), and reads the chi tables of the later variables (L/2 entries, one for
+    entries; from n = 512 the fold after round 2 writes the NC tables of length n/8 from the
+    digits (8 digits per entry) and round 3 reads them unfused; below, the fold after round 1
+    writes them at length n/4 and round 2 reads them (K tables).  Every later round is one fused
+    launch that reads the previous round's NC, f-var and Eval tables (2L entries) and writes the
+    folded ones (L entries), and reads the chi tables of the later variables (L/2 entries, one for
     eq_beta and one for eq_rin) in place of the eq tables.
     """
     ell_n = n.bit_length() - 1
@@ -65,21 +66,27 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
         nsc = 4 if s == 0 else 6
         out["round0_eval"] += k * D * n + nsc * 16 * n
         out["round1_eval"] += k * D * n + nsc * 16 * (n // 2)
-        # round 2 from the digits (int8 MFMA, 120 monomials of 8 digits) once n / 8 fills 64-pair blocks
-        if n >= 512:
+        # round 2 from the digits (int8 MFMA, 120 monomials of 8 digits) once n / 8 fills 64-pair
+        # blocks; then round 3 unfused on the tables folded from the digits after round 2
+        digit2 = n >= 512
+        if digit2:
             out["round_eval"] += k * D * n + nsc * 16 * (n >> 2)
+            out["round_eval"] += (k * D + nsc) * 16 * (n >> 3)
         else:
             out["round_eval"] += (k * D + nsc) * 16 * (n >> 2)
         nfold, ngt = (3, 1) if s == 0 else (4, 2)
-        for r in range(3, ell_n):
+        for r in range(4 if digit2 else 3, ell_n):
             L = n >> r
             out["round_fused"] += (k * D + nfold) * 16 * (2 * L + L) + ngt * 16 * (L // 2)
-        # separate folds: scalar tables after rounds 0 and 1, the NC tables from the digits after
-        # round 1, and the last round's fold (flushed before y)
-        for r in (0, 1):
+        # separate folds: scalar tables after rounds 0, 1 (and 2), the NC tables from the digits,
+        # and the last round's fold (flushed before y)
+        for r in (0, 1, 2) if digit2 else (0, 1):
             L = n >> r
             out["table_fold"] += nsc * 16 * (L + L // 2)
-        out["table_fold"] += k * D * n + k * D * 16 * (n // 4)
+        if digit2:
+            out["table_fold"] += k * D * n + k * D * 16 * (n // 8)
+        else:
+            out["table_fold"] += k * D * n + k * D * 16 * (n // 4)
         out["table_fold"] += (k * D + nsc) * 16 * (2 + 1)
         # y products: k witnesses at the row point + 12 DEC children, t = 4 matrices
         for nw in (k, 12):

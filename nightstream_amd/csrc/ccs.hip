@@ -1351,6 +1351,7 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
     launch_sum_partials(partials, nb_u + nb_fe, 5, out, sig, s);
     return;
   }
+  if (dmode == 2) throw std::logic_error("round 2 from the digits: tables outside the MFMA path's sizes");
   // Split the k*54 NC rows of a pair across G work items: long tables get long items (each item
   // ends with a few reductions and, in round 1, 40 coefficient products), short tables get as
   // many items as it takes to fill the GPU, down to one row per item.
@@ -1819,6 +1820,43 @@ void launch_fold_digits2(const int8_t* const* digits_dev, int k, size_t m, size_
   const K2 a0 = ksub(kone(), r0), a1 = ksub(kone(), r1);
   NS_LAUNCH_TIMED(KT_FOLD, k_fold_digits2, dim3((unsigned)((quarter + 255) / 256), (unsigned)(k * D)), dim3(256), 0,
                   s, digits_dev, m, len, dst, kmul(a0, a1), kmul(r0, a1), kmul(a0, r1), kmul(r0, r1));
+}
+
+// Tables after rounds 0-2 from the digits: 8 digits per entry, weights chi_(r0, r1, r2)(b) for
+// b = bit0 r0 + bit1 r1 + bit2 r2 order (digit 8i + b), signed selections summed lazily.
+struct Chi8 {
+  K2 w[8];
+};
+__global__ void k_fold_digits3(const int8_t* const* __restrict__ digits, size_t m, size_t eighth, K2* __restrict__ dst,
+                               Chi8 wts) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t row = blockIdx.y;
+  if (i >= eighth) return;
+  const uint2 w8 = *reinterpret_cast<const uint2*>(digits[row / D] + (row % D) * m + 8 * i);
+  KS1 acc = ks1_zero();
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    const int d = (int)(int8_t)((b < 4 ? w8.x : w8.y) >> (8 * (b & 3)));
+    const K2 w = wts.w[b];
+    ks1_add(acc, d > 0 ? w : (d < 0 ? kneg(w) : kzero()));
+  }
+  dst[row * eighth + i] = ks1_reduce(acc);
+}
+
+bool digit_round2_fits(size_t len, int k) {
+  const size_t pairs = len / 2;
+  return pairs >= 64 && pairs % 64 == 0 && k <= 32;
+}
+
+void launch_fold_digits3(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1, K2 r2,
+                         hipStream_t s) {
+  if (len % 8 != 0 || m % 8 != 0 || len > m) throw std::invalid_argument("fold_digits3: len and m multiples of 8, len <= m");
+  const size_t eighth = len / 8;
+  const K2 a0 = ksub(kone(), r0), a1 = ksub(kone(), r1), a2 = ksub(kone(), r2);
+  Chi8 w;
+  for (int b = 0; b < 8; b++) w.w[b] = kmul(kmul(b & 1 ? r0 : a0, b & 2 ? r1 : a1), b & 4 ? r2 : a2);
+  NS_LAUNCH_TIMED(KT_FOLD, k_fold_digits3, dim3((unsigned)((eighth + 255) / 256), (unsigned)(k * D)), dim3(256), 0, s,
+                  digits_dev, m, eighth, dst, w);
 }
 
 void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r, hipStream_t s) {

@@ -216,6 +216,13 @@ void launch_fold_k(const K2* src, size_t rows, size_t len, K2* dst, K2 r, hipStr
 void launch_fold_digits2(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1,
                          hipStream_t s);
 void launch_fold_digits(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r, hipStream_t s);
+// Tables after rounds 0, 1 and 2 straight from the digits (round 2 evaluated from them too):
+// T3[i] = sum_b chi_(r0, r1, r2)(b) D[8 i + b], [k*54][len / 8] for the digit rows of length len.
+void launch_fold_digits3(const int8_t* const* digits_dev, int k, size_t m, size_t len, K2* dst, K2 r0, K2 r1, K2 r2,
+                         hipStream_t s);
+// Whether round 2 of a table of `len` entries (len / 2 pairs) runs from the digits on int8 MFMA
+// (launch_round_eval with digit_round2): its block size and the instance limit
+bool digit_round2_fits(size_t len, int k);
 
 // ---- Pi_RLC / Pi_DEC ----
 // zmix[r][c] = sum_i sum_kk rho_i[r][kk] Z_i[kk][c]; rho given as int8 [k][54][54]

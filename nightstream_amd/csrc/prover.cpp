@@ -699,6 +699,7 @@ CcsOracle::CcsOracle(const Ccs& s, const int8_t* const* Zs, size_t k, const K2* 
     w_->gt.ensure(2 * g_total_ * sizeof(K2));
     launch_chi_suffix(bits, (int)ell_n, 3, g_total_, has_eval_ ? 2 : 1, w_->gt.as<K2>(), st);
   }
+  digit2_ = n_pad >= 8 && digit_round2_fits(n_pad / 4, (int)k) && m % 8 == 0 && n_pad <= m;
   w_->nc[0].ensure(k * D * (n_pad / 2) * sizeof(K2));
   // nc[1] also holds round 2's 120 x n/8 digit-monomial sums (larger than the tables when k = 1)
   w_->nc[1].ensure(std::max(k * D * std::max<size_t>(n_pad / 4, 1), 15 * n_pad) * sizeof(K2));
@@ -908,7 +909,7 @@ std::vector<K2> CcsOracle::round_coeffs() {
   T.nf = nf_;
   T.eqr = has_eval_ ? sc + (size_t)(1 + nf_) * len_ : nullptr;
   T.ev = has_eval_ ? sc + (size_t)(2 + nf_) * len_ : nullptr;
-  T.nc = round_ <= 1 ? nullptr : w_->nc[cur_nc_].as<K2>();
+  T.nc = round_ <= 1 || (round_ == 2 && digit2_) ? nullptr : w_->nc[cur_nc_].as<K2>();
   T.digits = w_->Zptrs.as<const int8_t*>();
   T.m = s_.m;
   T.k = (int)k_;
@@ -954,7 +955,7 @@ std::vector<K2> CcsOracle::round_coeffs() {
     const K2 al = ksub(kone(), r0_), be = r0_;
     monomial_polys({lin(al, kneg(al)), lin(be, kneg(be)), lin(kzero(), al), lin(kzero(), be)});
     T.digit_round1 = true;
-  } else if (round_ == 2 && len_ >= 2) {
+  } else if (round_ == 2 && digit2_) {
     // the pair's entries fold digits 8t..8t+3 and 8t+4..8t+7 by chi_(r0, r1)
     const K2 a0 = ksub(kone(), r0_), a1 = ksub(kone(), r1_);
     const K2 w[4] = {kmul(a0, a1), kmul(r0_, a1), kmul(a0, r1_), kmul(r0_, r1_)};
@@ -999,7 +1000,7 @@ void CcsOracle::fold(K2 r) {
     if (has_eval_) cr_ = kmul(cr_, eq_lin(r, rin_[round_]));
   }
   const size_t half = len_ / 2;
-  if (round_ >= 2 && nsc_ <= FUSED_MAX_SC && half >= 1) {
+  if (round_ >= 2 && nsc_ <= FUSED_MAX_SC && half >= 1 && !(round_ == 2 && digit2_)) {
     pending_ = true;
     pending_r_ = r;
     len_ = half;
@@ -1015,6 +1016,11 @@ void CcsOracle::fold(K2 r) {
       launch_fold_digits(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_, w_->nc[0].as<K2>(), r, st_);
       cur_nc_ = 0;
     }
+  } else if (round_ == 1 && digit2_) {
+    r1_ = r;  // the NC tables stay as digits through round 2
+  } else if (round_ == 2 && digit2_) {
+    launch_fold_digits3(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_ * 4, w_->nc[0].as<K2>(), r0_, r1_, r, st_);
+    cur_nc_ = 0;
   } else if (round_ == 1 && len_ * 2 >= 4) {
     r1_ = r;
     launch_fold_digits2(w_->Zptrs.as<const int8_t*>(), (int)k_, s_.m, len_ * 2, w_->nc[0].as<K2>(), r0_, r, st_);

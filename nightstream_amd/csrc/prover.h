@@ -303,6 +303,9 @@ class CcsOracle {
   std::vector<const int8_t*> zs_;
   K2 r0_ = K2{0, 0};  // round-0 challenge: round 1 is evaluated from the digits with it
   K2 r1_ = K2{0, 0};  // round-1 challenge: with r0, round 2 is evaluated from the digits
+  // Round 2 from the digits (int8 MFMA): the NC tables are first materialized after round 2,
+  // folded by r0, r1, r2 at once, and round 3 is evaluated unfused on them
+  bool digit2_ = false;
   // From round 2 on, fold(r) only records r: the next round_coeffs folds and evaluates in one
   // launch (launch_round_fused), and anything else that reads the tables flushes it first.
   bool pending_ = false;
