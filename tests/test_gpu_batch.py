@@ -101,7 +101,8 @@ def test_process_prover_equals_plain(shards):
 def test_shared_round_launches(ns, monkeypatch):
     """Eight CCS-only shards proved by eight concurrent contexts with shared fused-round launches
     give the plain path's bytes, and the shared launcher carried every fused round (rounds
-    3 .. ell_n - 1 of every step), in no more launches than rounds."""
+    4 .. ell_n - 1 of every step: from 2^9 rows round 2 runs from the digits and round 3 unfused
+    on the tables folded after it), in no more launches than rounds."""
     from nightstream_amd import batch
     from nightstream_amd.batch import ShardFactory, prove_batched, prove_sequential
     monkeypatch.setattr(batch, "SHARE_ROUNDS", True)
@@ -115,7 +116,7 @@ def test_shared_round_launches(ns, monkeypatch):
     bat, _ = prove_batched(shards, 8)
     l1, r1 = ns.shared_round_stats()
     assert bat == seq
-    want = sum(steps * (n.bit_length() - 1 - 3) for n, steps, _ in specs)
+    want = sum(steps * (n.bit_length() - 1 - 4) for n, steps, _ in specs)
     assert r1 - r0 == want
     assert 0 < l1 - l0 <= r1 - r0
     print(f"shared rounds: {r1 - r0} rounds in {l1 - l0} launches", flush=True)
