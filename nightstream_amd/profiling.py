@@ -54,9 +54,10 @@ def stats_table(path: str, cmd: str, top: int = 25) -> str:
     return "\n".join(out) + "\n"
 
 
-def fused_rounds_line(trace_csv: str, period: int, name: str = "k_round_fused", first: int = 3) -> str:
+def fused_rounds_line(trace_csv: str, period: int, name: str = "k_round_fused", first: int = 4) -> str:
     """Median duration of each fused Pi_CCS round over the fold steps of a kernel trace (the
-    fused launches in start order, `period` = ell_n - 3 of them per fold step)."""
+    fused launches in start order, `period` of them per fold step: ell_n - 4 from 2^9 rows, where
+    rounds 2 and 3 are not fused, else ell_n - 3)."""
     import statistics
     rows = [r for r in csv.DictReader(open(trace_csv)) if short_name(r["Kernel_Name"]) == name]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
@@ -81,7 +82,7 @@ def tail_line(trace_csv: str, prefix: str, count: int) -> str:
             f"(min {min(d):.1f}, max {max(d):.1f}); " + ", ".join(f"{k} x{v}" for k, v in sorted(names.items())))
 
 
-def sq_table(db: str, cmd: str, name: str = "k_round_fused", first: int = 3) -> str:
+def sq_table(db: str, cmd: str, name: str = "k_round_fused", first: int = 4) -> str:
     """SQ counters of each fused round (the last fold step of the run), per wave."""
     import sqlite3
     cur = sqlite3.connect(db).cursor()
@@ -173,12 +174,14 @@ def main(argv=None):
     s.add_argument("csv")
     s.add_argument("--cmd", required=True)
     s.add_argument("--trace", default=None)
-    s.add_argument("--rounds-per-step", type=int, default=13)
+    s.add_argument("--rounds-per-step", type=int, default=12)
+    s.add_argument("--first-round", type=int, default=4)
     s.add_argument("--tail-prefix", default=None)
     s.add_argument("--tail-count", type=int, default=17)
     q = sub.add_parser("sq")
     q.add_argument("db")
     q.add_argument("--cmd", required=True)
+    q.add_argument("--first-round", type=int, default=4)
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -191,11 +194,11 @@ def main(argv=None):
     if a.mode == "stats":
         sys.stdout.write(stats_table(a.csv, a.cmd))
         if a.trace:
-            sys.stdout.write("\n" + fused_rounds_line(a.trace, a.rounds_per_step) + "\n")
+            sys.stdout.write("\n" + fused_rounds_line(a.trace, a.rounds_per_step, first=a.first_round) + "\n")
             if a.tail_prefix:
                 sys.stdout.write("\n" + tail_line(a.trace, a.tail_prefix, a.tail_count) + "\n")
     elif a.mode == "sq":
-        sys.stdout.write(sq_table(a.db, a.cmd))
+        sys.stdout.write(sq_table(a.db, a.cmd, first=a.first_round))
     else:
         sys.stdout.write(traffic_table(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes, a.cmd, a.copy_cmd))
 
