@@ -90,7 +90,9 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
         out["table_fold"] += (k * D + nsc) * 16 * (2 + 1)
         # y products: k witnesses at the row point + 12 DEC children, t = 4 matrices
         for nw in (k, 12):
-            out["ydigits_gemm"] += 2.0 * (nw * D) * 64 * n
+            # digits x 64 byte-limb rows of v: 2 (nw 54) 64 n int8 ops on (nw 54 + 64) n bytes, 117
+            # ops per byte, under the MI355X's int8 ridge (~625): priced against HBM, by bytes
+            out["ydigits_gemm"] += (nw * D + 64) * n
         out["rlc_zmix"] += (k * D * n + D * n * 4) if k > 1 else D * n * 5
         out["dec_split"] += D * n * 4 + 12 * D * n
         out["ajtai_gemm"] += 2.0 * 6912 * D * 12 * n  # 12 DEC children per step
@@ -99,7 +101,7 @@ def algorithmic_work(n: int, steps: int, nw_mcs: int):
 
 
 def kernel_bound(cls: str):
-    if cls in ("ajtai_gemm", "ydigits_gemm"):
+    if cls == "ajtai_gemm":
         return "mfma", "TFLOP/s", INT8_PEAK_TOPS
     return "hbm", "GB/s", HBM_PEAK_GBS
 

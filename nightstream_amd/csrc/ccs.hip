@@ -412,6 +412,75 @@ __global__ __launch_bounds__(256) void k_ydigits_gemm(const int8_t* const* __res
     for (int v = 0; v < 4; v++) base[((lane >> 4) * 4 + v) * (NS * 16) + n * 16 + r16] = acc[n][v];
 }
 
+// The same product with both operands staged through LDS: per 256-byte K chunk a block loads its
+// 64 digit rows and the 16 NS limb rows of v with 16-byte loads, 16 threads per row (256
+// contiguous bytes of one row per 16 lanes, where the direct kernel's lanes read 64 bytes from
+// 16 rows apart), into a double-buffered tile (rows padded to 272 bytes: the 16-byte fragment
+// reads of 16 lanes hit distinct banks); the next chunk's loads are in flight during this
+// chunk's MFMAs.  kc must be a multiple of 256.
+constexpr int YD_PITCH = 272;
+template <int NS>
+__global__ __launch_bounds__(256) void k_ydigits_gemm_lds(const int8_t* const* __restrict__ Zs, size_t nrows,
+                                                         const int8_t* __restrict__ Vl, size_t m, size_t kc,
+                                                         int32_t* __restrict__ part, size_t mpad) {
+  __shared__ __align__(16) int8_t at[2][64 * YD_PITCH];
+  __shared__ __align__(16) int8_t bt[2][NS * 16 * YD_PITCH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t r0 = (size_t)blockIdx.x * 64;
+  const size_t k_begin = (size_t)blockIdx.y * kc;
+  const int kq = (lane >> 4) * 16, r16 = lane & 15;
+  // this thread's load slots: A rows (4 of them) and B rows (NS of them), 16 bytes each
+  const int seg = threadIdx.x & 15, rsub = threadIdx.x >> 4;  // rsub 0..15
+  const int8_t* asrc[4];
+  bool aok[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const size_t row = r0 + (size_t)(q * 16 + rsub);
+    aok[q] = row < nrows;
+    asrc[q] = aok[q] ? Zs[row / D] + (row % D) * m + k_begin + seg * 16 : nullptr;
+  }
+  v4i ra[4], rb[NS];
+  auto gload = [&](size_t k) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) ra[q] = aok[q] ? *(const v4i*)(asrc[q] + k) : v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < NS; n++) rb[n] = *(const v4i*)(Vl + (size_t)(n * 16 + rsub) * m + k_begin + k + seg * 16);
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) *(v4i*)(at[buf] + (q * 16 + rsub) * YD_PITCH + seg * 16) = ra[q];
+#pragma unroll
+    for (int n = 0; n < NS; n++) *(v4i*)(bt[buf] + (n * 16 + rsub) * YD_PITCH + seg * 16) = rb[n];
+  };
+  v4i acc[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) acc[n] = v4i{0, 0, 0, 0};
+  const size_t nch = kc / 256;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (size_t c = 0; c < nch; c++) {
+    const int buf = (int)(c & 1);
+    if (c + 1 < nch) gload((c + 1) * 256);
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+      const v4i a = *(const v4i*)(at[buf] + (wave * 16 + r16) * YD_PITCH + st * 64 + kq);
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        const v4i b = *(const v4i*)(bt[buf] + (n * 16 + r16) * YD_PITCH + st * 64 + kq);
+        acc[n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[n], 0, 0, 0);
+      }
+    }
+    if (c + 1 < nch) lstore(buf ^ 1);  // the other buffer's readers finished before the last barrier
+    __syncthreads();
+  }
+  int32_t* base = part + ((size_t)blockIdx.y * mpad + r0 + wave * 16) * (NS * 16);
+#pragma unroll
+  for (int n = 0; n < NS; n++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) base[((lane >> 4) * 4 + v) * (NS * 16) + n * 16 + r16] = acc[n][v];
+}
+
 // thread per (row, j): y[row][j] = sum_comp,l 256^l (G + 128 rowsum)
 __global__ void k_ydigits_epilogue(const int32_t* __restrict__ part, size_t splits, size_t mpad, size_t nrows,
                                    size_t t, const int32_t* __restrict__ rowsum, K2* __restrict__ y) {
@@ -472,7 +541,13 @@ void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t 
   hipLaunchKernelGGL(k_v_limbs, dim3((unsigned)((m + 255) / 256), (unsigned)t), dim3(256), 0, s, v, t, m, Vl);
   hipLaunchKernelGGL(k_digit_rowsum2, dim3((unsigned)nrows), dim3(256), 0, s, Zs_dev, m, rowsum);
   dim3 g((unsigned)(mpad / 64), (unsigned)splits);
-#define NS_YD(T) NS_LAUNCH_TIMED(KT_YD_GEMM, k_ydigits_gemm<T>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad)
+#define NS_YD(T)                                                                                              \
+  do {                                                                                                        \
+    if (kc % 256 == 0 && (T) <= 4)                                                                             \
+      NS_LAUNCH_TIMED(KT_YD_GEMM, k_ydigits_gemm_lds<T>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad); \
+    else                                                                                                      \
+      NS_LAUNCH_TIMED(KT_YD_GEMM, k_ydigits_gemm<T>, g, dim3(256), 0, s, Zs_dev, nrows, Vl, m, kc, part, mpad);     \
+  } while (0)
   switch (t) {
     case 1: NS_YD(1); break;
     case 2: NS_YD(2); break;
