@@ -326,6 +326,35 @@ void launch_weighted_digit_colsum(const int8_t* const* Zs_dev, size_t nw, const 
 
 // ---------------- y = Z v via int8 MFMA on byte limbs of v ----------------
 // Vl[(j*2 + comp)*8 + l][c] = byte_l(v_j[c].comp) - 128 (64 rows per 4 matrices).
+// Balanced base-256 limbs of v's K coordinates: component x in [0, q) is taken as the representative
+// v of its class with |v| <= 128 (2^64 - 1) / 255 (x itself up to 0x7F..7F, else x - q) and written
+// as 8 digits in [-128, 127] (digit-by-digit with carries), so the digit x limb products need no
+// offset correction.  Limb row (j 2 + comp) 8 + l of [16 t][m].
+__device__ __forceinline__ void balanced_limbs(uint64_t x, int8_t (&d)[8]) {
+  if (x <= 0x7F7F7F7F7F7F7F7Full) {
+    uint64_t u = x;  // non-negative: u = sum d_l 256^l
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+      int r = (int)(u & 255);
+      if (r >= 128) r -= 256;
+      d[l] = (int8_t)r;
+      u = (u - (uint64_t)(int64_t)r) >> 8;
+    }
+  } else {
+    uint64_t w = Q - x;  // v = -w: the digits of -w, with w's magnitude carried
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+      const int r = (int)(w & 255);
+      if (r <= 128) {
+        d[l] = (int8_t)(-r);
+        w = (w - (uint64_t)r) >> 8;
+      } else {
+        d[l] = (int8_t)(256 - r);
+        w = (w + (uint64_t)(256 - r)) >> 8;
+      }
+    }
+  }
+}
 __global__ void k_v_limbs(const K2* __restrict__ v, size_t t, size_t m, int8_t* __restrict__ Vl) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t j = blockIdx.y;
@@ -333,32 +362,15 @@ __global__ void k_v_limbs(const K2* __restrict__ v, size_t t, size_t m, int8_t* 
   K2 x = v[j * m + c];
   uint64_t comp[2] = {x.c0, x.c1};
 #pragma unroll
-  for (int cp = 0; cp < 2; cp++)
+  for (int cp = 0; cp < 2; cp++) {
+    int8_t d[8];
+    balanced_limbs(comp[cp], d);
 #pragma unroll
-    for (int l = 0; l < 8; l++)
-      Vl[((j * 2 + cp) * 8 + l) * m + c] = (int8_t)((int)((comp[cp] >> (8 * l)) & 0xFF) - 128);
-}
-
-__device__ __forceinline__ int sum4_i8(int x) {
-  return (x << 24 >> 24) + (x << 16 >> 24) + (x << 8 >> 24) + (x >> 24);
-}
-
-// S[w*54 + t] = sum_j Z_w[t][j]; 16-byte loads (m is a multiple of 64)
-__global__ __launch_bounds__(256) void k_digit_rowsum2(const int8_t* const* __restrict__ Zs, size_t m,
-                                                       int32_t* __restrict__ S) {
-  size_t w = blockIdx.x / D, t = blockIdx.x % D;
-  const int4* row = (const int4*)(Zs[w] + t * m);
-  int32_t acc = 0;
-  for (size_t j = threadIdx.x; j < m / 16; j += blockDim.x) {
-    int4 v = row[j];
-    acc += sum4_i8(v.x) + sum4_i8(v.y) + sum4_i8(v.z) + sum4_i8(v.w);
+    for (int l = 0; l < 8; l++) Vl[((j * 2 + cp) * 8 + l) * m + c] = d[l];
   }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  __shared__ int32_t red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) S[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
+
+
 
 constexpr int YD_SPLIT_MAX = 64;
 
@@ -481,9 +493,9 @@ __global__ __launch_bounds__(256) void k_ydigits_gemm_lds(const int8_t* const* _
     for (int v = 0; v < 4; v++) base[((lane >> 4) * 4 + v) * (NS * 16) + n * 16 + r16] = acc[n][v];
 }
 
-// thread per (row, j): y[row][j] = sum_comp,l 256^l (G + 128 rowsum)
+// thread per (row, j): y[row][j] = sum_comp,l 256^l G (balanced limbs: no offset correction)
 __global__ void k_ydigits_epilogue(const int32_t* __restrict__ part, size_t splits, size_t mpad, size_t nrows,
-                                   size_t t, const int32_t* __restrict__ rowsum, K2* __restrict__ y) {
+                                   size_t t, K2* __restrict__ y) {
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nrows * t) return;
   size_t row = idx / t, j = idx % t;
@@ -495,7 +507,6 @@ __global__ void k_ydigits_epilogue(const int32_t* __restrict__ part, size_t spli
       int64_t g = 0;
       size_t col = (j * 2 + cp) * 8 + l;
       for (size_t sp = 0; sp < splits; sp++) g += part[(sp * mpad + row) * ncols + col];
-      g += 128 * (int64_t)rowsum[row];
       acc = fadd(acc, fmul(from_i64(g), 1ull << (8 * l)));
     }
     comp[cp] = acc;
@@ -539,7 +550,6 @@ void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t 
   int32_t* rowsum = (int32_t*)(((uintptr_t)(Vl + 16 * t * m) + 255) & ~(uintptr_t)255);
   int32_t* red = (int32_t*)(((uintptr_t)(rowsum + nrows) + 255) & ~(uintptr_t)255);
   hipLaunchKernelGGL(k_v_limbs, dim3((unsigned)((m + 255) / 256), (unsigned)t), dim3(256), 0, s, v, t, m, Vl);
-  hipLaunchKernelGGL(k_digit_rowsum2, dim3((unsigned)nrows), dim3(256), 0, s, Zs_dev, m, rowsum);
   dim3 g((unsigned)(mpad / 64), (unsigned)splits);
 #define NS_YD(T)                                                                                              \
   do {                                                                                                        \
@@ -562,7 +572,7 @@ void launch_ydigits(const int8_t* const* Zs_dev, size_t nw, const K2* v, size_t 
   size_t nt = nrows * t;
   launch_sum_splits(part, splits, mpad * 16 * t, red, s);
   hipLaunchKernelGGL(k_ydigits_epilogue, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, red, (size_t)1, mpad,
-                     nrows, t, rowsum, y);
+                     nrows, t, y);
 }
 
 // ---------------- round polynomial ----------------
