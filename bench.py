@@ -38,7 +38,7 @@ KERNEL_NAMES = {
     "round_eval": ("k_digit_monomials8", "k_round_eval<6>", "k_round_eval<2>"),
     "table_fold": ("k_fold_k", "k_fold_digits3"),
     "ajtai_gemm": ("k_ajtai_gemm32p", "k_ajtai_gemm32", "k_ajtai_gemm"),
-    "ydigits_gemm": "k_ydigits_gemm<4>",
+    "ydigits_gemm": ("k_ydigits_gemm_lds<4>", "k_ydigits_gemm<4>"),
     "rlc_zmix": "k_rowcontract<4>",
     "dec_split": "k_dec_split",
     "round_fused": "k_round_fused",
