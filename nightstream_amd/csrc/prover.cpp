@@ -1107,14 +1107,22 @@ void Prover::rebind(const Ccs& s, const Pp& pp) {
 
 Prover::~Prover() { (void)hipStreamDestroy(stream_); }
 
-void Prover::commit_device(const int8_t* const* Zs_dev, size_t nw, uint64_t* host_out) {
+void Prover::commit_launch(const int8_t* const* Zs_dev, size_t nw) {
   ws_ajtai_.ensure(ajtai_workspace_bytes(s_->m, nw));
   ptrs_.ensure(nw * sizeof(void*));
   cout_.ensure(nw * KAPPA * D * 8);
   h2d(ptrs_.p, (const void* const*)Zs_dev, nw, stream_);
   ajtai_commit(pp_->limbs.as<int8_t>(), KAPPA, s_->m, ptrs_.as<const int8_t*>(), nw, cout_.as<uint64_t>(),
                ws_ajtai_.p, stream_);
+}
+
+void Prover::commit_fetch(size_t nw, uint64_t* host_out) {
   d2h(ows_.stage, host_out, cout_.p, nw * KAPPA * D, stream_);
+}
+
+void Prover::commit_device(const int8_t* const* Zs_dev, size_t nw, uint64_t* host_out) {
+  commit_launch(Zs_dev, nw);
+  commit_fetch(nw, host_out);
 }
 
 static size_t log2_pow2(size_t n) {
@@ -1336,7 +1344,7 @@ void Prover::vjs_at(const std::vector<K2>& r, DevBuf& vjs) {
 }
 
 // y[w][j][rho] = sum_c Z_w[rho][c] v_j[c] for device digit planes (int8 MFMA contraction)
-std::vector<K2> Prover::y_products(const std::vector<const int8_t*>& planes, const DevBuf& vjs) {
+void Prover::y_launch(const std::vector<const int8_t*>& planes, const DevBuf& vjs) {
   const Ccs& s = *s_;
   const size_t nw = planes.size();
   yptr_.ensure(nw * sizeof(void*));
@@ -1344,9 +1352,17 @@ std::vector<K2> Prover::y_products(const std::vector<const int8_t*>& planes, con
   ws_yd_.ensure(ydigits_ws_bytes(nw, s.t, s.m));
   ydev_.ensure(nw * s.t * D * sizeof(K2));
   launch_ydigits(yptr_.as<const int8_t*>(), nw, vjs.as<K2>(), s.t, s.m, ydev_.as<K2>(), ws_yd_.p, stream_);
-  std::vector<K2> out(nw * s.t * D);
+}
+
+std::vector<K2> Prover::y_fetch(size_t nw) {
+  std::vector<K2> out(nw * s_->t * D);
   d2h(ows_.stage, out.data(), ydev_.p, out.size(), stream_);
   return out;
+}
+
+std::vector<K2> Prover::y_products(const std::vector<const int8_t*>& planes, const DevBuf& vjs) {
+  y_launch(planes, vjs);
+  return y_fetch(planes.size());
 }
 
 void Prover::head_cols(const int8_t* const* planes, size_t np, size_t m_in, std::vector<std::vector<uint64_t>*> Xs) {
@@ -1467,12 +1483,16 @@ void Prover::rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const st
   }
   flags_.ensure(4);
   launch_dec_split(zsrc, (size_t)D * m, (int)k_dec, dec_pl, flags_.as<int>(), st);
+  // the children's commitments and y products queue behind the split; the host waits once (the
+  // range check reads the split's flags after them: an out-of-range split fails the step either way)
+  commit_launch(pptr.data(), k_dec);
+  y_launch(pptr, vjs);
   uint32_t dmask = 0;
   d2h(ows_.stage, &dmask, flags_.p, 1, st);
   if (dec_out_of_range(dmask, (int)k_dec)) throw Error(6, "step " + std::to_string(idx) + ": DEC split out of range");
   std::vector<uint64_t> child_c(k_dec * KAPPA * D, 0);
-  commit_device(pptr.data(), k_dec, child_c.data());
-  std::vector<K2> y_child = y_products(pptr, vjs);
+  commit_fetch(k_dec, child_c.data());
+  std::vector<K2> y_child = y_fetch(k_dec);
   out.children.assign(k_dec, MeHost{});
   {
     std::vector<std::vector<uint64_t>*> xs;

@@ -223,6 +223,9 @@ class Prover {
   uint64_t last_permutations = 0;
   // commit helpers exposed to the C ABI
   void commit_device(const int8_t* const* Zs_host_ptrs_dev, size_t nw, uint64_t* host_out);
+  // the same in two halves: the launches (queued on the prover's stream), then the copy out
+  void commit_launch(const int8_t* const* Zs_host_ptrs_dev, size_t nw);
+  void commit_fetch(size_t nw, uint64_t* host_out);
   hipStream_t stream() const { return stream_; }
   const Ccs& ccs() const { return *s_; }
   const Pp& pp() const { return *pp_; }
@@ -253,6 +256,8 @@ class Prover {
   void open_bus_i32(const int32_t* zmix, const BusLayout& L, MeHost& out, size_t t_core);
   void vjs_at(const std::vector<K2>& r, DevBuf& vjs);
   std::vector<K2> y_products(const std::vector<const int8_t*>& planes, const DevBuf& vjs);
+  void y_launch(const std::vector<const int8_t*>& planes, const DevBuf& vjs);
+  std::vector<K2> y_fetch(size_t nw);
   void head_cols(const int8_t* const* planes, size_t np, size_t m_in, std::vector<std::vector<uint64_t>*> Xs);
   void rlc_dec_lane(Transcript& tr, const char* lane, size_t idx, const std::vector<MeHost>& ins,
                     const std::vector<const int8_t*>& Zs, DevBuf* bank, const DevBuf& vjs, const BusLayout& L,
