@@ -1224,15 +1224,16 @@ __global__ __launch_bounds__(RB) void k_round_eval(RoundArgs A, const K2* __rest
         const uint64_t w7 = w7s[r];
         const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
         const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
-        ks2_mac(z1, wa, a2);
+        const uint64_t was = ksum(wa), wbs = ksum(wb), a2s = ksum(a2), b2s = ksum(b2);
+        ks2k_mac(z1, wa, was, a2, a2s);
         ks1_add(z2, wa);
-        ks2_mac(z3, wb, a2);
+        ks2k_mac(z3, wb, wbs, a2, a2s);
         ks1_add(z4, wb);
-        ks2_mac(z5, wa, b2);
-        ks2_mac(z6, wb, b2);
+        ks2k_mac(z5, wa, was, b2, b2s);
+        ks2k_mac(z6, wb, wbs, b2, b2s);
       }
-      const K2 s1 = ks2_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2_reduce(z3), s4 = ks1_reduce(z4),
-               s5 = ks2_reduce(z5), s6 = ks2_reduce(z6);
+      const K2 s1 = ks2k_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2k_reduce(z3), s4 = ks1_reduce(z4),
+               s5 = ks2k_reduce(z5), s6 = ks2k_reduce(z6);
       in[0] = kadd(in[0], ksub(s1, s2));
       in[1] = kadd(in[1], ksub(kadd(kadd(s3, s3), s3), s4));
       in[2] = kadd(in[2], kadd(kadd(s5, s5), s5));
@@ -1596,15 +1597,16 @@ __device__ __forceinline__ void fused_nc_rows(const FusedArgs& A, const K2* __re
     const uint64_t w7 = fmul7(wv.c1);
     const K2 wa = kmul_w7(wv, w7, a), wb = kmul_w7(wv, w7, b);
     const K2 a2 = ksqr_dev(a), b2 = ksqr_dev(b);
-    ks2_mac(z1, wa, a2);
+    const uint64_t was = ksum(wa), wbs = ksum(wb), a2s = ksum(a2), b2s = ksum(b2);
+    ks2k_mac(z1, wa, was, a2, a2s);
     ks1_add(z2, wa);
-    ks2_mac(z3, wb, a2);
+    ks2k_mac(z3, wb, wbs, a2, a2s);
     ks1_add(z4, wb);
-    ks2_mac(z5, wa, b2);
-    ks2_mac(z6, wb, b2);
+    ks2k_mac(z5, wa, was, b2, b2s);
+    ks2k_mac(z6, wb, wbs, b2, b2s);
   }
-  const K2 s1 = ks2_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2_reduce(z3), s4 = ks1_reduce(z4),
-           s5 = ks2_reduce(z5), s6 = ks2_reduce(z6);
+  const K2 s1 = ks2k_reduce(z1), s2 = ks1_reduce(z2), s3 = ks2k_reduce(z3), s4 = ks1_reduce(z4),
+           s5 = ks2k_reduce(z5), s6 = ks2k_reduce(z6);
   in[0] = ksub(s1, s2);
   in[1] = ksub(kadd(kadd(s3, s3), s3), s4);
   in[2] = kadd(kadd(s5, s5), s5);

@@ -211,6 +211,23 @@ __device__ __forceinline__ void ks2_mac(KS2& s, K2 x, K2 y) {
 __device__ __forceinline__ K2 ks2_reduce(const KS2& s) {
   return K2{fadd(s2_reduce(s.p00), fmul7(s2_reduce(s.p11))), s2_reduce(s.px)};
 }
+// Karatsuba form of the same lazy sum: px holds sum (x0 + x1)(y0 + y1) and the cross term is
+// px - p00 - p11, so a product costs three wide products; the operand sums (ksum, a congruent u64)
+// are formed once per operand and shared by every product it enters.
+__device__ __forceinline__ uint64_t ksum(K2 x) {
+  uint64_t s;
+  const bool c = __builtin_add_overflow(x.c0, x.c1, &s);
+  return s + ((uint64_t)0 - (uint64_t)c & EPS);  // x0 + x1 < 2^65 - 2^33: one fold of 2^64 = EPS
+}
+__device__ __forceinline__ void ks2k_mac(KS2& s, K2 x, uint64_t xs, K2 y, uint64_t ys) {
+  s2_mac(s.p00, x.c0, y.c0);
+  s2_mac(s.p11, x.c1, y.c1);
+  s2_mac(s.px, xs, ys);
+}
+__device__ __forceinline__ K2 ks2k_reduce(const KS2& s) {
+  const uint64_t r00 = s2_reduce(s.p00), r11 = s2_reduce(s.p11);
+  return K2{fadd(r00, fmul7(r11)), fsub(fsub(s2_reduce(s.px), r00), r11)};
+}
 // a * s for a small signed integer s in [-8, 8]
 NS_HD K2 kmul_small(K2 a, int s) {
   K2 r = kzero();
