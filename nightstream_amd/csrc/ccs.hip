@@ -1410,8 +1410,9 @@ void launch_round_eval(const RowTables& T, const FPoly& f, const K2* wg, bool ro
   const int dmode = round0 ? 0 : (T.digit_round1 ? 1 : (T.digit_round2 ? 2 : -1));
   const size_t nmono = dmode == 0 ? 4 : (dmode == 1 ? 20 : 120);
   const size_t min_pairs = dmode == 2 ? 64 : 128;
+  const size_t digits_per_pair = dmode == 0 ? 2 : (dmode == 1 ? 4 : 8);  // bytes read per row and pair
   if (dmode >= 0 && T.wl && T.ubuf && T.pc && pairs >= min_pairs && pairs % min_pairs == 0 && T.k <= 32 &&
-      T.ubuf_bytes >= nmono * pairs * sizeof(K2)) {
+      T.ubuf_bytes >= nmono * pairs * sizeof(K2) && digits_per_pair * pairs <= T.m) {
     const int kt = dmode == 0 ? KT_ROUND0 : (dmode == 1 ? KT_ROUND1 : KT_ROUND);
     if (dmode == 2) {
       NS_LAUNCH_TIMED(kt, k_digit_monomials8, dim3((unsigned)(pairs / 16)), dim3(64 * M8_NGROUPS), 0, s, T.digits,
